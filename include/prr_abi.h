@@ -1,0 +1,236 @@
+/*
+ * prr_abi.h — C ABI of libprr.so, the MI355X (gfx950) path-tracing hot path.
+ *
+ * This is the drop-in boundary that replaces the per-pixel CPU loop of the
+ * reference renderer:
+ *   Renderer::Render(const Scene&, const Integrator&, const RenderSettings&)
+ *       Renderer.hpp:57-59, Renderer.cpp:24-212
+ *   Integrator::castRay(const Ray&)          Integrator.hpp:23, Integrator.cpp:104-263
+ *   Scene::intersect / Scene::intersectP     Scene.cpp:29-35
+ *   BVHAccel::Intersect / IntersectP         BVH.hpp:94-95, BVH.cpp:311-473
+ *
+ * Conventions
+ *  - Every entry point returns an int status: PRR_OK (0) or a negative
+ *    PRR_ERR_* code; prr_last_error(ctx) returns the message. No C++
+ *    exception crosses this boundary.
+ *  - Scene descriptors are HOST memory, borrowed for the duration of the call
+ *    and copied. Device-side scene buffers are owned by the prr_scene and
+ *    released by prr_scene_free.
+ *  - Array arguments of the batch entry points (prr_trace_*, prr_bsdf_*,
+ *    prr_env_*, prr_camera_rays, prr_tonemap, prr_render outputs) are DEVICE
+ *    pointers (HBM), caller-allocated. Work is enqueued on the context's
+ *    stream (prr_set_stream) and is asynchronous unless stated otherwise.
+ *  - A context is externally synchronized: one host thread at a time.
+ *    Distinct contexts may be used concurrently.
+ *  - Vectors are packed float triples (x,y,z), row-major.
+ */
+#ifndef PRR_ABI_H
+#define PRR_ABI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PRR_OK 0
+#define PRR_ERR_INVALID (-1)     /* bad argument / shape */
+#define PRR_ERR_DEVICE (-2)      /* HIP runtime error or no device */
+#define PRR_ERR_NOMEM (-3)       /* device allocation failed / over budget */
+#define PRR_ERR_UNSUPPORTED (-4) /* feature not available for this scene */
+
+/* Material classes (Material.hpp:100-183). */
+enum prr_material_type {
+    PRR_MAT_DIFFUSE = 0,  /* DiffuseMaterial: Lambert*(1-F)*(1-m) + GGX (Material.cpp:28-158) */
+    PRR_MAT_GLASS = 1,    /* GlassMaterial: Fresnel dielectric, delta (Material.cpp:169-191) */
+    PRR_MAT_MIRROR = 2,   /* MirrorMaterial: delta reflection (Material.cpp:160-167) */
+    PRR_MAT_EMISSIVE = 3  /* EmissiveMaterial: emits, never scatters (Material.hpp:135-152) */
+};
+
+/* Object kinds of Scene::objects (Scene.hpp:18). */
+enum prr_object_kind {
+    PRR_OBJ_MESH = 0,   /* MeshTriangle (Triangle.hpp:112-180), owns a sub-BVH of <=4 prims/leaf */
+    PRR_OBJ_SPHERE = 1  /* Sphere (Sphere.hpp:11-66) */
+};
+
+/* Tone-map operators (ToneMapping.hpp:32-37) plus ACES (Narkowicz 2015 fit). */
+enum prr_tonemap_op {
+    PRR_TONEMAP_RAW = 0,
+    PRR_TONEMAP_STANDARD = 1,
+    PRR_TONEMAP_AGX = 2,
+    PRR_TONEMAP_PBR_NEUTRAL = 3,
+    PRR_TONEMAP_ACES = 4
+};
+
+/* Material record. Field meaning follows Material.hpp:42-47. */
+typedef struct prr_material_desc {
+    int32_t type;          /* prr_material_type */
+    float base_color[3];   /* Material::baseColor (default 0.8) */
+    float roughness;       /* Material::roughness; clamped to >=0.08 when shading */
+    float metallic;        /* Material::metallic */
+    float ior;             /* Material::ior (default 1.5) */
+    float emission[3];     /* Material::m_emission */
+} prr_material_desc;
+
+/* One entry of Scene::objects, in Scene::Add order. The order fixes the
+ * scene-level SAH build input, the emitter list order (Scene.cpp:19-25) and
+ * the global primitive ids: a triangle's id is its row in
+ * prr_scene_desc::triangles; sphere row s has id num_triangles + s. */
+typedef struct prr_object_desc {
+    int32_t kind;      /* prr_object_kind */
+    int32_t material;  /* row in prr_scene_desc::materials */
+    int32_t first;     /* mesh: first triangle row; sphere: sphere row */
+    int32_t count;     /* mesh: number of triangles (>=1); sphere: 1 */
+} prr_object_desc;
+
+/* Pinhole / thin-lens camera (Camera.hpp:8-20, init :43-56). */
+typedef struct prr_camera_desc {
+    float eye[3];
+    float target[3];
+    float up[3];
+    float fovh;            /* horizontal field of view, degrees (fovv is unused by the reference) */
+    float aperture;        /* 0 = pinhole */
+    float focus_distance;  /* <0: |target - eye| */
+} prr_camera_desc;
+
+/* Equirectangular linear-RGB environment (EnvMap.hpp). width == 0: none. */
+typedef struct prr_envmap_desc {
+    int32_t width;
+    int32_t height;
+    const float* pixels;   /* host, width*height*3 floats, row 0 = zenith (y-up) */
+} prr_envmap_desc;
+
+/* Point light (PointLight.hpp:9-16). The reference declares but never uses
+ * it; here it contributes NEE (intensity / d^2, delta) and photon emission. */
+typedef struct prr_point_light_desc {
+    float position[3];
+    float intensity[3];    /* radiant intensity (W/sr) per channel */
+} prr_point_light_desc;
+
+typedef struct prr_scene_desc {
+    int32_t num_objects;
+    const prr_object_desc* objects;
+    int32_t num_triangles;
+    const float* triangles;       /* 9 floats per triangle: v0, v1, v2 (counter-clockwise) */
+    int32_t num_spheres;
+    const float* spheres;         /* 4 floats per sphere: center xyz, radius */
+    int32_t num_materials;
+    const prr_material_desc* materials;
+    prr_envmap_desc env;
+    float background[3];          /* Scene::backgroundColor for escaped rays without env */
+    int32_t num_point_lights;
+    const prr_point_light_desc* point_lights;
+    prr_camera_desc camera;
+} prr_scene_desc;
+
+/* RenderSettings (Renderer.hpp:42-55) restricted to fixed spp, plus the
+ * Integrator parameters (Integrator.hpp:18-21). */
+typedef struct prr_render_params {
+    int32_t width;
+    int32_t height;
+    int32_t spp;               /* samples per pixel (minSPP = maxSPP) */
+    int32_t max_depth;         /* Integrator maxDepth */
+    float indirect_clamp;      /* Integrator::indirectClamp (<=0 disables) */
+    int32_t sample_offset;     /* index of the first sample (progressive / split renders) */
+    uint64_t seed;             /* per-pixel PCG32 seed */
+    /* caustic photon map (Photon.hpp); 0 photons = off */
+    int64_t num_photons;
+    float photon_radius;
+    int32_t photon_max_depth;
+    int32_t reserved[4];
+} prr_render_params;
+
+typedef struct prr_stats {
+    uint64_t rays_closest;     /* scene-level closest-hit queries */
+    uint64_t rays_any;         /* scene-level any-hit (shadow) queries */
+    uint64_t samples;          /* castRay-equivalents */
+    uint64_t photons_stored;   /* caustic photons stored */
+    double ms_total;           /* device time of the render, ms */
+} prr_stats;
+
+typedef struct prr_scene_info {
+    int32_t num_nodes;         /* interior nodes of the merged device BVH */
+    int32_t num_leaf_prims;    /* primitive slots in leaf order */
+    int32_t max_depth;         /* deepest root-to-leaf path */
+    int32_t num_emitters;      /* Scene::emissives */
+    float total_emit_area;     /* Scene::totalEmitArea */
+    float bounds[6];           /* scene root box min xyz, max xyz */
+    int64_t device_bytes;      /* HBM held by the scene */
+} prr_scene_info;
+
+typedef struct prr_context prr_context;
+typedef struct prr_scene prr_scene;
+
+/* ---- context ------------------------------------------------------------ */
+/* No reference counterpart: the reference has no device. */
+int prr_create(int device, prr_context** out);
+int prr_destroy(prr_context* ctx);
+const char* prr_last_error(const prr_context* ctx); /* ctx may be NULL */
+int prr_set_stream(prr_context* ctx, void* hip_stream); /* NULL = context's own stream */
+int prr_synchronize(prr_context* ctx);
+
+/* ---- scene: Scene::buildBVH (Scene.cpp:15-27) + upload ------------------ */
+/* Builds the reference's two-level binned-SAH BVH on the host (scene level
+ * maxPrims 1, mesh level maxPrims 4; BVH.cpp:13-280), merges it into one
+ * device tree, builds the emitter tables (Scene.cpp:19-25, BVH.cpp:478-514)
+ * and the environment CDF (EnvMap.hpp:189-224), and uploads everything. */
+int prr_scene_create(prr_context* ctx, const prr_scene_desc* desc, prr_scene** out);
+int prr_scene_free(prr_context* ctx, prr_scene* scene);
+int prr_scene_get_info(const prr_scene* scene, prr_scene_info* out);
+/* Host copy of the flattened reference-layout tree of one level, for tests:
+ * level -1 = scene BVH over objects, level k>=0 = sub-BVH of object k.
+ * Writes up to cap nodes (6 bounds floats [minX,maxX,minY,maxY,minZ,maxZ],
+ * first_child, meta as in BVH.hpp:34-67) and returns the node count. */
+int prr_scene_export_level(const prr_scene* scene, int32_t level, int32_t cap,
+                           float* bounds6, uint32_t* first_child, uint32_t* meta,
+                           int32_t* out_count);
+
+/* ---- traversal (BVHAccel::Intersect / IntersectP) ----------------------- */
+/* Closest hit of n rays (BVH.cpp:311-415 + Triangle.hpp:46-64 + Sphere.hpp:22-44).
+ * prim = global primitive id or -1; t, u, v = hit distance and barycentrics
+ * (0 for spheres). Rays may be unnormalized exactly as in the reference. */
+int prr_trace_closest(prr_context* ctx, const prr_scene* scene, int64_t n,
+                      const float* org, const float* dir,
+                      int32_t* prim, float* t, float* u, float* v);
+/* Any hit in (1e-3, tmax) (BVH.cpp:421-473, Triangle.hpp:75-92). */
+int prr_trace_any(prr_context* ctx, const prr_scene* scene, int64_t n,
+                  const float* org, const float* dir, const float* tmax, uint8_t* hit);
+
+/* ---- BSDF / environment / camera / tone map batch entry points ---------- */
+/* params: 8 floats per entry: type, base rgb, roughness, metallic, ior, pad.
+ * Shading frame is built from N alone (Material.hpp:80-84). */
+/* DiffuseMaterial::eval/pdf with LOBE_ALL (Material.cpp:80-158). */
+int prr_bsdf_eval(prr_context* ctx, int64_t n, const float* wi, const float* wo,
+                  const float* normal, const float* params, float* f, float* pdf);
+/* Material::sample with explicit uniforms u (3 per entry: u0,u1,u2 for
+ * DiffuseMaterial::sample Material.cpp:28-78, u0 for GlassMaterial::sample
+ * Material.cpp:169-191). lobe: 0 diffuse, 1 specular, 2 delta, -1 none. */
+int prr_bsdf_sample(prr_context* ctx, int64_t n, const float* wo, const float* normal,
+                    const float* params, const float* u, float* wi, float* f, float* pdf,
+                    int32_t* lobe);
+/* EnvMap::importanceSample with explicit uniforms (EnvMap.hpp:145-169). */
+int prr_env_sample(prr_context* ctx, const prr_scene* scene, int64_t n, const float* u2,
+                   float* dir, float* pdf);
+/* EnvMap::sample (bilinear radiance, :97-113) and importanceSamplePdf (:124-133). */
+int prr_env_eval(prr_context* ctx, const prr_scene* scene, int64_t n, const float* dir,
+                 float* radiance, float* pdf);
+/* Camera::generateRay (Camera.hpp:75-95) for pixel (i,j) with offsets (ox,oy);
+ * pix: 2 int32 per ray, offs: 2 floats per ray (+2 lens uniforms if aperture>0). */
+int prr_camera_rays(prr_context* ctx, const prr_scene* scene, int32_t width, int32_t height,
+                    int64_t n, const int32_t* pix, const float* offs, float* org, float* dir);
+/* exposure + sanitize (Renderer.cpp:149-161), tonemap::apply (ToneMapping.hpp:267-279)
+ * and 8-bit quantisation (Renderer.cpp:169-171). display or rgb8 may be NULL. */
+int prr_tonemap(prr_context* ctx, int32_t op, int64_t n, float exposure, const float* linear,
+                float* display, uint8_t* rgb8);
+
+/* ---- the render: Renderer::Render sample loop + Integrator::castRay ----- */
+/* Writes the per-pixel mean linear radiance (width*height*3 floats, device)
+ * of a fixed-spp render. Blocks until the render is done; stats may be NULL. */
+int prr_render(prr_context* ctx, const prr_scene* scene, const prr_render_params* params,
+               float* rgb_out, prr_stats* stats);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PRR_ABI_H */

@@ -1,0 +1,574 @@
+// prr_oracle.cpp — TEST INFRASTRUCTURE ONLY (see prr_oracle.h).
+//
+// C entry points over the CPU restatement in oracle_core.hpp /
+// oracle_scene.hpp. Two builds are made from this file (oracle/Makefile):
+//   liboracle.so       IEEE-strict (-O2 -ffp-contract=off): the parity checker
+//   liboracle_fast.so  the reference's release flags (-O3 -march=native
+//                      -ffast-math, OpenMP; CMakeLists.txt:15,41-52): the CPU
+//                      baseline timed beside the GPU.
+#include "prr_oracle.h"
+
+#include "oracle_scene.hpp"
+
+#include <atomic>
+#include <cstdlib>
+#include <unordered_map>
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+using namespace orc;
+
+struct orc_scene {
+    Scene sc;
+};
+
+static V3 v3(const float *p) { return V3(p[0], p[1], p[2]); }
+static void put3(float *p, const V3 &v) {
+    p[0] = v.x;
+    p[1] = v.y;
+    p[2] = v.z;
+}
+
+extern "C" int orc_version(void) { return 3; }
+
+extern "C" orc_scene *orc_scene_create(const prr_scene_desc *d) {
+    orc_scene *s = new orc_scene();
+    Scene &sc = s->sc;
+    sc.tris.resize(d->num_triangles);
+    for (int i = 0; i < d->num_triangles; i++) {
+        const float *t = d->triangles + 9 * (size_t) i;
+        sc.tris[i] = Tri(v3(t), v3(t + 3), v3(t + 6));
+    }
+    sc.sphs.resize(d->num_spheres);
+    for (int i = 0; i < d->num_spheres; i++) {
+        const float *p = d->spheres + 4 * (size_t) i;
+        Sph &sp = sc.sphs[i];
+        sp.c = v3(p);
+        sp.r = p[3];
+        sp.r2 = sp.r * sp.r;
+        sp.area = 4 * kPi * sp.r * sp.r;  // Sphere.hpp:16-17
+    }
+    sc.mats.resize(d->num_materials);
+    for (int i = 0; i < d->num_materials; i++) {
+        const prr_material_desc &m = d->materials[i];
+        Material &o = sc.mats[i];
+        o.type = m.type;
+        o.base = v3(m.base_color);
+        o.roughness = m.roughness;
+        o.metallic = m.metallic;
+        o.ior = m.ior;
+        o.emission = v3(m.emission);
+    }
+    sc.objs.resize(d->num_objects);
+    for (int k = 0; k < d->num_objects; k++) {
+        sc.objs[k].kind = d->objects[k].kind;
+        sc.objs[k].mat = d->objects[k].material;
+        sc.objs[k].first = d->objects[k].first;
+        sc.objs[k].count = d->objects[k].count;
+    }
+    sc.finalizeBuild();
+    if (d->env.width > 0 && d->env.height > 0 && d->env.pixels)
+        sc.env.build(d->env.width, d->env.height, d->env.pixels);
+    sc.background = v3(d->background);
+    for (int i = 0; i < d->num_point_lights; i++)
+        sc.points.push_back(PointLight{v3(d->point_lights[i].position), v3(d->point_lights[i].intensity)});
+    Camera &c = sc.camera;
+    c.eye = v3(d->camera.eye);
+    c.target = v3(d->camera.target);
+    c.up = v3(d->camera.up);
+    c.fovh = d->camera.fovh;
+    c.aperture = d->camera.aperture;
+    c.focusDistance = d->camera.focus_distance;
+    c.init();
+    return s;
+}
+
+extern "C" void orc_scene_free(orc_scene *s) { delete s; }
+
+extern "C" float orc_total_emit_area(const orc_scene *s) { return s->sc.totalEmitArea; }
+
+extern "C" int orc_export_level(const orc_scene *s, int level, int cap, float *b6, uint32_t *fc,
+                                uint32_t *meta) {
+    const BVH *bvh = nullptr;
+    if (level < 0)
+        bvh = &s->sc.top;
+    else if (level < (int) s->sc.objs.size() && s->sc.objs[level].kind == OBJ_MESH)
+        bvh = &s->sc.objs[level].sub;
+    if (!bvh) return -1;
+    int n = (int) bvh->nodes.size();
+    for (int i = 0; i < n && i < cap; i++) {
+        for (int k = 0; k < 6; k++) b6[6 * i + k] = bvh->nodes[i].b[k];
+        fc[i] = bvh->nodes[i].first;
+        meta[i] = bvh->nodes[i].meta;
+    }
+    return n;
+}
+
+extern "C" int orc_export_order(const orc_scene *s, int level, int cap, int32_t *order, float *areas) {
+    const BVH *bvh = nullptr;
+    int base = 0;
+    if (level < 0)
+        bvh = &s->sc.top;
+    else if (level < (int) s->sc.objs.size() && s->sc.objs[level].kind == OBJ_MESH) {
+        bvh = &s->sc.objs[level].sub;
+        base = s->sc.objs[level].first;
+    }
+    if (!bvh) return -1;
+    for (int i = 0; i < (int) bvh->ordered.size() && i < cap; i++) order[i] = bvh->ordered[i] - base;
+    for (int i = 0; i < (int) bvh->nodeAreas.size() && i < cap; i++) areas[i] = bvh->nodeAreas[i];
+    return (int) bvh->ordered.size();
+}
+
+// Scene::sampleLight with the reference's SplitMix64 stream from state seeds[k]
+extern "C" void orc_sample_light(const orc_scene *s, int64_t n, const uint64_t *seeds, float *pos,
+                                 float *nrm, float *pdf) {
+    for (int64_t k = 0; k < n; k++) {
+        SplitMix64 rng{seeds[k]};
+        V3 p, q;
+        int m;
+        float pd = 0.f;
+        if (!s->sc.sampleLight(rng, p, q, m, pd)) p = q = V3(0.f);
+        put3(pos + 3 * k, p);
+        put3(nrm + 3 * k, q);
+        pdf[k] = pd;
+    }
+}
+
+extern "C" void orc_trace_closest(const orc_scene *s, int64_t n, const float *org, const float *dir,
+                                  int32_t *prim, float *t, float *u, float *v, uint32_t *counters) {
+#pragma omp parallel for schedule(dynamic, 256)
+    for (int64_t i = 0; i < n; i++) {
+        Counters c;
+        Ray r{v3(org + 3 * i), v3(dir + 3 * i)};
+        Hit h = s->sc.intersect(r, &c);
+        prim[i] = h.happened ? h.prim : -1;
+        if (t) t[i] = h.happened ? h.t : kInf;
+        if (u) u[i] = h.u;
+        if (v) v[i] = h.v;
+        if (counters) {
+            counters[3 * i + 0] = (uint32_t) c.inner;
+            counters[3 * i + 1] = (uint32_t) c.tris;
+            counters[3 * i + 2] = (uint32_t) c.spheres;
+        }
+    }
+}
+
+extern "C" void orc_trace_any(const orc_scene *s, int64_t n, const float *org, const float *dir,
+                              const float *tmax, uint8_t *hit, uint32_t *counters) {
+#pragma omp parallel for schedule(dynamic, 256)
+    for (int64_t i = 0; i < n; i++) {
+        Counters c;
+        Ray r{v3(org + 3 * i), v3(dir + 3 * i)};
+        hit[i] = s->sc.intersectP(r, tmax[i], &c) ? 1 : 0;
+        if (counters) {
+            counters[3 * i + 0] = (uint32_t) c.inner;
+            counters[3 * i + 1] = (uint32_t) c.tris;
+            counters[3 * i + 2] = (uint32_t) c.spheres;
+        }
+    }
+}
+
+// Brute force: minimum t over all primitives (ties: lowest id), independent
+// of any tree. Uses the same primitive tests as the traversal.
+extern "C" void orc_brute_closest(const orc_scene *s, int64_t n, const float *org, const float *dir,
+                                  int32_t *prim, float *t) {
+    const Scene &sc = s->sc;
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int64_t i = 0; i < n; i++) {
+        Ray r{v3(org + 3 * i), v3(dir + 3 * i)};
+        float best = kInf;
+        int bp = -1;
+        for (int k = 0; k < (int) sc.tris.size(); k++) {
+            float tt, uu, vv;
+            if (triHit(sc.tris[k], r, tt, uu, vv) && tt < best) {
+                best = tt;
+                bp = k;
+            }
+        }
+        for (int k = 0; k < (int) sc.sphs.size(); k++) {
+            float tt;
+            if (sphHit(sc.sphs[k], r, tt) && tt < best) {
+                best = tt;
+                bp = sc.numTris() + k;
+            }
+        }
+        prim[i] = bp;
+        t[i] = best;
+    }
+}
+
+extern "C" void orc_brute_any(const orc_scene *s, int64_t n, const float *org, const float *dir,
+                              const float *tmax, uint8_t *hit) {
+    const Scene &sc = s->sc;
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int64_t i = 0; i < n; i++) {
+        Ray r{v3(org + 3 * i), v3(dir + 3 * i)};
+        bool occ = false;
+        for (int k = 0; k < (int) sc.tris.size() && !occ; k++) {
+            float tt = triAnyT(sc.tris[k], r, tmax[i]);
+            occ = tt >= 0.f && tt < tmax[i];
+        }
+        for (int k = 0; k < (int) sc.sphs.size() && !occ; k++) {
+            float tt;
+            occ = sphHit(sc.sphs[k], r, tt) && tt < tmax[i];
+        }
+        hit[i] = occ ? 1 : 0;
+    }
+}
+
+static Material paramMat(const float *p) {
+    Material m;
+    m.type = (int) p[0];
+    m.base = V3(p[1], p[2], p[3]);
+    m.roughness = p[4];
+    m.metallic = p[5];
+    m.ior = p[6];
+    return m;
+}
+
+extern "C" void orc_bsdf_eval(int64_t n, const float *wi, const float *wo, const float *nrm,
+                              const float *params, float *f, float *pdf) {
+    for (int64_t i = 0; i < n; i++) {
+        Material m = paramMat(params + 8 * i);
+        V3 fv(0.f);
+        float p = 0.f;
+        if (m.type == MAT_DIFFUSE) {
+            Shading sd = makeShading(m, v3(nrm + 3 * i));
+            fv = diffuseEval(v3(wi + 3 * i), v3(wo + 3 * i), sd);
+            p = diffusePdf(v3(wi + 3 * i), v3(wo + 3 * i), sd);
+        }
+        put3(f + 3 * i, fv);
+        pdf[i] = p;
+    }
+}
+
+extern "C" void orc_bsdf_sample(int64_t n, const float *wo, const float *nrm, const float *params,
+                                const float *u3, float *wi, float *f, float *pdf, int32_t *lobe) {
+    for (int64_t i = 0; i < n; i++) {
+        Material m = paramMat(params + 8 * i);
+        FixedU rng{u3 + 3 * i};
+        BSDFSample bs;
+        V3 N = v3(nrm + 3 * i), w = v3(wo + 3 * i);
+        if (m.type == MAT_DIFFUSE)
+            bs = diffuseSample(w, makeShading(m, N), rng);
+        else if (m.type == MAT_GLASS)
+            bs = glassSample(w, N, m.ior, rng);
+        else if (m.type == MAT_MIRROR)
+            bs = mirrorSample(w, N, m.base);
+        put3(wi + 3 * i, bs.wi);
+        put3(f + 3 * i, bs.f);
+        pdf[i] = bs.pdf;
+        lobe[i] = bs.lobe;
+    }
+}
+
+extern "C" void orc_ggx_terms(int64_t n, const float *in4, float *out6) {
+    for (int64_t i = 0; i < n; i++) {
+        float NdotH = in4[4 * i], NdotWo = in4[4 * i + 1], NdotWi = in4[4 * i + 2], alpha = in4[4 * i + 3];
+        out6[6 * i + 0] = D_GGX(NdotH, alpha);
+        out6[6 * i + 1] = lambdaGGX(NdotWo, alpha);
+        out6[6 * i + 2] = G1_GGX(NdotWo, alpha);
+        out6[6 * i + 3] = G2_GGX(NdotWo, NdotWi, alpha);
+        out6[6 * i + 4] = pdfVNDF(NdotWo, NdotH, NdotWo, alpha);
+        out6[6 * i + 5] = F_Schlick(NdotWo, V3(0.04f)).x;
+    }
+}
+
+extern "C" void orc_env_build(int w, int h, const float *rgb, float *cdf, float *marg, float *total) {
+    Env e;
+    e.build(w, h, rgb);
+    std::memcpy(cdf, e.cdf.data(), sizeof(float) * (size_t) w * h);
+    std::memcpy(marg, e.marg.data(), sizeof(float) * (size_t) h);
+    *total = e.total;
+}
+
+extern "C" void orc_env_sample(int w, int h, const float *rgb, int64_t n, const float *u2, float *dir,
+                               float *pdf) {
+    Env e;
+    e.build(w, h, rgb);
+    for (int64_t i = 0; i < n; i++) {
+        float p;
+        V3 d = e.importanceSample(u2[2 * i], u2[2 * i + 1], p);
+        put3(dir + 3 * i, d);
+        pdf[i] = p;
+    }
+}
+
+extern "C" void orc_env_eval(int w, int h, const float *rgb, int64_t n, const float *dir, float *rad,
+                             float *pdf) {
+    Env e;
+    e.build(w, h, rgb);
+    for (int64_t i = 0; i < n; i++) {
+        V3 d = v3(dir + 3 * i);
+        put3(rad + 3 * i, e.sample(d));
+        pdf[i] = e.pdf(d);
+    }
+}
+
+extern "C" void orc_camera_rays(const prr_camera_desc *cd, int width, int height, int64_t n,
+                                const int32_t *pix, const float *offs, float *org, float *dir) {
+    Camera c;
+    c.eye = v3(cd->eye);
+    c.target = v3(cd->target);
+    c.up = v3(cd->up);
+    c.fovh = cd->fovh;
+    c.aperture = cd->aperture;
+    c.focusDistance = cd->focus_distance;
+    c.init();
+    const int stride = c.aperture > 0.f ? 4 : 2;
+    for (int64_t i = 0; i < n; i++) {
+        FixedU lens{offs + stride * i + 2};
+        Ray r = c.generateRay(pix[2 * i], pix[2 * i + 1], width, height, offs[stride * i],
+                              offs[stride * i + 1], lens);
+        put3(org + 3 * i, r.o);
+        put3(dir + 3 * i, r.d);
+    }
+}
+
+extern "C" void orc_tonemap(int op, int64_t n, float exposure, const float *lin, float *disp,
+                            uint8_t *rgb8) {
+    for (int64_t i = 0; i < n; i++) {
+        V3 c = tonemap(op, exposeLinear(v3(lin + 3 * i), exposure));
+        if (disp) put3(disp + 3 * i, c);
+        if (rgb8) {
+            rgb8[3 * i + 0] = quantize(c.x);
+            rgb8[3 * i + 1] = quantize(c.y);
+            rgb8[3 * i + 2] = quantize(c.z);
+        }
+    }
+}
+
+extern "C" void orc_castray_splitmix(const orc_scene *s, int width, int height, int64_t n,
+                                     const int32_t *pix, const uint64_t *seeds, int max_depth,
+                                     float indirect_clamp, float *L) {
+    IntegratorParams ip;
+    ip.maxDepth = max_depth;
+    ip.indirectClamp = indirect_clamp;
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int64_t k = 0; k < n; k++) {
+        SplitMix64 rng{seeds[k]};
+        float ox = rng.next();
+        float oy = rng.next();
+        Ray r = s->sc.camera.generateRay(pix[2 * k], pix[2 * k + 1], width, height, ox, oy, rng);
+        V3 c = castRay(s->sc, r, rng, ip, nullptr);
+        if (std::isnan(c.x) || std::isnan(c.y) || std::isnan(c.z)) c = V3(0.f);  // Renderer.cpp:80-83
+        put3(L + 3 * k, c);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Caustic photon map (Photon.cpp:96-188 restated with the build's documented
+// changes: point-light emission, glass handled with the outward normal as in
+// castRay's delta branch (the reference flips it, Photon.cpp:105-119), a
+// photon is stored at the first diffuse hit after >=1 delta bounce and then
+// terminated; storage is a uniform hash grid of cell 2r).
+// ---------------------------------------------------------------------------
+struct Photon {
+    V3 pos, dir, power;
+};
+struct orc_photon_map {
+    std::vector<Photon> ph;
+    float r = 0.01f, cell = 0.02f;
+    std::unordered_map<uint64_t, std::vector<int>> grid;
+    static uint64_t key(int x, int y, int z) {
+        return ((uint64_t) (uint32_t) x * 73856093ULL) ^ ((uint64_t) (uint32_t) y * 19349663ULL) ^
+               ((uint64_t) (uint32_t) z * 83492791ULL) ^ ((uint64_t) (uint32_t) x << 40) ^
+               ((uint64_t) (uint32_t) y << 20);
+    }
+    V3 irradiance(const V3 &x) const {
+        V3 sum(0.f);
+        int cx = (int) std::floor(x.x / cell), cy = (int) std::floor(x.y / cell),
+            cz = (int) std::floor(x.z / cell);
+        for (int dz = -1; dz <= 1; dz++)
+            for (int dy = -1; dy <= 1; dy++)
+                for (int dx = -1; dx <= 1; dx++) {
+                    auto it = grid.find(key(cx + dx, cy + dy, cz + dz));
+                    if (it == grid.end()) continue;
+                    for (int i : it->second) {
+                        const Photon &p = ph[i];
+                        if (norm(p.pos - x) <= r) sum += p.power;
+                    }
+                }
+        return sum / (kPi * r * r);
+    }
+    V3 radiance(const V3 &x, const V3 &wo, const Shading &sd) const {
+        V3 sum(0.f);
+        int cx = (int) std::floor(x.x / cell), cy = (int) std::floor(x.y / cell),
+            cz = (int) std::floor(x.z / cell);
+        for (int dz = -1; dz <= 1; dz++)
+            for (int dy = -1; dy <= 1; dy++)
+                for (int dx = -1; dx <= 1; dx++) {
+                    auto it = grid.find(key(cx + dx, cy + dy, cz + dz));
+                    if (it == grid.end()) continue;
+                    for (int i : it->second) {
+                        const Photon &p = ph[i];
+                        if (norm(p.pos - x) > r) continue;
+                        V3 wi = -p.dir;
+                        float c = dot(wi, sd.N);
+                        if (c <= 1e-4f) continue;
+                        sum += diffuseEval(wi, wo, sd) * p.power / c;
+                    }
+                }
+        return sum / (kPi * r * r);
+    }
+};
+
+template <class R>
+static bool tracePhoton(const Scene &sc, Ray ray, V3 power, int maxDepth, R &rng, Photon &out) {
+    bool caustic = false;
+    for (int depth = 0; depth <= maxDepth; depth++) {
+        Hit h = sc.intersect(ray, nullptr);
+        if (!h.happened) return false;
+        const Material &m = sc.mats[sc.objs[h.obj].mat];
+        if (m.isEmissive()) return false;
+        V3 N = h.normal;
+        if (m.isDelta()) {
+            BSDFSample bs = (m.type == MAT_GLASS) ? glassSample(-ray.d, N, m.ior, rng)
+                                                   : mirrorSample(-ray.d, N, m.base);
+            power = power * bs.f / bs.pdf;
+            bool trans = dot(bs.wi, N) < 0.f;
+            if (trans) power = power * m.base;
+            ray = Ray{h.coords + (trans ? -N : N) * kEps, bs.wi};
+            caustic = true;
+            continue;
+        }
+        if (caustic && depth > 0) {
+            out = Photon{h.coords, ray.d, power};
+            return true;
+        }
+        return false;
+    }
+    return false;
+}
+
+extern "C" orc_photon_map *orc_photon_build(const orc_scene *s, int64_t num, float radius, int maxDepth,
+                                            uint64_t seed, int threads) {
+    const Scene &sc = s->sc;
+    orc_photon_map *pm = new orc_photon_map();
+    pm->r = radius;
+    pm->cell = 2.f * radius;
+    if (sc.points.empty() || num <= 0) return pm;
+    std::vector<Photon> tmp(num);
+    std::vector<uint8_t> ok(num, 0);
+    const int nl = (int) sc.points.size();
+#ifdef _OPENMP
+    if (threads > 0) omp_set_num_threads(threads);
+#endif
+#pragma omp parallel for schedule(dynamic, 1024)
+    for (int64_t i = 0; i < num; i++) {
+        Pcg32 rng = Pcg32::make(seed, (uint32_t) (i & 0xffffffffu), 0x7000000u + (uint32_t) (i >> 32));
+        const PointLight &pl = sc.points[(int) (i % nl)];
+        float u1 = rng.next(), u2 = rng.next();
+        float z = 1.f - 2.f * u1;
+        float rr = std::sqrt(std::max(0.f, 1.f - z * z));
+        float phi = 2.f * kPi * u2;
+        V3 d(rr * std::cos(phi), z, rr * std::sin(phi));
+        // power per photon: 4*pi*I / (photons per light)
+        float perLight = (float) ((num + nl - 1 - (i % nl)) / nl);
+        V3 power = pl.intensity * (4.f * kPi / perLight);
+        Photon p;
+        if (tracePhoton(sc, Ray{pl.pos, d}, power, maxDepth, rng, p)) {
+            tmp[i] = p;
+            ok[i] = 1;
+        }
+    }
+    for (int64_t i = 0; i < num; i++)
+        if (ok[i]) pm->ph.push_back(tmp[i]);
+    for (int i = 0; i < (int) pm->ph.size(); i++) {
+        const V3 &x = pm->ph[i].pos;
+        int cx = (int) std::floor(x.x / pm->cell), cy = (int) std::floor(x.y / pm->cell),
+            cz = (int) std::floor(x.z / pm->cell);
+        pm->grid[orc_photon_map::key(cx, cy, cz)].push_back(i);
+    }
+    return pm;
+}
+
+extern "C" void orc_photon_free(orc_photon_map *pm) { delete pm; }
+extern "C" int64_t orc_photon_count(const orc_photon_map *pm) { return (int64_t) pm->ph.size(); }
+extern "C" int64_t orc_photon_export(const orc_photon_map *pm, int64_t cap, float *o) {
+    int64_t n = std::min<int64_t>(cap, (int64_t) pm->ph.size());
+    for (int64_t i = 0; i < n; i++) {
+        put3(o + 9 * i, pm->ph[i].pos);
+        put3(o + 9 * i + 3, pm->ph[i].dir);
+        put3(o + 9 * i + 6, pm->ph[i].power);
+    }
+    return (int64_t) pm->ph.size();
+}
+extern "C" void orc_photon_irradiance(const orc_photon_map *pm, int64_t n, const float *pos, float *o) {
+    for (int64_t i = 0; i < n; i++) put3(o + 3 * i, pm->irradiance(v3(pos + 3 * i)));
+}
+
+// ---------------------------------------------------------------------------
+// Renderer::Render sample loop (Renderer.cpp:63-99) at fixed spp, PCG32 per
+// (pixel, sample), accumulated in sample order.
+// ---------------------------------------------------------------------------
+extern "C" void orc_render(const orc_scene *s, const prr_render_params *p, int row0, int row1,
+                           const orc_photon_map *pm, int threads, float *mean, float *var,
+                           uint64_t *counters) {
+    const Scene &sc = s->sc;
+    const int W = p->width, H = p->height;
+    if (row1 <= 0 || row1 > H) row1 = H;
+    IntegratorParams ip;
+    ip.maxDepth = p->max_depth;
+    ip.indirectClamp = p->indirect_clamp;
+    PhotonFn pf;
+    const PhotonFn *pfp = nullptr;
+    if (pm && !pm->ph.empty()) {
+        pf = [pm](const V3 &x, const V3 &wo, const Shading &sd) { return pm->radiance(x, wo, sd); };
+        pfp = &pf;
+    }
+    std::atomic<uint64_t> cClosest(0), cAny(0), cSamples(0), cInner(0), cTris(0);
+#ifdef _OPENMP
+    if (threads > 0) omp_set_num_threads(threads);
+#endif
+#pragma omp parallel
+    {
+        Counters cnt;
+        uint64_t samples = 0;
+#pragma omp for schedule(dynamic, 1)
+        for (int j = row0; j < row1; j++) {
+            for (int i = 0; i < W; i++) {
+                const uint32_t pix = (uint32_t) (j * W + i);
+                V3 sum(0.f);
+                double m1[3] = {0, 0, 0}, m2[3] = {0, 0, 0};
+                for (int sIdx = 0; sIdx < p->spp; sIdx++) {
+                    Pcg32 rng = Pcg32::make(p->seed, pix, (uint32_t) (p->sample_offset + sIdx));
+                    float ox = rng.next();
+                    float oy = rng.next();
+                    Ray r = sc.camera.generateRay(i, j, W, H, ox, oy, rng);
+                    V3 c = castRay(sc, r, rng, ip, &cnt, pfp);
+                    if (std::isnan(c.x) || std::isnan(c.y) || std::isnan(c.z)) c = V3(0.f);
+                    sum += c;
+                    samples++;
+                    if (var) {
+                        double cv[3] = {c.x, c.y, c.z};
+                        for (int k = 0; k < 3; k++) {
+                            double d = cv[k] - m1[k];
+                            m1[k] += d / (sIdx + 1);
+                            m2[k] += d * (cv[k] - m1[k]);
+                        }
+                    }
+                }
+                V3 mv = sum / (float) p->spp;
+                if (mean) put3(mean + 3 * (size_t) pix, mv);
+                if (var)
+                    for (int k = 0; k < 3; k++)
+                        var[3 * (size_t) pix + k] = p->spp > 1 ? (float) (m2[k] / (p->spp - 1)) : 0.f;
+            }
+        }
+        cClosest += cnt.closest;
+        cAny += cnt.any;
+        cSamples += samples;
+        cInner += cnt.inner;
+        cTris += cnt.tris;
+    }
+    if (counters) {
+        counters[0] = cClosest;
+        counters[1] = cAny;
+        counters[2] = cSamples;
+        counters[3] = cInner;
+        counters[4] = cTris;
+    }
+}
