@@ -138,8 +138,12 @@ typedef struct prr_render_params {
     int64_t num_photons;
     float photon_radius;
     int32_t photon_max_depth;
-    int32_t reserved[4];
+    int32_t flags;             /* PRR_RENDER_* bits */
+    int32_t reserved[3];
 } prr_render_params;
+
+/* prr_render_params::flags */
+#define PRR_RENDER_KERNEL_TIMING 1 /* time every kernel launch with HIP events (prr_stats::ms_*) */
 
 typedef struct prr_stats {
     uint64_t rays_closest;     /* scene-level closest-hit queries */
@@ -147,6 +151,12 @@ typedef struct prr_stats {
     uint64_t samples;          /* castRay-equivalents */
     uint64_t photons_stored;   /* caustic photons stored */
     double ms_total;           /* device time of the render, ms */
+    /* with PRR_RENDER_KERNEL_TIMING: summed launch durations per kernel */
+    double ms_extend;          /* closest-hit traversal (extension rays) */
+    double ms_shadow;          /* any-hit traversal (shadow rays) */
+    double ms_shade;           /* shading / NEE / BSDF sampling */
+    int64_t launches_extend;
+    int64_t launches_shadow;
 } prr_stats;
 
 typedef struct prr_scene_info {

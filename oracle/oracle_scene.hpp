@@ -42,6 +42,8 @@ struct Counters {
     uint64_t spheres = 0;  // sphere tests
     uint64_t closest = 0;  // scene-level closest-hit queries
     uint64_t any = 0;      // scene-level any-hit queries
+    uint64_t innerAny = 0; // interior expansions of any-hit queries
+    uint64_t trisAny = 0;  // triangle tests of any-hit queries
     void add(const Counters &o) {
         inner += o.inner;
         tris += o.tris;
@@ -411,7 +413,7 @@ struct BVH {
                     for (uint32_t i = 0; i < n.count(); i++)
                         if (anyFn(ordered[n.first + i])) return true;
                 } else {
-                    if (cnt) cnt->inner++;
+                    if (cnt) cnt->innerAny++;
                     uint32_t left = ni + 1, right = n.first, nearC, farC;
                     if (rd.oct[n.axis()] == 0) {
                         nearC = left;
@@ -611,7 +613,7 @@ struct Scene {
                     bool occ = o.sub.intersectP(
                         r, tMax,
                         [&](int ti) {
-                            if (cnt) cnt->tris++;
+                            if (cnt) cnt->trisAny++;
                             float tt = triAnyT(tris[ti], r, tMax);
                             return tt >= 0.f && tt < tMax;
                         },

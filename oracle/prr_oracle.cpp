@@ -163,8 +163,8 @@ extern "C" void orc_trace_any(const orc_scene *s, int64_t n, const float *org, c
         Ray r{v3(org + 3 * i), v3(dir + 3 * i)};
         hit[i] = s->sc.intersectP(r, tmax[i], &c) ? 1 : 0;
         if (counters) {
-            counters[3 * i + 0] = (uint32_t) c.inner;
-            counters[3 * i + 1] = (uint32_t) c.tris;
+            counters[3 * i + 0] = (uint32_t) c.innerAny;
+            counters[3 * i + 1] = (uint32_t) c.trisAny;
             counters[3 * i + 2] = (uint32_t) c.spheres;
         }
     }
@@ -519,7 +519,7 @@ extern "C" void orc_render(const orc_scene *s, const prr_render_params *p, int r
         pf = [pm](const V3 &x, const V3 &wo, const Shading &sd) { return pm->radiance(x, wo, sd); };
         pfp = &pf;
     }
-    std::atomic<uint64_t> cClosest(0), cAny(0), cSamples(0), cInner(0), cTris(0);
+    std::atomic<uint64_t> cClosest(0), cAny(0), cSamples(0), cInner(0), cTris(0), cInnerAny(0), cTrisAny(0);
 #ifdef _OPENMP
     if (threads > 0) omp_set_num_threads(threads);
 #endif
@@ -563,6 +563,8 @@ extern "C" void orc_render(const orc_scene *s, const prr_render_params *p, int r
         cSamples += samples;
         cInner += cnt.inner;
         cTris += cnt.tris;
+        cInnerAny += cnt.innerAny;
+        cTrisAny += cnt.trisAny;
     }
     if (counters) {
         counters[0] = cClosest;
@@ -570,5 +572,7 @@ extern "C" void orc_render(const orc_scene *s, const prr_render_params *p, int r
         counters[2] = cSamples;
         counters[3] = cInner;
         counters[4] = cTris;
+        counters[5] = cInnerAny;
+        counters[6] = cTrisAny;
     }
 }

@@ -88,7 +88,9 @@ void orc_castray_splitmix(const orc_scene* s, int width, int height, int64_t n,
 /* Fixed-spp render with PCG32 per (pixel, sample) (the GPU renderer's RNG).
  * mean: W*H*3 (sum in sample order / spp), var: W*H*3 sample variance of the
  * per-sample radiance (may be NULL). rows [row0,row1) only (row1<=0: all).
- * counters (may be NULL): closest, any, samples, inner, tris. threads: OpenMP
+ * counters (may be NULL, 7 entries): closest queries, any-hit queries, samples,
+ * closest-hit interior expansions, closest-hit triangle tests, any-hit
+ * interior expansions, any-hit triangle tests. threads: OpenMP
  * threads (<=0: default). Photon estimate if photon_map != NULL. */
 typedef struct orc_photon_map orc_photon_map;
 void orc_render(const orc_scene* s, const prr_render_params* p, int row0, int row1,

@@ -49,7 +49,7 @@ class RenderParams(C.Structure):
     _fields_ = [("width", C.c_int32), ("height", C.c_int32), ("spp", C.c_int32), ("max_depth", C.c_int32),
                 ("indirect_clamp", C.c_float), ("sample_offset", C.c_int32), ("seed", C.c_uint64),
                 ("num_photons", C.c_int64), ("photon_radius", C.c_float), ("photon_max_depth", C.c_int32),
-                ("reserved", C.c_int32 * 4)]
+                ("flags", C.c_int32), ("reserved", C.c_int32 * 3)]
 
 
 def _f(a):
@@ -250,11 +250,12 @@ class OracleScene:
                          sample_offset=sample_offset, seed=seed)
         mean = np.zeros((height, width, 3), np.float32)
         var = np.zeros((height, width, 3), np.float32) if want_var else None
-        cnt = np.zeros(5, np.uint64)
+        cnt = np.zeros(7, np.uint64)
         r0, r1 = rows if rows else (0, 0)
         self.L.orc_render(self.h, C.byref(p), r0, r1, photon_map.h if photon_map else None, threads,
                           _p(mean), _p(var), _p(cnt))
-        return mean, var, dict(zip(["closest", "any", "samples", "inner", "tris"], (int(x) for x in cnt)))
+        keys = ["closest", "any", "samples", "inner", "tris", "inner_any", "tris_any"]
+        return mean, var, dict(zip(keys, (int(x) for x in cnt)))
 
 
 class OraclePhotonMap:

@@ -1,0 +1,502 @@
+// abi.hip — the C ABI of libprr.so (include/prr_abi.h).
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/prr_abi.h"
+#include "device/dev_traverse.hpp"
+#include "device/launch.hpp"
+#include "device/launch_render.hpp"
+#include "host/scene_build.hpp"
+
+using namespace prr;
+
+static thread_local std::string g_lastError = "";
+
+struct prr_context {
+    int device = 0;
+    hipStream_t own = nullptr;
+    hipStream_t stream = nullptr;
+    std::string err;
+    int numCU = 1;
+    int traceBlocks = 1, shadeBlocks = 1;
+    uint2 *spill = nullptr;
+    int spillLanes = 0;
+    // render buffers
+    PathBufs pb{};
+    int64_t pbSlots = 0;
+    std::vector<void *> pbAllocs;
+    int maxDepthAlloc = 0;
+    float4 *fb = nullptr;
+    int64_t fbPixels = 0;
+    unsigned long long *stats = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::vector<hipEvent_t> evPool;  // per-launch timing events (PRR_RENDER_KERNEL_TIMING)
+};
+
+struct prr_scene {
+    host::HostScene hs;
+    SceneView view{};
+    std::vector<void *> allocs;
+    int64_t bytes = 0;
+    AgxConst agx{};
+};
+
+static int fail(prr_context *c, int code, const std::string &msg) {
+    if (c) c->err = msg;
+    g_lastError = msg;
+    return code;
+}
+#define HIPCK(ctx, expr)                                                                          \
+    do {                                                                                          \
+        hipError_t e_ = (expr);                                                                   \
+        if (e_ != hipSuccess)                                                                     \
+            return fail(ctx, PRR_ERR_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+template <class T>
+static hipError_t upload(prr_scene *s, T **dst, const std::vector<T> &src, hipStream_t st) {
+    const size_t bytes = std::max<size_t>(sizeof(T), src.size() * sizeof(T));
+    void *p = nullptr;
+    hipError_t e = hipMalloc(&p, bytes);
+    if (e != hipSuccess) return e;
+    s->allocs.push_back(p);
+    s->bytes += (int64_t) bytes;
+    *dst = (T *) p;
+    if (!src.empty()) e = hipMemcpyAsync(p, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice, st);
+    return e;
+}
+
+// AllenwpParams(1.25, 16, 1) (ToneMapping.hpp:125-147), evaluated on the host.
+static AgxConst agxConst() {
+    const float crossover = 0.1841865f, contrast = 1.25f, outMax = 1.0f;
+    const float high_clip = std::max(16.0f, outMax);
+    AgxConst p;
+    p.toe_a = ((1.f / crossover) - 1.f) * std::pow(crossover, contrast);
+    const float denom = std::pow(crossover, contrast) + p.toe_a;
+    p.slope = (contrast * std::pow(crossover, contrast - 1.f) * p.toe_a) / (denom * denom);
+    p.shoulder_max = outMax - crossover;
+    float w = high_clip - crossover;
+    w = w * w;
+    w = w / p.shoulder_max;
+    w = w * p.slope;
+    p.w = w;
+    return p;
+}
+
+extern "C" {
+
+const char *prr_last_error(const prr_context *ctx) { return ctx ? ctx->err.c_str() : g_lastError.c_str(); }
+
+int prr_create(int device, prr_context **out) {
+    if (!out) return fail(nullptr, PRR_ERR_INVALID, "out is NULL");
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return fail(nullptr, PRR_ERR_DEVICE, "no HIP device");
+    if (device < 0 || device >= n) return fail(nullptr, PRR_ERR_INVALID, "device index out of range");
+    prr_context *c = new (std::nothrow) prr_context();
+    if (!c) return fail(nullptr, PRR_ERR_NOMEM, "out of host memory");
+    c->device = device;
+    hipError_t e = hipSetDevice(device);
+    hipDeviceProp_t prop;
+    if (e == hipSuccess) e = hipGetDeviceProperties(&prop, device);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->own, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreate(&c->ev0);
+    if (e == hipSuccess) e = hipEventCreate(&c->ev1);
+    if (e != hipSuccess) {
+        std::string m = std::string("device init: ") + hipGetErrorString(e);
+        delete c;
+        return fail(nullptr, PRR_ERR_DEVICE, m);
+    }
+    c->stream = c->own;
+    c->numCU = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 1;
+    c->traceBlocks = c->numCU * 4;
+    c->shadeBlocks = c->numCU * 8;
+    c->spillLanes = c->traceBlocks * kTraceBlock;
+    e = hipMalloc(&c->spill, (size_t) c->spillLanes * (kSpill - kShort) * sizeof(uint2));
+    if (e == hipSuccess) e = hipMalloc(&c->stats, 8 * sizeof(unsigned long long));
+    if (e != hipSuccess) {
+        std::string m = std::string("device alloc: ") + hipGetErrorString(e);
+        delete c;
+        return fail(nullptr, PRR_ERR_NOMEM, m);
+    }
+    *out = c;
+    return PRR_OK;
+}
+
+static void freeRenderBufs(prr_context *c) {
+    for (void *p : c->pbAllocs) (void) hipFree(p);
+    c->pbAllocs.clear();
+    c->pbSlots = 0;
+    c->maxDepthAlloc = 0;
+}
+
+int prr_destroy(prr_context *c) {
+    if (!c) return PRR_OK;
+    (void) hipSetDevice(c->device);
+    (void) hipStreamSynchronize(c->stream);
+    freeRenderBufs(c);
+    if (c->fb) (void) hipFree(c->fb);
+    if (c->spill) (void) hipFree(c->spill);
+    if (c->stats) (void) hipFree(c->stats);
+    if (c->ev0) (void) hipEventDestroy(c->ev0);
+    if (c->ev1) (void) hipEventDestroy(c->ev1);
+    for (hipEvent_t e : c->evPool) (void) hipEventDestroy(e);
+    if (c->own) (void) hipStreamDestroy(c->own);
+    delete c;
+    return PRR_OK;
+}
+
+int prr_set_stream(prr_context *c, void *stream) {
+    if (!c) return fail(nullptr, PRR_ERR_INVALID, "context is NULL");
+    c->stream = stream ? (hipStream_t) stream : c->own;
+    return PRR_OK;
+}
+
+int prr_synchronize(prr_context *c) {
+    if (!c) return fail(nullptr, PRR_ERR_INVALID, "context is NULL");
+    HIPCK(c, hipStreamSynchronize(c->stream));
+    return PRR_OK;
+}
+
+int prr_scene_create(prr_context *c, const prr_scene_desc *d, prr_scene **out) {
+    if (!c || !out) return fail(c, PRR_ERR_INVALID, "NULL argument");
+    *out = nullptr;
+    prr_scene *s = new (std::nothrow) prr_scene();
+    if (!s) return fail(c, PRR_ERR_NOMEM, "out of host memory");
+    std::string err;
+    if (!host::buildHostScene(d, s->hs, err)) {
+        delete s;
+        return fail(c, PRR_ERR_INVALID, err);
+    }
+    HIPCK(c, hipSetDevice(c->device));
+    const host::HostScene &h = s->hs;
+    SceneView &v = s->view;
+    hipError_t e = hipSuccess;
+#define UP(dst, src)                                                               \
+    if (e == hipSuccess) e = upload(s, (std::remove_const<std::remove_reference<decltype(*dst)>::type>::type **) &dst, src, c->stream)
+    UP(v.nodes, h.nodes);
+    UP(v.tris, h.tris);
+    UP(v.sph, h.sph);
+    UP(v.sph_id, h.sphId);
+    UP(v.prim_n, h.primN);
+    UP(v.sph_by_id, h.sphById);
+    UP(v.mats, h.mats);
+    UP(v.emitters, h.emitters);
+    UP(v.lnodes, h.lnodes);
+    UP(v.ltri, h.ltri);
+    UP(v.env_px, h.envPx);
+    UP(v.env_cdf, h.envCdf);
+    UP(v.env_marg, h.envMarg);
+#undef UP
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) {
+        for (void *p : s->allocs) (void) hipFree(p);
+        delete s;
+        return fail(c, PRR_ERR_NOMEM, std::string("scene upload: ") + hipGetErrorString(e));
+    }
+    v.root = h.root;
+    v.num_tris = h.numTris;
+    v.num_emitters = h.totalEmitArea > 0.f ? (int) h.emitters.size() : 0;
+    v.total_emit_area = h.totalEmitArea;
+    for (int k = 0; k < 6; k++) v.root_box[k] = h.rootBox[k];
+    v.env_w = h.envW;
+    v.env_h = h.envH;
+    v.env_total = h.envTotal;
+    for (int k = 0; k < 3; k++) {
+        v.background[k] = h.background[k];
+        v.point_pos[k] = h.pointPos[k];
+        v.point_int[k] = h.pointInt[k];
+        v.cam_eye[k] = h.camEye[k];
+        v.cam_fwd[k] = h.camFwd[k];
+        v.cam_right[k] = h.camRight[k];
+        v.cam_up[k] = h.camUp[k];
+    }
+    v.num_points = h.numPoints;
+    v.cam_aperture = h.camAperture;
+    v.cam_focus = h.camFocus;
+    v.cam_tanhalf = h.camTanHalf;
+    s->agx = agxConst();
+    *out = s;
+    return PRR_OK;
+}
+
+int prr_scene_free(prr_context *c, prr_scene *s) {
+    if (!s) return PRR_OK;
+    if (c) {
+        (void) hipSetDevice(c->device);
+        (void) hipStreamSynchronize(c->stream);
+    }
+    for (void *p : s->allocs) (void) hipFree(p);
+    delete s;
+    return PRR_OK;
+}
+
+int prr_scene_get_info(const prr_scene *s, prr_scene_info *o) {
+    if (!s || !o) return fail(nullptr, PRR_ERR_INVALID, "NULL argument");
+    const host::HostScene &h = s->hs;
+    o->num_nodes = (int32_t) h.nodes.size();
+    o->num_leaf_prims = (int32_t) (h.tris.size() / 3);
+    o->max_depth = h.maxDepth;
+    o->num_emitters = h.totalEmitArea > 0.f ? (int32_t) h.emitters.size() : 0;
+    o->total_emit_area = h.totalEmitArea;
+    for (int a = 0; a < 3; a++) {
+        o->bounds[a] = h.rootBox[2 * a];
+        o->bounds[3 + a] = h.rootBox[2 * a + 1];
+    }
+    o->device_bytes = s->bytes;
+    return PRR_OK;
+}
+
+int prr_scene_export_level(const prr_scene *s, int32_t level, int32_t cap, float *b6, uint32_t *fc, uint32_t *meta,
+                           int32_t *count) {
+    if (!s || !count) return fail(nullptr, PRR_ERR_INVALID, "NULL argument");
+    const host::RefBVH *b = nullptr;
+    if (level < 0)
+        b = &s->hs.top;
+    else if (level < (int32_t) s->hs.meshes.size() && !s->hs.meshes[level].nodes.empty())
+        b = &s->hs.meshes[level];
+    if (!b) return fail(nullptr, PRR_ERR_INVALID, "no BVH at this level");
+    const int n = (int) b->nodes.size();
+    *count = n;
+    for (int i = 0; i < n && i < cap; i++) {
+        if (b6)
+            for (int k = 0; k < 6; k++) b6[6 * i + k] = b->nodes[i].b[k];
+        if (fc) fc[i] = b->nodes[i].first;
+        if (meta) meta[i] = b->nodes[i].meta;
+    }
+    return PRR_OK;
+}
+
+int prr_trace_closest(prr_context *c, const prr_scene *s, int64_t n, const float *org, const float *dir,
+                      int32_t *prim, float *t, float *u, float *v) {
+    if (!c || !s) return fail(c, PRR_ERR_INVALID, "NULL context or scene");
+    if (n < 0 || (n > 0 && (!org || !dir || !prim || !t))) return fail(c, PRR_ERR_INVALID, "bad arrays");
+    HIPCK(c, hipSetDevice(c->device));
+    HIPCK(c, launchTraceClosest(s->view, n, org, dir, prim, t, u, v, c->spill, c->spillLanes, c->stream));
+    return PRR_OK;
+}
+
+int prr_trace_any(prr_context *c, const prr_scene *s, int64_t n, const float *org, const float *dir,
+                  const float *tmax, uint8_t *hit) {
+    if (!c || !s) return fail(c, PRR_ERR_INVALID, "NULL context or scene");
+    if (n < 0 || (n > 0 && (!org || !dir || !tmax || !hit))) return fail(c, PRR_ERR_INVALID, "bad arrays");
+    HIPCK(c, hipSetDevice(c->device));
+    HIPCK(c, launchTraceAny(s->view, n, org, dir, tmax, hit, c->spill, c->spillLanes, c->stream));
+    return PRR_OK;
+}
+
+int prr_bsdf_eval(prr_context *c, int64_t n, const float *wi, const float *wo, const float *nrm, const float *params,
+                  float *f, float *pdf) {
+    if (!c) return fail(nullptr, PRR_ERR_INVALID, "context is NULL");
+    if (n < 0 || (n > 0 && (!wi || !wo || !nrm || !params || !f || !pdf))) return fail(c, PRR_ERR_INVALID, "bad arrays");
+    HIPCK(c, hipSetDevice(c->device));
+    HIPCK(c, launchBsdfEval(n, wi, wo, nrm, params, f, pdf, c->stream));
+    return PRR_OK;
+}
+
+int prr_bsdf_sample(prr_context *c, int64_t n, const float *wo, const float *nrm, const float *params, const float *u,
+                    float *wi, float *f, float *pdf, int32_t *lobe) {
+    if (!c) return fail(nullptr, PRR_ERR_INVALID, "context is NULL");
+    if (n < 0 || (n > 0 && (!wo || !nrm || !params || !u || !wi || !f || !pdf || !lobe)))
+        return fail(c, PRR_ERR_INVALID, "bad arrays");
+    HIPCK(c, hipSetDevice(c->device));
+    HIPCK(c, launchBsdfSample(n, wo, nrm, params, u, wi, f, pdf, lobe, c->stream));
+    return PRR_OK;
+}
+
+int prr_env_sample(prr_context *c, const prr_scene *s, int64_t n, const float *u2, float *dir, float *pdf) {
+    if (!c || !s) return fail(c, PRR_ERR_INVALID, "NULL context or scene");
+    if (s->view.env_w <= 0) return fail(c, PRR_ERR_UNSUPPORTED, "scene has no environment map");
+    if (n < 0 || (n > 0 && (!u2 || !dir || !pdf))) return fail(c, PRR_ERR_INVALID, "bad arrays");
+    HIPCK(c, hipSetDevice(c->device));
+    HIPCK(c, launchEnvSample(s->view, n, u2, dir, pdf, c->stream));
+    return PRR_OK;
+}
+
+int prr_env_eval(prr_context *c, const prr_scene *s, int64_t n, const float *dir, float *rad, float *pdf) {
+    if (!c || !s) return fail(c, PRR_ERR_INVALID, "NULL context or scene");
+    if (s->view.env_w <= 0) return fail(c, PRR_ERR_UNSUPPORTED, "scene has no environment map");
+    if (n < 0 || (n > 0 && (!dir || !rad || !pdf))) return fail(c, PRR_ERR_INVALID, "bad arrays");
+    HIPCK(c, hipSetDevice(c->device));
+    HIPCK(c, launchEnvEval(s->view, n, dir, rad, pdf, c->stream));
+    return PRR_OK;
+}
+
+int prr_camera_rays(prr_context *c, const prr_scene *s, int32_t W, int32_t H, int64_t n, const int32_t *pix,
+                    const float *offs, float *org, float *dir) {
+    if (!c || !s) return fail(c, PRR_ERR_INVALID, "NULL context or scene");
+    if (W <= 0 || H <= 0 || n < 0 || (n > 0 && (!pix || !offs || !org || !dir)))
+        return fail(c, PRR_ERR_INVALID, "bad arguments");
+    HIPCK(c, hipSetDevice(c->device));
+    HIPCK(c, launchCameraRays(s->view, W, H, n, pix, offs, org, dir, c->stream));
+    return PRR_OK;
+}
+
+int prr_tonemap(prr_context *c, int32_t op, int64_t n, float exposure, const float *lin, float *disp, uint8_t *rgb8) {
+    if (!c) return fail(nullptr, PRR_ERR_INVALID, "context is NULL");
+    if (op < PRR_TONEMAP_RAW || op > PRR_TONEMAP_ACES) return fail(c, PRR_ERR_INVALID, "unknown tone-map operator");
+    if (n < 0 || (n > 0 && !lin)) return fail(c, PRR_ERR_INVALID, "bad arrays");
+    HIPCK(c, hipSetDevice(c->device));
+    HIPCK(c, launchTonemap(op, n, exposure, lin, disp, rgb8, agxConst(), c->stream));
+    return PRR_OK;
+}
+
+// ---------------------------------------------------------------------------
+// render
+// ---------------------------------------------------------------------------
+static const int64_t kMaxSlots = (int64_t) 1 << 22;
+
+static int ensureRenderBufs(prr_context *c, int64_t slots, int maxDepth) {
+    if (slots <= c->pbSlots && maxDepth <= c->maxDepthAlloc) return PRR_OK;
+    HIPCK(c, hipStreamSynchronize(c->stream));
+    freeRenderBufs(c);
+    PathBufs &p = c->pb;
+    p = PathBufs{};
+    p.slots = slots;
+    hipError_t e = hipSuccess;
+    auto A = [&](auto **ptr, size_t count) {
+        using T = typename std::remove_reference<decltype(**ptr)>::type;
+        if (e != hipSuccess) return;
+        void *q = nullptr;
+        e = hipMalloc(&q, std::max<size_t>(1, count) * sizeof(T));
+        if (e == hipSuccess) {
+            c->pbAllocs.push_back(q);
+            *ptr = (T *) q;
+        }
+    };
+    A(&p.L, slots);
+    A(&p.beta, slots);
+    A(&p.bscale, slots);
+    A(&p.prevP, slots);
+    A(&p.pend, 3 * slots);
+    A(&p.rng, slots);
+    A(&p.flags, slots);
+    A(&p.vis, 3 * slots);
+    A(&p.result, slots);
+    for (int k = 0; k < 2; k++) {
+        A(&p.qSlot[k], slots);
+        A(&p.qOrg[k], slots);
+        A(&p.qDir[k], slots);
+    }
+    A(&p.hit, slots);
+    A(&p.sqSlot, 3 * slots);
+    A(&p.sqOrg, 3 * slots);
+    A(&p.sqDir, 3 * slots);
+    A(&p.qcnt, (size_t) maxDepth + 2);
+    A(&p.sqcnt, (size_t) maxDepth + 2);
+    if (e != hipSuccess) {
+        freeRenderBufs(c);
+        return fail(c, PRR_ERR_NOMEM, std::string("render buffers: ") + hipGetErrorString(e));
+    }
+    p.stats = c->stats;
+    c->pbSlots = slots;
+    c->maxDepthAlloc = maxDepth;
+    return PRR_OK;
+}
+
+int prr_render(prr_context *c, const prr_scene *s, const prr_render_params *p, float *rgb, prr_stats *stats) {
+    if (!c || !s || !p || !rgb) return fail(c, PRR_ERR_INVALID, "NULL argument");
+    if (p->width <= 0 || p->height <= 0 || p->spp <= 0 || p->max_depth <= 0 || p->max_depth > 64 ||
+        p->sample_offset < 0)
+        return fail(c, PRR_ERR_INVALID, "bad render parameters");
+    const int64_t pixels = (int64_t) p->width * p->height;
+    if (pixels > kMaxSlots) return fail(c, PRR_ERR_UNSUPPORTED, "image larger than 4M pixels");
+    if (p->num_photons > 0) return fail(c, PRR_ERR_UNSUPPORTED, "photon map requested (use the photon entry points)");
+    HIPCK(c, hipSetDevice(c->device));
+    const int spc = (int) std::max<int64_t>(1, std::min<int64_t>(p->spp, kMaxSlots / pixels));
+    int rc = ensureRenderBufs(c, pixels * spc, p->max_depth);
+    if (rc != PRR_OK) return rc;
+    if (c->fbPixels < pixels) {
+        if (c->fb) HIPCK(c, hipFree(c->fb));
+        c->fb = nullptr;
+        c->fbPixels = 0;
+        HIPCK(c, hipMalloc(&c->fb, (size_t) pixels * sizeof(float4)));
+        c->fbPixels = pixels;
+    }
+    hipStream_t st = c->stream;
+    HIPCK(c, hipMemsetAsync(c->stats, 0, 8 * sizeof(unsigned long long), st));
+    HIPCK(c, hipMemsetAsync(c->fb, 0, (size_t) pixels * sizeof(float4), st));
+    HIPCK(c, hipEventRecord(c->ev0, st));
+    const int traceGrid = c->traceBlocks;
+    const bool timing = (p->flags & PRR_RENDER_KERNEL_TIMING) != 0;
+    // per-launch event pairs: kind 0 extend, 1 shadow, 2 shade
+    std::vector<std::pair<int, int>> timed;  // (kind, first event index)
+    size_t evNext = 0;
+    auto evPair = [&](int kind) -> int {
+        if (evNext + 2 > c->evPool.size()) {
+            for (int k = 0; k < 256; k++) {
+                hipEvent_t e;
+                if (hipEventCreate(&e) != hipSuccess) return -1;
+                c->evPool.push_back(e);
+            }
+        }
+        const int i = (int) evNext;
+        evNext += 2;
+        timed.emplace_back(kind, i);
+        return i;
+    };
+#define TIMED(kind, launch)                                            \
+    do {                                                               \
+        int ei_ = timing ? evPair(kind) : -1;                          \
+        if (ei_ >= 0) HIPCK(c, hipEventRecord(c->evPool[ei_], st));    \
+        HIPCK(c, launch);                                              \
+        if (ei_ >= 0) HIPCK(c, hipEventRecord(c->evPool[ei_ + 1], st)); \
+    } while (0)
+    for (int s0 = 0; s0 < p->spp; s0 += spc) {
+        ChunkInfo ci;
+        ci.W = p->width;
+        ci.H = p->height;
+        ci.pixels = (int) pixels;
+        ci.spc = std::min(spc, p->spp - s0);
+        ci.sampleBase = p->sample_offset + s0;
+        ci.seed = p->seed;
+        ci.maxDepth = p->max_depth;
+        ci.clampInd = p->indirect_clamp;
+        HIPCK(c, hipMemsetAsync(c->pb.qcnt, 0, ((size_t) p->max_depth + 2) * sizeof(uint32_t), st));
+        HIPCK(c, hipMemsetAsync(c->pb.sqcnt, 0, ((size_t) p->max_depth + 2) * sizeof(uint32_t), st));
+        HIPCK(c, launchRaygen(s->view, c->pb, ci, c->shadeBlocks, st));
+        for (int b = 0; b <= p->max_depth; b++) {
+            TIMED(0, launchExtend(s->view, c->pb, b, c->spill, traceGrid, st));
+            if (b > 0) TIMED(1, launchShadow(s->view, c->pb, b - 1, c->spill, traceGrid, st));
+            TIMED(2, launchShade(s->view, c->pb, ci, b, c->shadeBlocks, st));
+        }
+        HIPCK(c, launchAccumulate(c->pb, ci, c->fb, st));
+    }
+#undef TIMED
+    HIPCK(c, launchResolve(c->fb, (int) pixels, p->spp, rgb, st));
+    HIPCK(c, hipEventRecord(c->ev1, st));
+    HIPCK(c, hipStreamSynchronize(st));
+    if (stats) {
+        unsigned long long h[8];
+        HIPCK(c, hipMemcpy(h, c->stats, sizeof(h), hipMemcpyDeviceToHost));
+        float ms = 0.f;
+        HIPCK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+        stats->rays_closest = h[0];
+        stats->rays_any = h[1];
+        stats->samples = h[2];
+        stats->photons_stored = h[3];
+        stats->ms_total = ms;
+        stats->ms_extend = stats->ms_shadow = stats->ms_shade = 0.0;
+        stats->launches_extend = stats->launches_shadow = 0;
+        for (const auto &t : timed) {
+            float dt = 0.f;
+            HIPCK(c, hipEventElapsedTime(&dt, c->evPool[t.second], c->evPool[t.second + 1]));
+            if (t.first == 0) {
+                stats->ms_extend += dt;
+                stats->launches_extend++;
+            } else if (t.first == 1) {
+                stats->ms_shadow += dt;
+                stats->launches_shadow++;
+            } else {
+                stats->ms_shade += dt;
+            }
+        }
+    }
+    return PRR_OK;
+}
+
+}  // extern "C"

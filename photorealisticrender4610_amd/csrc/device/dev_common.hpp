@@ -1,0 +1,591 @@
+// dev_common.hpp — device-side types and shading math for gfx950.
+//
+// Arithmetic keeps the reference's operation order (the library is compiled
+// with -ffp-contract=off, so no FMA contraction) so that geometry results are
+// bit-identical to the reference's CPU code and shading matches to ~1 ulp of
+// the transcendental functions. Reference file:line citations are relative to
+// the reference repository root.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace prr {
+
+#define PRR_DEV __device__ __forceinline__
+#define PRR_HD __host__ __device__ __forceinline__
+
+constexpr float kPi = 3.141592653589793f;  // global.hpp:9-10
+constexpr float kEps = 1e-3f;              // global.hpp:12
+
+// std::max / std::min semantics (a < b ? b : a), including NaN operands.
+PRR_HD float smax(float a, float b) { return (a < b) ? b : a; }
+PRR_HD float smin(float a, float b) { return (b < a) ? b : a; }
+
+struct V3 {
+    float x, y, z;
+    PRR_HD V3() : x(0.f), y(0.f), z(0.f) {}
+    PRR_HD explicit V3(float s) : x(s), y(s), z(s) {}
+    PRR_HD V3(float a, float b, float c) : x(a), y(b), z(c) {}
+    PRR_HD float operator[](int i) const { return i == 0 ? x : (i == 1 ? y : z); }
+};
+PRR_HD V3 operator+(const V3 &a, const V3 &b) { return V3(a.x + b.x, a.y + b.y, a.z + b.z); }
+PRR_HD V3 operator-(const V3 &a, const V3 &b) { return V3(a.x - b.x, a.y - b.y, a.z - b.z); }
+PRR_HD V3 operator*(const V3 &a, const V3 &b) { return V3(a.x * b.x, a.y * b.y, a.z * b.z); }
+PRR_HD V3 operator/(const V3 &a, const V3 &b) { return V3(a.x / b.x, a.y / b.y, a.z / b.z); }
+PRR_HD V3 operator*(const V3 &a, float r) { return V3(a.x * r, a.y * r, a.z * r); }
+PRR_HD V3 operator*(float r, const V3 &a) { return V3(a.x * r, a.y * r, a.z * r); }
+PRR_HD V3 operator/(const V3 &a, float r) { return V3(a.x / r, a.y / r, a.z / r); }
+PRR_HD V3 operator-(const V3 &a) { return V3(-a.x, -a.y, -a.z); }
+PRR_HD V3 &operator+=(V3 &a, const V3 &b) {
+    a.x += b.x;
+    a.y += b.y;
+    a.z += b.z;
+    return a;
+}
+PRR_HD float dot(const V3 &a, const V3 &b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+PRR_HD V3 cross(const V3 &a, const V3 &b) {
+    return V3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+// Vector.hpp:103-111
+PRR_HD V3 normalize(const V3 &v) {
+    float m2 = v.x * v.x + v.y * v.y + v.z * v.z;
+    if (m2 > 0) {
+        float inv = 1 / sqrtf(m2);
+        return V3(v.x * inv, v.y * inv, v.z * inv);
+    }
+    return v;
+}
+PRR_HD float norm(const V3 &v) { return sqrtf(v.x * v.x + v.y * v.y + v.z * v.z); }
+PRR_HD V3 normalized(const V3 &v) {  // Vector.hpp:28-31
+    float n = sqrtf(v.x * v.x + v.y * v.y + v.z * v.z);
+    return V3(v.x / n, v.y / n, v.z / n);
+}
+PRR_HD V3 lerp(const V3 &a, const V3 &b, float t) { return a * (1 - t) + b * t; }
+PRR_HD float max3(const V3 &v) { return smax(smax(v.x, v.y), v.z); }  // std::max({x,y,z})
+PRR_DEV bool finite3(const V3 &v) { return isfinite(v.x) && isfinite(v.y) && isfinite(v.z); }
+PRR_DEV V3 ld3(const float4 &f) { return V3(f.x, f.y, f.z); }
+PRR_DEV float4 st4(const V3 &v, float w) { return make_float4(v.x, v.y, v.z, w); }
+
+// ---------------------------------------------------------------------------
+// PCG32 per (pixel, sample) stream (O'Neill 2014, pcg32_random_r); 24-bit
+// floats like the reference's generator (global.hpp:49).
+// ---------------------------------------------------------------------------
+PRR_HD uint64_t mix64(uint64_t z) {
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+struct Pcg {
+    uint64_t state, inc;
+    PRR_HD uint32_t nextU() {
+        uint64_t old = state;
+        state = old * 6364136223846793005ULL + inc;
+        uint32_t xs = (uint32_t) (((old >> 18u) ^ old) >> 27u);
+        uint32_t rot = (uint32_t) (old >> 59u);
+        return (xs >> rot) | (xs << ((32u - rot) & 31u));
+    }
+    PRR_HD float next() { return (float) (nextU() >> 8) * 0x1p-24f; }
+    PRR_HD static Pcg make(uint64_t seed, uint32_t pixel, uint32_t sample) {
+        Pcg r;
+        r.state = 0;
+        r.inc = ((uint64_t) sample << 1) | 1u;
+        r.nextU();
+        r.state += mix64(seed * 0x9E3779B97F4A7C15ULL + pixel);
+        r.nextU();
+        return r;
+    }
+};
+
+// ---------------------------------------------------------------------------
+// Scene view: device pointers + constants, passed by value to kernels.
+// ---------------------------------------------------------------------------
+// Merged BVH node: the two children's boxes and references (64 B).
+struct __align__(16) DNode {
+    float4 a;  // L.minx L.maxx L.miny L.maxy
+    float4 b;  // L.minz L.maxz R.minx R.maxx
+    float4 c;  // R.miny R.maxy R.minz R.maxz
+    int4 d;    // Lref Rref pad pad
+};
+// Child reference: bit31 = leaf; leaf: bits 28..30 = count-1, bit 27 =
+// sphere leaf, bits 0..26 = first leaf-order primitive slot.
+constexpr uint32_t kLeafBit = 0x80000000u;
+constexpr uint32_t kSphereBit = 0x08000000u;
+PRR_HD bool refIsLeaf(uint32_t r) { return (r & kLeafBit) != 0u; }
+PRR_HD uint32_t refCount(uint32_t r) { return ((r >> 28) & 7u) + 1u; }
+PRR_HD uint32_t refFirst(uint32_t r) { return r & 0x07FFFFFFu; }
+PRR_HD bool refSphere(uint32_t r) { return (r & kSphereBit) != 0u; }
+PRR_HD uint32_t makeLeafRef(uint32_t first, uint32_t count, bool sphere) {
+    return kLeafBit | ((count - 1u) << 28) | (sphere ? kSphereBit : 0u) | first;
+}
+
+enum MatType { MAT_DIFFUSE = 0, MAT_GLASS = 1, MAT_MIRROR = 2, MAT_EMISSIVE = 3 };
+
+struct __align__(16) DMaterial {
+    float4 base_type;   // base rgb, type (as float)
+    float4 params;      // roughness (raw), metallic, ior, hasEmission (0/1)
+    float4 emission;    // emission rgb, pad
+};
+
+// Emitter record (Scene::emissives, Scene.cpp:19-25)
+struct __align__(16) DEmitter {
+    float area;        // Object::getArea
+    int kind;          // 0 mesh, 1 sphere
+    int material;
+    int lt_root;       // mesh: first node of its light tree; sphere: sphere row
+    float total;       // mesh sub-BVH totalArea (BVH.cpp:189)
+    int pad[3];
+};
+// Light tree node (reference flatten layout of the emitter mesh's sub-BVH):
+// first = right child (interior) or first leaf slot; count 0 = interior;
+// left_area = nodeAreas[this+1] (BVH.cpp:496).
+struct __align__(16) DLightNode {
+    int first;
+    int count;
+    float left_area;
+    int pad;
+};
+
+struct SceneView {
+    const DNode *nodes;
+    const float4 *tris;       // 3 float4 per leaf slot: v0.xyz e1.x | e1.yz e2.xy | e2.z id pad pad
+    const float4 *sph;        // leaf-order spheres: center.xyz radius
+    const int *sph_id;        // leaf-order sphere -> global prim id
+    const float4 *prim_n;     // per global prim: geometric normal (triangles), w = material (bits)
+    const float4 *sph_by_id;  // per sphere row: center, radius
+    const DMaterial *mats;
+    const DEmitter *emitters;
+    const DLightNode *lnodes;
+    const float4 *ltri;       // 4 float4 per light-tree leaf slot: v0.xyz area | v1 | v2 | normal
+    const float4 *env_px;     // W*H (rgb, lum)
+    const float *env_cdf;     // W*H
+    const float *env_marg;    // H
+    uint32_t root;            // root child reference
+    int num_tris;
+    int num_emitters;
+    float total_emit_area;
+    float root_box[6];        // [minX,maxX,minY,maxY,minZ,maxZ]
+    int env_w, env_h;
+    float env_total;
+    float background[3];
+    // point light (build extension): at most one
+    int num_points;
+    float point_pos[3];
+    float point_int[3];
+    // camera (Camera.hpp:43-56), focal length precomputed on the host
+    float cam_eye[3], cam_fwd[3], cam_right[3], cam_up[3];
+    float cam_aperture, cam_focus, cam_tanhalf;
+};
+
+// ---------------------------------------------------------------------------
+// Shading (BRDFUtils.hpp, Material.cpp)
+// ---------------------------------------------------------------------------
+struct Frame {
+    V3 T, B, N;
+};
+// buildTBN (BRDFUtils.hpp:15-30); the reference rebuilds it in every
+// toLocal/toWorld call, it is built once here (identical values).
+PRR_DEV Frame makeFrame(const V3 &N) {
+    Frame f;
+    f.N = N;
+    if (fabsf(N.x) > fabsf(N.y)) {
+        float invLen = 1.0f / sqrtf(N.x * N.x + N.z * N.z);
+        f.T = V3(N.z * invLen, 0.0f, -N.x * invLen);
+    } else {
+        float invLen = 1.0f / sqrtf(N.y * N.y + N.z * N.z);
+        f.T = V3(0.0f, N.z * invLen, -N.y * invLen);
+    }
+    f.B = cross(N, f.T);
+    return f;
+}
+PRR_DEV V3 toLocal(const Frame &f, const V3 &w) { return V3(dot(w, f.T), dot(w, f.B), dot(w, f.N)); }
+PRR_DEV V3 toWorld(const Frame &f, const V3 &l) { return l.x * f.T + l.y * f.B + l.z * f.N; }
+PRR_DEV V3 reflect(const V3 &wo, const V3 &N) { return 2.f * dot(wo, N) * N - wo; }
+
+PRR_DEV float D_GGX(float NdotH, float alpha) {  // BRDFUtils.hpp:90-94
+    float a2 = alpha * alpha;
+    float d = NdotH * NdotH * (a2 - 1.f) + 1.f;
+    return a2 / (kPi * d * d);
+}
+PRR_DEV float lambdaGGX(float NdotV, float alpha) {  // BRDFUtils.hpp:102-106
+    float c2 = NdotV * NdotV;
+    float t2 = (1.f - c2) / smax(c2, 1e-6f);
+    return 0.5f * (-1.f + sqrtf(1.f + alpha * alpha * t2));
+}
+PRR_DEV V3 F_Schlick(float WoDotH, const V3 &F0) {  // BRDFUtils.hpp:143-146
+    float t = powf(smax(0.f, 1.f - WoDotH), 5.f);
+    return F0 + (V3(1.f) - F0) * t;
+}
+PRR_DEV V3 sampleVNDF(const V3 &wo, float alpha, float u1, float u2) {  // BRDFUtils.hpp:163-181
+    V3 Vh = normalize(V3(alpha * wo.x, alpha * wo.y, wo.z));
+    float lensq = Vh.x * Vh.x + Vh.y * Vh.y;
+    V3 T1 = lensq > 0.f ? V3(-Vh.y, Vh.x, 0.f) / sqrtf(lensq) : V3(1.f, 0.f, 0.f);
+    V3 T2 = cross(Vh, T1);
+    float r = sqrtf(u1);
+    float phi = 2.f * kPi * u2;
+    float t1 = r * cosf(phi);
+    float t2 = r * sinf(phi);
+    float s = 0.5f * (1.f + Vh.z);
+    t2 = (1.f - s) * sqrtf(1.f - t1 * t1) + s * t2;
+    V3 Nh = t1 * T1 + t2 * T2 + sqrtf(smax(0.f, 1.f - t1 * t1 - t2 * t2)) * Vh;
+    return normalize(V3(alpha * Nh.x, alpha * Nh.y, smax(0.f, Nh.z)));
+}
+
+// Per-hit material constants of DiffuseMaterial (Material.cpp:28-158).
+struct Lobes {
+    V3 base, F0;
+    float alpha, metallic, specWeight;
+};
+PRR_DEV Lobes makeLobes(const V3 &base, float roughness, float metallic, float ior) {
+    Lobes l;
+    float r = smax(roughness, 0.08f);  // Material.cpp:15
+    l.alpha = r * r;
+    l.base = base;
+    l.metallic = metallic;
+    float f = (ior - 1.0f) / (ior + 1.0f);  // Material::computeF0 (Material.hpp:117-121)
+    f *= f;
+    l.F0 = lerp(V3(f), base, metallic);
+    l.specWeight = metallic + (1.f - metallic) * max3(l.F0);
+    return l;
+}
+
+// DiffuseMaterial::eval and ::pdf with LOBE_ALL, fused (they share H).
+PRR_DEV void evalPdf(const Lobes &m, const Frame &fr, const V3 &wi, const V3 &wo, V3 &f, float &pdf) {
+    V3 wiL = toLocal(fr, wi), woL = toLocal(fr, wo);
+    float NdotWi = wiL.z, NdotWo = woL.z;
+    f = V3(0.f);
+    pdf = 0.f;
+    if (NdotWi <= 0.f || NdotWo <= 0.f) return;
+    V3 H = wiL + woL;
+    if (dot(H, H) < 1e-8f) return;
+    H = normalize(H);
+    if (H.z < 0.f) H = -H;
+    float NdotH = smax(0.f, H.z);
+    float WoDotH = smax(0.f, dot(woL, H));
+    V3 F = F_Schlick(WoDotH, m.F0);
+    float D = D_GGX(NdotH, m.alpha);
+    float lo = lambdaGGX(NdotWo, m.alpha);
+    float li = lambdaGGX(NdotWi, m.alpha);
+    float G = 1.f / (1.f + lo + li);
+    float denom = 4.f * smax(NdotWo, 1e-4f) * smax(NdotWi, 1e-4f);
+    V3 spec = D * G * F / denom;
+    V3 kD = (V3(1.f) - F) * (1.f - m.metallic);
+    V3 diff = kD * m.base / kPi;
+    f = (diff + spec) * NdotWi;
+    float G1 = 1.f / (1.f + lo);
+    float ps = D * G1 * smax(0.f, WoDotH) / (smax(1e-6f, NdotWo) * 4.f * smax(1e-4f, WoDotH));
+    float pd = smax(0.f, NdotWi) / kPi;
+    pdf = m.specWeight * ps + (1.f - m.specWeight) * pd;
+}
+
+struct BsdfSample {
+    V3 wi, f;
+    float pdf;
+    int lobe;  // 0 diffuse, 1 specular, 2 delta, -1 none
+};
+
+// DiffuseMaterial::sample (Material.cpp:28-78); draws u0..u2 only above the
+// horizon, exactly like the reference.
+template <class R>
+PRR_DEV BsdfSample sampleDiffuse(const Lobes &m, const Frame &fr, const V3 &wo, R &rng) {
+    BsdfSample s;
+    s.wi = V3(0.f);
+    s.f = V3(0.f);
+    s.pdf = 0.f;
+    s.lobe = -1;
+    V3 woL = toLocal(fr, wo);
+    if (woL.z <= 0.f) return s;
+    float u0 = rng.next();
+    float u1 = rng.next();
+    float u2 = rng.next();
+    V3 wiL;
+    int lobe;
+    if (u0 < m.specWeight) {
+        V3 H = sampleVNDF(woL, m.alpha, u1, u2);
+        if (dot(H, H) < 1e-8f) return s;
+        wiL = reflect(woL, H);
+        lobe = 1;
+    } else {
+        float r = sqrtf(u1);
+        float phi = 2.f * kPi * u2;
+        wiL = V3(r * cosf(phi), r * sinf(phi), sqrtf(smax(0.f, 1.f - u1)));
+        lobe = 0;
+    }
+    if (wiL.z <= 0.f) return s;
+    V3 wi = toWorld(fr, wiL);
+    V3 f;
+    float pdf;
+    evalPdf(m, fr, wi, wo, f, pdf);
+    s.wi = wi;
+    s.lobe = lobe;
+    s.pdf = pdf;
+    s.f = (pdf > 1e-6f) ? f : V3(0.f);
+    return s;
+}
+
+PRR_DEV float clampf(float lo, float hi, float v) { return smax(lo, smin(hi, v)); }
+// refract / fresnel (BRDFUtils.hpp:232-260)
+PRR_DEV V3 refract(const V3 &I, const V3 &N, float ior) {
+    float cosi = clampf(-1.f, 1.f, dot(I, N));
+    float etai = 1.f, etat = ior;
+    V3 n = N;
+    if (cosi < 0.f) {
+        cosi = -cosi;
+    } else {
+        float t = etai;
+        etai = etat;
+        etat = t;
+        n = -N;
+    }
+    float eta = etai / etat;
+    float k = 1.f - eta * eta * (1.f - cosi * cosi);
+    return k < 0.f ? V3(0.f) : eta * I + (eta * cosi - sqrtf(k)) * n;
+}
+PRR_DEV float fresnel(const V3 &I, const V3 &N, float ior) {
+    float cosi = clampf(-1.f, 1.f, dot(I, N));
+    float etai = 1.f, etat = ior;
+    if (cosi > 0.f) {
+        float t = etai;
+        etai = etat;
+        etat = t;
+    }
+    float sint = etai / etat * sqrtf(smax(0.f, 1.f - cosi * cosi));
+    if (sint >= 1.f) return 1.f;
+    float cost = sqrtf(smax(0.f, 1.f - sint * sint));
+    cosi = fabsf(cosi);
+    float Rs = ((etat * cosi) - (etai * cost)) / ((etat * cosi) + (etai * cost));
+    float Rp = ((etai * cosi) - (etat * cost)) / ((etai * cosi) + (etat * cost));
+    return (Rs * Rs + Rp * Rp) * 0.5f;
+}
+// GlassMaterial::sample (Material.cpp:169-191): one draw, always.
+PRR_DEV BsdfSample sampleGlass(const V3 &wo, const V3 &N, float ior, float u) {
+    BsdfSample s;
+    s.pdf = 1.0f;
+    s.lobe = 2;
+    V3 inc = -wo;
+    float kr = fresnel(inc, N, ior);
+    V3 refr = refract(inc, N, ior);
+    bool tir = norm(refr) < 1e-6f;
+    bool entering = dot(inc, N) < 0.f;
+    float eta = entering ? (1.0f / ior) : ior;
+    if (u < kr || tir) {
+        s.wi = normalize(reflect(wo, N));
+        s.f = V3(1.0f);
+    } else {
+        s.wi = normalize(refr);
+        s.f = V3(eta * eta);
+    }
+    return s;
+}
+// MirrorMaterial::sample (Material.cpp:160-167)
+PRR_DEV BsdfSample sampleMirror(const V3 &wo, const V3 &N, const V3 &base) {
+    BsdfSample s;
+    s.wi = normalize(reflect(wo, N));
+    s.f = base;
+    s.pdf = 1.0f;
+    s.lobe = 2;
+    return s;
+}
+
+// ---------------------------------------------------------------------------
+// Environment map (EnvMap.hpp)
+// ---------------------------------------------------------------------------
+PRR_DEV float luminance(const V3 &c) { return 0.2126f * c.x + 0.7152f * c.y + 0.0722f * c.z; }
+PRR_DEV void dirToUV(const V3 &d, float &u, float &v) {  // EnvMap.hpp:71-75
+    float phi = atan2f(d.z, d.x);
+    float theta = acosf(smax(-1.f, smin(1.f, d.y)));
+    u = (phi + kPi) / (2.f * kPi);
+    v = theta / kPi;
+}
+PRR_DEV V3 uvToDir(float u, float v) {  // EnvMap.hpp:84-89
+    float phi = u * 2.f * kPi - kPi;
+    float theta = v * kPi;
+    float sinT = sinf(theta);
+    return V3(sinT * cosf(phi), cosf(theta), sinT * sinf(phi));
+}
+PRR_DEV V3 envLookup(const SceneView &s, const V3 &dir) {  // EnvMap.hpp:97-113
+    float u, v;
+    dirToUV(dir, u, v);
+    const int W = s.env_w, H = s.env_h;
+    float px = smax(0.f, u * (W - 1));
+    float py = smax(0.f, v * (H - 1));
+    int x0 = (int) px, y0 = (int) py;
+    int x1 = min(x0 + 1, W - 1);
+    int y1 = min(y0 + 1, H - 1);
+    float fx = px - x0, fy = py - y0;
+    V3 a = ld3(s.env_px[y0 * W + x0]), b = ld3(s.env_px[y0 * W + x1]);
+    V3 c = ld3(s.env_px[y1 * W + x0]), d = ld3(s.env_px[y1 * W + x1]);
+    V3 top = a + (b - a) * fx;
+    V3 bot = c + (d - c) * fx;
+    return top + (bot - top) * fy;
+}
+PRR_DEV float envPdf(const SceneView &s, const V3 &dir) {  // EnvMap.hpp:124-133
+    float u, v;
+    dirToUV(dir, u, v);
+    const int W = s.env_w, H = s.env_h;
+    int col = min((int) (u * W), W - 1);
+    int row = min((int) (v * H), H - 1);
+    float theta = v * kPi;
+    float sinT = smax(1e-4f, sinf(theta));
+    float lum = s.env_px[row * W + col].w;
+    return (lum * sinT / (s.env_total + 1e-6f)) * ((float) (W * H) / (2.f * kPi * kPi * sinT));
+}
+// first index i in [0,n) with a[i] >= x, or n (std::lower_bound)
+PRR_DEV int lowerBound(const float *a, int n, float x) {
+    int lo = 0, cnt = n;
+    while (cnt > 0) {
+        int step = cnt >> 1;
+        int mid = lo + step;
+        if (a[mid] < x) {
+            lo = mid + 1;
+            cnt -= step + 1;
+        } else {
+            cnt = step;
+        }
+    }
+    return lo;
+}
+PRR_DEV V3 envImportanceSample(const SceneView &s, float r1, float r2, float &pdf) {  // EnvMap.hpp:145-169
+    const int W = s.env_w, H = s.env_h;
+    int row = min(lowerBound(s.env_marg, H, r1), H - 1);
+    int col = min(lowerBound(s.env_cdf + (size_t) row * W, W, r2), W - 1);
+    float u = (col + 0.5f) / (float) W;
+    float v = (row + 0.5f) / (float) H;
+    V3 dir = uvToDir(u, v);
+    float lum = s.env_px[row * W + col].w;
+    float p = (lum / (s.env_total + 1e-6f)) * ((float) (W * H) / (2.f * kPi * kPi));
+    pdf = smax(1e-6f, p);
+    return dir;
+}
+
+// ---------------------------------------------------------------------------
+// Camera::generateRay (Camera.hpp:58-95)
+// ---------------------------------------------------------------------------
+PRR_DEV V3 sampleDisc(float u, float v) {
+    float a = 2.0f * u - 1.0f, b = 2.0f * v - 1.0f;
+    if (a == 0.0f && b == 0.0f) return V3(0.f);
+    float r, phi;
+    if (fabsf(a) > fabsf(b)) {
+        r = a;
+        phi = (kPi / 4.0f) * (b / a);
+    } else {
+        r = b;
+        phi = (kPi / 2.0f) - (kPi / 4.0f) * (a / b);
+    }
+    return V3(r * cosf(phi), r * sinf(phi), 0.0f);
+}
+template <class R>
+PRR_DEV void cameraRay(const SceneView &s, int i, int j, int W, int H, float ox, float oy, R &rng, V3 &org,
+                       V3 &dir) {
+    float fpx = ((float) W * 0.5f) / s.cam_tanhalf;
+    float x = ((float) i + ox - (float) W * 0.5f) / fpx;
+    float y = ((float) (H - j) + oy - (float) H * 0.5f) / fpx;
+    V3 fwd(s.cam_fwd[0], s.cam_fwd[1], s.cam_fwd[2]);
+    V3 right(s.cam_right[0], s.cam_right[1], s.cam_right[2]);
+    V3 up(s.cam_up[0], s.cam_up[1], s.cam_up[2]);
+    V3 eye(s.cam_eye[0], s.cam_eye[1], s.cam_eye[2]);
+    V3 d = normalize(fwd + x * right + y * up);
+    if (s.cam_aperture <= 0.0f) {
+        org = eye;
+        dir = d;
+        return;
+    }
+    V3 focus = eye + d * (s.cam_focus / dot(d, fwd));
+    float u = rng.next();
+    float v = rng.next();
+    V3 lo = sampleDisc(u, v);
+    V3 lens = eye + s.cam_aperture * (lo.x * right + lo.y * up);
+    org = lens;
+    dir = normalize(focus - lens);
+}
+
+// ---------------------------------------------------------------------------
+// Tone mapping (ToneMapping.hpp) + ACES (Narkowicz 2015 fit)
+// ---------------------------------------------------------------------------
+PRR_DEV float clamp01f(float x) { return smin(smax(x, 0.f), 1.f); }  // std::clamp(x, 0, 1)
+PRR_DEV float srgbOETF(float x) {
+    x = clamp01f(x);
+    return x < 0.0031308f ? 12.92f * x : 1.055f * powf(x, 1.f / 2.4f) - 0.055f;
+}
+PRR_DEV V3 srgb3(const V3 &c) { return V3(srgbOETF(c.x), srgbOETF(c.y), srgbOETF(c.z)); }
+struct AgxConst {
+    float toe_a, slope, w, shoulder_max;
+};
+PRR_DEV float allenwp(float x, const AgxConst &p) {  // ToneMapping.hpp:151-166
+    const float crossover = 0.1841865f;
+    x = smax(x, 0.f);
+    if (x < crossover) {
+        float t = powf(x, 1.25f);
+        return t / (t + p.toe_a);
+    }
+    float s = x - crossover;
+    float slope_s = p.slope * s;
+    s = slope_s * (1.f + s / p.w) / (1.f + (slope_s / p.shoulder_max));
+    s += crossover;
+    return s;
+}
+PRR_DEV V3 mat3mul(const float m[9], const V3 &v) {
+    return V3(m[0] * v.x + m[1] * v.y + m[2] * v.z, m[3] * v.x + m[4] * v.y + m[5] * v.z,
+              m[6] * v.x + m[7] * v.y + m[8] * v.z);
+}
+PRR_DEV V3 tonemapAgx(V3 c, const AgxConst &p) {  // ToneMapping.hpp:172-214
+    const float inset[9] = {0.544814746488245f, 0.373787398372697f, 0.0813978551390581f,
+                            0.140416948464053f, 0.754137554567394f, 0.105445496968552f,
+                            0.0888104196149096f, 0.178871756420858f, 0.732317823964232f};
+    const float outset[9] = {1.96488741169489f, -0.855988495690215f, -0.108898916004672f,
+                             -0.299313364904742f, 1.32639796461980f, -0.0270845997150571f,
+                             -0.164352742528393f, -0.238183969428088f, 1.40253671195648f};
+    c = V3(smax(c.x, 0.f), smax(c.y, 0.f), smax(c.z, 0.f));
+    c = mat3mul(inset, c);
+    c = V3(allenwp(c.x, p), allenwp(c.y, p), allenwp(c.z, p));
+    c = V3(smin(1.0f, c.x), smin(1.0f, c.y), smin(1.0f, c.z));
+    c = mat3mul(outset, c);
+    c = V3(clamp01f(c.x), clamp01f(c.y), clamp01f(c.z));
+    return srgb3(c);
+}
+PRR_DEV V3 tonemapPbrNeutral(V3 c) {  // ToneMapping.hpp:234-262
+    const float startCompression = 0.8f - 0.04f;
+    const float desaturation = 0.15f;
+    float x = smin(c.x, smin(c.y, c.z));
+    float offset = x < 0.08f ? x - 6.25f * x * x : 0.04f;
+    c.x -= offset;
+    c.y -= offset;
+    c.z -= offset;
+    float peak = smax(c.x, smax(c.y, c.z));
+    if (peak < startCompression) return srgb3(c);
+    float d = 1.f - startCompression;
+    float newPeak = 1.f - d * d / (peak + d - startCompression);
+    float scale = newPeak / peak;
+    c.x *= scale;
+    c.y *= scale;
+    c.z *= scale;
+    float g = 1.f - 1.f / (desaturation * (peak - newPeak) + 1.f);
+    c.x = c.x * (1.f - g) + newPeak * g;
+    c.y = c.y * (1.f - g) + newPeak * g;
+    c.z = c.z * (1.f - g) + newPeak * g;
+    return srgb3(c);
+}
+PRR_DEV float acesCurve(float x) {  // Narkowicz 2015
+    x = smax(x, 0.f);
+    float n = x * (2.51f * x + 0.03f);
+    float dd = x * (2.43f * x + 0.59f) + 0.14f;
+    return clamp01f(n / dd);
+}
+PRR_DEV V3 tonemapApply(int op, const V3 &c, const AgxConst &p) {
+    switch (op) {
+    case 0: return V3(clamp01f(c.x), clamp01f(c.y), clamp01f(c.z));
+    case 1: return srgb3(c);
+    case 2: return tonemapAgx(c, p);
+    case 3: return tonemapPbrNeutral(c);
+    default: return srgb3(V3(acesCurve(c.x), acesCurve(c.y), acesCurve(c.z)));
+    }
+}
+PRR_DEV V3 exposeLinear(const V3 &fb, float exposure) {  // Renderer.cpp:149-161
+    V3 c = fb * exposure;
+    if (!isfinite(c.x)) c.x = 0.f;
+    if (!isfinite(c.y)) c.y = 0.f;
+    if (!isfinite(c.z)) c.z = 0.f;
+    return V3(smax(0.f, c.x), smax(0.f, c.y), smax(0.f, c.z));
+}
+
+}  // namespace prr

@@ -1,0 +1,281 @@
+// dev_traverse.hpp — BVH closest-hit and any-hit traversal on gfx950.
+//
+// The reference's two-level BVH (scene level over objects, mesh level over
+// triangles; BVH.cpp:311-473) is merged on the host into one tree of 64-byte
+// nodes that hold both children's boxes, so an interior expansion is one
+// 64-byte read. Visit order, box test arithmetic, culling and the primitive
+// tests are those of BVHAccel::Intersect / IntersectP and Triangle::hitTest /
+// intersectT, which makes closest-hit ids (including ties) and t bit-identical
+// to the reference.
+//
+// Stack: each lane keeps the top kShort entries (node ref, box entry t) in
+// LDS, laid out [depth][lane] so a wave's 64 lanes hit 64 distinct banks;
+// deeper entries spill to a per-lane region of global memory.
+#pragma once
+
+#include "dev_common.hpp"
+
+namespace prr {
+
+constexpr int kShort = 16;        // LDS stack entries per lane
+constexpr int kSpill = 64;        // total stack capacity per lane (reference: 64)
+constexpr int kTraceBlock = 256;  // threads per block of traversal kernels
+
+struct RayPre {
+    float o[3], inv[3];
+    int oct[3];
+};
+// BVHAccel::precompute (BVH.cpp:290-299)
+PRR_DEV RayPre precompute(const V3 &o, const V3 &d) {
+    RayPre r;
+    const float dv[3] = {d.x, d.y, d.z};
+    const float ov[3] = {o.x, o.y, o.z};
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        r.o[a] = ov[a];
+        r.inv[a] = (dv[a] != 0.f) ? (1.f / dv[a]) : __builtin_inff();
+        r.oct[a] = (dv[a] < 0.f) ? 1 : 0;
+    }
+    return r;
+}
+// nodeIntersect (BVH.hpp:115-124) on one box [minX,maxX,minY,maxY,minZ,maxZ].
+PRR_DEV float slab(const RayPre &r, float x0, float x1, float y0, float y1, float z0, float z1) {
+    float tmin = 0.f, tmax = __builtin_inff();
+    float nx = r.oct[0] ? x1 : x0, fx = r.oct[0] ? x0 : x1;
+    float ny = r.oct[1] ? y1 : y0, fy = r.oct[1] ? y0 : y1;
+    float nz = r.oct[2] ? z1 : z0, fz = r.oct[2] ? z0 : z1;
+    float t0 = (nx - r.o[0]) * r.inv[0], t1 = (fx - r.o[0]) * r.inv[0];
+    tmin = fmaxf(tmin, t0);  // == std::max(tmin, t0) for non-NaN tmin
+    tmax = fminf(tmax, t1);
+    t0 = (ny - r.o[1]) * r.inv[1];
+    t1 = (fy - r.o[1]) * r.inv[1];
+    tmin = fmaxf(tmin, t0);
+    tmax = fminf(tmax, t1);
+    t0 = (nz - r.o[2]) * r.inv[2];
+    t1 = (fz - r.o[2]) * r.inv[2];
+    tmin = fmaxf(tmin, t0);
+    tmax = fminf(tmax, t1);
+    return tmin <= tmax ? tmin : __builtin_inff();
+}
+
+struct TriRec {
+    V3 v0, e1, e2;
+    int id;
+};
+PRR_DEV TriRec loadTri(const float4 *tris, uint32_t slot) {
+    const float4 a = tris[3 * slot], b = tris[3 * slot + 1], c = tris[3 * slot + 2];
+    TriRec t;
+    t.v0 = V3(a.x, a.y, a.z);
+    t.e1 = V3(a.w, b.x, b.y);
+    t.e2 = V3(b.z, b.w, c.x);
+    t.id = __float_as_int(c.y);
+    return t;
+}
+// Triangle::hitTest (Triangle.hpp:46-64)
+PRR_DEV bool triHit(const TriRec &tr, const V3 &o, const V3 &d, float &t, float &u, float &v) {
+    V3 P = cross(d, tr.e2);
+    float det = dot(P, tr.e1);
+    if (fabsf(det) < 1e-6f) return false;
+    float inv = 1.f / det;
+    V3 T = o - tr.v0;
+    u = dot(P, T) * inv;
+    if (u < 0.f) return false;
+    V3 Q = cross(T, tr.e1);
+    v = dot(Q, d) * inv;
+    if (v < 0.f || u + v > 1.f) return false;
+    t = dot(Q, tr.e2) * inv;
+    return t > 1e-3f;
+}
+// Triangle::intersectT (Triangle.hpp:75-92): occluder in (1e-3, tMax)
+PRR_DEV bool triAny(const TriRec &tr, const V3 &o, const V3 &d, float tMax) {
+    V3 P = cross(d, tr.e2);
+    float det = dot(P, tr.e1);
+    if (fabsf(det) < 1e-6f) return false;
+    float inv = 1.f / det;
+    V3 T = o - tr.v0;
+    float u = dot(P, T) * inv;
+    if (u < 0.f || u > 1.f) return false;
+    V3 Q = cross(T, tr.e1);
+    float v = dot(Q, d) * inv;
+    if (v < 0.f || u + v > 1.f) return false;
+    float t = dot(Q, tr.e2) * inv;
+    return t > 1e-3f && t < tMax;
+}
+// Sphere::getIntersection (Sphere.hpp:22-44) with solveQuadratic
+// (global.hpp:19-33) whose sqrt is the double overload.
+PRR_DEV bool sphHit(const float4 &s, const V3 &o, const V3 &d, float &t) {
+    V3 c(s.x, s.y, s.z);
+    V3 L = o - c;
+    float a = dot(d, d);
+    float b = 2 * dot(d, L);
+    float cc = dot(L, L) - s.w * s.w;
+    float discr = b * b - 4 * a * cc;
+    if (discr < 0) return false;
+    float x0, x1;
+    if (discr == 0) {
+        x0 = x1 = (float) (-0.5 * (double) b / (double) a);
+    } else {
+        double sq = sqrt((double) discr);
+        float q = (float) ((b > 0) ? -0.5 * ((double) b + sq) : -0.5 * ((double) b - sq));
+        x0 = q / a;
+        x1 = cc / q;
+    }
+    if (x0 > x1) {
+        float tt = x0;
+        x0 = x1;
+        x1 = tt;
+    }
+    if (x0 < 0) x0 = x1;
+    if (x0 < 0) return false;
+    t = x0;
+    return true;
+}
+
+// Per-lane stack: top kShort entries in LDS ([kShort][kTraceBlock] uint2),
+// the rest in a global spill region of (kSpill - kShort) entries per lane.
+struct LaneStack {
+    uint2 *lds;    // base of this block's LDS stack
+    uint2 *spill;  // this lane's spill region (kSpill - kShort entries)
+    int lane;      // threadIdx.x
+    int sp;
+    PRR_DEV void push(uint32_t ref, float t) {
+        const uint2 e = make_uint2(ref, __float_as_uint(t));
+        if (sp >= kShort) spill[sp - kShort] = lds[(sp % kShort) * kTraceBlock + lane];
+        lds[(sp % kShort) * kTraceBlock + lane] = e;
+        sp++;
+    }
+    PRR_DEV uint2 pop() {
+        sp--;
+        const int slot = (sp % kShort) * kTraceBlock + lane;
+        uint2 e = lds[slot];
+        if (sp >= kShort) lds[slot] = spill[sp - kShort];
+        return e;
+    }
+};
+
+struct HitOut {
+    float t, u, v;
+    int prim;  // global primitive id or -1
+};
+
+// BVHAccel::Intersect on the merged tree. Visits nodes in the reference's
+// order; the running closest t persists across mesh boundaries, which only
+// removes nodes the reference would reject at scene level (strict <), so the
+// winner is unchanged.
+PRR_DEV HitOut traceClosest(const SceneView &s, const V3 &o, const V3 &d, LaneStack &st) {
+    HitOut h;
+    h.t = __builtin_inff();
+    h.u = h.v = 0.f;
+    h.prim = -1;
+    const RayPre r = precompute(o, d);
+    st.sp = 0;
+    uint32_t cur = s.root;
+    while (true) {
+        if (refIsLeaf(cur)) {
+            const uint32_t first = refFirst(cur), cnt = refCount(cur);
+            if (refSphere(cur)) {
+                for (uint32_t i = 0; i < cnt; i++) {
+                    float t;
+                    if (sphHit(s.sph[first + i], o, d, t) && t < h.t) {
+                        h.t = t;
+                        h.u = h.v = 0.f;
+                        h.prim = s.sph_id[first + i];
+                    }
+                }
+            } else {
+                for (uint32_t i = 0; i < cnt; i++) {
+                    const TriRec tr = loadTri(s.tris, first + i);
+                    float t, u, v;
+                    if (triHit(tr, o, d, t, u, v) && t < h.t) {
+                        h.t = t;
+                        h.u = u;
+                        h.v = v;
+                        h.prim = tr.id;
+                    }
+                }
+            }
+        } else {
+            const DNode n = s.nodes[cur];
+            const float tL = slab(r, n.a.x, n.a.y, n.a.z, n.a.w, n.b.x, n.b.y);
+            const float tR = slab(r, n.b.z, n.b.w, n.c.x, n.c.y, n.c.z, n.c.w);
+            const uint32_t L = (uint32_t) n.d.x, R = (uint32_t) n.d.y;
+            if (tL <= tR) {
+                if (tL < h.t) {
+                    if (tR < h.t) st.push(R, tR);
+                    cur = L;
+                    continue;
+                }
+            } else {
+                if (tR < h.t) {
+                    if (tL < h.t) st.push(L, tL);
+                    cur = R;
+                    continue;
+                }
+            }
+        }
+        bool found = false;
+        while (st.sp > 0) {
+            const uint2 e = st.pop();
+            if (__uint_as_float(e.y) < h.t) {
+                cur = e.x;
+                found = true;
+                break;
+            }
+        }
+        if (!found) break;
+    }
+    return h;
+}
+
+// BVHAccel::IntersectP on the merged tree: occluded iff some primitive in a
+// leaf whose every ancestor box (root included) passes tBox < tMax has a hit
+// in range (triangles: (1e-3, tMax); spheres: [0, tMax)). The boolean does
+// not depend on visit order; near child first by box distance.
+PRR_DEV bool traceAny(const SceneView &s, const V3 &o, const V3 &d, float tMax, LaneStack &st) {
+    const RayPre r = precompute(o, d);
+    // the reference tests the root's own box first (BVH.cpp:437-438)
+    const float tb = slab(r, s.root_box[0], s.root_box[1], s.root_box[2], s.root_box[3], s.root_box[4],
+                          s.root_box[5]);
+    if (!(tb < tMax)) return false;
+    st.sp = 0;
+    uint32_t cur = s.root;
+    while (true) {
+        if (refIsLeaf(cur)) {
+            const uint32_t first = refFirst(cur), cnt = refCount(cur);
+            if (refSphere(cur)) {
+                for (uint32_t i = 0; i < cnt; i++) {
+                    float t;
+                    if (sphHit(s.sph[first + i], o, d, t) && t < tMax) return true;
+                }
+            } else {
+                for (uint32_t i = 0; i < cnt; i++)
+                    if (triAny(loadTri(s.tris, first + i), o, d, tMax)) return true;
+            }
+        } else {
+            const DNode n = s.nodes[cur];
+            const float tL = slab(r, n.a.x, n.a.y, n.a.z, n.a.w, n.b.x, n.b.y);
+            const float tR = slab(r, n.b.z, n.b.w, n.c.x, n.c.y, n.c.z, n.c.w);
+            const bool hl = tL < tMax, hr = tR < tMax;
+            const uint32_t L = (uint32_t) n.d.x, R = (uint32_t) n.d.y;
+            if (hl && hr) {
+                const bool lf = tL <= tR;
+                st.push(lf ? R : L, 0.f);
+                cur = lf ? L : R;
+                continue;
+            }
+            if (hl) {
+                cur = L;
+                continue;
+            }
+            if (hr) {
+                cur = R;
+                continue;
+            }
+        }
+        if (st.sp == 0) break;
+        cur = st.pop().x;
+    }
+    return false;
+}
+
+}  // namespace prr

@@ -1,0 +1,146 @@
+// k_misc.hip — batch kernels for the BSDF, environment, camera and tone-map
+// entry points. Each runs the same device functions the renderer uses.
+#include "dev_common.hpp"
+#include "launch.hpp"
+
+namespace prr {
+
+constexpr int kBlk = 256;
+
+PRR_DEV V3 ldv(const float *p, int64_t i) { return V3(p[3 * i], p[3 * i + 1], p[3 * i + 2]); }
+PRR_DEV void stv(float *p, int64_t i, const V3 &v) {
+    p[3 * i] = v.x;
+    p[3 * i + 1] = v.y;
+    p[3 * i + 2] = v.z;
+}
+
+struct FixedU {
+    const float *u;
+    int k;
+    PRR_DEV float next() { return u[k++]; }
+};
+
+__global__ void kBsdfEval(int64_t n, const float *wi, const float *wo, const float *nrm, const float *params, float *f,
+                          float *pdf) {
+    const int64_t i = (int64_t) blockIdx.x * kBlk + threadIdx.x;
+    if (i >= n) return;
+    const float *p = params + 8 * i;
+    V3 fv(0.f);
+    float pv = 0.f;
+    if ((int) p[0] == MAT_DIFFUSE) {
+        const Lobes m = makeLobes(V3(p[1], p[2], p[3]), p[4], p[5], p[6]);
+        const Frame fr = makeFrame(ldv(nrm, i));
+        evalPdf(m, fr, ldv(wi, i), ldv(wo, i), fv, pv);
+    }
+    stv(f, i, fv);
+    pdf[i] = pv;
+}
+
+__global__ void kBsdfSample(int64_t n, const float *wo, const float *nrm, const float *params, const float *u,
+                            float *wi, float *f, float *pdf, int32_t *lobe) {
+    const int64_t i = (int64_t) blockIdx.x * kBlk + threadIdx.x;
+    if (i >= n) return;
+    const float *p = params + 8 * i;
+    const V3 N = ldv(nrm, i), w = ldv(wo, i);
+    BsdfSample s;
+    s.wi = V3(0.f);
+    s.f = V3(0.f);
+    s.pdf = 0.f;
+    s.lobe = -1;
+    const int type = (int) p[0];
+    if (type == MAT_DIFFUSE) {
+        FixedU r{u + 3 * i, 0};
+        s = sampleDiffuse(makeLobes(V3(p[1], p[2], p[3]), p[4], p[5], p[6]), makeFrame(N), w, r);
+    } else if (type == MAT_GLASS) {
+        s = sampleGlass(w, N, p[6], u[3 * i]);
+    } else if (type == MAT_MIRROR) {
+        s = sampleMirror(w, N, V3(p[1], p[2], p[3]));
+    }
+    stv(wi, i, s.wi);
+    stv(f, i, s.f);
+    pdf[i] = s.pdf;
+    lobe[i] = s.lobe;
+}
+
+__global__ void kEnvSample(SceneView s, int64_t n, const float *u2, float *dir, float *pdf) {
+    const int64_t i = (int64_t) blockIdx.x * kBlk + threadIdx.x;
+    if (i >= n) return;
+    float p;
+    const V3 d = envImportanceSample(s, u2[2 * i], u2[2 * i + 1], p);
+    stv(dir, i, d);
+    pdf[i] = p;
+}
+
+__global__ void kEnvEval(SceneView s, int64_t n, const float *dir, float *rad, float *pdf) {
+    const int64_t i = (int64_t) blockIdx.x * kBlk + threadIdx.x;
+    if (i >= n) return;
+    const V3 d = ldv(dir, i);
+    stv(rad, i, envLookup(s, d));
+    pdf[i] = envPdf(s, d);
+}
+
+__global__ void kCameraRays(SceneView s, int W, int H, int64_t n, const int32_t *pix, const float *offs, float *org,
+                            float *dir) {
+    const int64_t i = (int64_t) blockIdx.x * kBlk + threadIdx.x;
+    if (i >= n) return;
+    const int stride = s.cam_aperture > 0.f ? 4 : 2;
+    FixedU lens{offs + stride * i + 2, 0};
+    V3 o, d;
+    cameraRay(s, pix[2 * i], pix[2 * i + 1], W, H, offs[stride * i], offs[stride * i + 1], lens, o, d);
+    stv(org, i, o);
+    stv(dir, i, d);
+}
+
+PRR_DEV uint8_t quantize(float v) { return (uint8_t) (clamp01f(v) * 255.f + 0.5f); }  // Renderer.cpp:169-171
+
+__global__ void kTonemap(int op, int64_t n, float exposure, const float *lin, float *disp, uint8_t *rgb8,
+                         AgxConst agx) {
+    const int64_t i = (int64_t) blockIdx.x * kBlk + threadIdx.x;
+    if (i >= n) return;
+    const V3 c = tonemapApply(op, exposeLinear(ldv(lin, i), exposure), agx);
+    if (disp) stv(disp, i, c);
+    if (rgb8) {
+        rgb8[3 * i] = quantize(c.x);
+        rgb8[3 * i + 1] = quantize(c.y);
+        rgb8[3 * i + 2] = quantize(c.z);
+    }
+}
+
+static dim3 grid(int64_t n) { return dim3((unsigned) ((n + kBlk - 1) / kBlk)); }
+
+hipError_t launchBsdfEval(int64_t n, const float *wi, const float *wo, const float *nrm, const float *params,
+                          float *f, float *pdf, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(kBsdfEval, grid(n), dim3(kBlk), 0, st, n, wi, wo, nrm, params, f, pdf);
+    return hipGetLastError();
+}
+hipError_t launchBsdfSample(int64_t n, const float *wo, const float *nrm, const float *params, const float *u,
+                            float *wi, float *f, float *pdf, int32_t *lobe, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(kBsdfSample, grid(n), dim3(kBlk), 0, st, n, wo, nrm, params, u, wi, f, pdf, lobe);
+    return hipGetLastError();
+}
+hipError_t launchEnvSample(const SceneView &s, int64_t n, const float *u2, float *dir, float *pdf, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(kEnvSample, grid(n), dim3(kBlk), 0, st, s, n, u2, dir, pdf);
+    return hipGetLastError();
+}
+hipError_t launchEnvEval(const SceneView &s, int64_t n, const float *dir, float *rad, float *pdf, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(kEnvEval, grid(n), dim3(kBlk), 0, st, s, n, dir, rad, pdf);
+    return hipGetLastError();
+}
+hipError_t launchCameraRays(const SceneView &s, int W, int H, int64_t n, const int32_t *pix, const float *offs,
+                            float *org, float *dir, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(kCameraRays, grid(n), dim3(kBlk), 0, st, s, W, H, n, pix, offs, org, dir);
+    return hipGetLastError();
+}
+hipError_t launchTonemap(int op, int64_t n, float exposure, const float *lin, float *disp, uint8_t *rgb8,
+                         const AgxConst &agx, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(kTonemap, grid(n), dim3(kBlk), 0, st, op, n, exposure, lin, disp, rgb8, agx);
+    return hipGetLastError();
+}
+
+}  // namespace prr

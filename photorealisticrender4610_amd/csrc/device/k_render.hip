@@ -1,0 +1,497 @@
+// k_render.hip — wavefront integrator kernels.
+//
+// One chunk = `spc` samples for every pixel, one path per (pixel, sample)
+// slot, slot = s * pixels + pixel. Per bounce b (0..maxDepth):
+//   kExtend  closest hit of the ray queue of bounce b
+//   kShadow  any hit of the shadow queue emitted at bounce b-1
+//   kShade   Integrator::castRay's loop body for bounce b (Integrator.cpp:122-257)
+// The NEE terms of bounce b are added at bounce b+1, once their shadow rays
+// are resolved, in the reference's order (area, env, then the BSDF-MIS term),
+// so the float sums match the reference's. kAccumulate adds the chunk's
+// results per pixel in sample order (Renderer.cpp:84-99).
+#include "dev_traverse.hpp"
+#include "launch_render.hpp"
+#include "render.hpp"
+
+namespace prr {
+
+constexpr int kShadeBlock = 256;
+
+PRR_DEV uint32_t laneId() { return __lane_id(); }
+
+// Wave-aggregated queue append: one atomic per wave. All lanes that reach
+// this point take part; lanes with want == false get no index.
+PRR_DEV uint32_t waveAppend(uint32_t *counter, bool want) {
+    const uint64_t mask = __ballot(want);
+    if (mask == 0) return 0;
+    const int leader = __ffsll((unsigned long long) mask) - 1;
+    uint32_t base = 0;
+    if ((int) laneId() == leader) base = atomicAdd(counter, (uint32_t) __popcll(mask));
+    base = __shfl(base, leader);
+    const uint64_t below = mask & ((1ull << laneId()) - 1ull);
+    return base + (uint32_t) __popcll(below);
+}
+
+PRR_DEV uint64_t pcgInc(uint32_t sample) { return ((uint64_t) sample << 1) | 1u; }
+
+PRR_DEV V3 clampInd(const V3 &c, float k) {  // Integrator.cpp:105-111
+    if (k <= 0.f) return c;
+    float lum = (c.x + c.y + c.z) / 3;
+    return (lum > k) ? c * (k / lum) : c;
+}
+PRR_DEV float mis(float a, float b) { return (a * a) / (a * a + b * b + 1e-6f); }  // Integrator.hpp:30-32
+
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kShadeBlock) void kRaygen(SceneView s, PathBufs pb, ChunkInfo ci) {
+    const int64_t n = (int64_t) ci.pixels * ci.spc;
+    const int64_t stride = (int64_t) gridDim.x * kShadeBlock;
+    for (int64_t slot = (int64_t) blockIdx.x * kShadeBlock + threadIdx.x; slot < n; slot += stride) {
+        const int p = (int) (slot % ci.pixels);
+        const int sIdx = ci.sampleBase + (int) (slot / ci.pixels);
+        Pcg rng = Pcg::make(ci.seed, (uint32_t) p, (uint32_t) sIdx);
+        const float ox = rng.next();  // Renderer.cpp:75-76
+        const float oy = rng.next();
+        V3 o, d;
+        cameraRay(s, p % ci.W, p / ci.W, ci.W, ci.H, ox, oy, rng, o, d);
+        pb.L[slot] = make_float4(0.f, 0.f, 0.f, 0.f);
+        pb.beta[slot] = make_float4(1.f, 1.f, 1.f, 0.f);
+        pb.flags[slot] = 0u;
+        pb.rng[slot] = rng.state;
+        pb.qSlot[0][slot] = (uint32_t) slot;
+        pb.qOrg[0][slot] = st4(o, 0.f);
+        pb.qDir[0][slot] = st4(d, 0.f);
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        pb.qcnt[0] = (uint32_t) n;
+        atomicAdd(&pb.stats[0], (unsigned long long) n);
+        atomicAdd(&pb.stats[2], (unsigned long long) n);
+    }
+}
+
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kTraceBlock) void kExtend(SceneView s, PathBufs pb, int bounce, uint2 *spill) {
+    __shared__ uint2 stk[kShort * kTraceBlock];
+    const int q = bounce & 1;
+    const int64_t n = pb.qcnt[bounce];
+    const int64_t lane = (int64_t) blockIdx.x * kTraceBlock + threadIdx.x;
+    LaneStack st;
+    st.lds = stk;
+    st.lane = threadIdx.x;
+    st.spill = spill + lane * (kSpill - kShort);
+    st.sp = 0;
+    const int64_t stride = (int64_t) gridDim.x * kTraceBlock;
+    for (int64_t i = lane; i < n; i += stride) {
+        const float4 o4 = pb.qOrg[q][i];
+        float4 h = make_float4(__builtin_inff(), 0.f, 0.f, __int_as_float(-1));
+        if (o4.w >= 0.f) {
+            const float4 d4 = pb.qDir[q][i];
+            const HitOut r = traceClosest(s, ld3(o4), ld3(d4), st);
+            h = make_float4(r.t, r.u, r.v, __int_as_float(r.prim));
+        }
+        pb.hit[i] = h;
+    }
+}
+
+__global__ __launch_bounds__(kTraceBlock) void kShadow(SceneView s, PathBufs pb, int bounce, uint2 *spill) {
+    __shared__ uint2 stk[kShort * kTraceBlock];
+    const int64_t n = pb.sqcnt[bounce];
+    const int64_t lane = (int64_t) blockIdx.x * kTraceBlock + threadIdx.x;
+    LaneStack st;
+    st.lds = stk;
+    st.lane = threadIdx.x;
+    st.spill = spill + lane * (kSpill - kShort);
+    st.sp = 0;
+    const int64_t stride = (int64_t) gridDim.x * kTraceBlock;
+    for (int64_t i = lane; i < n; i += stride) {
+        const uint32_t sk = pb.sqSlot[i];
+        const float4 o4 = pb.sqOrg[i];
+        const float4 d4 = pb.sqDir[i];
+        const bool occ = traceAny(s, ld3(o4), ld3(d4), o4.w, st);
+        pb.vis[(int64_t) (sk >> 30) * pb.slots + (sk & 0x3FFFFFFFu)] = occ ? 0 : 1;
+    }
+}
+
+// ---------------------------------------------------------------------------
+struct HitGeom {
+    V3 P, Ng;
+    int mat;
+};
+PRR_DEV HitGeom hitGeom(const SceneView &s, int prim, float t, const V3 &o, const V3 &d) {
+    HitGeom g;
+    const float4 pn = s.prim_n[prim];
+    g.mat = __float_as_int(pn.w);
+    if (prim < s.num_tris) {
+        g.P = o + t * d;  // Triangle::finalize (Triangle.cpp:34-59)
+        g.Ng = ld3(pn);
+    } else {
+        const float4 sc = s.sph_by_id[prim - s.num_tris];
+        g.P = V3(o + d * t);  // Sphere.hpp:34-35
+        g.Ng = normalize(V3(g.P - V3(sc.x, sc.y, sc.z)));
+    }
+    return g;
+}
+
+// Scene::sampleLight (Scene.cpp:37-58) -> MeshTriangle / BVHAccel::Sample
+// (BVH.cpp:478-514) -> Triangle::Sample / Sphere::Sample.
+PRR_DEV void sampleLight(const SceneView &s, Pcg &rng, V3 &pos, V3 &nrm, int &mat) {
+    const float p = rng.next() * s.total_emit_area;
+    float acc = 0.f;
+    int chosen = s.num_emitters - 1;
+    for (int k = 0; k < s.num_emitters; k++) {
+        acc += s.emitters[k].area;
+        if (p <= acc) {
+            chosen = k;
+            break;
+        }
+    }
+    const DEmitter e = s.emitters[chosen];
+    mat = e.material;
+    if (e.kind == 0) {
+        float q = sqrtf(rng.next()) * e.total;
+        int ni = e.lt_root;
+        int slot = -1;
+        while (slot < 0) {
+            const DLightNode n = s.lnodes[ni];
+            if (n.count > 0) {
+                for (int i = 0; i < n.count; i++) {
+                    q -= s.ltri[4 * (n.first + i)].w;
+                    if (q <= 0.f || i == n.count - 1) {
+                        slot = n.first + i;
+                        break;
+                    }
+                }
+            } else if (q < n.left_area) {
+                ni = ni + 1;
+            } else {
+                q -= n.left_area;
+                ni = n.first;
+            }
+        }
+        const float x = sqrtf(rng.next()), y = rng.next();
+        const V3 v0 = ld3(s.ltri[4 * slot]), v1 = ld3(s.ltri[4 * slot + 1]), v2 = ld3(s.ltri[4 * slot + 2]);
+        pos = v0 * (1.0f - x) + v1 * (x * (1.0f - y)) + v2 * (x * y);
+        nrm = ld3(s.ltri[4 * slot + 3]);
+    } else {
+        const float4 sc = s.sph_by_id[e.lt_root];
+        const float theta = (float) (2.0 * (double) kPi * (double) rng.next());
+        const float phi = kPi * rng.next();
+        const V3 dir(cosf(phi), sinf(phi) * cosf(theta), sinf(phi) * sinf(theta));
+        pos = V3(sc.x, sc.y, sc.z) + sc.w * dir;
+        nrm = dir;
+    }
+}
+
+__global__ __launch_bounds__(kShadeBlock) void kShade(SceneView s, PathBufs pb, ChunkInfo ci, int bounce) {
+    const int q = bounce & 1;
+    const int64_t n = pb.qcnt[bounce];
+    const int64_t stride = (int64_t) gridDim.x * kShadeBlock;
+    const int64_t i0 = (int64_t) blockIdx.x * kShadeBlock + threadIdx.x;
+    // uniform trip count per wave so every lane reaches the appends
+    const int64_t waveBase = i0 - (threadIdx.x & 63);
+    for (int64_t wb = waveBase; wb < n; wb += stride) {
+        const int64_t i = wb + (threadIdx.x & 63);
+        const bool active = i < n;
+        bool emitRay = false, emitA = false, emitE = false, emitP = false, done = false, termEntry = false;
+        V3 newO, newD, sOA, sDA, sOE, sDE, sOP, sDP;
+        float tA = 0.f, tP = 0.f;
+        uint32_t slot = 0;
+        if (active) {
+            slot = pb.qSlot[q][i];
+            uint32_t fl = pb.flags[slot];
+            V3 L = ld3(pb.L[slot]);
+            V3 beta = ld3(pb.beta[slot]);
+            const int sIdx = ci.sampleBase + (int) (slot / (uint32_t) ci.pixels);
+            Pcg rng;
+            rng.state = pb.rng[slot];
+            rng.inc = pcgInc((uint32_t) sIdx);
+            // 1. NEE of the previous bounce, in reference order
+            if (fl & F_PEND_A)
+                if (pb.vis[slot]) L += ld3(pb.pend[slot]);
+            if (fl & F_PEND_E)
+                if (pb.vis[pb.slots + slot]) L += ld3(pb.pend[pb.slots + slot]);
+            if (fl & F_PEND_P)
+                if (pb.vis[2 * pb.slots + slot]) L += ld3(pb.pend[2 * pb.slots + slot]);
+            fl &= ~F_PEND_ALL;
+            done = (fl & F_TERMINATING) != 0u;
+            const float4 h = pb.hit[i];
+            const int prim = __float_as_int(h.w);
+            const V3 o = ld3(pb.qOrg[q][i]), d = ld3(pb.qDir[q][i]);
+            // 2. the BSDF-sampled ray of the previous (diffuse) bounce
+            if (!done && (fl & F_PREV_DIFF)) {
+                const bool ind = bounce - 1 > 0;
+                const float4 bs4 = pb.bscale[slot];
+                const V3 bscale = ld3(bs4);
+                const float bpdf = bs4.w;
+                if (prim < 0) {  // Integrator.cpp:217-229
+                    if (s.env_w > 0) {
+                        const V3 envL = envLookup(s, d);
+                        const float w = mis(bpdf, envPdf(s, d));
+                        const V3 c = beta * bscale * w * envL;
+                        L += ind ? clampInd(c, ci.clampInd) : c;
+                    } else {
+                        const V3 c = beta * bscale * V3(s.background[0], s.background[1], s.background[2]);
+                        L += ind ? clampInd(c, ci.clampInd) : c;
+                    }
+                    done = true;
+                } else {
+                    const HitGeom g = hitGeom(s, prim, h.x, o, d);
+                    const DMaterial hm = s.mats[g.mat];
+                    if ((int) hm.base_type.w == MAT_EMISSIVE) {  // Integrator.cpp:231-242
+                        const float cosT = smax(0.f, dot(-d, normalized(g.Ng)));
+                        const V3 dd = g.P - ld3(pb.prevP[slot]);
+                        const float hd2 = dot(dd, dd);
+                        const float lpA = s.total_emit_area > 0.f ? 1.f / s.total_emit_area : 0.f;
+                        const float lpSA = lpA * hd2 / smax(cosT, 1e-4f);
+                        const float w = mis(bpdf, lpSA);
+                        const V3 c = beta * bscale * w * ld3(hm.emission);
+                        L += ind ? clampInd(c, ci.clampInd) : c;
+                        done = true;
+                    } else {
+                        beta = beta * bscale;
+                        if (!finite3(beta)) done = true;
+                    }
+                }
+                if (!done && bounce - 1 >= 3) {  // Russian roulette (Integrator.cpp:250-256)
+                    const float mc = max3(beta);
+                    const float qq = smax(0.05f, 1.f - mc);
+                    if (rng.next() < qq)
+                        done = true;
+                    else
+                        beta = beta * (1.f / (1.f - qq));
+                }
+                fl &= ~F_PREV_DIFF;
+            }
+            if (!done && bounce >= ci.maxDepth) done = true;
+            // 3. this bounce (Integrator.cpp:124-248)
+            if (!done) {
+                const bool spec = (fl & F_SPEC) != 0u;
+                if (bounce == 0 || spec) {
+                    if (prim < 0) {
+                        const V3 bg = s.env_w > 0 ? envLookup(s, d)
+                                                  : V3(s.background[0], s.background[1], s.background[2]);
+                        const V3 c = beta * bg;
+                        L += bounce > 0 ? clampInd(c, ci.clampInd) : c;
+                    }
+                }
+                if (prim < 0) {
+                    done = true;
+                } else {
+                    const HitGeom g = hitGeom(s, prim, h.x, o, d);
+                    const DMaterial m = s.mats[g.mat];
+                    const int mtype = (int) m.base_type.w;
+                    const V3 base = ld3(m.base_type);
+                    if (mtype == MAT_EMISSIVE) {
+                        if (bounce == 0 || spec) {
+                            const V3 c = beta * ld3(m.emission);
+                            L += bounce > 0 ? clampInd(c, ci.clampInd) : c;
+                        }
+                        // scatters nothing (EmissiveMaterial: f = pdf = 0)
+                        done = true;
+                    } else if (mtype == MAT_GLASS || mtype == MAT_MIRROR) {
+                        const BsdfSample bs = mtype == MAT_GLASS ? sampleGlass(-d, g.Ng, m.params.z, rng.next())
+                                                                 : sampleMirror(-d, g.Ng, base);
+                        beta = beta * bs.f / bs.pdf;
+                        const bool trans = dot(bs.wi, g.Ng) < 0.f;
+                        const V3 off = trans ? -g.Ng : g.Ng;
+                        if (trans) beta = beta * base;  // Integrator.cpp:161-165
+                        fl |= F_SPEC;
+                        if (bounce + 1 >= ci.maxDepth) {
+                            done = true;  // the loop ends before this ray is used
+                        } else {
+                            emitRay = true;
+                            newO = g.P + off * kEps;
+                            newD = bs.wi;
+                        }
+                    } else {
+                        fl &= ~F_SPEC;
+                        const V3 sN = dot(d, g.Ng) < 0.f ? g.Ng : -g.Ng;
+                        const Frame fr = makeFrame(sN);
+                        const Lobes lb = makeLobes(base, m.params.x, m.params.y, m.params.z);
+                        const V3 wo = -d;
+                        const bool ind = bounce > 0;
+                        // NEE area (Integrator.cpp:14-34, 78-87)
+                        if (s.total_emit_area > 0.f) {
+                            V3 lp, ln;
+                            int lm;
+                            sampleLight(s, rng, lp, ln, lm);
+                            const V3 toL = lp - g.P;
+                            const float d2 = dot(toL, toL);
+                            const float dist = sqrtf(d2);
+                            const V3 wi = toL / dist;
+                            const float cosL = smax(0.f, dot(-wi, normalized(ln)));
+                            if (cosL > 0.f) {
+                                const float pdfSA = (1.0f / s.total_emit_area) * d2 / cosL;
+                                V3 f;
+                                float bp;
+                                evalPdf(lb, fr, wi, wo, f, bp);
+                                const float w = mis(pdfSA, bp);
+                                V3 c = beta * (w * ld3(s.mats[lm].emission) * f / pdfSA);
+                                c = ind ? clampInd(c, ci.clampInd) : c;
+                                pb.pend[slot] = st4(c, 0.f);
+                                fl |= F_PEND_A;
+                                emitA = true;
+                                sOA = g.P + wi * kEps;
+                                sDA = wi;
+                                tA = dist - 2.f * kEps;
+                            }
+                        }
+                        // NEE environment (Integrator.cpp:39-66, 93-102)
+                        if (s.env_w > 0) {
+                            const float r1 = rng.next(), r2 = rng.next();
+                            float pe;
+                            const V3 dir = envImportanceSample(s, r1, r2, pe);
+                            if (!(pe < 1e-6f)) {
+                                const V3 Le = envLookup(s, dir);
+                                V3 f;
+                                float bp;
+                                evalPdf(lb, fr, dir, wo, f, bp);
+                                const float w = mis(pe, bp);
+                                V3 c = beta * (w * Le * f / (pe + 1e-6f));
+                                c = ind ? clampInd(c, ci.clampInd) : c;
+                                pb.pend[pb.slots + slot] = st4(c, 0.f);
+                                fl |= F_PEND_E;
+                                emitE = true;
+                                sOE = g.P + dir * kEps;
+                                sDE = dir;
+                            }
+                        }
+                        // NEE point light (build extension; no draws)
+                        if (s.num_points > 0) {
+                            const V3 pp(s.point_pos[0], s.point_pos[1], s.point_pos[2]);
+                            const V3 toL = pp - g.P;
+                            const float d2 = dot(toL, toL);
+                            const float dist = sqrtf(d2);
+                            const V3 wi = toL / dist;
+                            V3 f;
+                            float bp;
+                            evalPdf(lb, fr, wi, wo, f, bp);
+                            V3 c = beta * (V3(s.point_int[0], s.point_int[1], s.point_int[2]) * f / d2);
+                            c = ind ? clampInd(c, ci.clampInd) : c;
+                            pb.pend[2 * pb.slots + slot] = st4(c, 0.f);
+                            fl |= F_PEND_P;
+                            emitP = true;
+                            sOP = g.P + wi * kEps;
+                            sDP = wi;
+                            tP = dist - 2.f * kEps;
+                        }
+                        // BSDF sampling (Integrator.cpp:207-215)
+                        const BsdfSample bs = sampleDiffuse(lb, fr, wo, rng);
+                        if (bs.pdf < 1e-6f) {
+                            done = true;
+                        } else {
+                            const V3 bscale = bs.f / bs.pdf;
+                            const V3 off = dot(bs.wi, sN) >= 0.f ? sN : -sN;
+                            pb.bscale[slot] = st4(bscale, bs.pdf);
+                            pb.prevP[slot] = st4(g.P, 0.f);
+                            fl |= F_PREV_DIFF;
+                            emitRay = true;
+                            newO = g.P + off * kEps;
+                            newD = bs.wi;
+                        }
+                    }
+                }
+            }
+            // a path that ends with NEE still in flight comes back once more
+            if (done && (fl & F_PEND_ALL) && !(fl & F_TERMINATING)) {
+                fl |= F_TERMINATING;
+                emitRay = true;
+                done = false;
+            }
+            if (done) {
+                const V3 r = finite3(L) ? L : V3(0.f);  // Integrator.cpp:259-261
+                pb.result[slot] = st4(r, 0.f);
+            } else {
+                pb.L[slot] = st4(L, 0.f);
+                pb.beta[slot] = st4(beta, 0.f);
+                pb.rng[slot] = rng.state;
+            }
+            pb.flags[slot] = fl;
+            termEntry = (fl & F_TERMINATING) != 0u;
+        }
+        // queue appends (all lanes of the wave take part)
+        const uint32_t qi = waveAppend(&pb.qcnt[bounce + 1], emitRay);
+        if (emitRay) {
+            pb.qSlot[q ^ 1][qi] = slot;
+            pb.qOrg[q ^ 1][qi] = termEntry ? make_float4(0.f, 0.f, 0.f, -1.f) : st4(newO, 0.f);
+            pb.qDir[q ^ 1][qi] = termEntry ? make_float4(0.f, 0.f, 0.f, 0.f) : st4(newD, 0.f);
+        }
+        const uint32_t ia = waveAppend(&pb.sqcnt[bounce], emitA);
+        if (emitA) {
+            pb.sqSlot[ia] = slot;
+            pb.sqOrg[ia] = st4(sOA, tA);
+            pb.sqDir[ia] = st4(sDA, 0.f);
+        }
+        const uint32_t ie = waveAppend(&pb.sqcnt[bounce], emitE);
+        if (emitE) {
+            pb.sqSlot[ie] = slot | (1u << 30);
+            pb.sqOrg[ie] = st4(sOE, __builtin_inff());
+            pb.sqDir[ie] = st4(sDE, 0.f);
+        }
+        const uint32_t ip = waveAppend(&pb.sqcnt[bounce], emitP);
+        if (emitP) {
+            pb.sqSlot[ip] = slot | (2u << 30);
+            pb.sqOrg[ip] = st4(sOP, tP);
+            pb.sqDir[ip] = st4(sDP, 0.f);
+        }
+        {
+            const uint64_t mr = __ballot(emitRay && !termEntry);
+            const uint64_t ms = __ballot(emitA);
+            const uint64_t me = __ballot(emitE);
+            const uint64_t mp = __ballot(emitP);
+            if ((threadIdx.x & 63) == 0) {
+                if (mr) atomicAdd(&pb.stats[0], (unsigned long long) __popcll(mr));
+                const int nsh = __popcll(ms) + __popcll(me) + __popcll(mp);
+                if (nsh) atomicAdd(&pb.stats[1], (unsigned long long) nsh);
+            }
+        }
+    }
+}
+
+// Renderer.cpp:84-99: framebuffer[m] += s in sample order.
+__global__ void kAccumulate(PathBufs pb, ChunkInfo ci, float4 *fb) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= ci.pixels) return;
+    V3 acc = ld3(fb[p]);
+    for (int k = 0; k < ci.spc; k++) acc += ld3(pb.result[(int64_t) k * ci.pixels + p]);
+    fb[p] = st4(acc, 0.f);
+}
+
+__global__ void kResolve(const float4 *fb, int pixels, int spp, float *rgb) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= pixels) return;
+    const V3 m = ld3(fb[p]) / (float) spp;  // Renderer.cpp:99
+    rgb[3 * p] = m.x;
+    rgb[3 * p + 1] = m.y;
+    rgb[3 * p + 2] = m.z;
+}
+
+// ---------------------------------------------------------------------------
+hipError_t launchRaygen(const SceneView &s, const PathBufs &pb, const ChunkInfo &ci, int grid, hipStream_t st) {
+    hipLaunchKernelGGL(kRaygen, dim3(grid), dim3(kShadeBlock), 0, st, s, pb, ci);
+    return hipGetLastError();
+}
+hipError_t launchExtend(const SceneView &s, const PathBufs &pb, int bounce, uint2 *spill, int traceGrid,
+                        hipStream_t st) {
+    hipLaunchKernelGGL(kExtend, dim3(traceGrid), dim3(kTraceBlock), 0, st, s, pb, bounce, spill);
+    return hipGetLastError();
+}
+hipError_t launchShadow(const SceneView &s, const PathBufs &pb, int bounce, uint2 *spill, int traceGrid,
+                        hipStream_t st) {
+    hipLaunchKernelGGL(kShadow, dim3(traceGrid), dim3(kTraceBlock), 0, st, s, pb, bounce, spill);
+    return hipGetLastError();
+}
+hipError_t launchShade(const SceneView &s, const PathBufs &pb, const ChunkInfo &ci, int bounce, int grid,
+                       hipStream_t st) {
+    hipLaunchKernelGGL(kShade, dim3(grid), dim3(kShadeBlock), 0, st, s, pb, ci, bounce);
+    return hipGetLastError();
+}
+hipError_t launchAccumulate(const PathBufs &pb, const ChunkInfo &ci, float4 *fb, hipStream_t st) {
+    hipLaunchKernelGGL(kAccumulate, dim3((ci.pixels + 255) / 256), dim3(256), 0, st, pb, ci, fb);
+    return hipGetLastError();
+}
+hipError_t launchResolve(const float4 *fb, int pixels, int spp, float *rgb, hipStream_t st) {
+    hipLaunchKernelGGL(kResolve, dim3((pixels + 255) / 256), dim3(256), 0, st, fb, pixels, spp, rgb);
+    return hipGetLastError();
+}
+
+}  // namespace prr

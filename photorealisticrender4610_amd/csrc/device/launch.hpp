@@ -1,0 +1,29 @@
+// launch.hpp — host-side launchers of the device kernels (one per .hip file).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "dev_common.hpp"
+
+namespace prr {
+
+// k_trace.hip
+hipError_t launchTraceClosest(const SceneView &s, int64_t n, const float *org, const float *dir, int32_t *prim,
+                              float *t, float *u, float *v, uint2 *spill, int spillLanes, hipStream_t st);
+hipError_t launchTraceAny(const SceneView &s, int64_t n, const float *org, const float *dir, const float *tmax,
+                          uint8_t *hit, uint2 *spill, int spillLanes, hipStream_t st);
+
+// k_misc.hip
+hipError_t launchBsdfEval(int64_t n, const float *wi, const float *wo, const float *nrm, const float *params,
+                          float *f, float *pdf, hipStream_t st);
+hipError_t launchBsdfSample(int64_t n, const float *wo, const float *nrm, const float *params, const float *u,
+                            float *wi, float *f, float *pdf, int32_t *lobe, hipStream_t st);
+hipError_t launchEnvSample(const SceneView &s, int64_t n, const float *u2, float *dir, float *pdf, hipStream_t st);
+hipError_t launchEnvEval(const SceneView &s, int64_t n, const float *dir, float *rad, float *pdf, hipStream_t st);
+hipError_t launchCameraRays(const SceneView &s, int W, int H, int64_t n, const int32_t *pix, const float *offs,
+                            float *org, float *dir, hipStream_t st);
+hipError_t launchTonemap(int op, int64_t n, float exposure, const float *lin, float *disp, uint8_t *rgb8,
+                         const AgxConst &agx, hipStream_t st);
+
+}  // namespace prr

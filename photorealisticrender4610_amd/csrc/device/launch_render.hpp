@@ -1,0 +1,20 @@
+// launch_render.hpp — host-side launchers of the wavefront render kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "render.hpp"
+
+namespace prr {
+
+hipError_t launchRaygen(const SceneView &s, const PathBufs &pb, const ChunkInfo &ci, int grid, hipStream_t st);
+hipError_t launchExtend(const SceneView &s, const PathBufs &pb, int bounce, uint2 *spill, int traceGrid,
+                        hipStream_t st);
+hipError_t launchShadow(const SceneView &s, const PathBufs &pb, int bounce, uint2 *spill, int traceGrid,
+                        hipStream_t st);
+hipError_t launchShade(const SceneView &s, const PathBufs &pb, const ChunkInfo &ci, int bounce, int grid,
+                       hipStream_t st);
+hipError_t launchAccumulate(const PathBufs &pb, const ChunkInfo &ci, float4 *fb, hipStream_t st);
+hipError_t launchResolve(const float4 *fb, int pixels, int spp, float *rgb, hipStream_t st);
+
+}  // namespace prr

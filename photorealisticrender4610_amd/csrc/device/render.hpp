@@ -1,0 +1,51 @@
+// render.hpp — wavefront path state shared by the render kernels and the host
+// driver (abi.hip).
+#pragma once
+
+#include "dev_common.hpp"
+
+namespace prr {
+
+// path flags
+constexpr uint32_t F_SPEC = 1u;          // last bounce was a delta bounce (specularBounce)
+constexpr uint32_t F_PREV_DIFF = 2u;     // a BSDF-sampled ray from a diffuse vertex is in flight
+constexpr uint32_t F_PEND_A = 4u;        // area-light NEE waiting for its shadow ray
+constexpr uint32_t F_PEND_E = 8u;        // environment NEE waiting
+constexpr uint32_t F_PEND_P = 16u;       // point-light NEE waiting
+constexpr uint32_t F_TERMINATING = 32u;  // path ended; only pending NEE remains
+constexpr uint32_t F_PEND_ALL = F_PEND_A | F_PEND_E | F_PEND_P;
+
+struct PathBufs {
+    int64_t slots;           // capacity (paths per chunk)
+    float4 *L;               // radiance so far (xyz)
+    float4 *beta;            // throughput (xyz)
+    float4 *bscale;          // pending BSDF factor f/pdf (xyz), w = BSDF pdf
+    float4 *prevP;           // diffuse vertex that shot the BSDF ray (xyz)
+    float4 *pend;            // 3 * slots: area, env, point NEE contributions
+    uint64_t *rng;           // PCG32 state
+    uint32_t *flags;
+    uint8_t *vis;            // 3 * slots: visibility of the pending NEE rays
+    float4 *result;          // final L per slot
+    uint32_t *qSlot[2];      // ray queue (ping-pong)
+    float4 *qOrg[2];         // origin, w < 0: no ray (terminating entry)
+    float4 *qDir[2];
+    float4 *hit;             // t, u, v, prim (bits), aligned with the queue being shaded
+    uint32_t *sqSlot;        // shadow queue: slot | kind << 30
+    float4 *sqOrg;           // origin, tmax
+    float4 *sqDir;
+    uint32_t *qcnt;          // ray-queue count per bounce
+    uint32_t *sqcnt;         // shadow-queue count per bounce
+    unsigned long long *stats;  // [0] closest rays, [1] any rays, [2] samples, [3] photons stored
+};
+
+struct ChunkInfo {
+    int W, H;
+    int pixels;          // W * H
+    int spc;             // samples of this chunk per pixel
+    int sampleBase;      // absolute index of the chunk's first sample
+    uint64_t seed;
+    int maxDepth;
+    float clampInd;      // Integrator::indirectClamp
+};
+
+}  // namespace prr

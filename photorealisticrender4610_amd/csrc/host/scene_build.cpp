@@ -1,0 +1,625 @@
+// scene_build.cpp — see scene_build.hpp.
+#include "scene_build.hpp"
+
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <limits>
+#include <thread>
+
+namespace prr {
+namespace host {
+
+static const float kInfF = std::numeric_limits<float>::infinity();
+
+Box::Box() {
+    for (int a = 0; a < 3; a++) {
+        mn[a] = kInfF;   // (float) std::numeric_limits<double>::max()    (Bounds3.hpp:11-16)
+        mx[a] = -kInfF;  // (float) std::numeric_limits<double>::lowest()
+    }
+}
+Box::Box(const float *a, const float *b) {
+    for (int k = 0; k < 3; k++) {
+        mn[k] = fminf(a[k], b[k]);
+        mx[k] = fmaxf(a[k], b[k]);
+    }
+}
+void Box::unite(const Box &o) {  // Union(Bounds3, Bounds3) (Bounds3.hpp:111-117)
+    for (int k = 0; k < 3; k++) {
+        mn[k] = std::min(mn[k], o.mn[k]);
+        mx[k] = std::max(mx[k], o.mx[k]);
+    }
+}
+void Box::unite(const float *p) {  // Union(Bounds3, Vector3f) (Bounds3.hpp:119-125)
+    for (int k = 0; k < 3; k++) {
+        mn[k] = std::min(mn[k], p[k]);
+        mx[k] = std::max(mx[k], p[k]);
+    }
+}
+float Box::area() const {  // Bounds3::SurfaceArea (Bounds3.hpp:37-41)
+    float d0 = mx[0] - mn[0], d1 = mx[1] - mn[1], d2 = mx[2] - mn[2];
+    return 2 * (d0 * d1 + d0 * d2 + d1 * d2);
+}
+void Box::centroid(float *c) const {
+    for (int k = 0; k < 3; k++) c[k] = mn[k] * 0.5f + mx[k] * 0.5f;
+}
+int Box::maxExtent() const {  // Bounds3.hpp:26-35
+    float d0 = mx[0] - mn[0], d1 = mx[1] - mn[1], d2 = mx[2] - mn[2];
+    if (d0 > d1 && d0 > d2) return 0;
+    if (d1 > d2) return 1;
+    return 2;
+}
+
+// ---------------------------------------------------------------------------
+// Binned SAH (BVH.cpp:66-256) + depth-first flatten (BVH.cpp:263-280)
+// ---------------------------------------------------------------------------
+namespace {
+struct Builder {
+    const std::vector<Box> &bounds;
+    const std::vector<float> &areas;
+    int maxPrims;
+    struct BN {
+        Box b;
+        int child[2] = {-1, -1};
+        float area = 0.f;
+        int axis = 0, primOffset = 0, primCount = 0;
+    };
+    std::vector<BN> pool;
+    std::vector<int> ordered;
+
+    Builder(const std::vector<Box> &b, const std::vector<float> &a, int mp)
+        : bounds(b), areas(a), maxPrims(std::min(255, mp)) {}
+
+    int rec(std::vector<int> &prims, int start, int end) {
+        pool.emplace_back();
+        const int ni = (int) pool.size() - 1;
+        const int count = end - start;
+        std::vector<float> pc(3 * (size_t) count);
+        Box bb, cb;
+        for (int i = 0; i < count; i++) {
+            const Box &pb = bounds[prims[start + i]];
+            pb.centroid(&pc[3 * i]);
+            bb.unite(pb);
+            cb.unite(&pc[3 * i]);
+        }
+        pool[ni].b = bb;
+        if (count == 1) {
+            pool[ni].primOffset = (int) ordered.size();
+            pool[ni].primCount = 1;
+            pool[ni].area = areas[prims[start]];
+            ordered.push_back(prims[start]);
+            return ni;
+        }
+        if (count <= maxPrims) {
+            pool[ni].primOffset = (int) ordered.size();
+            pool[ni].primCount = count;
+            float a = 0.f;
+            for (int i = start; i < end; i++) {
+                a += areas[prims[i]];
+                ordered.push_back(prims[i]);
+            }
+            pool[ni].area = a;
+            return ni;
+        }
+        const int axis = cb.maxExtent();
+        pool[ni].axis = axis;
+        int mid = (start + end) / 2;
+        if (cb.mx[axis] == cb.mn[axis]) {
+            // all centroids coincide: median split, no reordering
+        } else if (count > 4) {
+            constexpr int NB = 12;
+            float bestCost = kInfF;
+            int bestAxis = axis, bestSplit = NB / 2;
+            const float parentArea = bb.area();
+            for (int a = 0; a < 3; a++) {
+                const float lo = cb.mn[a], hi = cb.mx[a];
+                if (lo == hi) continue;
+                int bcnt[NB] = {0};
+                Box bbox[NB];
+                const float span = hi - lo;
+                for (int i = 0; i < count; i++) {
+                    int b = std::clamp((int) (NB * (pc[3 * i + a] - lo) / span), 0, NB - 1);
+                    bcnt[b]++;
+                    bbox[b].unite(bounds[prims[start + i]]);
+                }
+                float lA[NB - 1], rA[NB - 1];
+                int lC[NB - 1], rC[NB - 1];
+                {
+                    Box bl;
+                    int c = 0;
+                    for (int i = 0; i < NB - 1; i++) {
+                        bl.unite(bbox[i]);
+                        c += bcnt[i];
+                        lA[i] = bl.area();
+                        lC[i] = c;
+                    }
+                }
+                {
+                    Box br;
+                    int c = 0;
+                    for (int i = NB - 2; i >= 0; i--) {
+                        br.unite(bbox[i + 1]);
+                        c += bcnt[i + 1];
+                        rA[i] = br.area();
+                        rC[i] = c;
+                    }
+                }
+                for (int i = 0; i < NB - 1; i++) {
+                    if (lC[i] == 0 || rC[i] == 0) continue;
+                    float cost = 1.f + (lC[i] * lA[i] + rC[i] * rA[i]) / parentArea;
+                    if (cost < bestCost) {
+                        bestCost = cost;
+                        bestAxis = a;
+                        bestSplit = i;
+                    }
+                }
+            }
+            const float lo = cb.mn[bestAxis];
+            const float span = cb.mx[bestAxis] - lo;
+            const float edge = lo + (bestSplit + 1) * span / NB;
+            std::vector<int> tmp;
+            tmp.reserve(count);
+            int nl = 0;
+            for (int i = 0; i < count; i++)
+                if (pc[3 * i + bestAxis] < edge) {
+                    tmp.push_back(prims[start + i]);
+                    nl++;
+                }
+            for (int i = 0; i < count; i++)
+                if (!(pc[3 * i + bestAxis] < edge)) tmp.push_back(prims[start + i]);
+            if (nl == 0 || nl == count) {
+                mid = (start + end) / 2;
+            } else {
+                std::copy(tmp.begin(), tmp.end(), prims.begin() + start);
+                mid = start + nl;
+            }
+        } else {
+            const std::vector<Box> &bd = bounds;
+            std::nth_element(prims.data() + start, prims.data() + mid, prims.data() + end,
+                             [axis, &bd](int x, int y) {
+                                 float cx[3], cy[3];
+                                 bd[x].centroid(cx);
+                                 bd[y].centroid(cy);
+                                 return cx[axis] < cy[axis];
+                             });
+        }
+        const int c0 = rec(prims, start, mid);
+        const int c1 = rec(prims, mid, end);
+        pool[ni].child[0] = c0;
+        pool[ni].child[1] = c1;
+        pool[ni].area = pool[c0].area + pool[c1].area;
+        return ni;
+    }
+
+    uint32_t flatten(RefBVH &out, int bi, uint32_t *off) {
+        const BN &b = pool[bi];
+        const uint32_t my = (*off)++;
+        RefNode &n = out.nodes[my];
+        n.b[0] = b.b.mn[0];
+        n.b[1] = b.b.mx[0];
+        n.b[2] = b.b.mn[1];
+        n.b[3] = b.b.mx[1];
+        n.b[4] = b.b.mn[2];
+        n.b[5] = b.b.mx[2];
+        out.nodeAreas[my] = b.area;
+        if (b.primCount > 0) {
+            out.nodes[my].first = (uint32_t) b.primOffset;
+            out.nodes[my].meta = (uint32_t) b.primCount & 0x00FFFFFFu;
+        } else {
+            out.nodes[my].meta = (uint32_t) b.axis << 24;
+            flatten(out, b.child[0], off);
+            const uint32_t r = flatten(out, b.child[1], off);
+            out.nodes[my].first = r;
+        }
+        return my;
+    }
+};
+}  // namespace
+
+RefBVH buildRefBVH(const std::vector<Box> &bounds, const std::vector<float> &areas, int maxPrims) {
+    RefBVH out;
+    if (bounds.empty()) return out;
+    Builder b(bounds, areas, maxPrims);
+    b.pool.reserve(2 * bounds.size());
+    b.ordered.reserve(bounds.size());
+    std::vector<int> prims(bounds.size());
+    for (size_t i = 0; i < prims.size(); i++) prims[i] = (int) i;
+    const int root = b.rec(prims, 0, (int) prims.size());
+    out.nodes.resize(b.pool.size());
+    out.nodeAreas.resize(b.pool.size());
+    uint32_t off = 0;
+    b.flatten(out, root, &off);
+    out.totalArea = b.pool[root].area;
+    out.ordered = std::move(b.ordered);
+    return out;
+}
+
+// ---------------------------------------------------------------------------
+// Scene
+// ---------------------------------------------------------------------------
+namespace {
+struct TriPre {
+    V3 v0, v1, v2, e1, e2, n;
+    float area;
+};
+TriPre prepTri(const float *t) {  // Triangle ctor (Triangle.hpp:28-34)
+    TriPre p;
+    p.v0 = V3(t[0], t[1], t[2]);
+    p.v1 = V3(t[3], t[4], t[5]);
+    p.v2 = V3(t[6], t[7], t[8]);
+    p.e1 = p.v1 - p.v0;
+    p.e2 = p.v2 - p.v0;
+    p.n = normalize(cross(p.e1, p.e2));
+    p.area = norm(cross(p.e1, p.e2)) * 0.5f;
+    return p;
+}
+Box triBox(const TriPre &p) {  // Triangle::getBounds (Triangle.hpp:182-184)
+    float a[3] = {p.v0.x, p.v0.y, p.v0.z}, b[3] = {p.v1.x, p.v1.y, p.v1.z}, c[3] = {p.v2.x, p.v2.y, p.v2.z};
+    Box bx(a, b);
+    bx.unite(c);
+    return bx;
+}
+
+struct Merger {
+    HostScene &S;
+    const prr_scene_desc *d;
+    const std::vector<TriPre> &tp;
+    std::vector<int> sphSlot;  // per sphere row: leaf slot
+
+    uint32_t leafTri(int obj, const RefNode &n, int maxDepthHere) {
+        (void) maxDepthHere;
+        const RefBVH &m = S.meshes[obj];
+        const int first = d->objects[obj].first;
+        const uint32_t slot0 = (uint32_t) (S.tris.size() / 3);
+        for (uint32_t i = 0; i < n.count(); i++) {
+            const int id = first + m.ordered[n.first + i];
+            const TriPre &t = tp[id];
+            S.tris.push_back(make_float4(t.v0.x, t.v0.y, t.v0.z, t.e1.x));
+            S.tris.push_back(make_float4(t.e1.y, t.e1.z, t.e2.x, t.e2.y));
+            S.tris.push_back(make_float4(t.e2.z, __builtin_bit_cast(float, id), 0.f, 0.f));
+        }
+        return makeLeafRef(slot0, n.count(), false);
+    }
+    static void setBoxes(DNode &dn, const float *l, const float *r) {
+        dn.a = make_float4(l[0], l[1], l[2], l[3]);
+        dn.b = make_float4(l[4], l[5], r[0], r[1]);
+        dn.c = make_float4(r[2], r[3], r[4], r[5]);
+    }
+    uint32_t mesh(int obj, int j, int depth) {
+        const RefBVH &m = S.meshes[obj];
+        const RefNode &n = m.nodes[j];
+        if (n.leaf()) return leafTri(obj, n, depth);
+        const int di = (int) S.nodes.size();
+        S.nodes.emplace_back();
+        S.maxDepth = std::max(S.maxDepth, depth + 1);
+        const int l = j + 1, r = (int) n.first;
+        const uint32_t L = mesh(obj, l, depth + 1);
+        const uint32_t R = mesh(obj, r, depth + 1);
+        setBoxes(S.nodes[di], m.nodes[l].b, m.nodes[r].b);
+        S.nodes[di].d = make_int4((int) L, (int) R, 0, 0);
+        return (uint32_t) di;
+    }
+    uint32_t object(int k, int depth) {
+        if (d->objects[k].kind == PRR_OBJ_SPHERE) {
+            const int row = d->objects[k].first;
+            const uint32_t slot = (uint32_t) S.sph.size();
+            S.sph.push_back(S.sphById[row]);
+            S.sphId.push_back(S.numTris + row);
+            return makeLeafRef(slot, 1, true);
+        }
+        return mesh(k, 0, depth);
+    }
+    uint32_t top(int i, int depth) {
+        const RefNode &n = S.top.nodes[i];
+        if (n.leaf()) return object(S.top.ordered[n.first], depth);
+        const int di = (int) S.nodes.size();
+        S.nodes.emplace_back();
+        S.maxDepth = std::max(S.maxDepth, depth + 1);
+        const int l = i + 1, r = (int) n.first;
+        const uint32_t L = top(l, depth + 1);
+        const uint32_t R = top(r, depth + 1);
+        setBoxes(S.nodes[di], S.top.nodes[l].b, S.top.nodes[r].b);
+        S.nodes[di].d = make_int4((int) L, (int) R, 0, 0);
+        return (uint32_t) di;
+    }
+};
+
+bool hasEmission(const prr_material_desc &m) {  // Material::hasEmission (Material.hpp:113-115)
+    return norm(V3(m.emission[0], m.emission[1], m.emission[2])) > kEps;
+}
+}  // namespace
+
+bool buildHostScene(const prr_scene_desc *d, HostScene &S, std::string &err) {
+    if (!d) {
+        err = "scene descriptor is NULL";
+        return false;
+    }
+    if (d->num_objects <= 0 || !d->objects) {
+        err = "scene has no objects";
+        return false;
+    }
+    if (d->num_materials <= 0 || !d->materials) {
+        err = "scene has no materials";
+        return false;
+    }
+    if (d->num_triangles < 0 || d->num_spheres < 0 || (d->num_triangles > 0 && !d->triangles) ||
+        (d->num_spheres > 0 && !d->spheres)) {
+        err = "bad triangle / sphere arrays";
+        return false;
+    }
+    if ((int64_t) d->num_triangles + d->num_spheres >= (int64_t) 1 << 27) {
+        err = "too many primitives (limit 2^27)";
+        return false;
+    }
+    if (d->num_point_lights > 1 || (d->num_point_lights > 0 && !d->point_lights)) {
+        err = "at most one point light is supported";
+        return false;
+    }
+    for (int k = 0; k < d->num_objects; k++) {
+        const prr_object_desc &o = d->objects[k];
+        if (o.material < 0 || o.material >= d->num_materials) {
+            err = "object " + std::to_string(k) + ": material index out of range";
+            return false;
+        }
+        if (o.kind == PRR_OBJ_MESH) {
+            if (o.count <= 0 || o.first < 0 || (int64_t) o.first + o.count > d->num_triangles) {
+                err = "object " + std::to_string(k) + ": triangle range out of bounds";
+                return false;
+            }
+        } else if (o.kind == PRR_OBJ_SPHERE) {
+            if (o.first < 0 || o.first >= d->num_spheres) {
+                err = "object " + std::to_string(k) + ": sphere row out of range";
+                return false;
+            }
+        } else {
+            err = "object " + std::to_string(k) + ": unknown kind";
+            return false;
+        }
+    }
+    for (int i = 0; i < d->num_materials; i++) {
+        const int t = d->materials[i].type;
+        if (t < PRR_MAT_DIFFUSE || t > PRR_MAT_EMISSIVE) {
+            err = "material " + std::to_string(i) + ": unknown type";
+            return false;
+        }
+    }
+    if (d->env.width < 0 || d->env.height < 0 || (d->env.width > 0 && (d->env.height <= 0 || !d->env.pixels))) {
+        err = "bad environment map";
+        return false;
+    }
+    S.numTris = d->num_triangles;
+    S.numSpheres = d->num_spheres;
+
+    // triangles (Triangle ctor), in parallel
+    std::vector<TriPre> tp(d->num_triangles);
+    const unsigned nth = std::max(1u, std::min(32u, std::thread::hardware_concurrency()));
+    {
+        std::vector<std::thread> th;
+        for (unsigned w = 0; w < nth; w++)
+            th.emplace_back([&, w]() {
+                for (int64_t i = w; i < d->num_triangles; i += nth) tp[i] = prepTri(d->triangles + 9 * i);
+            });
+        for (auto &t : th) t.join();
+    }
+    // spheres (Sphere ctor, Sphere.hpp:16-17)
+    S.sphById.resize(d->num_spheres);
+    for (int i = 0; i < d->num_spheres; i++) {
+        const float *p = d->spheres + 4 * (size_t) i;
+        if (!(p[3] > 0.f)) {
+            err = "sphere " + std::to_string(i) + ": radius must be > 0";
+            return false;
+        }
+        S.sphById[i] = make_float4(p[0], p[1], p[2], p[3]);
+    }
+    // per-object bounds / area and mesh sub-BVHs (maxPrims 4), meshes in parallel
+    const int nObj = d->num_objects;
+    S.meshes.assign(nObj, RefBVH());
+    S.objBounds.assign(nObj, Box());
+    S.objArea.assign(nObj, 0.f);
+    std::atomic<int> next(0);
+    {
+        std::vector<std::thread> th;
+        for (unsigned w = 0; w < nth; w++)
+            th.emplace_back([&]() {
+                for (int k = next++; k < nObj; k = next++) {
+                    const prr_object_desc &o = d->objects[k];
+                    if (o.kind == PRR_OBJ_SPHERE) {
+                        const float4 s = S.sphById[o.first];
+                        float a[3] = {s.x - s.w, s.y - s.w, s.z - s.w}, b[3] = {s.x + s.w, s.y + s.w, s.z + s.w};
+                        S.objBounds[k] = Box(a, b);                 // Sphere::getBounds (Sphere.hpp:47-50)
+                        S.objArea[k] = 4 * kPi * s.w * s.w;         // Sphere ctor
+                        continue;
+                    }
+                    float mn[3] = {kInfF, kInfF, kInfF}, mx[3] = {-kInfF, -kInfF, -kInfF};
+                    std::vector<Box> bx(o.count);
+                    std::vector<float> ar(o.count);
+                    float area = 0.f;
+                    for (int i = 0; i < o.count; i++) {
+                        const TriPre &t = tp[o.first + i];
+                        const V3 vs[3] = {t.v0, t.v1, t.v2};
+                        for (const V3 &p : vs) {
+                            mn[0] = std::min(mn[0], p.x);
+                            mn[1] = std::min(mn[1], p.y);
+                            mn[2] = std::min(mn[2], p.z);
+                            mx[0] = std::max(mx[0], p.x);
+                            mx[1] = std::max(mx[1], p.y);
+                            mx[2] = std::max(mx[2], p.z);
+                        }
+                        bx[i] = triBox(t);
+                        ar[i] = t.area;
+                        area += t.area;  // MeshTriangle ctor (Triangle.hpp:116-124)
+                    }
+                    S.objBounds[k] = Box(mn, mx);
+                    S.objArea[k] = area;
+                    S.meshes[k] = buildRefBVH(bx, ar, 4);
+                }
+            });
+        for (auto &t : th) t.join();
+    }
+    // scene BVH over objects (maxPrims 1, SAH)
+    S.top = buildRefBVH(S.objBounds, S.objArea, 1);
+
+    // merged device tree
+    S.primN.assign((size_t) d->num_triangles + d->num_spheres, make_float4(0, 0, 0, 0));
+    for (int k = 0; k < nObj; k++) {
+        const prr_object_desc &o = d->objects[k];
+        const float mbits = __builtin_bit_cast(float, o.material);
+        if (o.kind == PRR_OBJ_MESH) {
+            for (int i = 0; i < o.count; i++) {
+                const TriPre &t = tp[o.first + i];
+                S.primN[o.first + i] = make_float4(t.n.x, t.n.y, t.n.z, mbits);
+            }
+        } else {
+            S.primN[(size_t) d->num_triangles + o.first] = make_float4(0, 0, 0, mbits);
+        }
+    }
+    S.nodes.clear();
+    S.tris.clear();
+    S.tris.reserve(3 * (size_t) d->num_triangles);
+    S.maxDepth = 0;
+    Merger mg{S, d, tp, {}};
+    S.root = mg.top(0, 0);
+    for (int k = 0; k < 6; k++) S.rootBox[k] = S.top.nodes[0].b[k];
+    if (S.maxDepth + 1 > 64) {
+        err = "BVH deeper than the 64-entry traversal stack";
+        return false;
+    }
+    if (S.tris.empty()) S.tris.push_back(make_float4(0, 0, 0, 0));  // keep buffers non-empty
+    if (S.sph.empty()) {
+        S.sph.push_back(make_float4(0, 0, 0, 0));
+        S.sphId.push_back(-1);
+    }
+    if (S.nodes.empty()) S.nodes.emplace_back();
+    if (S.sphById.empty()) S.sphById.push_back(make_float4(0, 0, 0, 0));
+
+    // materials
+    S.mats.resize(d->num_materials);
+    for (int i = 0; i < d->num_materials; i++) {
+        const prr_material_desc &m = d->materials[i];
+        S.mats[i].base_type = make_float4(m.base_color[0], m.base_color[1], m.base_color[2], (float) m.type);
+        S.mats[i].params = make_float4(m.roughness, m.metallic, m.ior, hasEmission(m) ? 1.f : 0.f);
+        S.mats[i].emission = make_float4(m.emission[0], m.emission[1], m.emission[2], 0.f);
+    }
+    // emitters (Scene::buildBVH, Scene.cpp:19-25) and light trees
+    S.totalEmitArea = 0.f;
+    S.emitters.clear();
+    S.lnodes.clear();
+    S.ltri.clear();
+    for (int k = 0; k < nObj; k++) {
+        const prr_object_desc &o = d->objects[k];
+        if (!hasEmission(d->materials[o.material])) continue;
+        DEmitter e{};
+        e.area = S.objArea[k];
+        e.kind = o.kind == PRR_OBJ_SPHERE ? 1 : 0;
+        e.material = o.material;
+        if (o.kind == PRR_OBJ_SPHERE) {
+            e.lt_root = o.first;
+        } else {
+            const RefBVH &m = S.meshes[k];
+            const int nodeBase = (int) S.lnodes.size();
+            const int slotBase = (int) (S.ltri.size() / 4);
+            e.lt_root = nodeBase;
+            e.total = m.totalArea;
+            for (size_t j = 0; j < m.nodes.size(); j++) {
+                const RefNode &n = m.nodes[j];
+                DLightNode ln{};
+                if (n.leaf()) {
+                    ln.first = slotBase + (int) n.first;
+                    ln.count = (int) n.count();
+                } else {
+                    ln.first = nodeBase + (int) n.first;
+                    ln.count = 0;
+                    ln.left_area = m.nodeAreas[j + 1];
+                }
+                S.lnodes.push_back(ln);
+            }
+            for (int li : m.ordered) {
+                const TriPre &t = tp[o.first + li];
+                S.ltri.push_back(make_float4(t.v0.x, t.v0.y, t.v0.z, t.area));
+                S.ltri.push_back(make_float4(t.v1.x, t.v1.y, t.v1.z, 0.f));
+                S.ltri.push_back(make_float4(t.v2.x, t.v2.y, t.v2.z, 0.f));
+                S.ltri.push_back(make_float4(t.n.x, t.n.y, t.n.z, 0.f));
+            }
+        }
+        S.emitters.push_back(e);
+        S.totalEmitArea += S.objArea[k];
+    }
+    if (S.emitters.empty()) S.emitters.push_back(DEmitter{});
+    if (S.lnodes.empty()) S.lnodes.push_back(DLightNode{});
+    if (S.ltri.empty()) S.ltri.assign(4, make_float4(0, 0, 0, 0));
+
+    // environment (EnvMap.hpp:189-224)
+    S.envW = d->env.width;
+    S.envH = d->env.height;
+    if (S.envW > 0) {
+        const int W = S.envW, H = S.envH;
+        S.envPx.resize((size_t) W * H);
+        S.envCdf.assign((size_t) W * H, 0.f);
+        S.envMarg.assign(H, 0.f);
+        std::vector<float> rowSums(H, 0.f);
+        for (size_t i = 0; i < (size_t) W * H; i++) {
+            const float *p = d->env.pixels + 3 * i;
+            const float lum = 0.2126f * p[0] + 0.7152f * p[1] + 0.0722f * p[2];
+            S.envPx[i] = make_float4(p[0], p[1], p[2], lum);
+        }
+        for (int row = 0; row < H; row++) {
+            const float theta = (row + 0.5f) / (float) H * kPi;
+            const float sinT = std::sin(theta);
+            float rs = 0.f;
+            for (int col = 0; col < W; col++) {
+                rs += S.envPx[(size_t) row * W + col].w * sinT;
+                S.envCdf[(size_t) row * W + col] = rs;
+            }
+            if (rs > 0.f)
+                for (int col = 0; col < W; col++) S.envCdf[(size_t) row * W + col] /= rs;
+            rowSums[row] = rs;
+        }
+        float t = 0.f;
+        for (int row = 0; row < H; row++) {
+            t += rowSums[row];
+            S.envMarg[row] = t;
+        }
+        S.envTotal = t;
+        if (t > 0.f)
+            for (float &v : S.envMarg) v /= t;
+    } else {
+        S.envPx.assign(1, make_float4(0, 0, 0, 0));
+        S.envCdf.assign(1, 0.f);
+        S.envMarg.assign(1, 0.f);
+        S.envTotal = 0.f;
+    }
+
+    // camera (Camera::init, Camera.hpp:43-56)
+    const prr_camera_desc &c = d->camera;
+    V3 eye(c.eye[0], c.eye[1], c.eye[2]), tgt(c.target[0], c.target[1], c.target[2]), up(c.up[0], c.up[1], c.up[2]);
+    V3 fwd = normalize(tgt - eye);
+    V3 right = normalize(cross(fwd, up));
+    V3 cu = cross(right, fwd);
+    const float halfFovH = c.fovh * kPi / 180.0f * 0.5f;
+    float focus = c.focus_distance;
+    if (focus < 0.0f) {
+        V3 dd = tgt - eye;
+        focus = std::sqrt(dd.x * dd.x + dd.y * dd.y + dd.z * dd.z);
+    }
+    const V3 *vs[4] = {&eye, &fwd, &right, &cu};
+    float *dst[4] = {S.camEye, S.camFwd, S.camRight, S.camUp};
+    for (int i = 0; i < 4; i++) {
+        dst[i][0] = vs[i]->x;
+        dst[i][1] = vs[i]->y;
+        dst[i][2] = vs[i]->z;
+    }
+    S.camAperture = c.aperture;
+    S.camFocus = focus;
+    S.camTanHalf = std::tan(halfFovH);
+    for (int k = 0; k < 3; k++) S.background[k] = d->background[k];
+    S.numPoints = d->num_point_lights;
+    if (S.numPoints > 0)
+        for (int k = 0; k < 3; k++) {
+            S.pointPos[k] = d->point_lights[0].position[k];
+            S.pointInt[k] = d->point_lights[0].intensity[k];
+        }
+    return true;
+}
+
+}  // namespace host
+}  // namespace prr

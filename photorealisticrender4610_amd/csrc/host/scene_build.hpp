@@ -1,0 +1,90 @@
+// scene_build.hpp — host side of Scene::buildBVH for the device library.
+//
+// Builds the reference's two-level BVH with its binned-SAH rules (scene
+// level maxPrims 1 over objects, mesh level maxPrims 4 over triangles;
+// BVH.cpp:13-280, Scene.cpp:15-27, Triangle.hpp:173-179), keeps each level in
+// the reference's flattened layout for inspection, and produces the device
+// arrays: one merged tree of two-child 64-byte nodes, leaf-ordered triangle
+// records, emitter light trees (BVH.cpp:478-514) and the environment CDF
+// (EnvMap.hpp:189-224).
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../../include/prr_abi.h"
+#include "../device/dev_common.hpp"
+
+namespace prr {
+namespace host {
+
+struct Box {
+    float mn[3], mx[3];
+    Box();
+    Box(const float *a, const float *b);  // Bounds3(p1, p2): fmin / fmax
+    void unite(const Box &o);             // Union(Bounds3, Bounds3): std::min / std::max
+    void unite(const float *p);           // Union(Bounds3, Vector3f)
+    float area() const;                   // Bounds3::SurfaceArea (float expression)
+    void centroid(float *c) const;        // 0.5 * pMin + 0.5 * pMax
+    int maxExtent() const;
+};
+
+// One level of the reference BVH in its flattened layout (BVH.hpp:34-67).
+struct RefNode {
+    float b[6];
+    uint32_t first;
+    uint32_t meta;
+    bool leaf() const { return (meta & 0x00FFFFFFu) > 0u; }
+    uint32_t count() const { return meta & 0x00FFFFFFu; }
+};
+struct RefBVH {
+    std::vector<RefNode> nodes;
+    std::vector<int> ordered;      // local primitive index in leaf order
+    std::vector<float> nodeAreas;  // BVH.hpp:135
+    float totalArea = 0.f;
+};
+
+// Binned-SAH build of n primitives given their boxes and areas.
+RefBVH buildRefBVH(const std::vector<Box> &bounds, const std::vector<float> &areas, int maxPrims);
+
+struct HostScene {
+    // reference-layout levels
+    RefBVH top;                  // over objects
+    std::vector<RefBVH> meshes;  // per object (empty for spheres)
+    std::vector<Box> objBounds;
+    std::vector<float> objArea;
+    // device staging
+    std::vector<DNode> nodes;
+    std::vector<float4> tris;      // 3 per leaf slot
+    std::vector<float4> sph;       // leaf-order spheres
+    std::vector<int> sphId;        // leaf-order sphere -> global prim id
+    std::vector<float4> primN;     // per global prim id
+    std::vector<float4> sphById;   // per sphere row
+    std::vector<DMaterial> mats;
+    std::vector<DEmitter> emitters;
+    std::vector<DLightNode> lnodes;
+    std::vector<float4> ltri;      // 4 per light leaf slot
+    std::vector<float4> envPx;
+    std::vector<float> envCdf, envMarg;
+    float envTotal = 0.f;
+    int envW = 0, envH = 0;
+    uint32_t root = 0;
+    float rootBox[6] = {0, 0, 0, 0, 0, 0};
+    int maxDepth = 0;
+    float totalEmitArea = 0.f;
+    int numTris = 0, numSpheres = 0;
+    // camera
+    float camEye[3], camFwd[3], camRight[3], camUp[3];
+    float camAperture = 0.f, camFocus = 1.f, camTanHalf = 1.f;
+    float background[3] = {0, 0, 0};
+    int numPoints = 0;
+    float pointPos[3] = {0, 0, 0}, pointInt[3] = {0, 0, 0};
+};
+
+// Validates the descriptor and builds everything. Returns false with a
+// message on invalid input.
+bool buildHostScene(const prr_scene_desc *d, HostScene &out, std::string &err);
+
+}  // namespace host
+}  // namespace prr
