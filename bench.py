@@ -38,29 +38,28 @@ WORKLOADS = {
 
 def cpu_baseline(sc, w, h, depth, seed, target_s=12.0):
     """The oracle built with the reference's release flags (-O3 -ffast-math,
-    OpenMP), timed on this host's cores on a band of image rows at 1 spp."""
+    OpenMP), timed on this host's cores on a bounded sample of the same render
+    (a band of rows, or the whole frame at a few spp), sized to ~target_s."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle as O
     threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
     t0 = time.time()
     osc = O.OracleScene(sc, fast=True)
     build_s = time.time() - t0
-    # probe a small band, then size the sample to ~target_s
-    r0 = h // 2
-    rows = 8
+    r0, rows = h // 2 - 8, 16
     t0 = time.time()
-    _, _, cnt = osc.render(w, h, 1, depth, seed, rows=(r0, r0 + rows), threads=threads)
-    dt = time.time() - t0
-    per_row = max(dt / rows, 1e-4)
-    rows = int(min(h, max(rows, target_s / per_row)))
+    osc.render(w, h, 1, depth, seed, rows=(r0, r0 + rows), threads=threads)
+    per_row = max((time.time() - t0) / rows, 1e-5)
+    rows = int(min(h, max(16, target_s / per_row)))
+    spp = int(max(1, min(64, target_s / (per_row * rows))))
     r0 = max(0, h // 2 - rows // 2)
     t0 = time.time()
-    _, _, cnt = osc.render(w, h, 1, depth, seed, rows=(r0, r0 + rows), threads=threads)
+    _, _, cnt = osc.render(w, h, spp, depth, seed, rows=(r0, r0 + rows), threads=threads)
     dt = time.time() - t0
     rays = cnt["closest"] + cnt["any"]
-    return {"value": rays / dt / 1e6, "unit": "Mrays/s", "cores": threads, "kind": "port",
-            "sample": "rows %d-%d of the %dx%d image at 1 spp (%d samples, %d rays) in %.1f s; "
-                      "oracle scene build %.1f s excluded" % (r0, r0 + rows, w, h, cnt["samples"], rays, dt,
+    return {"value": round(rays / dt / 1e6, 3), "unit": "Mrays/s", "cores": threads, "kind": "port",
+            "sample": "rows %d-%d of the %dx%d image at %d spp (%d samples, %d rays) in %.1f s; "
+                      "oracle scene build %.1f s excluded" % (r0, r0 + rows, w, h, spp, cnt["samples"], rays, dt,
                                                               build_s)}
 
 

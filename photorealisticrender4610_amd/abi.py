@@ -201,8 +201,17 @@ class Context:
             pass
 
     def _bind_stream(self):
+        """Run on torch's current stream. torch's default stream has handle 0,
+        which the ABI reads as 'the context's own stream'; in that case the
+        call is followed by prr_synchronize (see _done)."""
         s = self.torch.cuda.current_stream(self.device)
-        self._check(lib().prr_set_stream(self.h, vp(s.cuda_stream)), self.h)
+        self._own = s.cuda_stream == 0
+        self._check(lib().prr_set_stream(self.h, vp(s.cuda_stream) if s.cuda_stream else None), self.h)
+
+    def _done(self, rc):
+        self._check(rc, self.h)
+        if getattr(self, "_own", True):
+            self._check(lib().prr_synchronize(self.h), self.h)
 
     def scene(self, sc):
         return Scene(self, sc)
@@ -220,8 +229,8 @@ class Context:
         prim = T.empty(n, dtype=T.int32, device=self.device)
         t, u, v = (T.empty(n, dtype=T.float32, device=self.device) for _ in range(3))
         self._bind_stream()
-        self._check(lib().prr_trace_closest(self.h, scene.h, n, _ptr(org), _ptr(d), _ptr(prim), _ptr(t), _ptr(u),
-                                            _ptr(v)), self.h)
+        self._done(lib().prr_trace_closest(self.h, scene.h, n, _ptr(org), _ptr(d), _ptr(prim), _ptr(t), _ptr(u),
+                                            _ptr(v)))
         return prim, t, u, v
 
     def trace_any(self, scene, org, d, tmax):
@@ -230,7 +239,7 @@ class Context:
         n = org.shape[0]
         hit = T.empty(n, dtype=T.uint8, device=self.device)
         self._bind_stream()
-        self._check(lib().prr_trace_any(self.h, scene.h, n, _ptr(org), _ptr(d), _ptr(tmax), _ptr(hit)), self.h)
+        self._done(lib().prr_trace_any(self.h, scene.h, n, _ptr(org), _ptr(d), _ptr(tmax), _ptr(hit)))
         return hit
 
     def bsdf_eval(self, wi, wo, nrm, params):
@@ -240,8 +249,7 @@ class Context:
         f = T.empty((n, 3), dtype=T.float32, device=self.device)
         pdf = T.empty(n, dtype=T.float32, device=self.device)
         self._bind_stream()
-        self._check(lib().prr_bsdf_eval(self.h, n, _ptr(wi), _ptr(wo), _ptr(nrm), _ptr(params), _ptr(f), _ptr(pdf)),
-                    self.h)
+        self._done(lib().prr_bsdf_eval(self.h, n, _ptr(wi), _ptr(wo), _ptr(nrm), _ptr(params), _ptr(f), _ptr(pdf)))
         return f, pdf
 
     def bsdf_sample(self, wo, nrm, params, u):
@@ -253,8 +261,8 @@ class Context:
         pdf = T.empty(n, dtype=T.float32, device=self.device)
         lobe = T.empty(n, dtype=T.int32, device=self.device)
         self._bind_stream()
-        self._check(lib().prr_bsdf_sample(self.h, n, _ptr(wo), _ptr(nrm), _ptr(params), _ptr(u), _ptr(wi), _ptr(f),
-                                          _ptr(pdf), _ptr(lobe)), self.h)
+        self._done(lib().prr_bsdf_sample(self.h, n, _ptr(wo), _ptr(nrm), _ptr(params), _ptr(u), _ptr(wi), _ptr(f),
+                                          _ptr(pdf), _ptr(lobe)))
         return wi, f, pdf, lobe
 
     def env_sample(self, scene, u2):
@@ -264,7 +272,7 @@ class Context:
         d = T.empty((n, 3), dtype=T.float32, device=self.device)
         pdf = T.empty(n, dtype=T.float32, device=self.device)
         self._bind_stream()
-        self._check(lib().prr_env_sample(self.h, scene.h, n, _ptr(u2), _ptr(d), _ptr(pdf)), self.h)
+        self._done(lib().prr_env_sample(self.h, scene.h, n, _ptr(u2), _ptr(d), _ptr(pdf)))
         return d, pdf
 
     def env_eval(self, scene, dirs):
@@ -274,7 +282,7 @@ class Context:
         rad = T.empty((n, 3), dtype=T.float32, device=self.device)
         pdf = T.empty(n, dtype=T.float32, device=self.device)
         self._bind_stream()
-        self._check(lib().prr_env_eval(self.h, scene.h, n, _ptr(dirs), _ptr(rad), _ptr(pdf)), self.h)
+        self._done(lib().prr_env_eval(self.h, scene.h, n, _ptr(dirs), _ptr(rad), _ptr(pdf)))
         return rad, pdf
 
     def camera_rays(self, scene, w, h, pix, offs):
@@ -285,7 +293,7 @@ class Context:
         o = T.empty((n, 3), dtype=T.float32, device=self.device)
         d = T.empty((n, 3), dtype=T.float32, device=self.device)
         self._bind_stream()
-        self._check(lib().prr_camera_rays(self.h, scene.h, w, h, n, _ptr(pix), _ptr(offs), _ptr(o), _ptr(d)), self.h)
+        self._done(lib().prr_camera_rays(self.h, scene.h, w, h, n, _ptr(pix), _ptr(offs), _ptr(o), _ptr(d)))
         return o, d
 
     def tonemap(self, op, lin, exposure=1.0):
@@ -296,7 +304,7 @@ class Context:
         rgb8 = T.empty((n, 3), dtype=T.uint8, device=self.device)
         opi = TONEMAP[op] if isinstance(op, str) else int(op)
         self._bind_stream()
-        self._check(lib().prr_tonemap(self.h, opi, n, exposure, _ptr(lin), _ptr(disp), _ptr(rgb8)), self.h)
+        self._done(lib().prr_tonemap(self.h, opi, n, exposure, _ptr(lin), _ptr(disp), _ptr(rgb8)))
         return disp, rgb8
 
     def render(self, scene, width, height, spp, max_depth, seed=0, clamp=10.0, sample_offset=0, out=None,
@@ -310,7 +318,7 @@ class Context:
                          sample_offset=sample_offset, seed=seed, flags=1 if kernel_timing else 0)
         st = Stats()
         self._bind_stream()
-        self._check(lib().prr_render(self.h, scene.h, C.byref(p), _ptr(out), C.byref(st)), self.h)
+        self._done(lib().prr_render(self.h, scene.h, C.byref(p), _ptr(out), C.byref(st)))
         return out, {"rays_closest": st.rays_closest, "rays_any": st.rays_any, "samples": st.samples,
                      "photons_stored": st.photons_stored, "ms": st.ms_total, "ms_extend": st.ms_extend,
                      "ms_shadow": st.ms_shadow, "ms_shade": st.ms_shade, "launches_extend": st.launches_extend,

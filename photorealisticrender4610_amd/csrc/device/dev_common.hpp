@@ -91,7 +91,7 @@ struct Pcg {
         r.state = 0;
         r.inc = ((uint64_t) sample << 1) | 1u;
         r.nextU();
-        r.state += mix64(seed * 0x9E3779B97F4A7C15ULL + pixel);
+        r.state += mix64(mix64(seed * 0x9E3779B97F4A7C15ULL + pixel) ^ ((uint64_t) sample * 0xD1B54A32D192ED03ULL));
         r.nextU();
         return r;
     }
