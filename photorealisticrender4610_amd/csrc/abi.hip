@@ -1,6 +1,7 @@
 // abi.hip — the C ABI of libprr.so (include/prr_abi.h).
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 #include <string>
@@ -23,6 +24,8 @@ struct prr_context {
     std::string err;
     int numCU = 1;
     int traceBlocks = 1, shadeBlocks = 1;
+    int refillAt = 32;
+    int stackDepth = 8;
     uint2 *spill = nullptr;
     int spillLanes = 0;
     // render buffers
@@ -113,10 +116,20 @@ int prr_create(int device, prr_context **out) {
     }
     c->stream = c->own;
     c->numCU = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 1;
-    c->traceBlocks = c->numCU * 4;
+    // persistent traversal grid: 8 blocks of 256 lanes per CU (with 8-entry
+    // LDS stacks, 16 KB per block); PRR_TRACE_BLOCKS_PER_CU, PRR_STACK_LDS and
+    // PRR_REFILL_AT override the measured defaults for tuning sweeps
+    int tbpc = 8;
+    if (const char *ev = getenv("PRR_TRACE_BLOCKS_PER_CU")) tbpc = std::max(1, std::min(16, atoi(ev)));
+    c->traceBlocks = c->numCU * tbpc;
+    if (const char *ev = getenv("PRR_REFILL_AT")) c->refillAt = std::max(1, std::min(64, atoi(ev)));
+    if (const char *ev = getenv("PRR_STACK_LDS")) {  // LDS stack entries per lane: 2, 4, 8, 16 or 32
+        const int v = atoi(ev);
+        if (v == 2 || v == 4 || v == 8 || v == 16 || v == 32) c->stackDepth = v;
+    }
     c->shadeBlocks = c->numCU * 8;
     c->spillLanes = c->traceBlocks * kTraceBlock;
-    e = hipMalloc(&c->spill, (size_t) c->spillLanes * (kSpill - kShort) * sizeof(uint2));
+    e = hipMalloc(&c->spill, (size_t) c->spillLanes * kSpill * sizeof(uint2));
     if (e == hipSuccess) e = hipMalloc(&c->stats, 8 * sizeof(unsigned long long));
     if (e != hipSuccess) {
         std::string m = std::string("device alloc: ") + hipGetErrorString(e);
@@ -276,7 +289,7 @@ int prr_trace_closest(prr_context *c, const prr_scene *s, int64_t n, const float
     if (!c || !s) return fail(c, PRR_ERR_INVALID, "NULL context or scene");
     if (n < 0 || (n > 0 && (!org || !dir || !prim || !t))) return fail(c, PRR_ERR_INVALID, "bad arrays");
     HIPCK(c, hipSetDevice(c->device));
-    HIPCK(c, launchTraceClosest(s->view, n, org, dir, prim, t, u, v, c->spill, c->spillLanes, c->stream));
+    HIPCK(c, launchTraceClosest(s->view, n, org, dir, prim, t, u, v, c->spill, c->spillLanes, c->stackDepth, c->stream));
     return PRR_OK;
 }
 
@@ -285,7 +298,7 @@ int prr_trace_any(prr_context *c, const prr_scene *s, int64_t n, const float *or
     if (!c || !s) return fail(c, PRR_ERR_INVALID, "NULL context or scene");
     if (n < 0 || (n > 0 && (!org || !dir || !tmax || !hit))) return fail(c, PRR_ERR_INVALID, "bad arrays");
     HIPCK(c, hipSetDevice(c->device));
-    HIPCK(c, launchTraceAny(s->view, n, org, dir, tmax, hit, c->spill, c->spillLanes, c->stream));
+    HIPCK(c, launchTraceAny(s->view, n, org, dir, tmax, hit, c->spill, c->spillLanes, c->stackDepth, c->stream));
     return PRR_OK;
 }
 
@@ -348,7 +361,10 @@ int prr_tonemap(prr_context *c, int32_t op, int64_t n, float exposure, const flo
 // ---------------------------------------------------------------------------
 // render
 // ---------------------------------------------------------------------------
-static const int64_t kMaxSlots = (int64_t) 1 << 22;
+// Paths in flight per chunk. Path state is ~340 B per slot, so 2^24 slots
+// hold ~5.7 GB of HBM; large chunks amortise the per-bounce tail of the
+// traversal kernels (a few long rays) over many samples.
+static const int64_t kMaxSlots = (int64_t) 1 << 24;
 
 static int ensureRenderBufs(prr_context *c, int64_t slots, int maxDepth) {
     if (slots <= c->pbSlots && maxDepth <= c->maxDepthAlloc) return PRR_OK;
@@ -388,6 +404,7 @@ static int ensureRenderBufs(prr_context *c, int64_t slots, int maxDepth) {
     A(&p.sqDir, 3 * slots);
     A(&p.qcnt, (size_t) maxDepth + 2);
     A(&p.sqcnt, (size_t) maxDepth + 2);
+    A(&p.fetch, 2 * ((size_t) maxDepth + 2));
     if (e != hipSuccess) {
         freeRenderBufs(c);
         return fail(c, PRR_ERR_NOMEM, std::string("render buffers: ") + hipGetErrorString(e));
@@ -404,7 +421,7 @@ int prr_render(prr_context *c, const prr_scene *s, const prr_render_params *p, f
         p->sample_offset < 0)
         return fail(c, PRR_ERR_INVALID, "bad render parameters");
     const int64_t pixels = (int64_t) p->width * p->height;
-    if (pixels > kMaxSlots) return fail(c, PRR_ERR_UNSUPPORTED, "image larger than 4M pixels");
+    if (pixels > kMaxSlots) return fail(c, PRR_ERR_UNSUPPORTED, "image larger than 16M pixels");
     if (p->num_photons > 0) return fail(c, PRR_ERR_UNSUPPORTED, "photon map requested (use the photon entry points)");
     HIPCK(c, hipSetDevice(c->device));
     const int spc = (int) std::max<int64_t>(1, std::min<int64_t>(p->spp, kMaxSlots / pixels));
@@ -458,10 +475,11 @@ int prr_render(prr_context *c, const prr_scene *s, const prr_render_params *p, f
         ci.clampInd = p->indirect_clamp;
         HIPCK(c, hipMemsetAsync(c->pb.qcnt, 0, ((size_t) p->max_depth + 2) * sizeof(uint32_t), st));
         HIPCK(c, hipMemsetAsync(c->pb.sqcnt, 0, ((size_t) p->max_depth + 2) * sizeof(uint32_t), st));
+        HIPCK(c, hipMemsetAsync(c->pb.fetch, 0, 2 * ((size_t) p->max_depth + 2) * sizeof(uint32_t), st));
         HIPCK(c, launchRaygen(s->view, c->pb, ci, c->shadeBlocks, st));
         for (int b = 0; b <= p->max_depth; b++) {
-            TIMED(0, launchExtend(s->view, c->pb, b, c->spill, traceGrid, st));
-            if (b > 0) TIMED(1, launchShadow(s->view, c->pb, b - 1, c->spill, traceGrid, st));
+            TIMED(0, launchExtend(s->view, c->pb, b, c->spill, traceGrid, c->refillAt, c->stackDepth, st));
+            if (b > 0) TIMED(1, launchShadow(s->view, c->pb, b - 1, c->spill, traceGrid, c->refillAt, c->stackDepth, st));
             TIMED(2, launchShade(s->view, c->pb, ci, b, c->shadeBlocks, st));
         }
         HIPCK(c, launchAccumulate(c->pb, ci, c->fb, st));

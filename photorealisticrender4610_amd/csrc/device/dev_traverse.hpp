@@ -17,8 +17,8 @@
 
 namespace prr {
 
-constexpr int kShort = 16;        // LDS stack entries per lane
-constexpr int kSpill = 64;        // total stack capacity per lane (reference: 64)
+constexpr int kShort = 16;        // default LDS stack entries per lane
+constexpr int kSpill = 64;        // spill entries per lane (the reference's stack holds 64)
 constexpr int kTraceBlock = 256;  // threads per block of traversal kernels
 
 struct RayPre {
@@ -131,24 +131,27 @@ PRR_DEV bool sphHit(const float4 &s, const V3 &o, const V3 &d, float &t) {
     return true;
 }
 
-// Per-lane stack: top kShort entries in LDS ([kShort][kTraceBlock] uint2),
-// the rest in a global spill region of (kSpill - kShort) entries per lane.
+// Per-lane stack: the top `depth` entries (a power of two) live in LDS laid
+// out [slot][lane] (uint2 node ref + box t), the rest in a global spill
+// region of kSpill entries per lane.
 struct LaneStack {
     uint2 *lds;    // base of this block's LDS stack
-    uint2 *spill;  // this lane's spill region (kSpill - kShort entries)
+    uint2 *spill;  // this lane's spill region (kSpill entries)
     int lane;      // threadIdx.x
     int sp;
+    int depth;     // LDS entries per lane (power of two)
     PRR_DEV void push(uint32_t ref, float t) {
         const uint2 e = make_uint2(ref, __float_as_uint(t));
-        if (sp >= kShort) spill[sp - kShort] = lds[(sp % kShort) * kTraceBlock + lane];
-        lds[(sp % kShort) * kTraceBlock + lane] = e;
+        const int slot = (sp & (depth - 1)) * kTraceBlock + lane;
+        if (sp >= depth) spill[sp - depth] = lds[slot];
+        lds[slot] = e;
         sp++;
     }
     PRR_DEV uint2 pop() {
         sp--;
-        const int slot = (sp % kShort) * kTraceBlock + lane;
+        const int slot = (sp & (depth - 1)) * kTraceBlock + lane;
         uint2 e = lds[slot];
-        if (sp >= kShort) lds[slot] = spill[sp - kShort];
+        if (sp >= depth) lds[slot] = spill[sp - depth];
         return e;
     }
 };
@@ -276,6 +279,153 @@ PRR_DEV bool traceAny(const SceneView &s, const V3 &o, const V3 &d, float tMax, 
         cur = st.pop().x;
     }
     return false;
+}
+
+// ---------------------------------------------------------------------------
+// Resumable traversal for persistent kernels: one node per call, so a lane
+// whose ray finished can fetch a new ray while the rest of its wave keeps
+// traversing (while-while with per-lane replenishment).
+// ---------------------------------------------------------------------------
+struct TravState {
+    RayPre r;
+    V3 o, d;
+    float tmax;  // any-hit range
+    uint32_t cur;
+    HitOut h;
+};
+
+PRR_DEV void travBegin(TravState &ts, const V3 &o, const V3 &d, float tmax, uint32_t root, LaneStack &st) {
+    ts.o = o;
+    ts.d = d;
+    ts.tmax = tmax;
+    ts.r = precompute(o, d);
+    ts.cur = root;
+    ts.h.t = __builtin_inff();
+    ts.h.u = ts.h.v = 0.f;
+    ts.h.prim = -1;
+    st.sp = 0;
+}
+
+// One step of BVHAccel::Intersect; returns true when the ray is finished.
+PRR_DEV bool closestStep(const SceneView &s, TravState &ts, LaneStack &st) {
+    const uint32_t cur = ts.cur;
+    if (refIsLeaf(cur)) {
+        const uint32_t first = refFirst(cur), cnt = refCount(cur);
+        if (refSphere(cur)) {
+            for (uint32_t i = 0; i < cnt; i++) {
+                float t;
+                if (sphHit(s.sph[first + i], ts.o, ts.d, t) && t < ts.h.t) {
+                    ts.h.t = t;
+                    ts.h.u = ts.h.v = 0.f;
+                    ts.h.prim = s.sph_id[first + i];
+                }
+            }
+        } else {
+            for (uint32_t i = 0; i < cnt; i++) {
+                const TriRec tr = loadTri(s.tris, first + i);
+                float t, u, v;
+                if (triHit(tr, ts.o, ts.d, t, u, v) && t < ts.h.t) {
+                    ts.h.t = t;
+                    ts.h.u = u;
+                    ts.h.v = v;
+                    ts.h.prim = tr.id;
+                }
+            }
+        }
+    } else {
+        const DNode n = s.nodes[cur];
+        const float tL = slab(ts.r, n.a.x, n.a.y, n.a.z, n.a.w, n.b.x, n.b.y);
+        const float tR = slab(ts.r, n.b.z, n.b.w, n.c.x, n.c.y, n.c.z, n.c.w);
+        const uint32_t L = (uint32_t) n.d.x, R = (uint32_t) n.d.y;
+        if (tL <= tR) {
+            if (tL < ts.h.t) {
+                if (tR < ts.h.t) st.push(R, tR);
+                ts.cur = L;
+                return false;
+            }
+        } else {
+            if (tR < ts.h.t) {
+                if (tL < ts.h.t) st.push(L, tL);
+                ts.cur = R;
+                return false;
+            }
+        }
+    }
+    while (st.sp > 0) {
+        const uint2 e = st.pop();
+        if (__uint_as_float(e.y) < ts.h.t) {
+            ts.cur = e.x;
+            return false;
+        }
+    }
+    return true;
+}
+
+// Root test of BVHAccel::IntersectP; false = the ray misses the scene box.
+PRR_DEV bool anyBegin(const SceneView &s, TravState &ts) {
+    const float tb = slab(ts.r, s.root_box[0], s.root_box[1], s.root_box[2], s.root_box[3], s.root_box[4],
+                          s.root_box[5]);
+    return tb < ts.tmax;
+}
+
+// One step of BVHAccel::IntersectP; returns true when finished, with
+// ts.h.prim = 1 if occluded, -1 otherwise.
+PRR_DEV bool anyStep(const SceneView &s, TravState &ts, LaneStack &st) {
+    const uint32_t cur = ts.cur;
+    if (refIsLeaf(cur)) {
+        const uint32_t first = refFirst(cur), cnt = refCount(cur);
+        if (refSphere(cur)) {
+            for (uint32_t i = 0; i < cnt; i++) {
+                float t;
+                if (sphHit(s.sph[first + i], ts.o, ts.d, t) && t < ts.tmax) {
+                    ts.h.prim = 1;
+                    return true;
+                }
+            }
+        } else {
+            for (uint32_t i = 0; i < cnt; i++)
+                if (triAny(loadTri(s.tris, first + i), ts.o, ts.d, ts.tmax)) {
+                    ts.h.prim = 1;
+                    return true;
+                }
+        }
+    } else {
+        const DNode n = s.nodes[cur];
+        const float tL = slab(ts.r, n.a.x, n.a.y, n.a.z, n.a.w, n.b.x, n.b.y);
+        const float tR = slab(ts.r, n.b.z, n.b.w, n.c.x, n.c.y, n.c.z, n.c.w);
+        const bool hl = tL < ts.tmax, hr = tR < ts.tmax;
+        const uint32_t L = (uint32_t) n.d.x, R = (uint32_t) n.d.y;
+        if (hl && hr) {
+            const bool lf = tL <= tR;
+            st.push(lf ? R : L, 0.f);
+            ts.cur = lf ? L : R;
+            return false;
+        }
+        if (hl || hr) {
+            ts.cur = hl ? L : R;
+            return false;
+        }
+    }
+    if (st.sp == 0) {
+        ts.h.prim = -1;
+        return true;
+    }
+    ts.cur = st.pop().x;
+    return false;
+}
+
+// Wave-batched work fetch: lanes with need == true get consecutive indices
+// from *counter (one atomic per wave); returns -1 if none was taken.
+PRR_DEV int64_t waveFetch(uint32_t *counter, bool need) {
+    const uint64_t mask = __ballot(need);
+    if (mask == 0) return -1;
+    const int lane = (int) __lane_id();
+    const int leader = __ffsll((unsigned long long) mask) - 1;
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(counter, (uint32_t) __popcll(mask));
+    base = __shfl(base, leader);
+    if (!need) return -1;
+    return (int64_t) base + __popcll(mask & ((1ull << lane) - 1ull));
 }
 
 }  // namespace prr

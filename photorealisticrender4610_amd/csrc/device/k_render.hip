@@ -69,45 +69,91 @@ __global__ __launch_bounds__(kShadeBlock) void kRaygen(SceneView s, PathBufs pb,
 }
 
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kTraceBlock) void kExtend(SceneView s, PathBufs pb, int bounce, uint2 *spill) {
-    __shared__ uint2 stk[kShort * kTraceBlock];
+// Persistent traversal: idle lanes refill from the queue with one atomic per
+// wave once `refillAt` lanes are idle (64: only when the whole wave is idle,
+// which keeps a wave on 64 consecutive, mostly coherent queue entries).
+
+__global__ __launch_bounds__(kTraceBlock) void kExtend(SceneView s, PathBufs pb, int bounce, uint2 *spill, int refillAt, int stackDepth) {
+    extern __shared__ uint2 stk[];
     const int q = bounce & 1;
     const int64_t n = pb.qcnt[bounce];
+    uint32_t *fetch = &pb.fetch[2 * bounce];
     const int64_t lane = (int64_t) blockIdx.x * kTraceBlock + threadIdx.x;
     LaneStack st;
     st.lds = stk;
     st.lane = threadIdx.x;
-    st.spill = spill + lane * (kSpill - kShort);
+    st.spill = spill + lane * kSpill;
+    st.depth = stackDepth;
     st.sp = 0;
-    const int64_t stride = (int64_t) gridDim.x * kTraceBlock;
-    for (int64_t i = lane; i < n; i += stride) {
-        const float4 o4 = pb.qOrg[q][i];
-        float4 h = make_float4(__builtin_inff(), 0.f, 0.f, __int_as_float(-1));
-        if (o4.w >= 0.f) {
-            const float4 d4 = pb.qDir[q][i];
-            const HitOut r = traceClosest(s, ld3(o4), ld3(d4), st);
-            h = make_float4(r.t, r.u, r.v, __int_as_float(r.prim));
+    TravState ts;
+    int64_t idx = -1;
+    bool exhausted = false;
+    while (true) {
+        const bool need = idx < 0 && !exhausted;
+        const uint64_t needMask = __ballot(need);
+        const bool refill = __popcll(needMask) >= refillAt || __ballot(idx >= 0) == 0;
+        const int64_t got = waveFetch(fetch, need && refill);
+        if (need && refill) {
+            if (got >= n) {
+                exhausted = true;
+            } else {
+                const float4 o4 = pb.qOrg[q][got];
+                if (o4.w < 0.f) {  // terminating entry: no ray
+                    pb.hit[got] = make_float4(__builtin_inff(), 0.f, 0.f, __int_as_float(-1));
+                } else {
+                    idx = got;
+                    travBegin(ts, ld3(o4), ld3(pb.qDir[q][got]), 0.f, s.root, st);
+                }
+            }
         }
-        pb.hit[i] = h;
+        if (__ballot(idx >= 0 || !exhausted) == 0) break;
+        if (idx >= 0 && closestStep(s, ts, st)) {
+            pb.hit[idx] = make_float4(ts.h.t, ts.h.u, ts.h.v, __int_as_float(ts.h.prim));
+            idx = -1;
+        }
     }
 }
 
-__global__ __launch_bounds__(kTraceBlock) void kShadow(SceneView s, PathBufs pb, int bounce, uint2 *spill) {
-    __shared__ uint2 stk[kShort * kTraceBlock];
+__global__ __launch_bounds__(kTraceBlock) void kShadow(SceneView s, PathBufs pb, int bounce, uint2 *spill, int refillAt, int stackDepth) {
+    extern __shared__ uint2 stk[];
     const int64_t n = pb.sqcnt[bounce];
+    uint32_t *fetch = &pb.fetch[2 * bounce + 1];
     const int64_t lane = (int64_t) blockIdx.x * kTraceBlock + threadIdx.x;
     LaneStack st;
     st.lds = stk;
     st.lane = threadIdx.x;
-    st.spill = spill + lane * (kSpill - kShort);
+    st.spill = spill + lane * kSpill;
+    st.depth = stackDepth;
     st.sp = 0;
-    const int64_t stride = (int64_t) gridDim.x * kTraceBlock;
-    for (int64_t i = lane; i < n; i += stride) {
-        const uint32_t sk = pb.sqSlot[i];
-        const float4 o4 = pb.sqOrg[i];
-        const float4 d4 = pb.sqDir[i];
-        const bool occ = traceAny(s, ld3(o4), ld3(d4), o4.w, st);
-        pb.vis[(int64_t) (sk >> 30) * pb.slots + (sk & 0x3FFFFFFFu)] = occ ? 0 : 1;
+    TravState ts;
+    int64_t idx = -1;
+    uint32_t sk = 0;
+    bool exhausted = false;
+    while (true) {
+        const bool need = idx < 0 && !exhausted;
+        const uint64_t needMask = __ballot(need);
+        const bool refill = __popcll(needMask) >= refillAt || __ballot(idx >= 0) == 0;
+        const int64_t got = waveFetch(fetch, need && refill);
+        if (need && refill) {
+            if (got >= n) {
+                exhausted = true;
+            } else {
+                const uint32_t k = pb.sqSlot[got];
+                const float4 o4 = pb.sqOrg[got];
+                travBegin(ts, ld3(o4), ld3(pb.sqDir[got]), o4.w, s.root, st);
+                if (anyBegin(s, ts)) {
+                    idx = got;
+                    sk = k;
+                } else {
+                    pb.vis[(int64_t) (k >> 30) * pb.slots + (k & 0x3FFFFFFFu)] = 1;
+                }
+            }
+        }
+        if (__ballot(idx >= 0 || !exhausted) == 0) break;
+        if (idx >= 0 && anyStep(s, ts, st)) {
+            pb.vis[(int64_t) (sk >> 30) * pb.slots + (sk & 0x3FFFFFFFu)] = ts.h.prim > 0 ? 0 : 1;
+            idx = -1;
+        }
     }
 }
 
@@ -471,13 +517,15 @@ hipError_t launchRaygen(const SceneView &s, const PathBufs &pb, const ChunkInfo 
     return hipGetLastError();
 }
 hipError_t launchExtend(const SceneView &s, const PathBufs &pb, int bounce, uint2 *spill, int traceGrid,
-                        hipStream_t st) {
-    hipLaunchKernelGGL(kExtend, dim3(traceGrid), dim3(kTraceBlock), 0, st, s, pb, bounce, spill);
+                        int refillAt, int stackDepth, hipStream_t st) {
+    hipLaunchKernelGGL(kExtend, dim3(traceGrid), dim3(kTraceBlock), stackDepth * kTraceBlock * sizeof(uint2), st, s,
+                       pb, bounce, spill, refillAt, stackDepth);
     return hipGetLastError();
 }
 hipError_t launchShadow(const SceneView &s, const PathBufs &pb, int bounce, uint2 *spill, int traceGrid,
-                        hipStream_t st) {
-    hipLaunchKernelGGL(kShadow, dim3(traceGrid), dim3(kTraceBlock), 0, st, s, pb, bounce, spill);
+                        int refillAt, int stackDepth, hipStream_t st) {
+    hipLaunchKernelGGL(kShadow, dim3(traceGrid), dim3(kTraceBlock), stackDepth * kTraceBlock * sizeof(uint2), st, s,
+                       pb, bounce, spill, refillAt, stackDepth);
     return hipGetLastError();
 }
 hipError_t launchShade(const SceneView &s, const PathBufs &pb, const ChunkInfo &ci, int bounce, int grid,

@@ -9,13 +9,15 @@ namespace prr {
 __global__ __launch_bounds__(kTraceBlock) void kTraceClosest(SceneView s, int64_t n, const float *__restrict__ org,
                                                              const float *__restrict__ dir, int32_t *__restrict__ prim,
                                                              float *__restrict__ tout, float *__restrict__ uout,
-                                                             float *__restrict__ vout, uint2 *__restrict__ spill) {
-    __shared__ uint2 stk[kShort * kTraceBlock];
+                                                             float *__restrict__ vout, uint2 *__restrict__ spill,
+                                                             int stackDepth) {
+    extern __shared__ uint2 stk[];
     const int64_t lane = (int64_t) blockIdx.x * kTraceBlock + threadIdx.x;
     LaneStack st;
     st.lds = stk;
     st.lane = threadIdx.x;
-    st.spill = spill + lane * (kSpill - kShort);
+    st.spill = spill + lane * kSpill;
+    st.depth = stackDepth;
     st.sp = 0;
     const int64_t stride = (int64_t) gridDim.x * kTraceBlock;
     for (int64_t i = lane; i < n; i += stride) {
@@ -31,13 +33,15 @@ __global__ __launch_bounds__(kTraceBlock) void kTraceClosest(SceneView s, int64_
 
 __global__ __launch_bounds__(kTraceBlock) void kTraceAny(SceneView s, int64_t n, const float *__restrict__ org,
                                                          const float *__restrict__ dir, const float *__restrict__ tmax,
-                                                         uint8_t *__restrict__ hit, uint2 *__restrict__ spill) {
-    __shared__ uint2 stk[kShort * kTraceBlock];
+                                                         uint8_t *__restrict__ hit, uint2 *__restrict__ spill,
+                                                         int stackDepth) {
+    extern __shared__ uint2 stk[];
     const int64_t lane = (int64_t) blockIdx.x * kTraceBlock + threadIdx.x;
     LaneStack st;
     st.lds = stk;
     st.lane = threadIdx.x;
-    st.spill = spill + lane * (kSpill - kShort);
+    st.spill = spill + lane * kSpill;
+    st.depth = stackDepth;
     st.sp = 0;
     const int64_t stride = (int64_t) gridDim.x * kTraceBlock;
     for (int64_t i = lane; i < n; i += stride) {
@@ -55,18 +59,19 @@ static int gridFor(int64_t n, int spillLanes) {
 }
 
 hipError_t launchTraceClosest(const SceneView &s, int64_t n, const float *org, const float *dir, int32_t *prim,
-                              float *t, float *u, float *v, uint2 *spill, int spillLanes, hipStream_t st) {
+                              float *t, float *u, float *v, uint2 *spill, int spillLanes, int stackDepth,
+                              hipStream_t st) {
     if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(kTraceClosest, dim3(gridFor(n, spillLanes)), dim3(kTraceBlock), 0, st, s, n, org, dir, prim,
-                       t, u, v, spill);
+    hipLaunchKernelGGL(kTraceClosest, dim3(gridFor(n, spillLanes)), dim3(kTraceBlock),
+                       stackDepth * kTraceBlock * sizeof(uint2), st, s, n, org, dir, prim, t, u, v, spill, stackDepth);
     return hipGetLastError();
 }
 
 hipError_t launchTraceAny(const SceneView &s, int64_t n, const float *org, const float *dir, const float *tmax,
-                          uint8_t *hit, uint2 *spill, int spillLanes, hipStream_t st) {
+                          uint8_t *hit, uint2 *spill, int spillLanes, int stackDepth, hipStream_t st) {
     if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(kTraceAny, dim3(gridFor(n, spillLanes)), dim3(kTraceBlock), 0, st, s, n, org, dir, tmax, hit,
-                       spill);
+    hipLaunchKernelGGL(kTraceAny, dim3(gridFor(n, spillLanes)), dim3(kTraceBlock),
+                       stackDepth * kTraceBlock * sizeof(uint2), st, s, n, org, dir, tmax, hit, spill, stackDepth);
     return hipGetLastError();
 }
 

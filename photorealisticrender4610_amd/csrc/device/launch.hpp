@@ -10,9 +10,10 @@ namespace prr {
 
 // k_trace.hip
 hipError_t launchTraceClosest(const SceneView &s, int64_t n, const float *org, const float *dir, int32_t *prim,
-                              float *t, float *u, float *v, uint2 *spill, int spillLanes, hipStream_t st);
+                              float *t, float *u, float *v, uint2 *spill, int spillLanes, int stackDepth,
+                              hipStream_t st);
 hipError_t launchTraceAny(const SceneView &s, int64_t n, const float *org, const float *dir, const float *tmax,
-                          uint8_t *hit, uint2 *spill, int spillLanes, hipStream_t st);
+                          uint8_t *hit, uint2 *spill, int spillLanes, int stackDepth, hipStream_t st);
 
 // k_misc.hip
 hipError_t launchBsdfEval(int64_t n, const float *wi, const float *wo, const float *nrm, const float *params,

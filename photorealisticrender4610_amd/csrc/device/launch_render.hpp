@@ -9,9 +9,9 @@ namespace prr {
 
 hipError_t launchRaygen(const SceneView &s, const PathBufs &pb, const ChunkInfo &ci, int grid, hipStream_t st);
 hipError_t launchExtend(const SceneView &s, const PathBufs &pb, int bounce, uint2 *spill, int traceGrid,
-                        hipStream_t st);
+                        int refillAt, int stackDepth, hipStream_t st);
 hipError_t launchShadow(const SceneView &s, const PathBufs &pb, int bounce, uint2 *spill, int traceGrid,
-                        hipStream_t st);
+                        int refillAt, int stackDepth, hipStream_t st);
 hipError_t launchShade(const SceneView &s, const PathBufs &pb, const ChunkInfo &ci, int bounce, int grid,
                        hipStream_t st);
 hipError_t launchAccumulate(const PathBufs &pb, const ChunkInfo &ci, float4 *fb, hipStream_t st);

@@ -35,6 +35,7 @@ struct PathBufs {
     float4 *sqDir;
     uint32_t *qcnt;          // ray-queue count per bounce
     uint32_t *sqcnt;         // shadow-queue count per bounce
+    uint32_t *fetch;         // persistent-kernel work counters: [2b] extend, [2b+1] shadow
     unsigned long long *stats;  // [0] closest rays, [1] any rays, [2] samples, [3] photons stored
 };
 
