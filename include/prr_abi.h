@@ -134,7 +134,9 @@ typedef struct prr_render_params {
     float indirect_clamp;      /* Integrator::indirectClamp (<=0 disables) */
     int32_t sample_offset;     /* index of the first sample (progressive / split renders) */
     uint64_t seed;             /* per-pixel PCG32 seed */
-    /* caustic photon map (Photon.hpp); 0 photons = off */
+    /* > 0: gather the scene's resident caustic photon map (prr_photon_build)
+     * at diffuse vertices reached from the camera or through delta bounces;
+     * radius / depth are those of the build (fields kept for reporting) */
     int64_t num_photons;
     float photon_radius;
     int32_t photon_max_depth;
@@ -233,6 +235,23 @@ int prr_camera_rays(prr_context* ctx, const prr_scene* scene, int32_t width, int
  * and 8-bit quantisation (Renderer.cpp:169-171). display or rgb8 may be NULL. */
 int prr_tonemap(prr_context* ctx, int32_t op, int64_t n, float exposure, const float* linear,
                 float* display, uint8_t* rgb8);
+
+/* ---- caustic photon map (Scene::buildPhotonMaps, Scene.cpp:8-13) -------- */
+/* Traces num photons from the scene's point light (Photon.cpp:96-188 with the
+ * build's changes: point-light emission, glass handled with the outward normal
+ * as in castRay's delta branch, storage at the first diffuse hit after >= 1
+ * delta bounce) and keeps the hash grid (cell = 2 * radius) resident in the
+ * scene. Blocks. stored may be NULL. */
+int prr_photon_build(prr_context* ctx, prr_scene* scene, int64_t num_photons, float radius,
+                     int32_t max_depth, uint64_t seed, int64_t* stored);
+/* Copies up to cap stored photons (pos3, dir3, power3 = 9 floats each, bucket
+ * order) to device memory out9; *count = photons stored. */
+int prr_photon_export(prr_context* ctx, const prr_scene* scene, int64_t cap, float* out9,
+                      int64_t* count);
+/* BSDF-weighted photon density estimate at n points (the term the renderer
+ * adds at camera- or delta-reached diffuse vertices); params as prr_bsdf_eval. */
+int prr_photon_radiance(prr_context* ctx, const prr_scene* scene, int64_t n, const float* pos,
+                        const float* wo, const float* normal, const float* params, float* out);
 
 /* ---- the render: Renderer::Render sample loop + Integrator::castRay ----- */
 /* Writes the per-pixel mean linear radiance (width*height*3 floats, device)

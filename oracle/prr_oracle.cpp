@@ -496,6 +496,15 @@ extern "C" int64_t orc_photon_export(const orc_photon_map *pm, int64_t cap, floa
     }
     return (int64_t) pm->ph.size();
 }
+extern "C" void orc_photon_radiance(const orc_photon_map *pm, int64_t n, const float *pos, const float *wo,
+                                    const float *nrm, const float *params, float *o) {
+    for (int64_t i = 0; i < n; i++) {
+        Material m = paramMat(params + 8 * i);
+        Shading sd = makeShading(m, v3(nrm + 3 * i));
+        put3(o + 3 * i, pm->radiance(v3(pos + 3 * i), v3(wo + 3 * i), sd));
+    }
+}
+
 extern "C" void orc_photon_irradiance(const orc_photon_map *pm, int64_t n, const float *pos, float *o) {
     for (int64_t i = 0; i < n; i++) put3(o + 3 * i, pm->irradiance(v3(pos + 3 * i)));
 }

@@ -107,6 +107,10 @@ int64_t orc_photon_count(const orc_photon_map* pm);
 int64_t orc_photon_export(const orc_photon_map* pm, int64_t cap, float* out9);
 /* irradiance estimate sum(power)/(pi r^2) over photons within r (Photon.cpp:82-94) */
 void orc_photon_irradiance(const orc_photon_map* pm, int64_t n, const float* pos, float* out3);
+/* BSDF-weighted estimate sum f(wi_p, wo) Phi_p / (pi r^2), f = eval / cos
+ * (the term the renderer adds at camera/delta-reached diffuse vertices) */
+void orc_photon_radiance(const orc_photon_map* pm, int64_t n, const float* pos, const float* wo,
+                         const float* nrm, const float* params, float* out3);
 
 int orc_version(void);
 

@@ -150,6 +150,7 @@ def lib(fast=False):
     L.orc_photon_export.restype = C.c_int64
     L.orc_photon_export.argtypes = [vp, C.c_int64, vp]
     L.orc_photon_irradiance.argtypes = [vp, C.c_int64, vp, vp]
+    L.orc_photon_radiance.argtypes = [vp, C.c_int64, vp, vp, vp, vp, vp]
     _LIBS[name] = L
     return L
 
@@ -276,6 +277,12 @@ class OraclePhotonMap:
         n = self.count()
         out = np.zeros((n, 9), np.float32)
         self.L.orc_photon_export(self.h, n, _p(out))
+        return out
+
+    def radiance(self, pos, wo, nrm, params):
+        pos, wo, nrm, params = _c(pos), _c(wo), _c(nrm), _c(params)
+        out = np.zeros_like(pos)
+        self.L.orc_photon_radiance(self.h, pos.shape[0], _p(pos), _p(wo), _p(nrm), _p(params), _p(out))
         return out
 
     def irradiance(self, pos):

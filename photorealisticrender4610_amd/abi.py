@@ -98,6 +98,9 @@ SIGNATURES = {
     "prr_camera_rays": [vp, vp, i32, i32, i64, vp, vp, vp, vp],
     "prr_tonemap": [vp, i32, i64, C.c_float, vp, vp, vp],
     "prr_render": [vp, vp, C.POINTER(RenderParams), vp, C.POINTER(Stats)],
+    "prr_photon_build": [vp, vp, i64, C.c_float, i32, C.c_uint64, C.POINTER(i64)],
+    "prr_photon_export": [vp, vp, i64, vp, C.POINTER(i64)],
+    "prr_photon_radiance": [vp, vp, i64, vp, vp, vp, vp, vp],
 }
 _RESTYPES = {"prr_last_error": C.c_char_p}
 
@@ -307,15 +310,43 @@ class Context:
         self._done(lib().prr_tonemap(self.h, opi, n, exposure, _ptr(lin), _ptr(disp), _ptr(rgb8)))
         return disp, rgb8
 
+    def photon_build(self, scene, num, radius, max_depth=30, seed=0xC2):
+        """Scene::buildPhotonMaps: trace num photons, keep the grid resident."""
+        stored = i64()
+        self._bind_stream()
+        self._done(lib().prr_photon_build(self.h, scene.h, int(num), float(radius), int(max_depth), int(seed),
+                                          C.byref(stored)))
+        scene.photons = int(num)
+        return stored.value
+
+    def photon_export(self, scene):
+        T = self.torch
+        cnt = i64()
+        self._check(lib().prr_photon_export(self.h, scene.h, 0, None, C.byref(cnt)), self.h)
+        out = T.empty((max(cnt.value, 1), 9), dtype=T.float32, device=self.device)
+        self._bind_stream()
+        self._done(lib().prr_photon_export(self.h, scene.h, cnt.value, _ptr(out), C.byref(cnt)))
+        return out[:cnt.value]
+
+    def photon_radiance(self, scene, pos, wo, nrm, params):
+        T = self.torch
+        pos, wo, nrm, params = (self._f32(a) for a in (pos, wo, nrm, params))
+        out = T.empty((pos.shape[0], 3), dtype=T.float32, device=self.device)
+        self._bind_stream()
+        self._done(lib().prr_photon_radiance(self.h, scene.h, pos.shape[0], _ptr(pos), _ptr(wo), _ptr(nrm),
+                                             _ptr(params), _ptr(out)))
+        return out
+
     def render(self, scene, width, height, spp, max_depth, seed=0, clamp=10.0, sample_offset=0, out=None,
-               kernel_timing=False):
+               kernel_timing=False, photons=False):
         """Renderer::Render at fixed spp: returns (H, W, 3) mean linear radiance
         (device tensor) and a stats dict. Blocks until done."""
         T = self.torch
         if out is None:
             out = T.empty((height, width, 3), dtype=T.float32, device=self.device)
         p = RenderParams(width=width, height=height, spp=spp, max_depth=max_depth, indirect_clamp=clamp,
-                         sample_offset=sample_offset, seed=seed, flags=1 if kernel_timing else 0)
+                         sample_offset=sample_offset, seed=seed, flags=1 if kernel_timing else 0,
+                         num_photons=getattr(scene, "photons", 0) if photons else 0)
         st = Stats()
         self._bind_stream()
         self._done(lib().prr_render(self.h, scene.h, C.byref(p), _ptr(out), C.byref(st)))

@@ -10,6 +10,7 @@
 #include "../../include/prr_abi.h"
 #include "device/dev_traverse.hpp"
 #include "device/launch.hpp"
+#include "device/launch_photon.hpp"
 #include "device/launch_render.hpp"
 #include "host/scene_build.hpp"
 
@@ -40,13 +41,30 @@ struct prr_context {
     std::vector<hipEvent_t> evPool;  // per-launch timing events (PRR_RENDER_KERNEL_TIMING)
 };
 
+struct PhotonMap {
+    std::vector<void *> allocs;
+    int64_t emitted = 0, stored = 0;
+    float radius = 0.f;
+    int maxDepth = 0;
+    uint64_t seed = 0;
+};
+
 struct prr_scene {
     host::HostScene hs;
     SceneView view{};
     std::vector<void *> allocs;
     int64_t bytes = 0;
     AgxConst agx{};
+    PhotonMap ph;
 };
+
+static void freePhotons(prr_scene *s) {
+    for (void *p : s->ph.allocs) (void) hipFree(p);
+    s->ph = PhotonMap();
+    s->view.ph_count = 0;
+    s->view.ph_pos = s->view.ph_dir = s->view.ph_pow = nullptr;
+    s->view.ph_start = s->view.ph_end = nullptr;
+}
 
 static int fail(prr_context *c, int code, const std::string &msg) {
     if (c) c->err = msg;
@@ -243,6 +261,7 @@ int prr_scene_free(prr_context *c, prr_scene *s) {
         (void) hipSetDevice(c->device);
         (void) hipStreamSynchronize(c->stream);
     }
+    freePhotons(s);
     for (void *p : s->allocs) (void) hipFree(p);
     delete s;
     return PRR_OK;
@@ -359,6 +378,130 @@ int prr_tonemap(prr_context *c, int32_t op, int64_t n, float exposure, const flo
 }
 
 // ---------------------------------------------------------------------------
+// caustic photon map (Scene::buildPhotonMaps, Scene.cpp:8-13; Photon.cpp)
+// ---------------------------------------------------------------------------
+int prr_photon_build(prr_context *c, prr_scene *s, int64_t num, float radius, int32_t maxDepth, uint64_t seed,
+                     int64_t *stored) {
+    if (!c || !s) return fail(c, PRR_ERR_INVALID, "NULL context or scene");
+    if (num <= 0 || num > ((int64_t) 1 << 31) || !(radius > 0.f) || maxDepth < 0 || maxDepth > 64)
+        return fail(c, PRR_ERR_INVALID, "bad photon parameters");
+    if (s->view.num_points < 1) return fail(c, PRR_ERR_UNSUPPORTED, "photon emission needs a point light");
+    HIPCK(c, hipSetDevice(c->device));
+    hipStream_t st = c->stream;
+    HIPCK(c, hipStreamSynchronize(st));
+    freePhotons(s);
+    PhotonMap &pm = s->ph;
+    auto A = [&](size_t bytes) -> void * {
+        void *p = nullptr;
+        if (hipMalloc(&p, std::max<size_t>(bytes, 16)) != hipSuccess) return nullptr;
+        pm.allocs.push_back(p);
+        return p;
+    };
+    // per-photon records (scratch, freed after the build)
+    std::vector<void *> tmp;
+    auto T = [&](size_t bytes) -> void * {
+        void *p = nullptr;
+        if (hipMalloc(&p, std::max<size_t>(bytes, 16)) != hipSuccess) return nullptr;
+        tmp.push_back(p);
+        return p;
+    };
+    auto freeTmp = [&]() {
+        for (void *p : tmp) (void) hipFree(p);
+    };
+    float4 *pos = (float4 *) T(num * sizeof(float4)), *dir = (float4 *) T(num * sizeof(float4));
+    float4 *pw = (float4 *) T(num * sizeof(float4));
+    uint32_t *cnt = (uint32_t *) T(sizeof(uint32_t));
+    if (!pos || !dir || !pw || !cnt) {
+        freeTmp();
+        return fail(c, PRR_ERR_NOMEM, "photon buffers");
+    }
+    HIPCK(c, hipMemsetAsync(cnt, 0, sizeof(uint32_t), st));
+    SceneView v = s->view;
+    v.ph_count = 0;
+    hipError_t e = launchPhotonTrace(v, num, seed, maxDepth, pos, dir, pw, cnt, c->spill, c->spillLanes,
+                                     c->stackDepth, st);
+    uint32_t m = 0;
+    if (e == hipSuccess) e = hipMemcpyAsync(&m, cnt, sizeof(uint32_t), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) {
+        freeTmp();
+        return fail(c, PRR_ERR_DEVICE, std::string("photon trace: ") + hipGetErrorString(e));
+    }
+    uint32_t buckets = 16;
+    while (buckets < 2u * std::max<uint32_t>(m, 1u) && buckets < (1u << 30)) buckets <<= 1;
+    const float cell = 2.f * radius;
+    uint32_t *kin = (uint32_t *) T(num * 4), *vin = (uint32_t *) T(num * 4);
+    uint32_t *kout = (uint32_t *) T(num * 4), *vout = (uint32_t *) T(num * 4);
+    float4 *spos = (float4 *) A((size_t) m * sizeof(float4)), *sdir = (float4 *) A((size_t) m * sizeof(float4));
+    float4 *spw = (float4 *) A((size_t) m * sizeof(float4));
+    uint32_t *bs = (uint32_t *) A((size_t) buckets * 4), *be = (uint32_t *) A((size_t) buckets * 4);
+    if (!kin || !vin || !kout || !vout || !spos || !sdir || !spw || !bs || !be) {
+        freeTmp();
+        freePhotons(s);
+        return fail(c, PRR_ERR_NOMEM, "photon grid buffers");
+    }
+    e = launchPhotonKeys(num, pos, cell, buckets - 1u, kin, vin, st);
+    size_t tb = 0;
+    if (e == hipSuccess) e = photonSort(nullptr, tb, kin, kout, vin, vout, (int) num, st);
+    void *temp = (e == hipSuccess) ? T(tb) : nullptr;
+    if (e == hipSuccess && !temp) e = hipErrorOutOfMemory;
+    if (e == hipSuccess) e = photonSort(temp, tb, kin, kout, vin, vout, (int) num, st);
+    if (e == hipSuccess) e = hipMemsetAsync(bs, 0, (size_t) buckets * 4, st);
+    if (e == hipSuccess) e = hipMemsetAsync(be, 0, (size_t) buckets * 4, st);
+    if (e == hipSuccess) e = launchPhotonFinish(m, kout, vout, pos, dir, pw, spos, sdir, spw, bs, be, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    freeTmp();
+    if (e != hipSuccess) {
+        freePhotons(s);
+        return fail(c, PRR_ERR_DEVICE, std::string("photon grid: ") + hipGetErrorString(e));
+    }
+    pm.emitted = num;
+    pm.stored = m;
+    pm.radius = radius;
+    pm.maxDepth = maxDepth;
+    pm.seed = seed;
+    s->view.ph_pos = spos;
+    s->view.ph_dir = sdir;
+    s->view.ph_pow = spw;
+    s->view.ph_start = bs;
+    s->view.ph_end = be;
+    s->view.ph_count = m;
+    s->view.ph_mask = buckets - 1u;
+    s->view.ph_radius = radius;
+    s->view.ph_cell = cell;
+    if (stored) *stored = m;
+    return PRR_OK;
+}
+
+int prr_photon_export(prr_context *c, const prr_scene *s, int64_t cap, float *out9, int64_t *count) {
+    if (!c || !s || !count) return fail(c, PRR_ERR_INVALID, "NULL argument");
+    *count = s->ph.stored;
+    const int64_t k = std::min<int64_t>(cap, s->ph.stored);
+    if (k <= 0) return PRR_OK;
+    if (!out9) return fail(c, PRR_ERR_INVALID, "NULL output");
+    HIPCK(c, hipSetDevice(c->device));
+    // pos, dir, power as 9 packed floats per photon (bucket order)
+    const size_t pitch = 9 * sizeof(float);
+    HIPCK(c, hipMemcpy2DAsync(out9, pitch, s->view.ph_pos, sizeof(float4), 3 * sizeof(float), k,
+                              hipMemcpyDeviceToDevice, c->stream));
+    HIPCK(c, hipMemcpy2DAsync(out9 + 3, pitch, s->view.ph_dir, sizeof(float4), 3 * sizeof(float), k,
+                              hipMemcpyDeviceToDevice, c->stream));
+    HIPCK(c, hipMemcpy2DAsync(out9 + 6, pitch, s->view.ph_pow, sizeof(float4), 3 * sizeof(float), k,
+                              hipMemcpyDeviceToDevice, c->stream));
+    return PRR_OK;
+}
+
+int prr_photon_radiance(prr_context *c, const prr_scene *s, int64_t n, const float *pos, const float *wo,
+                        const float *normal, const float *params, float *out) {
+    if (!c || !s) return fail(c, PRR_ERR_INVALID, "NULL context or scene");
+    if (s->ph.stored <= 0) return fail(c, PRR_ERR_UNSUPPORTED, "no photon map (prr_photon_build)");
+    if (n < 0 || (n > 0 && (!pos || !wo || !normal || !params || !out))) return fail(c, PRR_ERR_INVALID, "bad arrays");
+    HIPCK(c, hipSetDevice(c->device));
+    HIPCK(c, launchPhotonRadiance(s->view, n, pos, wo, normal, params, out, c->stream));
+    return PRR_OK;
+}
+
+// ---------------------------------------------------------------------------
 // render
 // ---------------------------------------------------------------------------
 // Paths in flight per chunk. Path state is ~340 B per slot, so 2^24 slots
@@ -388,7 +531,7 @@ static int ensureRenderBufs(prr_context *c, int64_t slots, int maxDepth) {
     A(&p.beta, slots);
     A(&p.bscale, slots);
     A(&p.prevP, slots);
-    A(&p.pend, 3 * slots);
+    A(&p.pend, 4 * slots);
     A(&p.rng, slots);
     A(&p.flags, slots);
     A(&p.vis, 3 * slots);
@@ -422,7 +565,8 @@ int prr_render(prr_context *c, const prr_scene *s, const prr_render_params *p, f
         return fail(c, PRR_ERR_INVALID, "bad render parameters");
     const int64_t pixels = (int64_t) p->width * p->height;
     if (pixels > kMaxSlots) return fail(c, PRR_ERR_UNSUPPORTED, "image larger than 16M pixels");
-    if (p->num_photons > 0) return fail(c, PRR_ERR_UNSUPPORTED, "photon map requested (use the photon entry points)");
+    if (p->num_photons > 0 && s->ph.stored <= 0 && s->ph.emitted <= 0)
+        return fail(c, PRR_ERR_INVALID, "num_photons > 0 but the scene has no photon map (prr_photon_build)");
     HIPCK(c, hipSetDevice(c->device));
     const int spc = (int) std::max<int64_t>(1, std::min<int64_t>(p->spp, kMaxSlots / pixels));
     int rc = ensureRenderBufs(c, pixels * spc, p->max_depth);
@@ -439,6 +583,8 @@ int prr_render(prr_context *c, const prr_scene *s, const prr_render_params *p, f
     HIPCK(c, hipMemsetAsync(c->fb, 0, (size_t) pixels * sizeof(float4), st));
     HIPCK(c, hipEventRecord(c->ev0, st));
     const int traceGrid = c->traceBlocks;
+    SceneView view = s->view;
+    if (p->num_photons <= 0) view.ph_count = 0;  // photon gather only on request
     const bool timing = (p->flags & PRR_RENDER_KERNEL_TIMING) != 0;
     // per-launch event pairs: kind 0 extend, 1 shadow, 2 shade
     std::vector<std::pair<int, int>> timed;  // (kind, first event index)
@@ -476,11 +622,11 @@ int prr_render(prr_context *c, const prr_scene *s, const prr_render_params *p, f
         HIPCK(c, hipMemsetAsync(c->pb.qcnt, 0, ((size_t) p->max_depth + 2) * sizeof(uint32_t), st));
         HIPCK(c, hipMemsetAsync(c->pb.sqcnt, 0, ((size_t) p->max_depth + 2) * sizeof(uint32_t), st));
         HIPCK(c, hipMemsetAsync(c->pb.fetch, 0, 2 * ((size_t) p->max_depth + 2) * sizeof(uint32_t), st));
-        HIPCK(c, launchRaygen(s->view, c->pb, ci, c->shadeBlocks, st));
+        HIPCK(c, launchRaygen(view, c->pb, ci, c->shadeBlocks, st));
         for (int b = 0; b <= p->max_depth; b++) {
-            TIMED(0, launchExtend(s->view, c->pb, b, c->spill, traceGrid, c->refillAt, c->stackDepth, st));
-            if (b > 0) TIMED(1, launchShadow(s->view, c->pb, b - 1, c->spill, traceGrid, c->refillAt, c->stackDepth, st));
-            TIMED(2, launchShade(s->view, c->pb, ci, b, c->shadeBlocks, st));
+            TIMED(0, launchExtend(view, c->pb, b, c->spill, traceGrid, c->refillAt, c->stackDepth, st));
+            if (b > 0) TIMED(1, launchShadow(view, c->pb, b - 1, c->spill, traceGrid, c->refillAt, c->stackDepth, st));
+            TIMED(2, launchShade(view, c->pb, ci, b, c->shadeBlocks, st));
         }
         HIPCK(c, launchAccumulate(c->pb, ci, c->fb, st));
     }
@@ -496,7 +642,7 @@ int prr_render(prr_context *c, const prr_scene *s, const prr_render_params *p, f
         stats->rays_closest = h[0];
         stats->rays_any = h[1];
         stats->samples = h[2];
-        stats->photons_stored = h[3];
+        stats->photons_stored = p->num_photons > 0 ? (uint64_t) s->ph.stored : 0;
         stats->ms_total = ms;
         stats->ms_extend = stats->ms_shadow = stats->ms_shade = 0.0;
         stats->launches_extend = stats->launches_shadow = 0;

@@ -172,6 +172,16 @@ struct SceneView {
     int num_points;
     float point_pos[3];
     float point_int[3];
+    // caustic photon map (build extension; count 0 = none): photons sorted by
+    // hash bucket of their cell (cell edge = 2 * radius)
+    const float4 *ph_pos;       // position, w unused
+    const float4 *ph_dir;       // direction of travel
+    const float4 *ph_pow;       // power (flux) rgb
+    const uint32_t *ph_start;   // per bucket: first photon
+    const uint32_t *ph_end;     // per bucket: one past the last photon
+    uint32_t ph_count;
+    uint32_t ph_mask;           // buckets - 1 (power of two)
+    float ph_radius, ph_cell;
     // camera (Camera.hpp:43-56), focal length precomputed on the host
     float cam_eye[3], cam_fwd[3], cam_right[3], cam_up[3];
     float cam_aperture, cam_focus, cam_tanhalf;
@@ -586,6 +596,50 @@ PRR_DEV V3 exposeLinear(const V3 &fb, float exposure) {  // Renderer.cpp:149-161
     if (!isfinite(c.y)) c.y = 0.f;
     if (!isfinite(c.z)) c.z = 0.f;
     return V3(smax(0.f, c.x), smax(0.f, c.y), smax(0.f, c.z));
+}
+
+// ---------------------------------------------------------------------------
+// Caustic photon map gather (build extension of Photon.cpp:82-94): the
+// photon density estimate weighted by the BSDF,
+//   L = sum_p f(wi_p, wo) * Phi_p / (pi r^2),  f = eval / cos(wi_p, N),
+// over photons within r. The cells overlapping [x - r, x + r] (at most 2 per
+// axis, cell = 2r) are hashed to buckets; duplicate buckets are visited once,
+// so every photon within r is counted exactly once even under collisions.
+// ---------------------------------------------------------------------------
+PRR_HD uint32_t photonHash(int cx, int cy, int cz, uint32_t mask) {
+    return (((uint32_t) cx * 73856093u) ^ ((uint32_t) cy * 19349663u) ^ ((uint32_t) cz * 83492791u)) & mask;
+}
+PRR_DEV V3 photonRadiance(const SceneView &s, const V3 &x, const V3 &wo, const Frame &fr, const Lobes &lb) {
+    V3 sum(0.f);
+    if (s.ph_count == 0) return sum;
+    const float r = s.ph_radius, cs = s.ph_cell;
+    const int x0 = (int) floorf((x.x - r) / cs), x1 = (int) floorf((x.x + r) / cs);
+    const int y0 = (int) floorf((x.y - r) / cs), y1 = (int) floorf((x.y + r) / cs);
+    const int z0 = (int) floorf((x.z - r) / cs), z1 = (int) floorf((x.z + r) / cs);
+    uint32_t seen[8];
+    int ns = 0;
+    for (int cz = z0; cz <= z1 && cz <= z0 + 1; cz++)
+        for (int cy = y0; cy <= y1 && cy <= y0 + 1; cy++)
+            for (int cx = x0; cx <= x1 && cx <= x0 + 1; cx++) {
+                const uint32_t h = photonHash(cx, cy, cz, s.ph_mask);
+                bool dup = false;
+                for (int k = 0; k < ns; k++) dup |= seen[k] == h;
+                if (dup) continue;
+                seen[ns++] = h;
+                const uint32_t b = s.ph_start[h], e = s.ph_end[h];
+                for (uint32_t i = b; i < e; i++) {
+                    const V3 p = ld3(s.ph_pos[i]);
+                    if (norm(p - x) > r) continue;
+                    const V3 wi = -ld3(s.ph_dir[i]);
+                    const float c = dot(wi, fr.N);
+                    if (c <= 1e-4f) continue;
+                    V3 f;
+                    float pdf;
+                    evalPdf(lb, fr, wi, wo, f, pdf);
+                    sum += f * ld3(s.ph_pow[i]) / c;
+                }
+            }
+    return sum / (kPi * r * r);
 }
 
 }  // namespace prr

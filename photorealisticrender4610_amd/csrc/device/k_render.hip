@@ -257,6 +257,7 @@ __global__ __launch_bounds__(kShadeBlock) void kShade(SceneView s, PathBufs pb, 
                 if (pb.vis[pb.slots + slot]) L += ld3(pb.pend[pb.slots + slot]);
             if (fl & F_PEND_P)
                 if (pb.vis[2 * pb.slots + slot]) L += ld3(pb.pend[2 * pb.slots + slot]);
+            if (fl & F_PEND_C) L += ld3(pb.pend[3 * pb.slots + slot]);
             fl &= ~F_PEND_ALL;
             done = (fl & F_TERMINATING) != 0u;
             const float4 h = pb.hit[i];
@@ -419,6 +420,14 @@ __global__ __launch_bounds__(kShadeBlock) void kShade(SceneView s, PathBufs pb, 
                             sOP = g.P + wi * kEps;
                             sDP = wi;
                             tP = dist - 2.f * kEps;
+                        }
+                        // caustic photon estimate at camera / delta-reached vertices
+                        // (build extension; no draws)
+                        if (s.ph_count > 0 && (bounce == 0 || spec)) {
+                            V3 c = beta * photonRadiance(s, g.P, wo, fr, lb);
+                            c = ind ? clampInd(c, ci.clampInd) : c;
+                            pb.pend[3 * pb.slots + slot] = st4(c, 0.f);
+                            fl |= F_PEND_C;
                         }
                         // BSDF sampling (Integrator.cpp:207-215)
                         const BsdfSample bs = sampleDiffuse(lb, fr, wo, rng);

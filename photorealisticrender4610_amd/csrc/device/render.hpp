@@ -13,7 +13,8 @@ constexpr uint32_t F_PEND_A = 4u;        // area-light NEE waiting for its shado
 constexpr uint32_t F_PEND_E = 8u;        // environment NEE waiting
 constexpr uint32_t F_PEND_P = 16u;       // point-light NEE waiting
 constexpr uint32_t F_TERMINATING = 32u;  // path ended; only pending NEE remains
-constexpr uint32_t F_PEND_ALL = F_PEND_A | F_PEND_E | F_PEND_P;
+constexpr uint32_t F_PEND_C = 64u;       // caustic photon estimate (no shadow ray)
+constexpr uint32_t F_PEND_ALL = F_PEND_A | F_PEND_E | F_PEND_P | F_PEND_C;
 
 struct PathBufs {
     int64_t slots;           // capacity (paths per chunk)
@@ -21,7 +22,7 @@ struct PathBufs {
     float4 *beta;            // throughput (xyz)
     float4 *bscale;          // pending BSDF factor f/pdf (xyz), w = BSDF pdf
     float4 *prevP;           // diffuse vertex that shot the BSDF ray (xyz)
-    float4 *pend;            // 3 * slots: area, env, point NEE contributions
+    float4 *pend;            // 4 * slots: area, env, point NEE and caustic-photon contributions
     uint64_t *rng;           // PCG32 state
     uint32_t *flags;
     uint8_t *vis;            // 3 * slots: visibility of the pending NEE rays
