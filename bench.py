@@ -30,6 +30,9 @@ WORKLOADS = {
     "sphere_field": ("sphere_field", 1920, 1080, 64, 12,
                      "sphere_field: 4096 UV spheres x 256 tris + ground (1,048,578 tris), 2048x1024 sky with "
                      "2-degree sun (peak 1e4), GGX materials, 1920x1080, 64 spp, depth 12, ACES"),
+    "glass_torus": ("glass_torus", 1024, 1024, 32, 12,
+                    "glass_torus: 65,536-tri glass torus (ior 1.5) on a Lambert plane, point light 1000 W; "
+                    "10M caustic photons (radius 0.01, hash grid) + path tracing, 1024x1024, 32 spp, depth 12"),
     "box_room": ("box_room", 256, 256, 16, 8,
                  "box_room: 10-tri box, GGX conductor + glass spheres, quad light (17,12,4), 256x256, 16 spp, "
                  "depth 8, NEE+MIS"),
@@ -107,8 +110,16 @@ def main():
     scene = ctx.scene(sc)
     log("host SAH build + upload in %.1f s: %s" % (time.time() - t0, scene.info()))
     out = torch.empty((H, W, 3), dtype=torch.float32, device="cuda")
+    photons = args.workload == "glass_torus"
+
+    def photon_pass(k):
+        if photons:
+            return ctx.photon_build(scene, 10_000_000, 0.01, 30, seed + 1000 + k)
+        return 0
+
     for k in range(args.warmup):
-        _, st = ctx.render(scene, W, H, SPP, DEPTH, seed=seed + k, out=out)
+        photon_pass(k)
+        _, st = ctx.render(scene, W, H, SPP, DEPTH, seed=seed + k, out=out, photons=photons)
         ctx.tonemap("aces", out.view(-1, 3), 1.0)
         log("warmup %d: %.1f ms, %d rays" % (k, st["ms"], st["rays_closest"] + st["rays_any"]))
     torch.cuda.synchronize()
@@ -116,8 +127,11 @@ def main():
     ms_ext = 0.0
     launches = 0
     t0 = time.perf_counter()
+    stored = 0
     for k in range(args.steps):
-        _, st = ctx.render(scene, W, H, SPP, DEPTH, seed=seed + 100 + k, out=out, kernel_timing=True)
+        stored = photon_pass(100 + k)
+        _, st = ctx.render(scene, W, H, SPP, DEPTH, seed=seed + 100 + k, out=out, kernel_timing=True,
+                           photons=photons)
         ctx.tonemap("aces", out.view(-1, 3), 1.0)
         rays += st["rays_closest"] + st["rays_any"]
         closest += st["rays_closest"]
@@ -151,6 +165,7 @@ def main():
                    "triangles": sc.num_triangles},
         "msamples_per_s": round(W * H * SPP * args.steps / dt / 1e6, 2),
         "rays_per_step": rays // args.steps,
+        "photons_stored": stored if photons else None,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

@@ -259,6 +259,14 @@ int prr_photon_radiance(prr_context* ctx, const prr_scene* scene, int64_t n, con
 int prr_render(prr_context* ctx, const prr_scene* scene, const prr_render_params* params,
                float* rgb_out, prr_stats* stats);
 
+/* Integrator::castRay (Integrator.cpp:104-263) for n caller rays (device
+ * arrays): ray k runs the full path with the PCG32 stream of (seed, k,
+ * sample 0) and writes its radiance (non-finite -> 0) to L[3k..3k+2].
+ * Blocks until done; stats may be NULL. */
+int prr_cast_rays(prr_context* ctx, const prr_scene* scene, int64_t n, const float* org,
+                  const float* dir, uint64_t seed, int32_t max_depth, float indirect_clamp, float* L,
+                  prr_stats* stats);
+
 #ifdef __cplusplus
 }
 #endif

@@ -358,6 +358,20 @@ extern "C" void orc_castray_splitmix(const orc_scene *s, int width, int height, 
     }
 }
 
+// castRay on caller rays with the GPU's PCG32 stream (seed, k, sample 0)
+extern "C" void orc_castray_pcg(const orc_scene *s, int64_t n, const float *org, const float *dir, uint64_t seed,
+                                int max_depth, float indirect_clamp, float *L) {
+    IntegratorParams ip;
+    ip.maxDepth = max_depth;
+    ip.indirectClamp = indirect_clamp;
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int64_t k = 0; k < n; k++) {
+        Pcg32 rng = Pcg32::make(seed, (uint32_t) k, 0u);
+        V3 c = castRay(s->sc, Ray{v3(org + 3 * k), v3(dir + 3 * k)}, rng, ip, nullptr);
+        put3(L + 3 * k, c);
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Caustic photon map (Photon.cpp:96-188 restated with the build's documented
 // changes: point-light emission, glass handled with the outward normal as in

@@ -85,6 +85,10 @@ void orc_castray_splitmix(const orc_scene* s, int width, int height, int64_t n,
                           const int32_t* pix, const uint64_t* seeds, int max_depth,
                           float indirect_clamp, float* L);
 
+/* castRay on caller rays with the PCG32 stream of (seed, k, sample 0) */
+void orc_castray_pcg(const orc_scene* s, int64_t n, const float* org, const float* dir, uint64_t seed,
+                     int max_depth, float indirect_clamp, float* L);
+
 /* Fixed-spp render with PCG32 per (pixel, sample) (the GPU renderer's RNG).
  * mean: W*H*3 (sum in sample order / spp), var: W*H*3 sample variance of the
  * per-sample radiance (may be NULL). rows [row0,row1) only (row1<=0: all).

@@ -141,6 +141,7 @@ def lib(fast=False):
     L.orc_camera_rays.argtypes = [C.POINTER(CameraDesc), C.c_int, C.c_int, C.c_int64, vp, vp, vp, vp]
     L.orc_tonemap.argtypes = [C.c_int, C.c_int64, C.c_float, vp, vp, vp]
     L.orc_castray_splitmix.argtypes = [vp, C.c_int, C.c_int, C.c_int64, vp, vp, C.c_int, C.c_float, vp]
+    L.orc_castray_pcg.argtypes = [vp, C.c_int64, vp, vp, C.c_uint64, C.c_int, C.c_float, vp]
     L.orc_render.argtypes = [vp, C.POINTER(RenderParams), C.c_int, C.c_int, vp, C.c_int, vp, vp, vp]
     L.orc_photon_build.restype = vp
     L.orc_photon_build.argtypes = [vp, C.c_int64, C.c_float, C.c_int, C.c_uint64, C.c_int]
@@ -243,6 +244,12 @@ class OracleScene:
         s64 = _c((s << np.uint64(32)) | s, np.uint64)
         L = np.zeros((pix.shape[0], 3), np.float32)
         self.L.orc_castray_splitmix(self.h, w, h, pix.shape[0], _p(pix), _p(s64), max_depth, clamp, _p(L))
+        return L
+
+    def castray_pcg(self, org, d, seed, max_depth, clamp=10.0):
+        org, d = _c(org), _c(d)
+        L = np.zeros((org.shape[0], 3), np.float32)
+        self.L.orc_castray_pcg(self.h, org.shape[0], _p(org), _p(d), seed, max_depth, clamp, _p(L))
         return L
 
     def render(self, width, height, spp, max_depth, seed, clamp=10.0, sample_offset=0, rows=None,

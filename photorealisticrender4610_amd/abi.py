@@ -98,6 +98,7 @@ SIGNATURES = {
     "prr_camera_rays": [vp, vp, i32, i32, i64, vp, vp, vp, vp],
     "prr_tonemap": [vp, i32, i64, C.c_float, vp, vp, vp],
     "prr_render": [vp, vp, C.POINTER(RenderParams), vp, C.POINTER(Stats)],
+    "prr_cast_rays": [vp, vp, i64, vp, vp, C.c_uint64, i32, C.c_float, vp, C.POINTER(Stats)],
     "prr_photon_build": [vp, vp, i64, C.c_float, i32, C.c_uint64, C.POINTER(i64)],
     "prr_photon_export": [vp, vp, i64, vp, C.POINTER(i64)],
     "prr_photon_radiance": [vp, vp, i64, vp, vp, vp, vp, vp],
@@ -309,6 +310,17 @@ class Context:
         self._bind_stream()
         self._done(lib().prr_tonemap(self.h, opi, n, exposure, _ptr(lin), _ptr(disp), _ptr(rgb8)))
         return disp, rgb8
+
+    def cast_rays(self, scene, org, d, max_depth, seed=0, clamp=10.0):
+        """Integrator::castRay for a batch of rays (device)."""
+        T = self.torch
+        org, d = self._f32(org), self._f32(d)
+        L = T.empty((org.shape[0], 3), dtype=T.float32, device=self.device)
+        st = Stats()
+        self._bind_stream()
+        self._done(lib().prr_cast_rays(self.h, scene.h, org.shape[0], _ptr(org), _ptr(d), int(seed), int(max_depth),
+                                       float(clamp), _ptr(L), C.byref(st)))
+        return L
 
     def photon_build(self, scene, num, radius, max_depth=30, seed=0xC2):
         """Scene::buildPhotonMaps: trace num photons, keep the grid resident."""

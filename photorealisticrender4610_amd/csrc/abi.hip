@@ -558,6 +558,80 @@ static int ensureRenderBufs(prr_context *c, int64_t slots, int maxDepth) {
     return PRR_OK;
 }
 
+struct Timing {
+    bool on = false;
+    std::vector<std::pair<int, int>> timed;  // (kind 0 extend / 1 shadow / 2 shade, first event)
+    size_t next = 0;
+};
+
+// Runs bounces 0..maxDepth of one chunk whose queue 0 is already filled.
+static int runBounces(prr_context *c, const SceneView &view, const ChunkInfo &ci, Timing &tm, hipStream_t st) {
+    auto evPair = [&](int kind) -> int {
+        if (!tm.on) return -1;
+        if (tm.next + 2 > c->evPool.size()) {
+            for (int k = 0; k < 256; k++) {
+                hipEvent_t e;
+                if (hipEventCreate(&e) != hipSuccess) return -1;
+                c->evPool.push_back(e);
+            }
+        }
+        const int i = (int) tm.next;
+        tm.next += 2;
+        tm.timed.emplace_back(kind, i);
+        return i;
+    };
+#define TIMED(kind, launch)                                             \
+    do {                                                                \
+        int ei_ = evPair(kind);                                         \
+        if (ei_ >= 0) HIPCK(c, hipEventRecord(c->evPool[ei_], st));     \
+        HIPCK(c, launch);                                               \
+        if (ei_ >= 0) HIPCK(c, hipEventRecord(c->evPool[ei_ + 1], st)); \
+    } while (0)
+    for (int b = 0; b <= ci.maxDepth; b++) {
+        TIMED(0, launchExtend(view, c->pb, b, c->spill, c->traceBlocks, c->refillAt, c->stackDepth, st));
+        if (b > 0) TIMED(1, launchShadow(view, c->pb, b - 1, c->spill, c->traceBlocks, c->refillAt, c->stackDepth, st));
+        TIMED(2, launchShade(view, c->pb, ci, b, c->shadeBlocks, st));
+    }
+#undef TIMED
+    return PRR_OK;
+}
+
+static int resetChunk(prr_context *c, int maxDepth, hipStream_t st) {
+    HIPCK(c, hipMemsetAsync(c->pb.qcnt, 0, ((size_t) maxDepth + 2) * sizeof(uint32_t), st));
+    HIPCK(c, hipMemsetAsync(c->pb.sqcnt, 0, ((size_t) maxDepth + 2) * sizeof(uint32_t), st));
+    HIPCK(c, hipMemsetAsync(c->pb.fetch, 0, 2 * ((size_t) maxDepth + 2) * sizeof(uint32_t), st));
+    return PRR_OK;
+}
+
+static int finishStats(prr_context *c, const Timing &tm, uint64_t photons, prr_stats *stats) {
+    if (!stats) return PRR_OK;
+    unsigned long long h[8];
+    HIPCK(c, hipMemcpy(h, c->stats, sizeof(h), hipMemcpyDeviceToHost));
+    float ms = 0.f;
+    HIPCK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    stats->rays_closest = h[0];
+    stats->rays_any = h[1];
+    stats->samples = h[2];
+    stats->photons_stored = photons;
+    stats->ms_total = ms;
+    stats->ms_extend = stats->ms_shadow = stats->ms_shade = 0.0;
+    stats->launches_extend = stats->launches_shadow = 0;
+    for (const auto &t : tm.timed) {
+        float dt = 0.f;
+        HIPCK(c, hipEventElapsedTime(&dt, c->evPool[t.second], c->evPool[t.second + 1]));
+        if (t.first == 0) {
+            stats->ms_extend += dt;
+            stats->launches_extend++;
+        } else if (t.first == 1) {
+            stats->ms_shadow += dt;
+            stats->launches_shadow++;
+        } else {
+            stats->ms_shade += dt;
+        }
+    }
+    return PRR_OK;
+}
+
 int prr_render(prr_context *c, const prr_scene *s, const prr_render_params *p, float *rgb, prr_stats *stats) {
     if (!c || !s || !p || !rgb) return fail(c, PRR_ERR_INVALID, "NULL argument");
     if (p->width <= 0 || p->height <= 0 || p->spp <= 0 || p->max_depth <= 0 || p->max_depth > 64 ||
@@ -582,33 +656,10 @@ int prr_render(prr_context *c, const prr_scene *s, const prr_render_params *p, f
     HIPCK(c, hipMemsetAsync(c->stats, 0, 8 * sizeof(unsigned long long), st));
     HIPCK(c, hipMemsetAsync(c->fb, 0, (size_t) pixels * sizeof(float4), st));
     HIPCK(c, hipEventRecord(c->ev0, st));
-    const int traceGrid = c->traceBlocks;
     SceneView view = s->view;
     if (p->num_photons <= 0) view.ph_count = 0;  // photon gather only on request
-    const bool timing = (p->flags & PRR_RENDER_KERNEL_TIMING) != 0;
-    // per-launch event pairs: kind 0 extend, 1 shadow, 2 shade
-    std::vector<std::pair<int, int>> timed;  // (kind, first event index)
-    size_t evNext = 0;
-    auto evPair = [&](int kind) -> int {
-        if (evNext + 2 > c->evPool.size()) {
-            for (int k = 0; k < 256; k++) {
-                hipEvent_t e;
-                if (hipEventCreate(&e) != hipSuccess) return -1;
-                c->evPool.push_back(e);
-            }
-        }
-        const int i = (int) evNext;
-        evNext += 2;
-        timed.emplace_back(kind, i);
-        return i;
-    };
-#define TIMED(kind, launch)                                            \
-    do {                                                               \
-        int ei_ = timing ? evPair(kind) : -1;                          \
-        if (ei_ >= 0) HIPCK(c, hipEventRecord(c->evPool[ei_], st));    \
-        HIPCK(c, launch);                                              \
-        if (ei_ >= 0) HIPCK(c, hipEventRecord(c->evPool[ei_ + 1], st)); \
-    } while (0)
+    Timing tm;
+    tm.on = (p->flags & PRR_RENDER_KERNEL_TIMING) != 0;
     for (int s0 = 0; s0 < p->spp; s0 += spc) {
         ChunkInfo ci;
         ci.W = p->width;
@@ -619,48 +670,52 @@ int prr_render(prr_context *c, const prr_scene *s, const prr_render_params *p, f
         ci.seed = p->seed;
         ci.maxDepth = p->max_depth;
         ci.clampInd = p->indirect_clamp;
-        HIPCK(c, hipMemsetAsync(c->pb.qcnt, 0, ((size_t) p->max_depth + 2) * sizeof(uint32_t), st));
-        HIPCK(c, hipMemsetAsync(c->pb.sqcnt, 0, ((size_t) p->max_depth + 2) * sizeof(uint32_t), st));
-        HIPCK(c, hipMemsetAsync(c->pb.fetch, 0, 2 * ((size_t) p->max_depth + 2) * sizeof(uint32_t), st));
+        if ((rc = resetChunk(c, p->max_depth, st)) != PRR_OK) return rc;
         HIPCK(c, launchRaygen(view, c->pb, ci, c->shadeBlocks, st));
-        for (int b = 0; b <= p->max_depth; b++) {
-            TIMED(0, launchExtend(view, c->pb, b, c->spill, traceGrid, c->refillAt, c->stackDepth, st));
-            if (b > 0) TIMED(1, launchShadow(view, c->pb, b - 1, c->spill, traceGrid, c->refillAt, c->stackDepth, st));
-            TIMED(2, launchShade(view, c->pb, ci, b, c->shadeBlocks, st));
-        }
+        if ((rc = runBounces(c, view, ci, tm, st)) != PRR_OK) return rc;
         HIPCK(c, launchAccumulate(c->pb, ci, c->fb, st));
     }
-#undef TIMED
     HIPCK(c, launchResolve(c->fb, (int) pixels, p->spp, rgb, st));
     HIPCK(c, hipEventRecord(c->ev1, st));
     HIPCK(c, hipStreamSynchronize(st));
-    if (stats) {
-        unsigned long long h[8];
-        HIPCK(c, hipMemcpy(h, c->stats, sizeof(h), hipMemcpyDeviceToHost));
-        float ms = 0.f;
-        HIPCK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
-        stats->rays_closest = h[0];
-        stats->rays_any = h[1];
-        stats->samples = h[2];
-        stats->photons_stored = p->num_photons > 0 ? (uint64_t) s->ph.stored : 0;
-        stats->ms_total = ms;
-        stats->ms_extend = stats->ms_shadow = stats->ms_shade = 0.0;
-        stats->launches_extend = stats->launches_shadow = 0;
-        for (const auto &t : timed) {
-            float dt = 0.f;
-            HIPCK(c, hipEventElapsedTime(&dt, c->evPool[t.second], c->evPool[t.second + 1]));
-            if (t.first == 0) {
-                stats->ms_extend += dt;
-                stats->launches_extend++;
-            } else if (t.first == 1) {
-                stats->ms_shadow += dt;
-                stats->launches_shadow++;
-            } else {
-                stats->ms_shade += dt;
-            }
-        }
+    return finishStats(c, tm, p->num_photons > 0 ? (uint64_t) s->ph.stored : 0, stats);
+}
+
+int prr_cast_rays(prr_context *c, const prr_scene *s, int64_t n, const float *org, const float *dir, uint64_t seed,
+                  int32_t max_depth, float indirect_clamp, float *L, prr_stats *stats) {
+    if (!c || !s) return fail(c, PRR_ERR_INVALID, "NULL context or scene");
+    if (n < 0 || max_depth <= 0 || max_depth > 64 || (n > 0 && (!org || !dir || !L)))
+        return fail(c, PRR_ERR_INVALID, "bad arguments");
+    if (n == 0) return PRR_OK;
+    HIPCK(c, hipSetDevice(c->device));
+    const int64_t chunk = std::min<int64_t>(n, kMaxSlots);
+    int rc = ensureRenderBufs(c, chunk, max_depth);
+    if (rc != PRR_OK) return rc;
+    hipStream_t st = c->stream;
+    HIPCK(c, hipMemsetAsync(c->stats, 0, 8 * sizeof(unsigned long long), st));
+    HIPCK(c, hipEventRecord(c->ev0, st));
+    SceneView view = s->view;
+    view.ph_count = 0;
+    Timing tm;
+    for (int64_t first = 0; first < n; first += chunk) {
+        const int64_t m = std::min(chunk, n - first);
+        ChunkInfo ci;
+        ci.W = (int) m;
+        ci.H = 1;
+        ci.pixels = (int) m;
+        ci.spc = 1;
+        ci.sampleBase = 0;
+        ci.seed = seed;
+        ci.maxDepth = max_depth;
+        ci.clampInd = indirect_clamp;
+        if ((rc = resetChunk(c, max_depth, st)) != PRR_OK) return rc;
+        HIPCK(c, launchRaygenRays(c->pb, m, org, dir, first, seed, c->shadeBlocks, st));
+        if ((rc = runBounces(c, view, ci, tm, st)) != PRR_OK) return rc;
+        HIPCK(c, launchCopyResults(c->pb, m, first, L, st));
     }
-    return PRR_OK;
+    HIPCK(c, hipEventRecord(c->ev1, st));
+    HIPCK(c, hipStreamSynchronize(st));
+    return finishStats(c, tm, 0, stats);
 }
 
 }  // extern "C"

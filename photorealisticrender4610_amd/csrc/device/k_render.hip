@@ -69,6 +69,38 @@ __global__ __launch_bounds__(kShadeBlock) void kRaygen(SceneView s, PathBufs pb,
 }
 
 // ---------------------------------------------------------------------------
+// Integrator::castRay on caller rays: slot k traces ray k with the PCG32
+// stream of (seed, k, sample 0); no camera draws.
+__global__ __launch_bounds__(kShadeBlock) void kRaygenRays(PathBufs pb, int64_t n, const float *org, const float *dir,
+                                                           int64_t first, uint64_t seed) {
+    const int64_t stride = (int64_t) gridDim.x * kShadeBlock;
+    for (int64_t slot = (int64_t) blockIdx.x * kShadeBlock + threadIdx.x; slot < n; slot += stride) {
+        const int64_t k = first + slot;
+        const Pcg rng = Pcg::make(seed, (uint32_t) k, 0u);
+        pb.L[slot] = make_float4(0.f, 0.f, 0.f, 0.f);
+        pb.beta[slot] = make_float4(1.f, 1.f, 1.f, 0.f);
+        pb.flags[slot] = 0u;
+        pb.rng[slot] = rng.state;
+        pb.qSlot[0][slot] = (uint32_t) slot;
+        pb.qOrg[0][slot] = make_float4(org[3 * k], org[3 * k + 1], org[3 * k + 2], 0.f);
+        pb.qDir[0][slot] = make_float4(dir[3 * k], dir[3 * k + 1], dir[3 * k + 2], 0.f);
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        pb.qcnt[0] = (uint32_t) n;
+        atomicAdd(&pb.stats[0], (unsigned long long) n);
+        atomicAdd(&pb.stats[2], (unsigned long long) n);
+    }
+}
+
+__global__ void kCopyResults(PathBufs pb, int64_t n, int64_t first, float *L) {
+    const int64_t i = (int64_t) blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const float4 r = pb.result[i];
+    L[3 * (first + i)] = r.x;
+    L[3 * (first + i) + 1] = r.y;
+    L[3 * (first + i) + 2] = r.z;
+}
+
 // Persistent traversal: idle lanes refill from the queue with one atomic per
 // wave once `refillAt` lanes are idle (64: only when the whole wave is idle,
 // which keeps a wave on 64 consecutive, mostly coherent queue entries).
@@ -523,6 +555,15 @@ __global__ void kResolve(const float4 *fb, int pixels, int spp, float *rgb) {
 // ---------------------------------------------------------------------------
 hipError_t launchRaygen(const SceneView &s, const PathBufs &pb, const ChunkInfo &ci, int grid, hipStream_t st) {
     hipLaunchKernelGGL(kRaygen, dim3(grid), dim3(kShadeBlock), 0, st, s, pb, ci);
+    return hipGetLastError();
+}
+hipError_t launchRaygenRays(const PathBufs &pb, int64_t n, const float *org, const float *dir, int64_t first,
+                            uint64_t seed, int grid, hipStream_t st) {
+    hipLaunchKernelGGL(kRaygenRays, dim3(grid), dim3(kShadeBlock), 0, st, pb, n, org, dir, first, seed);
+    return hipGetLastError();
+}
+hipError_t launchCopyResults(const PathBufs &pb, int64_t n, int64_t first, float *L, hipStream_t st) {
+    hipLaunchKernelGGL(kCopyResults, dim3((unsigned) ((n + 255) / 256)), dim3(256), 0, st, pb, n, first, L);
     return hipGetLastError();
 }
 hipError_t launchExtend(const SceneView &s, const PathBufs &pb, int bounce, uint2 *spill, int traceGrid,

@@ -8,6 +8,9 @@
 namespace prr {
 
 hipError_t launchRaygen(const SceneView &s, const PathBufs &pb, const ChunkInfo &ci, int grid, hipStream_t st);
+hipError_t launchRaygenRays(const PathBufs &pb, int64_t n, const float *org, const float *dir, int64_t first,
+                            uint64_t seed, int grid, hipStream_t st);
+hipError_t launchCopyResults(const PathBufs &pb, int64_t n, int64_t first, float *L, hipStream_t st);
 hipError_t launchExtend(const SceneView &s, const PathBufs &pb, int bounce, uint2 *spill, int traceGrid,
                         int refillAt, int stackDepth, hipStream_t st);
 hipError_t launchShadow(const SceneView &s, const PathBufs &pb, int bounce, uint2 *spill, int traceGrid,

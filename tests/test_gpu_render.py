@@ -111,3 +111,21 @@ def test_render_deterministic(ctx):
     a, _ = ctx.render(scene, 64, 64, 8, 8, seed=7)
     b, _ = ctx.render(scene, 64, 64, 8, 8, seed=7)
     assert np.array_equal(a.cpu().numpy(), b.cpu().numpy())
+
+
+def test_cast_rays_vs_oracle(ctx):
+    """prr_cast_rays (Integrator::castRay on caller rays) against the oracle's
+    castRay on the same PCG32 streams."""
+    import testscenes as TS
+    sc = TS.mini_field()
+    scene = ctx.scene(sc)
+    rng = np.random.default_rng(8)
+    n = 20000
+    org = np.tile(np.array([[-6.0, 6.0, -8.0]], np.float32), (n, 1)) + rng.normal(scale=0.5, size=(n, 3)).astype(np.float32)
+    d = (np.array([[5.0, 2.0, 5.0]]) - org + rng.normal(scale=3.0, size=(n, 3))).astype(np.float32)
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    g = ctx.cast_rays(scene, org, d, 12, seed=21).cpu().numpy()
+    o = O.OracleScene(sc).castray_pcg(org, d, 21, 12)
+    close = np.all(np.abs(g - o) <= 1e-4 * np.maximum(np.abs(o), 1e-2), axis=1)
+    assert close.mean() > 0.98, close.mean()
+    assert (o.sum(1) > 0).mean() > 0.5
