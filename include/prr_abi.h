@@ -124,12 +124,12 @@ typedef struct prr_scene_desc {
     prr_camera_desc camera;
 } prr_scene_desc;
 
-/* RenderSettings (Renderer.hpp:42-55) restricted to fixed spp, plus the
- * Integrator parameters (Integrator.hpp:18-21). */
+/* RenderSettings (Renderer.hpp:42-55) plus the Integrator parameters
+ * (Integrator.hpp:18-21). */
 typedef struct prr_render_params {
     int32_t width;
     int32_t height;
-    int32_t spp;               /* samples per pixel (minSPP = maxSPP) */
+    int32_t spp;               /* RenderSettings::maxSPP */
     int32_t max_depth;         /* Integrator maxDepth */
     float indirect_clamp;      /* Integrator::indirectClamp (<=0 disables) */
     int32_t sample_offset;     /* index of the first sample (progressive / split renders) */
@@ -141,7 +141,13 @@ typedef struct prr_render_params {
     float photon_radius;
     int32_t photon_max_depth;
     int32_t flags;             /* PRR_RENDER_* bits */
-    int32_t reserved[3];
+    /* Adaptive sampling (Renderer.cpp:74-99, Welford Renderer.hpp:14-40):
+     * once a pixel has min_spp samples it stops as soon as the 95% CI of all
+     * three channel means is below variance_threshold * |mean|. With
+     * variance_threshold <= 0 or min_spp >= spp every pixel takes spp samples. */
+    int32_t min_spp;           /* RenderSettings::minSPP */
+    float variance_threshold;  /* RenderSettings::varianceThreshold */
+    int32_t reserved;
 } prr_render_params;
 
 /* prr_render_params::flags */
@@ -255,9 +261,13 @@ int prr_photon_radiance(prr_context* ctx, const prr_scene* scene, int64_t n, con
 
 /* ---- the render: Renderer::Render sample loop + Integrator::castRay ----- */
 /* Writes the per-pixel mean linear radiance (width*height*3 floats, device)
- * of a fixed-spp render. Blocks until the render is done; stats may be NULL. */
+ * and, when spp_out is not NULL, the samples each pixel took (width*height
+ * int32, device; the reference's sampleCount, Renderer.cpp:98). Sample k of
+ * pixel p uses the PCG32 stream of (seed, p, sample_offset + k), so a pixel's
+ * result does not depend on the image size, the batching or the other
+ * pixels. Blocks until the render is done; stats may be NULL. */
 int prr_render(prr_context* ctx, const prr_scene* scene, const prr_render_params* params,
-               float* rgb_out, prr_stats* stats);
+               float* rgb_out, int32_t* spp_out, prr_stats* stats);
 
 /* Integrator::castRay (Integrator.cpp:104-263) for n caller rays (device
  * arrays): ray k runs the full path with the PCG32 stream of (seed, k,

@@ -524,11 +524,59 @@ extern "C" void orc_photon_irradiance(const orc_photon_map *pm, int64_t n, const
 }
 
 // ---------------------------------------------------------------------------
-// Renderer::Render sample loop (Renderer.cpp:63-99) at fixed spp, PCG32 per
-// (pixel, sample), accumulated in sample order.
+// Renderer::Render sample loop (Renderer.cpp:63-99), PCG32 per (pixel,
+// sample), accumulated in sample order, with the adaptive stop rule.
 // ---------------------------------------------------------------------------
+// Welford (Renderer.hpp:14-40): running mean / M2 of one channel in float.
+struct WelfordF {
+    int n = 0;
+    float mean = 0.f, m2 = 0.f;
+    void add(float x) {
+        n++;
+        float d = x - mean;
+        mean += d / (float) n;
+        m2 += d * (x - mean);
+    }
+    // 95% CI half-width relative to |mean|; 1 until two samples / nonzero mean
+    float ci() const {
+        if (n < 2 || std::fabs(mean) < 1e-6f) return 1.f;
+        return 1.96f * std::sqrt(m2 / (float) (n - 1) / (float) n) / std::fabs(mean);
+    }
+};
+struct PixelStop {
+    WelfordF w[3];
+    int minSpp;
+    float thr;
+    // Renderer.cpp:88-96: update, then stop once n >= minSPP and all CIs < thr
+    bool add(const V3 &c) {
+        w[0].add(c.x);
+        w[1].add(c.y);
+        w[2].add(c.z);
+        return w[0].n >= minSpp && w[0].ci() < thr && w[1].ci() < thr && w[2].ci() < thr;
+    }
+};
+
+extern "C" void orc_welford(int nseq, int len, const float *seq, const int32_t *min_spp, const float *thr,
+                            int32_t *count, float *mean) {
+    for (int q = 0; q < nseq; q++) {
+        PixelStop ps;
+        ps.minSpp = min_spp[q];
+        ps.thr = thr[q];
+        V3 sum(0.f);
+        int n = 0;
+        while (n < len) {
+            V3 c = v3(seq + 3 * ((size_t) q * len + n));
+            sum += c;
+            n++;
+            if (ps.add(c)) break;
+        }
+        count[q] = n;
+        put3(mean + 3 * q, sum / (float) n);
+    }
+}
+
 extern "C" void orc_render(const orc_scene *s, const prr_render_params *p, int row0, int row1,
-                           const orc_photon_map *pm, int threads, float *mean, float *var,
+                           const orc_photon_map *pm, int threads, float *mean, float *var, int32_t *spp_out,
                            uint64_t *counters) {
     const Scene &sc = s->sc;
     const int W = p->width, H = p->height;
@@ -556,6 +604,10 @@ extern "C" void orc_render(const orc_scene *s, const prr_render_params *p, int r
                 const uint32_t pix = (uint32_t) (j * W + i);
                 V3 sum(0.f);
                 double m1[3] = {0, 0, 0}, m2[3] = {0, 0, 0};
+                PixelStop ps;
+                ps.minSpp = p->min_spp;
+                ps.thr = p->variance_threshold;
+                int taken = 0;
                 for (int sIdx = 0; sIdx < p->spp; sIdx++) {
                     Pcg32 rng = Pcg32::make(p->seed, pix, (uint32_t) (p->sample_offset + sIdx));
                     float ox = rng.next();
@@ -565,6 +617,7 @@ extern "C" void orc_render(const orc_scene *s, const prr_render_params *p, int r
                     if (std::isnan(c.x) || std::isnan(c.y) || std::isnan(c.z)) c = V3(0.f);
                     sum += c;
                     samples++;
+                    taken++;
                     if (var) {
                         double cv[3] = {c.x, c.y, c.z};
                         for (int k = 0; k < 3; k++) {
@@ -573,12 +626,14 @@ extern "C" void orc_render(const orc_scene *s, const prr_render_params *p, int r
                             m2[k] += d * (cv[k] - m1[k]);
                         }
                     }
+                    if (ps.add(c)) break;
                 }
-                V3 mv = sum / (float) p->spp;
+                V3 mv = sum / (float) taken;
                 if (mean) put3(mean + 3 * (size_t) pix, mv);
+                if (spp_out) spp_out[pix] = taken;
                 if (var)
                     for (int k = 0; k < 3; k++)
-                        var[3 * (size_t) pix + k] = p->spp > 1 ? (float) (m2[k] / (p->spp - 1)) : 0.f;
+                        var[3 * (size_t) pix + k] = taken > 1 ? (float) (m2[k] / (taken - 1)) : 0.f;
             }
         }
         cClosest += cnt.closest;

@@ -89,9 +89,11 @@ void orc_castray_splitmix(const orc_scene* s, int width, int height, int64_t n,
 void orc_castray_pcg(const orc_scene* s, int64_t n, const float* org, const float* dir, uint64_t seed,
                      int max_depth, float indirect_clamp, float* L);
 
-/* Fixed-spp render with PCG32 per (pixel, sample) (the GPU renderer's RNG).
- * mean: W*H*3 (sum in sample order / spp), var: W*H*3 sample variance of the
- * per-sample radiance (may be NULL). rows [row0,row1) only (row1<=0: all).
+/* Render with PCG32 per (pixel, sample) (the GPU renderer's RNG).
+ * mean: W*H*3 (sum in sample order / samples taken), var: W*H*3 sample
+ * variance of the per-sample radiance (may be NULL), spp_out: samples taken
+ * per pixel (may be NULL; < spp only under the adaptive stop rule of
+ * prr_render_params). rows [row0,row1) only (row1<=0: all).
  * counters (may be NULL, 7 entries): closest queries, any-hit queries, samples,
  * closest-hit interior expansions, closest-hit triangle tests, any-hit
  * interior expansions, any-hit triangle tests. threads: OpenMP
@@ -99,7 +101,13 @@ void orc_castray_pcg(const orc_scene* s, int64_t n, const float* org, const floa
 typedef struct orc_photon_map orc_photon_map;
 void orc_render(const orc_scene* s, const prr_render_params* p, int row0, int row1,
                 const orc_photon_map* pm, int threads, float* mean, float* var,
-                uint64_t* counters);
+                int32_t* spp_out, uint64_t* counters);
+
+/* Renderer.cpp:71-99 on given per-sample values: nseq sequences of len RGB
+ * samples; stop rule of Welford (Renderer.hpp:14-40) per sequence with its own
+ * min_spp / threshold. Writes samples taken and their mean. */
+void orc_welford(int nseq, int len, const float* seq, const int32_t* min_spp, const float* thr,
+                 int32_t* count, float* mean);
 
 /* Caustic photon map (Photon.cpp, build extensions: point-light emission and
  * hash-grid storage). */

@@ -49,7 +49,8 @@ class RenderParams(C.Structure):
     _fields_ = [("width", C.c_int32), ("height", C.c_int32), ("spp", C.c_int32), ("max_depth", C.c_int32),
                 ("indirect_clamp", C.c_float), ("sample_offset", C.c_int32), ("seed", C.c_uint64),
                 ("num_photons", C.c_int64), ("photon_radius", C.c_float), ("photon_max_depth", C.c_int32),
-                ("flags", C.c_int32), ("reserved", C.c_int32 * 3)]
+                ("flags", C.c_int32), ("min_spp", C.c_int32), ("variance_threshold", C.c_float),
+                ("reserved", C.c_int32)]
 
 
 def _f(a):
@@ -142,7 +143,8 @@ def lib(fast=False):
     L.orc_tonemap.argtypes = [C.c_int, C.c_int64, C.c_float, vp, vp, vp]
     L.orc_castray_splitmix.argtypes = [vp, C.c_int, C.c_int, C.c_int64, vp, vp, C.c_int, C.c_float, vp]
     L.orc_castray_pcg.argtypes = [vp, C.c_int64, vp, vp, C.c_uint64, C.c_int, C.c_float, vp]
-    L.orc_render.argtypes = [vp, C.POINTER(RenderParams), C.c_int, C.c_int, vp, C.c_int, vp, vp, vp]
+    L.orc_render.argtypes = [vp, C.POINTER(RenderParams), C.c_int, C.c_int, vp, C.c_int, vp, vp, vp, vp]
+    L.orc_welford.argtypes = [C.c_int, C.c_int, vp, vp, vp, vp, vp]
     L.orc_photon_build.restype = vp
     L.orc_photon_build.argtypes = [vp, C.c_int64, C.c_float, C.c_int, C.c_uint64, C.c_int]
     L.orc_photon_free.argtypes = [vp]
@@ -253,17 +255,22 @@ class OracleScene:
         return L
 
     def render(self, width, height, spp, max_depth, seed, clamp=10.0, sample_offset=0, rows=None,
-               photon_map=None, threads=0, want_var=False):
+               photon_map=None, threads=0, want_var=False, min_spp=0, variance_threshold=0.0, want_spp=False):
         p = RenderParams(width=width, height=height, spp=spp, max_depth=max_depth, indirect_clamp=clamp,
-                         sample_offset=sample_offset, seed=seed)
+                         sample_offset=sample_offset, seed=seed, min_spp=min_spp,
+                         variance_threshold=variance_threshold)
         mean = np.zeros((height, width, 3), np.float32)
         var = np.zeros((height, width, 3), np.float32) if want_var else None
+        sppo = np.zeros((height, width), np.int32) if want_spp else None
         cnt = np.zeros(7, np.uint64)
         r0, r1 = rows if rows else (0, 0)
         self.L.orc_render(self.h, C.byref(p), r0, r1, photon_map.h if photon_map else None, threads,
-                          _p(mean), _p(var), _p(cnt))
+                          _p(mean), _p(var), _p(sppo), _p(cnt))
         keys = ["closest", "any", "samples", "inner", "tris", "inner_any", "tris_any"]
-        return mean, var, dict(zip(keys, (int(x) for x in cnt)))
+        stats = dict(zip(keys, (int(x) for x in cnt)))
+        if want_spp:
+            stats["spp"] = sppo
+        return mean, var, stats
 
 
 class OraclePhotonMap:
@@ -401,3 +408,16 @@ def splitmix_draws(seed_u32, k):
             z = z ^ (z >> np.uint64(31))
             out.append(np.float32(int(z >> np.uint64(40))) * np.float32(2.0 ** -24))
     return np.array(out, np.float32)
+
+
+def welford(seq, min_spp, thr):
+    """Renderer.cpp:71-99 stop rule on per-sample values seq (nseq, len, 3);
+    min_spp / thr per sequence. Returns (samples taken, mean)."""
+    seq = _c(seq)
+    nseq, ln = seq.shape[0], seq.shape[1]
+    ms = _c(np.broadcast_to(np.asarray(min_spp, np.int32), (nseq,)), np.int32)
+    th = _c(np.broadcast_to(np.asarray(thr, np.float32), (nseq,)))
+    cnt = np.zeros(nseq, np.int32)
+    mean = np.zeros((nseq, 3), np.float32)
+    lib().orc_welford(nseq, ln, _p(seq), _p(ms), _p(th), _p(cnt), _p(mean))
+    return cnt, mean

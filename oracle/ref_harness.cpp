@@ -3,7 +3,7 @@
 // TEST INFRASTRUCTURE ONLY.
 //
 // Usage: ref_harness <mode> <in.bundle> <out.bundle>
-//   modes: trace, bsdf, env, camera, tonemap, castray, image
+//   modes: trace, bsdf, env, camera, tonemap, castray, image, welford, hdr
 // A bundle is a sequence of records {char name[32]; int32 dtype; int64 count;
 // payload}, dtype 0=f32 1=i32 2=u64 3=u8 4=u32 (tests/golden/bundle.py).
 //
@@ -37,6 +37,7 @@
 #include "EnvMap.hpp"
 #include "Integrator.hpp"
 #include "Material.hpp"
+#include "Renderer.hpp"
 #include "Scene.hpp"
 #include "Sphere.hpp"
 #include "ToneMapping.hpp"
@@ -526,6 +527,63 @@ static int modeImage(const Bundle &b, const char *out) {
     return 0;
 }
 
+// Renderer.cpp:71-99 per pixel on given sample sequences, with the
+// reference's own Welford (Renderer.hpp:14-40): samples taken until n reaches
+// len or, once n >= minSPP, all three CIs are below the threshold.
+static int modeWelford(const Bundle &b, const char *out) {
+    const int32_t *cfg = b.at("cfg").as<int32_t>();  // nseq, len
+    const int nseq = cfg[0], len = cfg[1];
+    const float *seq = b.at("seq").as<float>();
+    const int32_t *minSpp = b.at("minspp").as<int32_t>();
+    const float *thr = b.at("thr").as<float>();
+    std::vector<int32_t> count(nseq);
+    std::vector<float> mean(3 * (size_t) nseq);
+    for (int q = 0; q < nseq; q++) {
+        Welford wR, wG, wB;
+        Vector3f fb(0.f);
+        int n = 0;
+        while (n < len) {
+            const float *x = seq + 3 * ((size_t) q * len + n);
+            Vector3f s(x[0], x[1], x[2]);
+            fb += s;
+            n++;
+            wR.update(s.x);
+            wG.update(s.y);
+            wB.update(s.z);
+            if (n >= minSpp[q])
+                if (wR.ci() < thr[q] && wG.ci() < thr[q] && wB.ci() < thr[q]) break;
+        }
+        fb = fb / (float) n;
+        count[q] = n;
+        mean[3 * q] = fb.x;
+        mean[3 * q + 1] = fb.y;
+        mean[3 * q + 2] = fb.z;
+    }
+    Writer w(out);
+    w.i32("n", count);
+    w.f32("mean", mean);
+    return 0;
+}
+
+// EnvMap::load (EnvMap.hpp:35-55) of the .hdr file named in the bundle:
+// decoded linear pixels.
+static int modeHdr(const Bundle &b, const char *out) {
+    const auto &pa = b.at("path");
+    std::string path(reinterpret_cast<const char *>(pa.bytes.data()), pa.bytes.size());
+    EnvMap env;
+    if (!env.load(path)) return 1;
+    std::vector<float> px(3 * env.pixels.size());
+    for (size_t i = 0; i < env.pixels.size(); i++) {
+        px[3 * i] = env.pixels[i].x;
+        px[3 * i + 1] = env.pixels[i].y;
+        px[3 * i + 2] = env.pixels[i].z;
+    }
+    Writer w(out);
+    w.i32("wh", {env.width, env.height});
+    w.f32("pixels", px);
+    return 0;
+}
+
 int main(int argc, char **argv) {
     if (argc != 4) {
         fprintf(stderr, "usage: %s <mode> <in> <out>\n", argv[0]);
@@ -540,6 +598,8 @@ int main(int argc, char **argv) {
     if (m == "tonemap") return modeTonemap(b, argv[3]);
     if (m == "castray") return modeCastray(b, argv[3]);
     if (m == "image") return modeImage(b, argv[3]);
+    if (m == "welford") return modeWelford(b, argv[3]);
+    if (m == "hdr") return modeHdr(b, argv[3]);
     fprintf(stderr, "unknown mode %s\n", argv[1]);
     return 2;
 }

@@ -57,7 +57,8 @@ class RenderParams(C.Structure):
     _fields_ = [("width", C.c_int32), ("height", C.c_int32), ("spp", C.c_int32), ("max_depth", C.c_int32),
                 ("indirect_clamp", C.c_float), ("sample_offset", C.c_int32), ("seed", C.c_uint64),
                 ("num_photons", C.c_int64), ("photon_radius", C.c_float), ("photon_max_depth", C.c_int32),
-                ("flags", C.c_int32), ("reserved", C.c_int32 * 3)]
+                ("flags", C.c_int32), ("min_spp", C.c_int32), ("variance_threshold", C.c_float),
+                ("reserved", C.c_int32)]
 
 
 class Stats(C.Structure):
@@ -97,7 +98,7 @@ SIGNATURES = {
     "prr_env_eval": [vp, vp, i64, vp, vp, vp],
     "prr_camera_rays": [vp, vp, i32, i32, i64, vp, vp, vp, vp],
     "prr_tonemap": [vp, i32, i64, C.c_float, vp, vp, vp],
-    "prr_render": [vp, vp, C.POINTER(RenderParams), vp, C.POINTER(Stats)],
+    "prr_render": [vp, vp, C.POINTER(RenderParams), vp, vp, C.POINTER(Stats)],
     "prr_cast_rays": [vp, vp, i64, vp, vp, C.c_uint64, i32, C.c_float, vp, C.POINTER(Stats)],
     "prr_photon_build": [vp, vp, i64, C.c_float, i32, C.c_uint64, C.POINTER(i64)],
     "prr_photon_export": [vp, vp, i64, vp, C.POINTER(i64)],
@@ -350,22 +351,30 @@ class Context:
         return out
 
     def render(self, scene, width, height, spp, max_depth, seed=0, clamp=10.0, sample_offset=0, out=None,
-               kernel_timing=False, photons=False):
-        """Renderer::Render at fixed spp: returns (H, W, 3) mean linear radiance
-        (device tensor) and a stats dict. Blocks until done."""
+               kernel_timing=False, photons=False, min_spp=0, variance_threshold=0.0, counts=False):
+        """Renderer::Render: returns (H, W, 3) mean linear radiance (device
+        tensor) and a stats dict; with counts=True the stats hold the (H, W)
+        int32 samples per pixel under "spp". spp is maxSPP; min_spp and
+        variance_threshold enable the Welford stop rule. Blocks until done."""
         T = self.torch
         if out is None:
             out = T.empty((height, width, 3), dtype=T.float32, device=self.device)
+        cnt = T.empty((height, width), dtype=T.int32, device=self.device) if counts else None
         p = RenderParams(width=width, height=height, spp=spp, max_depth=max_depth, indirect_clamp=clamp,
                          sample_offset=sample_offset, seed=seed, flags=1 if kernel_timing else 0,
-                         num_photons=getattr(scene, "photons", 0) if photons else 0)
+                         num_photons=getattr(scene, "photons", 0) if photons else 0, min_spp=min_spp,
+                         variance_threshold=variance_threshold)
         st = Stats()
         self._bind_stream()
-        self._done(lib().prr_render(self.h, scene.h, C.byref(p), _ptr(out), C.byref(st)))
-        return out, {"rays_closest": st.rays_closest, "rays_any": st.rays_any, "samples": st.samples,
-                     "photons_stored": st.photons_stored, "ms": st.ms_total, "ms_extend": st.ms_extend,
-                     "ms_shadow": st.ms_shadow, "ms_shade": st.ms_shade, "launches_extend": st.launches_extend,
-                     "launches_shadow": st.launches_shadow}
+        self._done(lib().prr_render(self.h, scene.h, C.byref(p), _ptr(out),
+                                    _ptr(cnt) if cnt is not None else None, C.byref(st)))
+        res = {"rays_closest": st.rays_closest, "rays_any": st.rays_any, "samples": st.samples,
+               "photons_stored": st.photons_stored, "ms": st.ms_total, "ms_extend": st.ms_extend,
+               "ms_shadow": st.ms_shadow, "ms_shade": st.ms_shade, "launches_extend": st.launches_extend,
+               "launches_shadow": st.launches_shadow}
+        if cnt is not None:
+            res["spp"] = cnt
+        return out, res
 
 
 class Scene:

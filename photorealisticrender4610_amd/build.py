@@ -62,6 +62,21 @@ def build_lib(jobs=None, verbose=True):
     return LIB
 
 
+def build_dropin_test(verbose=True):
+    """tests/cpp/dropin_main.cpp: a scene script against the C++ drop-in API
+    (include/prr/prr.hpp), linked to libprr.so."""
+    src = os.path.join(ROOT, "tests", "cpp", "dropin_main.cpp")
+    exe = os.path.join(OUT, "dropin_main")
+    cmd = ["g++", "-std=c++17", "-O2", "-I" + os.path.join(ROOT, "include"), src, "-o", exe,
+           "-L" + OUT, "-lprr", "-Wl,-rpath,$ORIGIN"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("drop-in test build failed: %s\n%s" % (r.stdout, r.stderr))
+    if verbose:
+        print("built", exe)
+    return exe
+
+
 def build_oracle(verbose=True):
     mk = shutil.which("make")
     cmd = [mk, "-C", os.path.join(ROOT, "oracle"), "-j4", "oracle"]
@@ -76,5 +91,6 @@ def build_oracle(verbose=True):
 
 if __name__ == "__main__":
     build_lib()
+    build_dropin_test()
     if "--no-oracle" not in sys.argv:
         build_oracle()

@@ -632,50 +632,116 @@ static int finishStats(prr_context *c, const Timing &tm, uint64_t photons, prr_s
     return PRR_OK;
 }
 
-int prr_render(prr_context *c, const prr_scene *s, const prr_render_params *p, float *rgb, prr_stats *stats) {
+// Framebuffer of the render: running sums, the adaptive sampler's Welford
+// state and its active-pixel list, in one allocation sized for `pixels`.
+static int ensureFrame(prr_context *c, int64_t pixels, AdaptiveBufs &ab) {
+    if (c->fbPixels < pixels) {
+        HIPCK(c, hipStreamSynchronize(c->stream));
+        if (c->fb) HIPCK(c, hipFree(c->fb));
+        c->fb = nullptr;
+        c->fbPixels = 0;
+        const size_t bytes = (size_t) pixels * (3 * sizeof(float4) + sizeof(int32_t)) + 256;
+        void *q = nullptr;
+        const hipError_t e = hipMalloc(&q, bytes);
+        if (e != hipSuccess) return fail(c, PRR_ERR_NOMEM, std::string("framebuffer: ") + hipGetErrorString(e));
+        c->fb = (float4 *) q;
+        c->fbPixels = pixels;
+    }
+    ab.mean = c->fb + c->fbPixels;
+    ab.m2 = ab.mean + c->fbPixels;
+    ab.list = (int32_t *) (ab.m2 + c->fbPixels);
+    ab.count = (uint32_t *) (ab.list + c->fbPixels);
+    return PRR_OK;
+}
+
+// Paths per adaptive pass: enough to keep every CU busy through the deep
+// bounces; the samples a pixel takes past its stop point are discarded.
+static const int64_t kAdaptivePassSlots = (int64_t) 1 << 22;
+
+int prr_render(prr_context *c, const prr_scene *s, const prr_render_params *p, float *rgb, int32_t *spp_out,
+               prr_stats *stats) {
     if (!c || !s || !p || !rgb) return fail(c, PRR_ERR_INVALID, "NULL argument");
     if (p->width <= 0 || p->height <= 0 || p->spp <= 0 || p->max_depth <= 0 || p->max_depth > 64 ||
-        p->sample_offset < 0)
+        p->sample_offset < 0 || p->min_spp < 0)
         return fail(c, PRR_ERR_INVALID, "bad render parameters");
     const int64_t pixels = (int64_t) p->width * p->height;
     if (pixels > kMaxSlots) return fail(c, PRR_ERR_UNSUPPORTED, "image larger than 16M pixels");
     if (p->num_photons > 0 && s->ph.stored <= 0 && s->ph.emitted <= 0)
         return fail(c, PRR_ERR_INVALID, "num_photons > 0 but the scene has no photon map (prr_photon_build)");
     HIPCK(c, hipSetDevice(c->device));
+    // Welford stopping can only end a pixel early when the threshold is
+    // positive and min_spp < spp (Renderer.cpp:92-96); otherwise every pixel
+    // takes spp samples.
+    const bool adaptive = p->variance_threshold > 0.f && p->min_spp < p->spp;
     const int spc = (int) std::max<int64_t>(1, std::min<int64_t>(p->spp, kMaxSlots / pixels));
-    int rc = ensureRenderBufs(c, pixels * spc, p->max_depth);
+    int rc = ensureRenderBufs(c, adaptive ? std::max<int64_t>(pixels * spc, std::min(kMaxSlots, kAdaptivePassSlots))
+                                          : pixels * spc,
+                              p->max_depth);
     if (rc != PRR_OK) return rc;
-    if (c->fbPixels < pixels) {
-        if (c->fb) HIPCK(c, hipFree(c->fb));
-        c->fb = nullptr;
-        c->fbPixels = 0;
-        HIPCK(c, hipMalloc(&c->fb, (size_t) pixels * sizeof(float4)));
-        c->fbPixels = pixels;
-    }
+    AdaptiveBufs ab;
+    if ((rc = ensureFrame(c, pixels, ab)) != PRR_OK) return rc;
     hipStream_t st = c->stream;
     HIPCK(c, hipMemsetAsync(c->stats, 0, 8 * sizeof(unsigned long long), st));
     HIPCK(c, hipMemsetAsync(c->fb, 0, (size_t) pixels * sizeof(float4), st));
+    if (adaptive) {
+        HIPCK(c, hipMemsetAsync(ab.mean, 0, (size_t) pixels * sizeof(float4), st));
+        HIPCK(c, hipMemsetAsync(ab.m2, 0, (size_t) pixels * sizeof(float4), st));
+    }
     HIPCK(c, hipEventRecord(c->ev0, st));
     SceneView view = s->view;
     if (p->num_photons <= 0) view.ph_count = 0;  // photon gather only on request
     Timing tm;
     tm.on = (p->flags & PRR_RENDER_KERNEL_TIMING) != 0;
-    for (int s0 = 0; s0 < p->spp; s0 += spc) {
-        ChunkInfo ci;
-        ci.W = p->width;
-        ci.H = p->height;
-        ci.pixels = (int) pixels;
-        ci.spc = std::min(spc, p->spp - s0);
-        ci.sampleBase = p->sample_offset + s0;
-        ci.seed = p->seed;
-        ci.maxDepth = p->max_depth;
-        ci.clampInd = p->indirect_clamp;
-        if ((rc = resetChunk(c, p->max_depth, st)) != PRR_OK) return rc;
-        HIPCK(c, launchRaygen(view, c->pb, ci, c->shadeBlocks, st));
-        if ((rc = runBounces(c, view, ci, tm, st)) != PRR_OK) return rc;
-        HIPCK(c, launchAccumulate(c->pb, ci, c->fb, st));
+    ChunkInfo ci;
+    ci.W = p->width;
+    ci.H = p->height;
+    ci.seed = p->seed;
+    ci.maxDepth = p->max_depth;
+    ci.clampInd = p->indirect_clamp;
+    // Renders samples [s0, s0 + count) of the listed pixels (all when list is
+    // null) in chunks of at most kMaxSlots paths.
+    auto renderSamples = [&](const int32_t *list, int64_t npix, int s0, int count) -> int {
+        const int per = (int) std::max<int64_t>(1, std::min<int64_t>(count, c->pbSlots / npix));
+        for (int k = 0; k < count; k += per) {
+            ci.pixList = list;
+            ci.pixels = (int) npix;
+            ci.spc = std::min(per, count - k);
+            ci.sampleBase = p->sample_offset + s0 + k;
+            int r;
+            if ((r = resetChunk(c, p->max_depth, st)) != PRR_OK) return r;
+            HIPCK(c, launchRaygen(view, c->pb, ci, c->shadeBlocks, st));
+            if ((r = runBounces(c, view, ci, tm, st)) != PRR_OK) return r;
+            if (adaptive)
+                HIPCK(c, launchAccumulateAdaptive(c->pb, ci, c->fb, ab, p->min_spp, p->variance_threshold, st));
+            else
+                HIPCK(c, launchAccumulate(c->pb, ci, c->fb, st));
+        }
+        return PRR_OK;
+    };
+    if (!adaptive) {
+        if ((rc = renderSamples(nullptr, pixels, 0, p->spp)) != PRR_OK) return rc;
+        HIPCK(c, launchResolve(c->fb, (int) pixels, p->spp, rgb, st));
+        if (spp_out) HIPCK(c, hipMemsetD32Async((hipDeviceptr_t) spp_out, p->spp, (size_t) pixels, st));
+    } else {
+        // First pass: min_spp samples everywhere (no pixel can stop earlier).
+        // Later passes: the pixels still sampling, several samples each when
+        // few remain. Every active pixel has taken the same number of samples.
+        int done = std::max(1, p->min_spp);
+        if ((rc = renderSamples(nullptr, pixels, 0, done)) != PRR_OK) return rc;
+        while (done < p->spp) {
+            HIPCK(c, hipMemsetAsync(ab.count, 0, sizeof(uint32_t), st));
+            HIPCK(c, launchCompactActive(ab, (int) pixels, p->spp, st));
+            uint32_t active = 0;
+            HIPCK(c, hipMemcpyAsync(&active, ab.count, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+            HIPCK(c, hipStreamSynchronize(st));
+            if (active == 0) break;
+            const int64_t want = (kAdaptivePassSlots + active - 1) / active;
+            const int batch = (int) std::min<int64_t>(p->spp - done, std::max<int64_t>(1, want));
+            if ((rc = renderSamples(ab.list, active, done, batch)) != PRR_OK) return rc;
+            done += batch;
+        }
+        HIPCK(c, launchResolveAdaptive(c->fb, ab, (int) pixels, rgb, spp_out, st));
     }
-    HIPCK(c, launchResolve(c->fb, (int) pixels, p->spp, rgb, st));
     HIPCK(c, hipEventRecord(c->ev1, st));
     HIPCK(c, hipStreamSynchronize(st));
     return finishStats(c, tm, p->num_photons > 0 ? (uint64_t) s->ph.stored : 0, stats);

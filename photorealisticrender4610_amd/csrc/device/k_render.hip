@@ -46,7 +46,8 @@ __global__ __launch_bounds__(kShadeBlock) void kRaygen(SceneView s, PathBufs pb,
     const int64_t n = (int64_t) ci.pixels * ci.spc;
     const int64_t stride = (int64_t) gridDim.x * kShadeBlock;
     for (int64_t slot = (int64_t) blockIdx.x * kShadeBlock + threadIdx.x; slot < n; slot += stride) {
-        const int p = (int) (slot % ci.pixels);
+        const int e = (int) (slot % ci.pixels);
+        const int p = ci.pixList ? ci.pixList[e] : e;
         const int sIdx = ci.sampleBase + (int) (slot / ci.pixels);
         Pcg rng = Pcg::make(ci.seed, (uint32_t) p, (uint32_t) sIdx);
         const float ox = rng.next();  // Renderer.cpp:75-76
@@ -543,6 +544,69 @@ __global__ void kAccumulate(PathBufs pb, ChunkInfo ci, float4 *fb) {
     fb[p] = st4(acc, 0.f);
 }
 
+// Welford::update (Renderer.hpp:20-26) on one channel.
+PRR_DEV void welfordUpdate(float x, int n, float &mean, float &m2) {
+    const float delta = x - mean;
+    mean += delta / (float) n;
+    m2 += delta * (x - mean);
+}
+// Welford::ci (Renderer.hpp:33-37): 95% confidence half-width / |mean|.
+PRR_DEV float welfordCi(int n, float mean, float m2) {
+    if (n < 2 || fabsf(mean) < 1e-6f) return 1.f;
+    const float var = m2 / (float) (n - 1);
+    return 1.96f * sqrtf(var / (float) n) / fabsf(mean);
+}
+
+// Renderer.cpp:74-99 with the adaptive stop rule: samples of the chunk are
+// taken in sample order until n >= minSpp and all three channel CIs are below
+// thr; later samples of a stopped pixel are discarded.
+__global__ void kAccumulateAdaptive(PathBufs pb, ChunkInfo ci, float4 *fb, AdaptiveBufs ab, int minSpp, float thr) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= ci.pixels) return;
+    const int p = ci.pixList ? ci.pixList[e] : e;
+    float4 m = ab.mean[p], q = ab.m2[p];
+    if (q.w != 0.f) return;
+    int n = __float_as_int(m.w);
+    V3 acc = ld3(fb[p]);
+    bool stop = false;
+    for (int k = 0; k < ci.spc && !stop; k++) {
+        const V3 sv = ld3(pb.result[(int64_t) k * ci.pixels + e]);
+        acc += sv;
+        n++;
+        welfordUpdate(sv.x, n, m.x, q.x);
+        welfordUpdate(sv.y, n, m.y, q.y);
+        welfordUpdate(sv.z, n, m.z, q.z);
+        if (n >= minSpp)
+            stop = welfordCi(n, m.x, q.x) < thr && welfordCi(n, m.y, q.y) < thr && welfordCi(n, m.z, q.z) < thr;
+    }
+    fb[p] = st4(acc, 0.f);
+    m.w = __int_as_float(n);
+    q.w = stop ? 1.f : 0.f;
+    ab.mean[p] = m;
+    ab.m2[p] = q;
+}
+
+// Active pixels (not stopped, fewer than spp samples) in pixel order within
+// each wave; one atomic per wave.
+__global__ void kCompactActive(AdaptiveBufs ab, int pixels, int spp) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    bool act = false;
+    if (p < pixels) act = ab.m2[p].w == 0.f && __float_as_int(ab.mean[p].w) < spp;
+    const uint32_t idx = waveAppend(ab.count, act);
+    if (act) ab.list[idx] = p;
+}
+
+__global__ void kResolveAdaptive(const float4 *fb, const AdaptiveBufs ab, int pixels, float *rgb, int32_t *sppOut) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= pixels) return;
+    const int n = __float_as_int(ab.mean[p].w);
+    const V3 m = ld3(fb[p]) / (float) n;  // Renderer.cpp:99
+    rgb[3 * p] = m.x;
+    rgb[3 * p + 1] = m.y;
+    rgb[3 * p + 2] = m.z;
+    if (sppOut) sppOut[p] = n;
+}
+
 __global__ void kResolve(const float4 *fb, int pixels, int spp, float *rgb) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= pixels) return;
@@ -585,6 +649,21 @@ hipError_t launchShade(const SceneView &s, const PathBufs &pb, const ChunkInfo &
 }
 hipError_t launchAccumulate(const PathBufs &pb, const ChunkInfo &ci, float4 *fb, hipStream_t st) {
     hipLaunchKernelGGL(kAccumulate, dim3((ci.pixels + 255) / 256), dim3(256), 0, st, pb, ci, fb);
+    return hipGetLastError();
+}
+hipError_t launchAccumulateAdaptive(const PathBufs &pb, const ChunkInfo &ci, float4 *fb, const AdaptiveBufs &ab,
+                                    int minSpp, float thr, hipStream_t st) {
+    hipLaunchKernelGGL(kAccumulateAdaptive, dim3((ci.pixels + 255) / 256), dim3(256), 0, st, pb, ci, fb, ab, minSpp,
+                       thr);
+    return hipGetLastError();
+}
+hipError_t launchCompactActive(const AdaptiveBufs &ab, int pixels, int spp, hipStream_t st) {
+    hipLaunchKernelGGL(kCompactActive, dim3((pixels + 255) / 256), dim3(256), 0, st, ab, pixels, spp);
+    return hipGetLastError();
+}
+hipError_t launchResolveAdaptive(const float4 *fb, const AdaptiveBufs &ab, int pixels, float *rgb, int32_t *sppOut,
+                                 hipStream_t st) {
+    hipLaunchKernelGGL(kResolveAdaptive, dim3((pixels + 255) / 256), dim3(256), 0, st, fb, ab, pixels, rgb, sppOut);
     return hipGetLastError();
 }
 hipError_t launchResolve(const float4 *fb, int pixels, int spp, float *rgb, hipStream_t st) {

@@ -18,6 +18,11 @@ hipError_t launchShadow(const SceneView &s, const PathBufs &pb, int bounce, uint
 hipError_t launchShade(const SceneView &s, const PathBufs &pb, const ChunkInfo &ci, int bounce, int grid,
                        hipStream_t st);
 hipError_t launchAccumulate(const PathBufs &pb, const ChunkInfo &ci, float4 *fb, hipStream_t st);
+hipError_t launchAccumulateAdaptive(const PathBufs &pb, const ChunkInfo &ci, float4 *fb, const AdaptiveBufs &ab,
+                                    int minSpp, float thr, hipStream_t st);
+hipError_t launchCompactActive(const AdaptiveBufs &ab, int pixels, int spp, hipStream_t st);
+hipError_t launchResolveAdaptive(const float4 *fb, const AdaptiveBufs &ab, int pixels, float *rgb, int32_t *sppOut,
+                                 hipStream_t st);
 hipError_t launchResolve(const float4 *fb, int pixels, int spp, float *rgb, hipStream_t st);
 
 }  // namespace prr

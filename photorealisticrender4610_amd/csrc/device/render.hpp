@@ -48,6 +48,19 @@ struct ChunkInfo {
     uint64_t seed;
     int maxDepth;
     float clampInd;      // Integrator::indirectClamp
+    // Pixel of slot entry e = slot % pixels: pixList[e], or e when null
+    // (adaptive passes render only the pixels still sampling).
+    const int32_t *pixList = nullptr;
+};
+
+// Per-pixel Welford state of the adaptive sampler (Renderer.hpp:14-40):
+// mean.xyz / m2.xyz per channel, mean.w = sample count (int bits),
+// m2.w = stopped flag.
+struct AdaptiveBufs {
+    float4 *mean;
+    float4 *m2;
+    int32_t *list;       // active pixel list
+    uint32_t *count;     // active pixel count
 };
 
 }  // namespace prr

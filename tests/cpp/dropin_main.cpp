@@ -1,0 +1,152 @@
+// dropin_main.cpp — a scene script written against the reference's API
+// (main.cpp:11-52 pattern) and linked to libprr.so instead of the CPU
+// renderer. Used by tests/test_dropin_cpp.py.
+//
+//   dropin_main render <outdir> <W> <H> <maxSPP> <minSPP> <threshold> <seed>
+//       box room (Cornell-style, 5 walls, quad light, metal + glass spheres)
+//       through Scene::buildBVH + Renderer::Render; writes the reference's
+//       output files into <outdir>, the linear image to <outdir>/linear.f32
+//       and single-query results (Scene::intersect / intersectP,
+//       Integrator::castRay) to <outdir>/queries.txt
+//   dropin_main hdr <file.hdr> <out.f32>     EnvMap::load; w, h, pixels
+//   dropin_main obj <file.obj> <dx> <dy> <dz> MeshTriangle from OBJ; prints
+//       triangle count, area and bounds
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "prr/prr.hpp"
+
+static std::vector<Triangle> quad(Vector3f p0, Vector3f p1, Vector3f p2, Vector3f p3, Material *m) {
+    return {Triangle(p0, p1, p2, m), Triangle(p0, p2, p3, m)};
+}
+
+static MeshTriangle *mesh(std::vector<Triangle> tris, Material *m) {
+    Bounds3 b;
+    for (auto &t : tris) b = Union(b, t.getBounds());
+    return new MeshTriangle(std::move(tris), b, m);
+}
+
+static Scene buildScene() {
+    Scene scene;
+    scene.backgroundColor = Vector3f(0.f);
+    auto *grey = new DiffuseMaterial();
+    grey->baseColor = Vector3f(0.73f, 0.73f, 0.73f);
+    grey->roughness = 1.0f;
+    auto *red = new DiffuseMaterial();
+    red->baseColor = Vector3f(0.65f, 0.05f, 0.05f);
+    red->roughness = 1.0f;
+    auto *green = new DiffuseMaterial();
+    green->baseColor = Vector3f(0.12f, 0.45f, 0.15f);
+    green->roughness = 1.0f;
+    auto *light = new EmissiveMaterial();
+    light->m_emission = Vector3f(17.f, 12.f, 4.f);
+    auto *metal = new DiffuseMaterial();
+    metal->baseColor = Vector3f(0.95f, 0.64f, 0.54f);
+    metal->roughness = 0.3f;
+    metal->metallic = 1.0f;
+    auto *glass = new GlassMaterial();
+    glass->baseColor = Vector3f(1.f);
+    scene.Add(mesh(quad({-1, 0, -1}, {-1, 0, 1}, {1, 0, 1}, {1, 0, -1}, grey), grey));
+    scene.Add(mesh(quad({-1, 2, -1}, {1, 2, -1}, {1, 2, 1}, {-1, 2, 1}, grey), grey));
+    scene.Add(mesh(quad({-1, 0, -1}, {1, 0, -1}, {1, 2, -1}, {-1, 2, -1}, grey), grey));
+    scene.Add(mesh(quad({-1, 0, -1}, {-1, 2, -1}, {-1, 2, 1}, {-1, 0, 1}, red), red));
+    scene.Add(mesh(quad({1, 0, -1}, {1, 0, 1}, {1, 2, 1}, {1, 2, -1}, green), green));
+    scene.Add(mesh(quad({-0.25f, 1.98f, -0.25f}, {0.25f, 1.98f, -0.25f}, {0.25f, 1.98f, 0.25f},
+                        {-0.25f, 1.98f, 0.25f}, light),
+                   light));
+    scene.Add(new Sphere(Vector3f(-0.45f, 0.4f, -0.3f), 0.4f, metal));
+    scene.Add(new Sphere(Vector3f(0.45f, 0.4f, 0.25f), 0.4f, glass));
+    scene.camera = Camera::create(Vector3f(0, 1, 3.9f), Vector3f(0, 1, 0), Vector3f(0, 1, 0), 40.f, 40.f);
+    return scene;
+}
+
+static int render(int argc, char **argv) {
+    if (argc < 9) return 2;
+    const std::string out = argv[2];
+    RenderSettings settings;
+    settings.width = std::atoi(argv[3]);
+    settings.height = std::atoi(argv[4]);
+    settings.maxSPP = std::atoi(argv[5]);
+    settings.minSPP = std::atoi(argv[6]);
+    settings.varianceThreshold = (float) std::atof(argv[7]);
+    settings.seed = std::strtoull(argv[8], nullptr, 10);
+    settings.maxDepth = 8;
+    settings.exposure = 1.0f;
+    settings.toneMapper = tonemap::fromString("pbrneutral");
+    settings.outputBase = out + "/output";
+    settings.sampleCountBase = out + "/sample_count";
+
+    Scene scene = buildScene();
+    scene.camera.init(settings.width, settings.height);
+    Integrator integrator(scene, settings.maxDepth, settings.russianRoulette);
+    scene.buildBVH();
+    Renderer r;
+    r.Render(scene, integrator, settings);
+
+    FILE *f = fopen((out + "/linear.f32").c_str(), "wb");
+    if (!f) return 1;
+    fwrite(r.lastImage.data(), sizeof(Vector3f), r.lastImage.size(), f);
+    fclose(f);
+
+    // single queries: rays from the camera through a few points
+    FILE *q = fopen((out + "/queries.txt").c_str(), "w");
+    if (!q) return 1;
+    fprintf(q, "emit_area %.9g\n", scene.totalEmitArea);
+    const Vector3f targets[4] = {Vector3f(-0.45f, 0.4f, -0.3f), Vector3f(0.45f, 0.4f, 0.25f),
+                                 Vector3f(0.f, 2.f, 0.f), Vector3f(0.f, 1.f, -1.f)};
+    for (const Vector3f &tg : targets) {
+        Ray ray(scene.camera.eye, normalize(tg - scene.camera.eye));
+        Intersection it = scene.intersect(ray);
+        const bool occl = scene.intersectP(ray, 100.f);
+        const bool occlShort = scene.intersectP(ray, 0.5f);
+        Vector3f L = integrator.castRay(ray);
+        fprintf(q, "hit %d t %.9g n %.9g %.9g %.9g occluded %d %d L %.9g %.9g %.9g\n", it.happened ? 1 : 0,
+                it.tnear, it.normal.x, it.normal.y, it.normal.z, occl ? 1 : 0, occlShort ? 1 : 0, L.x, L.y, L.z);
+    }
+    fclose(q);
+    return 0;
+}
+
+static int hdr(int argc, char **argv) {
+    if (argc < 4) return 2;
+    EnvMap env;
+    if (!env.load(argv[2])) return 1;
+    FILE *f = fopen(argv[3], "wb");
+    if (!f) return 1;
+    const int wh[2] = {env.width, env.height};
+    fwrite(wh, sizeof(int), 2, f);
+    fwrite(env.pixels.data(), sizeof(Vector3f), env.pixels.size(), f);
+    fclose(f);
+    return 0;
+}
+
+static int obj(int argc, char **argv) {
+    if (argc < 6) return 2;
+    DiffuseMaterial m;
+    MeshTriangle mesh(argv[2], Vector3f((float) std::atof(argv[3]), (float) std::atof(argv[4]),
+                                        (float) std::atof(argv[5])),
+                      &m);
+    Bounds3 b = mesh.getBounds();
+    printf("tris %zu area %.9g bounds %.9g %.9g %.9g %.9g %.9g %.9g\n", mesh.triangles.size(), mesh.getArea(),
+           b.pMin.x, b.pMin.y, b.pMin.z, b.pMax.x, b.pMax.y, b.pMax.z);
+    return 0;
+}
+
+int main(int argc, char **argv) {
+    if (argc < 2) {
+        fprintf(stderr, "usage: %s render|hdr|obj ...\n", argv[0]);
+        return 2;
+    }
+    try {
+        if (!std::strcmp(argv[1], "render")) return render(argc, argv);
+        if (!std::strcmp(argv[1], "hdr")) return hdr(argc, argv);
+        if (!std::strcmp(argv[1], "obj")) return obj(argc, argv);
+    } catch (const std::exception &e) {
+        fprintf(stderr, "error: %s\n", e.what());
+        return 1;
+    }
+    return 2;
+}

@@ -3,7 +3,9 @@
 Runs in the development container only: builds oracle/_ref/ref_harness from
 the reference's own sources (oracle/Makefile target ``ref``), feeds it the
 seeded inputs below, and stores inputs + reference outputs together, so the
-tests never need the reference. Usage: python tests/golden/make_golden.py
+tests never need the reference. Usage:
+python tests/golden/make_golden.py [trace bsdf env camera tonemap castray image welford hdr]
+(no names: all).
 """
 import os
 import subprocess
@@ -142,23 +144,137 @@ def gen_image(sc, w, h, depth, spp):
     save("image_%s.npz" % sc.name, res)
 
 
+def welford_sequences(seed=61, nseq=720, ln=64):
+    """Per-sample radiance sequences that reach every branch of the stop rule:
+    constant (zero variance), all-zero and one-zero channels (|mean| < 1e-6),
+    Gaussian noise of several widths, heavy-tailed lognormal, and sparse
+    fireflies; each paired with a min_spp and a threshold."""
+    rng = np.random.default_rng(seed)
+    seq = np.zeros((nseq, ln, 3), np.float32)
+    kinds = nseq // 6
+    for q in range(nseq):
+        k = q // kinds
+        if k == 0:
+            seq[q] = rng.uniform(0.01, 5.0, 3)
+        elif k == 1:
+            seq[q] = 0.0
+            if q % 2:
+                seq[q, :, 0] = rng.uniform(0.5, 1.5, ln)
+                seq[q, :, 1] = rng.uniform(0.5, 1.5, ln)
+        elif k == 2:
+            sd = [0.01, 0.05, 0.2, 1.0][q % 4]
+            seq[q] = rng.normal(1.0, sd, (ln, 3))
+        elif k == 3:
+            seq[q] = rng.lognormal(0.0, [0.3, 0.8, 1.5][q % 3], (ln, 3))
+        elif k == 4:
+            seq[q] = 0.2 + 0.01 * rng.random((ln, 3))
+            hits = rng.random(ln) < 0.05
+            seq[q, hits] *= 500.0
+        else:
+            seq[q] = rng.exponential(1.0, (ln, 3)) * rng.uniform(1e-3, 10.0)
+    min_spp = rng.choice(np.array([0, 1, 2, 4, 16, 63, 64], np.int32), nseq).astype(np.int32)
+    thr = rng.choice(np.array([0.0, 0.01, 0.05, 0.2, 1.5], np.float32), nseq).astype(np.float32)
+    return seq, min_spp, thr
+
+
+def gen_welford():
+    seq, min_spp, thr = welford_sequences()
+    out = run("welford", dict(cfg=np.array(seq.shape[:2], np.int32), seq=seq.ravel(), minspp=min_spp, thr=thr))
+    save("welford.npz", dict(seq=seq, min_spp=min_spp, thr=thr, ref_n=out["n"],
+                             ref_mean=out["mean"].reshape(-1, 3)))
+
+
+def hdr_bytes(rgbe, rle):
+    """Radiance .hdr file bytes for an (H, W, 4) uint8 RGBE image: flat
+    scanlines, or new-style RLE (per channel; runs of >= 4 equal bytes as
+    128+n, the rest as literal groups of <= 128)."""
+    h, w, _ = rgbe.shape
+    out = bytearray(b"#?RADIANCE\n# prr test image\nFORMAT=32-bit_rle_rgbe\n\n")
+    out += b"-Y %d +X %d\n" % (h, w)
+    for y in range(h):
+        row = rgbe[y]
+        if not rle:
+            out += row.tobytes()
+            continue
+        out += bytes([2, 2, w >> 8, w & 255])
+        for ch in range(4):
+            data = row[:, ch].tolist()
+            x = 0
+            lit = []
+            while x < w:
+                n = 1
+                while x + n < w and n < 127 and data[x + n] == data[x]:
+                    n += 1
+                if n >= 4:
+                    while lit:
+                        out += bytes([len(lit[:128])]) + bytes(lit[:128])
+                        lit = lit[128:]
+                    out += bytes([128 + n, data[x]])
+                    x += n
+                else:
+                    lit.append(data[x])
+                    x += 1
+            while lit:
+                out += bytes([len(lit[:128])]) + bytes(lit[:128])
+                lit = lit[128:]
+    return bytes(out)
+
+
+def gen_hdr(seed=71):
+    """EnvMap::load of two small .hdr files (RLE and flat scanlines) through
+    the reference's own loader."""
+    rng = np.random.default_rng(seed)
+    res = {}
+    for name, (h, w), rle in (("rle", (12, 40), True), ("flat", (5, 7), False)):
+        img = rng.integers(0, 256, (h, w, 4), dtype=np.uint8)
+        img[..., 3] = rng.integers(100, 160, (h, w))
+        img[0, :5, 3] = 0                      # zero exponent: black
+        img[1, 3:20] = img[1, 3]               # runs in every channel
+        img[2, :, 3] = 128
+        data = hdr_bytes(img, rle)
+        path = os.path.join(HERE, "env_%s.hdr" % name)
+        with open(path, "wb") as f:
+            f.write(data)
+        out = run("hdr", dict(path=np.frombuffer(path.encode(), np.uint8)))
+        res[name + "_wh"] = out["wh"]
+        res[name + "_ref_pixels"] = out["pixels"].reshape(h, w, 3)
+        res[name + "_rgbe"] = img
+    save("hdr.npz", res)
+
+
 def main():
     if not os.path.isdir("/root/reference"):
         print("reference sources absent; fixtures are committed, nothing to do")
         return
     subprocess.run(["make", "-C", os.path.join(ROOT, "oracle"), "ref"], check=True)
+    only = set(sys.argv[1:])
+
+    def want(name):
+        return not only or name in only
+
     box = S.box_room()
     field = TS.mini_field()
     soup = TS.soup()
-    for sc, seed in ((box, 101), (field, 102), (soup, 103)):
-        gen_trace(sc, 4000, seed)
-    gen_bsdf()
-    gen_env()
-    gen_camera()
-    gen_tonemap()
-    gen_castray(box, 256, 256, 8, 3000, 201)
-    gen_castray(field, 320, 240, 12, 3000, 202)
-    gen_image(box, 64, 64, 8, 256)
+    if want("trace"):
+        for sc, seed in ((box, 101), (field, 102), (soup, 103)):
+            gen_trace(sc, 4000, seed)
+    if want("bsdf"):
+        gen_bsdf()
+    if want("env"):
+        gen_env()
+    if want("camera"):
+        gen_camera()
+    if want("tonemap"):
+        gen_tonemap()
+    if want("castray"):
+        gen_castray(box, 256, 256, 8, 3000, 201)
+        gen_castray(field, 320, 240, 12, 3000, 202)
+    if want("image"):
+        gen_image(box, 64, 64, 8, 256)
+    if want("welford"):
+        gen_welford()
+    if want("hdr"):
+        gen_hdr()
 
 
 if __name__ == "__main__":

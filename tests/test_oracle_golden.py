@@ -207,3 +207,50 @@ def test_castray_matches_reference(name):
     assert exact.mean() > 0.995, exact.mean()
     assert close.mean() > 0.999, close.mean()
     assert (ref.sum(1) > 0).mean() > 0.3
+
+
+def test_welford_stop_rule_matches_reference():
+    """Adaptive sampling (Renderer.cpp:71-99 with Welford, Renderer.hpp:14-40):
+    samples taken and the mean, bit-exact against the reference's own Welford
+    on the same sequences."""
+    g = load_golden("welford.npz")
+    n, mean = O.welford(g["seq"], g["min_spp"], g["thr"])
+    assert np.array_equal(n, g["ref_n"])
+    assert np.array_equal(mean, g["ref_mean"])
+    ln = g["seq"].shape[1]
+    # the fixture reaches the branches that matter: early stops, stops exactly
+    # at min_spp, and sequences that never stop
+    assert (g["ref_n"] < ln).mean() > 0.2 and (g["ref_n"] == ln).mean() > 0.2
+    assert np.any((g["ref_n"] == g["min_spp"]) & (g["min_spp"] >= 2))
+
+
+def test_welford_closed_forms():
+    """Zero variance stops at max(min_spp, 2); |mean| < 1e-6 (and n < 2) gives
+    CI 1, so it stops only for thresholds above 1, at the first sample;
+    threshold <= 0 never stops."""
+    ln = 32
+    const = np.full((4, ln, 3), 0.5, np.float32)
+    n, m = O.welford(const, np.array([0, 1, 2, 9], np.int32), 0.05)
+    assert n.tolist() == [2, 2, 2, 9]
+    assert np.all(m == 0.5)
+    zero = np.zeros((2, ln, 3), np.float32)
+    n, _ = O.welford(zero, 0, np.array([0.5, 1.5], np.float32))
+    assert n.tolist() == [ln, 1]
+    rng = np.random.default_rng(3)
+    noisy = rng.random((3, ln, 3)).astype(np.float32)
+    n, m = O.welford(noisy, 0, 0.0)
+    assert np.all(n == ln)
+    np.testing.assert_allclose(m, noisy.mean(1), rtol=1e-5)
+    # the CI rule itself, in float64 on the stopping prefix: the last sample
+    # is the first whose CIs are all below the threshold
+    seq = rng.lognormal(0.0, 0.5, (200, 64, 3)).astype(np.float32)
+    n, _ = O.welford(seq, 4, 0.2)
+    for q in range(200):
+        def ci(k):
+            x = seq[q, :k].astype(np.float64)
+            return 1.96 * np.sqrt(x.var(0, ddof=1) / k) / np.abs(x.mean(0))
+        k = int(n[q])
+        if k < 64:
+            assert np.all(ci(k) < 0.2 + 1e-4)
+        if k > 4:
+            assert not np.all(ci(k - 1) < 0.2 - 1e-4)
