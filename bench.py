@@ -80,6 +80,20 @@ def traversal_bytes(sc, w, h, depth, seed):
     return per_ray, cnt
 
 
+def profiled_traffic(workload, kernel="kExtend"):
+    """HBM bytes per launch of `kernel` on this workload, measured by
+    rocprofv3 --pmc (FETCH_SIZE and WRITE_SIZE per dispatch, each scaled by
+    what the counter reads for a plain device copy of known size) and kept in
+    the committed summary profiles/pmc_traversal_<workload>.md; None when no
+    summary covers the workload."""
+    import re
+    path = os.path.join(ROOT, "profiles", "pmc_traversal_%s.md" % workload)
+    if not os.path.exists(path):
+        return None
+    m = re.search(r"^%s HBM bytes per launch [^:]*: (\d+)$" % kernel, open(path).read(), re.M)
+    return int(m.group(1)) if m else None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -90,8 +104,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py runs on one GPU only: this build has no multi-GPU path (the render shards by "
-              "pixels/samples with one final reduce; see DESIGN.md). Use --gpus 1.", file=sys.stderr)
+        print("bench.py runs on one GPU only: this build has no multi-GPU path (the render would shard "
+              "by pixels/samples with one final reduce). Use --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np
@@ -146,7 +160,9 @@ def main():
     per_ray, ocnt = traversal_bytes(sc, W, H, DEPTH, seed)
     achieved = (closest * per_ray / 1e9) / (ms_ext / 1e3) if ms_ext > 0 else 0.0
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": profiled_traffic(args.workload) if args.spp == 0 else None,
+                "traffic_unit": "HBM bytes per launch (rocprofv3 --pmc, copy-calibrated)",
                 "kernel": "kExtend (closest-hit traversal)",
                 "bytes_per_ray": round(per_ray, 1),
                 "avg_launch_ms": round(ms_ext / max(launches, 1), 4)}
