@@ -277,6 +277,50 @@ int prr_cast_rays(prr_context* ctx, const prr_scene* scene, int64_t n, const flo
                   const float* dir, uint64_t seed, int32_t max_depth, float indirect_clamp, float* L,
                   prr_stats* stats);
 
+/* ---- learned radiance cache -------------------------------------------- */
+/* No counterpart in the reference (its paths always run to full depth). An
+ * FP16 MLP (Mueller et al. 2021, "Real-time Neural Radiance Caching"):
+ * 96 encoded inputs, 5 hidden layers of 64 (ReLU), 3 outputs, no biases;
+ * MFMA with FP32 accumulation; Adam on FP32 master weights. Trained online on
+ * radiance the tracer produces; paths that use it stop after a short prefix
+ * and add throughput * cache(vertex). */
+#define PRR_CACHE_WEIGHTS 22720 /* 96*64 + 4*64*64 + 64*3 */
+#define PRR_CACHE_QUERY 13      /* floats per query: position3, wo3 (towards the viewer), normal3,
+                                   roughness, albedo3 */
+
+typedef struct prr_cache prr_cache;
+
+typedef struct prr_cache_params {
+    float learning_rate;   /* Adam step size (1e-2) */
+    float beta1, beta2;    /* Adam moment decays (0.9, 0.99) */
+    float epsilon;         /* Adam epsilon (1e-8) */
+    float grad_clip;       /* bound on |dLoss/dy| per output before the FP16 backward pass (100) */
+    float bounds[6];       /* position-encoding box: min xyz, max xyz */
+    uint64_t seed;         /* He-uniform initialisation */
+    int32_t train_batch;   /* samples per Adam step (65536) */
+    int32_t reserved[3];
+} prr_cache_params;
+
+int prr_cache_create(prr_context* ctx, const prr_cache_params* params, prr_cache** out);
+int prr_cache_free(prr_context* ctx, prr_cache* cache);
+/* FP32 master weights, PRR_CACHE_WEIGHTS floats (host): per layer W^T
+ * [out][in] row-major, layers in order. set_weights also resets Adam. */
+int prr_cache_get_weights(prr_context* ctx, const prr_cache* cache, float* weights);
+int prr_cache_set_weights(prr_context* ctx, prr_cache* cache, const float* weights);
+/* Forward pass on n queries (device arrays: query PRR_CACHE_QUERY floats
+ * each, out 3 floats each). */
+int prr_cache_infer(prr_context* ctx, const prr_cache* cache, int64_t n, const float* query, float* out);
+/* Gradient of the mean relative L2 loss sum_c (y_c - t_c)^2 / (y_c^2 + 0.01)
+ * / (3 n) over n samples (device arrays), without an update. grad (host,
+ * PRR_CACHE_WEIGHTS floats) and loss (host) may be NULL. */
+int prr_cache_gradient(prr_context* ctx, prr_cache* cache, int64_t n, const float* query,
+                       const float* target, float* grad, float* loss);
+/* steps Adam steps; step s uses samples [a, min(a + B, n)) with B =
+ * train_batch and a = (s * B) mod n. loss (host, may be NULL): mean loss of
+ * the last step, before its update. */
+int prr_cache_train(prr_context* ctx, prr_cache* cache, int64_t n, const float* query,
+                    const float* target, int32_t steps, float* loss);
+
 #ifdef __cplusplus
 }
 #endif

@@ -8,6 +8,7 @@
 //                      baseline timed beside the GPU.
 #include "prr_oracle.h"
 
+#include "oracle_cache.hpp"
 #include "oracle_scene.hpp"
 
 #include <atomic>
@@ -653,4 +654,32 @@ extern "C" void orc_render(const orc_scene *s, const prr_render_params *p, int r
         counters[5] = cInnerAny;
         counters[6] = cTrisAny;
     }
+}
+
+// ---------------------------------------------------------------------------
+// Radiance-cache network (oracle_cache.hpp)
+// ---------------------------------------------------------------------------
+extern "C" void orc_to_half(int64_t n, const float *in, float *out) {
+    for (int64_t i = 0; i < n; i++) out[i] = orc_cache::toHalf(in[i]);
+}
+extern "C" void orc_cache_encode(int64_t n, const float *q, const float *bounds, float *x) {
+    for (int64_t i = 0; i < n; i++) orc_cache::encode(q + orc_cache::kQ * i, bounds, x + orc_cache::kIn * i);
+}
+extern "C" void orc_cache_forward(int64_t n, const float *q, const float *bounds, const float *w, int half,
+                                  float *y) {
+    const orc_cache::Round R{half != 0};
+    std::vector<float> acts[orc_cache::kLayers];
+    float x0[orc_cache::kIn];
+    for (int64_t i = 0; i < n; i++) {
+        orc_cache::encode(q + orc_cache::kQ * i, bounds, x0);
+        orc_cache::forward(w, x0, R, acts, y + 3 * i);
+    }
+}
+extern "C" double orc_cache_gradient(int64_t n, const float *q, const float *t, const float *bounds, const float *w,
+                                     int half, float clip, float *grad) {
+    return orc_cache::gradient(w, n, q, t, bounds, orc_cache::Round{half != 0}, clip, grad);
+}
+extern "C" void orc_cache_adam(float *w, float *m1, float *m2, const float *grad, int64_t step, float lr, float b1,
+                               float b2, float eps) {
+    orc_cache::adam(w, m1, m2, grad, step, lr, b1, b2, eps);
 }

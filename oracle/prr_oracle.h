@@ -126,6 +126,17 @@ void orc_photon_radiance(const orc_photon_map* pm, int64_t n, const float* pos, 
 
 int orc_version(void);
 
+/* Radiance-cache network (oracle_cache.hpp). half != 0 reproduces the
+ * device's binary16 rounding points; half == 0 is exact float arithmetic. */
+void orc_to_half(int64_t n, const float* in, float* out);
+void orc_cache_encode(int64_t n, const float* query13, const float* bounds6, float* x96);
+void orc_cache_forward(int64_t n, const float* query13, const float* bounds6, const float* weights,
+                       int half, float* y3);
+double orc_cache_gradient(int64_t n, const float* query13, const float* target3, const float* bounds6,
+                          const float* weights, int half, float clip, float* grad);
+void orc_cache_adam(float* w, float* m1, float* m2, const float* grad, int64_t step, float lr,
+                    float beta1, float beta2, float eps);
+
 #ifdef __cplusplus
 }
 #endif

@@ -145,6 +145,12 @@ def lib(fast=False):
     L.orc_castray_pcg.argtypes = [vp, C.c_int64, vp, vp, C.c_uint64, C.c_int, C.c_float, vp]
     L.orc_render.argtypes = [vp, C.POINTER(RenderParams), C.c_int, C.c_int, vp, C.c_int, vp, vp, vp, vp]
     L.orc_welford.argtypes = [C.c_int, C.c_int, vp, vp, vp, vp, vp]
+    L.orc_to_half.argtypes = [C.c_int64, vp, vp]
+    L.orc_cache_encode.argtypes = [C.c_int64, vp, vp, vp]
+    L.orc_cache_forward.argtypes = [C.c_int64, vp, vp, vp, C.c_int, vp]
+    L.orc_cache_gradient.argtypes = [C.c_int64, vp, vp, vp, vp, C.c_int, C.c_float, vp]
+    L.orc_cache_gradient.restype = C.c_double
+    L.orc_cache_adam.argtypes = [vp, vp, vp, vp, C.c_int64, C.c_float, C.c_float, C.c_float, C.c_float]
     L.orc_photon_build.restype = vp
     L.orc_photon_build.argtypes = [vp, C.c_int64, C.c_float, C.c_int, C.c_uint64, C.c_int]
     L.orc_photon_free.argtypes = [vp]
@@ -421,3 +427,41 @@ def welford(seq, min_spp, thr):
     mean = np.zeros((nseq, 3), np.float32)
     lib().orc_welford(nseq, ln, _p(seq), _p(ms), _p(th), _p(cnt), _p(mean))
     return cnt, mean
+
+
+# ---------------------------------------------------------------------------
+# radiance-cache network
+CACHE_WEIGHTS = 22720
+
+
+def to_half(a):
+    a = _c(a)
+    out = np.empty_like(a)
+    lib().orc_to_half(a.size, _p(a), _p(out))
+    return out
+
+
+def cache_encode(q, bounds):
+    q, b = _c(q), _c(bounds)
+    x = np.zeros((q.shape[0], 96), np.float32)
+    lib().orc_cache_encode(q.shape[0], _p(q), _p(b), _p(x))
+    return x
+
+
+def cache_forward(q, bounds, w, half=True):
+    q, b, w = _c(q), _c(bounds), _c(w)
+    y = np.zeros((q.shape[0], 3), np.float32)
+    lib().orc_cache_forward(q.shape[0], _p(q), _p(b), _p(w), int(half), _p(y))
+    return y
+
+
+def cache_gradient(q, t, bounds, w, half=True, clip=100.0):
+    q, t, b, w = _c(q), _c(t), _c(bounds), _c(w)
+    g = np.zeros(CACHE_WEIGHTS, np.float32)
+    loss = lib().orc_cache_gradient(q.shape[0], _p(q), _p(t), _p(b), _p(w), int(half), clip, _p(g))
+    return g, loss
+
+
+def cache_adam(w, m1, m2, grad, step, lr=1e-2, b1=0.9, b2=0.99, eps=1e-8):
+    """One Adam step in place on float32 arrays."""
+    lib().orc_cache_adam(_p(w), _p(m1), _p(m2), _p(_c(grad)), step, lr, b1, b2, eps)

@@ -74,6 +74,15 @@ class SceneInfo(C.Structure):
                 ("device_bytes", C.c_int64)]
 
 
+class CacheParams(C.Structure):
+    _fields_ = [("learning_rate", C.c_float), ("beta1", C.c_float), ("beta2", C.c_float), ("epsilon", C.c_float),
+                ("grad_clip", C.c_float), ("bounds", C.c_float * 6), ("seed", C.c_uint64),
+                ("train_batch", C.c_int32), ("reserved", C.c_int32 * 3)]
+
+
+CACHE_WEIGHTS = 22720
+CACHE_QUERY = 13
+
 _lib = None
 vp = C.c_void_p
 i64 = C.c_int64
@@ -103,6 +112,13 @@ SIGNATURES = {
     "prr_photon_build": [vp, vp, i64, C.c_float, i32, C.c_uint64, C.POINTER(i64)],
     "prr_photon_export": [vp, vp, i64, vp, C.POINTER(i64)],
     "prr_photon_radiance": [vp, vp, i64, vp, vp, vp, vp, vp],
+    "prr_cache_create": [vp, C.POINTER(CacheParams), C.POINTER(vp)],
+    "prr_cache_free": [vp, vp],
+    "prr_cache_get_weights": [vp, vp, vp],
+    "prr_cache_set_weights": [vp, vp, vp],
+    "prr_cache_infer": [vp, vp, i64, vp, vp],
+    "prr_cache_gradient": [vp, vp, i64, vp, vp, vp, C.POINTER(C.c_float)],
+    "prr_cache_train": [vp, vp, i64, vp, vp, i32, C.POINTER(C.c_float)],
 }
 _RESTYPES = {"prr_last_error": C.c_char_p}
 
@@ -220,6 +236,10 @@ class Context:
 
     def scene(self, sc):
         return Scene(self, sc)
+
+    def cache(self, bounds, seed=0xC4, lr=1e-2, beta1=0.9, beta2=0.99, eps=1e-8, grad_clip=100.0,
+              train_batch=65536):
+        return Cache(self, bounds, seed, lr, beta1, beta2, eps, grad_clip, train_batch)
 
     # -- batch entry points on torch tensors -------------------------------
     def _f32(self, a, shape=None):
@@ -417,3 +437,65 @@ class Scene:
         self.ctx._check(lib().prr_scene_export_level(self.h, level, k, vp(b.ctypes.data), vp(fc.ctypes.data),
                                                      vp(meta.ctypes.data), C.byref(n)), self.ctx.h)
         return b, fc, meta
+
+
+class Cache:
+    """The learned radiance cache (prr_cache_*)."""
+
+    def __init__(self, ctx, bounds, seed, lr, beta1, beta2, eps, grad_clip, train_batch):
+        self.ctx = ctx
+        p = CacheParams(learning_rate=lr, beta1=beta1, beta2=beta2, epsilon=eps, grad_clip=grad_clip, seed=seed,
+                        train_batch=train_batch)
+        for i, b in enumerate(bounds):
+            p.bounds[i] = float(b)
+        h = vp()
+        ctx._check(lib().prr_cache_create(ctx.h, C.byref(p), C.byref(h)), ctx.h)
+        self.h = h
+        self.bounds = np.asarray(bounds, np.float32)
+
+    def close(self):
+        if getattr(self, "h", None) and self.ctx.h:
+            lib().prr_cache_free(self.ctx.h, self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def get_weights(self):
+        w = np.zeros(CACHE_WEIGHTS, np.float32)
+        self.ctx._check(lib().prr_cache_get_weights(self.ctx.h, self.h, w.ctypes.data_as(vp)), self.ctx.h)
+        return w
+
+    def set_weights(self, w):
+        w = np.ascontiguousarray(w, np.float32)
+        assert w.size == CACHE_WEIGHTS
+        self.ctx._check(lib().prr_cache_set_weights(self.ctx.h, self.h, w.ctypes.data_as(vp)), self.ctx.h)
+
+    def infer(self, q):
+        c = self.ctx
+        q = c._f32(q)
+        out = c.torch.empty((q.shape[0], 3), dtype=c.torch.float32, device=c.device)
+        c._bind_stream()
+        c._done(lib().prr_cache_infer(c.h, self.h, q.shape[0], _ptr(q), _ptr(out)))
+        return out
+
+    def gradient(self, q, t):
+        c = self.ctx
+        q, t = c._f32(q), c._f32(t)
+        g = np.zeros(CACHE_WEIGHTS, np.float32)
+        loss = C.c_float()
+        c._bind_stream()
+        c._done(lib().prr_cache_gradient(c.h, self.h, q.shape[0], _ptr(q), _ptr(t), g.ctypes.data_as(vp),
+                                         C.byref(loss)))
+        return g, loss.value
+
+    def train(self, q, t, steps):
+        c = self.ctx
+        q, t = c._f32(q), c._f32(t)
+        loss = C.c_float()
+        c._bind_stream()
+        c._done(lib().prr_cache_train(c.h, self.h, q.shape[0], _ptr(q), _ptr(t), steps, C.byref(loss)))
+        return loss.value

@@ -12,51 +12,18 @@
 #include "device/launch.hpp"
 #include "device/launch_photon.hpp"
 #include "device/launch_render.hpp"
+#include "context.hpp"
 #include "host/scene_build.hpp"
 
 using namespace prr;
 
 static thread_local std::string g_lastError = "";
 
-struct prr_context {
-    int device = 0;
-    hipStream_t own = nullptr;
-    hipStream_t stream = nullptr;
-    std::string err;
-    int numCU = 1;
-    int traceBlocks = 1, shadeBlocks = 1;
-    int refillAt = 32;
-    int stackDepth = 8;
-    uint2 *spill = nullptr;
-    int spillLanes = 0;
-    // render buffers
-    PathBufs pb{};
-    int64_t pbSlots = 0;
-    std::vector<void *> pbAllocs;
-    int maxDepthAlloc = 0;
-    float4 *fb = nullptr;
-    int64_t fbPixels = 0;
-    unsigned long long *stats = nullptr;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    std::vector<hipEvent_t> evPool;  // per-launch timing events (PRR_RENDER_KERNEL_TIMING)
-};
-
-struct PhotonMap {
-    std::vector<void *> allocs;
-    int64_t emitted = 0, stored = 0;
-    float radius = 0.f;
-    int maxDepth = 0;
-    uint64_t seed = 0;
-};
-
-struct prr_scene {
-    host::HostScene hs;
-    SceneView view{};
-    std::vector<void *> allocs;
-    int64_t bytes = 0;
-    AgxConst agx{};
-    PhotonMap ph;
-};
+int fail(prr_context *c, int code, const std::string &msg) {
+    if (c) c->err = msg;
+    g_lastError = msg;
+    return code;
+}
 
 static void freePhotons(prr_scene *s) {
     for (void *p : s->ph.allocs) (void) hipFree(p);
@@ -65,18 +32,6 @@ static void freePhotons(prr_scene *s) {
     s->view.ph_pos = s->view.ph_dir = s->view.ph_pow = nullptr;
     s->view.ph_start = s->view.ph_end = nullptr;
 }
-
-static int fail(prr_context *c, int code, const std::string &msg) {
-    if (c) c->err = msg;
-    g_lastError = msg;
-    return code;
-}
-#define HIPCK(ctx, expr)                                                                          \
-    do {                                                                                          \
-        hipError_t e_ = (expr);                                                                   \
-        if (e_ != hipSuccess)                                                                     \
-            return fail(ctx, PRR_ERR_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
-    } while (0)
 
 template <class T>
 static hipError_t upload(prr_scene *s, T **dst, const std::vector<T> &src, hipStream_t st) {
@@ -107,6 +62,167 @@ static AgxConst agxConst() {
     p.w = w;
     return p;
 }
+
+namespace prr_host {
+
+void freeRenderBufs(prr_context *c) {
+    for (void *p : c->pbAllocs) (void) hipFree(p);
+    c->pbAllocs.clear();
+    c->pbSlots = 0;
+    c->maxDepthAlloc = 0;
+}
+
+// ---------------------------------------------------------------------------
+// render
+// ---------------------------------------------------------------------------
+
+int ensureRenderBufs(prr_context *c, int64_t slots, int maxDepth) {
+    if (slots <= c->pbSlots && maxDepth <= c->maxDepthAlloc) return PRR_OK;
+    HIPCK(c, hipStreamSynchronize(c->stream));
+    freeRenderBufs(c);
+    PathBufs &p = c->pb;
+    p = PathBufs{};
+    p.slots = slots;
+    hipError_t e = hipSuccess;
+    auto A = [&](auto **ptr, size_t count) {
+        using T = typename std::remove_reference<decltype(**ptr)>::type;
+        if (e != hipSuccess) return;
+        void *q = nullptr;
+        e = hipMalloc(&q, std::max<size_t>(1, count) * sizeof(T));
+        if (e == hipSuccess) {
+            c->pbAllocs.push_back(q);
+            *ptr = (T *) q;
+        }
+    };
+    A(&p.L, slots);
+    A(&p.beta, slots);
+    A(&p.bscale, slots);
+    A(&p.prevP, slots);
+    A(&p.pend, 4 * slots);
+    A(&p.rng, slots);
+    A(&p.flags, slots);
+    A(&p.vis, 3 * slots);
+    A(&p.result, slots);
+    for (int k = 0; k < 2; k++) {
+        A(&p.qSlot[k], slots);
+        A(&p.qOrg[k], slots);
+        A(&p.qDir[k], slots);
+    }
+    A(&p.hit, slots);
+    A(&p.sqSlot, 3 * slots);
+    A(&p.sqOrg, 3 * slots);
+    A(&p.sqDir, 3 * slots);
+    A(&p.qcnt, (size_t) maxDepth + 2);
+    A(&p.sqcnt, (size_t) maxDepth + 2);
+    A(&p.fetch, 2 * ((size_t) maxDepth + 2));
+    if (e != hipSuccess) {
+        freeRenderBufs(c);
+        return fail(c, PRR_ERR_NOMEM, std::string("render buffers: ") + hipGetErrorString(e));
+    }
+    p.stats = c->stats;
+    c->pbSlots = slots;
+    c->maxDepthAlloc = maxDepth;
+    return PRR_OK;
+}
+
+
+// Runs bounces 0..maxDepth of one chunk whose queue 0 is already filled.
+int runBounces(prr_context *c, const SceneView &view, const ChunkInfo &ci, Timing &tm, hipStream_t st) {
+    auto evPair = [&](int kind) -> int {
+        if (!tm.on) return -1;
+        if (tm.next + 2 > c->evPool.size()) {
+            for (int k = 0; k < 256; k++) {
+                hipEvent_t e;
+                if (hipEventCreate(&e) != hipSuccess) return -1;
+                c->evPool.push_back(e);
+            }
+        }
+        const int i = (int) tm.next;
+        tm.next += 2;
+        tm.timed.emplace_back(kind, i);
+        return i;
+    };
+#define TIMED(kind, launch)                                             \
+    do {                                                                \
+        int ei_ = evPair(kind);                                         \
+        if (ei_ >= 0) HIPCK(c, hipEventRecord(c->evPool[ei_], st));     \
+        HIPCK(c, launch);                                               \
+        if (ei_ >= 0) HIPCK(c, hipEventRecord(c->evPool[ei_ + 1], st)); \
+    } while (0)
+    for (int b = 0; b <= ci.maxDepth; b++) {
+        TIMED(0, launchExtend(view, c->pb, b, c->spill, c->traceBlocks, c->refillAt, c->stackDepth, st));
+        if (b > 0) TIMED(1, launchShadow(view, c->pb, b - 1, c->spill, c->traceBlocks, c->refillAt, c->stackDepth, st));
+        TIMED(2, launchShade(view, c->pb, ci, b, c->shadeBlocks, st));
+    }
+#undef TIMED
+    return PRR_OK;
+}
+
+int resetChunk(prr_context *c, int maxDepth, hipStream_t st) {
+    HIPCK(c, hipMemsetAsync(c->pb.qcnt, 0, ((size_t) maxDepth + 2) * sizeof(uint32_t), st));
+    HIPCK(c, hipMemsetAsync(c->pb.sqcnt, 0, ((size_t) maxDepth + 2) * sizeof(uint32_t), st));
+    HIPCK(c, hipMemsetAsync(c->pb.fetch, 0, 2 * ((size_t) maxDepth + 2) * sizeof(uint32_t), st));
+    return PRR_OK;
+}
+
+int finishStats(prr_context *c, const Timing &tm, uint64_t photons, prr_stats *stats) {
+    if (!stats) return PRR_OK;
+    unsigned long long h[8];
+    HIPCK(c, hipMemcpy(h, c->stats, sizeof(h), hipMemcpyDeviceToHost));
+    float ms = 0.f;
+    HIPCK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    stats->rays_closest = h[0];
+    stats->rays_any = h[1];
+    stats->samples = h[2];
+    stats->photons_stored = photons;
+    stats->ms_total = ms;
+    stats->ms_extend = stats->ms_shadow = stats->ms_shade = 0.0;
+    stats->launches_extend = stats->launches_shadow = 0;
+    for (const auto &t : tm.timed) {
+        float dt = 0.f;
+        HIPCK(c, hipEventElapsedTime(&dt, c->evPool[t.second], c->evPool[t.second + 1]));
+        if (t.first == 0) {
+            stats->ms_extend += dt;
+            stats->launches_extend++;
+        } else if (t.first == 1) {
+            stats->ms_shadow += dt;
+            stats->launches_shadow++;
+        } else {
+            stats->ms_shade += dt;
+        }
+    }
+    return PRR_OK;
+}
+
+// Framebuffer of the render: running sums, the adaptive sampler's Welford
+// state and its active-pixel list, in one allocation sized for `pixels`.
+int ensureFrame(prr_context *c, int64_t pixels, AdaptiveBufs &ab) {
+    if (c->fbPixels < pixels) {
+        HIPCK(c, hipStreamSynchronize(c->stream));
+        if (c->fb) HIPCK(c, hipFree(c->fb));
+        c->fb = nullptr;
+        c->fbPixels = 0;
+        const size_t bytes = (size_t) pixels * (3 * sizeof(float4) + sizeof(int32_t)) + 256;
+        void *q = nullptr;
+        const hipError_t e = hipMalloc(&q, bytes);
+        if (e != hipSuccess) return fail(c, PRR_ERR_NOMEM, std::string("framebuffer: ") + hipGetErrorString(e));
+        c->fb = (float4 *) q;
+        c->fbPixels = pixels;
+    }
+    ab.mean = c->fb + c->fbPixels;
+    ab.m2 = ab.mean + c->fbPixels;
+    ab.list = (int32_t *) (ab.m2 + c->fbPixels);
+    ab.count = (uint32_t *) (ab.list + c->fbPixels);
+    return PRR_OK;
+}
+
+// Paths per adaptive pass: enough to keep every CU busy through the deep
+// bounces; the samples a pixel takes past its stop point are discarded.
+static const int64_t kAdaptivePassSlots = (int64_t) 1 << 22;
+
+}  // namespace prr_host
+
+using namespace prr_host;
 
 extern "C" {
 
@@ -156,13 +272,6 @@ int prr_create(int device, prr_context **out) {
     }
     *out = c;
     return PRR_OK;
-}
-
-static void freeRenderBufs(prr_context *c) {
-    for (void *p : c->pbAllocs) (void) hipFree(p);
-    c->pbAllocs.clear();
-    c->pbSlots = 0;
-    c->maxDepthAlloc = 0;
 }
 
 int prr_destroy(prr_context *c) {
@@ -500,163 +609,6 @@ int prr_photon_radiance(prr_context *c, const prr_scene *s, int64_t n, const flo
     HIPCK(c, launchPhotonRadiance(s->view, n, pos, wo, normal, params, out, c->stream));
     return PRR_OK;
 }
-
-// ---------------------------------------------------------------------------
-// render
-// ---------------------------------------------------------------------------
-// Paths in flight per chunk. Path state is ~340 B per slot, so 2^24 slots
-// hold ~5.7 GB of HBM; large chunks amortise the per-bounce tail of the
-// traversal kernels (a few long rays) over many samples.
-static const int64_t kMaxSlots = (int64_t) 1 << 24;
-
-static int ensureRenderBufs(prr_context *c, int64_t slots, int maxDepth) {
-    if (slots <= c->pbSlots && maxDepth <= c->maxDepthAlloc) return PRR_OK;
-    HIPCK(c, hipStreamSynchronize(c->stream));
-    freeRenderBufs(c);
-    PathBufs &p = c->pb;
-    p = PathBufs{};
-    p.slots = slots;
-    hipError_t e = hipSuccess;
-    auto A = [&](auto **ptr, size_t count) {
-        using T = typename std::remove_reference<decltype(**ptr)>::type;
-        if (e != hipSuccess) return;
-        void *q = nullptr;
-        e = hipMalloc(&q, std::max<size_t>(1, count) * sizeof(T));
-        if (e == hipSuccess) {
-            c->pbAllocs.push_back(q);
-            *ptr = (T *) q;
-        }
-    };
-    A(&p.L, slots);
-    A(&p.beta, slots);
-    A(&p.bscale, slots);
-    A(&p.prevP, slots);
-    A(&p.pend, 4 * slots);
-    A(&p.rng, slots);
-    A(&p.flags, slots);
-    A(&p.vis, 3 * slots);
-    A(&p.result, slots);
-    for (int k = 0; k < 2; k++) {
-        A(&p.qSlot[k], slots);
-        A(&p.qOrg[k], slots);
-        A(&p.qDir[k], slots);
-    }
-    A(&p.hit, slots);
-    A(&p.sqSlot, 3 * slots);
-    A(&p.sqOrg, 3 * slots);
-    A(&p.sqDir, 3 * slots);
-    A(&p.qcnt, (size_t) maxDepth + 2);
-    A(&p.sqcnt, (size_t) maxDepth + 2);
-    A(&p.fetch, 2 * ((size_t) maxDepth + 2));
-    if (e != hipSuccess) {
-        freeRenderBufs(c);
-        return fail(c, PRR_ERR_NOMEM, std::string("render buffers: ") + hipGetErrorString(e));
-    }
-    p.stats = c->stats;
-    c->pbSlots = slots;
-    c->maxDepthAlloc = maxDepth;
-    return PRR_OK;
-}
-
-struct Timing {
-    bool on = false;
-    std::vector<std::pair<int, int>> timed;  // (kind 0 extend / 1 shadow / 2 shade, first event)
-    size_t next = 0;
-};
-
-// Runs bounces 0..maxDepth of one chunk whose queue 0 is already filled.
-static int runBounces(prr_context *c, const SceneView &view, const ChunkInfo &ci, Timing &tm, hipStream_t st) {
-    auto evPair = [&](int kind) -> int {
-        if (!tm.on) return -1;
-        if (tm.next + 2 > c->evPool.size()) {
-            for (int k = 0; k < 256; k++) {
-                hipEvent_t e;
-                if (hipEventCreate(&e) != hipSuccess) return -1;
-                c->evPool.push_back(e);
-            }
-        }
-        const int i = (int) tm.next;
-        tm.next += 2;
-        tm.timed.emplace_back(kind, i);
-        return i;
-    };
-#define TIMED(kind, launch)                                             \
-    do {                                                                \
-        int ei_ = evPair(kind);                                         \
-        if (ei_ >= 0) HIPCK(c, hipEventRecord(c->evPool[ei_], st));     \
-        HIPCK(c, launch);                                               \
-        if (ei_ >= 0) HIPCK(c, hipEventRecord(c->evPool[ei_ + 1], st)); \
-    } while (0)
-    for (int b = 0; b <= ci.maxDepth; b++) {
-        TIMED(0, launchExtend(view, c->pb, b, c->spill, c->traceBlocks, c->refillAt, c->stackDepth, st));
-        if (b > 0) TIMED(1, launchShadow(view, c->pb, b - 1, c->spill, c->traceBlocks, c->refillAt, c->stackDepth, st));
-        TIMED(2, launchShade(view, c->pb, ci, b, c->shadeBlocks, st));
-    }
-#undef TIMED
-    return PRR_OK;
-}
-
-static int resetChunk(prr_context *c, int maxDepth, hipStream_t st) {
-    HIPCK(c, hipMemsetAsync(c->pb.qcnt, 0, ((size_t) maxDepth + 2) * sizeof(uint32_t), st));
-    HIPCK(c, hipMemsetAsync(c->pb.sqcnt, 0, ((size_t) maxDepth + 2) * sizeof(uint32_t), st));
-    HIPCK(c, hipMemsetAsync(c->pb.fetch, 0, 2 * ((size_t) maxDepth + 2) * sizeof(uint32_t), st));
-    return PRR_OK;
-}
-
-static int finishStats(prr_context *c, const Timing &tm, uint64_t photons, prr_stats *stats) {
-    if (!stats) return PRR_OK;
-    unsigned long long h[8];
-    HIPCK(c, hipMemcpy(h, c->stats, sizeof(h), hipMemcpyDeviceToHost));
-    float ms = 0.f;
-    HIPCK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
-    stats->rays_closest = h[0];
-    stats->rays_any = h[1];
-    stats->samples = h[2];
-    stats->photons_stored = photons;
-    stats->ms_total = ms;
-    stats->ms_extend = stats->ms_shadow = stats->ms_shade = 0.0;
-    stats->launches_extend = stats->launches_shadow = 0;
-    for (const auto &t : tm.timed) {
-        float dt = 0.f;
-        HIPCK(c, hipEventElapsedTime(&dt, c->evPool[t.second], c->evPool[t.second + 1]));
-        if (t.first == 0) {
-            stats->ms_extend += dt;
-            stats->launches_extend++;
-        } else if (t.first == 1) {
-            stats->ms_shadow += dt;
-            stats->launches_shadow++;
-        } else {
-            stats->ms_shade += dt;
-        }
-    }
-    return PRR_OK;
-}
-
-// Framebuffer of the render: running sums, the adaptive sampler's Welford
-// state and its active-pixel list, in one allocation sized for `pixels`.
-static int ensureFrame(prr_context *c, int64_t pixels, AdaptiveBufs &ab) {
-    if (c->fbPixels < pixels) {
-        HIPCK(c, hipStreamSynchronize(c->stream));
-        if (c->fb) HIPCK(c, hipFree(c->fb));
-        c->fb = nullptr;
-        c->fbPixels = 0;
-        const size_t bytes = (size_t) pixels * (3 * sizeof(float4) + sizeof(int32_t)) + 256;
-        void *q = nullptr;
-        const hipError_t e = hipMalloc(&q, bytes);
-        if (e != hipSuccess) return fail(c, PRR_ERR_NOMEM, std::string("framebuffer: ") + hipGetErrorString(e));
-        c->fb = (float4 *) q;
-        c->fbPixels = pixels;
-    }
-    ab.mean = c->fb + c->fbPixels;
-    ab.m2 = ab.mean + c->fbPixels;
-    ab.list = (int32_t *) (ab.m2 + c->fbPixels);
-    ab.count = (uint32_t *) (ab.list + c->fbPixels);
-    return PRR_OK;
-}
-
-// Paths per adaptive pass: enough to keep every CU busy through the deep
-// bounces; the samples a pixel takes past its stop point are discarded.
-static const int64_t kAdaptivePassSlots = (int64_t) 1 << 22;
 
 int prr_render(prr_context *c, const prr_scene *s, const prr_render_params *p, float *rgb, int32_t *spp_out,
                prr_stats *stats) {

@@ -1,0 +1,86 @@
+// context.hpp — host-side state behind the C ABI handles (prr_context,
+// prr_scene) and the render-driver helpers shared by abi.hip (plain renders)
+// and cache_abi.hip (radiance-cache renders).
+#pragma once
+
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "../../include/prr_abi.h"
+#include "device/dev_common.hpp"
+#include "device/render.hpp"
+#include "host/scene_build.hpp"
+
+struct prr_context {
+    int device = 0;
+    hipStream_t own = nullptr;
+    hipStream_t stream = nullptr;
+    std::string err;
+    int numCU = 1;
+    int traceBlocks = 1, shadeBlocks = 1;
+    int refillAt = 32;
+    int stackDepth = 8;
+    uint2 *spill = nullptr;
+    int spillLanes = 0;
+    // render buffers
+    prr::PathBufs pb{};
+    int64_t pbSlots = 0;
+    std::vector<void *> pbAllocs;
+    int maxDepthAlloc = 0;
+    float4 *fb = nullptr;
+    int64_t fbPixels = 0;
+    unsigned long long *stats = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::vector<hipEvent_t> evPool;  // per-launch timing events (PRR_RENDER_KERNEL_TIMING)
+};
+
+struct PhotonMap {
+    std::vector<void *> allocs;
+    int64_t emitted = 0, stored = 0;
+    float radius = 0.f;
+    int maxDepth = 0;
+    uint64_t seed = 0;
+};
+
+struct prr_scene {
+    prr::host::HostScene hs;
+    prr::SceneView view{};
+    std::vector<void *> allocs;
+    int64_t bytes = 0;
+    prr::AgxConst agx{};
+    PhotonMap ph;
+};
+
+// Records msg as the context's (and the thread's) last error; returns code.
+int fail(prr_context *c, int code, const std::string &msg);
+
+#define HIPCK(ctx, expr)                                                                          \
+    do {                                                                                          \
+        hipError_t e_ = (expr);                                                                   \
+        if (e_ != hipSuccess)                                                                     \
+            return fail(ctx, PRR_ERR_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+namespace prr_host {
+
+// Paths in flight per chunk. Path state is ~340 B per slot, so 2^24 slots
+// hold ~5.7 GB of HBM; large chunks amortise the per-bounce tail of the
+// traversal kernels (a few long rays) over many samples.
+constexpr int64_t kMaxSlots = (int64_t) 1 << 24;
+
+struct Timing {
+    bool on = false;
+    std::vector<std::pair<int, int>> timed;  // (kind 0 extend / 1 shadow / 2 shade, first event)
+    size_t next = 0;
+};
+
+void freeRenderBufs(prr_context *c);
+int ensureRenderBufs(prr_context *c, int64_t slots, int maxDepth);
+// Runs bounces 0..maxDepth of one chunk whose queue 0 is already filled.
+int runBounces(prr_context *c, const prr::SceneView &view, const prr::ChunkInfo &ci, Timing &tm, hipStream_t st);
+int resetChunk(prr_context *c, int maxDepth, hipStream_t st);
+int finishStats(prr_context *c, const Timing &tm, uint64_t photons, prr_stats *stats);
+int ensureFrame(prr_context *c, int64_t pixels, prr::AdaptiveBufs &ab);
+
+}  // namespace prr_host

@@ -1,0 +1,351 @@
+// k_cache.hip — radiance-cache network kernels.
+//
+//   kCacheInfer  fully fused forward pass: weights in LDS, each wave runs 32
+//                queries at a time through all six layers in registers (MFMA 16x16x16
+//                FP16, FP32 accumulate); writes y or adds beta * max(y, 0) to
+//                the path's result
+//   kCacheTrain  fused forward + backward for 128 queries per block: layer
+//                inputs kept in LDS ([feature][query] FP16), loss gradient of
+//                the relative L2 loss, weight gradients as MFMA products over
+//                the block's queries, written as one partial per block
+//   kCacheAdam   sums the block partials in block order (deterministic), then
+//                Adam, FP32 master weights and the two FP16 layouts
+#include "cache.hpp"
+#include "launch_cache.hpp"
+
+namespace prr {
+
+constexpr int kCacheBlock = 256;
+constexpr int kTrainQ = 128;       // queries per training block (4 waves x 32)
+constexpr int kXsStride = 132;     // halves per LDS row: 128 queries + 4 pad
+constexpr int kXsRows = kCacheIn + 5 * kCacheH;
+PRR_HD constexpr int xsRow(int l) { return l == 0 ? 0 : kCacheIn + (l - 1) * kCacheH; }
+
+PRR_DEV void loadQuery(const float *__restrict__ q, int64_t qi, int64_t n, float *qq) {
+    if (qi < n) {
+#pragma unroll
+        for (int k = 0; k < kCacheQ; k++) qq[k] = q[qi * kCacheQ + k];
+    } else {
+#pragma unroll
+        for (int k = 0; k < kCacheQ; k++) qq[k] = 0.f;
+        qq[4] = 1.f;
+        qq[7] = 1.f;
+    }
+}
+
+PRR_DEV h4 ldh4(const _Float16 *p) { return *(const h4 *) p; }
+
+// One hidden layer (64 outputs) on QB query blocks: x[qb][0..KB) -> x[qb][0..4).
+template <int QB, int KB>
+PRR_DEV void hiddenLayer(const _Float16 *wT, int stride, int g, int c, h4 (&x)[QB][6]) {
+    f4 acc[QB][4];
+#pragma unroll
+    for (int qb = 0; qb < QB; qb++)
+#pragma unroll
+        for (int ob = 0; ob < 4; ob++) acc[qb][ob] = f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int ob = 0; ob < 4; ob++)
+#pragma unroll
+        for (int kb = 0; kb < KB; kb++) {
+            const h4 a = ldh4(wT + (ob * 16 + c) * stride + kb * 16 + 4 * g);
+#pragma unroll
+            for (int qb = 0; qb < QB; qb++) acc[qb][ob] = mfma16(a, x[qb][kb], acc[qb][ob]);
+        }
+#pragma unroll
+    for (int qb = 0; qb < QB; qb++)
+#pragma unroll
+        for (int ob = 0; ob < 4; ob++) x[qb][ob] = reluHalf(acc[qb][ob]);
+}
+
+template <int QB>
+PRR_DEV void outputLayer(const _Float16 *wT, int g, int c, const h4 (&x)[QB][6], f4 (&y)[QB]) {
+#pragma unroll
+    for (int qb = 0; qb < QB; qb++) y[qb] = f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int kb = 0; kb < 4; kb++) {
+        const h4 a = ldh4(wT + c * cacheStrideT(5) + kb * 16 + 4 * g);
+#pragma unroll
+        for (int qb = 0; qb < QB; qb++) y[qb] = mfma16(a, x[qb][kb], y[qb]);
+    }
+}
+
+// ---------------------------------------------------------------------------
+constexpr int kInferQB = 2;  // query blocks of 16 per wave and trip
+
+__global__ __launch_bounds__(kCacheBlock) void kCacheInfer(CacheView cv, int64_t n, const float *__restrict__ q,
+                                                           float *__restrict__ out, const uint32_t *__restrict__ slot,
+                                                           const float4 *__restrict__ beta,
+                                                           float4 *__restrict__ result) {
+    __shared__ __align__(16) _Float16 w[kCacheHalfT];
+    {
+        const uint4 *src = (const uint4 *) cv.wt;
+        uint4 *dst = (uint4 *) w;
+        for (int i = threadIdx.x; i < kCacheHalfT / 8; i += kCacheBlock) dst[i] = src[i];
+    }
+    __syncthreads();
+    const int lane = threadIdx.x & 63, g = lane >> 4, c = lane & 15;
+    const int64_t wave = (int64_t) blockIdx.x * (kCacheBlock / 64) + (threadIdx.x >> 6);
+    const int64_t nWaves = (int64_t) gridDim.x * (kCacheBlock / 64);
+    constexpr int QB = kInferQB;
+    for (int64_t base = wave * 16 * QB; base < n; base += nWaves * 16 * QB) {
+        // compiler-only barrier: weight fragments are re-read from LDS each
+        // trip instead of being hoisted into ~190 registers
+        asm volatile("" ::: "memory");
+        h4 x[QB][6];
+#pragma unroll
+        for (int qb = 0; qb < QB; qb++) {
+            float qq[kCacheQ];
+            loadQuery(q, base + qb * 16 + c, n, qq);
+            const CacheQuery cq = cacheQuery(cv, qq);
+#pragma unroll
+            for (int kb = 0; kb < 6; kb++) x[qb][kb] = cacheInputFrag(cq, kb, g);
+        }
+        hiddenLayer<QB, 6>(w + cacheOffT(0), cacheStrideT(0), g, c, x);
+#pragma unroll
+        for (int l = 1; l < 5; l++) hiddenLayer<QB, 4>(w + cacheOffT(l), cacheStrideT(l), g, c, x);
+        f4 y[QB];
+        outputLayer<QB>(w + cacheOffT(5), g, c, x, y);
+        if (g == 0) {
+#pragma unroll
+            for (int qb = 0; qb < QB; qb++) {
+                const int64_t qi = base + qb * 16 + c;
+                if (qi >= n) continue;
+                if (result) {
+                    const uint32_t s = slot[qi];
+                    const float4 b = beta[qi];
+                    float4 r = result[s];
+                    r.x += b.x * fmaxf(y[qb][0], 0.f);
+                    r.y += b.y * fmaxf(y[qb][1], 0.f);
+                    r.z += b.z * fmaxf(y[qb][2], 0.f);
+                    result[s] = r;
+                } else {
+                    out[3 * qi] = y[qb][0];
+                    out[3 * qi + 1] = y[qb][1];
+                    out[3 * qi + 2] = y[qb][2];
+                }
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Training: layer inputs of the block's 128 queries as [feature][query] FP16.
+template <int KB>
+PRR_DEV void storeX(_Float16 *xs, int l, int g, int c, int wv, const h4 (&x)[2][6]) {
+#pragma unroll
+    for (int qb = 0; qb < 2; qb++)
+#pragma unroll
+        for (int kb = 0; kb < KB; kb++)
+#pragma unroll
+            for (int i = 0; i < 4; i++)
+                xs[(xsRow(l) + kb * 16 + 4 * g + i) * kXsStride + wv * 32 + qb * 16 + c] = x[qb][kb][i];
+}
+
+// Backward step of layer L: d holds dLoss/d(output of layer L) for this
+// wave's queries (D layout). Writes the block's dW of layer L to part and
+// leaves dLoss/d(output of layer L-1) in d.
+template <int L>
+PRR_DEV void backwardLayer(const CacheView &cv, _Float16 *xs, _Float16 *ds, float *part, int g, int c, int wv,
+                           h4 (&d)[2][4]) {
+    constexpr int OB = L == 5 ? 1 : 4;
+    constexpr int IB = L == 0 ? 6 : 4;
+#pragma unroll
+    for (int qb = 0; qb < 2; qb++)
+#pragma unroll
+        for (int ob = 0; ob < OB; ob++)
+#pragma unroll
+            for (int i = 0; i < 4; i++) ds[(ob * 16 + 4 * g + i) * kXsStride + wv * 32 + qb * 16 + c] = d[qb][ob][i];
+    __syncthreads();
+    // dW^T[out][in] = sum over the block's queries of delta[out][q] x[in][q]
+    for (int t = wv; t < OB * IB; t += kCacheBlock / 64) {
+        const int ob = t / IB, ib = t - ob * IB;
+        f4 acc = f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int k = 0; k < kTrainQ / 16; k++) {
+            const h4 a = ldh4(ds + (ob * 16 + c) * kXsStride + k * 16 + 4 * g);
+            const h4 b = ldh4(xs + (xsRow(L) + ib * 16 + c) * kXsStride + k * 16 + 4 * g);
+            acc = mfma16(a, b, acc);
+        }
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const int o = ob * 16 + 4 * g + i;
+            if (o < cacheOut(L)) part[cacheOff(L) + o * cacheIn(L) + ib * 16 + c] = acc[i];
+        }
+    }
+    if constexpr (L > 0) {
+        // delta of the layer's input: W delta, times ReLU' of that input
+        const _Float16 *wn = cv.wn + cacheOffN(L);
+        constexpr int sN = cacheStrideN(L);
+        h4 nd[2][4];
+#pragma unroll
+        for (int ib = 0; ib < 4; ib++) {
+            f4 acc[2] = {f4{0.f, 0.f, 0.f, 0.f}, f4{0.f, 0.f, 0.f, 0.f}};
+#pragma unroll
+            for (int ob = 0; ob < OB; ob++) {
+                const h4 a = ldh4(wn + (ib * 16 + c) * sN + ob * 16 + 4 * g);
+#pragma unroll
+                for (int qb = 0; qb < 2; qb++) acc[qb] = mfma16(a, d[qb][ob], acc[qb]);
+            }
+#pragma unroll
+            for (int qb = 0; qb < 2; qb++)
+#pragma unroll
+                for (int i = 0; i < 4; i++) {
+                    const float xv = (float) xs[(xsRow(L) + ib * 16 + 4 * g + i) * kXsStride + wv * 32 + qb * 16 + c];
+                    nd[qb][ib][i] = (_Float16) (xv > 0.f ? acc[qb][i] : 0.f);
+                }
+        }
+#pragma unroll
+        for (int qb = 0; qb < 2; qb++)
+#pragma unroll
+            for (int ib = 0; ib < 4; ib++) d[qb][ib] = nd[qb][ib];
+    }
+    __syncthreads();
+}
+
+// Relative L2 loss (Mueller et al. 2021): (y - t)^2 / (sg(y)^2 + 0.01) per
+// channel; dL/dy = 2 (y - t) / (y^2 + 0.01), clipped to +-clip (the mean's
+// 1/(3n) is applied to the summed gradient).
+__global__ __launch_bounds__(kCacheBlock) void kCacheTrain(CacheView cv, int64_t n, const float *__restrict__ q,
+                                                           const float *__restrict__ target,
+                                                           float *__restrict__ partial,
+                                                           float *__restrict__ lossPart, float clip) {
+    __shared__ __align__(16) _Float16 xs[kXsRows * kXsStride];
+    __shared__ __align__(16) _Float16 ds[kCacheH * kXsStride];
+    __shared__ float lossW[kCacheBlock / 64];
+    const int lane = threadIdx.x & 63, g = lane >> 4, c = lane & 15;
+    const int wv = threadIdx.x >> 6;
+    const int64_t base = (int64_t) blockIdx.x * kTrainQ + wv * 32;
+    const _Float16 *wt = cv.wt;
+
+    // forward: this wave's 32 queries
+    h4 x[2][6];
+#pragma unroll
+    for (int qb = 0; qb < 2; qb++) {
+        float qq[kCacheQ];
+        loadQuery(q, base + qb * 16 + c, n, qq);
+        const CacheQuery cq = cacheQuery(cv, qq);
+#pragma unroll
+        for (int kb = 0; kb < 6; kb++) x[qb][kb] = cacheInputFrag(cq, kb, g);
+    }
+    storeX<6>(xs, 0, g, c, wv, x);
+    hiddenLayer<2, 6>(wt + cacheOffT(0), cacheStrideT(0), g, c, x);
+#pragma unroll
+    for (int l = 1; l < 5; l++) {
+        storeX<4>(xs, l, g, c, wv, x);
+        hiddenLayer<2, 4>(wt + cacheOffT(l), cacheStrideT(l), g, c, x);
+    }
+    storeX<4>(xs, 5, g, c, wv, x);
+    f4 y[2];
+    outputLayer<2>(wt + cacheOffT(5), g, c, x, y);
+
+    // loss gradient at the outputs (lanes g == 0 hold channels 0..2)
+    h4 d[2][4];
+    float loss = 0.f;
+#pragma unroll
+    for (int qb = 0; qb < 2; qb++) {
+        const int64_t qi = base + qb * 16 + c;
+        f4 dv = f4{0.f, 0.f, 0.f, 0.f};
+        if (g == 0 && qi < n) {
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                const float yv = y[qb][k], t = target[3 * qi + k];
+                const float den = yv * yv + 0.01f;
+                loss += (yv - t) * (yv - t) / den;
+                dv[k] = fminf(clip, fmaxf(-clip, 2.f * (yv - t) / den));
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < 4; i++) d[qb][0][i] = (_Float16) dv[i];
+    }
+    for (int o = 32; o > 0; o >>= 1) loss += __shfl_xor(loss, o);
+    if (lane == 0) lossW[wv] = loss;
+
+    float *part = partial + (int64_t) blockIdx.x * kCacheWeights;
+    backwardLayer<5>(cv, xs, ds, part, g, c, wv, d);
+    backwardLayer<4>(cv, xs, ds, part, g, c, wv, d);
+    backwardLayer<3>(cv, xs, ds, part, g, c, wv, d);
+    backwardLayer<2>(cv, xs, ds, part, g, c, wv, d);
+    backwardLayer<1>(cv, xs, ds, part, g, c, wv, d);
+    backwardLayer<0>(cv, xs, ds, part, g, c, wv, d);
+    if (threadIdx.x == 0) lossPart[blockIdx.x] = lossW[0] + lossW[1] + lossW[2] + lossW[3];
+}
+
+// ---------------------------------------------------------------------------
+// Index of master weight w in the FP16 layouts.
+PRR_DEV void cacheIndex(int w, int &l, int &o, int &i) {
+    l = 0;
+    while (l < kCacheLayers - 1 && w >= cacheOff(l + 1)) l++;
+    const int r = w - cacheOff(l);
+    o = r / cacheIn(l);
+    i = r - o * cacheIn(l);
+}
+
+__global__ void kCacheAdam(float *__restrict__ master, float *__restrict__ m1, float *__restrict__ m2,
+                           const float *__restrict__ partial, int nblocks, CacheOpt opt, _Float16 *__restrict__ wt,
+                           _Float16 *__restrict__ wn, float *__restrict__ gradOut) {
+    const int w = blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= kCacheWeights) return;
+    float wv = master[w];
+    if (partial) {
+        float gsum = 0.f;
+        for (int b = 0; b < nblocks; b++) gsum += partial[(int64_t) b * kCacheWeights + w];
+        const float gr = gsum * opt.gscale;
+        if (gradOut) {
+            gradOut[w] = gr;
+            return;
+        }
+        const float a = opt.beta1 * m1[w] + (1.f - opt.beta1) * gr;
+        const float v = opt.beta2 * m2[w] + (1.f - opt.beta2) * gr * gr;
+        m1[w] = a;
+        m2[w] = v;
+        wv -= opt.lr * (a / opt.bias1) / (sqrtf(v / opt.bias2) + opt.eps);
+        master[w] = wv;
+    }
+    int l, o, i;
+    cacheIndex(w, l, o, i);
+    const _Float16 h = (_Float16) wv;
+    wt[cacheOffT(l) + o * cacheStrideT(l) + i] = h;
+    if (l > 0) wn[cacheOffN(l) + i * cacheStrideN(l) + o] = h;
+}
+
+__global__ void kCacheLossSum(const float *lossPart, int nblocks, float *out) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    float s = 0.f;
+    for (int b = 0; b < nblocks; b++) s += lossPart[b];
+    *out = s;
+}
+
+// ---------------------------------------------------------------------------
+int cacheTrainBlocks(int64_t n) { return (int) ((n + kTrainQ - 1) / kTrainQ); }
+
+hipError_t launchCacheInfer(const CacheView &cv, int64_t n, const float *q, float *out, const uint32_t *slot,
+                            const float4 *beta, float4 *result, int grid, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    const int64_t need = (n + 4 * 16 * kInferQB - 1) / (4 * 16 * kInferQB);
+    const int blocks = (int) (need < grid ? need : grid);
+    hipLaunchKernelGGL(kCacheInfer, dim3(blocks > 0 ? blocks : 1), dim3(kCacheBlock), 0, st, cv, n, q, out, slot, beta,
+                       result);
+    return hipGetLastError();
+}
+
+hipError_t launchCacheTrain(const CacheView &cv, int64_t n, const float *q, const float *target, float *partial,
+                            float *lossPart, float clip, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(kCacheTrain, dim3(cacheTrainBlocks(n)), dim3(kCacheBlock), 0, st, cv, n, q, target, partial,
+                       lossPart, clip);
+    return hipGetLastError();
+}
+
+hipError_t launchCacheAdam(float *master, float *m1, float *m2, const float *partial, int nblocks, const CacheOpt &opt,
+                           _Float16 *wt, _Float16 *wn, float *gradOut, hipStream_t st) {
+    hipLaunchKernelGGL(kCacheAdam, dim3((kCacheWeights + 255) / 256), dim3(256), 0, st, master, m1, m2, partial,
+                       nblocks, opt, wt, wn, gradOut);
+    return hipGetLastError();
+}
+
+hipError_t launchCacheLossSum(const float *lossPart, int nblocks, float *out, hipStream_t st) {
+    hipLaunchKernelGGL(kCacheLossSum, dim3(1), dim3(64), 0, st, lossPart, nblocks, out);
+    return hipGetLastError();
+}
+
+}  // namespace prr

@@ -1,0 +1,163 @@
+"""The radiance-cache oracle (oracle/oracle_cache.hpp) pinned by what the
+mathematics fixes: binary16 rounding against numpy's float16, the encoding's
+closed forms, the analytic gradient against central finite differences of
+the same loss, and Adam's first-step property."""
+import numpy as np
+import pytest
+
+import pyoracle as O
+
+BOUNDS = np.array([-1.0, -2.0, 0.0, 3.0, 2.0, 5.0], np.float32)
+
+
+def _queries(n, seed):
+    rng = np.random.default_rng(seed)
+    q = np.zeros((n, 13), np.float32)
+    q[:, 0:3] = BOUNDS[:3] + rng.random((n, 3)) * (BOUNDS[3:] - BOUNDS[:3])
+    wo = rng.normal(size=(n, 3))
+    nr = rng.normal(size=(n, 3))
+    q[:, 3:6] = wo / np.linalg.norm(wo, axis=1, keepdims=True)
+    q[:, 6:9] = nr / np.linalg.norm(nr, axis=1, keepdims=True)
+    q[:, 9] = rng.random(n)
+    q[:, 10:13] = rng.random((n, 3))
+    return q
+
+
+def _weights(seed, scale=1.0):
+    rng = np.random.default_rng(seed)
+    w = np.zeros(O.CACHE_WEIGHTS, np.float32)
+    off = 0
+    for fan_in, fan_out in [(96, 64)] + [(64, 64)] * 4 + [(64, 3)]:
+        lim = np.sqrt(6.0 / fan_in) * scale
+        w[off:off + fan_in * fan_out] = rng.uniform(-lim, lim, fan_in * fan_out)
+        off += fan_in * fan_out
+    assert off == O.CACHE_WEIGHTS
+    return w
+
+
+def test_to_half_matches_numpy_float16():
+    rng = np.random.default_rng(1)
+    x = np.concatenate([
+        rng.normal(size=20000).astype(np.float32) * 10 ** rng.uniform(-9, 5, 20000).astype(np.float32),
+        np.array([0.0, -0.0, 65504.0, 65519.99, 65520.0, 1e-8, 2.98e-8, 5.96e-8, 6.1e-5, 6.103515625e-05,
+                  1.0 + 2 ** -11, 1.0 + 3 * 2 ** -11, -2049.0, 2051.0], np.float32)])
+    with np.errstate(over="ignore"):
+        want = x.astype(np.float16).astype(np.float32)
+    got = O.to_half(x)
+    assert np.array_equal(got, want)
+
+
+def test_encoding_closed_forms():
+    q = _queries(64, 2)
+    x = O.cache_encode(q, BOUNDS)
+    # the normalised position in float32, as the encoder forms it
+    inv = (np.float32(1.0) / (BOUNDS[3:] - BOUNDS[:3])).astype(np.float32)
+    p = ((q[:, :3] - BOUNDS[:3]) * inv).astype(np.float32).astype(np.float64)
+    for G in range(18):
+        a, k = G % 3, 2 * (G // 3)
+        np.testing.assert_allclose(x[:, 4 * G], np.sin(np.pi * 2.0 ** k * p[:, a]), atol=2e-5)
+        np.testing.assert_allclose(x[:, 4 * G + 1], np.cos(np.pi * 2.0 ** k * p[:, a]), atol=2e-5)
+        # next octave by the double-angle formulas
+        np.testing.assert_allclose(x[:, 4 * G + 2], np.sin(np.pi * 2.0 ** (k + 1) * p[:, a]), atol=1e-4)
+        np.testing.assert_allclose(x[:, 4 * G + 3], np.cos(np.pi * 2.0 ** (k + 1) * p[:, a]), atol=1e-4)
+    theta = np.arccos(np.clip(q[:, 4], -1, 1)) / np.pi
+    blob = x[:, 72:76]
+    centers = (np.arange(4) + 0.5) / 4
+    np.testing.assert_allclose(blob, np.exp(-0.5 * (4 * (theta[:, None] - centers)) ** 2), atol=1e-6)
+    np.testing.assert_allclose(x[:, 88:92], np.exp(-0.5 * (4 * (q[:, 9:10] - centers)) ** 2), atol=1e-6)
+    assert np.array_equal(x[:, 92:95], q[:, 10:13])
+    assert np.all(x[:, 95] == 1.0)
+    # phi: atan2(z, x) mapped to [0, 1]
+    phi = np.arctan2(q[:, 5], q[:, 3]) / (2 * np.pi) + 0.5
+    np.testing.assert_allclose(x[:, 76:80], np.exp(-0.5 * (4 * (phi[:, None] - centers)) ** 2), atol=1e-6)
+
+
+def test_forward_is_a_relu_mlp():
+    """Exact mode equals a numpy float64 MLP on the same encoding."""
+    q = _queries(50, 3)
+    w = _weights(4)
+    y = O.cache_forward(q, BOUNDS, w, half=False)
+    h = O.cache_encode(q, BOUNDS).astype(np.float64)
+    off = 0
+    for l, (fi, fo) in enumerate([(96, 64)] + [(64, 64)] * 4 + [(64, 3)]):
+        W = w[off:off + fi * fo].reshape(fo, fi).astype(np.float64)
+        off += fi * fo
+        h = h @ W.T
+        if l < 5:
+            h = np.maximum(h, 0.0)
+    np.testing.assert_allclose(y, h, rtol=1e-4, atol=1e-5)
+
+
+def _frozen_loss(q, t, w, den):
+    y = O.cache_forward(q, BOUNDS, w, half=False).astype(np.float64)
+    return np.sum((y - t) ** 2 / den) / (3 * len(q))
+
+
+def test_gradient_matches_finite_differences():
+    """Exact-mode gradient = central differences of the relative L2 loss with
+    its denominator held at the base point (the stop-gradient)."""
+    q = _queries(40, 5)
+    w = _weights(6, scale=0.7)
+    y0 = O.cache_forward(q, BOUNDS, w, half=False).astype(np.float64)
+    rng = np.random.default_rng(7)
+    t = (y0 + rng.normal(size=y0.shape) * 0.3).astype(np.float32)
+    den = y0 ** 2 + 0.01
+    g, loss = O.cache_gradient(q, t, BOUNDS, w, half=False, clip=1e9)
+    assert abs(loss - _frozen_loss(q, t, w, den)) <= 1e-6 * max(1.0, loss)
+    offs = [0, 6144, 6144 + 4096, 6144 + 2 * 4096, 6144 + 3 * 4096, 6144 + 4 * 4096]
+    sizes = [6144, 4096, 4096, 4096, 4096, 192]
+    for off, size in zip(offs, sizes):
+        idx = off + rng.choice(size, 6, replace=False)
+        # the largest gradients of the layer too, where the check is sharpest
+        idx = np.concatenate([idx, off + np.argsort(-np.abs(g[off:off + size]))[:4]])
+        for i in idx:
+            h = 1e-2 * max(abs(float(w[i])), 0.05)
+            wp, wm = w.copy(), w.copy()
+            wp[i] += h
+            wm[i] -= h
+            fd = (_frozen_loss(q, t, wp, den) - _frozen_loss(q, t, wm, den)) / (float(wp[i]) - float(wm[i]))
+            assert abs(fd - g[i]) <= 2e-2 * abs(g[i]) + 1e-6, (i, fd, g[i])
+
+
+def test_gradient_clip_bounds_output_delta():
+    """A planted large target: with clip c the output-layer gradient is that
+    of a delta of magnitude c."""
+    q = _queries(1, 8)
+    w = _weights(9)
+    t = np.array([[1e6, 0.0, 0.0]], np.float32)
+    g1, _ = O.cache_gradient(q, t, BOUNDS, w, half=False, clip=10.0)
+    g2, _ = O.cache_gradient(q, t, BOUNDS, w, half=False, clip=20.0)
+    out = slice(O.CACHE_WEIGHTS - 192, O.CACHE_WEIGHTS - 128)  # output row 0
+    nz = np.abs(g1[out]) > 0
+    assert nz.any()
+    np.testing.assert_allclose(g2[out][nz], 2 * g1[out][nz], rtol=1e-6)
+
+
+def test_half_mode_close_to_exact():
+    q = _queries(256, 10)
+    w = _weights(11)
+    ye = O.cache_forward(q, BOUNDS, w, half=False)
+    yh = O.cache_forward(q, BOUNDS, w, half=True)
+    assert np.abs(yh - ye).max() <= 2e-2 * max(1.0, np.abs(ye).max())
+    t = ye + 0.5
+    ge, _ = O.cache_gradient(q, t, BOUNDS, w, half=False)
+    gh, _ = O.cache_gradient(q, t, BOUNDS, w, half=True)
+    assert np.linalg.norm(gh - ge) <= 3e-2 * np.linalg.norm(ge)
+
+
+def test_adam_first_steps():
+    """Bias correction makes the first step lr * sign(g) (|g| >> eps), and a
+    constant gradient keeps that step size."""
+    rng = np.random.default_rng(12)
+    w = rng.normal(size=O.CACHE_WEIGHTS).astype(np.float32)
+    sign = np.where(rng.random(O.CACHE_WEIGHTS) < 0.5, -1.0, 1.0)
+    g = (sign * 10 ** rng.uniform(-4, 1, O.CACHE_WEIGHTS)).astype(np.float32)
+    m1 = np.zeros_like(w)
+    m2 = np.zeros_like(w)
+    w0 = w.copy()
+    O.cache_adam(w, m1, m2, g, 1, lr=1e-2)
+    np.testing.assert_allclose(w0 - w, 1e-2 * np.sign(g), rtol=1e-3)
+    w1 = w.copy()
+    O.cache_adam(w, m1, m2, g, 2, lr=1e-2)
+    np.testing.assert_allclose(w1 - w, 1e-2 * np.sign(g), rtol=1e-3)
+    assert np.allclose(m1, 0.19 * g, rtol=1e-5)
