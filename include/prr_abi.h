@@ -321,6 +321,42 @@ int prr_cache_gradient(prr_context* ctx, prr_cache* cache, int64_t n, const floa
 int prr_cache_train(prr_context* ctx, prr_cache* cache, int64_t n, const float* query,
                     const float* target, int32_t steps, float* loss);
 
+/* One frame of a render that uses the cache. */
+typedef struct prr_cache_render_params {
+    int32_t use_cache;        /* 1: non-training paths stop at their first diffuse vertex at or after
+                                 query_bounce and add throughput * cache; 0: plain paths (the fallback,
+                                 per request) */
+    int32_t train;            /* 1: one path in train_every runs to full depth, its diffuse vertices
+                                 from bounce 1 on become training samples, and Adam runs after the frame */
+    int32_t query_bounce;     /* 2 */
+    int32_t train_every;      /* power of two (16) */
+    int32_t max_train_steps;  /* Adam steps per frame at most (batches of train_batch) */
+    int32_t train_capacity;   /* training samples kept per frame at most (0: 4M) */
+    float divergence_loss;    /* a training loss above this, or not finite, marks the network as
+                                 diverged: the next frame re-initialises it and renders plain */
+    int32_t reserved;
+} prr_cache_render_params;
+
+typedef struct prr_cache_stats {
+    prr_stats render;         /* rays, samples, device ms of the frame (render + cache) */
+    uint64_t queries;         /* paths ended by a cache query */
+    uint64_t train_samples;   /* training samples recorded (before the capacity cap) */
+    int32_t train_steps;      /* Adam steps run */
+    int32_t fell_back;        /* 1: the frame rendered plain because the network had diverged */
+    float train_loss;         /* mean loss of the last step, before its update */
+    int32_t diverged;         /* 1: this frame's training diverged (next frame falls back) */
+    double ms_render;         /* raygen .. accumulate, including cache inference */
+    double ms_train;          /* training targets + Adam steps */
+} prr_cache_stats;
+
+/* Renderer::Render with the cache: fixed spp (params->variance_threshold must
+ * be <= 0), the PCG32 streams of prr_render. Writes the mean linear radiance
+ * (device, width*height*3) and blocks until the frame and its training are
+ * done. stats may be NULL. */
+int prr_cache_render(prr_context* ctx, const prr_scene* scene, prr_cache* cache,
+                     const prr_render_params* params, const prr_cache_render_params* cache_params,
+                     float* rgb_out, prr_cache_stats* stats);
+
 #ifdef __cplusplus
 }
 #endif

@@ -679,6 +679,10 @@ struct Scene {
 struct IntegratorParams {
     int maxDepth = 8;
     float indirectClamp = 10.0f;
+    // > 0: the path ends at its first scattering (diffuse) vertex at bounce >=
+    // truncateAt, before that vertex's NEE (a radiance-cache query path whose
+    // cache contributes nothing)
+    int truncateAt = 0;
 };
 
 using PhotonFn = std::function<V3(const V3 &pos, const V3 &wo, const Shading &sd)>;
@@ -731,6 +735,7 @@ V3 castRay(const Scene &sc, const Ray &ray0, R &rng, const IntegratorParams &ip,
             V3 wo = -cur.d;
             bool indirect = bounce > 0;
             const bool scatters = (mat.type == MAT_DIFFUSE);  // EmissiveMaterial: f = pdf = 0
+            if (scatters && ip.truncateAt > 0 && bounce >= ip.truncateAt) break;
             // NEE area (Integrator.cpp:14-34, 78-87)
             if (sc.totalEmitArea > 0.f) {
                 V3 lp, ln;

@@ -576,9 +576,36 @@ extern "C" void orc_welford(int nseq, int len, const float *seq, const int32_t *
     }
 }
 
+// Radiance-cache training-path selection: lowbias32 mix of pixel and sample.
+static uint32_t trainHash(uint32_t p, uint32_t s) {
+    uint32_t h = p * 0x9E3779B1u + s * 0x85EBCA77u;
+    h ^= h >> 16;
+    h *= 0x7FEB352Du;
+    h ^= h >> 15;
+    h *= 0x846CA68Bu;
+    h ^= h >> 16;
+    return h;
+}
+
+static void renderImpl(const orc_scene *s, const prr_render_params *p, int row0, int row1, const orc_photon_map *pm,
+                       int threads, float *mean, float *var, int32_t *spp_out, uint64_t *counters, int truncateAt,
+                       uint32_t trainMask);
+
 extern "C" void orc_render(const orc_scene *s, const prr_render_params *p, int row0, int row1,
                            const orc_photon_map *pm, int threads, float *mean, float *var, int32_t *spp_out,
                            uint64_t *counters) {
+    renderImpl(s, p, row0, row1, pm, threads, mean, var, spp_out, counters, 0, 0u);
+}
+
+extern "C" void orc_render_truncated(const orc_scene *s, const prr_render_params *p, int row0, int row1,
+                                     int threads, int query_bounce, uint32_t train_mask, float *mean,
+                                     uint64_t *counters) {
+    renderImpl(s, p, row0, row1, nullptr, threads, mean, nullptr, nullptr, counters, query_bounce, train_mask);
+}
+
+static void renderImpl(const orc_scene *s, const prr_render_params *p, int row0, int row1, const orc_photon_map *pm,
+                       int threads, float *mean, float *var, int32_t *spp_out, uint64_t *counters, int truncateAt,
+                       uint32_t trainMask) {
     const Scene &sc = s->sc;
     const int W = p->width, H = p->height;
     if (row1 <= 0 || row1 > H) row1 = H;
@@ -614,7 +641,11 @@ extern "C" void orc_render(const orc_scene *s, const prr_render_params *p, int r
                     float ox = rng.next();
                     float oy = rng.next();
                     Ray r = sc.camera.generateRay(i, j, W, H, ox, oy, rng);
-                    V3 c = castRay(sc, r, rng, ip, &cnt, pfp);
+                    IntegratorParams ipp = ip;
+                    if (truncateAt > 0 &&
+                        (trainHash(pix, (uint32_t) (p->sample_offset + sIdx)) & trainMask) != 0u)
+                        ipp.truncateAt = truncateAt;
+                    V3 c = castRay(sc, r, rng, ipp, &cnt, pfp);
                     if (std::isnan(c.x) || std::isnan(c.y) || std::isnan(c.z)) c = V3(0.f);
                     sum += c;
                     samples++;

@@ -145,6 +145,8 @@ def lib(fast=False):
     L.orc_castray_pcg.argtypes = [vp, C.c_int64, vp, vp, C.c_uint64, C.c_int, C.c_float, vp]
     L.orc_render.argtypes = [vp, C.POINTER(RenderParams), C.c_int, C.c_int, vp, C.c_int, vp, vp, vp, vp]
     L.orc_welford.argtypes = [C.c_int, C.c_int, vp, vp, vp, vp, vp]
+    L.orc_render_truncated.argtypes = [vp, C.POINTER(RenderParams), C.c_int, C.c_int, C.c_int, C.c_int, C.c_uint32,
+                                       vp, vp]
     L.orc_to_half.argtypes = [C.c_int64, vp, vp]
     L.orc_cache_encode.argtypes = [C.c_int64, vp, vp, vp]
     L.orc_cache_forward.argtypes = [C.c_int64, vp, vp, vp, C.c_int, vp]
@@ -277,6 +279,21 @@ class OracleScene:
         if want_spp:
             stats["spp"] = sppo
         return mean, var, stats
+
+
+    def render_truncated(self, width, height, spp, max_depth, seed, query_bounce, train_every, clamp=10.0,
+                         sample_offset=0, threads=0):
+        """The cache render with a zero network: non-training paths end at
+        their first diffuse vertex at bounce >= query_bounce (train_every
+        None: no training paths)."""
+        p = RenderParams(width=width, height=height, spp=spp, max_depth=max_depth, indirect_clamp=clamp,
+                         sample_offset=sample_offset, seed=seed)
+        mean = np.zeros((height, width, 3), np.float32)
+        cnt = np.zeros(7, np.uint64)
+        mask = 0xFFFFFFFF if train_every is None else train_every - 1
+        self.L.orc_render_truncated(self.h, C.byref(p), 0, 0, threads, query_bounce, mask, _p(mean), _p(cnt))
+        keys = ["closest", "any", "samples", "inner", "tris", "inner_any", "tris_any"]
+        return mean, dict(zip(keys, (int(x) for x in cnt)))
 
 
 class OraclePhotonMap:

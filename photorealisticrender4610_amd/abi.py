@@ -80,6 +80,18 @@ class CacheParams(C.Structure):
                 ("train_batch", C.c_int32), ("reserved", C.c_int32 * 3)]
 
 
+class CacheRenderParams(C.Structure):
+    _fields_ = [("use_cache", C.c_int32), ("train", C.c_int32), ("query_bounce", C.c_int32),
+                ("train_every", C.c_int32), ("max_train_steps", C.c_int32), ("train_capacity", C.c_int32),
+                ("divergence_loss", C.c_float), ("reserved", C.c_int32)]
+
+
+class CacheStats(C.Structure):
+    _fields_ = [("render", Stats), ("queries", C.c_uint64), ("train_samples", C.c_uint64),
+                ("train_steps", C.c_int32), ("fell_back", C.c_int32), ("train_loss", C.c_float),
+                ("diverged", C.c_int32), ("ms_render", C.c_double), ("ms_train", C.c_double)]
+
+
 CACHE_WEIGHTS = 22720
 CACHE_QUERY = 13
 
@@ -119,6 +131,8 @@ SIGNATURES = {
     "prr_cache_infer": [vp, vp, i64, vp, vp],
     "prr_cache_gradient": [vp, vp, i64, vp, vp, vp, C.POINTER(C.c_float)],
     "prr_cache_train": [vp, vp, i64, vp, vp, i32, C.POINTER(C.c_float)],
+    "prr_cache_render": [vp, vp, vp, C.POINTER(RenderParams), C.POINTER(CacheRenderParams), vp,
+                         C.POINTER(CacheStats)],
 }
 _RESTYPES = {"prr_last_error": C.c_char_p}
 
@@ -499,3 +513,28 @@ class Cache:
         c._bind_stream()
         c._done(lib().prr_cache_train(c.h, self.h, q.shape[0], _ptr(q), _ptr(t), steps, C.byref(loss)))
         return loss.value
+
+    def render(self, scene, width, height, spp, max_depth, seed=0, clamp=10.0, sample_offset=0, out=None,
+               use_cache=True, train=True, query_bounce=2, train_every=16, max_train_steps=16, train_capacity=0,
+               divergence_loss=1e3, kernel_timing=False):
+        """One frame of Renderer::Render with the cache (prr_cache_render):
+        (H, W, 3) device tensor and a stats dict. Blocks until the frame and
+        its training are done."""
+        c = self.ctx
+        T = c.torch
+        if out is None:
+            out = T.empty((height, width, 3), dtype=T.float32, device=c.device)
+        p = RenderParams(width=width, height=height, spp=spp, max_depth=max_depth, indirect_clamp=clamp,
+                         sample_offset=sample_offset, seed=seed, flags=1 if kernel_timing else 0)
+        cp = CacheRenderParams(use_cache=int(use_cache), train=int(train), query_bounce=query_bounce,
+                               train_every=train_every, max_train_steps=max_train_steps,
+                               train_capacity=train_capacity, divergence_loss=divergence_loss)
+        st = CacheStats()
+        c._bind_stream()
+        c._done(lib().prr_cache_render(c.h, scene.h, self.h, C.byref(p), C.byref(cp), _ptr(out), C.byref(st)))
+        r = st.render
+        return out, {"rays_closest": r.rays_closest, "rays_any": r.rays_any, "samples": r.samples, "ms": r.ms_total,
+                     "ms_extend": r.ms_extend, "ms_shadow": r.ms_shadow, "ms_shade": r.ms_shade,
+                     "queries": st.queries, "train_samples": st.train_samples, "train_steps": st.train_steps,
+                     "fell_back": st.fell_back, "train_loss": st.train_loss, "diverged": st.diverged,
+                     "ms_render": st.ms_render, "ms_train": st.ms_train}

@@ -63,6 +63,7 @@ PRR_HD V3 normalized(const V3 &v) {  // Vector.hpp:28-31
 }
 PRR_HD V3 lerp(const V3 &a, const V3 &b, float t) { return a * (1 - t) + b * t; }
 PRR_HD float max3(const V3 &v) { return smax(smax(v.x, v.y), v.z); }  // std::max({x,y,z})
+PRR_HD float min3(const V3 &v) { return smin(smin(v.x, v.y), v.z); }
 PRR_DEV bool finite3(const V3 &v) { return isfinite(v.x) && isfinite(v.y) && isfinite(v.z); }
 PRR_DEV V3 ld3(const float4 &f) { return V3(f.x, f.y, f.z); }
 PRR_DEV float4 st4(const V3 &v, float w) { return make_float4(v.x, v.y, v.z, w); }

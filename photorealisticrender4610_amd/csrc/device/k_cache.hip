@@ -72,10 +72,15 @@ PRR_DEV void outputLayer(const _Float16 *wT, int g, int c, const h4 (&x)[QB][6],
 // ---------------------------------------------------------------------------
 constexpr int kInferQB = 2;  // query blocks of 16 per wave and trip
 
-__global__ __launch_bounds__(kCacheBlock) void kCacheInfer(CacheView cv, int64_t n, const float *__restrict__ q,
-                                                           float *__restrict__ out, const uint32_t *__restrict__ slot,
+// With result != null the outputs are path contributions: result[slot] +=
+// beta * max(y, 0), rescaled to mean clampInd when brighter (the renderer's
+// indirect clamp); n is then read from nDev when nDev != null (capped by n).
+__global__ __launch_bounds__(kCacheBlock) void kCacheInfer(CacheView cv, int64_t n, const uint32_t *__restrict__ nDev,
+                                                           const float *__restrict__ q, float *__restrict__ out,
+                                                           const uint32_t *__restrict__ slot,
                                                            const float4 *__restrict__ beta,
-                                                           float4 *__restrict__ result) {
+                                                           float4 *__restrict__ result, float clampInd) {
+    if (nDev) n = min(n, (int64_t) *nDev);
     __shared__ __align__(16) _Float16 w[kCacheHalfT];
     {
         const uint4 *src = (const uint4 *) cv.wt;
@@ -113,10 +118,19 @@ __global__ __launch_bounds__(kCacheBlock) void kCacheInfer(CacheView cv, int64_t
                 if (result) {
                     const uint32_t s = slot[qi];
                     const float4 b = beta[qi];
+                    float cx = b.x * fmaxf(y[qb][0], 0.f), cy = b.y * fmaxf(y[qb][1], 0.f),
+                          cz = b.z * fmaxf(y[qb][2], 0.f);
+                    const float lum = (cx + cy + cz) / 3;
+                    if (clampInd > 0.f && lum > clampInd) {
+                        const float k = clampInd / lum;
+                        cx *= k, cy *= k, cz *= k;
+                    }
                     float4 r = result[s];
-                    r.x += b.x * fmaxf(y[qb][0], 0.f);
-                    r.y += b.y * fmaxf(y[qb][1], 0.f);
-                    r.z += b.z * fmaxf(y[qb][2], 0.f);
+                    if (isfinite(cx + cy + cz)) {
+                        r.x += cx;
+                        r.y += cy;
+                        r.z += cz;
+                    }
                     result[s] = r;
                 } else {
                     out[3 * qi] = y[qb][0];
@@ -205,10 +219,14 @@ PRR_DEV void backwardLayer(const CacheView &cv, _Float16 *xs, _Float16 *ds, floa
 // Relative L2 loss (Mueller et al. 2021): (y - t)^2 / (sg(y)^2 + 0.01) per
 // channel; dL/dy = 2 (y - t) / (y^2 + 0.01), clipped to +-clip (the mean's
 // 1/(3n) is applied to the summed gradient).
+// Sample j < n of the batch is record (start + j * mult) mod total of q /
+// target: mult = 1 takes a contiguous slice, a mult coprime to total spreads
+// the batch over the whole record set (mult < 2^32, start < total).
 __global__ __launch_bounds__(kCacheBlock) void kCacheTrain(CacheView cv, int64_t n, const float *__restrict__ q,
                                                            const float *__restrict__ target,
                                                            float *__restrict__ partial,
-                                                           float *__restrict__ lossPart, float clip) {
+                                                           float *__restrict__ lossPart, float clip, int64_t start,
+                                                           int64_t total, int64_t mult) {
     __shared__ __align__(16) _Float16 xs[kXsRows * kXsStride];
     __shared__ __align__(16) _Float16 ds[kCacheH * kXsStride];
     __shared__ float lossW[kCacheBlock / 64];
@@ -217,12 +235,21 @@ __global__ __launch_bounds__(kCacheBlock) void kCacheTrain(CacheView cv, int64_t
     const int64_t base = (int64_t) blockIdx.x * kTrainQ + wv * 32;
     const _Float16 *wt = cv.wt;
 
+    // j < 2^31 and mult < 2^32: the product fits in 64 bits
+    auto rec = [&](int64_t j) { return (int64_t) (((uint64_t) j * (uint64_t) mult + (uint64_t) start) % (uint64_t) total); };
     // forward: this wave's 32 queries
     h4 x[2][6];
 #pragma unroll
     for (int qb = 0; qb < 2; qb++) {
         float qq[kCacheQ];
-        loadQuery(q, base + qb * 16 + c, n, qq);
+        const int64_t j = base + qb * 16 + c;
+        if (j < n) {
+            const int64_t r = rec(j);
+#pragma unroll
+            for (int k = 0; k < kCacheQ; k++) qq[k] = q[r * kCacheQ + k];
+        } else {
+            loadQuery(q, 0, 0, qq);
+        }
         const CacheQuery cq = cacheQuery(cv, qq);
 #pragma unroll
         for (int kb = 0; kb < 6; kb++) x[qb][kb] = cacheInputFrag(cq, kb, g);
@@ -246,9 +273,10 @@ __global__ __launch_bounds__(kCacheBlock) void kCacheTrain(CacheView cv, int64_t
         const int64_t qi = base + qb * 16 + c;
         f4 dv = f4{0.f, 0.f, 0.f, 0.f};
         if (g == 0 && qi < n) {
+            const int64_t r = rec(qi);
 #pragma unroll
             for (int k = 0; k < 3; k++) {
-                const float yv = y[qb][k], t = target[3 * qi + k];
+                const float yv = y[qb][k], t = target[3 * r + k];
                 const float den = yv * yv + 0.01f;
                 loss += (yv - t) * (yv - t) / den;
                 dv[k] = fminf(clip, fmaxf(-clip, 2.f * (yv - t) / den));
@@ -318,21 +346,23 @@ __global__ void kCacheLossSum(const float *lossPart, int nblocks, float *out) {
 // ---------------------------------------------------------------------------
 int cacheTrainBlocks(int64_t n) { return (int) ((n + kTrainQ - 1) / kTrainQ); }
 
-hipError_t launchCacheInfer(const CacheView &cv, int64_t n, const float *q, float *out, const uint32_t *slot,
-                            const float4 *beta, float4 *result, int grid, hipStream_t st) {
+hipError_t launchCacheInfer(const CacheView &cv, int64_t n, const uint32_t *nDev, const float *q, float *out,
+                            const uint32_t *slot, const float4 *beta, float4 *result, float clampInd, int grid,
+                            hipStream_t st) {
     if (n <= 0) return hipSuccess;
     const int64_t need = (n + 4 * 16 * kInferQB - 1) / (4 * 16 * kInferQB);
     const int blocks = (int) (need < grid ? need : grid);
-    hipLaunchKernelGGL(kCacheInfer, dim3(blocks > 0 ? blocks : 1), dim3(kCacheBlock), 0, st, cv, n, q, out, slot, beta,
-                       result);
+    hipLaunchKernelGGL(kCacheInfer, dim3(blocks > 0 ? blocks : 1), dim3(kCacheBlock), 0, st, cv, n, nDev, q, out,
+                       slot, beta, result, clampInd);
     return hipGetLastError();
 }
 
 hipError_t launchCacheTrain(const CacheView &cv, int64_t n, const float *q, const float *target, float *partial,
-                            float *lossPart, float clip, hipStream_t st) {
+                            float *lossPart, float clip, int64_t start, int64_t total, int64_t mult,
+                            hipStream_t st) {
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(kCacheTrain, dim3(cacheTrainBlocks(n)), dim3(kCacheBlock), 0, st, cv, n, q, target, partial,
-                       lossPart, clip);
+                       lossPart, clip, start, total, mult);
     return hipGetLastError();
 }
 

@@ -34,6 +34,27 @@ PRR_DEV uint32_t waveAppend(uint32_t *counter, bool want) {
 
 PRR_DEV uint64_t pcgInc(uint32_t sample) { return ((uint64_t) sample << 1) | 1u; }
 
+// Pixel/sample hash selecting radiance-cache training paths (lowbias32 mix).
+PRR_DEV uint32_t trainHash(uint32_t p, uint32_t s) {
+    uint32_t h = p * 0x9E3779B1u + s * 0x85EBCA77u;
+    h ^= h >> 16;
+    h *= 0x7FEB352Du;
+    h ^= h >> 15;
+    h *= 0x846CA68Bu;
+    h ^= h >> 16;
+    return h;
+}
+
+// One radiance-cache query / training record: position, wo, shading normal,
+// roughness, albedo (the cache's 13 query floats).
+PRR_DEV void putQuery(float *q, const V3 &P, const V3 &wo, const V3 &n, float rough, const V3 &alb) {
+    q[0] = P.x, q[1] = P.y, q[2] = P.z;
+    q[3] = wo.x, q[4] = wo.y, q[5] = wo.z;
+    q[6] = n.x, q[7] = n.y, q[8] = n.z;
+    q[9] = rough;
+    q[10] = alb.x, q[11] = alb.y, q[12] = alb.z;
+}
+
 PRR_DEV V3 clampInd(const V3 &c, float k) {  // Integrator.cpp:105-111
     if (k <= 0.f) return c;
     float lum = (c.x + c.y + c.z) / 3;
@@ -56,7 +77,9 @@ __global__ __launch_bounds__(kShadeBlock) void kRaygen(SceneView s, PathBufs pb,
         cameraRay(s, p % ci.W, p / ci.W, ci.W, ci.H, ox, oy, rng, o, d);
         pb.L[slot] = make_float4(0.f, 0.f, 0.f, 0.f);
         pb.beta[slot] = make_float4(1.f, 1.f, 1.f, 0.f);
-        pb.flags[slot] = 0u;
+        pb.flags[slot] = ci.cache.train && (trainHash((uint32_t) p, (uint32_t) sIdx) & ci.cache.trainMask) == 0u
+                             ? F_TRAIN
+                             : 0u;
         pb.rng[slot] = rng.state;
         pb.qSlot[0][slot] = (uint32_t) slot;
         pb.qOrg[0][slot] = st4(o, 0.f);
@@ -385,9 +408,32 @@ __global__ __launch_bounds__(kShadeBlock) void kShade(SceneView s, PathBufs pb, 
                     } else {
                         fl &= ~F_SPEC;
                         const V3 sN = dot(d, g.Ng) < 0.f ? g.Ng : -g.Ng;
+                        const V3 wo = -d;
+                        const CacheRec &cr = ci.cache;
+                        bool cacheQuery = false;
+                        if (cr.query | cr.train) {
+                            if (cr.query && !(fl & F_TRAIN) && bounce >= cr.queryBounce) {
+                                // the path ends here; the cache adds beta * L_out(P, wo)
+                                cacheQuery = true;
+                                done = true;
+                                const uint32_t k = waveAppend(cr.qCount, true);
+                                putQuery(cr.q + (int64_t) k * kCacheQueryFloats, g.P, wo, sN, m.params.x, base);
+                                cr.qSlot[k] = slot;
+                                cr.qBeta[k] = st4(beta, 0.f);
+                            } else if ((fl & F_TRAIN) && bounce >= 1 && min3(beta) > 1e-3f) {
+                                // target later: (final L - L so far) / beta
+                                const uint32_t k = waveAppend(cr.tCount, true);
+                                if (k < cr.tCap) {
+                                    putQuery(cr.tq + (int64_t) k * kCacheQueryFloats, g.P, wo, sN, m.params.x, base);
+                                    cr.tSlot[k] = slot;
+                                    cr.tBeta[k] = st4(beta, 0.f);
+                                    cr.tLpre[k] = st4(L, 0.f);
+                                }
+                            }
+                        }
+                        if (!cacheQuery) {
                         const Frame fr = makeFrame(sN);
                         const Lobes lb = makeLobes(base, m.params.x, m.params.y, m.params.z);
-                        const V3 wo = -d;
                         const bool ind = bounce > 0;
                         // NEE area (Integrator.cpp:14-34, 78-87)
                         if (s.total_emit_area > 0.f) {
@@ -476,6 +522,7 @@ __global__ __launch_bounds__(kShadeBlock) void kShade(SceneView s, PathBufs pb, 
                             newO = g.P + off * kEps;
                             newD = bs.wi;
                         }
+                        }  // !cacheQuery
                     }
                 }
             }
@@ -533,6 +580,19 @@ __global__ __launch_bounds__(kShadeBlock) void kShade(SceneView s, PathBufs pb, 
             }
         }
     }
+}
+
+// Radiance-cache training targets of records [r0, r1): the radiance the
+// path gathered after the vertex, per unit throughput.
+__global__ void kCacheTargets(PathBufs pb, CacheRec cr, uint32_t r0, uint32_t r1, float *target) {
+    const uint32_t r = r0 + blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= r1) return;
+    const V3 res = ld3(pb.result[cr.tSlot[r]]);
+    const V3 lp = ld3(cr.tLpre[r]);
+    const V3 b = ld3(cr.tBeta[r]);
+    target[3 * (int64_t) r] = (res.x - lp.x) / b.x;
+    target[3 * (int64_t) r + 1] = (res.y - lp.y) / b.y;
+    target[3 * (int64_t) r + 2] = (res.z - lp.z) / b.z;
 }
 
 // Renderer.cpp:84-99: framebuffer[m] += s in sample order.
@@ -664,6 +724,12 @@ hipError_t launchCompactActive(const AdaptiveBufs &ab, int pixels, int spp, hipS
 hipError_t launchResolveAdaptive(const float4 *fb, const AdaptiveBufs &ab, int pixels, float *rgb, int32_t *sppOut,
                                  hipStream_t st) {
     hipLaunchKernelGGL(kResolveAdaptive, dim3((pixels + 255) / 256), dim3(256), 0, st, fb, ab, pixels, rgb, sppOut);
+    return hipGetLastError();
+}
+hipError_t launchCacheTargets(const PathBufs &pb, const CacheRec &cr, uint32_t r0, uint32_t r1, float *target,
+                              hipStream_t st) {
+    if (r1 <= r0) return hipSuccess;
+    hipLaunchKernelGGL(kCacheTargets, dim3((r1 - r0 + 255) / 256), dim3(256), 0, st, pb, cr, r0, r1, target);
     return hipGetLastError();
 }
 hipError_t launchResolve(const float4 *fb, int pixels, int spp, float *rgb, hipStream_t st) {

@@ -14,6 +14,7 @@ constexpr uint32_t F_PEND_E = 8u;        // environment NEE waiting
 constexpr uint32_t F_PEND_P = 16u;       // point-light NEE waiting
 constexpr uint32_t F_TERMINATING = 32u;  // path ended; only pending NEE remains
 constexpr uint32_t F_PEND_C = 64u;       // caustic photon estimate (no shadow ray)
+constexpr uint32_t F_TRAIN = 128u;       // radiance-cache training path: full depth, records its vertices
 constexpr uint32_t F_PEND_ALL = F_PEND_A | F_PEND_E | F_PEND_P | F_PEND_C;
 
 struct PathBufs {
@@ -40,6 +41,29 @@ struct PathBufs {
     unsigned long long *stats;  // [0] closest rays, [1] any rays, [2] samples, [3] photons stored
 };
 
+constexpr int kCacheQueryFloats = 13;  // position, wo, normal, roughness, albedo
+
+// Radiance-cache mode of a render. Training paths (hash of pixel and sample
+// index & trainMask == 0) run to full depth and record each diffuse vertex
+// from bounce 1 on; the other paths stop at their first diffuse vertex at
+// bounce >= queryBounce and record a query there.
+struct CacheRec {
+    int query = 0;               // paths other than training paths stop and query
+    int train = 0;               // training paths are selected and recorded
+    int queryBounce = 2;
+    uint32_t trainMask = 15;
+    uint32_t tCap = 0;           // training-record capacity
+    uint32_t *qCount = nullptr;  // queries: slot, 13 query floats, throughput
+    uint32_t *qSlot = nullptr;
+    float *q = nullptr;
+    float4 *qBeta = nullptr;
+    uint32_t *tCount = nullptr;  // training records: slot, 13 query floats, throughput, radiance so far
+    uint32_t *tSlot = nullptr;
+    float *tq = nullptr;
+    float4 *tBeta = nullptr;
+    float4 *tLpre = nullptr;
+};
+
 struct ChunkInfo {
     int W, H;
     int pixels;          // W * H
@@ -51,6 +75,7 @@ struct ChunkInfo {
     // Pixel of slot entry e = slot % pixels: pixList[e], or e when null
     // (adaptive passes render only the pixels still sampling).
     const int32_t *pixList = nullptr;
+    CacheRec cache;
 };
 
 // Per-pixel Welford state of the adaptive sampler (Renderer.hpp:14-40):
