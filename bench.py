@@ -6,14 +6,21 @@ One step = one full render (Renderer::Render equivalent: every camera,
 extension and shadow ray of all 64 spp) plus the ACES tone map to 8 bits, with
 the scene already resident in HBM. Prints ONE JSON line.
 
-Usage: python bench.py [--gpus 1] [--steps 3] [--warmup 1] [--workload sphere_field|box_room]
-                       [--no-cpu] [--spp N]
+Usage: python bench.py [--gpus 1] [--steps 3] [--warmup 1]
+                       [--workload sphere_field|glass_torus|box_room|radiance_cache]
+                       [--no-cpu] [--spp N] [--ref-spp N] [--frames N]
+
+--workload radiance_cache prints the equal-time relMSE line of the learned
+radiance cache against the plain tracer (60 frames of 4 spp, plain reference
+of --ref-spp samples per pixel).
 """
 import argparse
 import json
 import os
 import sys
 import time
+
+import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
@@ -37,6 +44,13 @@ WORKLOADS = {
                  "box_room: 10-tri box, GGX conductor + glass spheres, quad light (17,12,4), 256x256, 16 spp, "
                  "depth 8, NEE+MIS"),
 }
+# radiance cache on the sphere_field scene: frames of 4 spp, plain vs cache
+CACHE_WORKLOAD = ("sphere_field", 1920, 1080, 4, 12, 60,
+                  "radiance_cache: sphere_field (1,048,578 tris), 1920x1080, 4 spp per frame x 60 frames, depth 12; "
+                  "FP16 MLP 96-64x5-3 trained online (Adam lr 1e-2) on 1/16 full-depth paths, other paths stop at "
+                  "bounce 2 and query; relMSE against a plain reference image")
+MFMA_FP16_PEAK_TFLOPS = 2500.0  # MI355X dense FP16 MFMA
+CACHE_FLOPS_PER_QUERY = 2 * 22720  # one multiply-add per weight
 
 
 def cpu_baseline(sc, w, h, depth, seed, target_s=12.0):
@@ -94,12 +108,110 @@ def profiled_traffic(workload, kernel="kExtend"):
     return int(m.group(1)) if m else None
 
 
+def rel_mse(x, ref):
+    """mean over pixels of sum over channels of (x - ref)^2 / (ref^2 + 1e-2)."""
+    return float((((x - ref) ** 2) / (ref ** 2 + 1e-2)).sum(-1).mean())
+
+
+def cache_bench(args):
+    """Equal-time comparison of the plain tracer and the radiance cache:
+    progressive 4-spp frames of each method, wall time per frame (render +
+    training), relMSE of the running mean after every frame against a plain
+    reference of --ref-spp samples per pixel."""
+    import torch
+    from photorealisticrender4610_amd import abi, scenes
+    fac, W, H, SPF, DEPTH, FRAMES, desc = CACHE_WORKLOAD
+    if args.frames > 0:
+        FRAMES = args.frames
+    seed = 0xC4
+    sc = getattr(scenes, fac)()
+    ctx = abi.Context(0)
+    scene = ctx.scene(sc)
+    bounds = scene.info()["bounds"]
+    # reference: ref_spp samples in chunks of 256 from sample index 1 << 20 on
+    ref = torch.zeros((H, W, 3), dtype=torch.float64, device="cuda")
+    chunk = min(256, args.ref_spp)
+    t0 = time.time()
+    for k in range(0, args.ref_spp, chunk):
+        img, _ = ctx.render(scene, W, H, chunk, DEPTH, seed=seed, sample_offset=(1 << 20) + k)
+        ref += img.double()
+        if (k // chunk) % 8 == 0:
+            log("reference %d/%d spp (%.0f s)" % (k + chunk, args.ref_spp, time.time() - t0))
+    ref = (ref / (args.ref_spp // chunk)).float()
+
+    def run(use_cache):
+        cache = ctx.cache(bounds, seed=seed) if use_cache else None
+        acc = torch.zeros((H, W, 3), dtype=torch.float32, device="cuda")
+        curve = []
+        t = 0.0
+        infer_ms = train_ms = 0.0
+        queries = rays = 0
+        for f in range(FRAMES):
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            if use_cache:
+                img, st = cache.render(scene, W, H, SPF, DEPTH, seed=seed, sample_offset=f * SPF)
+                infer_ms += st["ms_infer"]
+                train_ms += st["ms_train"]
+                queries += st["queries"]
+            else:
+                img, st = ctx.render(scene, W, H, SPF, DEPTH, seed=seed, sample_offset=f * SPF)
+            torch.cuda.synchronize()
+            t += time.perf_counter() - t1
+            rays += st["rays_closest"] + st["rays_any"]
+            acc += img
+            curve.append((t, rel_mse(acc / (f + 1), ref)))
+        log("%s: %d frames in %.3f s, final relMSE %.5g" % ("cache" if use_cache else "plain", FRAMES, t,
+                                                             curve[-1][1]))
+        return curve, {"infer_ms": infer_ms, "train_ms": train_ms, "queries": queries, "rays": rays}
+
+    plain, pst = run(False)
+    cached, cst = run(True)
+    t_plain, e_plain = plain[-1]
+    # time the cache needs to reach the plain tracer's final relMSE
+    t_reach = None
+    for (ta, ea), (tb, eb) in zip([(0.0, float("inf"))] + cached[:-1], cached):
+        if eb <= e_plain:
+            t_reach = ta + (tb - ta) * ((ea - e_plain) / (ea - eb) if np.isfinite(ea) and ea > eb else 1.0)
+            break
+
+    def at(curve, t):  # relMSE of the running mean at time t (last frame done by t)
+        e = None
+        for tt, ee in curve:
+            if tt <= t:
+                e = ee
+        return e
+    t_eq = min(t_plain, cached[-1][0])
+    infer_tflops = (cst["queries"] * CACHE_FLOPS_PER_QUERY / 1e12) / (cst["infer_ms"] / 1e3) if cst["infer_ms"] else 0
+    line = {
+        "metric": "equal-time relMSE (radiance cache vs plain)", "value": at(cached, t_eq), "unit": "relMSE",
+        "n_gpus": 1, "steps": FRAMES, "warmup": 0, "ms_per_step": round(cached[-1][0] * 1e3 / FRAMES, 2),
+        "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "f32 (MLP f16, f32 accumulate)",
+        "data": "synthetic (seeded procedural scene)",
+        "config": {"workload": desc, "width": W, "height": H, "spp_per_frame": SPF, "frames": FRAMES,
+                   "max_depth": DEPTH, "reference_spp": args.ref_spp},
+        "plain_relmse_at_equal_time": at(plain, t_eq), "equal_time_s": round(t_eq, 4),
+        "plain_final": {"relmse": e_plain, "time_s": round(t_plain, 4), "rays": pst["rays"]},
+        "cache_final": {"relmse": cached[-1][1], "time_s": round(cached[-1][0], 4), "rays": cst["rays"],
+                        "queries": cst["queries"], "train_ms": round(cst["train_ms"], 2),
+                        "infer_ms": round(cst["infer_ms"], 2)},
+        "cache_time_to_plain_relmse_ratio": round(t_reach / t_plain, 4) if t_reach is not None else None,
+        "roofline": {"bound": "mfma", "achieved": round(infer_tflops, 2), "peak": MFMA_FP16_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": round(infer_tflops / MFMA_FP16_PEAK_TFLOPS, 5), "traffic": None,
+                     "kernel": "kCacheInfer (fused MLP forward, %d FLOP/query)" % CACHE_FLOPS_PER_QUERY},
+        "cpu_baseline": None,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="sphere_field", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="sphere_field", choices=sorted(WORKLOADS) + ["radiance_cache"])
+    ap.add_argument("--ref-spp", type=int, default=16384, help="radiance_cache: reference samples per pixel")
+    ap.add_argument("--frames", type=int, default=0, help="radiance_cache: override the frame count")
     ap.add_argument("--spp", type=int, default=0, help="override spp (profiling runs only)")
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
@@ -108,7 +220,8 @@ def main():
               "by pixels/samples with one final reduce). Use --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
-    import numpy as np
+    if args.workload == "radiance_cache":
+        return cache_bench(args)
     import torch
     from photorealisticrender4610_amd import abi, scenes
 

@@ -346,7 +346,8 @@ typedef struct prr_cache_stats {
     float train_loss;         /* mean loss of the last step, before its update */
     int32_t diverged;         /* 1: this frame's training diverged (next frame falls back) */
     double ms_render;         /* raygen .. accumulate, including cache inference */
-    double ms_train;          /* training targets + Adam steps */
+    double ms_train;          /* Adam steps (forward + backward + update) */
+    double ms_infer;          /* cache inference launches (part of ms_render) */
 } prr_cache_stats;
 
 /* Renderer::Render with the cache: fixed spp (params->variance_threshold must

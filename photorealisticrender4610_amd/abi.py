@@ -89,7 +89,8 @@ class CacheRenderParams(C.Structure):
 class CacheStats(C.Structure):
     _fields_ = [("render", Stats), ("queries", C.c_uint64), ("train_samples", C.c_uint64),
                 ("train_steps", C.c_int32), ("fell_back", C.c_int32), ("train_loss", C.c_float),
-                ("diverged", C.c_int32), ("ms_render", C.c_double), ("ms_train", C.c_double)]
+                ("diverged", C.c_int32), ("ms_render", C.c_double), ("ms_train", C.c_double),
+                ("ms_infer", C.c_double)]
 
 
 CACHE_WEIGHTS = 22720
@@ -537,4 +538,4 @@ class Cache:
                      "ms_extend": r.ms_extend, "ms_shadow": r.ms_shadow, "ms_shade": r.ms_shade,
                      "queries": st.queries, "train_samples": st.train_samples, "train_steps": st.train_steps,
                      "fell_back": st.fell_back, "train_loss": st.train_loss, "diverged": st.diverged,
-                     "ms_render": st.ms_render, "ms_train": st.ms_train}
+                     "ms_render": st.ms_render, "ms_train": st.ms_train, "ms_infer": st.ms_infer}

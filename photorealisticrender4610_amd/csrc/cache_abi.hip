@@ -32,7 +32,7 @@ struct prr_cache {
     float *q = nullptr, *tq = nullptr, *tTarget = nullptr;
     float4 *qBeta = nullptr, *tBeta = nullptr, *tLpre = nullptr;
     std::vector<void *> frameAllocs;
-    hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+    hipEvent_t ev[3] = {nullptr, nullptr, nullptr};  // render end, inference start / end
     CacheView view() const {
         CacheView v;
         v.wt = wt;
@@ -373,6 +373,7 @@ int prr_cache_render(prr_context *c, const prr_scene *s, prr_cache *k, const prr
     ci.cache = cr;
     uint64_t queries = 0;
     uint32_t tDone = 0, tSeen = 0;
+    double msInfer = 0.0;
     for (int s0 = 0; s0 < p->spp; s0 += spc) {
         ci.pixels = (int) pixels;
         ci.spc = std::min(spc, p->spp - s0);
@@ -381,12 +382,20 @@ int prr_cache_render(prr_context *c, const prr_scene *s, prr_cache *k, const prr
         HIPCK(c, hipMemsetAsync(k->counts, 0, sizeof(uint32_t), st));
         HIPCK(c, launchRaygen(view, c->pb, ci, c->shadeBlocks, st));
         if ((rc = runBounces(c, view, ci, tm, st)) != PRR_OK) return rc;
-        if (useCache)
+        if (useCache) {
+            HIPCK(c, hipEventRecord(k->ev[1], st));
             HIPCK(c, launchCacheInfer(k->view(), pixels * ci.spc, k->counts, k->q, nullptr, k->qSlot, k->qBeta,
                                       c->pb.result, p->indirect_clamp, c->numCU * 3, st));
+            HIPCK(c, hipEventRecord(k->ev[2], st));
+        }
         uint32_t cnt[2] = {0, 0};
         HIPCK(c, hipMemcpyAsync(cnt, k->counts, sizeof(cnt), hipMemcpyDeviceToHost, st));
         HIPCK(c, hipStreamSynchronize(st));
+        if (useCache) {
+            float dt = 0.f;
+            HIPCK(c, hipEventElapsedTime(&dt, k->ev[1], k->ev[2]));
+            msInfer += dt;
+        }
         queries += cnt[0];
         tSeen = cnt[1];
         const uint32_t tNow = (uint32_t) std::min<int64_t>(cnt[1], tCap);
@@ -438,6 +447,7 @@ int prr_cache_render(prr_context *c, const prr_scene *s, prr_cache *k, const prr
         HIPCK(c, hipEventElapsedTime(&b, k->ev[0], c->ev1));
         cs->ms_render = a;
         cs->ms_train = b;
+        cs->ms_infer = msInfer;
         cs->queries = queries;
         cs->train_samples = tSeen;
         cs->train_steps = steps;
