@@ -277,6 +277,70 @@ int prr_cast_rays(prr_context* ctx, const prr_scene* scene, int64_t n, const flo
                   const float* dir, uint64_t seed, int32_t max_depth, float indirect_clamp, float* L,
                   prr_stats* stats);
 
+/* ---- serving: continuous batching of render requests ------------------- */
+/* No counterpart in the reference (Renderer::Render is single-shot,
+ * Renderer.cpp:24). A server keeps scenes resident, admits requests in
+ * arrival order while their memory fits the HBM budget, and runs batches of
+ * up to path_slots paths through the wavefront kernels; every batch packs the
+ * next samples of all admitted requests (one segment per request, oldest
+ * first), so requests that arrive while others render join at the next
+ * batch. A request's image depends only on (scene, camera, size, spp, seed):
+ * each pixel takes samples in order with the PCG32 streams of prr_render, so
+ * it is the same alone and in any mix. The server owns its context while it
+ * runs (its worker thread makes every device call). */
+typedef struct prr_server prr_server;
+
+typedef struct prr_server_params {
+    int64_t path_slots;    /* paths per batch (1 << 22) */
+    int64_t hbm_budget;    /* bytes of admitted request memory (16 B per pixel); 0: half the free HBM */
+    int32_t reserved[4];
+} prr_server_params;
+
+typedef struct prr_request_desc {
+    int32_t scene;         /* index into the server's scenes */
+    int32_t width, height;
+    int32_t spp;
+    prr_camera_desc camera;
+    uint64_t seed;         /* per-pixel PCG32 seed (prr_render_params::seed) */
+} prr_request_desc;
+
+typedef struct prr_request_stats {
+    double t_submit_ms;    /* server clock (ms since prr_server_create) */
+    double t_admit_ms;
+    double t_done_ms;
+    int64_t samples;
+    int32_t batches;       /* batches the request took part in */
+    int32_t reserved;
+} prr_request_stats;
+
+typedef struct prr_server_stats {
+    int64_t requests_done;
+    int64_t batches;
+    int64_t samples;       /* camera samples rendered */
+    int64_t rays;          /* closest + any-hit queries */
+    double busy_ms;        /* wall time the worker spent in batches */
+    int64_t peak_admitted_bytes;
+    int32_t peak_active;   /* most requests admitted at once */
+    int32_t reserved;
+} prr_server_stats;
+
+/* scenes: num_scenes resident scenes created on ctx; max_depth and
+ * indirect_clamp per scene (Integrator parameters). */
+int prr_server_create(prr_context* ctx, const prr_scene* const* scenes, const int32_t* max_depth,
+                      const float* indirect_clamp, int32_t num_scenes, const prr_server_params* params,
+                      prr_server** out);
+/* Stops the worker after the admitted requests finish; frees the server. */
+int prr_server_destroy(prr_server* server);
+/* Thread-safe; returns at once with the request id. A request larger than
+ * the whole budget is rejected (PRR_ERR_NOMEM). */
+int prr_server_submit(prr_server* server, const prr_request_desc* request, int64_t* id);
+/* Blocks until request id is done; copies its mean linear radiance
+ * (width*height*3 floats, HOST memory, may be NULL) and stats, then forgets
+ * the request. */
+int prr_server_wait(prr_server* server, int64_t id, float* rgb, prr_request_stats* stats);
+int prr_server_get_stats(prr_server* server, prr_server_stats* stats);
+const char* prr_server_last_error(const prr_server* server);
+
 /* ---- learned radiance cache -------------------------------------------- */
 /* No counterpart in the reference (its paths always run to full depth). An
  * FP16 MLP (Mueller et al. 2021, "Real-time Neural Radiance Caching"):

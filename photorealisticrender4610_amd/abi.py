@@ -93,6 +93,26 @@ class CacheStats(C.Structure):
                 ("ms_infer", C.c_double)]
 
 
+class ServerParams(C.Structure):
+    _fields_ = [("path_slots", C.c_int64), ("hbm_budget", C.c_int64), ("reserved", C.c_int32 * 4)]
+
+
+class RequestDesc(C.Structure):
+    _fields_ = [("scene", C.c_int32), ("width", C.c_int32), ("height", C.c_int32), ("spp", C.c_int32),
+                ("camera", CameraDesc), ("seed", C.c_uint64)]
+
+
+class RequestStats(C.Structure):
+    _fields_ = [("t_submit_ms", C.c_double), ("t_admit_ms", C.c_double), ("t_done_ms", C.c_double),
+                ("samples", C.c_int64), ("batches", C.c_int32), ("reserved", C.c_int32)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("requests_done", C.c_int64), ("batches", C.c_int64), ("samples", C.c_int64), ("rays", C.c_int64),
+                ("busy_ms", C.c_double), ("peak_admitted_bytes", C.c_int64), ("peak_active", C.c_int32),
+                ("reserved", C.c_int32)]
+
+
 CACHE_WEIGHTS = 22720
 CACHE_QUERY = 13
 
@@ -134,8 +154,15 @@ SIGNATURES = {
     "prr_cache_train": [vp, vp, i64, vp, vp, i32, C.POINTER(C.c_float)],
     "prr_cache_render": [vp, vp, vp, C.POINTER(RenderParams), C.POINTER(CacheRenderParams), vp,
                          C.POINTER(CacheStats)],
+    "prr_server_create": [vp, C.POINTER(vp), C.POINTER(i32), C.POINTER(C.c_float), i32, C.POINTER(ServerParams),
+                          C.POINTER(vp)],
+    "prr_server_destroy": [vp],
+    "prr_server_submit": [vp, C.POINTER(RequestDesc), C.POINTER(i64)],
+    "prr_server_wait": [vp, i64, vp, C.POINTER(RequestStats)],
+    "prr_server_get_stats": [vp, C.POINTER(ServerStats)],
+    "prr_server_last_error": [vp],
 }
-_RESTYPES = {"prr_last_error": C.c_char_p}
+_RESTYPES = {"prr_last_error": C.c_char_p, "prr_server_last_error": C.c_char_p}
 
 
 def lib():
@@ -251,6 +278,9 @@ class Context:
 
     def scene(self, sc):
         return Scene(self, sc)
+
+    def server(self, scenes, depths, clamps=None, path_slots=1 << 22, hbm_budget=0):
+        return Server(self, scenes, depths, clamps, path_slots, hbm_budget)
 
     def cache(self, bounds, seed=0xC4, lr=1e-2, beta1=0.9, beta2=0.99, eps=1e-8, grad_clip=100.0,
               train_batch=65536):
@@ -539,3 +569,66 @@ class Cache:
                      "queries": st.queries, "train_samples": st.train_samples, "train_steps": st.train_steps,
                      "fell_back": st.fell_back, "train_loss": st.train_loss, "diverged": st.diverged,
                      "ms_render": st.ms_render, "ms_train": st.ms_train, "ms_infer": st.ms_infer}
+
+
+def camera_desc(eye, target, up=(0.0, 1.0, 0.0), fovh=40.0, aperture=0.0, focus_distance=-1.0):
+    d = CameraDesc()
+    for k in range(3):
+        d.eye[k], d.target[k], d.up[k] = float(eye[k]), float(target[k]), float(up[k])
+    d.fovh, d.aperture, d.focus_distance = float(fovh), float(aperture), float(focus_distance)
+    return d
+
+
+class Server:
+    """Continuous-batching render server (prr_server_*). The server's worker
+    thread owns the context until close(); submit() and wait() may be called
+    from any thread."""
+
+    def __init__(self, ctx, scenes, depths, clamps, path_slots, hbm_budget):
+        self.ctx = ctx
+        n = len(scenes)
+        self._scenes = list(scenes)  # keep the scenes alive
+        arr = (vp * n)(*[s.h for s in scenes])
+        dep = (i32 * n)(*depths)
+        cl = (C.c_float * n)(*(clamps if clamps is not None else [10.0] * n))
+        p = ServerParams(path_slots=path_slots, hbm_budget=hbm_budget)
+        h = vp()
+        ctx._check(lib().prr_server_create(ctx.h, arr, dep, cl, n, C.byref(p), C.byref(h)), ctx.h)
+        self.h = h
+        self.sizes = {}
+
+    def _check(self, rc):
+        if rc != PRR_OK:
+            msg = lib().prr_server_last_error(self.h)
+            raise PrrError("prr server error %d: %s" % (rc, msg.decode() if msg else ""))
+
+    def submit(self, scene, width, height, spp, camera, seed=0):
+        d = RequestDesc(scene=scene, width=width, height=height, spp=spp, camera=camera, seed=seed)
+        rid = i64()
+        self._check(lib().prr_server_submit(self.h, C.byref(d), C.byref(rid)))
+        self.sizes[rid.value] = (height, width)
+        return rid.value
+
+    def wait(self, rid, image=True):
+        h, w = self.sizes.pop(rid)
+        out = np.zeros((h, w, 3), np.float32) if image else None
+        st = RequestStats()
+        self._check(lib().prr_server_wait(self.h, rid, out.ctypes.data_as(vp) if image else None, C.byref(st)))
+        return out, {"t_submit_ms": st.t_submit_ms, "t_admit_ms": st.t_admit_ms, "t_done_ms": st.t_done_ms,
+                     "samples": st.samples, "batches": st.batches}
+
+    def stats(self):
+        st = ServerStats()
+        self._check(lib().prr_server_get_stats(self.h, C.byref(st)))
+        return {k: getattr(st, k) for k, _ in ServerStats._fields_ if k != "reserved"}
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().prr_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

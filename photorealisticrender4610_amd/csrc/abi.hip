@@ -350,15 +350,9 @@ int prr_scene_create(prr_context *c, const prr_scene_desc *d, prr_scene **out) {
         v.background[k] = h.background[k];
         v.point_pos[k] = h.pointPos[k];
         v.point_int[k] = h.pointInt[k];
-        v.cam_eye[k] = h.camEye[k];
-        v.cam_fwd[k] = h.camFwd[k];
-        v.cam_right[k] = h.camRight[k];
-        v.cam_up[k] = h.camUp[k];
     }
     v.num_points = h.numPoints;
-    v.cam_aperture = h.camAperture;
-    v.cam_focus = h.camFocus;
-    v.cam_tanhalf = h.camTanHalf;
+    v.cam = h.cam;
     s->agx = agxConst();
     *out = s;
     return PRR_OK;

@@ -147,6 +147,13 @@ struct __align__(16) DLightNode {
     int pad;
 };
 
+// Camera (Camera.hpp:43-56): basis, aperture, focus distance and
+// tan(fovh / 2), precomputed on the host.
+struct CamView {
+    float eye[3], fwd[3], right[3], up[3];
+    float aperture, focus, tanhalf;
+};
+
 struct SceneView {
     const DNode *nodes;
     const float4 *tris;       // 3 float4 per leaf slot: v0.xyz e1.x | e1.yz e2.xy | e2.z id pad pad
@@ -183,9 +190,7 @@ struct SceneView {
     uint32_t ph_count;
     uint32_t ph_mask;           // buckets - 1 (power of two)
     float ph_radius, ph_cell;
-    // camera (Camera.hpp:43-56), focal length precomputed on the host
-    float cam_eye[3], cam_fwd[3], cam_right[3], cam_up[3];
-    float cam_aperture, cam_focus, cam_tanhalf;
+    CamView cam;  // the scene's camera
 };
 
 // ---------------------------------------------------------------------------
@@ -486,26 +491,26 @@ PRR_DEV V3 sampleDisc(float u, float v) {
     return V3(r * cosf(phi), r * sinf(phi), 0.0f);
 }
 template <class R>
-PRR_DEV void cameraRay(const SceneView &s, int i, int j, int W, int H, float ox, float oy, R &rng, V3 &org,
+PRR_DEV void cameraRay(const CamView &c, int i, int j, int W, int H, float ox, float oy, R &rng, V3 &org,
                        V3 &dir) {
-    float fpx = ((float) W * 0.5f) / s.cam_tanhalf;
+    float fpx = ((float) W * 0.5f) / c.tanhalf;
     float x = ((float) i + ox - (float) W * 0.5f) / fpx;
     float y = ((float) (H - j) + oy - (float) H * 0.5f) / fpx;
-    V3 fwd(s.cam_fwd[0], s.cam_fwd[1], s.cam_fwd[2]);
-    V3 right(s.cam_right[0], s.cam_right[1], s.cam_right[2]);
-    V3 up(s.cam_up[0], s.cam_up[1], s.cam_up[2]);
-    V3 eye(s.cam_eye[0], s.cam_eye[1], s.cam_eye[2]);
+    V3 fwd(c.fwd[0], c.fwd[1], c.fwd[2]);
+    V3 right(c.right[0], c.right[1], c.right[2]);
+    V3 up(c.up[0], c.up[1], c.up[2]);
+    V3 eye(c.eye[0], c.eye[1], c.eye[2]);
     V3 d = normalize(fwd + x * right + y * up);
-    if (s.cam_aperture <= 0.0f) {
+    if (c.aperture <= 0.0f) {
         org = eye;
         dir = d;
         return;
     }
-    V3 focus = eye + d * (s.cam_focus / dot(d, fwd));
+    V3 focus = eye + d * (c.focus / dot(d, fwd));
     float u = rng.next();
     float v = rng.next();
     V3 lo = sampleDisc(u, v);
-    V3 lens = eye + s.cam_aperture * (lo.x * right + lo.y * up);
+    V3 lens = eye + c.aperture * (lo.x * right + lo.y * up);
     org = lens;
     dir = normalize(focus - lens);
 }

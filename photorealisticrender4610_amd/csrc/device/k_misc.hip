@@ -83,10 +83,10 @@ __global__ void kCameraRays(SceneView s, int W, int H, int64_t n, const int32_t 
                             float *dir) {
     const int64_t i = (int64_t) blockIdx.x * kBlk + threadIdx.x;
     if (i >= n) return;
-    const int stride = s.cam_aperture > 0.f ? 4 : 2;
+    const int stride = s.cam.aperture > 0.f ? 4 : 2;
     FixedU lens{offs + stride * i + 2, 0};
     V3 o, d;
-    cameraRay(s, pix[2 * i], pix[2 * i + 1], W, H, offs[stride * i], offs[stride * i + 1], lens, o, d);
+    cameraRay(s.cam, pix[2 * i], pix[2 * i + 1], W, H, offs[stride * i], offs[stride * i + 1], lens, o, d);
     stv(org, i, o);
     stv(dir, i, d);
 }

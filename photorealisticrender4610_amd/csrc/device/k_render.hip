@@ -74,7 +74,7 @@ __global__ __launch_bounds__(kShadeBlock) void kRaygen(SceneView s, PathBufs pb,
         const float ox = rng.next();  // Renderer.cpp:75-76
         const float oy = rng.next();
         V3 o, d;
-        cameraRay(s, p % ci.W, p / ci.W, ci.W, ci.H, ox, oy, rng, o, d);
+        cameraRay(s.cam, p % ci.W, p / ci.W, ci.W, ci.H, ox, oy, rng, o, d);
         pb.L[slot] = make_float4(0.f, 0.f, 0.f, 0.f);
         pb.beta[slot] = make_float4(1.f, 1.f, 1.f, 0.f);
         pb.flags[slot] = ci.cache.train && (trainHash((uint32_t) p, (uint32_t) sIdx) & ci.cache.trainMask) == 0u
@@ -90,6 +90,73 @@ __global__ __launch_bounds__(kShadeBlock) void kRaygen(SceneView s, PathBufs pb,
         atomicAdd(&pb.stats[0], (unsigned long long) n);
         atomicAdd(&pb.stats[2], (unsigned long long) n);
     }
+}
+
+// ---------------------------------------------------------------------------
+// Serving batches: slot -> segment by binary search over slot0 (segments in
+// increasing slot0), then the segment's pixel and sample; camera rays from the
+// request's own camera, PCG32 stream of (request seed, pixel, sample).
+PRR_DEV int findSegment(const Segment *segs, int nseg, int64_t slot) {
+    int lo = 0, hi = nseg - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (segs[mid].slot0 <= slot)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
+
+__global__ __launch_bounds__(kShadeBlock) void kRaygenSeg(PathBufs pb, const Segment *__restrict__ segs, int nseg,
+                                                          int64_t n, uint32_t *__restrict__ sampleOf) {
+    const int64_t stride = (int64_t) gridDim.x * kShadeBlock;
+    for (int64_t slot = (int64_t) blockIdx.x * kShadeBlock + threadIdx.x; slot < n; slot += stride) {
+        const Segment &sg = segs[findSegment(segs, nseg, slot)];
+        const int64_t local = slot - sg.slot0;
+        const int p = sg.p0 + (int) (local % sg.np);
+        const int sIdx = sg.s0 + (int) (local / sg.np);
+        Pcg rng = Pcg::make(sg.seed, (uint32_t) p, (uint32_t) sIdx);
+        const float ox = rng.next();  // Renderer.cpp:75-76
+        const float oy = rng.next();
+        V3 o, d;
+        cameraRay(sg.cam, p % sg.W, p / sg.W, sg.W, sg.H, ox, oy, rng, o, d);
+        pb.L[slot] = make_float4(0.f, 0.f, 0.f, 0.f);
+        pb.beta[slot] = make_float4(1.f, 1.f, 1.f, 0.f);
+        pb.flags[slot] = 0u;
+        pb.rng[slot] = rng.state;
+        sampleOf[slot] = (uint32_t) sIdx;
+        pb.qSlot[0][slot] = (uint32_t) slot;
+        pb.qOrg[0][slot] = st4(o, 0.f);
+        pb.qDir[0][slot] = st4(d, 0.f);
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        pb.qcnt[0] = (uint32_t) n;
+        atomicAdd(&pb.stats[0], (unsigned long long) n);
+        atomicAdd(&pb.stats[2], (unsigned long long) n);
+    }
+}
+
+// Renderer.cpp:84-99 per request pixel: its ns samples of the batch added in
+// sample order. pixStart[k] = pixels of segments before k.
+__global__ void kAccumulateSeg(PathBufs pb, const Segment *__restrict__ segs, const int64_t *__restrict__ pixStart,
+                               int nseg, int64_t total) {
+    const int64_t t = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= total) return;
+    int lo = 0, hi = nseg - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (pixStart[mid] <= t)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    const Segment &sg = segs[lo];
+    const int64_t pp = t - pixStart[lo];
+    float4 *fb = sg.fb + sg.p0 + pp;
+    V3 acc = ld3(*fb);
+    for (int k = 0; k < sg.ns; k++) acc += ld3(pb.result[sg.slot0 + (int64_t) k * sg.np + pp]);
+    *fb = st4(acc, 0.f);
 }
 
 // ---------------------------------------------------------------------------
@@ -302,7 +369,7 @@ __global__ __launch_bounds__(kShadeBlock) void kShade(SceneView s, PathBufs pb, 
             uint32_t fl = pb.flags[slot];
             V3 L = ld3(pb.L[slot]);
             V3 beta = ld3(pb.beta[slot]);
-            const int sIdx = ci.sampleBase + (int) (slot / (uint32_t) ci.pixels);
+            const int sIdx = ci.sampleOf ? (int) ci.sampleOf[slot] : ci.sampleBase + (int) (slot / (uint32_t) ci.pixels);
             Pcg rng;
             rng.state = pb.rng[slot];
             rng.inc = pcgInc((uint32_t) sIdx);
@@ -724,6 +791,18 @@ hipError_t launchCompactActive(const AdaptiveBufs &ab, int pixels, int spp, hipS
 hipError_t launchResolveAdaptive(const float4 *fb, const AdaptiveBufs &ab, int pixels, float *rgb, int32_t *sppOut,
                                  hipStream_t st) {
     hipLaunchKernelGGL(kResolveAdaptive, dim3((pixels + 255) / 256), dim3(256), 0, st, fb, ab, pixels, rgb, sppOut);
+    return hipGetLastError();
+}
+hipError_t launchRaygenSeg(const PathBufs &pb, const Segment *segs, int nseg, int64_t n, uint32_t *sampleOf, int grid,
+                           hipStream_t st) {
+    hipLaunchKernelGGL(kRaygenSeg, dim3(grid), dim3(kShadeBlock), 0, st, pb, segs, nseg, n, sampleOf);
+    return hipGetLastError();
+}
+hipError_t launchAccumulateSeg(const PathBufs &pb, const Segment *segs, const int64_t *pixStart, int nseg,
+                               int64_t total, hipStream_t st) {
+    if (total <= 0) return hipSuccess;
+    hipLaunchKernelGGL(kAccumulateSeg, dim3((unsigned) ((total + 255) / 256)), dim3(256), 0, st, pb, segs, pixStart,
+                       nseg, total);
     return hipGetLastError();
 }
 hipError_t launchCacheTargets(const PathBufs &pb, const CacheRec &cr, uint32_t r0, uint32_t r1, float *target,

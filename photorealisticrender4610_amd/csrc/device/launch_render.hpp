@@ -23,6 +23,10 @@ hipError_t launchAccumulateAdaptive(const PathBufs &pb, const ChunkInfo &ci, flo
 hipError_t launchCompactActive(const AdaptiveBufs &ab, int pixels, int spp, hipStream_t st);
 hipError_t launchResolveAdaptive(const float4 *fb, const AdaptiveBufs &ab, int pixels, float *rgb, int32_t *sppOut,
                                  hipStream_t st);
+hipError_t launchRaygenSeg(const PathBufs &pb, const Segment *segs, int nseg, int64_t n, uint32_t *sampleOf, int grid,
+                           hipStream_t st);
+hipError_t launchAccumulateSeg(const PathBufs &pb, const Segment *segs, const int64_t *pixStart, int nseg,
+                               int64_t total, hipStream_t st);
 hipError_t launchCacheTargets(const PathBufs &pb, const CacheRec &cr, uint32_t r0, uint32_t r1, float *target,
                               hipStream_t st);
 hipError_t launchResolve(const float4 *fb, int pixels, int spp, float *rgb, hipStream_t st);

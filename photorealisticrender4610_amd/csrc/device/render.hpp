@@ -64,6 +64,18 @@ struct CacheRec {
     float4 *tLpre = nullptr;
 };
 
+// Serving: one request's share of a batch, samples [s0, s0 + ns) of pixels
+// [p0, p0 + np) in sample-major slot order from slot0. A request has at most
+// one segment per batch, so each of its pixels takes its samples in order.
+struct Segment {
+    CamView cam;
+    int W, H;
+    int p0, np, s0, ns;
+    int64_t slot0;
+    uint64_t seed;
+    float4 *fb;  // the request's running sums
+};
+
 struct ChunkInfo {
     int W, H;
     int pixels;          // W * H
@@ -76,6 +88,8 @@ struct ChunkInfo {
     // (adaptive passes render only the pixels still sampling).
     const int32_t *pixList = nullptr;
     CacheRec cache;
+    // per-slot sample index (serving batches); null: sampleBase + slot / pixels
+    const uint32_t *sampleOf = nullptr;
 };
 
 // Per-pixel Welford state of the adaptive sampler (Renderer.hpp:14-40):

@@ -589,8 +589,18 @@ bool buildHostScene(const prr_scene_desc *d, HostScene &S, std::string &err) {
         S.envTotal = 0.f;
     }
 
-    // camera (Camera::init, Camera.hpp:43-56)
-    const prr_camera_desc &c = d->camera;
+    S.cam = makeCamView(d->camera);
+    for (int k = 0; k < 3; k++) S.background[k] = d->background[k];
+    S.numPoints = d->num_point_lights;
+    if (S.numPoints > 0)
+        for (int k = 0; k < 3; k++) {
+            S.pointPos[k] = d->point_lights[0].position[k];
+            S.pointInt[k] = d->point_lights[0].intensity[k];
+        }
+    return true;
+}
+
+CamView makeCamView(const prr_camera_desc &c) {
     V3 eye(c.eye[0], c.eye[1], c.eye[2]), tgt(c.target[0], c.target[1], c.target[2]), up(c.up[0], c.up[1], c.up[2]);
     V3 fwd = normalize(tgt - eye);
     V3 right = normalize(cross(fwd, up));
@@ -601,24 +611,18 @@ bool buildHostScene(const prr_scene_desc *d, HostScene &S, std::string &err) {
         V3 dd = tgt - eye;
         focus = std::sqrt(dd.x * dd.x + dd.y * dd.y + dd.z * dd.z);
     }
+    CamView v;
     const V3 *vs[4] = {&eye, &fwd, &right, &cu};
-    float *dst[4] = {S.camEye, S.camFwd, S.camRight, S.camUp};
+    float *dst[4] = {v.eye, v.fwd, v.right, v.up};
     for (int i = 0; i < 4; i++) {
         dst[i][0] = vs[i]->x;
         dst[i][1] = vs[i]->y;
         dst[i][2] = vs[i]->z;
     }
-    S.camAperture = c.aperture;
-    S.camFocus = focus;
-    S.camTanHalf = std::tan(halfFovH);
-    for (int k = 0; k < 3; k++) S.background[k] = d->background[k];
-    S.numPoints = d->num_point_lights;
-    if (S.numPoints > 0)
-        for (int k = 0; k < 3; k++) {
-            S.pointPos[k] = d->point_lights[0].position[k];
-            S.pointInt[k] = d->point_lights[0].intensity[k];
-        }
-    return true;
+    v.aperture = c.aperture;
+    v.focus = focus;
+    v.tanhalf = std::tan(halfFovH);
+    return v;
 }
 
 }  // namespace host

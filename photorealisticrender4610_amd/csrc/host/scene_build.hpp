@@ -74,9 +74,7 @@ struct HostScene {
     int maxDepth = 0;
     float totalEmitArea = 0.f;
     int numTris = 0, numSpheres = 0;
-    // camera
-    float camEye[3], camFwd[3], camRight[3], camUp[3];
-    float camAperture = 0.f, camFocus = 1.f, camTanHalf = 1.f;
+    CamView cam{};
     float background[3] = {0, 0, 0};
     int numPoints = 0;
     float pointPos[3] = {0, 0, 0}, pointInt[3] = {0, 0, 0};
@@ -85,6 +83,8 @@ struct HostScene {
 // Validates the descriptor and builds everything. Returns false with a
 // message on invalid input.
 bool buildHostScene(const prr_scene_desc *d, HostScene &out, std::string &err);
+// Camera::init (Camera.hpp:43-56): basis, focus distance, tan(fovh / 2).
+CamView makeCamView(const prr_camera_desc &c);
 
 }  // namespace host
 }  // namespace prr
