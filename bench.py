@@ -7,12 +7,14 @@ extension and shadow ray of all 64 spp) plus the ACES tone map to 8 bits, with
 the scene already resident in HBM. Prints ONE JSON line.
 
 Usage: python bench.py [--gpus 1] [--steps 3] [--warmup 1]
-                       [--workload sphere_field|glass_torus|box_room|radiance_cache]
-                       [--no-cpu] [--spp N] [--ref-spp N] [--frames N]
+                       [--workload sphere_field|glass_torus|box_room|radiance_cache|serving]
+                       [--no-cpu] [--spp N] [--ref-spp N] [--frames N] [--requests N]
 
 --workload radiance_cache prints the equal-time relMSE line of the learned
 radiance cache against the plain tracer (60 frames of 4 spp, plain reference
-of --ref-spp samples per pixel).
+of --ref-spp samples per pixel). --workload serving prints the p50/p99
+latency line of the continuous-batching server against running the same
+Poisson request stream one request at a time.
 """
 import argparse
 import json
@@ -204,12 +206,127 @@ def cache_bench(args):
     print(json.dumps(line), flush=True)
 
 
+SERVING = {"requests": 400, "rate": 20.0, "sizes": (256, 512, 1024), "spps": (1, 4, 16, 64), "seed": 0xC3,
+           "desc": "serving: box_room (depth 8) and sphere_field (1,048,578 tris, depth 12) resident; Poisson "
+                   "arrivals at 20 req/s; resolution uniform in {256^2, 512^2, 1024^2}, spp uniform in {1, 4, 16, "
+                   "64}, camera on a sphere of 2.5x the scene's bounding radius aimed at its centre; 400 requests"}
+
+
+def serving_requests(n, infos, seed):
+    """The seeded request list: (arrival s, scene, size, spp, eye, center, fovh)."""
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    out = []
+    for i in range(n):
+        t += rng.exponential(1.0 / SERVING["rate"])
+        si = int(rng.integers(0, 2))
+        size = int(rng.choice(SERVING["sizes"]))
+        spp = int(rng.choice(SERVING["spps"]))
+        b = np.asarray(infos[si]["bounds"], np.float64)
+        center = 0.5 * (b[:3] + b[3:])
+        radius = 0.5 * np.linalg.norm(b[3:] - b[:3])
+        v = rng.normal(size=3)
+        v /= np.linalg.norm(v)
+        out.append((t, si, size, spp, center + 2.5 * radius * v, center, infos[si]["fovh"]))
+    return out
+
+
+def serving_bench(args):
+    """Continuous batching vs one request at a time on the same request list:
+    p50 / p99 latency (completion - arrival) and aggregate Msamples/s."""
+    import threading
+    from photorealisticrender4610_amd import abi, scenes
+    ctx = abi.Context(0)
+    scs = [scenes.box_room(), scenes.sphere_field()]
+    dev = [ctx.scene(sc) for sc in scs]
+    infos = [dict(d.info(), fovh=float(sc.camera[9])) for d, sc in zip(dev, scs)]
+    n = args.requests if args.requests > 0 else SERVING["requests"]
+    reqs = serving_requests(n, infos, SERVING["seed"])
+    depths = [8, 12]
+
+    def submit(srv, k):
+        t, si, size, spp, eye, center, fovh = reqs[k]
+        cam = abi.camera_desc(eye, center, (0.0, 1.0, 0.0), fovh)
+        return srv.submit(si, size, size, spp, cam, seed=k)
+
+    def run(mode, policy="fair"):
+        srv = ctx.server(dev, depths, path_slots=1 << 22, policy=policy)
+        ids = [None] * n
+        arrive = [None] * n  # server clock, ms
+        done = []
+        t0 = time.perf_counter()
+        if mode == "batched":
+            def feeder():
+                for k in range(n):
+                    dt = reqs[k][0] - (time.perf_counter() - t0)
+                    if dt > 0:
+                        time.sleep(dt)
+                    ids[k] = submit(srv, k)
+                    if k % 50 == 49:
+                        log("batched: submitted %d/%d" % (k + 1, n))
+            th = threading.Thread(target=feeder)
+            th.start()
+            for k in range(n):
+                while ids[k] is None:
+                    time.sleep(0.0005)
+                _, st = srv.wait(ids[k], image=False)
+                arrive[k] = st["t_submit_ms"]
+                done.append(st)
+            th.join()
+        else:
+            anchor = None
+            for k in range(n):
+                dt = reqs[k][0] - (time.perf_counter() - t0)
+                if dt > 0:
+                    time.sleep(dt)
+                _, st = srv.wait(submit(srv, k), image=False)
+                if anchor is None:
+                    anchor = st["t_submit_ms"] - reqs[0][0] * 1e3
+                arrive[k] = anchor + reqs[k][0] * 1e3
+                done.append(st)
+                if k % 50 == 49:
+                    log("one at a time: %d/%d done" % (k + 1, n))
+        sst = srv.stats()
+        srv.close()
+        lat = np.array([d["t_done_ms"] - a for d, a in zip(done, arrive)])
+        span = (max(d["t_done_ms"] for d in done) - min(arrive)) / 1e3
+        samples = sum(d["samples"] for d in done)
+        res = {"p50_ms": round(float(np.percentile(lat, 50)), 2), "p99_ms": round(float(np.percentile(lat, 99)), 2),
+               "mean_ms": round(float(lat.mean()), 2), "msamples_per_s": round(samples / span / 1e6, 2),
+               "span_s": round(span, 3), "samples": int(samples), "rays": int(sst["rays"]),
+               "batches": int(sst["batches"]), "peak_active": int(sst["peak_active"]),
+               "busy_s": round(sst["busy_ms"] / 1e3, 3)}
+        log("%s %s: %s" % (mode, policy, res))
+        return res
+
+    batched = run("batched", "aged")
+    others = {p: run("batched", p) for p in ("oldest", "shortest", "fair")}
+    single = run("one_at_a_time")
+    line = {
+        "metric": "p50/p99 request latency", "value": batched["p99_ms"], "unit": "ms (p99)", "n_gpus": 1,
+        "steps": n, "warmup": 0, "ms_per_step": round(batched["span_s"] * 1e3 / n, 2), "higher_is_better": False,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded procedural scenes)",
+        "config": {"workload": SERVING["desc"], "requests": n, "rate_per_s": SERVING["rate"]},
+        "p50_ms": batched["p50_ms"], "p99_ms": batched["p99_ms"],
+        "msamples_per_s": batched["msamples_per_s"],
+        "mrays_per_s": round(batched["rays"] / batched["span_s"] / 1e6, 2),
+        "policy": "aged", "batched": batched, "one_at_a_time": single,
+        "other_policies": {p: {k: r[k] for k in ("p50_ms", "p99_ms", "mean_ms", "msamples_per_s")}
+                           for p, r in others.items()},
+        "p99_speedup_vs_one_at_a_time": round(single["p99_ms"] / batched["p99_ms"], 2),
+        "roofline": None, "cpu_baseline": None,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="sphere_field", choices=sorted(WORKLOADS) + ["radiance_cache"])
+    ap.add_argument("--workload", default="sphere_field",
+                    choices=sorted(WORKLOADS) + ["radiance_cache", "serving"])
+    ap.add_argument("--requests", type=int, default=0, help="serving: override the request count")
     ap.add_argument("--ref-spp", type=int, default=16384, help="radiance_cache: reference samples per pixel")
     ap.add_argument("--frames", type=int, default=0, help="radiance_cache: override the frame count")
     ap.add_argument("--spp", type=int, default=0, help="override spp (profiling runs only)")
@@ -222,6 +339,8 @@ def main():
 
     if args.workload == "radiance_cache":
         return cache_bench(args)
+    if args.workload == "serving":
+        return serving_bench(args)
     import torch
     from photorealisticrender4610_amd import abi, scenes
 

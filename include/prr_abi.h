@@ -282,18 +282,26 @@ int prr_cast_rays(prr_context* ctx, const prr_scene* scene, int64_t n, const flo
  * Renderer.cpp:24). A server keeps scenes resident, admits requests in
  * arrival order while their memory fits the HBM budget, and runs batches of
  * up to path_slots paths through the wavefront kernels; every batch packs the
- * next samples of all admitted requests (one segment per request, oldest
- * first), so requests that arrive while others render join at the next
- * batch. A request's image depends only on (scene, camera, size, spp, seed):
+ * next samples of the admitted requests (one segment per request, shares set
+ * by the policy below), so requests that arrive while others render join at
+ * the next batch. A request's image depends only on (scene, camera, size, spp, seed):
  * each pixel takes samples in order with the PCG32 streams of prr_render, so
  * it is the same alone and in any mix. The server owns its context while it
  * runs (its worker thread makes every device call). */
 typedef struct prr_server prr_server;
 
+/* Batch shares (prr_server_params::policy) */
+#define PRR_SERVE_AGED 0     /* shortest remaining work first, after subtracting the samples the
+                                server renders in the time the request has waited */
+#define PRR_SERVE_OLDEST 1   /* arrival order, each takes all it can */
+#define PRR_SERVE_SHORTEST 2 /* shortest remaining work first, each takes all it can */
+#define PRR_SERVE_FAIR 3     /* max-min fair: by remaining work, each takes min(need, equal share of the rest) */
+
 typedef struct prr_server_params {
     int64_t path_slots;    /* paths per batch (1 << 22) */
     int64_t hbm_budget;    /* bytes of admitted request memory (16 B per pixel); 0: half the free HBM */
-    int32_t reserved[4];
+    int32_t policy;        /* PRR_SERVE_* */
+    int32_t reserved[3];
 } prr_server_params;
 
 typedef struct prr_request_desc {

@@ -94,7 +94,11 @@ class CacheStats(C.Structure):
 
 
 class ServerParams(C.Structure):
-    _fields_ = [("path_slots", C.c_int64), ("hbm_budget", C.c_int64), ("reserved", C.c_int32 * 4)]
+    _fields_ = [("path_slots", C.c_int64), ("hbm_budget", C.c_int64), ("policy", C.c_int32),
+                ("reserved", C.c_int32 * 3)]
+
+
+SERVE_POLICIES = {"aged": 0, "oldest": 1, "shortest": 2, "fair": 3}
 
 
 class RequestDesc(C.Structure):
@@ -279,8 +283,8 @@ class Context:
     def scene(self, sc):
         return Scene(self, sc)
 
-    def server(self, scenes, depths, clamps=None, path_slots=1 << 22, hbm_budget=0):
-        return Server(self, scenes, depths, clamps, path_slots, hbm_budget)
+    def server(self, scenes, depths, clamps=None, path_slots=1 << 22, hbm_budget=0, policy="aged"):
+        return Server(self, scenes, depths, clamps, path_slots, hbm_budget, policy)
 
     def cache(self, bounds, seed=0xC4, lr=1e-2, beta1=0.9, beta2=0.99, eps=1e-8, grad_clip=100.0,
               train_batch=65536):
@@ -584,14 +588,14 @@ class Server:
     thread owns the context until close(); submit() and wait() may be called
     from any thread."""
 
-    def __init__(self, ctx, scenes, depths, clamps, path_slots, hbm_budget):
+    def __init__(self, ctx, scenes, depths, clamps, path_slots, hbm_budget, policy="aged"):
         self.ctx = ctx
         n = len(scenes)
         self._scenes = list(scenes)  # keep the scenes alive
         arr = (vp * n)(*[s.h for s in scenes])
         dep = (i32 * n)(*depths)
         cl = (C.c_float * n)(*(clamps if clamps is not None else [10.0] * n))
-        p = ServerParams(path_slots=path_slots, hbm_budget=hbm_budget)
+        p = ServerParams(path_slots=path_slots, hbm_budget=hbm_budget, policy=SERVE_POLICIES[policy])
         h = vp()
         ctx._check(lib().prr_server_create(ctx.h, arr, dep, cl, n, C.byref(p), C.byref(h)), ctx.h)
         self.h = h

@@ -2,8 +2,8 @@
 //
 // A worker thread owns the context. Each loop: admit pending requests in
 // arrival order while their framebuffers fit the HBM budget; plan one batch
-// of at most path_slots paths with one segment per admitted request (oldest
-// first); run it per scene (segment ray generation, the wavefront bounces,
+// of at most path_slots paths with one segment per admitted request (the
+// share of each set by the server's policy); run it per scene (segment ray generation, the wavefront bounces,
 // per-request accumulation in sample order); resolve and copy out the
 // requests the batch completed. Requests submitted meanwhile wait only for
 // the current batch to end.
@@ -44,6 +44,7 @@ struct Request {
     int batches = 0;
     std::vector<float> rgb;
     std::string err;
+    int64_t remaining() const { return (int64_t) (d.spp - sCur) * pixels - pCur; }
 };
 
 struct Planned {
@@ -191,6 +192,45 @@ static bool planSegment(Request *r, int64_t room, Planned &pl) {
     return true;
 }
 
+// One batch of at most path_slots paths, one segment per admitted request.
+//   PRR_SERVE_FAIR      max-min fair shares: requests in order of remaining
+//                       work each take min(their remaining work, an equal
+//                       share of the room left), so small requests finish in
+//                       the next batch and large ones split the rest evenly
+//   PRR_SERVE_OLDEST    arrival order, each takes all it can
+//   PRR_SERVE_SHORTEST  shortest remaining work first, each takes all it can
+//   PRR_SERVE_AGED      shortest remaining work first after crediting each
+//                       request with the samples the server renders in the
+//                       time it has waited: waiting requests gain priority
+//                       at the service rate, so large ones are not starved
+static void planBatch(prr_server *S, std::vector<Planned> &plan) {
+    std::vector<Request *> order(S->active.begin(), S->active.end());
+    if (S->p.policy == PRR_SERVE_AGED) {
+        const double now = S->now();
+        const double rate = S->stats.busy_ms > 0.0 ? (double) S->stats.samples / S->stats.busy_ms : 2e5;
+        std::vector<std::pair<double, Request *>> key;
+        for (Request *r : order) key.emplace_back((double) r->remaining() - rate * (now - r->tSubmit), r);
+        std::stable_sort(key.begin(), key.end(),
+                         [](const std::pair<double, Request *> &a, const std::pair<double, Request *> &b) {
+                             return a.first < b.first;
+                         });
+        for (size_t i = 0; i < key.size(); i++) order[i] = key[i].second;
+    } else if (S->p.policy != PRR_SERVE_OLDEST) {
+        std::stable_sort(order.begin(), order.end(),
+                         [](const Request *a, const Request *b) { return a->remaining() < b->remaining(); });
+    }
+    int64_t room = S->p.path_slots;
+    const int n = (int) std::min<size_t>(order.size(), (size_t) S->segCap);
+    for (int i = 0; i < n && room > 0; i++) {
+        int64_t give = room;
+        if (S->p.policy == PRR_SERVE_FAIR) give = std::max<int64_t>(1, room / (n - i));
+        Planned pl;
+        if (!planSegment(order[i], give, pl)) continue;
+        room -= (int64_t) pl.seg.np * pl.seg.ns;
+        plan.push_back(pl);
+    }
+}
+
 static void workerLoop(prr_server *S) {
     (void) hipSetDevice(S->ctx->device);
     std::unique_lock<std::mutex> lk(S->mu);
@@ -227,14 +267,7 @@ static void workerLoop(prr_server *S) {
             continue;
         }
         std::vector<Planned> plan;
-        int64_t room = S->p.path_slots;
-        for (Request *r : S->active) {
-            if (room <= 0 || (int) plan.size() >= S->segCap) break;
-            Planned pl;
-            if (!planSegment(r, room, pl)) continue;
-            room -= (int64_t) pl.seg.np * pl.seg.ns;
-            plan.push_back(pl);
-        }
+        planBatch(S, plan);
         lk.unlock();
         const double tb = S->now();
         int64_t rays = 0;
@@ -290,7 +323,8 @@ int prr_server_create(prr_context *c, const prr_scene *const *scenes, const int3
     if (!c || !scenes || !max_depth || !indirect_clamp || !p || !out || num_scenes <= 0)
         return fail(c, PRR_ERR_INVALID, "NULL argument");
     *out = nullptr;
-    if (p->path_slots <= 0 || p->path_slots > kMaxSlots || p->hbm_budget < 0)
+    if (p->path_slots <= 0 || p->path_slots > kMaxSlots || p->hbm_budget < 0 || p->policy < PRR_SERVE_AGED ||
+        p->policy > PRR_SERVE_FAIR)
         return fail(c, PRR_ERR_INVALID, "bad server parameters");
     int maxDepth = 1;
     for (int i = 0; i < num_scenes; i++) {
