@@ -55,10 +55,52 @@ MFMA_FP16_PEAK_TFLOPS = 2500.0  # MI355X dense FP16 MFMA
 CACHE_FLOPS_PER_QUERY = 2 * 22720  # one multiply-add per weight
 
 
+def reference_baseline(exe, sc, w, h, depth, target_s=12.0):
+    """The reference renderer itself (oracle/_ref/ref_bench: its Integrator,
+    BVH, materials and Render sample loop compiled from its own sources with
+    its release flags), timed on this host's cores: rows of the image at 16
+    spp, top to bottom, until target_s has passed (threads stop taking new
+    rows); closest / any-hit queries counted at Scene::intersect /
+    intersectP."""
+    import subprocess
+    import tempfile
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    from bundle import read_bundle, write_bundle
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
+    spp = 16
+    r0, r1 = 0, h
+    arr = sc.bundle_arrays()
+    arr["cfg"] = np.array([w, h, depth, spp, r0, r1, threads, int(target_s * 1000)], np.int32)
+    with tempfile.TemporaryDirectory() as td:
+        fin, fout = os.path.join(td, "in.bin"), os.path.join(td, "out.bin")
+        write_bundle(fin, arr)
+        env = dict(os.environ, OMP_NUM_THREADS=str(threads))
+        subprocess.run([exe, "bench", fin, fout], check=True, env=env, timeout=600, stdout=subprocess.DEVNULL,
+                       stderr=subprocess.DEVNULL)
+        out = read_bundle(fout)
+    sec, build_s = float(out["seconds"][0]), float(out["seconds"][1])
+    c = [int(x) for x in out["counts"]]
+    closest, anyq, samples, used = (c[0] << 32) | c[1], (c[2] << 32) | c[3], (c[4] << 32) | c[5], c[6]
+    rays = closest + anyq
+    return {"value": round(rays / sec / 1e6, 3), "unit": "Mrays/s", "cores": used, "kind": "reference",
+            "sample": "the reference's own castRay / BVH / Render loop (oracle/_ref/ref_bench, -O3 -ffast-math "
+                      "-march=x86-64-v3 OpenMP, no LTO) on rows %d.. of the %dx%d image at %d spp: %d samples, "
+                      "%d rays (%d closest, %d any-hit) in %.1f s; its scene build (%.1f s) excluded"
+                      % (r0, w, h, spp, samples, rays, closest, anyq, sec, build_s)}
+
+
 def cpu_baseline(sc, w, h, depth, seed, target_s=12.0):
-    """The oracle built with the reference's release flags (-O3 -ffast-math,
-    OpenMP), timed on this host's cores on a bounded sample of the same render
-    (a band of rows, or the whole frame at a few spp), sized to ~target_s."""
+    """The CPU baseline: the reference renderer itself when its build
+    (oracle/_ref/ref_bench) is present, else the oracle built with the
+    reference's release flags (-O3 -ffast-math, OpenMP), timed on this
+    host's cores on a bounded sample of the same render (a band of rows, or
+    the whole frame at a few spp), sized to ~target_s."""
+    exe = os.path.join(ROOT, "oracle", "_ref", "ref_bench")
+    if os.path.exists(exe):
+        try:
+            return reference_baseline(exe, sc, w, h, depth, target_s)
+        except Exception as e:  # the port below is the fallback
+            log("reference baseline failed (%s); timing the port" % e)
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle as O
     threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)

@@ -3,7 +3,8 @@
 // TEST INFRASTRUCTURE ONLY.
 //
 // Usage: ref_harness <mode> <in.bundle> <out.bundle>
-//   modes: trace, bsdf, env, camera, tonemap, castray, image, welford, hdr
+//   modes: trace, bsdf, env, camera, tonemap, castray, image, welford, hdr,
+//          bench (only in the ref_bench build: PRR_REF_BENCH)
 // A bundle is a sequence of records {char name[32]; int32 dtype; int64 count;
 // payload}, dtype 0=f32 1=i32 2=u64 3=u8 4=u32 (tests/golden/bundle.py).
 //
@@ -12,6 +13,7 @@
 // `public` in this translation unit only.
 #include <algorithm>
 #include <array>
+#include <chrono>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -584,6 +586,77 @@ static int modeHdr(const Bundle &b, const char *out) {
     return 0;
 }
 
+#ifdef PRR_REF_BENCH
+// Scene-level query counts without editing the reference: the link wraps
+// Scene::intersect / Scene::intersectP (-Wl,--wrap, oracle/Makefile), so every
+// call from castRay's translation unit passes through these counters.
+static thread_local uint64_t t_closest = 0, t_any = 0;
+Intersection realIntersect(const Scene *, const Ray &) __asm__("__real__ZNK5Scene9intersectERK3Ray");
+bool realIntersectP(const Scene *, const Ray &, float) __asm__("__real__ZNK5Scene10intersectPERK3Rayf");
+Intersection wrapIntersect(const Scene *s, const Ray &r) __asm__("__wrap__ZNK5Scene9intersectERK3Ray");
+bool wrapIntersectP(const Scene *s, const Ray &r, float t) __asm__("__wrap__ZNK5Scene10intersectPERK3Rayf");
+Intersection wrapIntersect(const Scene *s, const Ray &r) {
+    t_closest++;
+    return realIntersect(s, r);
+}
+bool wrapIntersectP(const Scene *s, const Ray &r, float t) {
+    t_any++;
+    return realIntersectP(s, r, t);
+}
+
+// The reference's own Render sample loop (Renderer.cpp:63-99, fixed spp,
+// rows handed out one at a time) on rows [row0, row1), timed with the wall
+// clock; threads stop taking rows once budget_ms have passed (0: no budget).
+// The scene build is not timed.
+static int modeBench(const Bundle &b, const char *out) {
+    RefScene rs;
+    const auto tb0 = std::chrono::steady_clock::now();
+    buildScene(b, rs);
+    const double buildS = std::chrono::duration<double>(std::chrono::steady_clock::now() - tb0).count();
+    const int32_t *cfg = b.at("cfg").as<int32_t>();  // W, H, maxDepth, spp, row0, row1, threads, budget_ms
+    const int W = cfg[0], H = cfg[1], spp = cfg[3], row0 = cfg[4], row1 = cfg[5], threads = cfg[6];
+    const double budget = cfg[7] / 1000.0;
+    rs.scene.camera.init(W, H);
+    Integrator integ(rs.scene, cfg[2], 0.95f);
+    prr_refseed::counter_mode().store(1);
+#ifdef _OPENMP
+    if (threads > 0) omp_set_num_threads(threads);
+#endif
+    uint64_t closest = 0, any = 0, samples = 0;
+    int used = 1;
+    std::atomic<int> nextRow(row0);
+    const auto t0 = std::chrono::steady_clock::now();
+#pragma omp parallel reduction(+ : closest, any, samples)
+    {
+        t_closest = t_any = 0;
+        while (true) {
+            if (budget > 0 &&
+                std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count() > budget)
+                break;
+            const int j = nextRow.fetch_add(1);
+            if (j >= row1) break;
+            for (int i = 0; i < W; i++)
+                for (int k = 0; k < spp; k++) {
+                    oneSample(rs, integ, i, j, W, H);
+                    samples++;
+                }
+        }
+        closest += t_closest;
+        any += t_any;
+#ifdef _OPENMP
+#pragma omp single
+        used = omp_get_num_threads();
+#endif
+    }
+    const double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    Writer w(out);
+    w.f32("seconds", {(float) dt, (float) buildS});
+    w.u32("counts", {(uint32_t) (closest >> 32), (uint32_t) closest, (uint32_t) (any >> 32), (uint32_t) any,
+                     (uint32_t) (samples >> 32), (uint32_t) samples, (uint32_t) used});
+    return 0;
+}
+#endif
+
 int main(int argc, char **argv) {
     if (argc != 4) {
         fprintf(stderr, "usage: %s <mode> <in> <out>\n", argv[0]);
@@ -600,6 +673,9 @@ int main(int argc, char **argv) {
     if (m == "image") return modeImage(b, argv[3]);
     if (m == "welford") return modeWelford(b, argv[3]);
     if (m == "hdr") return modeHdr(b, argv[3]);
+#ifdef PRR_REF_BENCH
+    if (m == "bench") return modeBench(b, argv[3]);
+#endif
     fprintf(stderr, "unknown mode %s\n", argv[1]);
     return 2;
 }

@@ -150,9 +150,12 @@ int runBounces(prr_context *c, const SceneView &view, const ChunkInfo &ci, Timin
         if (ei_ >= 0) HIPCK(c, hipEventRecord(c->evPool[ei_ + 1], st)); \
     } while (0)
     for (int b = 0; b <= ci.maxDepth; b++) {
-        TIMED(0, launchExtend(view, c->pb, b, c->spill, c->traceBlocks, c->refillAt, c->stackDepth, st));
-        if (b > 0) TIMED(1, launchShadow(view, c->pb, b - 1, c->spill, c->traceBlocks, c->refillAt, c->stackDepth, st));
-        TIMED(2, launchShade(view, c->pb, ci, b, c->shadeBlocks, st));
+        TIMED(0, launchExtend(view, c->pb, b, c->spill, c->traceBlocks, c->refillAt, c->stackDepth, c->whileWhile,
+                              st));
+        if (b > 0)
+            TIMED(1, launchShadow(view, c->pb, b - 1, c->spill, c->traceBlocks, c->refillAt, c->stackDepth,
+                                  c->whileWhile, st));
+        TIMED(2, launchShade(view, c->pb, ci, b, c->shadeBlocks, c->shadeOcc4, st));
     }
 #undef TIMED
     return PRR_OK;
@@ -251,12 +254,23 @@ int prr_create(int device, prr_context **out) {
     c->stream = c->own;
     c->numCU = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 1;
     // persistent traversal grid: 8 blocks of 256 lanes per CU (with 8-entry
-    // LDS stacks, 16 KB per block); PRR_TRACE_BLOCKS_PER_CU, PRR_STACK_LDS and
-    // PRR_REFILL_AT override the measured defaults for tuning sweeps
+    // LDS stacks, 16 KB per block); PRR_TRACE_BLOCKS_PER_CU, PRR_STACK_LDS,
+    // PRR_REFILL_AT, PRR_WHILE_WHILE and PRR_SHADE_OCC4 override the measured
+    // defaults for tuning sweeps
     int tbpc = 8;
     if (const char *ev = getenv("PRR_TRACE_BLOCKS_PER_CU")) tbpc = std::max(1, std::min(16, atoi(ev)));
     c->traceBlocks = c->numCU * tbpc;
     if (const char *ev = getenv("PRR_REFILL_AT")) c->refillAt = std::max(1, std::min(64, atoi(ev)));
+    {
+        size_t freeB = 0, totalB = 0;
+        if (hipMemGetInfo(&freeB, &totalB) == hipSuccess && freeB > 0)
+            c->maxSlots = std::max<int64_t>((int64_t) 1 << 20,
+                                            std::min<int64_t>(kMaxSlots, (int64_t) (0.4 * (double) freeB) / kPathBytes));
+        if (const char *ev = getenv("PRR_MAX_SLOTS"))
+            c->maxSlots = std::max<int64_t>(1 << 16, std::min<int64_t>(kMaxSlots, atoll(ev)));
+    }
+    if (const char *ev = getenv("PRR_WHILE_WHILE")) c->whileWhile = atoi(ev) != 0;
+    if (const char *ev = getenv("PRR_SHADE_OCC4")) c->shadeOcc4 = atoi(ev) != 0;
     if (const char *ev = getenv("PRR_STACK_LDS")) {  // LDS stack entries per lane: 2, 4, 8, 16 or 32
         const int v = atoi(ev);
         if (v == 2 || v == 4 || v == 8 || v == 16 || v == 32) c->stackDepth = v;
@@ -611,7 +625,7 @@ int prr_render(prr_context *c, const prr_scene *s, const prr_render_params *p, f
         p->sample_offset < 0 || p->min_spp < 0)
         return fail(c, PRR_ERR_INVALID, "bad render parameters");
     const int64_t pixels = (int64_t) p->width * p->height;
-    if (pixels > kMaxSlots) return fail(c, PRR_ERR_UNSUPPORTED, "image larger than 16M pixels");
+    if (pixels > c->maxSlots) return fail(c, PRR_ERR_UNSUPPORTED, "image larger than the context's path slots");
     if (p->num_photons > 0 && s->ph.stored <= 0 && s->ph.emitted <= 0)
         return fail(c, PRR_ERR_INVALID, "num_photons > 0 but the scene has no photon map (prr_photon_build)");
     HIPCK(c, hipSetDevice(c->device));
@@ -619,8 +633,8 @@ int prr_render(prr_context *c, const prr_scene *s, const prr_render_params *p, f
     // positive and min_spp < spp (Renderer.cpp:92-96); otherwise every pixel
     // takes spp samples.
     const bool adaptive = p->variance_threshold > 0.f && p->min_spp < p->spp;
-    const int spc = (int) std::max<int64_t>(1, std::min<int64_t>(p->spp, kMaxSlots / pixels));
-    int rc = ensureRenderBufs(c, adaptive ? std::max<int64_t>(pixels * spc, std::min(kMaxSlots, kAdaptivePassSlots))
+    const int spc = (int) std::max<int64_t>(1, std::min<int64_t>(p->spp, c->maxSlots / pixels));
+    int rc = ensureRenderBufs(c, adaptive ? std::max<int64_t>(pixels * spc, std::min(c->maxSlots, kAdaptivePassSlots))
                                           : pixels * spc,
                               p->max_depth);
     if (rc != PRR_OK) return rc;
@@ -645,7 +659,7 @@ int prr_render(prr_context *c, const prr_scene *s, const prr_render_params *p, f
     ci.maxDepth = p->max_depth;
     ci.clampInd = p->indirect_clamp;
     // Renders samples [s0, s0 + count) of the listed pixels (all when list is
-    // null) in chunks of at most kMaxSlots paths.
+    // null) in chunks of at most the context's path slots.
     auto renderSamples = [&](const int32_t *list, int64_t npix, int s0, int count) -> int {
         const int per = (int) std::max<int64_t>(1, std::min<int64_t>(count, c->pbSlots / npix));
         for (int k = 0; k < count; k += per) {
@@ -700,7 +714,7 @@ int prr_cast_rays(prr_context *c, const prr_scene *s, int64_t n, const float *or
         return fail(c, PRR_ERR_INVALID, "bad arguments");
     if (n == 0) return PRR_OK;
     HIPCK(c, hipSetDevice(c->device));
-    const int64_t chunk = std::min<int64_t>(n, kMaxSlots);
+    const int64_t chunk = std::min<int64_t>(n, c->maxSlots);
     int rc = ensureRenderBufs(c, chunk, max_depth);
     if (rc != PRR_OK) return rc;
     hipStream_t st = c->stream;

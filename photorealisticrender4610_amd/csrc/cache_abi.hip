@@ -320,9 +320,9 @@ int prr_cache_render(prr_context *c, const prr_scene *s, prr_cache *k, const prr
         cp->max_train_steps < 0 || cp->train_capacity < 0)
         return fail(c, PRR_ERR_INVALID, "bad cache render parameters");
     const int64_t pixels = (int64_t) p->width * p->height;
-    if (pixels > kMaxSlots) return fail(c, PRR_ERR_UNSUPPORTED, "image larger than 16M pixels");
+    if (pixels > c->maxSlots) return fail(c, PRR_ERR_UNSUPPORTED, "image larger than the context's path slots");
     HIPCK(c, hipSetDevice(c->device));
-    const int spc = (int) std::max<int64_t>(1, std::min<int64_t>(p->spp, kMaxSlots / pixels));
+    const int spc = (int) std::max<int64_t>(1, std::min<int64_t>(p->spp, c->maxSlots / pixels));
     int rc = ensureRenderBufs(c, pixels * spc, p->max_depth);
     if (rc != PRR_OK) return rc;
     AdaptiveBufs ab;

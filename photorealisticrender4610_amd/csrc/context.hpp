@@ -21,6 +21,9 @@ struct prr_context {
     int traceBlocks = 1, shadeBlocks = 1;
     int refillAt = 32;
     int stackDepth = 8;
+    int whileWhile = 1;     // traversal iteration: 1 interior nodes then a leaf, 0 one node per work check
+    bool shadeOcc4 = true;  // shading kernel compiled for 4 waves per SIMD
+    int64_t maxSlots = 1 << 24;  // paths per chunk (set at creation from free HBM)
     uint2 *spill = nullptr;
     int spillLanes = 0;
     // render buffers
@@ -64,10 +67,14 @@ int fail(prr_context *c, int code, const std::string &msg);
 
 namespace prr_host {
 
-// Paths in flight per chunk. Path state is ~340 B per slot, so 2^24 slots
-// hold ~5.7 GB of HBM; large chunks amortise the per-bounce tail of the
-// traversal kernels (a few long rays) over many samples.
-constexpr int64_t kMaxSlots = (int64_t) 1 << 24;
+// Paths in flight per chunk, at most. Path state is kPathBytes per slot, so
+// 2^27 slots hold ~47 GB of HBM: a whole 1080p frame at 64 spp in one chunk.
+// Large chunks pay the per-bounce tail of the traversal kernels (the last few
+// long rays of each bounce) once per frame instead of once per chunk. The
+// context's own limit (prr_context::maxSlots) also keeps path state within
+// 40 % of the HBM free when the context is created.
+constexpr int64_t kMaxSlots = (int64_t) 1 << 27;
+constexpr int64_t kPathBytes = 360;
 
 struct Timing {
     bool on = false;

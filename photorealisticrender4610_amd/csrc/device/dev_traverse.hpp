@@ -306,51 +306,9 @@ PRR_DEV void travBegin(TravState &ts, const V3 &o, const V3 &d, float tmax, uint
     st.sp = 0;
 }
 
-// One step of BVHAccel::Intersect; returns true when the ray is finished.
-PRR_DEV bool closestStep(const SceneView &s, TravState &ts, LaneStack &st) {
-    const uint32_t cur = ts.cur;
-    if (refIsLeaf(cur)) {
-        const uint32_t first = refFirst(cur), cnt = refCount(cur);
-        if (refSphere(cur)) {
-            for (uint32_t i = 0; i < cnt; i++) {
-                float t;
-                if (sphHit(s.sph[first + i], ts.o, ts.d, t) && t < ts.h.t) {
-                    ts.h.t = t;
-                    ts.h.u = ts.h.v = 0.f;
-                    ts.h.prim = s.sph_id[first + i];
-                }
-            }
-        } else {
-            for (uint32_t i = 0; i < cnt; i++) {
-                const TriRec tr = loadTri(s.tris, first + i);
-                float t, u, v;
-                if (triHit(tr, ts.o, ts.d, t, u, v) && t < ts.h.t) {
-                    ts.h.t = t;
-                    ts.h.u = u;
-                    ts.h.v = v;
-                    ts.h.prim = tr.id;
-                }
-            }
-        }
-    } else {
-        const DNode n = s.nodes[cur];
-        const float tL = slab(ts.r, n.a.x, n.a.y, n.a.z, n.a.w, n.b.x, n.b.y);
-        const float tR = slab(ts.r, n.b.z, n.b.w, n.c.x, n.c.y, n.c.z, n.c.w);
-        const uint32_t L = (uint32_t) n.d.x, R = (uint32_t) n.d.y;
-        if (tL <= tR) {
-            if (tL < ts.h.t) {
-                if (tR < ts.h.t) st.push(R, tR);
-                ts.cur = L;
-                return false;
-            }
-        } else {
-            if (tR < ts.h.t) {
-                if (tL < ts.h.t) st.push(L, tL);
-                ts.cur = R;
-                return false;
-            }
-        }
-    }
+// Next stack entry whose box entry t is below the closest hit so far
+// (BVH.cpp:399-407); true when the stack is empty (the ray is finished).
+PRR_DEV bool popNext(TravState &ts, LaneStack &st) {
     while (st.sp > 0) {
         const uint2 e = st.pop();
         if (__uint_as_float(e.y) < ts.h.t) {
@@ -361,6 +319,72 @@ PRR_DEV bool closestStep(const SceneView &s, TravState &ts, LaneStack &st) {
     return true;
 }
 
+// BVHAccel::Intersect at a leaf (ts.cur): test its primitives, then pop.
+PRR_DEV bool closestLeaf(const SceneView &s, TravState &ts, LaneStack &st) {
+    const uint32_t cur = ts.cur;
+    const uint32_t first = refFirst(cur), cnt = refCount(cur);
+    if (refSphere(cur)) {
+        for (uint32_t i = 0; i < cnt; i++) {
+            float t;
+            if (sphHit(s.sph[first + i], ts.o, ts.d, t) && t < ts.h.t) {
+                ts.h.t = t;
+                ts.h.u = ts.h.v = 0.f;
+                ts.h.prim = s.sph_id[first + i];
+            }
+        }
+    } else {
+        for (uint32_t i = 0; i < cnt; i++) {
+            const TriRec tr = loadTri(s.tris, first + i);
+            float t, u, v;
+            if (triHit(tr, ts.o, ts.d, t, u, v) && t < ts.h.t) {
+                ts.h.t = t;
+                ts.h.u = u;
+                ts.h.v = v;
+                ts.h.prim = tr.id;
+            }
+        }
+    }
+    return popNext(ts, st);
+}
+
+// BVHAccel::Intersect at an interior node: near child first, the far one
+// pushed with its entry t, or pop when neither box is closer than the hit.
+PRR_DEV bool closestInterior(const SceneView &s, TravState &ts, LaneStack &st) {
+    const DNode n = s.nodes[ts.cur];
+    const float tL = slab(ts.r, n.a.x, n.a.y, n.a.z, n.a.w, n.b.x, n.b.y);
+    const float tR = slab(ts.r, n.b.z, n.b.w, n.c.x, n.c.y, n.c.z, n.c.w);
+    const uint32_t L = (uint32_t) n.d.x, R = (uint32_t) n.d.y;
+    if (tL <= tR) {
+        if (tL < ts.h.t) {
+            if (tR < ts.h.t) st.push(R, tR);
+            ts.cur = L;
+            return false;
+        }
+    } else {
+        if (tR < ts.h.t) {
+            if (tL < ts.h.t) st.push(L, tL);
+            ts.cur = R;
+            return false;
+        }
+    }
+    return popNext(ts, st);
+}
+
+// One step of BVHAccel::Intersect; returns true when the ray is finished.
+PRR_DEV bool closestStep(const SceneView &s, TravState &ts, LaneStack &st) {
+    return refIsLeaf(ts.cur) ? closestLeaf(s, ts, st) : closestInterior(s, ts, st);
+}
+
+// While-while iteration (Aila and Laine 2009): interior steps until this
+// lane stands on a leaf, then that leaf, so a wave tests its leaves
+// together. Each ray visits the same nodes in the same order as with
+// closestStep.
+PRR_DEV bool closestIter(const SceneView &s, TravState &ts, LaneStack &st) {
+    while (!refIsLeaf(ts.cur))
+        if (closestInterior(s, ts, st)) return true;
+    return closestLeaf(s, ts, st);
+}
+
 // Root test of BVHAccel::IntersectP; false = the ray misses the scene box.
 PRR_DEV bool anyBegin(const SceneView &s, TravState &ts) {
     const float tb = slab(ts.r, s.root_box[0], s.root_box[1], s.root_box[2], s.root_box[3], s.root_box[4],
@@ -368,50 +392,68 @@ PRR_DEV bool anyBegin(const SceneView &s, TravState &ts) {
     return tb < ts.tmax;
 }
 
-// One step of BVHAccel::IntersectP; returns true when finished, with
-// ts.h.prim = 1 if occluded, -1 otherwise.
-PRR_DEV bool anyStep(const SceneView &s, TravState &ts, LaneStack &st) {
-    const uint32_t cur = ts.cur;
-    if (refIsLeaf(cur)) {
-        const uint32_t first = refFirst(cur), cnt = refCount(cur);
-        if (refSphere(cur)) {
-            for (uint32_t i = 0; i < cnt; i++) {
-                float t;
-                if (sphHit(s.sph[first + i], ts.o, ts.d, t) && t < ts.tmax) {
-                    ts.h.prim = 1;
-                    return true;
-                }
-            }
-        } else {
-            for (uint32_t i = 0; i < cnt; i++)
-                if (triAny(loadTri(s.tris, first + i), ts.o, ts.d, ts.tmax)) {
-                    ts.h.prim = 1;
-                    return true;
-                }
-        }
-    } else {
-        const DNode n = s.nodes[cur];
-        const float tL = slab(ts.r, n.a.x, n.a.y, n.a.z, n.a.w, n.b.x, n.b.y);
-        const float tR = slab(ts.r, n.b.z, n.b.w, n.c.x, n.c.y, n.c.z, n.c.w);
-        const bool hl = tL < ts.tmax, hr = tR < ts.tmax;
-        const uint32_t L = (uint32_t) n.d.x, R = (uint32_t) n.d.y;
-        if (hl && hr) {
-            const bool lf = tL <= tR;
-            st.push(lf ? R : L, 0.f);
-            ts.cur = lf ? L : R;
-            return false;
-        }
-        if (hl || hr) {
-            ts.cur = hl ? L : R;
-            return false;
-        }
-    }
+// BVHAccel::IntersectP at a leaf; true when finished (ts.h.prim = 1 if
+// occluded, -1 otherwise).
+PRR_DEV bool anyPop(TravState &ts, LaneStack &st) {
     if (st.sp == 0) {
         ts.h.prim = -1;
         return true;
     }
     ts.cur = st.pop().x;
     return false;
+}
+
+PRR_DEV bool anyLeaf(const SceneView &s, TravState &ts, LaneStack &st) {
+    const uint32_t cur = ts.cur;
+    const uint32_t first = refFirst(cur), cnt = refCount(cur);
+    if (refSphere(cur)) {
+        for (uint32_t i = 0; i < cnt; i++) {
+            float t;
+            if (sphHit(s.sph[first + i], ts.o, ts.d, t) && t < ts.tmax) {
+                ts.h.prim = 1;
+                return true;
+            }
+        }
+    } else {
+        for (uint32_t i = 0; i < cnt; i++)
+            if (triAny(loadTri(s.tris, first + i), ts.o, ts.d, ts.tmax)) {
+                ts.h.prim = 1;
+                return true;
+            }
+    }
+    return anyPop(ts, st);
+}
+
+PRR_DEV bool anyInterior(const SceneView &s, TravState &ts, LaneStack &st) {
+    const DNode n = s.nodes[ts.cur];
+    const float tL = slab(ts.r, n.a.x, n.a.y, n.a.z, n.a.w, n.b.x, n.b.y);
+    const float tR = slab(ts.r, n.b.z, n.b.w, n.c.x, n.c.y, n.c.z, n.c.w);
+    const bool hl = tL < ts.tmax, hr = tR < ts.tmax;
+    const uint32_t L = (uint32_t) n.d.x, R = (uint32_t) n.d.y;
+    if (hl && hr) {
+        const bool lf = tL <= tR;
+        st.push(lf ? R : L, 0.f);
+        ts.cur = lf ? L : R;
+        return false;
+    }
+    if (hl || hr) {
+        ts.cur = hl ? L : R;
+        return false;
+    }
+    return anyPop(ts, st);
+}
+
+// One step of BVHAccel::IntersectP; returns true when finished, with
+// ts.h.prim = 1 if occluded, -1 otherwise.
+PRR_DEV bool anyStep(const SceneView &s, TravState &ts, LaneStack &st) {
+    return refIsLeaf(ts.cur) ? anyLeaf(s, ts, st) : anyInterior(s, ts, st);
+}
+
+// While-while iteration of IntersectP (see closestIter).
+PRR_DEV bool anyIter(const SceneView &s, TravState &ts, LaneStack &st) {
+    while (!refIsLeaf(ts.cur))
+        if (anyInterior(s, ts, st)) return true;
+    return anyLeaf(s, ts, st);
 }
 
 // Wave-batched work fetch: lanes with need == true get consecutive indices

@@ -196,7 +196,9 @@ __global__ void kCopyResults(PathBufs pb, int64_t n, int64_t first, float *L) {
 // wave once `refillAt` lanes are idle (64: only when the whole wave is idle,
 // which keeps a wave on 64 consecutive, mostly coherent queue entries).
 
-__global__ __launch_bounds__(kTraceBlock) void kExtend(SceneView s, PathBufs pb, int bounce, uint2 *spill, int refillAt, int stackDepth) {
+template <bool WW>
+__global__ __launch_bounds__(kTraceBlock) void kExtend(SceneView s, PathBufs pb, int bounce, uint2 *spill, int refillAt,
+                                                       int stackDepth) {
     extern __shared__ uint2 stk[];
     const int q = bounce & 1;
     const int64_t n = pb.qcnt[bounce];
@@ -230,14 +232,16 @@ __global__ __launch_bounds__(kTraceBlock) void kExtend(SceneView s, PathBufs pb,
             }
         }
         if (__ballot(idx >= 0 || !exhausted) == 0) break;
-        if (idx >= 0 && closestStep(s, ts, st)) {
+        if (idx >= 0 && (WW ? closestIter(s, ts, st) : closestStep(s, ts, st))) {
             pb.hit[idx] = make_float4(ts.h.t, ts.h.u, ts.h.v, __int_as_float(ts.h.prim));
             idx = -1;
         }
     }
 }
 
-__global__ __launch_bounds__(kTraceBlock) void kShadow(SceneView s, PathBufs pb, int bounce, uint2 *spill, int refillAt, int stackDepth) {
+template <bool WW>
+__global__ __launch_bounds__(kTraceBlock) void kShadow(SceneView s, PathBufs pb, int bounce, uint2 *spill, int refillAt,
+                                                       int stackDepth) {
     extern __shared__ uint2 stk[];
     const int64_t n = pb.sqcnt[bounce];
     uint32_t *fetch = &pb.fetch[2 * bounce + 1];
@@ -273,7 +277,7 @@ __global__ __launch_bounds__(kTraceBlock) void kShadow(SceneView s, PathBufs pb,
             }
         }
         if (__ballot(idx >= 0 || !exhausted) == 0) break;
-        if (idx >= 0 && anyStep(s, ts, st)) {
+        if (idx >= 0 && (WW ? anyIter(s, ts, st) : anyStep(s, ts, st))) {
             pb.vis[(int64_t) (sk >> 30) * pb.slots + (sk & 0x3FFFFFFFu)] = ts.h.prim > 0 ? 0 : 1;
             idx = -1;
         }
@@ -350,7 +354,7 @@ PRR_DEV void sampleLight(const SceneView &s, Pcg &rng, V3 &pos, V3 &nrm, int &ma
     }
 }
 
-__global__ __launch_bounds__(kShadeBlock) void kShade(SceneView s, PathBufs pb, ChunkInfo ci, int bounce) {
+__device__ __forceinline__ void shadeImpl(const SceneView &s, const PathBufs &pb, const ChunkInfo &ci, int bounce) {
     const int q = bounce & 1;
     const int64_t n = pb.qcnt[bounce];
     const int64_t stride = (int64_t) gridDim.x * kShadeBlock;
@@ -649,6 +653,15 @@ __global__ __launch_bounds__(kShadeBlock) void kShade(SceneView s, PathBufs pb, 
     }
 }
 
+// kShade: one shading pass; kShadeO4 is the same code compiled for at least
+// 4 waves per SIMD (fewer registers, more latency hiding).
+__global__ __launch_bounds__(kShadeBlock) void kShade(SceneView s, PathBufs pb, ChunkInfo ci, int bounce) {
+    shadeImpl(s, pb, ci, bounce);
+}
+__global__ __launch_bounds__(kShadeBlock, 4) void kShadeO4(SceneView s, PathBufs pb, ChunkInfo ci, int bounce) {
+    shadeImpl(s, pb, ci, bounce);
+}
+
 // Radiance-cache training targets of records [r0, r1): the radiance the
 // path gathered after the vertex, per unit throughput.
 __global__ void kCacheTargets(PathBufs pb, CacheRec cr, uint32_t r0, uint32_t r1, float *target) {
@@ -758,20 +771,31 @@ hipError_t launchCopyResults(const PathBufs &pb, int64_t n, int64_t first, float
     return hipGetLastError();
 }
 hipError_t launchExtend(const SceneView &s, const PathBufs &pb, int bounce, uint2 *spill, int traceGrid,
-                        int refillAt, int stackDepth, hipStream_t st) {
-    hipLaunchKernelGGL(kExtend, dim3(traceGrid), dim3(kTraceBlock), stackDepth * kTraceBlock * sizeof(uint2), st, s,
-                       pb, bounce, spill, refillAt, stackDepth);
+                        int refillAt, int stackDepth, int whileWhile, hipStream_t st) {
+    if (whileWhile)
+        hipLaunchKernelGGL(kExtend<true>, dim3(traceGrid), dim3(kTraceBlock), stackDepth * kTraceBlock * sizeof(uint2),
+                           st, s, pb, bounce, spill, refillAt, stackDepth);
+    else
+        hipLaunchKernelGGL(kExtend<false>, dim3(traceGrid), dim3(kTraceBlock), stackDepth * kTraceBlock * sizeof(uint2),
+                           st, s, pb, bounce, spill, refillAt, stackDepth);
     return hipGetLastError();
 }
 hipError_t launchShadow(const SceneView &s, const PathBufs &pb, int bounce, uint2 *spill, int traceGrid,
-                        int refillAt, int stackDepth, hipStream_t st) {
-    hipLaunchKernelGGL(kShadow, dim3(traceGrid), dim3(kTraceBlock), stackDepth * kTraceBlock * sizeof(uint2), st, s,
-                       pb, bounce, spill, refillAt, stackDepth);
+                        int refillAt, int stackDepth, int whileWhile, hipStream_t st) {
+    if (whileWhile)
+        hipLaunchKernelGGL(kShadow<true>, dim3(traceGrid), dim3(kTraceBlock), stackDepth * kTraceBlock * sizeof(uint2),
+                           st, s, pb, bounce, spill, refillAt, stackDepth);
+    else
+        hipLaunchKernelGGL(kShadow<false>, dim3(traceGrid), dim3(kTraceBlock), stackDepth * kTraceBlock * sizeof(uint2),
+                           st, s, pb, bounce, spill, refillAt, stackDepth);
     return hipGetLastError();
 }
 hipError_t launchShade(const SceneView &s, const PathBufs &pb, const ChunkInfo &ci, int bounce, int grid,
-                       hipStream_t st) {
-    hipLaunchKernelGGL(kShade, dim3(grid), dim3(kShadeBlock), 0, st, s, pb, ci, bounce);
+                       bool occ4, hipStream_t st) {
+    if (occ4)
+        hipLaunchKernelGGL(kShadeO4, dim3(grid), dim3(kShadeBlock), 0, st, s, pb, ci, bounce);
+    else
+        hipLaunchKernelGGL(kShade, dim3(grid), dim3(kShadeBlock), 0, st, s, pb, ci, bounce);
     return hipGetLastError();
 }
 hipError_t launchAccumulate(const PathBufs &pb, const ChunkInfo &ci, float4 *fb, hipStream_t st) {

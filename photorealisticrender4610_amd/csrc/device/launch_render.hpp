@@ -12,11 +12,11 @@ hipError_t launchRaygenRays(const PathBufs &pb, int64_t n, const float *org, con
                             uint64_t seed, int grid, hipStream_t st);
 hipError_t launchCopyResults(const PathBufs &pb, int64_t n, int64_t first, float *L, hipStream_t st);
 hipError_t launchExtend(const SceneView &s, const PathBufs &pb, int bounce, uint2 *spill, int traceGrid,
-                        int refillAt, int stackDepth, hipStream_t st);
+                        int refillAt, int stackDepth, int whileWhile, hipStream_t st);
 hipError_t launchShadow(const SceneView &s, const PathBufs &pb, int bounce, uint2 *spill, int traceGrid,
-                        int refillAt, int stackDepth, hipStream_t st);
+                        int refillAt, int stackDepth, int whileWhile, hipStream_t st);
 hipError_t launchShade(const SceneView &s, const PathBufs &pb, const ChunkInfo &ci, int bounce, int grid,
-                       hipStream_t st);
+                       bool occ4, hipStream_t st);
 hipError_t launchAccumulate(const PathBufs &pb, const ChunkInfo &ci, float4 *fb, hipStream_t st);
 hipError_t launchAccumulateAdaptive(const PathBufs &pb, const ChunkInfo &ci, float4 *fb, const AdaptiveBufs &ab,
                                     int minSpp, float thr, hipStream_t st);

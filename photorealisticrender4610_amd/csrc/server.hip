@@ -323,7 +323,7 @@ int prr_server_create(prr_context *c, const prr_scene *const *scenes, const int3
     if (!c || !scenes || !max_depth || !indirect_clamp || !p || !out || num_scenes <= 0)
         return fail(c, PRR_ERR_INVALID, "NULL argument");
     *out = nullptr;
-    if (p->path_slots <= 0 || p->path_slots > kMaxSlots || p->hbm_budget < 0 || p->policy < PRR_SERVE_AGED ||
+    if (p->path_slots <= 0 || p->path_slots > c->maxSlots || p->hbm_budget < 0 || p->policy < PRR_SERVE_AGED ||
         p->policy > PRR_SERVE_FAIR)
         return fail(c, PRR_ERR_INVALID, "bad server parameters");
     int maxDepth = 1;

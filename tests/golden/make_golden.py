@@ -18,8 +18,9 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(os.path.dirname(HERE))
 sys.path.insert(0, ROOT)
 sys.path.insert(0, HERE)
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
 
-from bundle import read_bundle, write_bundle  # noqa: E402
+from bundle import read_bundle, write_bundle  # noqa: E402  (oracle/bundle.py)
 import testscenes as TS  # noqa: E402
 from photorealisticrender4610_amd import scenes as S  # noqa: E402
 
