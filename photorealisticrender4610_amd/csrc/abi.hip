@@ -155,7 +155,7 @@ int runBounces(prr_context *c, const SceneView &view, const ChunkInfo &ci, Timin
         if (b > 0)
             TIMED(1, launchShadow(view, c->pb, b - 1, c->spill, c->traceBlocks, c->refillAt, c->stackDepth,
                                   c->whileWhile, st));
-        TIMED(2, launchShade(view, c->pb, ci, b, c->shadeBlocks, c->shadeOcc4, st));
+        TIMED(2, launchShade(view, c->pb, ci, b, c->shadeBlocks, c->shadeWaves, st));
     }
 #undef TIMED
     return PRR_OK;
@@ -254,8 +254,10 @@ int prr_create(int device, prr_context **out) {
     c->stream = c->own;
     c->numCU = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 1;
     // persistent traversal grid: 8 blocks of 256 lanes per CU (with 8-entry
-    // LDS stacks, 16 KB per block); PRR_TRACE_BLOCKS_PER_CU, PRR_STACK_LDS,
-    // PRR_REFILL_AT, PRR_WHILE_WHILE and PRR_SHADE_OCC4 override the measured
+    // LDS stacks, 16 KB per block), a wave refills once 40 of its lanes are
+    // idle, at most 8 interior steps per work check (measured on the 1M-tri
+    // scene); PRR_TRACE_BLOCKS_PER_CU, PRR_STACK_LDS,
+    // PRR_REFILL_AT, PRR_WHILE_WHILE and PRR_SHADE_WAVES override the measured
     // defaults for tuning sweeps
     int tbpc = 8;
     if (const char *ev = getenv("PRR_TRACE_BLOCKS_PER_CU")) tbpc = std::max(1, std::min(16, atoi(ev)));
@@ -269,8 +271,8 @@ int prr_create(int device, prr_context **out) {
         if (const char *ev = getenv("PRR_MAX_SLOTS"))
             c->maxSlots = std::max<int64_t>(1 << 16, std::min<int64_t>(kMaxSlots, atoll(ev)));
     }
-    if (const char *ev = getenv("PRR_WHILE_WHILE")) c->whileWhile = atoi(ev) != 0;
-    if (const char *ev = getenv("PRR_SHADE_OCC4")) c->shadeOcc4 = atoi(ev) != 0;
+    if (const char *ev = getenv("PRR_WHILE_WHILE")) c->whileWhile = std::max(0, atoi(ev));
+    if (const char *ev = getenv("PRR_SHADE_WAVES")) c->shadeWaves = std::max(1, std::min(5, atoi(ev)));
     if (const char *ev = getenv("PRR_STACK_LDS")) {  // LDS stack entries per lane: 2, 4, 8, 16 or 32
         const int v = atoi(ev);
         if (v == 2 || v == 4 || v == 8 || v == 16 || v == 32) c->stackDepth = v;
@@ -345,6 +347,8 @@ int prr_scene_create(prr_context *c, const prr_scene_desc *d, prr_scene **out) {
     UP(v.env_px, h.envPx);
     UP(v.env_cdf, h.envCdf);
     UP(v.env_marg, h.envMarg);
+    UP(v.env_gm, h.envGuideM);
+    UP(v.env_gc, h.envGuideC);
 #undef UP
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) {
@@ -360,6 +364,7 @@ int prr_scene_create(prr_context *c, const prr_scene_desc *d, prr_scene **out) {
     v.env_w = h.envW;
     v.env_h = h.envH;
     v.env_total = h.envTotal;
+    v.env_gk = h.envGuideK;
     for (int k = 0; k < 3; k++) {
         v.background[k] = h.background[k];
         v.point_pos[k] = h.pointPos[k];

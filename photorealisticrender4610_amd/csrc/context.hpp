@@ -19,10 +19,11 @@ struct prr_context {
     std::string err;
     int numCU = 1;
     int traceBlocks = 1, shadeBlocks = 1;
-    int refillAt = 32;
+    int refillAt = 40;
     int stackDepth = 8;
-    int whileWhile = 1;     // traversal iteration: 1 interior nodes then a leaf, 0 one node per work check
-    bool shadeOcc4 = true;  // shading kernel compiled for 4 waves per SIMD
+    int whileWhile = 8;     // traversal iteration: 0 one node per work check, 1 interior nodes then a leaf,
+                            // n > 1 the same with at most n interior steps per check
+    int shadeWaves = 4;     // shading kernel compiled for at least this many waves per SIMD (1, 4, 5)
     int64_t maxSlots = 1 << 24;  // paths per chunk (set at creation from free HBM)
     uint2 *spill = nullptr;
     int spillLanes = 0;

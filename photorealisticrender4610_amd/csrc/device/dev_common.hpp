@@ -168,6 +168,9 @@ struct SceneView {
     const float4 *env_px;     // W*H (rgb, lum)
     const float *env_cdf;     // W*H
     const float *env_marg;    // H
+    const int *env_gm;        // marginal guide: env_gk + 1 entries
+    const int *env_gc;        // conditional guides: env_gk + 1 per row
+    int env_gk;               // guide buckets (power of two), 0: full searches
     uint32_t root;            // root child reference
     int num_tris;
     int num_emitters;
@@ -461,10 +464,35 @@ PRR_DEV int lowerBound(const float *a, int n, float x) {
     }
     return lo;
 }
+// lower_bound(a, n, x) for x in [0, 1) restricted to the guide bucket of x
+// (same index: see buildEnvGuides on the host).
+PRR_DEV int guidedLowerBound(const float *a, float x, const int *g, int K) {
+    const int k = min(max((int) (x * (float) K), 0), K - 1);
+    int lo = g[k], cnt = g[k + 1] - lo;
+    while (cnt > 0) {
+        int step = cnt >> 1;
+        int mid = lo + step;
+        if (a[mid] < x) {
+            lo = mid + 1;
+            cnt -= step + 1;
+        } else {
+            cnt = step;
+        }
+    }
+    return lo;
+}
 PRR_DEV V3 envImportanceSample(const SceneView &s, float r1, float r2, float &pdf) {  // EnvMap.hpp:145-169
     const int W = s.env_w, H = s.env_h;
-    int row = min(lowerBound(s.env_marg, H, r1), H - 1);
-    int col = min(lowerBound(s.env_cdf + (size_t) row * W, W, r2), W - 1);
+    int row, col;
+    if (s.env_gk > 0 && r1 < 1.f && r2 < 1.f && r1 >= 0.f && r2 >= 0.f) {
+        row = min(guidedLowerBound(s.env_marg, r1, s.env_gm, s.env_gk), H - 1);
+        col = min(guidedLowerBound(s.env_cdf + (size_t) row * W, r2, s.env_gc + (size_t) row * (s.env_gk + 1),
+                                   s.env_gk),
+                  W - 1);
+    } else {
+        row = min(lowerBound(s.env_marg, H, r1), H - 1);
+        col = min(lowerBound(s.env_cdf + (size_t) row * W, W, r2), W - 1);
+    }
     float u = (col + 0.5f) / (float) W;
     float v = (row + 0.5f) / (float) H;
     V3 dir = uvToDir(u, v);

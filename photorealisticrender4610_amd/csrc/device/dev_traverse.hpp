@@ -379,9 +379,14 @@ PRR_DEV bool closestStep(const SceneView &s, TravState &ts, LaneStack &st) {
 // lane stands on a leaf, then that leaf, so a wave tests its leaves
 // together. Each ray visits the same nodes in the same order as with
 // closestStep.
-PRR_DEV bool closestIter(const SceneView &s, TravState &ts, LaneStack &st) {
-    while (!refIsLeaf(ts.cur))
+// At most cap interior steps per call (cap <= 0: no limit); the descent then
+// resumes at the next call.
+PRR_DEV bool closestIter(const SceneView &s, TravState &ts, LaneStack &st, int cap) {
+    int k = 0;
+    while (!refIsLeaf(ts.cur)) {
         if (closestInterior(s, ts, st)) return true;
+        if (++k == cap) return false;
+    }
     return closestLeaf(s, ts, st);
 }
 
@@ -450,9 +455,12 @@ PRR_DEV bool anyStep(const SceneView &s, TravState &ts, LaneStack &st) {
 }
 
 // While-while iteration of IntersectP (see closestIter).
-PRR_DEV bool anyIter(const SceneView &s, TravState &ts, LaneStack &st) {
-    while (!refIsLeaf(ts.cur))
+PRR_DEV bool anyIter(const SceneView &s, TravState &ts, LaneStack &st, int cap) {
+    int k = 0;
+    while (!refIsLeaf(ts.cur)) {
         if (anyInterior(s, ts, st)) return true;
+        if (++k == cap) return false;
+    }
     return anyLeaf(s, ts, st);
 }
 

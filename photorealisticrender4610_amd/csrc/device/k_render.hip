@@ -198,7 +198,7 @@ __global__ void kCopyResults(PathBufs pb, int64_t n, int64_t first, float *L) {
 
 template <bool WW>
 __global__ __launch_bounds__(kTraceBlock) void kExtend(SceneView s, PathBufs pb, int bounce, uint2 *spill, int refillAt,
-                                                       int stackDepth) {
+                                                       int stackDepth, int wwCap) {
     extern __shared__ uint2 stk[];
     const int q = bounce & 1;
     const int64_t n = pb.qcnt[bounce];
@@ -232,7 +232,7 @@ __global__ __launch_bounds__(kTraceBlock) void kExtend(SceneView s, PathBufs pb,
             }
         }
         if (__ballot(idx >= 0 || !exhausted) == 0) break;
-        if (idx >= 0 && (WW ? closestIter(s, ts, st) : closestStep(s, ts, st))) {
+        if (idx >= 0 && (WW ? closestIter(s, ts, st, wwCap) : closestStep(s, ts, st))) {
             pb.hit[idx] = make_float4(ts.h.t, ts.h.u, ts.h.v, __int_as_float(ts.h.prim));
             idx = -1;
         }
@@ -241,7 +241,7 @@ __global__ __launch_bounds__(kTraceBlock) void kExtend(SceneView s, PathBufs pb,
 
 template <bool WW>
 __global__ __launch_bounds__(kTraceBlock) void kShadow(SceneView s, PathBufs pb, int bounce, uint2 *spill, int refillAt,
-                                                       int stackDepth) {
+                                                       int stackDepth, int wwCap) {
     extern __shared__ uint2 stk[];
     const int64_t n = pb.sqcnt[bounce];
     uint32_t *fetch = &pb.fetch[2 * bounce + 1];
@@ -277,7 +277,7 @@ __global__ __launch_bounds__(kTraceBlock) void kShadow(SceneView s, PathBufs pb,
             }
         }
         if (__ballot(idx >= 0 || !exhausted) == 0) break;
-        if (idx >= 0 && (WW ? anyIter(s, ts, st) : anyStep(s, ts, st))) {
+        if (idx >= 0 && (WW ? anyIter(s, ts, st, wwCap) : anyStep(s, ts, st))) {
             pb.vis[(int64_t) (sk >> 30) * pb.slots + (sk & 0x3FFFFFFFu)] = ts.h.prim > 0 ? 0 : 1;
             idx = -1;
         }
@@ -354,6 +354,10 @@ PRR_DEV void sampleLight(const SceneView &s, Pcg &rng, V3 &pos, V3 &nrm, int &ma
     }
 }
 
+// PHOTON / CACHE: whether the caustic photon gather and the radiance-cache
+// query / training code are compiled in (the kernel's registers are those of
+// its heaviest branch, so renders without them use a leaner variant).
+template <bool PHOTON, bool CACHE>
 __device__ __forceinline__ void shadeImpl(const SceneView &s, const PathBufs &pb, const ChunkInfo &ci, int bounce) {
     const int q = bounce & 1;
     const int64_t n = pb.qcnt[bounce];
@@ -482,7 +486,7 @@ __device__ __forceinline__ void shadeImpl(const SceneView &s, const PathBufs &pb
                         const V3 wo = -d;
                         const CacheRec &cr = ci.cache;
                         bool cacheQuery = false;
-                        if (cr.query | cr.train) {
+                        if (CACHE && (cr.query | cr.train)) {
                             if (cr.query && !(fl & F_TRAIN) && bounce >= cr.queryBounce) {
                                 // the path ends here; the cache adds beta * L_out(P, wo)
                                 cacheQuery = true;
@@ -573,7 +577,7 @@ __device__ __forceinline__ void shadeImpl(const SceneView &s, const PathBufs &pb
                         }
                         // caustic photon estimate at camera / delta-reached vertices
                         // (build extension; no draws)
-                        if (s.ph_count > 0 && (bounce == 0 || spec)) {
+                        if (PHOTON && s.ph_count > 0 && (bounce == 0 || spec)) {
                             V3 c = beta * photonRadiance(s, g.P, wo, fr, lb);
                             c = ind ? clampInd(c, ci.clampInd) : c;
                             pb.pend[3 * pb.slots + slot] = st4(c, 0.f);
@@ -653,13 +657,11 @@ __device__ __forceinline__ void shadeImpl(const SceneView &s, const PathBufs &pb
     }
 }
 
-// kShade: one shading pass; kShadeO4 is the same code compiled for at least
-// 4 waves per SIMD (fewer registers, more latency hiding).
-__global__ __launch_bounds__(kShadeBlock) void kShade(SceneView s, PathBufs pb, ChunkInfo ci, int bounce) {
-    shadeImpl(s, pb, ci, bounce);
-}
-__global__ __launch_bounds__(kShadeBlock, 4) void kShadeO4(SceneView s, PathBufs pb, ChunkInfo ci, int bounce) {
-    shadeImpl(s, pb, ci, bounce);
+// kShade: one shading pass, compiled for at least W waves per SIMD (fewer
+// registers, spilled if need be, for more latency hiding).
+template <int W, bool PHOTON, bool CACHE>
+__global__ __launch_bounds__(kShadeBlock, W) void kShade(SceneView s, PathBufs pb, ChunkInfo ci, int bounce) {
+    shadeImpl<PHOTON, CACHE>(s, pb, ci, bounce);
 }
 
 // Radiance-cache training targets of records [r0, r1): the radiance the
@@ -772,30 +774,46 @@ hipError_t launchCopyResults(const PathBufs &pb, int64_t n, int64_t first, float
 }
 hipError_t launchExtend(const SceneView &s, const PathBufs &pb, int bounce, uint2 *spill, int traceGrid,
                         int refillAt, int stackDepth, int whileWhile, hipStream_t st) {
+    const int wwCap = whileWhile > 1 ? whileWhile : 0;
     if (whileWhile)
         hipLaunchKernelGGL(kExtend<true>, dim3(traceGrid), dim3(kTraceBlock), stackDepth * kTraceBlock * sizeof(uint2),
-                           st, s, pb, bounce, spill, refillAt, stackDepth);
+                           st, s, pb, bounce, spill, refillAt, stackDepth, wwCap);
     else
         hipLaunchKernelGGL(kExtend<false>, dim3(traceGrid), dim3(kTraceBlock), stackDepth * kTraceBlock * sizeof(uint2),
-                           st, s, pb, bounce, spill, refillAt, stackDepth);
+                           st, s, pb, bounce, spill, refillAt, stackDepth, wwCap);
     return hipGetLastError();
 }
 hipError_t launchShadow(const SceneView &s, const PathBufs &pb, int bounce, uint2 *spill, int traceGrid,
                         int refillAt, int stackDepth, int whileWhile, hipStream_t st) {
+    const int wwCap = whileWhile > 1 ? whileWhile : 0;
     if (whileWhile)
         hipLaunchKernelGGL(kShadow<true>, dim3(traceGrid), dim3(kTraceBlock), stackDepth * kTraceBlock * sizeof(uint2),
-                           st, s, pb, bounce, spill, refillAt, stackDepth);
+                           st, s, pb, bounce, spill, refillAt, stackDepth, wwCap);
     else
         hipLaunchKernelGGL(kShadow<false>, dim3(traceGrid), dim3(kTraceBlock), stackDepth * kTraceBlock * sizeof(uint2),
-                           st, s, pb, bounce, spill, refillAt, stackDepth);
+                           st, s, pb, bounce, spill, refillAt, stackDepth, wwCap);
     return hipGetLastError();
 }
-hipError_t launchShade(const SceneView &s, const PathBufs &pb, const ChunkInfo &ci, int bounce, int grid,
-                       bool occ4, hipStream_t st) {
-    if (occ4)
-        hipLaunchKernelGGL(kShadeO4, dim3(grid), dim3(kShadeBlock), 0, st, s, pb, ci, bounce);
+template <bool P, bool C>
+static void shadeLaunch(const SceneView &s, const PathBufs &pb, const ChunkInfo &ci, int bounce, int grid, int waves,
+                        hipStream_t st) {
+    if (waves >= 5)
+        hipLaunchKernelGGL((kShade<5, P, C>), dim3(grid), dim3(kShadeBlock), 0, st, s, pb, ci, bounce);
+    else if (waves == 4)
+        hipLaunchKernelGGL((kShade<4, P, C>), dim3(grid), dim3(kShadeBlock), 0, st, s, pb, ci, bounce);
     else
-        hipLaunchKernelGGL(kShade, dim3(grid), dim3(kShadeBlock), 0, st, s, pb, ci, bounce);
+        hipLaunchKernelGGL((kShade<1, P, C>), dim3(grid), dim3(kShadeBlock), 0, st, s, pb, ci, bounce);
+}
+
+hipError_t launchShade(const SceneView &s, const PathBufs &pb, const ChunkInfo &ci, int bounce, int grid, int waves,
+                       hipStream_t st) {
+    const bool photon = s.ph_count > 0, cache = ci.cache.query || ci.cache.train;
+    if (photon)
+        cache ? shadeLaunch<true, true>(s, pb, ci, bounce, grid, waves, st)
+              : shadeLaunch<true, false>(s, pb, ci, bounce, grid, waves, st);
+    else
+        cache ? shadeLaunch<false, true>(s, pb, ci, bounce, grid, waves, st)
+              : shadeLaunch<false, false>(s, pb, ci, bounce, grid, waves, st);
     return hipGetLastError();
 }
 hipError_t launchAccumulate(const PathBufs &pb, const ChunkInfo &ci, float4 *fb, hipStream_t st) {

@@ -11,12 +11,15 @@ hipError_t launchRaygen(const SceneView &s, const PathBufs &pb, const ChunkInfo 
 hipError_t launchRaygenRays(const PathBufs &pb, int64_t n, const float *org, const float *dir, int64_t first,
                             uint64_t seed, int grid, hipStream_t st);
 hipError_t launchCopyResults(const PathBufs &pb, int64_t n, int64_t first, float *L, hipStream_t st);
+// whileWhile: 0 one node per work check; 1 interior nodes then a leaf; n > 1
+// the same with at most n interior steps per work check
 hipError_t launchExtend(const SceneView &s, const PathBufs &pb, int bounce, uint2 *spill, int traceGrid,
                         int refillAt, int stackDepth, int whileWhile, hipStream_t st);
 hipError_t launchShadow(const SceneView &s, const PathBufs &pb, int bounce, uint2 *spill, int traceGrid,
                         int refillAt, int stackDepth, int whileWhile, hipStream_t st);
-hipError_t launchShade(const SceneView &s, const PathBufs &pb, const ChunkInfo &ci, int bounce, int grid,
-                       bool occ4, hipStream_t st);
+// waves: minimum waves per SIMD the shading kernel is compiled for (1, 4 or 5)
+hipError_t launchShade(const SceneView &s, const PathBufs &pb, const ChunkInfo &ci, int bounce, int grid, int waves,
+                       hipStream_t st);
 hipError_t launchAccumulate(const PathBufs &pb, const ChunkInfo &ci, float4 *fb, hipStream_t st);
 hipError_t launchAccumulateAdaptive(const PathBufs &pb, const ChunkInfo &ci, float4 *fb, const AdaptiveBufs &ab,
                                     int minSpp, float thr, hipStream_t st);

@@ -582,11 +582,16 @@ bool buildHostScene(const prr_scene_desc *d, HostScene &S, std::string &err) {
         S.envTotal = t;
         if (t > 0.f)
             for (float &v : S.envMarg) v /= t;
+        buildEnvGuides(S);
     } else {
         S.envPx.assign(1, make_float4(0, 0, 0, 0));
         S.envCdf.assign(1, 0.f);
         S.envMarg.assign(1, 0.f);
         S.envTotal = 0.f;
+    }
+    if (S.envGuideK == 0) {
+        S.envGuideM.assign(1, 0);
+        S.envGuideC.assign(1, 0);
     }
 
     S.cam = makeCamView(d->camera);
@@ -598,6 +603,34 @@ bool buildHostScene(const prr_scene_desc *d, HostScene &S, std::string &err) {
             S.pointInt[k] = d->point_lights[0].intensity[k];
         }
     return true;
+}
+
+// Guide tables for the two lower_bound searches of EnvMap::importanceSample
+// (EnvMap.hpp:145-169). For a non-decreasing CDF a and K a power of two, a
+// draw x in [0, 1) lies in bucket k = floor(x K) (exact: x K only scales the
+// exponent), and lower_bound(a, x) lies in [g[k], g[k + 1]] with g[k] =
+// lower_bound(a, k / K): the device searches that range only and finds the
+// same index. A CDF that is not non-decreasing (NaN, negative luminance)
+// leaves the guides off.
+void buildEnvGuides(HostScene &S) {
+    const int K = 256, W = S.envW, H = S.envH;
+    auto monotone = [](const float *a, int n) {
+        for (int i = 0; i + 1 < n; i++)
+            if (!(a[i] <= a[i + 1])) return false;
+        return n == 0 || a[0] == a[0];
+    };
+    if (!monotone(S.envMarg.data(), H)) return;
+    for (int r = 0; r < H; r++)
+        if (!monotone(S.envCdf.data() + (size_t) r * W, W)) return;
+    auto guides = [&](const float *a, int n, int32_t *g) {
+        for (int k = 0; k <= K; k++)
+            g[k] = (int32_t) (std::lower_bound(a, a + n, (float) k / (float) K) - a);
+    };
+    S.envGuideM.assign(K + 1, 0);
+    S.envGuideC.assign((size_t) H * (K + 1), 0);
+    guides(S.envMarg.data(), H, S.envGuideM.data());
+    for (int r = 0; r < H; r++) guides(S.envCdf.data() + (size_t) r * W, W, S.envGuideC.data() + (size_t) r * (K + 1));
+    S.envGuideK = K;
 }
 
 CamView makeCamView(const prr_camera_desc &c) {

@@ -69,6 +69,10 @@ struct HostScene {
     std::vector<float> envCdf, envMarg;
     float envTotal = 0.f;
     int envW = 0, envH = 0;
+    // guide tables: entry k of a CDF = its lower_bound of k / envGuideK
+    // (marginal: K + 1 entries; conditional: K + 1 per row); K = 0: none
+    int envGuideK = 0;
+    std::vector<int32_t> envGuideM, envGuideC;
     uint32_t root = 0;
     float rootBox[6] = {0, 0, 0, 0, 0, 0};
     int maxDepth = 0;
@@ -83,6 +87,8 @@ struct HostScene {
 // Validates the descriptor and builds everything. Returns false with a
 // message on invalid input.
 bool buildHostScene(const prr_scene_desc *d, HostScene &out, std::string &err);
+// Guide tables of the environment CDFs (sets envGuideK, or leaves it 0).
+void buildEnvGuides(HostScene &S);
 // Camera::init (Camera.hpp:43-56): basis, focus distance, tan(fovh / 2).
 CamView makeCamView(const prr_camera_desc &c);
 
