@@ -365,6 +365,7 @@ __device__ __forceinline__ void shadeImpl(const SceneView &s, const PathBufs &pb
     const int64_t i0 = (int64_t) blockIdx.x * kShadeBlock + threadIdx.x;
     // uniform trip count per wave so every lane reaches the appends
     const int64_t waveBase = i0 - (threadIdx.x & 63);
+    uint32_t nRays = 0, nShadow = 0;
     for (int64_t wb = waveBase; wb < n; wb += stride) {
         const int64_t i = wb + (threadIdx.x & 63);
         const bool active = i < n;
@@ -625,35 +626,48 @@ __device__ __forceinline__ void shadeImpl(const SceneView &s, const PathBufs &pb
             pb.qOrg[q ^ 1][qi] = termEntry ? make_float4(0.f, 0.f, 0.f, -1.f) : st4(newO, 0.f);
             pb.qDir[q ^ 1][qi] = termEntry ? make_float4(0.f, 0.f, 0.f, 0.f) : st4(newD, 0.f);
         }
-        const uint32_t ia = waveAppend(&pb.sqcnt[bounce], emitA);
-        if (emitA) {
-            pb.sqSlot[ia] = slot;
-            pb.sqOrg[ia] = st4(sOA, tA);
-            pb.sqDir[ia] = st4(sDA, 0.f);
-        }
-        const uint32_t ie = waveAppend(&pb.sqcnt[bounce], emitE);
-        if (emitE) {
-            pb.sqSlot[ie] = slot | (1u << 30);
-            pb.sqOrg[ie] = st4(sOE, __builtin_inff());
-            pb.sqDir[ie] = st4(sDE, 0.f);
-        }
-        const uint32_t ip = waveAppend(&pb.sqcnt[bounce], emitP);
-        if (emitP) {
-            pb.sqSlot[ip] = slot | (2u << 30);
-            pb.sqOrg[ip] = st4(sOP, tP);
-            pb.sqDir[ip] = st4(sDP, 0.f);
-        }
+        // the wave's shadow rays (area, env, point) take one atomic together
         {
-            const uint64_t mr = __ballot(emitRay && !termEntry);
-            const uint64_t ms = __ballot(emitA);
-            const uint64_t me = __ballot(emitE);
-            const uint64_t mp = __ballot(emitP);
-            if ((threadIdx.x & 63) == 0) {
-                if (mr) atomicAdd(&pb.stats[0], (unsigned long long) __popcll(mr));
-                const int nsh = __popcll(ms) + __popcll(me) + __popcll(mp);
-                if (nsh) atomicAdd(&pb.stats[1], (unsigned long long) nsh);
+            const uint64_t mA = __ballot(emitA), mE = __ballot(emitE), mP = __ballot(emitP);
+            const uint32_t nA = (uint32_t) __popcll(mA), nE = (uint32_t) __popcll(mE);
+            const uint32_t total = nA + nE + (uint32_t) __popcll(mP);
+            uint32_t base = 0;
+            if (total) {
+                if (laneId() == 0) base = atomicAdd(&pb.sqcnt[bounce], total);
+                base = __shfl(base, 0);
+            }
+            const uint64_t below = (1ull << laneId()) - 1ull;
+            if (emitA) {
+                const uint32_t ia = base + (uint32_t) __popcll(mA & below);
+                pb.sqSlot[ia] = slot;
+                pb.sqOrg[ia] = st4(sOA, tA);
+                pb.sqDir[ia] = st4(sDA, 0.f);
+            }
+            if (emitE) {
+                const uint32_t ie = base + nA + (uint32_t) __popcll(mE & below);
+                pb.sqSlot[ie] = slot | (1u << 30);
+                pb.sqOrg[ie] = st4(sOE, __builtin_inff());
+                pb.sqDir[ie] = st4(sDE, 0.f);
+            }
+            if (emitP) {
+                const uint32_t ip = base + nA + nE + (uint32_t) __popcll(mP & below);
+                pb.sqSlot[ip] = slot | (2u << 30);
+                pb.sqOrg[ip] = st4(sOP, tP);
+                pb.sqDir[ip] = st4(sDP, 0.f);
             }
         }
+        nRays += (emitRay && !termEntry) ? 1u : 0u;
+        nShadow += (emitA ? 1u : 0u) + (emitE ? 1u : 0u) + (emitP ? 1u : 0u);
+    }
+    // query counts: one atomic per wave and launch (contended device-scope
+    // atomics per wave-iteration would serialise the kernel)
+    for (int o = 32; o > 0; o >>= 1) {
+        nRays += __shfl_xor(nRays, o);
+        nShadow += __shfl_xor(nShadow, o);
+    }
+    if (laneId() == 0) {
+        if (nRays) atomicAdd(&pb.stats[0], (unsigned long long) nRays);
+        if (nShadow) atomicAdd(&pb.stats[1], (unsigned long long) nShadow);
     }
 }
 
