@@ -464,18 +464,49 @@ PRR_DEV bool anyIter(const SceneView &s, TravState &ts, LaneStack &st, int cap) 
     return anyLeaf(s, ts, st);
 }
 
-// Wave-batched work fetch: lanes with need == true get consecutive indices
-// from *counter (one atomic per wave); returns -1 if none was taken.
-PRR_DEV int64_t waveFetch(uint32_t *counter, bool need) {
+// Work fetch through a wave-private range of queue entries. A wave claims a
+// run of consecutive entries from *counter with one atomic and hands them to
+// its idle lanes over several refills; the run is sized by guided
+// self-scheduling, max(64, min(maxRun, remaining / (4 * waves))), so that
+// the atomics on the one shared counter fall from one per refill to a few
+// per wave while the runs shrink to a single refill's worth near the end of
+// the queue (no wave sits on a long run while others are idle). `remaining`
+// uses the counter value the wave saw on its last claim. Entries come out in
+// queue order within a wave. Lanes with need get an index (>= n once the
+// queue is exhausted); others get -1.
+struct WaveRange {
+    int64_t lo = 0, hi = 0;  // entries [lo, hi) claimed and not yet handed out
+    int64_t seen = 0;        // counter value after this wave's last claim
+};
+
+PRR_DEV int64_t waveTake(WaveRange &r, uint32_t *counter, int64_t n, bool need, int64_t maxRun, int64_t waves) {
     const uint64_t mask = __ballot(need);
     if (mask == 0) return -1;
     const int lane = (int) __lane_id();
-    const int leader = __ffsll((unsigned long long) mask) - 1;
-    uint32_t base = 0;
-    if (lane == leader) base = atomicAdd(counter, (uint32_t) __popcll(mask));
-    base = __shfl(base, leader);
-    if (!need) return -1;
-    return (int64_t) base + __popcll(mask & ((1ull << lane) - 1ull));
+    const int64_t k = __popcll(mask);
+    const int64_t rank = __popcll(mask & ((1ull << lane) - 1ull));
+    const int64_t have = r.hi - r.lo;
+    int64_t idx;
+    if (have >= k) {
+        idx = r.lo + rank;
+        r.lo += k;
+    } else if (r.seen >= n) {  // queue exhausted when this wave last claimed
+        idx = rank < have ? r.lo + rank : n;
+        r.lo = r.hi;
+    } else {
+        int64_t run = (n - r.seen) / (4 * waves);
+        run = run < 64 ? 64 : (run > maxRun ? maxRun : run);
+        const int leader = __ffsll((unsigned long long) mask) - 1;
+        uint32_t b = 0;
+        if (lane == leader) b = atomicAdd(counter, (uint32_t) run);
+        const int64_t base = (int64_t) __shfl(b, leader);
+        r.seen = base + run;
+        const int64_t end = r.seen < n ? r.seen : n;
+        idx = rank < have ? r.lo + rank : base + (rank - have);
+        r.lo = base + (k - have);
+        r.hi = end > r.lo ? end : r.lo;
+    }
+    return need ? idx : -1;
 }
 
 }  // namespace prr

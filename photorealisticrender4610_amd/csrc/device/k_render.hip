@@ -192,13 +192,14 @@ __global__ void kCopyResults(PathBufs pb, int64_t n, int64_t first, float *L) {
     L[3 * (first + i) + 2] = r.z;
 }
 
-// Persistent traversal: idle lanes refill from the queue with one atomic per
-// wave once `refillAt` lanes are idle (64: only when the whole wave is idle,
-// which keeps a wave on 64 consecutive, mostly coherent queue entries).
+// Persistent traversal: idle lanes refill from the queue once `refillAt`
+// lanes of the wave are idle (64: only when the whole wave is idle), taking
+// consecutive entries from the wave's claimed run (waveTake), which costs an
+// atomic on the bounce's counter only when the run is used up.
 
 template <bool WW>
 __global__ __launch_bounds__(kTraceBlock) void kExtend(SceneView s, PathBufs pb, int bounce, uint2 *spill, int refillAt,
-                                                       int stackDepth, int wwCap) {
+                                                       int stackDepth, int wwCap, int fetchRun) {
     extern __shared__ uint2 stk[];
     const int q = bounce & 1;
     const int64_t n = pb.qcnt[bounce];
@@ -213,11 +214,13 @@ __global__ __launch_bounds__(kTraceBlock) void kExtend(SceneView s, PathBufs pb,
     TravState ts;
     int64_t idx = -1;
     bool exhausted = false;
+    WaveRange wr;
+    const int64_t waves = (int64_t) gridDim.x * (kTraceBlock / 64);
     while (true) {
         const bool need = idx < 0 && !exhausted;
         const uint64_t needMask = __ballot(need);
         const bool refill = __popcll(needMask) >= refillAt || __ballot(idx >= 0) == 0;
-        const int64_t got = waveFetch(fetch, need && refill);
+        const int64_t got = waveTake(wr, fetch, n, need && refill, fetchRun, waves);
         if (need && refill) {
             if (got >= n) {
                 exhausted = true;
@@ -241,7 +244,7 @@ __global__ __launch_bounds__(kTraceBlock) void kExtend(SceneView s, PathBufs pb,
 
 template <bool WW>
 __global__ __launch_bounds__(kTraceBlock) void kShadow(SceneView s, PathBufs pb, int bounce, uint2 *spill, int refillAt,
-                                                       int stackDepth, int wwCap) {
+                                                       int stackDepth, int wwCap, int fetchRun) {
     extern __shared__ uint2 stk[];
     const int64_t n = pb.sqcnt[bounce];
     uint32_t *fetch = &pb.fetch[2 * bounce + 1];
@@ -256,11 +259,13 @@ __global__ __launch_bounds__(kTraceBlock) void kShadow(SceneView s, PathBufs pb,
     int64_t idx = -1;
     uint32_t sk = 0;
     bool exhausted = false;
+    WaveRange wr;
+    const int64_t waves = (int64_t) gridDim.x * (kTraceBlock / 64);
     while (true) {
         const bool need = idx < 0 && !exhausted;
         const uint64_t needMask = __ballot(need);
         const bool refill = __popcll(needMask) >= refillAt || __ballot(idx >= 0) == 0;
-        const int64_t got = waveFetch(fetch, need && refill);
+        const int64_t got = waveTake(wr, fetch, n, need && refill, fetchRun, waves);
         if (need && refill) {
             if (got >= n) {
                 exhausted = true;
@@ -787,25 +792,25 @@ hipError_t launchCopyResults(const PathBufs &pb, int64_t n, int64_t first, float
     return hipGetLastError();
 }
 hipError_t launchExtend(const SceneView &s, const PathBufs &pb, int bounce, uint2 *spill, int traceGrid,
-                        int refillAt, int stackDepth, int whileWhile, hipStream_t st) {
+                        int refillAt, int stackDepth, int whileWhile, int fetchRun, hipStream_t st) {
     const int wwCap = whileWhile > 1 ? whileWhile : 0;
     if (whileWhile)
         hipLaunchKernelGGL(kExtend<true>, dim3(traceGrid), dim3(kTraceBlock), stackDepth * kTraceBlock * sizeof(uint2),
-                           st, s, pb, bounce, spill, refillAt, stackDepth, wwCap);
+                           st, s, pb, bounce, spill, refillAt, stackDepth, wwCap, fetchRun);
     else
         hipLaunchKernelGGL(kExtend<false>, dim3(traceGrid), dim3(kTraceBlock), stackDepth * kTraceBlock * sizeof(uint2),
-                           st, s, pb, bounce, spill, refillAt, stackDepth, wwCap);
+                           st, s, pb, bounce, spill, refillAt, stackDepth, wwCap, fetchRun);
     return hipGetLastError();
 }
 hipError_t launchShadow(const SceneView &s, const PathBufs &pb, int bounce, uint2 *spill, int traceGrid,
-                        int refillAt, int stackDepth, int whileWhile, hipStream_t st) {
+                        int refillAt, int stackDepth, int whileWhile, int fetchRun, hipStream_t st) {
     const int wwCap = whileWhile > 1 ? whileWhile : 0;
     if (whileWhile)
         hipLaunchKernelGGL(kShadow<true>, dim3(traceGrid), dim3(kTraceBlock), stackDepth * kTraceBlock * sizeof(uint2),
-                           st, s, pb, bounce, spill, refillAt, stackDepth, wwCap);
+                           st, s, pb, bounce, spill, refillAt, stackDepth, wwCap, fetchRun);
     else
         hipLaunchKernelGGL(kShadow<false>, dim3(traceGrid), dim3(kTraceBlock), stackDepth * kTraceBlock * sizeof(uint2),
-                           st, s, pb, bounce, spill, refillAt, stackDepth, wwCap);
+                           st, s, pb, bounce, spill, refillAt, stackDepth, wwCap, fetchRun);
     return hipGetLastError();
 }
 template <bool P, bool C>
