@@ -175,6 +175,8 @@ typedef struct prr_scene_info {
     float total_emit_area;     /* Scene::totalEmitArea */
     float bounds[6];           /* scene root box min xyz, max xyz */
     int64_t device_bytes;      /* HBM held by the scene */
+    int32_t num_wide_nodes;    /* nodes of the four-wide occlusion tree (shadow rays, prr_trace_any) */
+    int32_t wide_levels_kept;  /* binary levels left uncollapsed there (a child box outside its parent's) */
 } prr_scene_info;
 
 typedef struct prr_context prr_context;

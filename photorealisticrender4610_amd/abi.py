@@ -71,7 +71,7 @@ class Stats(C.Structure):
 class SceneInfo(C.Structure):
     _fields_ = [("num_nodes", C.c_int32), ("num_leaf_prims", C.c_int32), ("max_depth", C.c_int32),
                 ("num_emitters", C.c_int32), ("total_emit_area", C.c_float), ("bounds", C.c_float * 6),
-                ("device_bytes", C.c_int64)]
+                ("device_bytes", C.c_int64), ("num_wide_nodes", C.c_int32), ("wide_levels_kept", C.c_int32)]
 
 
 class CacheParams(C.Structure):
@@ -474,7 +474,8 @@ class Scene:
         self.ctx._check(lib().prr_scene_get_info(self.h, C.byref(inf)), self.ctx.h)
         return {"num_nodes": inf.num_nodes, "num_leaf_prims": inf.num_leaf_prims, "max_depth": inf.max_depth,
                 "num_emitters": inf.num_emitters, "total_emit_area": inf.total_emit_area,
-                "bounds": list(inf.bounds), "device_bytes": inf.device_bytes}
+                "bounds": list(inf.bounds), "device_bytes": inf.device_bytes, "num_wide_nodes": inf.num_wide_nodes,
+                "wide_levels_kept": inf.wide_levels_kept}
 
     def export_level(self, level):
         n = i32()

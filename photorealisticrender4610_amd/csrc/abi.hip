@@ -154,7 +154,7 @@ int runBounces(prr_context *c, const SceneView &view, const ChunkInfo &ci, Timin
                               c->fetchRun, st));
         if (b > 0)
             TIMED(1, launchShadow(view, c->pb, b - 1, c->spill, c->traceBlocks, c->refillAt, c->stackDepth,
-                                  c->whileWhile, c->fetchRun, st));
+                                  c->whileWhile, c->fetchRun, c->shadowWide != 0, st));
         TIMED(2, launchShade(view, c->pb, ci, b, c->shadeBlocks, c->shadeWaves, st));
     }
 #undef TIMED
@@ -257,7 +257,7 @@ int prr_create(int device, prr_context **out) {
     // LDS stacks, 16 KB per block), a wave refills once 40 of its lanes are
     // idle, at most 8 interior steps per work check (measured on the 1M-tri
     // scene); PRR_TRACE_BLOCKS_PER_CU, PRR_STACK_LDS, PRR_REFILL_AT,
-    // PRR_WHILE_WHILE, PRR_FETCH_RUN and PRR_SHADE_WAVES override the measured
+    // PRR_WHILE_WHILE, PRR_FETCH_RUN, PRR_SHADOW_WIDE and PRR_SHADE_WAVES override the measured
     // defaults for tuning sweeps
     int tbpc = 8;
     if (const char *ev = getenv("PRR_TRACE_BLOCKS_PER_CU")) tbpc = std::max(1, std::min(16, atoi(ev)));
@@ -272,6 +272,7 @@ int prr_create(int device, prr_context **out) {
             c->maxSlots = std::max<int64_t>(1 << 16, std::min<int64_t>(kMaxSlots, atoll(ev)));
     }
     if (const char *ev = getenv("PRR_WHILE_WHILE")) c->whileWhile = std::max(0, atoi(ev));
+    if (const char *ev = getenv("PRR_SHADOW_WIDE")) c->shadowWide = atoi(ev) != 0;
     if (const char *ev = getenv("PRR_FETCH_RUN")) c->fetchRun = std::max(64, std::min(1 << 16, atoi(ev)));
     if (const char *ev = getenv("PRR_SHADE_WAVES")) c->shadeWaves = std::max(1, std::min(5, atoi(ev)));
     if (const char *ev = getenv("PRR_STACK_LDS")) {  // LDS stack entries per lane: 2, 4, 8, 16 or 32
@@ -336,6 +337,7 @@ int prr_scene_create(prr_context *c, const prr_scene_desc *d, prr_scene **out) {
 #define UP(dst, src)                                                               \
     if (e == hipSuccess) e = upload(s, (std::remove_const<std::remove_reference<decltype(*dst)>::type>::type **) &dst, src, c->stream)
     UP(v.nodes, h.nodes);
+    UP(v.nodes4, h.nodes4);
     UP(v.tris, h.tris);
     UP(v.sph, h.sph);
     UP(v.sph_id, h.sphId);
@@ -358,6 +360,7 @@ int prr_scene_create(prr_context *c, const prr_scene_desc *d, prr_scene **out) {
         return fail(c, PRR_ERR_NOMEM, std::string("scene upload: ") + hipGetErrorString(e));
     }
     v.root = h.root;
+    v.root4 = h.root4;
     v.num_tris = h.numTris;
     v.num_emitters = h.totalEmitArea > 0.f ? (int) h.emitters.size() : 0;
     v.total_emit_area = h.totalEmitArea;
@@ -403,6 +406,8 @@ int prr_scene_get_info(const prr_scene *s, prr_scene_info *o) {
         o->bounds[3 + a] = h.rootBox[2 * a + 1];
     }
     o->device_bytes = s->bytes;
+    o->num_wide_nodes = (int32_t) h.nodes4.size();
+    o->wide_levels_kept = h.wideLevelsKept;
     return PRR_OK;
 }
 

@@ -120,6 +120,17 @@ PRR_HD uint32_t makeLeafRef(uint32_t first, uint32_t count, bool sphere) {
     return kLeafBit | ((count - 1u) << 28) | (sphere ? kSphereBit : 0u) | first;
 }
 
+// Four-wide node of the occlusion tree (128 B): up to four children's boxes
+// (structure of arrays, child k in component k) and references. The tree is
+// the merged binary tree with levels collapsed, keeping its leaves; empty
+// slots hold an inverted box and kNoChild.
+struct __align__(16) DNode4 {
+    float4 lox, hix, loy, hiy, loz, hiz;
+    uint4 ref;
+    uint4 pad;
+};
+constexpr uint32_t kNoChild = 0x7FFFFFFFu;
+
 enum MatType { MAT_DIFFUSE = 0, MAT_GLASS = 1, MAT_MIRROR = 2, MAT_EMISSIVE = 3 };
 
 struct __align__(16) DMaterial {
@@ -156,6 +167,7 @@ struct CamView {
 
 struct SceneView {
     const DNode *nodes;
+    const DNode4 *nodes4;     // occlusion tree (four-wide)
     const float4 *tris;       // 3 float4 per leaf slot: v0.xyz e1.x | e1.yz e2.xy | e2.z id pad pad
     const float4 *sph;        // leaf-order spheres: center.xyz radius
     const int *sph_id;        // leaf-order sphere -> global prim id
@@ -172,6 +184,7 @@ struct SceneView {
     const int *env_gc;        // conditional guides: env_gk + 1 per row
     int env_gk;               // guide buckets (power of two), 0: full searches
     uint32_t root;            // root child reference
+    uint32_t root4;           // occlusion tree root reference
     int num_tris;
     int num_emitters;
     float total_emit_area;

@@ -230,57 +230,6 @@ PRR_DEV HitOut traceClosest(const SceneView &s, const V3 &o, const V3 &d, LaneSt
     return h;
 }
 
-// BVHAccel::IntersectP on the merged tree: occluded iff some primitive in a
-// leaf whose every ancestor box (root included) passes tBox < tMax has a hit
-// in range (triangles: (1e-3, tMax); spheres: [0, tMax)). The boolean does
-// not depend on visit order; near child first by box distance.
-PRR_DEV bool traceAny(const SceneView &s, const V3 &o, const V3 &d, float tMax, LaneStack &st) {
-    const RayPre r = precompute(o, d);
-    // the reference tests the root's own box first (BVH.cpp:437-438)
-    const float tb = slab(r, s.root_box[0], s.root_box[1], s.root_box[2], s.root_box[3], s.root_box[4],
-                          s.root_box[5]);
-    if (!(tb < tMax)) return false;
-    st.sp = 0;
-    uint32_t cur = s.root;
-    while (true) {
-        if (refIsLeaf(cur)) {
-            const uint32_t first = refFirst(cur), cnt = refCount(cur);
-            if (refSphere(cur)) {
-                for (uint32_t i = 0; i < cnt; i++) {
-                    float t;
-                    if (sphHit(s.sph[first + i], o, d, t) && t < tMax) return true;
-                }
-            } else {
-                for (uint32_t i = 0; i < cnt; i++)
-                    if (triAny(loadTri(s.tris, first + i), o, d, tMax)) return true;
-            }
-        } else {
-            const DNode n = s.nodes[cur];
-            const float tL = slab(r, n.a.x, n.a.y, n.a.z, n.a.w, n.b.x, n.b.y);
-            const float tR = slab(r, n.b.z, n.b.w, n.c.x, n.c.y, n.c.z, n.c.w);
-            const bool hl = tL < tMax, hr = tR < tMax;
-            const uint32_t L = (uint32_t) n.d.x, R = (uint32_t) n.d.y;
-            if (hl && hr) {
-                const bool lf = tL <= tR;
-                st.push(lf ? R : L, 0.f);
-                cur = lf ? L : R;
-                continue;
-            }
-            if (hl) {
-                cur = L;
-                continue;
-            }
-            if (hr) {
-                cur = R;
-                continue;
-            }
-        }
-        if (st.sp == 0) break;
-        cur = st.pop().x;
-    }
-    return false;
-}
-
 // ---------------------------------------------------------------------------
 // Resumable traversal for persistent kernels: one node per call, so a lane
 // whose ray finished can fetch a new ray while the rest of its wave keeps
@@ -306,17 +255,19 @@ PRR_DEV void travBegin(TravState &ts, const V3 &o, const V3 &d, float tmax, uint
     st.sp = 0;
 }
 
-// Next stack entry whose box entry t is below the closest hit so far
-// (BVH.cpp:399-407); true when the stack is empty (the ray is finished).
-PRR_DEV bool popNext(TravState &ts, LaneStack &st) {
-    while (st.sp > 0) {
-        const uint2 e = st.pop();
-        if (__uint_as_float(e.y) < ts.h.t) {
-            ts.cur = e.x;
-            return false;
-        }
-    }
-    return true;
+// ts.cur after a pop whose entry fails the t test: the next step pops again.
+// Never a node index (the node array is far smaller) and not a leaf ref.
+constexpr uint32_t kCulled = 0x7FFFFFFEu;
+
+// One pop of BVHAccel::Intersect's stack (BVH.cpp:399-407): the entry is
+// visited next if its box entry t is below the closest hit so far, else the
+// next step pops again; true when the stack is empty (the ray is finished).
+// Popping one entry per step keeps the traversal loop free of a nested loop.
+PRR_DEV bool popOne(TravState &ts, LaneStack &st) {
+    if (st.sp == 0) return true;
+    const uint2 e = st.pop();
+    ts.cur = __uint_as_float(e.y) < ts.h.t ? e.x : kCulled;
+    return false;
 }
 
 // BVHAccel::Intersect at a leaf (ts.cur): test its primitives, then pop.
@@ -344,30 +295,28 @@ PRR_DEV bool closestLeaf(const SceneView &s, TravState &ts, LaneStack &st) {
             }
         }
     }
-    return popNext(ts, st);
+    return popOne(ts, st);
 }
 
 // BVHAccel::Intersect at an interior node: near child first, the far one
 // pushed with its entry t, or pop when neither box is closer than the hit.
+// Near is the left child when tL <= tR; since tNear <= tFar, a far child that
+// passes implies a near one that passes, as in the reference's nesting.
 PRR_DEV bool closestInterior(const SceneView &s, TravState &ts, LaneStack &st) {
+    if (ts.cur == kCulled) return popOne(ts, st);
     const DNode n = s.nodes[ts.cur];
     const float tL = slab(ts.r, n.a.x, n.a.y, n.a.z, n.a.w, n.b.x, n.b.y);
     const float tR = slab(ts.r, n.b.z, n.b.w, n.c.x, n.c.y, n.c.z, n.c.w);
     const uint32_t L = (uint32_t) n.d.x, R = (uint32_t) n.d.y;
-    if (tL <= tR) {
-        if (tL < ts.h.t) {
-            if (tR < ts.h.t) st.push(R, tR);
-            ts.cur = L;
-            return false;
-        }
-    } else {
-        if (tR < ts.h.t) {
-            if (tL < ts.h.t) st.push(L, tL);
-            ts.cur = R;
-            return false;
-        }
+    const bool lFirst = tL <= tR;
+    const float tNear = lFirst ? tL : tR, tFar = lFirst ? tR : tL;
+    const uint32_t rNear = lFirst ? L : R, rFar = lFirst ? R : L;
+    if (tFar < ts.h.t) st.push(rFar, tFar);
+    if (tNear < ts.h.t) {
+        ts.cur = rNear;
+        return false;
     }
-    return popNext(ts, st);
+    return popOne(ts, st);
 }
 
 // One step of BVHAccel::Intersect; returns true when the ray is finished.
@@ -378,13 +327,29 @@ PRR_DEV bool closestStep(const SceneView &s, TravState &ts, LaneStack &st) {
 // While-while iteration (Aila and Laine 2009): interior steps until this
 // lane stands on a leaf, then that leaf, so a wave tests its leaves
 // together. Each ray visits the same nodes in the same order as with
-// closestStep.
-// At most cap interior steps per call (cap <= 0: no limit); the descent then
-// resumes at the next call.
+// closestStep. At most cap interior steps per call (cap <= 0: no limit); the
+// descent then resumes at the next call. The interior step is written with
+// one pop site so the loop carries few divergent exits.
 PRR_DEV bool closestIter(const SceneView &s, TravState &ts, LaneStack &st, int cap) {
     int k = 0;
     while (!refIsLeaf(ts.cur)) {
-        if (closestInterior(s, ts, st)) return true;
+        bool pop = true;
+        if (ts.cur != kCulled) {
+            const DNode n = s.nodes[ts.cur];
+            const float tL = slab(ts.r, n.a.x, n.a.y, n.a.z, n.a.w, n.b.x, n.b.y);
+            const float tR = slab(ts.r, n.b.z, n.b.w, n.c.x, n.c.y, n.c.z, n.c.w);
+            const bool lFirst = tL <= tR;
+            const float tNear = lFirst ? tL : tR, tFar = lFirst ? tR : tL;
+            const uint32_t rNear = (uint32_t) (lFirst ? n.d.x : n.d.y), rFar = (uint32_t) (lFirst ? n.d.y : n.d.x);
+            if (tFar < ts.h.t) st.push(rFar, tFar);
+            pop = !(tNear < ts.h.t);
+            ts.cur = rNear;
+        }
+        if (pop) {
+            if (st.sp == 0) return true;
+            const uint2 e = st.pop();
+            ts.cur = __uint_as_float(e.y) < ts.h.t ? e.x : kCulled;
+        }
         if (++k == cap) return false;
     }
     return closestLeaf(s, ts, st);
@@ -462,6 +427,71 @@ PRR_DEV bool anyIter(const SceneView &s, TravState &ts, LaneStack &st, int cap) 
         if (++k == cap) return false;
     }
     return anyLeaf(s, ts, st);
+}
+
+// Any-hit at a node of the occlusion tree (DNode4): the nearest child whose
+// box is entered before tmax next, the other entered children pushed. Empty
+// slots hold an inverted box, which the slab test never enters. The answer
+// equals BVHAccel::IntersectP's on the binary tree: the occlusion tree keeps
+// its leaves and skips only boxes that contain their children's boxes.
+PRR_DEV bool anyInterior4(const SceneView &s, TravState &ts, LaneStack &st) {
+    const DNode4 *n = s.nodes4 + ts.cur;
+    const float4 lx = n->lox, hx = n->hix, ly = n->loy, hy = n->hiy, lz = n->loz, hz = n->hiz;
+    const uint4 rf = n->ref;
+    const float t0 = slab(ts.r, lx.x, hx.x, ly.x, hy.x, lz.x, hz.x);
+    const float t1 = slab(ts.r, lx.y, hx.y, ly.y, hy.y, lz.y, hz.y);
+    const float t2 = slab(ts.r, lx.z, hx.z, ly.z, hy.z, lz.z, hz.z);
+    const float t3 = slab(ts.r, lx.w, hx.w, ly.w, hy.w, lz.w, hz.w);
+    uint32_t best = kNoChild;
+    float bt = ts.tmax;
+    if (t0 < bt) {
+        bt = t0;
+        best = rf.x;
+    }
+    if (t1 < bt) {
+        bt = t1;
+        best = rf.y;
+    }
+    if (t2 < bt) {
+        bt = t2;
+        best = rf.z;
+    }
+    if (t3 < bt) {
+        bt = t3;
+        best = rf.w;
+    }
+    if (best == kNoChild) return anyPop(ts, st);
+    if (t0 < ts.tmax && rf.x != best) st.push(rf.x, 0.f);
+    if (t1 < ts.tmax && rf.y != best) st.push(rf.y, 0.f);
+    if (t2 < ts.tmax && rf.z != best) st.push(rf.z, 0.f);
+    if (t3 < ts.tmax && rf.w != best) st.push(rf.w, 0.f);
+    ts.cur = best;
+    return false;
+}
+
+PRR_DEV bool anyStep4(const SceneView &s, TravState &ts, LaneStack &st) {
+    return refIsLeaf(ts.cur) ? anyLeaf(s, ts, st) : anyInterior4(s, ts, st);
+}
+
+// While-while iteration on the occlusion tree (see closestIter).
+PRR_DEV bool anyIter4(const SceneView &s, TravState &ts, LaneStack &st, int cap) {
+    int k = 0;
+    while (!refIsLeaf(ts.cur)) {
+        if (anyInterior4(s, ts, st)) return true;
+        if (++k == cap) return false;
+    }
+    return anyLeaf(s, ts, st);
+}
+
+// BVHAccel::IntersectP through the occlusion tree; the scene box test first
+// (BVH.cpp:437-438).
+PRR_DEV bool traceAny4(const SceneView &s, const V3 &o, const V3 &d, float tMax, LaneStack &st) {
+    TravState ts;
+    travBegin(ts, o, d, tMax, s.root4, st);
+    if (!anyBegin(s, ts)) return false;
+    while (!anyStep4(s, ts, st)) {
+    }
+    return ts.h.prim > 0;
 }
 
 // Work fetch through a wave-private range of queue entries. A wave claims a

@@ -242,7 +242,7 @@ __global__ __launch_bounds__(kTraceBlock) void kExtend(SceneView s, PathBufs pb,
     }
 }
 
-template <bool WW>
+template <bool WW, bool WIDE>
 __global__ __launch_bounds__(kTraceBlock) void kShadow(SceneView s, PathBufs pb, int bounce, uint2 *spill, int refillAt,
                                                        int stackDepth, int wwCap, int fetchRun) {
     extern __shared__ uint2 stk[];
@@ -272,7 +272,7 @@ __global__ __launch_bounds__(kTraceBlock) void kShadow(SceneView s, PathBufs pb,
             } else {
                 const uint32_t k = pb.sqSlot[got];
                 const float4 o4 = pb.sqOrg[got];
-                travBegin(ts, ld3(o4), ld3(pb.sqDir[got]), o4.w, s.root, st);
+                travBegin(ts, ld3(o4), ld3(pb.sqDir[got]), o4.w, WIDE ? s.root4 : s.root, st);
                 if (anyBegin(s, ts)) {
                     idx = got;
                     sk = k;
@@ -282,7 +282,8 @@ __global__ __launch_bounds__(kTraceBlock) void kShadow(SceneView s, PathBufs pb,
             }
         }
         if (__ballot(idx >= 0 || !exhausted) == 0) break;
-        if (idx >= 0 && (WW ? anyIter(s, ts, st, wwCap) : anyStep(s, ts, st))) {
+        if (idx >= 0 && (WIDE ? (WW ? anyIter4(s, ts, st, wwCap) : anyStep4(s, ts, st))
+                              : (WW ? anyIter(s, ts, st, wwCap) : anyStep(s, ts, st)))) {
             pb.vis[(int64_t) (sk >> 30) * pb.slots + (sk & 0x3FFFFFFFu)] = ts.h.prim > 0 ? 0 : 1;
             idx = -1;
         }
@@ -803,14 +804,21 @@ hipError_t launchExtend(const SceneView &s, const PathBufs &pb, int bounce, uint
     return hipGetLastError();
 }
 hipError_t launchShadow(const SceneView &s, const PathBufs &pb, int bounce, uint2 *spill, int traceGrid,
-                        int refillAt, int stackDepth, int whileWhile, int fetchRun, hipStream_t st) {
+                        int refillAt, int stackDepth, int whileWhile, int fetchRun, bool wide, hipStream_t st) {
     const int wwCap = whileWhile > 1 ? whileWhile : 0;
-    if (whileWhile)
-        hipLaunchKernelGGL(kShadow<true>, dim3(traceGrid), dim3(kTraceBlock), stackDepth * kTraceBlock * sizeof(uint2),
-                           st, s, pb, bounce, spill, refillAt, stackDepth, wwCap, fetchRun);
+    const size_t lds = stackDepth * kTraceBlock * sizeof(uint2);
+#define PRR_SHADOW(W, V)                                                                                             \
+    hipLaunchKernelGGL((kShadow<W, V>), dim3(traceGrid), dim3(kTraceBlock), lds, st, s, pb, bounce, spill, refillAt, \
+                       stackDepth, wwCap, fetchRun)
+    if (whileWhile && wide)
+        PRR_SHADOW(true, true);
+    else if (whileWhile)
+        PRR_SHADOW(true, false);
+    else if (wide)
+        PRR_SHADOW(false, true);
     else
-        hipLaunchKernelGGL(kShadow<false>, dim3(traceGrid), dim3(kTraceBlock), stackDepth * kTraceBlock * sizeof(uint2),
-                           st, s, pb, bounce, spill, refillAt, stackDepth, wwCap, fetchRun);
+        PRR_SHADOW(false, false);
+#undef PRR_SHADOW
     return hipGetLastError();
 }
 template <bool P, bool C>

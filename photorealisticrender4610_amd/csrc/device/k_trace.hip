@@ -47,7 +47,7 @@ __global__ __launch_bounds__(kTraceBlock) void kTraceAny(SceneView s, int64_t n,
     for (int64_t i = lane; i < n; i += stride) {
         const V3 o(org[3 * i], org[3 * i + 1], org[3 * i + 2]);
         const V3 d(dir[3 * i], dir[3 * i + 1], dir[3 * i + 2]);
-        hit[i] = traceAny(s, o, d, tmax[i], st) ? 1 : 0;
+        hit[i] = traceAny4(s, o, d, tmax[i], st) ? 1 : 0;
     }
 }
 

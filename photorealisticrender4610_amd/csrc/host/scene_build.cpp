@@ -324,6 +324,92 @@ struct Merger {
     }
 };
 
+// Occlusion tree: the merged binary tree with levels collapsed into nodes of
+// up to four children, keeping its leaves. An any-hit query only asks whether
+// some reached leaf holds an occluder. The slab test's arithmetic is monotone
+// in the box bounds, so a box passes whenever a box inside it passes, and a
+// leaf is then reached in the binary tree exactly when its own box passes.
+// Skipping the test of a box that contains its children's boxes therefore
+// changes no answer; a level where a child's box sticks out is kept.
+struct Widener {
+    HostScene &S;
+    int maxPush = 0;
+    struct Cand {
+        float b[6];  // minX maxX minY maxY minZ maxZ
+        uint32_t ref;
+    };
+    static void split(const DNode &n, Cand &l, Cand &r) {
+        const float lb[6] = {n.a.x, n.a.y, n.a.z, n.a.w, n.b.x, n.b.y};
+        const float rb[6] = {n.b.z, n.b.w, n.c.x, n.c.y, n.c.z, n.c.w};
+        for (int k = 0; k < 6; k++) {
+            l.b[k] = lb[k];
+            r.b[k] = rb[k];
+        }
+        l.ref = (uint32_t) n.d.x;
+        r.ref = (uint32_t) n.d.y;
+    }
+    static bool inside(const float *in, const float *out) {
+        return in[0] >= out[0] && in[1] <= out[1] && in[2] >= out[2] && in[3] <= out[3] && in[4] >= out[4] &&
+               in[5] <= out[5];
+    }
+    static double area(const float *b) {
+        const double x = (double) b[1] - b[0], y = (double) b[3] - b[2], z = (double) b[5] - b[4];
+        return x * y + y * z + z * x;
+    }
+    // pushes: stack entries a traversal may hold on arriving here
+    uint32_t build(uint32_t ref, int pushes) {
+        if (refIsLeaf(ref)) return ref;
+        Cand c[4];
+        int n = 2;
+        split(S.nodes[ref], c[0], c[1]);
+        while (n < 4) {  // open the largest interior child whose children lie inside its box
+            int best = -1;
+            double bestA = -1.0;
+            for (int k = 0; k < n; k++) {
+                if (refIsLeaf(c[k].ref)) continue;
+                Cand l, r;
+                split(S.nodes[c[k].ref], l, r);
+                if (!inside(l.b, c[k].b) || !inside(r.b, c[k].b)) continue;
+                const double a = area(c[k].b);
+                if (a > bestA) {
+                    bestA = a;
+                    best = k;
+                }
+            }
+            if (best < 0) break;
+            Cand l, r;
+            split(S.nodes[c[best].ref], l, r);
+            c[best] = l;
+            c[n++] = r;
+        }
+        for (int k = 0; k < n; k++)
+            if (!refIsLeaf(c[k].ref) && n < 4) S.wideLevelsKept++;
+        const uint32_t idx = (uint32_t) S.nodes4.size();
+        S.nodes4.emplace_back();
+        const int below = pushes + n - 1;
+        maxPush = std::max(maxPush, below);
+        uint32_t refs[4] = {kNoChild, kNoChild, kNoChild, kNoChild};
+        for (int k = 0; k < n; k++) refs[k] = build(c[k].ref, below);
+        const float inf = std::numeric_limits<float>::infinity();
+        float lo[3][4], hi[3][4];
+        for (int k = 0; k < 4; k++)
+            for (int a = 0; a < 3; a++) {
+                lo[a][k] = k < n ? c[k].b[2 * a] : inf;
+                hi[a][k] = k < n ? c[k].b[2 * a + 1] : -inf;
+            }
+        DNode4 &w = S.nodes4[idx];
+        w.lox = make_float4(lo[0][0], lo[0][1], lo[0][2], lo[0][3]);
+        w.hix = make_float4(hi[0][0], hi[0][1], hi[0][2], hi[0][3]);
+        w.loy = make_float4(lo[1][0], lo[1][1], lo[1][2], lo[1][3]);
+        w.hiy = make_float4(hi[1][0], hi[1][1], hi[1][2], hi[1][3]);
+        w.loz = make_float4(lo[2][0], lo[2][1], lo[2][2], lo[2][3]);
+        w.hiz = make_float4(hi[2][0], hi[2][1], hi[2][2], hi[2][3]);
+        w.ref = make_uint4(refs[0], refs[1], refs[2], refs[3]);
+        w.pad = make_uint4(0, 0, 0, 0);
+        return idx;
+    }
+};
+
 bool hasEmission(const prr_material_desc &m) {  // Material::hasEmission (Material.hpp:113-115)
     return norm(V3(m.emission[0], m.emission[1], m.emission[2])) > kEps;
 }
@@ -484,6 +570,17 @@ bool buildHostScene(const prr_scene_desc *d, HostScene &S, std::string &err) {
         err = "BVH deeper than the 64-entry traversal stack";
         return false;
     }
+    S.nodes4.clear();
+    S.wideLevelsKept = 0;
+    {
+        Widener wd{S};
+        S.root4 = wd.build(S.root, 0);
+        if (wd.maxPush > 64) {  // the traversal stack's spill region holds 64 entries
+            err = "occlusion tree needs more than 64 stack entries";
+            return false;
+        }
+    }
+    if (S.nodes4.empty()) S.nodes4.emplace_back();
     if (S.tris.empty()) S.tris.push_back(make_float4(0, 0, 0, 0));  // keep buffers non-empty
     if (S.sph.empty()) {
         S.sph.push_back(make_float4(0, 0, 0, 0));

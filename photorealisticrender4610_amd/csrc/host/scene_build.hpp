@@ -56,6 +56,7 @@ struct HostScene {
     std::vector<float> objArea;
     // device staging
     std::vector<DNode> nodes;
+    std::vector<DNode4> nodes4;    // occlusion tree
     std::vector<float4> tris;      // 3 per leaf slot
     std::vector<float4> sph;       // leaf-order spheres
     std::vector<int> sphId;        // leaf-order sphere -> global prim id
@@ -74,6 +75,8 @@ struct HostScene {
     int envGuideK = 0;
     std::vector<int32_t> envGuideM, envGuideC;
     uint32_t root = 0;
+    uint32_t root4 = 0;
+    int wideLevelsKept = 0;        // binary levels not collapsed (a child box not inside its parent's)
     float rootBox[6] = {0, 0, 0, 0, 0, 0};
     int maxDepth = 0;
     float totalEmitArea = 0.f;
