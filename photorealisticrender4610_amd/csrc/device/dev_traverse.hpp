@@ -156,6 +156,72 @@ struct LaneStack {
     }
 };
 
+// Any-hit stack: node references only (an any-hit pop tests no entry t),
+// so the same LDS bytes hold twice the entries; [slot][lane] layout and a
+// global spill region of 2 * kSpill entries per lane (the same bytes as a
+// LaneStack's).
+struct RefStack {
+    uint32_t *lds;
+    uint32_t *spill;
+    int lane;
+    int sp;
+    int depth;  // LDS entries per lane (power of two)
+    PRR_DEV void push(uint32_t ref) {
+        const int slot = (sp & (depth - 1)) * kTraceBlock + lane;
+        if (sp >= depth) spill[sp - depth] = lds[slot];
+        lds[slot] = ref;
+        sp++;
+    }
+    // Pushes the first m (0..3) of a, b, c. Without wrap-around (sp + 3 <=
+    // depth) the three LDS slots above the top are free, so all three are
+    // written and sp advances by m: no branch on m.
+    PRR_DEV void push3(uint32_t a, uint32_t b, uint32_t c, int m) {
+        if (sp + 3 <= depth) {
+            uint32_t *q = lds + sp * kTraceBlock + lane;
+            q[0] = a;
+            q[kTraceBlock] = b;
+            q[2 * kTraceBlock] = c;
+            sp += m;
+        } else {
+            if (m > 0) push(a);
+            if (m > 1) push(b);
+            if (m > 2) push(c);
+        }
+    }
+    PRR_DEV uint32_t pop() {
+        sp--;
+        const int slot = (sp & (depth - 1)) * kTraceBlock + lane;
+        const uint32_t e = lds[slot];
+        if (sp >= depth) lds[slot] = spill[sp - depth];
+        return e;
+    }
+};
+
+// Stack over a block's LDS stack area (ldsDepth uint2 entries per lane) and
+// a lane's spill region (kSpill uint2 entries).
+template <class S>
+PRR_DEV S makeStack(uint2 *lds, uint2 *spill, int lane, int ldsDepth);
+template <>
+PRR_DEV LaneStack makeStack<LaneStack>(uint2 *lds, uint2 *spill, int lane, int ldsDepth) {
+    LaneStack st;
+    st.lds = lds;
+    st.spill = spill;
+    st.lane = lane;
+    st.depth = ldsDepth;
+    st.sp = 0;
+    return st;
+}
+template <>
+PRR_DEV RefStack makeStack<RefStack>(uint2 *lds, uint2 *spill, int lane, int ldsDepth) {
+    RefStack st;
+    st.lds = reinterpret_cast<uint32_t *>(lds);
+    st.spill = reinterpret_cast<uint32_t *>(spill);
+    st.lane = lane;
+    st.depth = 2 * ldsDepth;
+    st.sp = 0;
+    return st;
+}
+
 struct HitOut {
     float t, u, v;
     int prim;  // global primitive id or -1
@@ -243,7 +309,8 @@ struct TravState {
     HitOut h;
 };
 
-PRR_DEV void travBegin(TravState &ts, const V3 &o, const V3 &d, float tmax, uint32_t root, LaneStack &st) {
+template <class Stack>
+PRR_DEV void travBegin(TravState &ts, const V3 &o, const V3 &d, float tmax, uint32_t root, Stack &st) {
     ts.o = o;
     ts.d = d;
     ts.tmax = tmax;
@@ -364,16 +431,21 @@ PRR_DEV bool anyBegin(const SceneView &s, TravState &ts) {
 
 // BVHAccel::IntersectP at a leaf; true when finished (ts.h.prim = 1 if
 // occluded, -1 otherwise).
-PRR_DEV bool anyPop(TravState &ts, LaneStack &st) {
+PRR_DEV uint32_t popRef(LaneStack &st) { return st.pop().x; }
+PRR_DEV uint32_t popRef(RefStack &st) { return st.pop(); }
+
+template <class Stack>
+PRR_DEV bool anyPop(TravState &ts, Stack &st) {
     if (st.sp == 0) {
         ts.h.prim = -1;
         return true;
     }
-    ts.cur = st.pop().x;
+    ts.cur = popRef(st);
     return false;
 }
 
-PRR_DEV bool anyLeaf(const SceneView &s, TravState &ts, LaneStack &st) {
+template <class Stack>
+PRR_DEV bool anyLeaf(const SceneView &s, TravState &ts, Stack &st) {
     const uint32_t cur = ts.cur;
     const uint32_t first = refFirst(cur), cnt = refCount(cur);
     if (refSphere(cur)) {
@@ -430,51 +502,48 @@ PRR_DEV bool anyIter(const SceneView &s, TravState &ts, LaneStack &st, int cap) 
 }
 
 // Any-hit at a node of the occlusion tree (DNode4): the nearest child whose
-// box is entered before tmax next, the other entered children pushed. Empty
-// slots hold an inverted box, which the slab test never enters. The answer
-// equals BVHAccel::IntersectP's on the binary tree: the occlusion tree keeps
-// its leaves and skips only boxes that contain their children's boxes.
-PRR_DEV bool anyInterior4(const SceneView &s, TravState &ts, LaneStack &st) {
+// box is entered before tmax next, the other entered children pushed (in
+// child order), or a pop when none is entered. Empty slots hold an inverted
+// box, which the slab test never enters. The answer equals
+// BVHAccel::IntersectP's on the binary tree: the occlusion tree keeps its
+// leaves and skips only boxes that contain their children's boxes.
+PRR_DEV bool anyInterior4(const SceneView &s, TravState &ts, RefStack &st) {
     const DNode4 *n = s.nodes4 + ts.cur;
     const float4 lx = n->lox, hx = n->hix, ly = n->loy, hy = n->hiy, lz = n->loz, hz = n->hiz;
     const uint4 rf = n->ref;
+    const float tm = ts.tmax;
     const float t0 = slab(ts.r, lx.x, hx.x, ly.x, hy.x, lz.x, hz.x);
     const float t1 = slab(ts.r, lx.y, hx.y, ly.y, hy.y, lz.y, hz.y);
     const float t2 = slab(ts.r, lx.z, hx.z, ly.z, hy.z, lz.z, hz.z);
     const float t3 = slab(ts.r, lx.w, hx.w, ly.w, hy.w, lz.w, hz.w);
-    uint32_t best = kNoChild;
-    float bt = ts.tmax;
-    if (t0 < bt) {
-        bt = t0;
-        best = rf.x;
-    }
-    if (t1 < bt) {
-        bt = t1;
-        best = rf.y;
-    }
-    if (t2 < bt) {
-        bt = t2;
-        best = rf.z;
-    }
-    if (t3 < bt) {
-        bt = t3;
-        best = rf.w;
-    }
-    if (best == kNoChild) return anyPop(ts, st);
-    if (t0 < ts.tmax && rf.x != best) st.push(rf.x, 0.f);
-    if (t1 < ts.tmax && rf.y != best) st.push(rf.y, 0.f);
-    if (t2 < ts.tmax && rf.z != best) st.push(rf.z, 0.f);
-    if (t3 < ts.tmax && rf.w != best) st.push(rf.w, 0.f);
-    ts.cur = best;
+    // nearest entered child (slot index, so equal refs never arise)
+    int bk = -1;
+    float bt = tm;
+    bk = t0 < bt ? 0 : bk;
+    bt = t0 < bt ? t0 : bt;
+    bk = t1 < bt ? 1 : bk;
+    bt = t1 < bt ? t1 : bt;
+    bk = t2 < bt ? 2 : bk;
+    bt = t2 < bt ? t2 : bt;
+    bk = t3 < bt ? 3 : bk;
+    if (bk < 0) return anyPop(ts, st);
+    const bool h0 = t0 < tm && bk != 0, h1 = t1 < tm && bk != 1, h2 = t2 < tm && bk != 2, h3 = t3 < tm && bk != 3;
+    // compact the other entered children into p0..p2
+    const uint32_t p0 = h0 ? rf.x : (h1 ? rf.y : (h2 ? rf.z : rf.w));
+    const uint32_t p1 = h0 ? (h1 ? rf.y : (h2 ? rf.z : rf.w)) : (h1 ? (h2 ? rf.z : rf.w) : rf.w);
+    const uint32_t p2 = bk == 3 ? rf.z : rf.w;  // the third, when all three others were entered
+    const int m = (int) h0 + (int) h1 + (int) h2 + (int) h3;
+    st.push3(p0, p1, p2, m);
+    ts.cur = bk == 0 ? rf.x : (bk == 1 ? rf.y : (bk == 2 ? rf.z : rf.w));
     return false;
 }
 
-PRR_DEV bool anyStep4(const SceneView &s, TravState &ts, LaneStack &st) {
+PRR_DEV bool anyStep4(const SceneView &s, TravState &ts, RefStack &st) {
     return refIsLeaf(ts.cur) ? anyLeaf(s, ts, st) : anyInterior4(s, ts, st);
 }
 
 // While-while iteration on the occlusion tree (see closestIter).
-PRR_DEV bool anyIter4(const SceneView &s, TravState &ts, LaneStack &st, int cap) {
+PRR_DEV bool anyIter4(const SceneView &s, TravState &ts, RefStack &st, int cap) {
     int k = 0;
     while (!refIsLeaf(ts.cur)) {
         if (anyInterior4(s, ts, st)) return true;
@@ -485,7 +554,7 @@ PRR_DEV bool anyIter4(const SceneView &s, TravState &ts, LaneStack &st, int cap)
 
 // BVHAccel::IntersectP through the occlusion tree; the scene box test first
 // (BVH.cpp:437-438).
-PRR_DEV bool traceAny4(const SceneView &s, const V3 &o, const V3 &d, float tMax, LaneStack &st) {
+PRR_DEV bool traceAny4(const SceneView &s, const V3 &o, const V3 &d, float tMax, RefStack &st) {
     TravState ts;
     travBegin(ts, o, d, tMax, s.root4, st);
     if (!anyBegin(s, ts)) return false;

@@ -9,6 +9,8 @@
 // are resolved, in the reference's order (area, env, then the BSDF-MIS term),
 // so the float sums match the reference's. kAccumulate adds the chunk's
 // results per pixel in sample order (Renderer.cpp:84-99).
+#include <type_traits>
+
 #include "dev_traverse.hpp"
 #include "launch_render.hpp"
 #include "render.hpp"
@@ -249,12 +251,8 @@ __global__ __launch_bounds__(kTraceBlock) void kShadow(SceneView s, PathBufs pb,
     const int64_t n = pb.sqcnt[bounce];
     uint32_t *fetch = &pb.fetch[2 * bounce + 1];
     const int64_t lane = (int64_t) blockIdx.x * kTraceBlock + threadIdx.x;
-    LaneStack st;
-    st.lds = stk;
-    st.lane = threadIdx.x;
-    st.spill = spill + lane * kSpill;
-    st.depth = stackDepth;
-    st.sp = 0;
+    using Stack = typename std::conditional<WIDE, RefStack, LaneStack>::type;
+    Stack st = makeStack<Stack>(stk, spill + lane * kSpill, threadIdx.x, stackDepth);
     TravState ts;
     int64_t idx = -1;
     uint32_t sk = 0;
@@ -282,8 +280,14 @@ __global__ __launch_bounds__(kTraceBlock) void kShadow(SceneView s, PathBufs pb,
             }
         }
         if (__ballot(idx >= 0 || !exhausted) == 0) break;
-        if (idx >= 0 && (WIDE ? (WW ? anyIter4(s, ts, st, wwCap) : anyStep4(s, ts, st))
-                              : (WW ? anyIter(s, ts, st, wwCap) : anyStep(s, ts, st)))) {
+        bool done = false;
+        if (idx >= 0) {
+            if constexpr (WIDE)
+                done = WW ? anyIter4(s, ts, st, wwCap) : anyStep4(s, ts, st);
+            else
+                done = WW ? anyIter(s, ts, st, wwCap) : anyStep(s, ts, st);
+        }
+        if (done) {
             pb.vis[(int64_t) (sk >> 30) * pb.slots + (sk & 0x3FFFFFFFu)] = ts.h.prim > 0 ? 0 : 1;
             idx = -1;
         }

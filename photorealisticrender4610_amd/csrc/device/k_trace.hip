@@ -37,12 +37,7 @@ __global__ __launch_bounds__(kTraceBlock) void kTraceAny(SceneView s, int64_t n,
                                                          int stackDepth) {
     extern __shared__ uint2 stk[];
     const int64_t lane = (int64_t) blockIdx.x * kTraceBlock + threadIdx.x;
-    LaneStack st;
-    st.lds = stk;
-    st.lane = threadIdx.x;
-    st.spill = spill + lane * kSpill;
-    st.depth = stackDepth;
-    st.sp = 0;
+    RefStack st = makeStack<RefStack>(stk, spill + lane * kSpill, threadIdx.x, stackDepth);
     const int64_t stride = (int64_t) gridDim.x * kTraceBlock;
     for (int64_t i = lane; i < n; i += stride) {
         const V3 o(org[3 * i], org[3 * i + 1], org[3 * i + 2]);
