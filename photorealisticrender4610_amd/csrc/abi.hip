@@ -154,7 +154,8 @@ int runBounces(prr_context *c, const SceneView &view, const ChunkInfo &ci, Timin
                               c->fetchRun, st));
         if (b > 0)
             TIMED(1, launchShadow(view, c->pb, b - 1, c->spill, c->traceBlocks, c->refillAt, c->stackDepth,
-                                  c->whileWhile, c->fetchRun, c->shadowWide != 0, st));
+                                  c->shadowWide ? c->whileWhileAny : c->whileWhile, c->fetchRun, c->shadowWide != 0,
+                                  st));
         TIMED(2, launchShade(view, c->pb, ci, b, c->shadeBlocks, c->shadeWaves, st));
     }
 #undef TIMED
@@ -255,9 +256,9 @@ int prr_create(int device, prr_context **out) {
     c->numCU = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 1;
     // persistent traversal grid: 8 blocks of 256 lanes per CU (with 8-entry
     // LDS stacks, 16 KB per block), a wave refills once 40 of its lanes are
-    // idle, at most 8 interior steps per work check (measured on the 1M-tri
+    // idle, at most 8 interior steps (4 for shadow rays) per work check (measured on the 1M-tri
     // scene); PRR_TRACE_BLOCKS_PER_CU, PRR_STACK_LDS, PRR_REFILL_AT,
-    // PRR_WHILE_WHILE, PRR_FETCH_RUN, PRR_SHADOW_WIDE and PRR_SHADE_WAVES override the measured
+    // PRR_WHILE_WHILE(_ANY), PRR_FETCH_RUN, PRR_SHADOW_WIDE and PRR_SHADE_WAVES override the measured
     // defaults for tuning sweeps
     int tbpc = 8;
     if (const char *ev = getenv("PRR_TRACE_BLOCKS_PER_CU")) tbpc = std::max(1, std::min(16, atoi(ev)));
@@ -272,6 +273,7 @@ int prr_create(int device, prr_context **out) {
             c->maxSlots = std::max<int64_t>(1 << 16, std::min<int64_t>(kMaxSlots, atoll(ev)));
     }
     if (const char *ev = getenv("PRR_WHILE_WHILE")) c->whileWhile = std::max(0, atoi(ev));
+    if (const char *ev = getenv("PRR_WHILE_WHILE_ANY")) c->whileWhileAny = std::max(0, atoi(ev));
     if (const char *ev = getenv("PRR_SHADOW_WIDE")) c->shadowWide = atoi(ev) != 0;
     if (const char *ev = getenv("PRR_FETCH_RUN")) c->fetchRun = std::max(64, std::min(1 << 16, atoi(ev)));
     if (const char *ev = getenv("PRR_SHADE_WAVES")) c->shadeWaves = std::max(1, std::min(5, atoi(ev)));

@@ -23,6 +23,7 @@ struct prr_context {
     int stackDepth = 8;
     int whileWhile = 8;     // traversal iteration: 0 one node per work check, 1 interior nodes then a leaf,
                             // n > 1 the same with at most n interior steps per check
+    int whileWhileAny = 4;  // the same for shadow rays (on the four-wide tree)
     int fetchRun = 256;     // traversal work fetch: a wave claims at most this many queue entries per atomic
     int shadowWide = 1;     // shadow rays on the four-wide occlusion tree (0: the binary tree)
     int shadeWaves = 4;     // shading kernel compiled for at least this many waves per SIMD (1, 4, 5)

@@ -398,8 +398,8 @@ PRR_DEV bool closestStep(const SceneView &s, TravState &ts, LaneStack &st) {
 // descent then resumes at the next call. The interior step is written with
 // one pop site so the loop carries few divergent exits.
 PRR_DEV bool closestIter(const SceneView &s, TravState &ts, LaneStack &st, int cap) {
-    int k = 0;
-    while (!refIsLeaf(ts.cur)) {
+    bool done = false;
+    for (int k = 0; !done && k != cap && !refIsLeaf(ts.cur); k++) {  // one loop exit
         bool pop = true;
         if (ts.cur != kCulled) {
             const DNode n = s.nodes[ts.cur];
@@ -413,12 +413,15 @@ PRR_DEV bool closestIter(const SceneView &s, TravState &ts, LaneStack &st, int c
             ts.cur = rNear;
         }
         if (pop) {
-            if (st.sp == 0) return true;
-            const uint2 e = st.pop();
-            ts.cur = __uint_as_float(e.y) < ts.h.t ? e.x : kCulled;
+            done = st.sp == 0;
+            if (!done) {
+                const uint2 e = st.pop();
+                ts.cur = __uint_as_float(e.y) < ts.h.t ? e.x : kCulled;
+            }
         }
-        if (++k == cap) return false;
     }
+    if (done) return true;
+    if (!refIsLeaf(ts.cur)) return false;  // step cap reached
     return closestLeaf(s, ts, st);
 }
 
