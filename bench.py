@@ -49,8 +49,10 @@ WORKLOADS = {
 # radiance cache on the sphere_field scene: frames of 4 spp, plain vs cache
 CACHE_WORKLOAD = ("sphere_field", 1920, 1080, 4, 12, 60,
                   "radiance_cache: sphere_field (1,048,578 tris), 1920x1080, 4 spp per frame x 60 frames, depth 12; "
-                  "FP16 MLP 96-64x5-3 trained online (Adam lr 1e-2) on 1/16 full-depth paths, other paths stop at "
-                  "bounce 2 and query; relMSE against a plain reference image")
+                  "FP16 MLP 96-64x5-3 trained online (Adam lr 1e-2) on 1/16 of the paths, which stop one bounce "
+                  "later and take the cache's value as their tail (self-training); other paths stop at bounce 2 and "
+                  "query; relMSE against a plain reference image")
+CACHE_RENDER = {"query_bounce": 2, "train_every": 16, "train_suffix": 1}
 MFMA_FP16_PEAK_TFLOPS = 2500.0  # MI355X dense FP16 MFMA
 CACHE_FLOPS_PER_QUERY = 2 * 22720  # one multiply-add per weight
 
@@ -194,7 +196,7 @@ def cache_bench(args):
             torch.cuda.synchronize()
             t1 = time.perf_counter()
             if use_cache:
-                img, st = cache.render(scene, W, H, SPF, DEPTH, seed=seed, sample_offset=f * SPF)
+                img, st = cache.render(scene, W, H, SPF, DEPTH, seed=seed, sample_offset=f * SPF, **CACHE_RENDER)
                 infer_ms += st["ms_infer"]
                 train_ms += st["ms_train"]
                 queries += st["queries"]

@@ -400,15 +400,18 @@ typedef struct prr_cache_render_params {
     int32_t use_cache;        /* 1: non-training paths stop at their first diffuse vertex at or after
                                  query_bounce and add throughput * cache; 0: plain paths (the fallback,
                                  per request) */
-    int32_t train;            /* 1: one path in train_every runs to full depth, its diffuse vertices
-                                 from bounce 1 on become training samples, and Adam runs after the frame */
+    int32_t train;            /* 1: one path in train_every runs on (to full depth, or see train_suffix),
+                                 its diffuse vertices from bounce 1 on become training samples, and Adam
+                                 runs after the frame */
     int32_t query_bounce;     /* 2 */
     int32_t train_every;      /* power of two (16) */
     int32_t max_train_steps;  /* Adam steps per frame at most (batches of train_batch) */
     int32_t train_capacity;   /* training samples kept per frame at most (0: 4M) */
     float divergence_loss;    /* a training loss above this, or not finite, marks the network as
                                  diverged: the next frame re-initialises it and renders plain */
-    int32_t reserved;
+    int32_t train_suffix;     /* 0: training paths run to full depth; n > 0 (with use_cache): they stop
+                                 at their first diffuse vertex at or after query_bounce + n and take the
+                                 cache's value there as their tail (self-training) */
 } prr_cache_render_params;
 
 typedef struct prr_cache_stats {

@@ -589,23 +589,28 @@ static uint32_t trainHash(uint32_t p, uint32_t s) {
 
 static void renderImpl(const orc_scene *s, const prr_render_params *p, int row0, int row1, const orc_photon_map *pm,
                        int threads, float *mean, float *var, int32_t *spp_out, uint64_t *counters, int truncateAt,
-                       uint32_t trainMask);
+                       uint32_t trainMask, int truncateTrainAt);
 
 extern "C" void orc_render(const orc_scene *s, const prr_render_params *p, int row0, int row1,
                            const orc_photon_map *pm, int threads, float *mean, float *var, int32_t *spp_out,
                            uint64_t *counters) {
-    renderImpl(s, p, row0, row1, pm, threads, mean, var, spp_out, counters, 0, 0u);
+    renderImpl(s, p, row0, row1, pm, threads, mean, var, spp_out, counters, 0, 0u, 0);
 }
 
+// The radiance-cache render with a network that outputs 0: paths with
+// trainHash & train_mask != 0 end at their first scattering vertex at bounce
+// >= query_bounce; the others (training paths) at bounce >= train_stop, or
+// run to full depth when train_stop is 0.
 extern "C" void orc_render_truncated(const orc_scene *s, const prr_render_params *p, int row0, int row1,
-                                     int threads, int query_bounce, uint32_t train_mask, float *mean,
-                                     uint64_t *counters) {
-    renderImpl(s, p, row0, row1, nullptr, threads, mean, nullptr, nullptr, counters, query_bounce, train_mask);
+                                     int threads, int query_bounce, uint32_t train_mask, int train_stop,
+                                     float *mean, uint64_t *counters) {
+    renderImpl(s, p, row0, row1, nullptr, threads, mean, nullptr, nullptr, counters, query_bounce, train_mask,
+               train_stop);
 }
 
 static void renderImpl(const orc_scene *s, const prr_render_params *p, int row0, int row1, const orc_photon_map *pm,
                        int threads, float *mean, float *var, int32_t *spp_out, uint64_t *counters, int truncateAt,
-                       uint32_t trainMask) {
+                       uint32_t trainMask, int truncateTrainAt) {
     const Scene &sc = s->sc;
     const int W = p->width, H = p->height;
     if (row1 <= 0 || row1 > H) row1 = H;
@@ -642,9 +647,10 @@ static void renderImpl(const orc_scene *s, const prr_render_params *p, int row0,
                     float oy = rng.next();
                     Ray r = sc.camera.generateRay(i, j, W, H, ox, oy, rng);
                     IntegratorParams ipp = ip;
-                    if (truncateAt > 0 &&
-                        (trainHash(pix, (uint32_t) (p->sample_offset + sIdx)) & trainMask) != 0u)
-                        ipp.truncateAt = truncateAt;
+                    if (truncateAt > 0)
+                        ipp.truncateAt = (trainHash(pix, (uint32_t) (p->sample_offset + sIdx)) & trainMask) != 0u
+                                             ? truncateAt
+                                             : truncateTrainAt;
                     V3 c = castRay(sc, r, rng, ipp, &cnt, pfp);
                     if (std::isnan(c.x) || std::isnan(c.y) || std::isnan(c.z)) c = V3(0.f);
                     sum += c;

@@ -105,10 +105,11 @@ void orc_render(const orc_scene* s, const prr_render_params* p, int row0, int ro
 
 /* orc_render at fixed spp where every path that is not a radiance-cache
  * training path (lowbias32(pixel, sample) & train_mask != 0) ends at its first
- * diffuse vertex at bounce >= query_bounce: the cache render with a network
- * that outputs zero. mean: W*H*3. */
+ * diffuse vertex at bounce >= query_bounce, and training paths at bounce >=
+ * train_stop (0: they run to full depth): the cache render with a network that
+ * outputs zero. mean: W*H*3. */
 void orc_render_truncated(const orc_scene* s, const prr_render_params* p, int row0, int row1,
-                          int threads, int query_bounce, uint32_t train_mask, float* mean,
+                          int threads, int query_bounce, uint32_t train_mask, int train_stop, float* mean,
                           uint64_t* counters);
 
 /* Renderer.cpp:71-99 on given per-sample values: nseq sequences of len RGB

@@ -146,7 +146,7 @@ def lib(fast=False):
     L.orc_render.argtypes = [vp, C.POINTER(RenderParams), C.c_int, C.c_int, vp, C.c_int, vp, vp, vp, vp]
     L.orc_welford.argtypes = [C.c_int, C.c_int, vp, vp, vp, vp, vp]
     L.orc_render_truncated.argtypes = [vp, C.POINTER(RenderParams), C.c_int, C.c_int, C.c_int, C.c_int, C.c_uint32,
-                                       vp, vp]
+                                       C.c_int, vp, vp]
     L.orc_to_half.argtypes = [C.c_int64, vp, vp]
     L.orc_cache_encode.argtypes = [C.c_int64, vp, vp, vp]
     L.orc_cache_forward.argtypes = [C.c_int64, vp, vp, vp, C.c_int, vp]
@@ -282,18 +282,34 @@ class OracleScene:
 
 
     def render_truncated(self, width, height, spp, max_depth, seed, query_bounce, train_every, clamp=10.0,
-                         sample_offset=0, threads=0):
+                         sample_offset=0, threads=0, train_suffix=0):
         """The cache render with a zero network: non-training paths end at
         their first diffuse vertex at bounce >= query_bounce (train_every
-        None: no training paths)."""
+        None: no training paths); training paths end at their first diffuse
+        vertex at bounce >= query_bounce + train_suffix (0: full depth)."""
         p = RenderParams(width=width, height=height, spp=spp, max_depth=max_depth, indirect_clamp=clamp,
                          sample_offset=sample_offset, seed=seed)
         mean = np.zeros((height, width, 3), np.float32)
         cnt = np.zeros(7, np.uint64)
         mask = 0xFFFFFFFF if train_every is None else train_every - 1
-        self.L.orc_render_truncated(self.h, C.byref(p), 0, 0, threads, query_bounce, mask, _p(mean), _p(cnt))
+        stop = query_bounce + train_suffix if train_suffix > 0 else 0
+        self.L.orc_render_truncated(self.h, C.byref(p), 0, 0, threads, query_bounce, mask, stop, _p(mean), _p(cnt))
         keys = ["closest", "any", "samples", "inner", "tris", "inner_any", "tris_any"]
         return mean, dict(zip(keys, (int(x) for x in cnt)))
+
+
+def train_hash(pixel, sample):
+    """Training-path selection hash of the cache render (prr_oracle.cpp
+    trainHash): lowbias32 mix of pixel and sample index; a path is a training
+    path when train_hash & (train_every - 1) == 0."""
+    m = 0xFFFFFFFF
+    h = (pixel * 0x9E3779B1 + sample * 0x85EBCA77) & m
+    h ^= h >> 16
+    h = (h * 0x7FEB352D) & m
+    h ^= h >> 15
+    h = (h * 0x846CA68B) & m
+    h ^= h >> 16
+    return h
 
 
 class OraclePhotonMap:

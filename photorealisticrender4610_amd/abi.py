@@ -83,7 +83,7 @@ class CacheParams(C.Structure):
 class CacheRenderParams(C.Structure):
     _fields_ = [("use_cache", C.c_int32), ("train", C.c_int32), ("query_bounce", C.c_int32),
                 ("train_every", C.c_int32), ("max_train_steps", C.c_int32), ("train_capacity", C.c_int32),
-                ("divergence_loss", C.c_float), ("reserved", C.c_int32)]
+                ("divergence_loss", C.c_float), ("train_suffix", C.c_int32)]
 
 
 class CacheStats(C.Structure):
@@ -552,7 +552,7 @@ class Cache:
 
     def render(self, scene, width, height, spp, max_depth, seed=0, clamp=10.0, sample_offset=0, out=None,
                use_cache=True, train=True, query_bounce=2, train_every=16, max_train_steps=16, train_capacity=0,
-               divergence_loss=1e3, kernel_timing=False):
+               divergence_loss=1e3, kernel_timing=False, train_suffix=0):
         """One frame of Renderer::Render with the cache (prr_cache_render):
         (H, W, 3) device tensor and a stats dict. Blocks until the frame and
         its training are done."""
@@ -564,7 +564,8 @@ class Cache:
                          sample_offset=sample_offset, seed=seed, flags=1 if kernel_timing else 0)
         cp = CacheRenderParams(use_cache=int(use_cache), train=int(train), query_bounce=query_bounce,
                                train_every=train_every, max_train_steps=max_train_steps,
-                               train_capacity=train_capacity, divergence_loss=divergence_loss)
+                               train_capacity=train_capacity, divergence_loss=divergence_loss,
+                               train_suffix=train_suffix)
         st = CacheStats()
         c._bind_stream()
         c._done(lib().prr_cache_render(c.h, scene.h, self.h, C.byref(p), C.byref(cp), _ptr(out), C.byref(st)))

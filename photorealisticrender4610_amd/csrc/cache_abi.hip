@@ -317,7 +317,7 @@ int prr_cache_render(prr_context *c, const prr_scene *s, prr_cache *k, const prr
         return fail(c, PRR_ERR_UNSUPPORTED, "adaptive sampling is not available with the cache");
     if (p->num_photons > 0) return fail(c, PRR_ERR_UNSUPPORTED, "photon gather is not available with the cache");
     if (cp->train_every <= 0 || (cp->train_every & (cp->train_every - 1)) || cp->query_bounce < 1 ||
-        cp->max_train_steps < 0 || cp->train_capacity < 0)
+        cp->max_train_steps < 0 || cp->train_capacity < 0 || cp->train_suffix < 0)
         return fail(c, PRR_ERR_INVALID, "bad cache render parameters");
     const int64_t pixels = (int64_t) p->width * p->height;
     if (pixels > c->maxSlots) return fail(c, PRR_ERR_UNSUPPORTED, "image larger than the context's path slots");
@@ -341,6 +341,7 @@ int prr_cache_render(prr_context *c, const prr_scene *s, prr_cache *k, const prr
     cr.query = useCache ? 1 : 0;
     cr.train = cp->train ? 1 : 0;
     cr.queryBounce = cp->query_bounce;
+    cr.trainStop = useCache && cp->train_suffix > 0 ? cp->query_bounce + cp->train_suffix : 0;
     cr.trainMask = (uint32_t) cp->train_every - 1u;
     cr.tCap = (uint32_t) std::min<int64_t>(tCap, 0xFFFFFFFFll);
     cr.qCount = k->counts;

@@ -498,8 +498,10 @@ __device__ __forceinline__ void shadeImpl(const SceneView &s, const PathBufs &pb
                         const CacheRec &cr = ci.cache;
                         bool cacheQuery = false;
                         if (CACHE && (cr.query | cr.train)) {
-                            if (cr.query && !(fl & F_TRAIN) && bounce >= cr.queryBounce) {
-                                // the path ends here; the cache adds beta * L_out(P, wo)
+                            const int stopAt = (fl & F_TRAIN) ? cr.trainStop : cr.queryBounce;
+                            if (cr.query && stopAt > 0 && bounce >= stopAt) {
+                                // the path ends here; the cache adds beta * L_out(P, wo) (for a
+                                // training path, also to the targets of its recorded vertices)
                                 cacheQuery = true;
                                 done = true;
                                 const uint32_t k = waveAppend(cr.qCount, true);

@@ -51,6 +51,7 @@ struct CacheRec {
     int query = 0;               // paths other than training paths stop and query
     int train = 0;               // training paths are selected and recorded
     int queryBounce = 2;
+    int trainStop = 0;           // training paths stop and query at bounce >= trainStop (0: full depth)
     uint32_t trainMask = 15;
     uint32_t tCap = 0;           // training-record capacity
     uint32_t *qCount = nullptr;  // queries: slot, 13 query floats, throughput

@@ -66,6 +66,27 @@ def test_zero_network_matches_truncated_oracle(ctx):
     assert st2["train_samples"] == 0
 
 
+def test_training_suffix_matches_truncated_oracle(ctx):
+    """Training paths stopped at query_bounce + train_suffix with a zero
+    network: per-path parity with the oracle's truncated render whose
+    training paths stop at the same bounce; fewer rays than full-depth
+    training paths, and training samples are still recorded."""
+    print("case cache render training suffix vs oracle", flush=True)
+    sc, scene, bounds = _box(ctx)
+    cache = ctx.cache(bounds, train_batch=16384)
+    cache.set_weights(np.zeros(22720, np.float32))
+    w, h = 96, 96
+    kw = dict(seed=5, use_cache=True, train=True, query_bounce=2, train_every=4, max_train_steps=0)
+    img, st = cache.render(scene, w, h, 1, 8, train_suffix=2, **kw)
+    ref, cnt = O.OracleScene(sc).render_truncated(w, h, 1, 8, 5, query_bounce=2, train_every=4, train_suffix=2)
+    close = np.all(np.abs(img.cpu().numpy() - ref) <= 1e-4 * np.maximum(np.abs(ref), 1e-2), axis=2)
+    assert close.mean() > 0.98, close.mean()
+    assert abs(st["rays_closest"] - cnt["closest"]) <= 0.01 * cnt["closest"]
+    assert st["train_samples"] > 0
+    _, st_full = cache.render(scene, w, h, 1, 8, **kw)
+    assert st["rays_closest"] < st_full["rays_closest"]
+
+
 def test_trained_cache_recovers_truncated_light(ctx):
     """What the trained network gains over the method without it: paths that
     stop at bounce 2 lose the box room's multiple bounces; the cache trained
