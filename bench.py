@@ -8,7 +8,7 @@ the scene already resident in HBM. Prints ONE JSON line.
 
 Usage: python bench.py [--gpus 1] [--steps 3] [--warmup 1]
                        [--workload sphere_field|glass_torus|box_room|radiance_cache|serving]
-                       [--no-cpu] [--spp N] [--ref-spp N] [--frames N] [--requests N]
+                       [--no-cpu] [--spp N] [--ref-spp N] [--frames N] [--requests N] [--rate R]
 
 --workload radiance_cache prints the equal-time relMSE line of the learned
 radiance cache against the plain tracer (60 frames of 4 spp, plain reference
@@ -51,7 +51,8 @@ CACHE_WORKLOAD = ("sphere_field", 1920, 1080, 4, 12, 60,
                   "radiance_cache: sphere_field (1,048,578 tris), 1920x1080, 4 spp per frame x 60 frames, depth 12; "
                   "FP16 MLP 96-64x5-3 trained online (Adam lr 1e-2) on 1/16 of the paths, which stop one bounce "
                   "later and take the cache's value as their tail (self-training); other paths stop at bounce 2 and "
-                  "query; relMSE against a plain reference image")
+                  "query (full_depth_training: the same with training paths run to full depth); relMSE against a "
+                  "plain reference image")
 CACHE_RENDER = {"query_bounce": 2, "train_every": 16, "train_suffix": 1}
 MFMA_FP16_PEAK_TFLOPS = 2500.0  # MI355X dense FP16 MFMA
 CACHE_FLOPS_PER_QUERY = 2 * 22720  # one multiply-add per weight
@@ -185,7 +186,8 @@ def cache_bench(args):
             log("reference %d/%d spp (%.0f s)" % (k + chunk, args.ref_spp, time.time() - t0))
     ref = (ref / (args.ref_spp // chunk)).float()
 
-    def run(use_cache):
+    def run(use_cache, render_args=None):
+        render_args = CACHE_RENDER if render_args is None else render_args
         cache = ctx.cache(bounds, seed=seed) if use_cache else None
         acc = torch.zeros((H, W, 3), dtype=torch.float32, device="cuda")
         curve = []
@@ -196,7 +198,7 @@ def cache_bench(args):
             torch.cuda.synchronize()
             t1 = time.perf_counter()
             if use_cache:
-                img, st = cache.render(scene, W, H, SPF, DEPTH, seed=seed, sample_offset=f * SPF, **CACHE_RENDER)
+                img, st = cache.render(scene, W, H, SPF, DEPTH, seed=seed, sample_offset=f * SPF, **render_args)
                 infer_ms += st["ms_infer"]
                 train_ms += st["ms_train"]
                 queries += st["queries"]
@@ -213,13 +215,15 @@ def cache_bench(args):
 
     plain, pst = run(False)
     cached, cst = run(True)
+    full, fst = run(True, dict(CACHE_RENDER, train_suffix=0))  # training paths to full depth
     t_plain, e_plain = plain[-1]
-    # time the cache needs to reach the plain tracer's final relMSE
-    t_reach = None
-    for (ta, ea), (tb, eb) in zip([(0.0, float("inf"))] + cached[:-1], cached):
-        if eb <= e_plain:
-            t_reach = ta + (tb - ta) * ((ea - e_plain) / (ea - eb) if np.isfinite(ea) and ea > eb else 1.0)
-            break
+
+    def reach(curve):  # time a curve needs to reach the plain tracer's final relMSE
+        for (ta, ea), (tb, eb) in zip([(0.0, float("inf"))] + curve[:-1], curve):
+            if eb <= e_plain:
+                return ta + (tb - ta) * ((ea - e_plain) / (ea - eb) if np.isfinite(ea) and ea > eb else 1.0)
+        return None
+    t_reach, t_reach_full = reach(cached), reach(full)
 
     def at(curve, t):  # relMSE of the running mean at time t (last frame done by t)
         e = None
@@ -242,6 +246,9 @@ def cache_bench(args):
                         "queries": cst["queries"], "train_ms": round(cst["train_ms"], 2),
                         "infer_ms": round(cst["infer_ms"], 2)},
         "cache_time_to_plain_relmse_ratio": round(t_reach / t_plain, 4) if t_reach is not None else None,
+        "full_depth_training": {"relmse": full[-1][1], "time_s": round(full[-1][0], 4), "rays": fst["rays"],
+                                "time_to_plain_relmse_ratio": round(t_reach_full / t_plain, 4)
+                                if t_reach_full is not None else None},
         "roofline": {"bound": "mfma", "achieved": round(infer_tflops, 2), "peak": MFMA_FP16_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": round(infer_tflops / MFMA_FP16_PEAK_TFLOPS, 5), "traffic": None,
                      "kernel": "kCacheInfer (fused MLP forward, %d FLOP/query)" % CACHE_FLOPS_PER_QUERY},
@@ -256,13 +263,13 @@ SERVING = {"requests": 400, "rate": 20.0, "sizes": (256, 512, 1024), "spps": (1,
                    "64}, camera on a sphere of 2.5x the scene's bounding radius aimed at its centre; 400 requests"}
 
 
-def serving_requests(n, infos, seed):
+def serving_requests(n, infos, seed, rate):
     """The seeded request list: (arrival s, scene, size, spp, eye, center, fovh)."""
     rng = np.random.default_rng(seed)
     t = 0.0
     out = []
     for i in range(n):
-        t += rng.exponential(1.0 / SERVING["rate"])
+        t += rng.exponential(1.0 / rate)
         si = int(rng.integers(0, 2))
         size = int(rng.choice(SERVING["sizes"]))
         spp = int(rng.choice(SERVING["spps"]))
@@ -285,7 +292,8 @@ def serving_bench(args):
     dev = [ctx.scene(sc) for sc in scs]
     infos = [dict(d.info(), fovh=float(sc.camera[9])) for d, sc in zip(dev, scs)]
     n = args.requests if args.requests > 0 else SERVING["requests"]
-    reqs = serving_requests(n, infos, SERVING["seed"])
+    rate = args.rate if args.rate > 0 else SERVING["rate"]
+    reqs = serving_requests(n, infos, SERVING["seed"], rate)
     depths = [8, 12]
 
     def submit(srv, k):
@@ -350,7 +358,9 @@ def serving_bench(args):
         "metric": "p50/p99 request latency", "value": batched["p99_ms"], "unit": "ms (p99)", "n_gpus": 1,
         "steps": n, "warmup": 0, "ms_per_step": round(batched["span_s"] * 1e3 / n, 2), "higher_is_better": False,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded procedural scenes)",
-        "config": {"workload": SERVING["desc"], "requests": n, "rate_per_s": SERVING["rate"]},
+        "config": {"workload": SERVING["desc"] if rate == SERVING["rate"] else
+                   SERVING["desc"].replace("%g req/s" % SERVING["rate"], "%g req/s" % rate), "requests": n,
+                   "rate_per_s": rate},
         "p50_ms": batched["p50_ms"], "p99_ms": batched["p99_ms"],
         "msamples_per_s": batched["msamples_per_s"],
         "mrays_per_s": round(batched["rays"] / batched["span_s"] / 1e6, 2),
@@ -371,6 +381,7 @@ def main():
     ap.add_argument("--workload", default="sphere_field",
                     choices=sorted(WORKLOADS) + ["radiance_cache", "serving"])
     ap.add_argument("--requests", type=int, default=0, help="serving: override the request count")
+    ap.add_argument("--rate", type=float, default=0.0, help="serving: override the arrival rate (requests/s)")
     ap.add_argument("--ref-spp", type=int, default=16384, help="radiance_cache: reference samples per pixel")
     ap.add_argument("--frames", type=int, default=0, help="radiance_cache: override the frame count")
     ap.add_argument("--spp", type=int, default=0, help="override spp (profiling runs only)")
