@@ -432,44 +432,41 @@ PRR_DEV bool anyBegin(const SceneView &s, TravState &ts) {
     return tb < ts.tmax;
 }
 
-// BVHAccel::IntersectP at a leaf; true when finished (ts.h.prim = 1 if
-// occluded, -1 otherwise).
+// Any-hit steps return kAnyGo (continue at ts.cur), kAnyClear (stack empty:
+// not occluded) or kAnyHit (occluded). The answer travels in the return
+// value, not in TravState, so the compiler never merges stores to different
+// TravState fields into one address-selected store (which would put the
+// traversal state in scratch memory).
+constexpr int kAnyGo = 0, kAnyClear = 1, kAnyHit = 2;
+
+// BVHAccel::IntersectP at a leaf.
 PRR_DEV uint32_t popRef(LaneStack &st) { return st.pop().x; }
 PRR_DEV uint32_t popRef(RefStack &st) { return st.pop(); }
 
 template <class Stack>
-PRR_DEV bool anyPop(TravState &ts, Stack &st) {
-    if (st.sp == 0) {
-        ts.h.prim = -1;
-        return true;
-    }
+PRR_DEV int anyPop(TravState &ts, Stack &st) {
+    if (st.sp == 0) return kAnyClear;
     ts.cur = popRef(st);
-    return false;
+    return kAnyGo;
 }
 
 template <class Stack>
-PRR_DEV bool anyLeaf(const SceneView &s, TravState &ts, Stack &st) {
+PRR_DEV int anyLeaf(const SceneView &s, TravState &ts, Stack &st) {
     const uint32_t cur = ts.cur;
     const uint32_t first = refFirst(cur), cnt = refCount(cur);
     if (refSphere(cur)) {
         for (uint32_t i = 0; i < cnt; i++) {
             float t;
-            if (sphHit(s.sph[first + i], ts.o, ts.d, t) && t < ts.tmax) {
-                ts.h.prim = 1;
-                return true;
-            }
+            if (sphHit(s.sph[first + i], ts.o, ts.d, t) && t < ts.tmax) return kAnyHit;
         }
     } else {
         for (uint32_t i = 0; i < cnt; i++)
-            if (triAny(loadTri(s.tris, first + i), ts.o, ts.d, ts.tmax)) {
-                ts.h.prim = 1;
-                return true;
-            }
+            if (triAny(loadTri(s.tris, first + i), ts.o, ts.d, ts.tmax)) return kAnyHit;
     }
     return anyPop(ts, st);
 }
 
-PRR_DEV bool anyInterior(const SceneView &s, TravState &ts, LaneStack &st) {
+PRR_DEV int anyInterior(const SceneView &s, TravState &ts, LaneStack &st) {
     const DNode n = s.nodes[ts.cur];
     const float tL = slab(ts.r, n.a.x, n.a.y, n.a.z, n.a.w, n.b.x, n.b.y);
     const float tR = slab(ts.r, n.b.z, n.b.w, n.c.x, n.c.y, n.c.z, n.c.w);
@@ -479,27 +476,27 @@ PRR_DEV bool anyInterior(const SceneView &s, TravState &ts, LaneStack &st) {
         const bool lf = tL <= tR;
         st.push(lf ? R : L, 0.f);
         ts.cur = lf ? L : R;
-        return false;
+        return kAnyGo;
     }
     if (hl || hr) {
         ts.cur = hl ? L : R;
-        return false;
+        return kAnyGo;
     }
     return anyPop(ts, st);
 }
 
-// One step of BVHAccel::IntersectP; returns true when finished, with
-// ts.h.prim = 1 if occluded, -1 otherwise.
-PRR_DEV bool anyStep(const SceneView &s, TravState &ts, LaneStack &st) {
+// One step of BVHAccel::IntersectP.
+PRR_DEV int anyStep(const SceneView &s, TravState &ts, LaneStack &st) {
     return refIsLeaf(ts.cur) ? anyLeaf(s, ts, st) : anyInterior(s, ts, st);
 }
 
 // While-while iteration of IntersectP (see closestIter).
-PRR_DEV bool anyIter(const SceneView &s, TravState &ts, LaneStack &st, int cap) {
+PRR_DEV int anyIter(const SceneView &s, TravState &ts, LaneStack &st, int cap) {
     int k = 0;
     while (!refIsLeaf(ts.cur)) {
-        if (anyInterior(s, ts, st)) return true;
-        if (++k == cap) return false;
+        const int r = anyInterior(s, ts, st);
+        if (r != kAnyGo) return r;
+        if (++k == cap) return kAnyGo;
     }
     return anyLeaf(s, ts, st);
 }
@@ -510,7 +507,7 @@ PRR_DEV bool anyIter(const SceneView &s, TravState &ts, LaneStack &st, int cap) 
 // box, which the slab test never enters. The answer equals
 // BVHAccel::IntersectP's on the binary tree: the occlusion tree keeps its
 // leaves and skips only boxes that contain their children's boxes.
-PRR_DEV bool anyInterior4(const SceneView &s, TravState &ts, RefStack &st) {
+PRR_DEV int anyInterior4(const SceneView &s, TravState &ts, RefStack &st) {
     const DNode4 *n = s.nodes4 + ts.cur;
     const float4 lx = n->lox, hx = n->hix, ly = n->loy, hy = n->hiy, lz = n->loz, hz = n->hiz;
     const uint4 rf = n->ref;
@@ -538,19 +535,20 @@ PRR_DEV bool anyInterior4(const SceneView &s, TravState &ts, RefStack &st) {
     const int m = (int) h0 + (int) h1 + (int) h2 + (int) h3;
     st.push3(p0, p1, p2, m);
     ts.cur = bk == 0 ? rf.x : (bk == 1 ? rf.y : (bk == 2 ? rf.z : rf.w));
-    return false;
+    return kAnyGo;
 }
 
-PRR_DEV bool anyStep4(const SceneView &s, TravState &ts, RefStack &st) {
+PRR_DEV int anyStep4(const SceneView &s, TravState &ts, RefStack &st) {
     return refIsLeaf(ts.cur) ? anyLeaf(s, ts, st) : anyInterior4(s, ts, st);
 }
 
 // While-while iteration on the occlusion tree (see closestIter).
-PRR_DEV bool anyIter4(const SceneView &s, TravState &ts, RefStack &st, int cap) {
+PRR_DEV int anyIter4(const SceneView &s, TravState &ts, RefStack &st, int cap) {
     int k = 0;
     while (!refIsLeaf(ts.cur)) {
-        if (anyInterior4(s, ts, st)) return true;
-        if (++k == cap) return false;
+        const int r = anyInterior4(s, ts, st);
+        if (r != kAnyGo) return r;
+        if (++k == cap) return kAnyGo;
     }
     return anyLeaf(s, ts, st);
 }
@@ -561,9 +559,10 @@ PRR_DEV bool traceAny4(const SceneView &s, const V3 &o, const V3 &d, float tMax,
     TravState ts;
     travBegin(ts, o, d, tMax, s.root4, st);
     if (!anyBegin(s, ts)) return false;
-    while (!anyStep4(s, ts, st)) {
+    int r;
+    while ((r = anyStep4(s, ts, st)) == kAnyGo) {
     }
-    return ts.h.prim > 0;
+    return r == kAnyHit;
 }
 
 // Work fetch through a wave-private range of queue entries. A wave claims a

@@ -280,15 +280,15 @@ __global__ __launch_bounds__(kTraceBlock) void kShadow(SceneView s, PathBufs pb,
             }
         }
         if (__ballot(idx >= 0 || !exhausted) == 0) break;
-        bool done = false;
+        int res = kAnyGo;
         if (idx >= 0) {
             if constexpr (WIDE)
-                done = WW ? anyIter4(s, ts, st, wwCap) : anyStep4(s, ts, st);
+                res = WW ? anyIter4(s, ts, st, wwCap) : anyStep4(s, ts, st);
             else
-                done = WW ? anyIter(s, ts, st, wwCap) : anyStep(s, ts, st);
+                res = WW ? anyIter(s, ts, st, wwCap) : anyStep(s, ts, st);
         }
-        if (done) {
-            pb.vis[(int64_t) (sk >> 30) * pb.slots + (sk & 0x3FFFFFFFu)] = ts.h.prim > 0 ? 0 : 1;
+        if (res != kAnyGo) {
+            pb.vis[(int64_t) (sk >> 30) * pb.slots + (sk & 0x3FFFFFFFu)] = res == kAnyHit ? 0 : 1;
             idx = -1;
         }
     }
