@@ -142,21 +142,35 @@ int runBounces(prr_context *c, const SceneView &view, const ChunkInfo &ci, Timin
         tm.timed.emplace_back(kind, i);
         return i;
     };
-#define TIMED(kind, launch)                                             \
-    do {                                                                \
-        int ei_ = evPair(kind);                                         \
-        if (ei_ >= 0) HIPCK(c, hipEventRecord(c->evPool[ei_], st));     \
-        HIPCK(c, launch);                                               \
-        if (ei_ >= 0) HIPCK(c, hipEventRecord(c->evPool[ei_ + 1], st)); \
+#define TIMED(kind, strm, launch)                                         \
+    do {                                                                  \
+        int ei_ = evPair(kind);                                           \
+        if (ei_ >= 0) HIPCK(c, hipEventRecord(c->evPool[ei_], strm));     \
+        HIPCK(c, launch);                                                 \
+        if (ei_ >= 0) HIPCK(c, hipEventRecord(c->evPool[ei_ + 1], strm)); \
     } while (0)
+    // Bounce b: extension rays of b and shadow rays of b - 1 (both written by
+    // shade b - 1, independent of each other), then shade b, which reads both.
+    const bool conc = c->concurrentShadow && c->aux;
+    const int wwShadow = c->shadowWide ? c->whileWhileAny : c->whileWhile;
     for (int b = 0; b <= ci.maxDepth; b++) {
-        TIMED(0, launchExtend(view, c->pb, b, c->spill, c->traceBlocks, c->refillAt, c->stackDepth, c->whileWhile,
-                              c->fetchRun, st));
-        if (b > 0)
-            TIMED(1, launchShadow(view, c->pb, b - 1, c->spill, c->traceBlocks, c->refillAt, c->stackDepth,
-                                  c->shadowWide ? c->whileWhileAny : c->whileWhile, c->fetchRun, c->shadowWide != 0,
-                                  st));
-        TIMED(2, launchShade(view, c->pb, ci, b, c->shadeBlocks, c->shadeWaves, st));
+        if (b > 0 && conc) {
+            HIPCK(c, hipEventRecord(c->evFork, st));
+            HIPCK(c, hipStreamWaitEvent(c->aux, c->evFork, 0));
+            TIMED(1, c->aux,
+                  launchShadow(view, c->pb, b - 1, c->spillAux, c->traceBlocks, c->refillAt, c->stackDepth, wwShadow,
+                               c->fetchRun, c->shadowWide != 0, c->aux));
+            HIPCK(c, hipEventRecord(c->evJoin, c->aux));
+        }
+        TIMED(0, st,
+              launchExtend(view, c->pb, b, c->spill, c->traceBlocks, c->refillAt, c->stackDepth, c->whileWhile,
+                           c->fetchRun, st));
+        if (b > 0 && !conc)
+            TIMED(1, st,
+                  launchShadow(view, c->pb, b - 1, c->spill, c->traceBlocks, c->refillAt, c->stackDepth, wwShadow,
+                               c->fetchRun, c->shadowWide != 0, st));
+        if (b > 0 && conc) HIPCK(c, hipStreamWaitEvent(st, c->evJoin, 0));
+        TIMED(2, st, launchShade(view, c->pb, ci, b, c->shadeBlocks, c->shadeWaves, st));
     }
 #undef TIMED
     return PRR_OK;
@@ -245,8 +259,11 @@ int prr_create(int device, prr_context **out) {
     hipDeviceProp_t prop;
     if (e == hipSuccess) e = hipGetDeviceProperties(&prop, device);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->own, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->aux, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreate(&c->ev0);
     if (e == hipSuccess) e = hipEventCreate(&c->ev1);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&c->evFork, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&c->evJoin, hipEventDisableTiming);
     if (e != hipSuccess) {
         std::string m = std::string("device init: ") + hipGetErrorString(e);
         delete c;
@@ -256,10 +273,11 @@ int prr_create(int device, prr_context **out) {
     c->numCU = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 1;
     // persistent traversal grid: 8 blocks of 256 lanes per CU (with 8-entry
     // LDS stacks, 16 KB per block), a wave refills once 40 of its lanes are
-    // idle, at most 8 interior steps (4 for shadow rays) per work check (measured on the 1M-tri
-    // scene); PRR_TRACE_BLOCKS_PER_CU, PRR_STACK_LDS, PRR_REFILL_AT,
-    // PRR_WHILE_WHILE(_ANY), PRR_FETCH_RUN, PRR_SHADOW_WIDE and PRR_SHADE_WAVES override the measured
-    // defaults for tuning sweeps
+    // idle, at most 8 interior steps (4 for shadow rays) per work check
+    // (measured on the 1M-tri scene). PRR_TRACE_BLOCKS_PER_CU, PRR_STACK_LDS,
+    // PRR_REFILL_AT, PRR_WHILE_WHILE(_ANY), PRR_FETCH_RUN, PRR_SHADOW_WIDE,
+    // PRR_CONCURRENT_SHADOW and PRR_SHADE_WAVES override the measured defaults
+    // for tuning sweeps.
     int tbpc = 8;
     if (const char *ev = getenv("PRR_TRACE_BLOCKS_PER_CU")) tbpc = std::max(1, std::min(16, atoi(ev)));
     c->traceBlocks = c->numCU * tbpc;
@@ -275,6 +293,7 @@ int prr_create(int device, prr_context **out) {
     if (const char *ev = getenv("PRR_WHILE_WHILE")) c->whileWhile = std::max(0, atoi(ev));
     if (const char *ev = getenv("PRR_WHILE_WHILE_ANY")) c->whileWhileAny = std::max(0, atoi(ev));
     if (const char *ev = getenv("PRR_SHADOW_WIDE")) c->shadowWide = atoi(ev) != 0;
+    if (const char *ev = getenv("PRR_CONCURRENT_SHADOW")) c->concurrentShadow = atoi(ev) != 0;
     if (const char *ev = getenv("PRR_FETCH_RUN")) c->fetchRun = std::max(64, std::min(1 << 16, atoi(ev)));
     if (const char *ev = getenv("PRR_SHADE_WAVES")) c->shadeWaves = std::max(1, std::min(5, atoi(ev)));
     if (const char *ev = getenv("PRR_STACK_LDS")) {  // LDS stack entries per lane: 2, 4, 8, 16 or 32
@@ -284,6 +303,7 @@ int prr_create(int device, prr_context **out) {
     c->shadeBlocks = c->numCU * 8;
     c->spillLanes = c->traceBlocks * kTraceBlock;
     e = hipMalloc(&c->spill, (size_t) c->spillLanes * kSpill * sizeof(uint2));
+    if (e == hipSuccess) e = hipMalloc(&c->spillAux, (size_t) c->spillLanes * kSpill * sizeof(uint2));
     if (e == hipSuccess) e = hipMalloc(&c->stats, 8 * sizeof(unsigned long long));
     if (e != hipSuccess) {
         std::string m = std::string("device alloc: ") + hipGetErrorString(e);
@@ -298,14 +318,19 @@ int prr_destroy(prr_context *c) {
     if (!c) return PRR_OK;
     (void) hipSetDevice(c->device);
     (void) hipStreamSynchronize(c->stream);
+    if (c->aux) (void) hipStreamSynchronize(c->aux);
     freeRenderBufs(c);
     if (c->fb) (void) hipFree(c->fb);
     if (c->spill) (void) hipFree(c->spill);
+    if (c->spillAux) (void) hipFree(c->spillAux);
     if (c->stats) (void) hipFree(c->stats);
     if (c->ev0) (void) hipEventDestroy(c->ev0);
     if (c->ev1) (void) hipEventDestroy(c->ev1);
     for (hipEvent_t e : c->evPool) (void) hipEventDestroy(e);
+    if (c->evFork) (void) hipEventDestroy(c->evFork);
+    if (c->evJoin) (void) hipEventDestroy(c->evJoin);
     if (c->own) (void) hipStreamDestroy(c->own);
+    if (c->aux) (void) hipStreamDestroy(c->aux);
     delete c;
     return PRR_OK;
 }

@@ -30,6 +30,13 @@ struct prr_context {
     int64_t maxSlots = 1 << 24;  // paths per chunk (set at creation from free HBM)
     uint2 *spill = nullptr;
     int spillLanes = 0;
+    // shadow rays of bounce b - 1 are traced on `aux` while the extension
+    // rays of bounce b are traced on the render stream (each launch's tail
+    // overlaps the other's bulk); the aux launches use their own spill region
+    int concurrentShadow = 1;
+    hipStream_t aux = nullptr;
+    hipEvent_t evFork = nullptr, evJoin = nullptr;
+    uint2 *spillAux = nullptr;
     // render buffers
     prr::PathBufs pb{};
     int64_t pbSlots = 0;
