@@ -8,8 +8,9 @@
 //                inputs kept in LDS ([feature][query] FP16), loss gradient of
 //                the relative L2 loss, weight gradients as MFMA products over
 //                the block's queries, written as one partial per block
-//   kCacheAdam   sums the block partials in block order (deterministic), then
-//                Adam, FP32 master weights and the two FP16 layouts
+//   kCacheAdam   sums the block partials in a fixed order (eight runs of
+//                consecutive blocks, then the runs; deterministic), then Adam,
+//                FP32 master weights and the two FP16 layouts
 #include "cache.hpp"
 #include "launch_cache.hpp"
 
@@ -308,15 +309,33 @@ PRR_DEV void cacheIndex(int w, int &l, int &o, int &i) {
     i = r - o * cacheIn(l);
 }
 
-__global__ void kCacheAdam(float *__restrict__ master, float *__restrict__ m1, float *__restrict__ m2,
-                           const float *__restrict__ partial, int nblocks, CacheOpt opt, _Float16 *__restrict__ wt,
-                           _Float16 *__restrict__ wn, float *__restrict__ gradOut) {
-    const int w = blockIdx.x * blockDim.x + threadIdx.x;
-    if (w >= kCacheWeights) return;
+// Block of 256 threads = 32 weights x 8 groups: group g sums the partials of
+// blocks [g nb / 8, (g + 1) nb / 8) in block order, then the group sums are
+// added in group order (deterministic; spreads the 4 * nblocks * 22720-byte
+// read over many CUs instead of one thread per weight).
+constexpr int kAdamW = 32, kAdamG = 8;
+__global__ __launch_bounds__(kAdamW * kAdamG) void kCacheAdam(float *__restrict__ master, float *__restrict__ m1,
+                                                               float *__restrict__ m2, const float *__restrict__ partial,
+                                                               int nblocks, CacheOpt opt, _Float16 *__restrict__ wt,
+                                                               _Float16 *__restrict__ wn, float *__restrict__ gradOut) {
+    __shared__ float gpart[kAdamG][kAdamW];
+    const int wl = threadIdx.x % kAdamW, g = threadIdx.x / kAdamW;
+    const int w = blockIdx.x * kAdamW + wl;
+    if (partial) {
+        float gsum = 0.f;
+        if (w < kCacheWeights) {
+            const int b0 = (int) ((int64_t) g * nblocks / kAdamG), b1 = (int) ((int64_t) (g + 1) * nblocks / kAdamG);
+            for (int b = b0; b < b1; b++) gsum += partial[(int64_t) b * kCacheWeights + w];
+        }
+        gpart[g][wl] = gsum;
+    }
+    __syncthreads();
+    if (g != 0 || w >= kCacheWeights) return;
     float wv = master[w];
     if (partial) {
         float gsum = 0.f;
-        for (int b = 0; b < nblocks; b++) gsum += partial[(int64_t) b * kCacheWeights + w];
+#pragma unroll
+        for (int k = 0; k < kAdamG; k++) gsum += gpart[k][wl];
         const float gr = gsum * opt.gscale;
         if (gradOut) {
             gradOut[w] = gr;
@@ -368,8 +387,8 @@ hipError_t launchCacheTrain(const CacheView &cv, int64_t n, const float *q, cons
 
 hipError_t launchCacheAdam(float *master, float *m1, float *m2, const float *partial, int nblocks, const CacheOpt &opt,
                            _Float16 *wt, _Float16 *wn, float *gradOut, hipStream_t st) {
-    hipLaunchKernelGGL(kCacheAdam, dim3((kCacheWeights + 255) / 256), dim3(256), 0, st, master, m1, m2, partial,
-                       nblocks, opt, wt, wn, gradOut);
+    hipLaunchKernelGGL(kCacheAdam, dim3((kCacheWeights + kAdamW - 1) / kAdamW), dim3(kAdamW * kAdamG), 0, st, master,
+                       m1, m2, partial, nblocks, opt, wt, wn, gradOut);
     return hipGetLastError();
 }
 
