@@ -663,29 +663,33 @@ PRR_DEV V3 photonRadiance(const SceneView &s, const V3 &x, const V3 &wo, const F
     const int x0 = (int) floorf((x.x - r) / cs), x1 = (int) floorf((x.x + r) / cs);
     const int y0 = (int) floorf((x.y - r) / cs), y1 = (int) floorf((x.y + r) / cs);
     const int z0 = (int) floorf((x.z - r) / cs), z1 = (int) floorf((x.z + r) / cs);
-    uint32_t seen[8];
-    int ns = 0;
-    for (int cz = z0; cz <= z1 && cz <= z0 + 1; cz++)
-        for (int cy = y0; cy <= y1 && cy <= y0 + 1; cy++)
-            for (int cx = x0; cx <= x1 && cx <= x0 + 1; cx++) {
-                const uint32_t h = photonHash(cx, cy, cz, s.ph_mask);
-                bool dup = false;
-                for (int k = 0; k < ns; k++) dup |= seen[k] == h;
-                if (dup) continue;
-                seen[ns++] = h;
-                const uint32_t b = s.ph_start[h], e = s.ph_end[h];
-                for (uint32_t i = b; i < e; i++) {
-                    const V3 p = ld3(s.ph_pos[i]);
-                    if (norm(p - x) > r) continue;
-                    const V3 wi = -ld3(s.ph_dir[i]);
-                    const float c = dot(wi, fr.N);
-                    if (c <= 1e-4f) continue;
-                    V3 f;
-                    float pdf;
-                    evalPdf(lb, fr, wi, wo, f, pdf);
-                    sum += f * ld3(s.ph_pow[i]) / c;
-                }
-            }
+    // cells (cx, cy, cz) in [x0, min(x1, x0 + 1)] x ... in z, y, x order; a
+    // cell whose hash bucket an earlier cell already visited is skipped (its
+    // hash is recomputed rather than kept in a per-lane array, which would
+    // live in scratch memory)
+    for (int k = 0; k < 8; k++) {
+        const int cx = x0 + (k & 1), cy = y0 + ((k >> 1) & 1), cz = z0 + (k >> 2);
+        if (cx > x1 || cy > y1 || cz > z1) continue;
+        const uint32_t h = photonHash(cx, cy, cz, s.ph_mask);
+        bool dup = false;
+        for (int j = 0; j < k; j++) {
+            const int jx = x0 + (j & 1), jy = y0 + ((j >> 1) & 1), jz = z0 + (j >> 2);
+            dup |= jx <= x1 && jy <= y1 && jz <= z1 && photonHash(jx, jy, jz, s.ph_mask) == h;
+        }
+        if (dup) continue;
+        const uint32_t b = s.ph_start[h], e = s.ph_end[h];
+        for (uint32_t i = b; i < e; i++) {
+            const V3 p = ld3(s.ph_pos[i]);
+            if (norm(p - x) > r) continue;
+            const V3 wi = -ld3(s.ph_dir[i]);
+            const float c = dot(wi, fr.N);
+            if (c <= 1e-4f) continue;
+            V3 f;
+            float pdf;
+            evalPdf(lb, fr, wi, wo, f, pdf);
+            sum += f * ld3(s.ph_pow[i]) / c;
+        }
+    }
     return sum / (kPi * r * r);
 }
 
