@@ -398,30 +398,49 @@ PRR_DEV bool closestStep(const SceneView &s, TravState &ts, LaneStack &st) {
 // descent then resumes at the next call. The interior step is written with
 // one pop site so the loop carries few divergent exits.
 PRR_DEV bool closestIter(const SceneView &s, TravState &ts, LaneStack &st, int cap) {
+    // Loop-carried state in locals; the closest t does not change during
+    // interior steps (only leaves update it).
+    uint32_t cur = ts.cur;
+    int sp = st.sp;
+    const float best = ts.h.t;
+    const char *nodes = reinterpret_cast<const char *>(s.nodes);
+    uint2 *lds = st.lds + st.lane;
+    const int dmask = st.depth - 1;
     bool done = false;
-    for (int k = 0; !done && k != cap && !refIsLeaf(ts.cur); k++) {  // one loop exit
+    for (int k = 0; !done && k != cap && !refIsLeaf(cur); k++) {  // one loop exit
         bool pop = true;
-        if (ts.cur != kCulled) {
-            const DNode n = s.nodes[ts.cur];
+        if (cur != kCulled) {
+            // node byte offset fits 32 bits (at most 2^26 nodes of 64 B)
+            const DNode n = *reinterpret_cast<const DNode *>(nodes + (cur << 6));
             const float tL = slab(ts.r, n.a.x, n.a.y, n.a.z, n.a.w, n.b.x, n.b.y);
             const float tR = slab(ts.r, n.b.z, n.b.w, n.c.x, n.c.y, n.c.z, n.c.w);
             const bool lFirst = tL <= tR;
             const float tNear = lFirst ? tL : tR, tFar = lFirst ? tR : tL;
             const uint32_t rNear = (uint32_t) (lFirst ? n.d.x : n.d.y), rFar = (uint32_t) (lFirst ? n.d.y : n.d.x);
-            if (tFar < ts.h.t) st.push(rFar, tFar);
-            pop = !(tNear < ts.h.t);
-            ts.cur = rNear;
+            if (tFar < best) {  // LaneStack::push
+                uint2 *slot = lds + (sp & dmask) * kTraceBlock;
+                if (sp >= st.depth) st.spill[sp - st.depth] = *slot;
+                *slot = make_uint2(rFar, __float_as_uint(tFar));
+                sp++;
+            }
+            pop = !(tNear < best);
+            cur = rNear;
         }
         if (pop) {
-            done = st.sp == 0;
-            if (!done) {
-                const uint2 e = st.pop();
-                ts.cur = __uint_as_float(e.y) < ts.h.t ? e.x : kCulled;
+            done = sp == 0;
+            if (!done) {  // LaneStack::pop
+                sp--;
+                uint2 *slot = lds + (sp & dmask) * kTraceBlock;
+                const uint2 e = *slot;
+                if (sp >= st.depth) *slot = st.spill[sp - st.depth];
+                cur = __uint_as_float(e.y) < best ? e.x : kCulled;
             }
         }
     }
+    ts.cur = cur;
+    st.sp = sp;
     if (done) return true;
-    if (!refIsLeaf(ts.cur)) return false;  // step cap reached
+    if (!refIsLeaf(cur)) return false;  // step cap reached
     return closestLeaf(s, ts, st);
 }
 
