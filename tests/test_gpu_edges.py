@@ -1,0 +1,156 @@
+"""Edge cases of the traversal and render entry points against the oracle:
+empty and ragged batches, axis-aligned rays (zero direction components make
+the slab test's 1/d infinite and (bound - o) * inf NaN when the origin lies on
+a box plane), rays along shared triangle edges, any-hit range limits, scenes
+of one triangle or spheres only, tiny and ragged images, and parameter errors.
+Geometry is bit-exact (closest through the binary tree, any-hit through the
+four-wide occlusion tree)."""
+import numpy as np
+import pytest
+
+import pyoracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    from photorealisticrender4610_amd import abi
+    c = abi.Context(0)
+    yield c
+    c.close()
+
+
+def _np(t):
+    return t.detach().cpu().numpy()
+
+
+def _check(ctx, sc, org, d, tmax):
+    scene = ctx.scene(sc)
+    orc = O.OracleScene(sc)
+    org = np.ascontiguousarray(org, np.float32)
+    d = np.ascontiguousarray(d, np.float32)
+    tmax = np.ascontiguousarray(tmax, np.float32)
+    prim, t, u, v = (_np(x) for x in ctx.trace_closest(scene, org, d))
+    oprim, ot, ou, ov = orc.closest(org, d)
+    assert np.array_equal(prim, oprim)
+    hit = prim >= 0
+    assert np.array_equal(t[hit], ot[hit]) and np.array_equal(u[hit], ou[hit]) and np.array_equal(v[hit], ov[hit])
+    anyh = _np(ctx.trace_any(scene, org, d, tmax))
+    assert np.array_equal(anyh, orc.any(org, d, tmax))
+    return prim, anyh
+
+
+def test_empty_and_ragged_batches(ctx):
+    from photorealisticrender4610_amd import scenes
+    print("case edges: batches", flush=True)
+    sc = scenes.box_room()
+    scene = ctx.scene(sc)
+    z = np.zeros((0, 3), np.float32)
+    prim, t, u, v = ctx.trace_closest(scene, z, z)
+    assert prim.shape == (0,) and t.shape == (0,)
+    assert ctx.trace_any(scene, z, z, np.zeros(0, np.float32)).shape == (0,)
+    rng = np.random.default_rng(3)
+    for n in (1, 63, 65, 257, 1001):
+        org = rng.uniform(-0.9, 0.9, (n, 3)).astype(np.float32)
+        org[:, 1] += 1.0
+        d = rng.normal(size=(n, 3)).astype(np.float32)
+        d /= np.linalg.norm(d, axis=1, keepdims=True)
+        _check(ctx, sc, org, d, rng.uniform(0, 3, n).astype(np.float32))
+
+
+def test_axis_aligned_rays_on_box_planes(ctx):
+    """Axis directions (two zero components), origins on the walls' planes
+    (x = +-1, y = 0 / 2, z = -1), on the light's plane and inside, and
+    directions along the quads' shared diagonals."""
+    from photorealisticrender4610_amd import scenes
+    print("case edges: axis-aligned rays", flush=True)
+    sc = scenes.box_room()
+    coords = np.array([-1.0, -0.5, -0.25, 0.0, 0.25, 0.45, 0.5, 1.0], np.float32)
+    ys = np.array([0.0, 0.4, 1.0, 1.98, 2.0], np.float32)
+    org = np.array([(x, y, z) for x in coords for y in ys for z in coords], np.float32)
+    axes = np.array([(1, 0, 0), (-1, 0, 0), (0, 1, 0), (0, -1, 0), (0, 0, 1), (0, 0, -1),
+                     (0.70710677, 0, 0.70710677), (-0.70710677, 0, -0.70710677),
+                     (0.70710677, 0.70710677, 0), (0, -0.70710677, 0.70710677)], np.float32)
+    O_ = np.repeat(org, len(axes), 0)
+    D_ = np.tile(axes, (len(org), 1))
+    tmax = np.tile(np.array([0.0, 0.5, 1.0, 2.0, np.inf], np.float32), len(O_) // 5 + 1)[:len(O_)]
+    prim, anyh = _check(ctx, sc, O_, D_, tmax)
+    assert (prim >= 0).mean() > 0.5
+    assert 0 < anyh.mean() < 1
+
+
+def test_any_hit_range_limits(ctx):
+    """tmax of 0, negative, tiny, the hit distance itself and infinity: the
+    same answers as IntersectP's (1e-3, tmax) range."""
+    from photorealisticrender4610_amd import scenes
+    print("case edges: any-hit ranges", flush=True)
+    sc = scenes.box_room()
+    rng = np.random.default_rng(5)
+    n = 4000
+    org = np.stack([rng.uniform(-0.9, 0.9, n), rng.uniform(0.1, 1.9, n), rng.uniform(-0.9, 0.9, n)], 1)
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    org, d = org.astype(np.float32), d.astype(np.float32)
+    _, t, _, _ = (_np(x) for x in ctx.trace_closest(ctx.scene(sc), org, d))
+    for tm in (np.zeros(n), -np.ones(n), np.full(n, 1e-3), np.full(n, 1.5e-3), t, np.nextafter(t, np.inf),
+               np.full(n, np.inf)):
+        _check(ctx, sc, org, d, np.asarray(tm, np.float32))
+
+
+def test_single_triangle_and_spheres_only(ctx):
+    from photorealisticrender4610_amd import scenes as S
+    print("case edges: one triangle, spheres only", flush=True)
+    rng = np.random.default_rng(8)
+    b = S.SceneBuilder()
+    m = b.material()
+    b.mesh([[0, 0, 0, 1, 0, 0, 0, 1, 0]], m)
+    one = b.build("one_triangle")
+    b = S.SceneBuilder()
+    m = b.material()
+    for k in range(40):
+        b.sphere(rng.uniform(-3, 3, 3), rng.uniform(0.1, 0.6), m)
+    balls = b.build("spheres_only")
+    for sc, lo, hi in ((one, -0.5, 1.5), (balls, -4, 4)):
+        n = 3000
+        org = rng.uniform(lo, hi, (n, 3)).astype(np.float32)
+        org[: n // 2, 2] = -2.0  # half aimed from in front
+        d = rng.normal(size=(n, 3)).astype(np.float32)
+        d[: n // 2] = (rng.uniform(lo, hi, (n // 2, 3)) * np.array([1, 1, 0]) - org[: n // 2] * np.array([1, 1, 1])
+                       + np.array([0, 0, 0.01])).astype(np.float32)
+        d /= np.linalg.norm(d, axis=1, keepdims=True)
+        prim, _ = _check(ctx, sc, org, d, rng.uniform(0, 8, n).astype(np.float32))
+        assert (prim >= 0).any()
+
+
+def test_tiny_and_ragged_images(ctx):
+    """1x1 and 3x2 images, 1 and 3 spp, depth 1 and 8: per-path parity with
+    the oracle (same PCG streams)."""
+    from photorealisticrender4610_amd import scenes
+    print("case edges: tiny images", flush=True)
+    sc = scenes.box_room()
+    scene = ctx.scene(sc)
+    orc = O.OracleScene(sc)
+    for w, h, spp, depth in ((1, 1, 1, 1), (1, 1, 3, 8), (3, 2, 2, 8), (17, 5, 1, 4)):
+        img, st = ctx.render(scene, w, h, spp, depth, seed=9)
+        ref, _, _ = orc.render(w, h, spp, depth, 9)
+        g = img.cpu().numpy()
+        assert g.shape == (h, w, 3) and st["samples"] == w * h * spp
+        close = np.all(np.abs(g - ref) <= 1e-4 * np.maximum(np.abs(ref), 1e-2), axis=2)
+        assert close.mean() >= 0.9, (w, h, spp, depth, close.mean())
+
+
+def test_bad_parameters_raise(ctx):
+    from photorealisticrender4610_amd import abi, scenes
+    print("case edges: bad parameters", flush=True)
+    scene = ctx.scene(scenes.box_room())
+    for args in ((0, 8, 1, 4), (8, 0, 1, 4), (8, 8, 0, 4), (8, 8, 1, 0), (8, 8, 1, 65)):
+        with pytest.raises(abi.PrrError):
+            ctx.render(scene, *args)
+    b = scenes.SceneBuilder()
+    b.material()
+    with pytest.raises(Exception):
+        ctx.scene(b.build("no_objects"))
+    # the context still renders after the errors
+    img, st = ctx.render(scene, 8, 8, 1, 4, seed=1)
+    assert st["samples"] == 64
