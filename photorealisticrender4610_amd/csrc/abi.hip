@@ -159,7 +159,7 @@ int runBounces(prr_context *c, const SceneView &view, const ChunkInfo &ci, Timin
             HIPCK(c, hipStreamWaitEvent(c->aux, c->evFork, 0));
             TIMED(1, c->aux,
                   launchShadow(view, c->pb, b - 1, c->spillAux, c->traceBlocks, c->refillAt, c->stackDepth, wwShadow,
-                               c->fetchRun, c->shadowWide != 0, c->aux));
+                               c->fetchRun, c->shadowWide, c->aux));
             HIPCK(c, hipEventRecord(c->evJoin, c->aux));
         }
         TIMED(0, st,
@@ -168,7 +168,7 @@ int runBounces(prr_context *c, const SceneView &view, const ChunkInfo &ci, Timin
         if (b > 0 && !conc)
             TIMED(1, st,
                   launchShadow(view, c->pb, b - 1, c->spill, c->traceBlocks, c->refillAt, c->stackDepth, wwShadow,
-                               c->fetchRun, c->shadowWide != 0, st));
+                               c->fetchRun, c->shadowWide, st));
         if (b > 0 && conc) HIPCK(c, hipStreamWaitEvent(st, c->evJoin, 0));
         TIMED(2, st, launchShade(view, c->pb, ci, b, c->shadeBlocks, c->shadeWaves, st));
     }
@@ -292,7 +292,7 @@ int prr_create(int device, prr_context **out) {
     }
     if (const char *ev = getenv("PRR_WHILE_WHILE")) c->whileWhile = std::max(0, atoi(ev));
     if (const char *ev = getenv("PRR_WHILE_WHILE_ANY")) c->whileWhileAny = std::max(0, atoi(ev));
-    if (const char *ev = getenv("PRR_SHADOW_WIDE")) c->shadowWide = atoi(ev) != 0;
+    if (const char *ev = getenv("PRR_SHADOW_WIDE")) c->shadowWide = std::max(0, std::min(2, atoi(ev)));
     if (const char *ev = getenv("PRR_CONCURRENT_SHADOW")) c->concurrentShadow = atoi(ev) != 0;
     if (const char *ev = getenv("PRR_FETCH_RUN")) c->fetchRun = std::max(64, std::min(1 << 16, atoi(ev)));
     if (const char *ev = getenv("PRR_SHADE_WAVES")) c->shadeWaves = std::max(1, std::min(5, atoi(ev)));
@@ -365,6 +365,8 @@ int prr_scene_create(prr_context *c, const prr_scene_desc *d, prr_scene **out) {
     if (e == hipSuccess) e = upload(s, (std::remove_const<std::remove_reference<decltype(*dst)>::type>::type **) &dst, src, c->stream)
     UP(v.nodes, h.nodes);
     UP(v.nodes4, h.nodes4);
+    UP(v.nodes4q, h.nodes4q);
+    UP(v.leavesq, h.leavesq);
     UP(v.tris, h.tris);
     UP(v.sph, h.sph);
     UP(v.sph_id, h.sphId);
@@ -388,6 +390,7 @@ int prr_scene_create(prr_context *c, const prr_scene_desc *d, prr_scene **out) {
     }
     v.root = h.root;
     v.root4 = h.root4;
+    v.root4q = h.root4q;
     v.num_tris = h.numTris;
     v.num_emitters = h.totalEmitArea > 0.f ? (int) h.emitters.size() : 0;
     v.total_emit_area = h.totalEmitArea;
@@ -472,7 +475,8 @@ int prr_trace_any(prr_context *c, const prr_scene *s, int64_t n, const float *or
     if (!c || !s) return fail(c, PRR_ERR_INVALID, "NULL context or scene");
     if (n < 0 || (n > 0 && (!org || !dir || !tmax || !hit))) return fail(c, PRR_ERR_INVALID, "bad arrays");
     HIPCK(c, hipSetDevice(c->device));
-    HIPCK(c, launchTraceAny(s->view, n, org, dir, tmax, hit, c->spill, c->spillLanes, c->stackDepth, c->stream));
+    HIPCK(c, launchTraceAny(s->view, n, org, dir, tmax, hit, c->spill, c->spillLanes, c->stackDepth,
+                            c->shadowWide == 2, c->stream));
     return PRR_OK;
 }
 

@@ -25,7 +25,7 @@ struct prr_context {
                             // n > 1 the same with at most n interior steps per check
     int whileWhileAny = 4;  // the same for shadow rays (on the four-wide tree)
     int fetchRun = 256;     // traversal work fetch: a wave claims at most this many queue entries per atomic
-    int shadowWide = 1;     // shadow rays on the four-wide occlusion tree (0: the binary tree)
+    int shadowWide = 2;     // shadow rays: 0 the binary tree, 1 the four-wide occlusion tree, 2 its quantised copy
     int shadeWaves = 4;     // shading kernel compiled for at least this many waves per SIMD (1, 4, 5)
     int64_t maxSlots = 1 << 24;  // paths per chunk (set at creation from free HBM)
     uint2 *spill = nullptr;

@@ -131,6 +131,25 @@ struct __align__(16) DNode4 {
 };
 constexpr uint32_t kNoChild = 0x7FFFFFFFu;
 
+// Quantised four-wide occlusion node (64 B, one cache line): the same
+// children as a DNode4, boxes stored as bytes q on a per-axis grid, bound =
+// fma(q, scale, origin) (exact: q * scale is a power-of-two multiple). The
+// builder checks that every decoded box contains the child's box, so a ray
+// that enters a child's box enters its decoded box (the slab test is monotone
+// in the bounds). Leaf children refer to a DLeafQ (kLeafBit | index) whose
+// exact box is tested before the leaf's primitives.
+//   w[0..2] origin xyz, w[3..5] scale xyz,
+//   w[6..11] lo x, hi x, lo y, hi y, lo z, hi z (child k in byte k),
+//   w[12..15] child refs (kNoChild: empty; decoded box inverted)
+struct __align__(16) DNode4Q {
+    uint4 w0, w1, w2, w3;
+};
+// Leaf of the quantised tree: exact box [minX, maxX, minY, maxY], [minZ,
+// maxZ, leaf ref bits, 0].
+struct __align__(16) DLeafQ {
+    float4 a, b;
+};
+
 enum MatType { MAT_DIFFUSE = 0, MAT_GLASS = 1, MAT_MIRROR = 2, MAT_EMISSIVE = 3 };
 
 struct __align__(16) DMaterial {
@@ -168,6 +187,8 @@ struct CamView {
 struct SceneView {
     const DNode *nodes;
     const DNode4 *nodes4;     // occlusion tree (four-wide)
+    const DNode4Q *nodes4q;   // the same tree quantised (one line per node)
+    const DLeafQ *leavesq;    // its leaves with exact boxes
     const float4 *tris;       // 3 float4 per leaf slot: v0.xyz e1.x | e1.yz e2.xy | e2.z id pad pad
     const float4 *sph;        // leaf-order spheres: center.xyz radius
     const int *sph_id;        // leaf-order sphere -> global prim id
@@ -185,6 +206,7 @@ struct SceneView {
     int env_gk;               // guide buckets (power of two), 0: full searches
     uint32_t root;            // root child reference
     uint32_t root4;           // occlusion tree root reference
+    uint32_t root4q;          // quantised occlusion tree root reference
     int num_tris;
     int num_emitters;
     float total_emit_area;

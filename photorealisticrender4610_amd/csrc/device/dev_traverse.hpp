@@ -584,6 +584,102 @@ PRR_DEV bool traceAny4(const SceneView &s, const V3 &o, const V3 &d, float tMax,
     return r == kAnyHit;
 }
 
+// slab() with the near and far plane of each axis already chosen.
+PRR_DEV float slabNF(const RayPre &r, float nx, float fx, float ny, float fy, float nz, float fz) {
+    float tmin = 0.f, tmax = __builtin_inff();
+    float t0 = (nx - r.o[0]) * r.inv[0], t1 = (fx - r.o[0]) * r.inv[0];
+    tmin = fmaxf(tmin, t0);
+    tmax = fminf(tmax, t1);
+    t0 = (ny - r.o[1]) * r.inv[1];
+    t1 = (fy - r.o[1]) * r.inv[1];
+    tmin = fmaxf(tmin, t0);
+    tmax = fminf(tmax, t1);
+    t0 = (nz - r.o[2]) * r.inv[2];
+    t1 = (fz - r.o[2]) * r.inv[2];
+    tmin = fmaxf(tmin, t0);
+    tmax = fminf(tmax, t1);
+    return tmin <= tmax ? tmin : __builtin_inff();
+}
+
+// Bound of byte k of a quantised word: fma(q, scale, origin), exact.
+PRR_DEV float decQ(uint32_t w, int k, float s, float o) {
+    return __builtin_fmaf((float) ((w >> (8 * k)) & 255u), s, o);
+}
+
+// Any-hit at a node of the quantised occlusion tree (DNode4Q): the decoded
+// boxes contain the children's boxes, so every child the exact test would
+// enter is entered (the slab test is monotone in the bounds); leaves repeat
+// the exact test (anyLeafQ), so the reached leaves, and the answer, are the
+// binary tree's. The near/far choice per axis is made once per node on the
+// packed words. Child order and pushes as anyInterior4.
+PRR_DEV int anyInterior4Q(const SceneView &s, TravState &ts, RefStack &st) {
+    const DNode4Q nq = s.nodes4q[ts.cur];
+    const float ox = __uint_as_float(nq.w0.x), oy = __uint_as_float(nq.w0.y), oz = __uint_as_float(nq.w0.z);
+    const float sx = __uint_as_float(nq.w0.w), sy = __uint_as_float(nq.w1.x), sz = __uint_as_float(nq.w1.y);
+    const uint32_t nxw = ts.r.oct[0] ? nq.w1.w : nq.w1.z, fxw = ts.r.oct[0] ? nq.w1.z : nq.w1.w;
+    const uint32_t nyw = ts.r.oct[1] ? nq.w2.y : nq.w2.x, fyw = ts.r.oct[1] ? nq.w2.x : nq.w2.y;
+    const uint32_t nzw = ts.r.oct[2] ? nq.w2.w : nq.w2.z, fzw = ts.r.oct[2] ? nq.w2.z : nq.w2.w;
+#define PRR_TQ(k)                                                                                             \
+    slabNF(ts.r, decQ(nxw, k, sx, ox), decQ(fxw, k, sx, ox), decQ(nyw, k, sy, oy), decQ(fyw, k, sy, oy), \
+           decQ(nzw, k, sz, oz), decQ(fzw, k, sz, oz))
+    const float t0 = PRR_TQ(0), t1 = PRR_TQ(1), t2 = PRR_TQ(2), t3 = PRR_TQ(3);
+#undef PRR_TQ
+    const uint4 rf = nq.w3;
+    const float tm = ts.tmax;
+    int bk = -1;
+    float bt = tm;
+    bk = t0 < bt ? 0 : bk;
+    bt = t0 < bt ? t0 : bt;
+    bk = t1 < bt ? 1 : bk;
+    bt = t1 < bt ? t1 : bt;
+    bk = t2 < bt ? 2 : bk;
+    bt = t2 < bt ? t2 : bt;
+    bk = t3 < bt ? 3 : bk;
+    if (bk < 0) return anyPop(ts, st);
+    const bool h0 = t0 < tm && bk != 0, h1 = t1 < tm && bk != 1, h2 = t2 < tm && bk != 2, h3 = t3 < tm && bk != 3;
+    const uint32_t p0 = h0 ? rf.x : (h1 ? rf.y : (h2 ? rf.z : rf.w));
+    const uint32_t p1 = h0 ? (h1 ? rf.y : (h2 ? rf.z : rf.w)) : (h1 ? (h2 ? rf.z : rf.w) : rf.w);
+    const uint32_t p2 = bk == 3 ? rf.z : rf.w;
+    const int m = (int) h0 + (int) h1 + (int) h2 + (int) h3;
+    st.push3(p0, p1, p2, m);
+    ts.cur = bk == 0 ? rf.x : (bk == 1 ? rf.y : (bk == 2 ? rf.z : rf.w));
+    return kAnyGo;
+}
+
+// A leaf of the quantised tree: its exact box first (the binary tree's test
+// of that leaf, BVH.cpp:445-459), then its primitives.
+PRR_DEV int anyLeafQ(const SceneView &s, TravState &ts, RefStack &st) {
+    const DLeafQ lf = s.leavesq[ts.cur & ~kLeafBit];
+    const float tb = slab(ts.r, lf.a.x, lf.a.y, lf.a.z, lf.a.w, lf.b.x, lf.b.y);
+    if (!(tb < ts.tmax)) return anyPop(ts, st);
+    ts.cur = __float_as_uint(lf.b.z);
+    return anyLeaf(s, ts, st);
+}
+
+PRR_DEV int anyStep4Q(const SceneView &s, TravState &ts, RefStack &st) {
+    return refIsLeaf(ts.cur) ? anyLeafQ(s, ts, st) : anyInterior4Q(s, ts, st);
+}
+
+PRR_DEV int anyIter4Q(const SceneView &s, TravState &ts, RefStack &st, int cap) {
+    int k = 0;
+    while (!refIsLeaf(ts.cur)) {
+        const int r = anyInterior4Q(s, ts, st);
+        if (r != kAnyGo) return r;
+        if (++k == cap) return kAnyGo;
+    }
+    return anyLeafQ(s, ts, st);
+}
+
+PRR_DEV bool traceAny4Q(const SceneView &s, const V3 &o, const V3 &d, float tMax, RefStack &st) {
+    TravState ts;
+    travBegin(ts, o, d, tMax, s.root4q, st);
+    if (!anyBegin(s, ts)) return false;
+    int r;
+    while ((r = anyStep4Q(s, ts, st)) == kAnyGo) {
+    }
+    return r == kAnyHit;
+}
+
 // Work fetch through a wave-private range of queue entries. A wave claims a
 // run of consecutive entries from *counter with one atomic and hands them to
 // its idle lanes over several refills; the run is sized by guided

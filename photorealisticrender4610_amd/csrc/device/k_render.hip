@@ -244,14 +244,15 @@ __global__ __launch_bounds__(kTraceBlock) void kExtend(SceneView s, PathBufs pb,
     }
 }
 
-template <bool WW, bool WIDE>
+// TREE: 0 the binary tree, 1 the four-wide occlusion tree, 2 its quantised copy.
+template <bool WW, int TREE>
 __global__ __launch_bounds__(kTraceBlock) void kShadow(SceneView s, PathBufs pb, int bounce, uint2 *spill, int refillAt,
                                                        int stackDepth, int wwCap, int fetchRun) {
     extern __shared__ uint2 stk[];
     const int64_t n = pb.sqcnt[bounce];
     uint32_t *fetch = &pb.fetch[2 * bounce + 1];
     const int64_t lane = (int64_t) blockIdx.x * kTraceBlock + threadIdx.x;
-    using Stack = typename std::conditional<WIDE, RefStack, LaneStack>::type;
+    using Stack = typename std::conditional<TREE != 0, RefStack, LaneStack>::type;
     Stack st = makeStack<Stack>(stk, spill + lane * kSpill, threadIdx.x, stackDepth);
     TravState ts;
     int64_t idx = -1;
@@ -270,7 +271,8 @@ __global__ __launch_bounds__(kTraceBlock) void kShadow(SceneView s, PathBufs pb,
             } else {
                 const uint32_t k = pb.sqSlot[got];
                 const float4 o4 = pb.sqOrg[got];
-                travBegin(ts, ld3(o4), ld3(pb.sqDir[got]), o4.w, WIDE ? s.root4 : s.root, st);
+                travBegin(ts, ld3(o4), ld3(pb.sqDir[got]), o4.w, TREE == 2 ? s.root4q : (TREE == 1 ? s.root4 : s.root),
+                          st);
                 if (anyBegin(s, ts)) {
                     idx = got;
                     sk = k;
@@ -282,7 +284,9 @@ __global__ __launch_bounds__(kTraceBlock) void kShadow(SceneView s, PathBufs pb,
         if (__ballot(idx >= 0 || !exhausted) == 0) break;
         int res = kAnyGo;
         if (idx >= 0) {
-            if constexpr (WIDE)
+            if constexpr (TREE == 2)
+                res = WW ? anyIter4Q(s, ts, st, wwCap) : anyStep4Q(s, ts, st);
+            else if constexpr (TREE == 1)
                 res = WW ? anyIter4(s, ts, st, wwCap) : anyStep4(s, ts, st);
             else
                 res = WW ? anyIter(s, ts, st, wwCap) : anyStep(s, ts, st);
@@ -810,20 +814,24 @@ hipError_t launchExtend(const SceneView &s, const PathBufs &pb, int bounce, uint
     return hipGetLastError();
 }
 hipError_t launchShadow(const SceneView &s, const PathBufs &pb, int bounce, uint2 *spill, int traceGrid,
-                        int refillAt, int stackDepth, int whileWhile, int fetchRun, bool wide, hipStream_t st) {
+                        int refillAt, int stackDepth, int whileWhile, int fetchRun, int tree, hipStream_t st) {
     const int wwCap = whileWhile > 1 ? whileWhile : 0;
     const size_t lds = stackDepth * kTraceBlock * sizeof(uint2);
 #define PRR_SHADOW(W, V)                                                                                             \
     hipLaunchKernelGGL((kShadow<W, V>), dim3(traceGrid), dim3(kTraceBlock), lds, st, s, pb, bounce, spill, refillAt, \
                        stackDepth, wwCap, fetchRun)
-    if (whileWhile && wide)
-        PRR_SHADOW(true, true);
+    if (whileWhile && tree == 2)
+        PRR_SHADOW(true, 2);
+    else if (whileWhile && tree == 1)
+        PRR_SHADOW(true, 1);
     else if (whileWhile)
-        PRR_SHADOW(true, false);
-    else if (wide)
-        PRR_SHADOW(false, true);
+        PRR_SHADOW(true, 0);
+    else if (tree == 2)
+        PRR_SHADOW(false, 2);
+    else if (tree == 1)
+        PRR_SHADOW(false, 1);
     else
-        PRR_SHADOW(false, false);
+        PRR_SHADOW(false, 0);
 #undef PRR_SHADOW
     return hipGetLastError();
 }

@@ -34,7 +34,7 @@ __global__ __launch_bounds__(kTraceBlock) void kTraceClosest(SceneView s, int64_
 __global__ __launch_bounds__(kTraceBlock) void kTraceAny(SceneView s, int64_t n, const float *__restrict__ org,
                                                          const float *__restrict__ dir, const float *__restrict__ tmax,
                                                          uint8_t *__restrict__ hit, uint2 *__restrict__ spill,
-                                                         int stackDepth) {
+                                                         int stackDepth, int quant) {
     extern __shared__ uint2 stk[];
     const int64_t lane = (int64_t) blockIdx.x * kTraceBlock + threadIdx.x;
     RefStack st = makeStack<RefStack>(stk, spill + lane * kSpill, threadIdx.x, stackDepth);
@@ -42,7 +42,7 @@ __global__ __launch_bounds__(kTraceBlock) void kTraceAny(SceneView s, int64_t n,
     for (int64_t i = lane; i < n; i += stride) {
         const V3 o(org[3 * i], org[3 * i + 1], org[3 * i + 2]);
         const V3 d(dir[3 * i], dir[3 * i + 1], dir[3 * i + 2]);
-        hit[i] = traceAny4(s, o, d, tmax[i], st) ? 1 : 0;
+        hit[i] = (quant ? traceAny4Q(s, o, d, tmax[i], st) : traceAny4(s, o, d, tmax[i], st)) ? 1 : 0;
     }
 }
 
@@ -63,10 +63,10 @@ hipError_t launchTraceClosest(const SceneView &s, int64_t n, const float *org, c
 }
 
 hipError_t launchTraceAny(const SceneView &s, int64_t n, const float *org, const float *dir, const float *tmax,
-                          uint8_t *hit, uint2 *spill, int spillLanes, int stackDepth, hipStream_t st) {
+                          uint8_t *hit, uint2 *spill, int spillLanes, int stackDepth, int quant, hipStream_t st) {
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(kTraceAny, dim3(gridFor(n, spillLanes)), dim3(kTraceBlock),
-                       stackDepth * kTraceBlock * sizeof(uint2), st, s, n, org, dir, tmax, hit, spill, stackDepth);
+                       stackDepth * kTraceBlock * sizeof(uint2), st, s, n, org, dir, tmax, hit, spill, stackDepth, quant);
     return hipGetLastError();
 }
 

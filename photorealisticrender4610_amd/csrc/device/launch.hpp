@@ -13,7 +13,7 @@ hipError_t launchTraceClosest(const SceneView &s, int64_t n, const float *org, c
                               float *t, float *u, float *v, uint2 *spill, int spillLanes, int stackDepth,
                               hipStream_t st);
 hipError_t launchTraceAny(const SceneView &s, int64_t n, const float *org, const float *dir, const float *tmax,
-                          uint8_t *hit, uint2 *spill, int spillLanes, int stackDepth, hipStream_t st);
+                          uint8_t *hit, uint2 *spill, int spillLanes, int stackDepth, int quant, hipStream_t st);
 
 // k_misc.hip
 hipError_t launchBsdfEval(int64_t n, const float *wi, const float *wo, const float *nrm, const float *params,

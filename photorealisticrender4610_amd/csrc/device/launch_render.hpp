@@ -15,9 +15,9 @@ hipError_t launchCopyResults(const PathBufs &pb, int64_t n, int64_t first, float
 // the same with at most n interior steps per work check
 hipError_t launchExtend(const SceneView &s, const PathBufs &pb, int bounce, uint2 *spill, int traceGrid,
                         int refillAt, int stackDepth, int whileWhile, int fetchRun, hipStream_t st);
-// wide: trace on the four-wide occlusion tree instead of the binary tree
+// tree: 0 the binary tree, 1 the four-wide occlusion tree, 2 its quantised copy
 hipError_t launchShadow(const SceneView &s, const PathBufs &pb, int bounce, uint2 *spill, int traceGrid,
-                        int refillAt, int stackDepth, int whileWhile, int fetchRun, bool wide, hipStream_t st);
+                        int refillAt, int stackDepth, int whileWhile, int fetchRun, int tree, hipStream_t st);
 // waves: minimum waves per SIMD the shading kernel is compiled for (1, 4 or 5)
 hipError_t launchShade(const SceneView &s, const PathBufs &pb, const ChunkInfo &ci, int bounce, int grid, int waves,
                        hipStream_t st);

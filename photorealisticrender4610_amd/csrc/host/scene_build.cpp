@@ -410,6 +410,97 @@ struct Widener {
     }
 };
 
+// Quantised copy of the occlusion tree (DNode4Q, DLeafQ). Per node and axis
+// the origin is the children's minimum and the scale a power of two with
+// extent / scale <= 255; each child's bytes are moved outwards until the
+// decoded bound fma(q, scale, origin), evaluated in float as on the device,
+// lies outside the child's box, and the scale doubles if a byte runs out.
+struct Quantiser {
+    HostScene &S;
+    static float dec(uint32_t q, float s, float o) { return std::fma((float) q, s, o); }
+    // bytes of one axis for the n valid children; false if the scale is too fine
+    static bool axis(const float *lo, const float *hi, int n, float o, float s, uint32_t &wlo, uint32_t &whi) {
+        wlo = whi = 0;
+        for (int k = 0; k < 4; k++) {
+            uint32_t ql = 255, qh = 0;  // empty slot: inverted
+            if (k < n) {
+                double fl = std::floor(((double) lo[k] - o) / s), fh = std::ceil(((double) hi[k] - o) / s);
+                ql = (uint32_t) std::min(255.0, std::max(0.0, fl));
+                qh = (uint32_t) std::min(255.0, std::max(0.0, fh));
+                while (ql > 0 && !(dec(ql, s, o) <= lo[k])) ql--;
+                while (qh < 255 && !(dec(qh, s, o) >= hi[k])) qh++;
+                if (!(dec(ql, s, o) <= lo[k]) || !(dec(qh, s, o) >= hi[k])) return false;
+            }
+            wlo |= ql << (8 * k);
+            whi |= qh << (8 * k);
+        }
+        return true;
+    }
+    uint32_t leaf(const float *b, uint32_t ref) {
+        DLeafQ l;
+        l.a = make_float4(b[0], b[1], b[2], b[3]);
+        l.b = make_float4(b[4], b[5], __builtin_bit_cast(float, ref), 0.f);
+        S.leavesq.push_back(l);
+        return kLeafBit | (uint32_t) (S.leavesq.size() - 1);
+    }
+    bool run(std::string &err) {
+        S.nodes4q.assign(S.nodes4.size(), DNode4Q{});
+        S.leavesq.clear();
+        for (size_t i = 0; i < S.nodes4.size(); i++) {
+            const DNode4 &w = S.nodes4[i];
+            const float lo3[3][4] = {{w.lox.x, w.lox.y, w.lox.z, w.lox.w},
+                                     {w.loy.x, w.loy.y, w.loy.z, w.loy.w},
+                                     {w.loz.x, w.loz.y, w.loz.z, w.loz.w}};
+            const float hi3[3][4] = {{w.hix.x, w.hix.y, w.hix.z, w.hix.w},
+                                     {w.hiy.x, w.hiy.y, w.hiy.z, w.hiy.w},
+                                     {w.hiz.x, w.hiz.y, w.hiz.z, w.hiz.w}};
+            const uint32_t rf[4] = {w.ref.x, w.ref.y, w.ref.z, w.ref.w};
+            int n = 0;
+            while (n < 4 && rf[n] != kNoChild) n++;
+            float org[3] = {0.f, 0.f, 0.f}, scl[3] = {1.f, 1.f, 1.f};
+            uint32_t wq[6] = {0xFFFFFFFFu, 0u, 0xFFFFFFFFu, 0u, 0xFFFFFFFFu, 0u};
+            for (int a = 0; a < 3 && n > 0; a++) {
+                float mn = lo3[a][0], mx = hi3[a][0];
+                for (int k = 1; k < n; k++) {
+                    mn = std::min(mn, lo3[a][k]);
+                    mx = std::max(mx, hi3[a][k]);
+                }
+                org[a] = mn;
+                const double ext = (double) mx - (double) mn;
+                int e = 0;
+                if (ext > 0.0) {
+                    std::frexp(ext / 255.0, &e);  // ext / 255 < 2^e
+                    e = std::max(e, -126);
+                }
+                bool ok = false;
+                for (int tries = 0; tries < 8 && !ok; tries++, e++) {
+                    scl[a] = std::ldexp(1.f, e);
+                    ok = axis(lo3[a], hi3[a], n, org[a], scl[a], wq[2 * a], wq[2 * a + 1]);
+                }
+                if (!ok) {
+                    err = "occlusion tree: a child box does not fit its node's grid";
+                    return false;
+                }
+            }
+            uint32_t refs[4] = {kNoChild, kNoChild, kNoChild, kNoChild};
+            for (int k = 0; k < n; k++) {
+                const float b[6] = {lo3[0][k], hi3[0][k], lo3[1][k], hi3[1][k], lo3[2][k], hi3[2][k]};
+                refs[k] = refIsLeaf(rf[k]) ? leaf(b, rf[k]) : rf[k];
+            }
+            DNode4Q &q = S.nodes4q[i];
+            q.w0 = make_uint4(__builtin_bit_cast(uint32_t, org[0]), __builtin_bit_cast(uint32_t, org[1]),
+                              __builtin_bit_cast(uint32_t, org[2]), __builtin_bit_cast(uint32_t, scl[0]));
+            q.w1 = make_uint4(__builtin_bit_cast(uint32_t, scl[1]), __builtin_bit_cast(uint32_t, scl[2]), wq[0], wq[1]);
+            q.w2 = make_uint4(wq[2], wq[3], wq[4], wq[5]);
+            q.w3 = make_uint4(refs[0], refs[1], refs[2], refs[3]);
+        }
+        S.root4q = refIsLeaf(S.root4) ? leaf(S.rootBox, S.root4) : S.root4;
+        if (S.nodes4q.empty()) S.nodes4q.emplace_back();
+        if (S.leavesq.empty()) S.leavesq.emplace_back();
+        return true;
+    }
+};
+
 bool hasEmission(const prr_material_desc &m) {  // Material::hasEmission (Material.hpp:113-115)
     return norm(V3(m.emission[0], m.emission[1], m.emission[2])) > kEps;
 }
@@ -581,6 +672,7 @@ bool buildHostScene(const prr_scene_desc *d, HostScene &S, std::string &err) {
         }
     }
     if (S.nodes4.empty()) S.nodes4.emplace_back();
+    if (!Quantiser{S}.run(err)) return false;
     if (S.tris.empty()) S.tris.push_back(make_float4(0, 0, 0, 0));  // keep buffers non-empty
     if (S.sph.empty()) {
         S.sph.push_back(make_float4(0, 0, 0, 0));

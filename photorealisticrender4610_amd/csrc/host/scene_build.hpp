@@ -57,6 +57,8 @@ struct HostScene {
     // device staging
     std::vector<DNode> nodes;
     std::vector<DNode4> nodes4;    // occlusion tree
+    std::vector<DNode4Q> nodes4q;  // the same tree quantised
+    std::vector<DLeafQ> leavesq;   // its leaves with exact boxes
     std::vector<float4> tris;      // 3 per leaf slot
     std::vector<float4> sph;       // leaf-order spheres
     std::vector<int> sphId;        // leaf-order sphere -> global prim id
@@ -76,6 +78,7 @@ struct HostScene {
     std::vector<int32_t> envGuideM, envGuideC;
     uint32_t root = 0;
     uint32_t root4 = 0;
+    uint32_t root4q = 0;
     int wideLevelsKept = 0;        // binary levels not collapsed (a child box not inside its parent's)
     float rootBox[6] = {0, 0, 0, 0, 0, 0};
     int maxDepth = 0;

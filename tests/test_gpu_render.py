@@ -106,10 +106,12 @@ def test_estimator_vs_reference_image(ctx):
 
 
 def test_shadow_trees_agree(ctx, monkeypatch):
-    """Shadow rays traced on the four-wide occlusion tree (the default) and on
-    the binary tree give the same image bit for bit: an any-hit answer does
-    not depend on the tree, and the occlusion tree skips only boxes that
-    contain their children's boxes."""
+    """Shadow rays traced on the quantised four-wide occlusion tree (the
+    default), on its full-precision copy and on the binary tree give the same
+    image bit for bit, and batch any-hit queries agree with the oracle on
+    each: an any-hit answer does not depend on the tree, the occlusion tree
+    skips only boxes that contain their children's boxes, and the quantised
+    boxes contain the exact ones with the exact leaf test kept."""
     from photorealisticrender4610_amd import abi
     import testscenes as TS
     sc = TS.mini_field()
@@ -119,16 +121,31 @@ def test_shadow_trees_agree(ctx, monkeypatch):
     print("occlusion tree: %d nodes, %d levels kept (binary tree: %d nodes)"
           % (info["num_wide_nodes"], info["wide_levels_kept"], info["num_nodes"]), flush=True)
     assert 0 < info["num_wide_nodes"] < info["num_nodes"]
-    img_w, st_w = ctx.render(scene, 160, 120, 4, 12, seed=5)
-    monkeypatch.setenv("PRR_SHADOW_WIDE", "0")
-    c2 = abi.Context(0)
-    try:
-        img_b, st_b = c2.render(c2.scene(sc), 160, 120, 4, 12, seed=5)
-        img_b = img_b.cpu().numpy()
-    finally:
-        c2.close()
-    assert np.array_equal(img_w.cpu().numpy(), img_b)
-    assert st_w["rays_any"] == st_b["rays_any"] and st_w["rays_closest"] == st_b["rays_closest"]
+    img_q, st_q = ctx.render(scene, 160, 120, 4, 12, seed=5)
+    img_q = img_q.cpu().numpy()
+    rng = np.random.default_rng(12)
+    n = 50000
+    b = np.asarray(info["bounds"], np.float32)
+    org = (b[:3] + rng.random((n, 3)) * (b[3:] - b[:3])).astype(np.float32)
+    d = rng.normal(size=(n, 3)).astype(np.float32)
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    tmax = (rng.random(n) * np.linalg.norm(b[3:] - b[:3])).astype(np.float32)
+    want = O.OracleScene(sc).any(org, d, tmax)
+    assert np.array_equal(ctx.trace_any(scene, org, d, tmax).cpu().numpy(), want)
+    for mode in ("1", "0"):
+        monkeypatch.setenv("PRR_SHADOW_WIDE", mode)
+        c2 = abi.Context(0)
+        try:
+            s2 = c2.scene(sc)
+            img, st = c2.render(s2, 160, 120, 4, 12, seed=5)
+            img = img.cpu().numpy()
+            anyh = c2.trace_any(s2, org, d, tmax).cpu().numpy()
+            del s2
+        finally:
+            c2.close()
+        assert np.array_equal(img_q, img), mode
+        assert st_q["rays_any"] == st["rays_any"] and st_q["rays_closest"] == st["rays_closest"]
+        assert np.array_equal(anyh, want), mode
 
 
 def test_render_deterministic(ctx):
