@@ -1,5 +1,5 @@
 // context.hpp — host-side state behind the C ABI handles (prr_context,
-// prr_scene) and the render-driver helpers shared by abi.hip (plain renders)
+// prr_scene) and the render helpers shared by abi.hip (plain renders)
 // and cache_abi.hip (radiance-cache renders).
 #pragma once
 

@@ -1,5 +1,5 @@
 // cache.hpp — neural radiance cache: network layout, query encoding and the
-// MFMA fragment helpers shared by the cache kernels and the host driver.
+// MFMA fragment helpers shared by the cache kernels and the host code.
 //
 // Network (Mueller et al. 2021, "Real-time Neural Radiance Caching"): 96
 // encoded inputs -> 5 hidden layers of 64 (ReLU) -> 3 outputs (linear), no

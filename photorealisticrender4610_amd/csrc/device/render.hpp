@@ -1,5 +1,5 @@
 // render.hpp — wavefront path state shared by the render kernels and the host
-// driver (abi.hip).
+// code (abi.hip).
 #pragma once
 
 #include "dev_common.hpp"
