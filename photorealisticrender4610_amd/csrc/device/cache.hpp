@@ -4,8 +4,8 @@
 // Network (Mueller et al. 2021, "Real-time Neural Radiance Caching"): 96
 // encoded inputs -> 5 hidden layers of 64 (ReLU) -> 3 outputs (linear), no
 // biases (input 95 is a constant 1). FP16 weights and activations on MFMA
-// (v_mfma_f32_16x16x16_f16), FP32 accumulation, FP32 master weights and Adam
-// moments.
+// (v_mfma_f32_16x16x16_f16, and gfx950's v_mfma_f32_16x16x32_f16 for the
+// forward layers), FP32 accumulation, FP32 master weights and Adam moments.
 //
 // Fragment layout of v_mfma_f32_16x16x16_f16 for D = A * B (16x16x16), lane
 // l = c + 16 g (c = l % 16, g = l / 16):
@@ -148,6 +148,12 @@ PRR_DEV h4 cacheInputGroup(const CacheQuery &q, int G) {
 PRR_DEV h4 cacheInputFrag(const CacheQuery &q, int kb, int g) { return cacheInputGroup(q, 4 * kb + g); }
 
 PRR_DEV f4 mfma16(h4 a, h4 b, f4 c) { return __builtin_amdgcn_mfma_f32_16x16x16f16(a, b, c, 0, 0, 0); }
+// gfx950's double-depth form: each lane's 8 K-elements are two 16-deep
+// fragments (K-blocks 2j and 2j + 1) side by side, in A and B alike, so one
+// instruction does the work of two mfma16 on the same operands.
+typedef _Float16 h8 __attribute__((ext_vector_type(8)));
+PRR_DEV h8 cat8(h4 a, h4 b) { return __builtin_shufflevector(a, b, 0, 1, 2, 3, 4, 5, 6, 7); }
+PRR_DEV f4 mfma32(h8 a, h8 b, f4 c) { return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0); }
 
 PRR_DEV h4 reluHalf(f4 v) {
     h4 r;

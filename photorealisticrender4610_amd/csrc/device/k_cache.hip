@@ -44,13 +44,15 @@ PRR_DEV void hiddenLayer(const _Float16 *wT, int stride, int g, int c, h4 (&x)[Q
     for (int qb = 0; qb < QB; qb++)
 #pragma unroll
         for (int ob = 0; ob < 4; ob++) acc[qb][ob] = f4{0.f, 0.f, 0.f, 0.f};
+    static_assert(KB % 2 == 0, "K-blocks are taken in pairs");
 #pragma unroll
     for (int ob = 0; ob < 4; ob++)
 #pragma unroll
-        for (int kb = 0; kb < KB; kb++) {
-            const h4 a = ldh4(wT + (ob * 16 + c) * stride + kb * 16 + 4 * g);
+        for (int kb = 0; kb < KB; kb += 2) {
+            const _Float16 *row = wT + (ob * 16 + c) * stride + 4 * g;
+            const h8 a = cat8(ldh4(row + kb * 16), ldh4(row + kb * 16 + 16));
 #pragma unroll
-            for (int qb = 0; qb < QB; qb++) acc[qb][ob] = mfma16(a, x[qb][kb], acc[qb][ob]);
+            for (int qb = 0; qb < QB; qb++) acc[qb][ob] = mfma32(a, cat8(x[qb][kb], x[qb][kb + 1]), acc[qb][ob]);
         }
 #pragma unroll
     for (int qb = 0; qb < QB; qb++)
@@ -63,10 +65,11 @@ PRR_DEV void outputLayer(const _Float16 *wT, int g, int c, const h4 (&x)[QB][6],
 #pragma unroll
     for (int qb = 0; qb < QB; qb++) y[qb] = f4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-    for (int kb = 0; kb < 4; kb++) {
-        const h4 a = ldh4(wT + c * cacheStrideT(5) + kb * 16 + 4 * g);
+    for (int kb = 0; kb < 4; kb += 2) {
+        const _Float16 *row = wT + c * cacheStrideT(5) + 4 * g;
+        const h8 a = cat8(ldh4(row + kb * 16), ldh4(row + kb * 16 + 16));
 #pragma unroll
-        for (int qb = 0; qb < QB; qb++) y[qb] = mfma16(a, x[qb][kb], y[qb]);
+        for (int qb = 0; qb < QB; qb++) y[qb] = mfma32(a, cat8(x[qb][kb], x[qb][kb + 1]), y[qb]);
     }
 }
 
