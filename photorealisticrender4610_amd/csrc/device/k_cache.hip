@@ -178,12 +178,12 @@ PRR_DEV void backwardLayer(const CacheView &cv, _Float16 *xs, _Float16 *ds, floa
     for (int t = wv; t < OB * IB; t += kCacheBlock / 64) {
         const int ob = t / IB, ib = t - ob * IB;
         f4 acc = f4{0.f, 0.f, 0.f, 0.f};
+        const _Float16 *ar = ds + (ob * 16 + c) * kXsStride + 4 * g;
+        const _Float16 *br = xs + (xsRow(L) + ib * 16 + c) * kXsStride + 4 * g;
 #pragma unroll
-        for (int k = 0; k < kTrainQ / 16; k++) {
-            const h4 a = ldh4(ds + (ob * 16 + c) * kXsStride + k * 16 + 4 * g);
-            const h4 b = ldh4(xs + (xsRow(L) + ib * 16 + c) * kXsStride + k * 16 + 4 * g);
-            acc = mfma16(a, b, acc);
-        }
+        for (int k = 0; k < kTrainQ / 16; k += 2)
+            acc = mfma32(cat8(ldh4(ar + k * 16), ldh4(ar + k * 16 + 16)), cat8(ldh4(br + k * 16), ldh4(br + k * 16 + 16)),
+                         acc);
 #pragma unroll
         for (int i = 0; i < 4; i++) {
             const int o = ob * 16 + 4 * g + i;
@@ -198,11 +198,18 @@ PRR_DEV void backwardLayer(const CacheView &cv, _Float16 *xs, _Float16 *ds, floa
 #pragma unroll
         for (int ib = 0; ib < 4; ib++) {
             f4 acc[2] = {f4{0.f, 0.f, 0.f, 0.f}, f4{0.f, 0.f, 0.f, 0.f}};
+            const _Float16 *row = wn + (ib * 16 + c) * sN + 4 * g;
+            if constexpr (OB == 1) {
+                const h4 a = ldh4(row);
 #pragma unroll
-            for (int ob = 0; ob < OB; ob++) {
-                const h4 a = ldh4(wn + (ib * 16 + c) * sN + ob * 16 + 4 * g);
+                for (int qb = 0; qb < 2; qb++) acc[qb] = mfma16(a, d[qb][0], acc[qb]);
+            } else {
 #pragma unroll
-                for (int qb = 0; qb < 2; qb++) acc[qb] = mfma16(a, d[qb][ob], acc[qb]);
+                for (int ob = 0; ob < OB; ob += 2) {
+                    const h8 a = cat8(ldh4(row + ob * 16), ldh4(row + ob * 16 + 16));
+#pragma unroll
+                    for (int qb = 0; qb < 2; qb++) acc[qb] = mfma32(a, cat8(d[qb][ob], d[qb][ob + 1]), acc[qb]);
+                }
             }
 #pragma unroll
             for (int qb = 0; qb < 2; qb++)
