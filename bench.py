@@ -435,11 +435,12 @@ def main():
                            photons=photons)
         ctx.tonemap("aces", out.view(-1, 3), 1.0)
         rays += st["rays_closest"] + st["rays_any"]
-        closest += st["rays_closest"]
+        closest += st["rays_closest"] - st["rays_closest_finish"]  # the rays kExtend traced
         ms_ext += st["ms_extend"]
         launches += st["launches_extend"]
-        log("step %d: %.1f ms (extend %.1f, shadow %.1f, shade %.1f), %d rays" % (
-            k, st["ms"], st["ms_extend"], st["ms_shadow"], st["ms_shade"], st["rays_closest"] + st["rays_any"]))
+        log("step %d: %.1f ms (extend %.1f, shadow %.1f, shade %.1f, finish %.1f), %d rays" % (
+            k, st["ms"], st["ms_extend"], st["ms_shadow"], st["ms_shade"], st["ms_finish"],
+            st["rays_closest"] + st["rays_any"]))
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
     value = rays / dt / 1e6

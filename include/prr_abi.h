@@ -165,6 +165,11 @@ typedef struct prr_stats {
     double ms_shade;           /* shading / NEE / BSDF sampling */
     int64_t launches_extend;
     int64_t launches_shadow;
+    /* path completion: once few paths remain, one launch runs each of them to
+       its end (closest hits, shading and shadow rays in one kernel) */
+    uint64_t rays_closest_finish; /* closest-hit queries traced there (part of rays_closest) */
+    double ms_finish;             /* its duration (with PRR_RENDER_KERNEL_TIMING) */
+    int64_t launches_finish;
 } prr_stats;
 
 typedef struct prr_scene_info {

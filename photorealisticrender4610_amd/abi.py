@@ -65,7 +65,8 @@ class Stats(C.Structure):
     _fields_ = [("rays_closest", C.c_uint64), ("rays_any", C.c_uint64), ("samples", C.c_uint64),
                 ("photons_stored", C.c_uint64), ("ms_total", C.c_double),
                 ("ms_extend", C.c_double), ("ms_shadow", C.c_double), ("ms_shade", C.c_double),
-                ("launches_extend", C.c_int64), ("launches_shadow", C.c_int64)]
+                ("launches_extend", C.c_int64), ("launches_shadow", C.c_int64),
+                ("rays_closest_finish", C.c_uint64), ("ms_finish", C.c_double), ("launches_finish", C.c_int64)]
 
 
 class SceneInfo(C.Structure):
@@ -440,7 +441,8 @@ class Context:
         res = {"rays_closest": st.rays_closest, "rays_any": st.rays_any, "samples": st.samples,
                "photons_stored": st.photons_stored, "ms": st.ms_total, "ms_extend": st.ms_extend,
                "ms_shadow": st.ms_shadow, "ms_shade": st.ms_shade, "launches_extend": st.launches_extend,
-               "launches_shadow": st.launches_shadow}
+               "launches_shadow": st.launches_shadow, "rays_closest_finish": st.rays_closest_finish,
+               "ms_finish": st.ms_finish, "launches_finish": st.launches_finish}
         if cnt is not None:
             res["spp"] = cnt
         return out, res
@@ -572,7 +574,7 @@ class Cache:
         r = st.render
         return out, {"rays_closest": r.rays_closest, "rays_any": r.rays_any, "samples": r.samples, "ms": r.ms_total,
                      "ms_extend": r.ms_extend, "ms_shadow": r.ms_shadow, "ms_shade": r.ms_shade,
-                     "queries": st.queries, "train_samples": st.train_samples, "train_steps": st.train_steps,
+                     "ms_finish": r.ms_finish, "queries": st.queries, "train_samples": st.train_samples, "train_steps": st.train_steps,
                      "fell_back": st.fell_back, "train_loss": st.train_loss, "diverged": st.diverged,
                      "ms_render": st.ms_render, "ms_train": st.ms_train, "ms_infer": st.ms_infer}
 

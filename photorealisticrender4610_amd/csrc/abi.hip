@@ -151,9 +151,18 @@ int runBounces(prr_context *c, const SceneView &view, const ChunkInfo &ci, Timin
     } while (0)
     // Bounce b: extension rays of b and shadow rays of b - 1 (both written by
     // shade b - 1, independent of each other), then shade b, which reads both.
+    // Once bounce b's queue holds at most tailPaths entries, the shadow rays
+    // of b - 1 are traced and one kFinish launch completes every remaining
+    // path. The count is read back after shade b - 1 (its shadow rays of
+    // b - 1 already running on the aux stream while the host waits).
     const bool conc = c->concurrentShadow && c->aux;
     const int wwShadow = c->shadowWide ? c->whileWhileAny : c->whileWhile;
+    const bool tail = c->tailPaths > 0 && c->cntHost && c->evCnt;
     for (int b = 0; b <= ci.maxDepth; b++) {
+        if (tail) {
+            HIPCK(c, hipMemcpyAsync(c->cntHost, c->pb.qcnt + b, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+            HIPCK(c, hipEventRecord(c->evCnt, st));
+        }
         if (b > 0 && conc) {
             HIPCK(c, hipEventRecord(c->evFork, st));
             HIPCK(c, hipStreamWaitEvent(c->aux, c->evFork, 0));
@@ -161,6 +170,20 @@ int runBounces(prr_context *c, const SceneView &view, const ChunkInfo &ci, Timin
                   launchShadow(view, c->pb, b - 1, c->spillAux, c->traceBlocks, c->refillAt, c->stackDepth, wwShadow,
                                c->fetchRun, c->shadowWide, c->aux));
             HIPCK(c, hipEventRecord(c->evJoin, c->aux));
+        }
+        if (tail) {
+            HIPCK(c, hipEventSynchronize(c->evCnt));
+            if ((int64_t) *c->cntHost <= c->tailPaths) {
+                if (b > 0 && !conc)
+                    TIMED(1, st,
+                          launchShadow(view, c->pb, b - 1, c->spill, c->traceBlocks, c->refillAt, c->stackDepth,
+                                       wwShadow, c->fetchRun, c->shadowWide, st));
+                if (b > 0 && conc) HIPCK(c, hipStreamWaitEvent(st, c->evJoin, 0));
+                TIMED(3, st,
+                      launchFinish(view, c->pb, ci, b, c->spill, c->traceBlocks, c->stackDepth, c->whileWhile,
+                                   c->shadowWide, st));
+                break;
+            }
         }
         TIMED(0, st,
               launchExtend(view, c->pb, b, c->spill, c->traceBlocks, c->refillAt, c->stackDepth, c->whileWhile,
@@ -194,8 +217,9 @@ int finishStats(prr_context *c, const Timing &tm, uint64_t photons, prr_stats *s
     stats->samples = h[2];
     stats->photons_stored = photons;
     stats->ms_total = ms;
-    stats->ms_extend = stats->ms_shadow = stats->ms_shade = 0.0;
-    stats->launches_extend = stats->launches_shadow = 0;
+    stats->rays_closest_finish = h[4];
+    stats->ms_extend = stats->ms_shadow = stats->ms_shade = stats->ms_finish = 0.0;
+    stats->launches_extend = stats->launches_shadow = stats->launches_finish = 0;
     for (const auto &t : tm.timed) {
         float dt = 0.f;
         HIPCK(c, hipEventElapsedTime(&dt, c->evPool[t.second], c->evPool[t.second + 1]));
@@ -205,8 +229,11 @@ int finishStats(prr_context *c, const Timing &tm, uint64_t photons, prr_stats *s
         } else if (t.first == 1) {
             stats->ms_shadow += dt;
             stats->launches_shadow++;
-        } else {
+        } else if (t.first == 2) {
             stats->ms_shade += dt;
+        } else {
+            stats->ms_finish += dt;
+            stats->launches_finish++;
         }
     }
     return PRR_OK;
@@ -264,6 +291,8 @@ int prr_create(int device, prr_context **out) {
     if (e == hipSuccess) e = hipEventCreate(&c->ev1);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&c->evFork, hipEventDisableTiming);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&c->evJoin, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&c->evCnt, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipHostMalloc((void **) &c->cntHost, 64, hipHostMallocDefault);
     if (e != hipSuccess) {
         std::string m = std::string("device init: ") + hipGetErrorString(e);
         delete c;
@@ -276,8 +305,8 @@ int prr_create(int device, prr_context **out) {
     // idle, at most 8 interior steps (4 for shadow rays) per work check
     // (measured on the 1M-tri scene). PRR_TRACE_BLOCKS_PER_CU, PRR_STACK_LDS,
     // PRR_REFILL_AT, PRR_WHILE_WHILE(_ANY), PRR_FETCH_RUN, PRR_SHADOW_WIDE,
-    // PRR_CONCURRENT_SHADOW and PRR_SHADE_WAVES override the measured defaults
-    // for tuning sweeps.
+    // PRR_CONCURRENT_SHADOW, PRR_SHADE_WAVES and PRR_TAIL_PATHS override the
+    // measured defaults for tuning sweeps.
     int tbpc = 8;
     if (const char *ev = getenv("PRR_TRACE_BLOCKS_PER_CU")) tbpc = std::max(1, std::min(16, atoi(ev)));
     c->traceBlocks = c->numCU * tbpc;
@@ -296,6 +325,7 @@ int prr_create(int device, prr_context **out) {
     if (const char *ev = getenv("PRR_CONCURRENT_SHADOW")) c->concurrentShadow = atoi(ev) != 0;
     if (const char *ev = getenv("PRR_FETCH_RUN")) c->fetchRun = std::max(64, std::min(1 << 16, atoi(ev)));
     if (const char *ev = getenv("PRR_SHADE_WAVES")) c->shadeWaves = std::max(1, std::min(5, atoi(ev)));
+    if (const char *ev = getenv("PRR_TAIL_PATHS")) c->tailPaths = std::max<int64_t>(0, atoll(ev));
     if (const char *ev = getenv("PRR_STACK_LDS")) {  // LDS stack entries per lane: 2, 4, 8, 16 or 32
         const int v = atoi(ev);
         if (v == 2 || v == 4 || v == 8 || v == 16 || v == 32) c->stackDepth = v;
@@ -307,7 +337,7 @@ int prr_create(int device, prr_context **out) {
     if (e == hipSuccess) e = hipMalloc(&c->stats, 8 * sizeof(unsigned long long));
     if (e != hipSuccess) {
         std::string m = std::string("device alloc: ") + hipGetErrorString(e);
-        delete c;
+        prr_destroy(c);
         return fail(nullptr, PRR_ERR_NOMEM, m);
     }
     *out = c;
@@ -329,6 +359,8 @@ int prr_destroy(prr_context *c) {
     for (hipEvent_t e : c->evPool) (void) hipEventDestroy(e);
     if (c->evFork) (void) hipEventDestroy(c->evFork);
     if (c->evJoin) (void) hipEventDestroy(c->evJoin);
+    if (c->evCnt) (void) hipEventDestroy(c->evCnt);
+    if (c->cntHost) (void) hipHostFree(c->cntHost);
     if (c->own) (void) hipStreamDestroy(c->own);
     if (c->aux) (void) hipStreamDestroy(c->aux);
     delete c;

@@ -37,6 +37,13 @@ struct prr_context {
     hipStream_t aux = nullptr;
     hipEvent_t evFork = nullptr, evJoin = nullptr;
     uint2 *spillAux = nullptr;
+    // path completion: once a bounce's queue holds at most tailPaths entries
+    // (0: never), one kFinish launch runs those paths to their end instead of
+    // three launches per remaining bounce. The host reads each bounce's count
+    // into cntHost (pinned) and waits on evCnt before choosing.
+    int64_t tailPaths = 1 << 18;
+    uint32_t *cntHost = nullptr;
+    hipEvent_t evCnt = nullptr;
     // render buffers
     prr::PathBufs pb{};
     int64_t pbSlots = 0;
@@ -89,7 +96,7 @@ constexpr int64_t kPathBytes = 360;
 
 struct Timing {
     bool on = false;
-    std::vector<std::pair<int, int>> timed;  // (kind 0 extend / 1 shadow / 2 shade, first event)
+    std::vector<std::pair<int, int>> timed;  // (kind 0 extend / 1 shadow / 2 shade / 3 finish, first event)
     size_t next = 0;
 };
 

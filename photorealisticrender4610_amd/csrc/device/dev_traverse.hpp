@@ -407,7 +407,8 @@ PRR_DEV bool closestIter(const SceneView &s, TravState &ts, LaneStack &st, int c
     uint2 *lds = st.lds + st.lane;
     const int dmask = st.depth - 1;
     bool done = false;
-    for (int k = 0; !done && k != cap && !refIsLeaf(cur); k++) {  // one loop exit
+    const int lim = cap > 0 ? cap : -1;  // k counts up from 0: -1 is never reached
+    for (int k = 0; !done && k != lim && !refIsLeaf(cur); k++) {  // one loop exit
         bool pop = true;
         if (cur != kCulled) {
             // node byte offset fits 32 bits (at most 2^26 nodes of 64 B)

@@ -8,7 +8,8 @@
 // The NEE terms of bounce b are added at bounce b+1, once their shadow rays
 // are resolved, in the reference's order (area, env, then the BSDF-MIS term),
 // so the float sums match the reference's. kAccumulate adds the chunk's
-// results per pixel in sample order (Renderer.cpp:84-99).
+// results per pixel in sample order (Renderer.cpp:84-99). Once few paths
+// remain, kFinish runs the rest of each remaining path in one launch.
 #include <type_traits>
 
 #include "dev_traverse.hpp"
@@ -368,9 +369,270 @@ PRR_DEV void sampleLight(const SceneView &s, Pcg &rng, V3 &pos, V3 &nrm, int &ma
     }
 }
 
+// What shading one path at one bounce emits: its next extension ray (or a
+// terminating entry that brings it back once more for its pending NEE) and
+// up to three shadow rays (area light, environment, point light).
+struct ShadeOut {
+    bool emitRay = false, termEntry = false, emitA = false, emitE = false, emitP = false;
+    V3 newO, newD, sOA, sDA, sOE, sDE, sOP, sDP;
+    float tA = 0.f, tP = 0.f;
+};
+
+// Shading of the path in `slot` at `bounce` (Integrator.cpp:104-263, one
+// loop iteration): h is the closest hit of the ray (o, d) the path traced
+// (t, u, v, prim bits; prim -1 for a miss or a terminating entry).
 // PHOTON / CACHE: whether the caustic photon gather and the radiance-cache
 // query / training code are compiled in (the kernel's registers are those of
 // its heaviest branch, so renders without them use a leaner variant).
+template <bool PHOTON, bool CACHE>
+__device__ __forceinline__ void shadePath(const SceneView &s, const PathBufs &pb, const ChunkInfo &ci, int bounce,
+                                          uint32_t slot, const float4 h, const V3 o, const V3 d, ShadeOut &so) {
+    bool done = false;
+    uint32_t fl = pb.flags[slot];
+    V3 L = ld3(pb.L[slot]);
+    V3 beta = ld3(pb.beta[slot]);
+    const int sIdx = ci.sampleOf ? (int) ci.sampleOf[slot] : ci.sampleBase + (int) (slot / (uint32_t) ci.pixels);
+    Pcg rng;
+    rng.state = pb.rng[slot];
+    rng.inc = pcgInc((uint32_t) sIdx);
+    // 1. NEE of the previous bounce, in reference order
+    if (fl & F_PEND_A)
+        if (pb.vis[slot]) L += ld3(pb.pend[slot]);
+    if (fl & F_PEND_E)
+        if (pb.vis[pb.slots + slot]) L += ld3(pb.pend[pb.slots + slot]);
+    if (fl & F_PEND_P)
+        if (pb.vis[2 * pb.slots + slot]) L += ld3(pb.pend[2 * pb.slots + slot]);
+    if (fl & F_PEND_C) L += ld3(pb.pend[3 * pb.slots + slot]);
+    fl &= ~F_PEND_ALL;
+    done = (fl & F_TERMINATING) != 0u;
+    const int prim = __float_as_int(h.w);
+    // 2. the BSDF-sampled ray of the previous (diffuse) bounce
+    if (!done && (fl & F_PREV_DIFF)) {
+        const bool ind = bounce - 1 > 0;
+        const float4 bs4 = pb.bscale[slot];
+        const V3 bscale = ld3(bs4);
+        const float bpdf = bs4.w;
+        if (prim < 0) {  // Integrator.cpp:217-229
+            if (s.env_w > 0) {
+                const V3 envL = envLookup(s, d);
+                const float w = mis(bpdf, envPdf(s, d));
+                const V3 c = beta * bscale * w * envL;
+                L += ind ? clampInd(c, ci.clampInd) : c;
+            } else {
+                const V3 c = beta * bscale * V3(s.background[0], s.background[1], s.background[2]);
+                L += ind ? clampInd(c, ci.clampInd) : c;
+            }
+            done = true;
+        } else {
+            const HitGeom g = hitGeom(s, prim, h.x, o, d);
+            const DMaterial hm = s.mats[g.mat];
+            if ((int) hm.base_type.w == MAT_EMISSIVE) {  // Integrator.cpp:231-242
+                const float cosT = smax(0.f, dot(-d, normalized(g.Ng)));
+                const V3 dd = g.P - ld3(pb.prevP[slot]);
+                const float hd2 = dot(dd, dd);
+                const float lpA = s.total_emit_area > 0.f ? 1.f / s.total_emit_area : 0.f;
+                const float lpSA = lpA * hd2 / smax(cosT, 1e-4f);
+                const float w = mis(bpdf, lpSA);
+                const V3 c = beta * bscale * w * ld3(hm.emission);
+                L += ind ? clampInd(c, ci.clampInd) : c;
+                done = true;
+            } else {
+                beta = beta * bscale;
+                if (!finite3(beta)) done = true;
+            }
+        }
+        if (!done && bounce - 1 >= 3) {  // Russian roulette (Integrator.cpp:250-256)
+            const float mc = max3(beta);
+            const float qq = smax(0.05f, 1.f - mc);
+            if (rng.next() < qq)
+                done = true;
+            else
+                beta = beta * (1.f / (1.f - qq));
+        }
+        fl &= ~F_PREV_DIFF;
+    }
+    if (!done && bounce >= ci.maxDepth) done = true;
+    // 3. this bounce (Integrator.cpp:124-248)
+    if (!done) {
+        const bool spec = (fl & F_SPEC) != 0u;
+        if (bounce == 0 || spec) {
+            if (prim < 0) {
+                const V3 bg = s.env_w > 0 ? envLookup(s, d)
+                                          : V3(s.background[0], s.background[1], s.background[2]);
+                const V3 c = beta * bg;
+                L += bounce > 0 ? clampInd(c, ci.clampInd) : c;
+            }
+        }
+        if (prim < 0) {
+            done = true;
+        } else {
+            const HitGeom g = hitGeom(s, prim, h.x, o, d);
+            const DMaterial m = s.mats[g.mat];
+            const int mtype = (int) m.base_type.w;
+            const V3 base = ld3(m.base_type);
+            if (mtype == MAT_EMISSIVE) {
+                if (bounce == 0 || spec) {
+                    const V3 c = beta * ld3(m.emission);
+                    L += bounce > 0 ? clampInd(c, ci.clampInd) : c;
+                }
+                // scatters nothing (EmissiveMaterial: f = pdf = 0)
+                done = true;
+            } else if (mtype == MAT_GLASS || mtype == MAT_MIRROR) {
+                const BsdfSample bs = mtype == MAT_GLASS ? sampleGlass(-d, g.Ng, m.params.z, rng.next())
+                                                         : sampleMirror(-d, g.Ng, base);
+                beta = beta * bs.f / bs.pdf;
+                const bool trans = dot(bs.wi, g.Ng) < 0.f;
+                const V3 off = trans ? -g.Ng : g.Ng;
+                if (trans) beta = beta * base;  // Integrator.cpp:161-165
+                fl |= F_SPEC;
+                if (bounce + 1 >= ci.maxDepth) {
+                    done = true;  // the loop ends before this ray is used
+                } else {
+                    so.emitRay = true;
+                    so.newO = g.P + off * kEps;
+                    so.newD = bs.wi;
+                }
+            } else {
+                fl &= ~F_SPEC;
+                const V3 sN = dot(d, g.Ng) < 0.f ? g.Ng : -g.Ng;
+                const V3 wo = -d;
+                const CacheRec &cr = ci.cache;
+                bool cacheQuery = false;
+                if (CACHE && (cr.query | cr.train)) {
+                    const int stopAt = (fl & F_TRAIN) ? cr.trainStop : cr.queryBounce;
+                    if (cr.query && stopAt > 0 && bounce >= stopAt) {
+                        // the path ends here; the cache adds beta * L_out(P, wo) (for a
+                        // training path, also to the targets of its recorded vertices)
+                        cacheQuery = true;
+                        done = true;
+                        const uint32_t k = waveAppend(cr.qCount, true);
+                        putQuery(cr.q + (int64_t) k * kCacheQueryFloats, g.P, wo, sN, m.params.x, base);
+                        cr.qSlot[k] = slot;
+                        cr.qBeta[k] = st4(beta, 0.f);
+                    } else if ((fl & F_TRAIN) && bounce >= 1 && min3(beta) > 1e-3f) {
+                        // target later: (final L - L so far) / beta
+                        const uint32_t k = waveAppend(cr.tCount, true);
+                        if (k < cr.tCap) {
+                            putQuery(cr.tq + (int64_t) k * kCacheQueryFloats, g.P, wo, sN, m.params.x, base);
+                            cr.tSlot[k] = slot;
+                            cr.tBeta[k] = st4(beta, 0.f);
+                            cr.tLpre[k] = st4(L, 0.f);
+                        }
+                    }
+                }
+                if (!cacheQuery) {
+                const Frame fr = makeFrame(sN);
+                const Lobes lb = makeLobes(base, m.params.x, m.params.y, m.params.z);
+                const bool ind = bounce > 0;
+                // NEE area (Integrator.cpp:14-34, 78-87)
+                if (s.total_emit_area > 0.f) {
+                    V3 lp, ln;
+                    int lm;
+                    sampleLight(s, rng, lp, ln, lm);
+                    const V3 toL = lp - g.P;
+                    const float d2 = dot(toL, toL);
+                    const float dist = sqrtf(d2);
+                    const V3 wi = toL / dist;
+                    const float cosL = smax(0.f, dot(-wi, normalized(ln)));
+                    if (cosL > 0.f) {
+                        const float pdfSA = (1.0f / s.total_emit_area) * d2 / cosL;
+                        V3 f;
+                        float bp;
+                        evalPdf(lb, fr, wi, wo, f, bp);
+                        const float w = mis(pdfSA, bp);
+                        V3 c = beta * (w * ld3(s.mats[lm].emission) * f / pdfSA);
+                        c = ind ? clampInd(c, ci.clampInd) : c;
+                        pb.pend[slot] = st4(c, 0.f);
+                        fl |= F_PEND_A;
+                        so.emitA = true;
+                        so.sOA = g.P + wi * kEps;
+                        so.sDA = wi;
+                        so.tA = dist - 2.f * kEps;
+                    }
+                }
+                // NEE environment (Integrator.cpp:39-66, 93-102)
+                if (s.env_w > 0) {
+                    const float r1 = rng.next(), r2 = rng.next();
+                    float pe;
+                    const V3 dir = envImportanceSample(s, r1, r2, pe);
+                    if (!(pe < 1e-6f)) {
+                        const V3 Le = envLookup(s, dir);
+                        V3 f;
+                        float bp;
+                        evalPdf(lb, fr, dir, wo, f, bp);
+                        const float w = mis(pe, bp);
+                        V3 c = beta * (w * Le * f / (pe + 1e-6f));
+                        c = ind ? clampInd(c, ci.clampInd) : c;
+                        pb.pend[pb.slots + slot] = st4(c, 0.f);
+                        fl |= F_PEND_E;
+                        so.emitE = true;
+                        so.sOE = g.P + dir * kEps;
+                        so.sDE = dir;
+                    }
+                }
+                // NEE point light (build extension; no draws)
+                if (s.num_points > 0) {
+                    const V3 pp(s.point_pos[0], s.point_pos[1], s.point_pos[2]);
+                    const V3 toL = pp - g.P;
+                    const float d2 = dot(toL, toL);
+                    const float dist = sqrtf(d2);
+                    const V3 wi = toL / dist;
+                    V3 f;
+                    float bp;
+                    evalPdf(lb, fr, wi, wo, f, bp);
+                    V3 c = beta * (V3(s.point_int[0], s.point_int[1], s.point_int[2]) * f / d2);
+                    c = ind ? clampInd(c, ci.clampInd) : c;
+                    pb.pend[2 * pb.slots + slot] = st4(c, 0.f);
+                    fl |= F_PEND_P;
+                    so.emitP = true;
+                    so.sOP = g.P + wi * kEps;
+                    so.sDP = wi;
+                    so.tP = dist - 2.f * kEps;
+                }
+                // caustic photon estimate at camera / delta-reached vertices
+                // (build extension; no draws)
+                if (PHOTON && s.ph_count > 0 && (bounce == 0 || spec)) {
+                    V3 c = beta * photonRadiance(s, g.P, wo, fr, lb);
+                    c = ind ? clampInd(c, ci.clampInd) : c;
+                    pb.pend[3 * pb.slots + slot] = st4(c, 0.f);
+                    fl |= F_PEND_C;
+                }
+                // BSDF sampling (Integrator.cpp:207-215)
+                const BsdfSample bs = sampleDiffuse(lb, fr, wo, rng);
+                if (bs.pdf < 1e-6f) {
+                    done = true;
+                } else {
+                    const V3 bscale = bs.f / bs.pdf;
+                    const V3 off = dot(bs.wi, sN) >= 0.f ? sN : -sN;
+                    pb.bscale[slot] = st4(bscale, bs.pdf);
+                    pb.prevP[slot] = st4(g.P, 0.f);
+                    fl |= F_PREV_DIFF;
+                    so.emitRay = true;
+                    so.newO = g.P + off * kEps;
+                    so.newD = bs.wi;
+                }
+                }  // !cacheQuery
+            }
+        }
+    }
+    // a path that ends with NEE still in flight comes back once more
+    if (done && (fl & F_PEND_ALL) && !(fl & F_TERMINATING)) {
+        fl |= F_TERMINATING;
+        so.emitRay = true;
+        done = false;
+    }
+    if (done) {
+        const V3 r = finite3(L) ? L : V3(0.f);  // Integrator.cpp:259-261
+        pb.result[slot] = st4(r, 0.f);
+    } else {
+        pb.L[slot] = st4(L, 0.f);
+        pb.beta[slot] = st4(beta, 0.f);
+        pb.rng[slot] = rng.state;
+    }
+    pb.flags[slot] = fl;
+    so.termEntry = (fl & F_TERMINATING) != 0u;
+}
+
 template <bool PHOTON, bool CACHE>
 __device__ __forceinline__ void shadeImpl(const SceneView &s, const PathBufs &pb, const ChunkInfo &ci, int bounce) {
     const int q = bounce & 1;
@@ -382,265 +644,20 @@ __device__ __forceinline__ void shadeImpl(const SceneView &s, const PathBufs &pb
     uint32_t nRays = 0, nShadow = 0;
     for (int64_t wb = waveBase; wb < n; wb += stride) {
         const int64_t i = wb + (threadIdx.x & 63);
-        const bool active = i < n;
-        bool emitRay = false, emitA = false, emitE = false, emitP = false, done = false, termEntry = false;
-        V3 newO, newD, sOA, sDA, sOE, sDE, sOP, sDP;
-        float tA = 0.f, tP = 0.f;
         uint32_t slot = 0;
-        if (active) {
+        ShadeOut so;
+        if (i < n) {
             slot = pb.qSlot[q][i];
-            uint32_t fl = pb.flags[slot];
-            V3 L = ld3(pb.L[slot]);
-            V3 beta = ld3(pb.beta[slot]);
-            const int sIdx = ci.sampleOf ? (int) ci.sampleOf[slot] : ci.sampleBase + (int) (slot / (uint32_t) ci.pixels);
-            Pcg rng;
-            rng.state = pb.rng[slot];
-            rng.inc = pcgInc((uint32_t) sIdx);
-            // 1. NEE of the previous bounce, in reference order
-            if (fl & F_PEND_A)
-                if (pb.vis[slot]) L += ld3(pb.pend[slot]);
-            if (fl & F_PEND_E)
-                if (pb.vis[pb.slots + slot]) L += ld3(pb.pend[pb.slots + slot]);
-            if (fl & F_PEND_P)
-                if (pb.vis[2 * pb.slots + slot]) L += ld3(pb.pend[2 * pb.slots + slot]);
-            if (fl & F_PEND_C) L += ld3(pb.pend[3 * pb.slots + slot]);
-            fl &= ~F_PEND_ALL;
-            done = (fl & F_TERMINATING) != 0u;
-            const float4 h = pb.hit[i];
-            const int prim = __float_as_int(h.w);
-            const V3 o = ld3(pb.qOrg[q][i]), d = ld3(pb.qDir[q][i]);
-            // 2. the BSDF-sampled ray of the previous (diffuse) bounce
-            if (!done && (fl & F_PREV_DIFF)) {
-                const bool ind = bounce - 1 > 0;
-                const float4 bs4 = pb.bscale[slot];
-                const V3 bscale = ld3(bs4);
-                const float bpdf = bs4.w;
-                if (prim < 0) {  // Integrator.cpp:217-229
-                    if (s.env_w > 0) {
-                        const V3 envL = envLookup(s, d);
-                        const float w = mis(bpdf, envPdf(s, d));
-                        const V3 c = beta * bscale * w * envL;
-                        L += ind ? clampInd(c, ci.clampInd) : c;
-                    } else {
-                        const V3 c = beta * bscale * V3(s.background[0], s.background[1], s.background[2]);
-                        L += ind ? clampInd(c, ci.clampInd) : c;
-                    }
-                    done = true;
-                } else {
-                    const HitGeom g = hitGeom(s, prim, h.x, o, d);
-                    const DMaterial hm = s.mats[g.mat];
-                    if ((int) hm.base_type.w == MAT_EMISSIVE) {  // Integrator.cpp:231-242
-                        const float cosT = smax(0.f, dot(-d, normalized(g.Ng)));
-                        const V3 dd = g.P - ld3(pb.prevP[slot]);
-                        const float hd2 = dot(dd, dd);
-                        const float lpA = s.total_emit_area > 0.f ? 1.f / s.total_emit_area : 0.f;
-                        const float lpSA = lpA * hd2 / smax(cosT, 1e-4f);
-                        const float w = mis(bpdf, lpSA);
-                        const V3 c = beta * bscale * w * ld3(hm.emission);
-                        L += ind ? clampInd(c, ci.clampInd) : c;
-                        done = true;
-                    } else {
-                        beta = beta * bscale;
-                        if (!finite3(beta)) done = true;
-                    }
-                }
-                if (!done && bounce - 1 >= 3) {  // Russian roulette (Integrator.cpp:250-256)
-                    const float mc = max3(beta);
-                    const float qq = smax(0.05f, 1.f - mc);
-                    if (rng.next() < qq)
-                        done = true;
-                    else
-                        beta = beta * (1.f / (1.f - qq));
-                }
-                fl &= ~F_PREV_DIFF;
-            }
-            if (!done && bounce >= ci.maxDepth) done = true;
-            // 3. this bounce (Integrator.cpp:124-248)
-            if (!done) {
-                const bool spec = (fl & F_SPEC) != 0u;
-                if (bounce == 0 || spec) {
-                    if (prim < 0) {
-                        const V3 bg = s.env_w > 0 ? envLookup(s, d)
-                                                  : V3(s.background[0], s.background[1], s.background[2]);
-                        const V3 c = beta * bg;
-                        L += bounce > 0 ? clampInd(c, ci.clampInd) : c;
-                    }
-                }
-                if (prim < 0) {
-                    done = true;
-                } else {
-                    const HitGeom g = hitGeom(s, prim, h.x, o, d);
-                    const DMaterial m = s.mats[g.mat];
-                    const int mtype = (int) m.base_type.w;
-                    const V3 base = ld3(m.base_type);
-                    if (mtype == MAT_EMISSIVE) {
-                        if (bounce == 0 || spec) {
-                            const V3 c = beta * ld3(m.emission);
-                            L += bounce > 0 ? clampInd(c, ci.clampInd) : c;
-                        }
-                        // scatters nothing (EmissiveMaterial: f = pdf = 0)
-                        done = true;
-                    } else if (mtype == MAT_GLASS || mtype == MAT_MIRROR) {
-                        const BsdfSample bs = mtype == MAT_GLASS ? sampleGlass(-d, g.Ng, m.params.z, rng.next())
-                                                                 : sampleMirror(-d, g.Ng, base);
-                        beta = beta * bs.f / bs.pdf;
-                        const bool trans = dot(bs.wi, g.Ng) < 0.f;
-                        const V3 off = trans ? -g.Ng : g.Ng;
-                        if (trans) beta = beta * base;  // Integrator.cpp:161-165
-                        fl |= F_SPEC;
-                        if (bounce + 1 >= ci.maxDepth) {
-                            done = true;  // the loop ends before this ray is used
-                        } else {
-                            emitRay = true;
-                            newO = g.P + off * kEps;
-                            newD = bs.wi;
-                        }
-                    } else {
-                        fl &= ~F_SPEC;
-                        const V3 sN = dot(d, g.Ng) < 0.f ? g.Ng : -g.Ng;
-                        const V3 wo = -d;
-                        const CacheRec &cr = ci.cache;
-                        bool cacheQuery = false;
-                        if (CACHE && (cr.query | cr.train)) {
-                            const int stopAt = (fl & F_TRAIN) ? cr.trainStop : cr.queryBounce;
-                            if (cr.query && stopAt > 0 && bounce >= stopAt) {
-                                // the path ends here; the cache adds beta * L_out(P, wo) (for a
-                                // training path, also to the targets of its recorded vertices)
-                                cacheQuery = true;
-                                done = true;
-                                const uint32_t k = waveAppend(cr.qCount, true);
-                                putQuery(cr.q + (int64_t) k * kCacheQueryFloats, g.P, wo, sN, m.params.x, base);
-                                cr.qSlot[k] = slot;
-                                cr.qBeta[k] = st4(beta, 0.f);
-                            } else if ((fl & F_TRAIN) && bounce >= 1 && min3(beta) > 1e-3f) {
-                                // target later: (final L - L so far) / beta
-                                const uint32_t k = waveAppend(cr.tCount, true);
-                                if (k < cr.tCap) {
-                                    putQuery(cr.tq + (int64_t) k * kCacheQueryFloats, g.P, wo, sN, m.params.x, base);
-                                    cr.tSlot[k] = slot;
-                                    cr.tBeta[k] = st4(beta, 0.f);
-                                    cr.tLpre[k] = st4(L, 0.f);
-                                }
-                            }
-                        }
-                        if (!cacheQuery) {
-                        const Frame fr = makeFrame(sN);
-                        const Lobes lb = makeLobes(base, m.params.x, m.params.y, m.params.z);
-                        const bool ind = bounce > 0;
-                        // NEE area (Integrator.cpp:14-34, 78-87)
-                        if (s.total_emit_area > 0.f) {
-                            V3 lp, ln;
-                            int lm;
-                            sampleLight(s, rng, lp, ln, lm);
-                            const V3 toL = lp - g.P;
-                            const float d2 = dot(toL, toL);
-                            const float dist = sqrtf(d2);
-                            const V3 wi = toL / dist;
-                            const float cosL = smax(0.f, dot(-wi, normalized(ln)));
-                            if (cosL > 0.f) {
-                                const float pdfSA = (1.0f / s.total_emit_area) * d2 / cosL;
-                                V3 f;
-                                float bp;
-                                evalPdf(lb, fr, wi, wo, f, bp);
-                                const float w = mis(pdfSA, bp);
-                                V3 c = beta * (w * ld3(s.mats[lm].emission) * f / pdfSA);
-                                c = ind ? clampInd(c, ci.clampInd) : c;
-                                pb.pend[slot] = st4(c, 0.f);
-                                fl |= F_PEND_A;
-                                emitA = true;
-                                sOA = g.P + wi * kEps;
-                                sDA = wi;
-                                tA = dist - 2.f * kEps;
-                            }
-                        }
-                        // NEE environment (Integrator.cpp:39-66, 93-102)
-                        if (s.env_w > 0) {
-                            const float r1 = rng.next(), r2 = rng.next();
-                            float pe;
-                            const V3 dir = envImportanceSample(s, r1, r2, pe);
-                            if (!(pe < 1e-6f)) {
-                                const V3 Le = envLookup(s, dir);
-                                V3 f;
-                                float bp;
-                                evalPdf(lb, fr, dir, wo, f, bp);
-                                const float w = mis(pe, bp);
-                                V3 c = beta * (w * Le * f / (pe + 1e-6f));
-                                c = ind ? clampInd(c, ci.clampInd) : c;
-                                pb.pend[pb.slots + slot] = st4(c, 0.f);
-                                fl |= F_PEND_E;
-                                emitE = true;
-                                sOE = g.P + dir * kEps;
-                                sDE = dir;
-                            }
-                        }
-                        // NEE point light (build extension; no draws)
-                        if (s.num_points > 0) {
-                            const V3 pp(s.point_pos[0], s.point_pos[1], s.point_pos[2]);
-                            const V3 toL = pp - g.P;
-                            const float d2 = dot(toL, toL);
-                            const float dist = sqrtf(d2);
-                            const V3 wi = toL / dist;
-                            V3 f;
-                            float bp;
-                            evalPdf(lb, fr, wi, wo, f, bp);
-                            V3 c = beta * (V3(s.point_int[0], s.point_int[1], s.point_int[2]) * f / d2);
-                            c = ind ? clampInd(c, ci.clampInd) : c;
-                            pb.pend[2 * pb.slots + slot] = st4(c, 0.f);
-                            fl |= F_PEND_P;
-                            emitP = true;
-                            sOP = g.P + wi * kEps;
-                            sDP = wi;
-                            tP = dist - 2.f * kEps;
-                        }
-                        // caustic photon estimate at camera / delta-reached vertices
-                        // (build extension; no draws)
-                        if (PHOTON && s.ph_count > 0 && (bounce == 0 || spec)) {
-                            V3 c = beta * photonRadiance(s, g.P, wo, fr, lb);
-                            c = ind ? clampInd(c, ci.clampInd) : c;
-                            pb.pend[3 * pb.slots + slot] = st4(c, 0.f);
-                            fl |= F_PEND_C;
-                        }
-                        // BSDF sampling (Integrator.cpp:207-215)
-                        const BsdfSample bs = sampleDiffuse(lb, fr, wo, rng);
-                        if (bs.pdf < 1e-6f) {
-                            done = true;
-                        } else {
-                            const V3 bscale = bs.f / bs.pdf;
-                            const V3 off = dot(bs.wi, sN) >= 0.f ? sN : -sN;
-                            pb.bscale[slot] = st4(bscale, bs.pdf);
-                            pb.prevP[slot] = st4(g.P, 0.f);
-                            fl |= F_PREV_DIFF;
-                            emitRay = true;
-                            newO = g.P + off * kEps;
-                            newD = bs.wi;
-                        }
-                        }  // !cacheQuery
-                    }
-                }
-            }
-            // a path that ends with NEE still in flight comes back once more
-            if (done && (fl & F_PEND_ALL) && !(fl & F_TERMINATING)) {
-                fl |= F_TERMINATING;
-                emitRay = true;
-                done = false;
-            }
-            if (done) {
-                const V3 r = finite3(L) ? L : V3(0.f);  // Integrator.cpp:259-261
-                pb.result[slot] = st4(r, 0.f);
-            } else {
-                pb.L[slot] = st4(L, 0.f);
-                pb.beta[slot] = st4(beta, 0.f);
-                pb.rng[slot] = rng.state;
-            }
-            pb.flags[slot] = fl;
-            termEntry = (fl & F_TERMINATING) != 0u;
+            shadePath<PHOTON, CACHE>(s, pb, ci, bounce, slot, pb.hit[i], ld3(pb.qOrg[q][i]), ld3(pb.qDir[q][i]), so);
         }
+        const bool emitRay = so.emitRay, termEntry = so.termEntry, emitA = so.emitA, emitE = so.emitE,
+                   emitP = so.emitP;
         // queue appends (all lanes of the wave take part)
         const uint32_t qi = waveAppend(&pb.qcnt[bounce + 1], emitRay);
         if (emitRay) {
             pb.qSlot[q ^ 1][qi] = slot;
-            pb.qOrg[q ^ 1][qi] = termEntry ? make_float4(0.f, 0.f, 0.f, -1.f) : st4(newO, 0.f);
-            pb.qDir[q ^ 1][qi] = termEntry ? make_float4(0.f, 0.f, 0.f, 0.f) : st4(newD, 0.f);
+            pb.qOrg[q ^ 1][qi] = termEntry ? make_float4(0.f, 0.f, 0.f, -1.f) : st4(so.newO, 0.f);
+            pb.qDir[q ^ 1][qi] = termEntry ? make_float4(0.f, 0.f, 0.f, 0.f) : st4(so.newD, 0.f);
         }
         // the wave's shadow rays (area, env, point) take one atomic together
         {
@@ -656,20 +673,20 @@ __device__ __forceinline__ void shadeImpl(const SceneView &s, const PathBufs &pb
             if (emitA) {
                 const uint32_t ia = base + (uint32_t) __popcll(mA & below);
                 pb.sqSlot[ia] = slot;
-                pb.sqOrg[ia] = st4(sOA, tA);
-                pb.sqDir[ia] = st4(sDA, 0.f);
+                pb.sqOrg[ia] = st4(so.sOA, so.tA);
+                pb.sqDir[ia] = st4(so.sDA, 0.f);
             }
             if (emitE) {
                 const uint32_t ie = base + nA + (uint32_t) __popcll(mE & below);
                 pb.sqSlot[ie] = slot | (1u << 30);
-                pb.sqOrg[ie] = st4(sOE, __builtin_inff());
-                pb.sqDir[ie] = st4(sDE, 0.f);
+                pb.sqOrg[ie] = st4(so.sOE, __builtin_inff());
+                pb.sqDir[ie] = st4(so.sDE, 0.f);
             }
             if (emitP) {
                 const uint32_t ip = base + nA + nE + (uint32_t) __popcll(mP & below);
                 pb.sqSlot[ip] = slot | (2u << 30);
-                pb.sqOrg[ip] = st4(sOP, tP);
-                pb.sqDir[ip] = st4(sDP, 0.f);
+                pb.sqOrg[ip] = st4(so.sOP, so.tP);
+                pb.sqDir[ip] = st4(so.sDP, 0.f);
             }
         }
         nRays += (emitRay && !termEntry) ? 1u : 0u;
@@ -692,6 +709,93 @@ __device__ __forceinline__ void shadeImpl(const SceneView &s, const PathBufs &pb
 template <int W, bool PHOTON, bool CACHE>
 __global__ __launch_bounds__(kShadeBlock, W) void kShade(SceneView s, PathBufs pb, ChunkInfo ci, int bounce) {
     shadeImpl<PHOTON, CACHE>(s, pb, ci, bounce);
+}
+
+// ---------------------------------------------------------------------------
+// Path completion. Each bounce of the wavefront costs three launches, and
+// each launch lasts as long as its slowest ray; once few paths remain, those
+// tails cost more than the work. kFinish takes the entries of bounce b0's
+// queue (the shadow rays of bounce b0 - 1 already resolved) and runs each
+// path to its end in one lane: closest hit, shading, then the shadow rays
+// shading emitted, traced at once instead of in the next launch. A path
+// meets the same traversals and the same shading code, bounce by bounce, as
+// in the wavefront kernels, so every slot ends with the same result.
+
+// IntersectP of one shadow ray on the chosen tree (0 the binary tree, 1 the
+// four-wide occlusion tree, 2 its quantised copy), traced to its end.
+PRR_DEV bool occluded(const SceneView &s, const V3 &o, const V3 &d, float tmax, int tree, int cap, LaneStack &lst,
+                      RefStack &rst) {
+    if (tree == 2) return traceAny4Q(s, o, d, tmax, rst);
+    if (tree == 1) return traceAny4(s, o, d, tmax, rst);
+    TravState ts;
+    travBegin(ts, o, d, tmax, s.root, lst);
+    if (!anyBegin(s, ts)) return false;
+    int r;
+    while ((r = anyIter(s, ts, lst, cap)) == kAnyGo) {
+    }
+    return r == kAnyHit;
+}
+
+template <bool PHOTON, bool CACHE>
+__global__ __launch_bounds__(kTraceBlock) void kFinish(SceneView s, PathBufs pb, ChunkInfo ci, int bounce0,
+                                                       uint2 *spill, int stackDepth, int wwCap, int tree) {
+    extern __shared__ uint2 stk[];
+    const int q = bounce0 & 1;
+    const int64_t n = pb.qcnt[bounce0];
+    uint32_t *fetch = &pb.fetch[2 * bounce0];
+    const int64_t lane = (int64_t) blockIdx.x * kTraceBlock + threadIdx.x;
+    // Two LDS stack areas: the closest-hit stack (uint2 entries) and the
+    // shadow-ray stack (uint32 entries). Sharing one area would overlap a
+    // lane's entries with those of another lane (lane / 2 at twice the
+    // entry size), which may belong to a different wave of the block that is
+    // traversing at the same time. A lane runs one traversal at a time, so
+    // its two stacks share its spill region.
+    LaneStack lst = makeStack<LaneStack>(stk, spill + lane * kSpill, threadIdx.x, stackDepth);
+    RefStack rst = makeStack<RefStack>(stk + stackDepth * kTraceBlock, spill + lane * kSpill, threadIdx.x, stackDepth);
+    WaveRange wr;
+    const int64_t waves = (int64_t) gridDim.x * (kTraceBlock / 64);
+    uint32_t nRays = 0, nShadow = 0, nTraced = 0;
+    while (true) {
+        // every lane that reaches this point takes one entry (>= n: none left)
+        const int64_t i = waveTake(wr, fetch, n, true, 64, waves);
+        if (i >= n) break;
+        const uint32_t slot = pb.qSlot[q][i];
+        float4 o4 = pb.qOrg[q][i], d4 = pb.qDir[q][i];
+        for (int b = bounce0;; b++) {
+            float4 h = make_float4(__builtin_inff(), 0.f, 0.f, __int_as_float(-1));
+            if (!(o4.w < 0.f)) {  // w < 0: terminating entry, no ray
+                TravState ts;
+                travBegin(ts, ld3(o4), ld3(d4), 0.f, s.root, lst);
+                while (!closestIter(s, ts, lst, wwCap)) {
+                }
+                h = make_float4(ts.h.t, ts.h.u, ts.h.v, __int_as_float(ts.h.prim));
+                nTraced++;
+            }
+            ShadeOut so;
+            shadePath<PHOTON, CACHE>(s, pb, ci, b, slot, h, ld3(o4), ld3(d4), so);
+            if (so.emitA) pb.vis[slot] = occluded(s, so.sOA, so.sDA, so.tA, tree, wwCap, lst, rst) ? 0 : 1;
+            if (so.emitE)
+                pb.vis[pb.slots + slot] =
+                    occluded(s, so.sOE, so.sDE, __builtin_inff(), tree, wwCap, lst, rst) ? 0 : 1;
+            if (so.emitP) pb.vis[2 * pb.slots + slot] = occluded(s, so.sOP, so.sDP, so.tP, tree, wwCap, lst, rst) ? 0 : 1;
+            nRays += (so.emitRay && !so.termEntry) ? 1u : 0u;
+            nShadow += (so.emitA ? 1u : 0u) + (so.emitE ? 1u : 0u) + (so.emitP ? 1u : 0u);
+            // the wavefront shades bounces 0..maxDepth, never the queue past it
+            if (!so.emitRay || b >= ci.maxDepth) break;
+            o4 = so.termEntry ? make_float4(0.f, 0.f, 0.f, -1.f) : st4(so.newO, 0.f);
+            d4 = st4(so.newD, 0.f);
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        nRays += __shfl_xor(nRays, o);
+        nShadow += __shfl_xor(nShadow, o);
+        nTraced += __shfl_xor(nTraced, o);
+    }
+    if (laneId() == 0) {
+        if (nRays) atomicAdd(&pb.stats[0], (unsigned long long) nRays);
+        if (nShadow) atomicAdd(&pb.stats[1], (unsigned long long) nShadow);
+        if (nTraced) atomicAdd(&pb.stats[4], (unsigned long long) nTraced);
+    }
 }
 
 // Radiance-cache training targets of records [r0, r1): the radiance the
@@ -855,6 +959,27 @@ hipError_t launchShade(const SceneView &s, const PathBufs &pb, const ChunkInfo &
     else
         cache ? shadeLaunch<false, true>(s, pb, ci, bounce, grid, waves, st)
               : shadeLaunch<false, false>(s, pb, ci, bounce, grid, waves, st);
+    return hipGetLastError();
+}
+template <bool P, bool C>
+static void finishLaunch(const SceneView &s, const PathBufs &pb, const ChunkInfo &ci, int bounce0, uint2 *spill,
+                         int grid, int stackDepth, int wwCap, int tree, hipStream_t st) {
+    // two stack areas (closest-hit and shadow rays)
+    hipLaunchKernelGGL((kFinish<P, C>), dim3(grid), dim3(kTraceBlock), 2 * stackDepth * kTraceBlock * sizeof(uint2), st,
+                       s, pb, ci, bounce0, spill, stackDepth, wwCap, tree);
+}
+hipError_t launchFinish(const SceneView &s, const PathBufs &pb, const ChunkInfo &ci, int bounce0, uint2 *spill,
+                        int traceGrid, int stackDepth, int whileWhile, int tree, hipStream_t st) {
+    const int wwCap = whileWhile > 1 ? whileWhile : 0;
+    // two stack areas of 2 KB per LDS entry: at most 64 KB per block
+    stackDepth = stackDepth < 16 ? stackDepth : 16;
+    const bool photon = s.ph_count > 0, cache = ci.cache.query || ci.cache.train;
+    if (photon)
+        cache ? finishLaunch<true, true>(s, pb, ci, bounce0, spill, traceGrid, stackDepth, wwCap, tree, st)
+              : finishLaunch<true, false>(s, pb, ci, bounce0, spill, traceGrid, stackDepth, wwCap, tree, st);
+    else
+        cache ? finishLaunch<false, true>(s, pb, ci, bounce0, spill, traceGrid, stackDepth, wwCap, tree, st)
+              : finishLaunch<false, false>(s, pb, ci, bounce0, spill, traceGrid, stackDepth, wwCap, tree, st);
     return hipGetLastError();
 }
 hipError_t launchAccumulate(const PathBufs &pb, const ChunkInfo &ci, float4 *fb, hipStream_t st) {

@@ -38,7 +38,8 @@ struct PathBufs {
     uint32_t *qcnt;          // ray-queue count per bounce
     uint32_t *sqcnt;         // shadow-queue count per bounce
     uint32_t *fetch;         // persistent-kernel work counters: [2b] extend, [2b+1] shadow
-    unsigned long long *stats;  // [0] closest rays, [1] any rays, [2] samples, [3] photons stored
+    unsigned long long *stats;  // [0] closest rays, [1] any rays, [2] samples, [3] photons stored,
+                                // [4] closest rays traced by kFinish
 };
 
 constexpr int kCacheQueryFloats = 13;  // position, wo, normal, roughness, albedo
