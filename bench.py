@@ -49,8 +49,9 @@ WORKLOADS = {
 # radiance cache on the sphere_field scene: frames of 4 spp, plain vs cache
 CACHE_WORKLOAD = ("sphere_field", 1920, 1080, 4, 12, 60,
                   "radiance_cache: sphere_field (1,048,578 tris), 1920x1080, 4 spp per frame x 60 frames, depth 12; "
-                  "FP16 MLP 96-64x5-3 trained online (Adam lr 1e-2) on 1/16 of the paths, which stop one bounce "
-                  "later and take the cache's value as their tail (self-training); other paths stop at bounce 2 and "
+                  "FP16 MLP 96-64x5-3 trained online (Adam lr 1e-2, batches of 32768, at most 16 steps per frame) "
+                  "on 1/16 of the paths, which stop one bounce later and take the cache's value as their tail "
+                  "(self-training); other paths stop at bounce 2 and "
                   "query (full_depth_training: the same with training paths run to full depth); relMSE against a "
                   "plain reference image")
 CACHE_RENDER = {"query_bounce": 2, "train_every": 16, "train_suffix": 1}

@@ -376,7 +376,7 @@ typedef struct prr_cache_params {
     float grad_clip;       /* bound on |dLoss/dy| per output before the FP16 backward pass (100) */
     float bounds[6];       /* position-encoding box: min xyz, max xyz */
     uint64_t seed;         /* He-uniform initialisation */
-    int32_t train_batch;   /* samples per Adam step (65536) */
+    int32_t train_batch;   /* samples per Adam step (32768) */
     int32_t reserved[3];
 } prr_cache_params;
 

@@ -288,7 +288,7 @@ class Context:
         return Server(self, scenes, depths, clamps, path_slots, hbm_budget, policy)
 
     def cache(self, bounds, seed=0xC4, lr=1e-2, beta1=0.9, beta2=0.99, eps=1e-8, grad_clip=100.0,
-              train_batch=65536):
+              train_batch=32768):
         return Cache(self, bounds, seed, lr, beta1, beta2, eps, grad_clip, train_batch)
 
     # -- batch entry points on torch tensors -------------------------------

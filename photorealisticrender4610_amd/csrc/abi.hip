@@ -173,7 +173,8 @@ int runBounces(prr_context *c, const SceneView &view, const ChunkInfo &ci, Timin
         }
         if (tail) {
             HIPCK(c, hipEventSynchronize(c->evCnt));
-            if ((int64_t) *c->cntHost <= c->tailPaths) {
+            const int64_t left = (int64_t) *c->cntHost;
+            if (left <= c->tailPaths) {
                 if (b > 0 && !conc)
                     TIMED(1, st,
                           launchShadow(view, c->pb, b - 1, c->spill, c->traceBlocks, c->refillAt, c->stackDepth,
@@ -181,7 +182,7 @@ int runBounces(prr_context *c, const SceneView &view, const ChunkInfo &ci, Timin
                 if (b > 0 && conc) HIPCK(c, hipStreamWaitEvent(st, c->evJoin, 0));
                 TIMED(3, st,
                       launchFinish(view, c->pb, ci, b, c->spill, c->traceBlocks, c->stackDepth, c->whileWhile,
-                                   c->shadowWide, st));
+                                   c->shadowWide, left, c->numCU, st));
                 break;
             }
         }

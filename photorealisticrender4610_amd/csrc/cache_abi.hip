@@ -365,6 +365,7 @@ int prr_cache_render(prr_context *c, const prr_scene *s, prr_cache *k, const prr
         return v;
     }();
     Timing tm;
+    tm.on = (p->flags & PRR_RENDER_KERNEL_TIMING) != 0;
     ChunkInfo ci;
     ci.W = p->width;
     ci.H = p->height;

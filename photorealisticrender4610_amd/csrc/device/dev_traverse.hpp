@@ -684,7 +684,7 @@ PRR_DEV bool traceAny4Q(const SceneView &s, const V3 &o, const V3 &d, float tMax
 // Work fetch through a wave-private range of queue entries. A wave claims a
 // run of consecutive entries from *counter with one atomic and hands them to
 // its idle lanes over several refills; the run is sized by guided
-// self-scheduling, max(64, min(maxRun, remaining / (4 * waves))), so that
+// self-scheduling, max(minRun, min(maxRun, remaining / (4 * waves))), so that
 // the atomics on the one shared counter fall from one per refill to a few
 // per wave while the runs shrink to a single refill's worth near the end of
 // the queue (no wave sits on a long run while others are idle). `remaining`
@@ -696,7 +696,9 @@ struct WaveRange {
     int64_t seen = 0;        // counter value after this wave's last claim
 };
 
-PRR_DEV int64_t waveTake(WaveRange &r, uint32_t *counter, int64_t n, bool need, int64_t maxRun, int64_t waves) {
+// minRun must be at least the number of lanes that can need an entry at once.
+PRR_DEV int64_t waveTake(WaveRange &r, uint32_t *counter, int64_t n, bool need, int64_t maxRun, int64_t waves,
+                         int64_t minRun = 64) {
     const uint64_t mask = __ballot(need);
     if (mask == 0) return -1;
     const int lane = (int) __lane_id();
@@ -712,7 +714,8 @@ PRR_DEV int64_t waveTake(WaveRange &r, uint32_t *counter, int64_t n, bool need, 
         r.lo = r.hi;
     } else {
         int64_t run = (n - r.seen) / (4 * waves);
-        run = run < 64 ? 64 : (run > maxRun ? maxRun : run);
+        run = run > maxRun ? maxRun : run;
+        run = run < minRun ? minRun : run;
         const int leader = __ffsll((unsigned long long) mask) - 1;
         uint32_t b = 0;
         if (lane == leader) b = atomicAdd(counter, (uint32_t) run);

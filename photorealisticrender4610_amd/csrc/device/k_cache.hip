@@ -335,6 +335,8 @@ __global__ __launch_bounds__(kAdamW * kAdamG) void kCacheAdam(float *__restrict_
         float gsum = 0.f;
         if (w < kCacheWeights) {
             const int b0 = (int) ((int64_t) g * nblocks / kAdamG), b1 = (int) ((int64_t) (g + 1) * nblocks / kAdamG);
+            // loads of 8 blocks in flight at once; the adds stay in block order
+#pragma unroll 8
             for (int b = b0; b < b1; b++) gsum += partial[(int64_t) b * kCacheWeights + w];
         }
         gpart[g][wl] = gsum;
@@ -365,11 +367,14 @@ __global__ __launch_bounds__(kAdamW * kAdamG) void kCacheAdam(float *__restrict_
     if (l > 0) wn[cacheOffN(l) + i * cacheStrideN(l) + o] = h;
 }
 
+// One wave: lane l sums blocks l, l + 64, ... in order, then a fixed
+// butterfly adds the lanes' sums (the same order on every run).
 __global__ void kCacheLossSum(const float *lossPart, int nblocks, float *out) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const int l = threadIdx.x;
     float s = 0.f;
-    for (int b = 0; b < nblocks; b++) s += lossPart[b];
-    *out = s;
+    for (int b = l; b < nblocks; b += 64) s += lossPart[b];
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+    if (l == 0) *out = s;
 }
 
 // ---------------------------------------------------------------------------

@@ -736,9 +736,12 @@ PRR_DEV bool occluded(const SceneView &s, const V3 &o, const V3 &d, float tmax, 
     return r == kAnyHit;
 }
 
+// Compiled for 3 waves per SIMD (at most 168 VGPRs): latency hiding for the
+// traversal loads matters more here than registers for shading.
 template <bool PHOTON, bool CACHE>
-__global__ __launch_bounds__(kTraceBlock) void kFinish(SceneView s, PathBufs pb, ChunkInfo ci, int bounce0,
-                                                       uint2 *spill, int stackDepth, int wwCap, int tree) {
+__global__ __launch_bounds__(kTraceBlock, 3) void kFinish(SceneView s, PathBufs pb, ChunkInfo ci, int bounce0,
+                                                       uint2 *spill, int stackDepth, int wwCap, int tree,
+                                                       int lanes) {
     extern __shared__ uint2 stk[];
     const int q = bounce0 & 1;
     const int64_t n = pb.qcnt[bounce0];
@@ -755,9 +758,13 @@ __global__ __launch_bounds__(kTraceBlock) void kFinish(SceneView s, PathBufs pb,
     WaveRange wr;
     const int64_t waves = (int64_t) gridDim.x * (kTraceBlock / 64);
     uint32_t nRays = 0, nShadow = 0, nTraced = 0;
-    while (true) {
+    // Only the first `lanes` lanes of a wave take paths: a wave waits at each
+    // bounce for its slowest path, so when few paths remain they are spread
+    // over more waves, fewer per wave.
+    const bool mine = (int) laneId() < lanes;
+    while (mine) {
         // every lane that reaches this point takes one entry (>= n: none left)
-        const int64_t i = waveTake(wr, fetch, n, true, 64, waves);
+        const int64_t i = waveTake(wr, fetch, n, true, lanes, waves, lanes);
         if (i >= n) break;
         const uint32_t slot = pb.qSlot[q][i];
         float4 o4 = pb.qOrg[q][i], d4 = pb.qDir[q][i];
@@ -961,25 +968,41 @@ hipError_t launchShade(const SceneView &s, const PathBufs &pb, const ChunkInfo &
               : shadeLaunch<false, false>(s, pb, ci, bounce, grid, waves, st);
     return hipGetLastError();
 }
+// Paths per wave: the remaining paths spread over the waves the GPU holds at
+// once (a power of two, 1..64).
+static int finishLanes(const void *kernel, size_t lds, int64_t paths, int numCU) {
+    int perCU = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, kernel, kTraceBlock, lds) != hipSuccess || perCU < 1)
+        perCU = 1;
+    const int64_t waves = (int64_t) perCU * (kTraceBlock / 64) * (numCU > 0 ? numCU : 1);
+    const int64_t want = (paths + waves - 1) / waves;
+    int lanes = 1;
+    while (lanes < 64 && lanes < want) lanes *= 2;
+    return lanes;
+}
 template <bool P, bool C>
 static void finishLaunch(const SceneView &s, const PathBufs &pb, const ChunkInfo &ci, int bounce0, uint2 *spill,
-                         int grid, int stackDepth, int wwCap, int tree, hipStream_t st) {
+                         int grid, int stackDepth, int wwCap, int tree, int64_t paths, int numCU, hipStream_t st) {
     // two stack areas (closest-hit and shadow rays)
-    hipLaunchKernelGGL((kFinish<P, C>), dim3(grid), dim3(kTraceBlock), 2 * stackDepth * kTraceBlock * sizeof(uint2), st,
-                       s, pb, ci, bounce0, spill, stackDepth, wwCap, tree);
+    const size_t lds = 2 * stackDepth * kTraceBlock * sizeof(uint2);
+    const int lanes = finishLanes(reinterpret_cast<const void *>(&kFinish<P, C>), lds, paths, numCU);
+    hipLaunchKernelGGL((kFinish<P, C>), dim3(grid), dim3(kTraceBlock), lds, st, s, pb, ci, bounce0, spill, stackDepth,
+                       wwCap, tree, lanes);
 }
 hipError_t launchFinish(const SceneView &s, const PathBufs &pb, const ChunkInfo &ci, int bounce0, uint2 *spill,
-                        int traceGrid, int stackDepth, int whileWhile, int tree, hipStream_t st) {
+                        int traceGrid, int stackDepth, int whileWhile, int tree, int64_t paths, int numCU,
+                        hipStream_t st) {
     const int wwCap = whileWhile > 1 ? whileWhile : 0;
     // two stack areas of 2 KB per LDS entry: at most 64 KB per block
     stackDepth = stackDepth < 16 ? stackDepth : 16;
     const bool photon = s.ph_count > 0, cache = ci.cache.query || ci.cache.train;
+#define PRR_FINISH(P, C) \
+    finishLaunch<P, C>(s, pb, ci, bounce0, spill, traceGrid, stackDepth, wwCap, tree, paths, numCU, st)
     if (photon)
-        cache ? finishLaunch<true, true>(s, pb, ci, bounce0, spill, traceGrid, stackDepth, wwCap, tree, st)
-              : finishLaunch<true, false>(s, pb, ci, bounce0, spill, traceGrid, stackDepth, wwCap, tree, st);
+        cache ? PRR_FINISH(true, true) : PRR_FINISH(true, false);
     else
-        cache ? finishLaunch<false, true>(s, pb, ci, bounce0, spill, traceGrid, stackDepth, wwCap, tree, st)
-              : finishLaunch<false, false>(s, pb, ci, bounce0, spill, traceGrid, stackDepth, wwCap, tree, st);
+        cache ? PRR_FINISH(false, true) : PRR_FINISH(false, false);
+#undef PRR_FINISH
     return hipGetLastError();
 }
 hipError_t launchAccumulate(const PathBufs &pb, const ChunkInfo &ci, float4 *fb, hipStream_t st) {

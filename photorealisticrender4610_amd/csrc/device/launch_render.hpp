@@ -21,10 +21,12 @@ hipError_t launchShadow(const SceneView &s, const PathBufs &pb, int bounce, uint
 // waves: minimum waves per SIMD the shading kernel is compiled for (1, 4 or 5)
 hipError_t launchShade(const SceneView &s, const PathBufs &pb, const ChunkInfo &ci, int bounce, int grid, int waves,
                        hipStream_t st);
-// path completion: every entry of bounce0's queue traced and shaded to the
-// end of its path in one launch (the shadow rays of bounce0 - 1 resolved)
+// path completion: every entry of bounce0's queue (`paths` of them) traced
+// and shaded to the end of its path in one launch (the shadow rays of
+// bounce0 - 1 resolved)
 hipError_t launchFinish(const SceneView &s, const PathBufs &pb, const ChunkInfo &ci, int bounce0, uint2 *spill,
-                        int traceGrid, int stackDepth, int whileWhile, int tree, hipStream_t st);
+                        int traceGrid, int stackDepth, int whileWhile, int tree, int64_t paths, int numCU,
+                        hipStream_t st);
 hipError_t launchAccumulate(const PathBufs &pb, const ChunkInfo &ci, float4 *fb, hipStream_t st);
 hipError_t launchAccumulateAdaptive(const PathBufs &pb, const ChunkInfo &ci, float4 *fb, const AdaptiveBufs &ab,
                                     int minSpp, float thr, hipStream_t st);
