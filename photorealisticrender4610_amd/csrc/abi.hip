@@ -151,17 +151,27 @@ int runBounces(prr_context *c, const SceneView &view, const ChunkInfo &ci, Timin
     } while (0)
     // Bounce b: extension rays of b and shadow rays of b - 1 (both written by
     // shade b - 1, independent of each other), then shade b, which reads both.
-    // Once bounce b's queue holds at most tailPaths entries, the shadow rays
-    // of b - 1 are traced and one kFinish launch completes every remaining
-    // path. The count is read back after shade b - 1 (its shadow rays of
-    // b - 1 already running on the aux stream while the host waits).
+    // With path completion on, kFinish is enqueued at every bounce too: the
+    // first bounce whose queue holds at most tailPaths entries runs there and
+    // kExtend / kShade of it and of every later bounce do nothing. Each
+    // bounce's count is copied to the host behind an event. Before enqueuing
+    // bounce b the host waits for the count of b - 1 (final once shade b - 2
+    // ended, while bounce b - 1's kernels still occupy the GPU) and stops
+    // once it shows the switch, so no launch of a later bounce is wasted and
+    // the GPU never waits for the host.
     const bool conc = c->concurrentShadow && c->aux;
     const int wwShadow = c->shadowWide ? c->whileWhileAny : c->whileWhile;
-    const bool tail = c->tailPaths > 0 && c->cntHost && c->evCnt;
+    const bool tail = c->tailPaths > 0 && c->cntHost && ci.maxDepth + 1 < prr_context::kMaxBounces;
+    ChunkInfo cit = ci;
+    cit.tailPaths = tail ? c->tailPaths : 0;
     for (int b = 0; b <= ci.maxDepth; b++) {
+        if (tail && b > 0) {
+            HIPCK(c, hipEventSynchronize(c->evCnt[b - 1]));
+            if ((int64_t) c->cntHost[b - 1] <= c->tailPaths) break;
+        }
         if (tail) {
-            HIPCK(c, hipMemcpyAsync(c->cntHost, c->pb.qcnt + b, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
-            HIPCK(c, hipEventRecord(c->evCnt, st));
+            HIPCK(c, hipMemcpyAsync(c->cntHost + b, c->pb.qcnt + b, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+            HIPCK(c, hipEventRecord(c->evCnt[b], st));
         }
         if (b > 0 && conc) {
             HIPCK(c, hipEventRecord(c->evFork, st));
@@ -171,30 +181,19 @@ int runBounces(prr_context *c, const SceneView &view, const ChunkInfo &ci, Timin
                                c->fetchRun, c->shadowWide, c->aux));
             HIPCK(c, hipEventRecord(c->evJoin, c->aux));
         }
-        if (tail) {
-            HIPCK(c, hipEventSynchronize(c->evCnt));
-            const int64_t left = (int64_t) *c->cntHost;
-            if (left <= c->tailPaths) {
-                if (b > 0 && !conc)
-                    TIMED(1, st,
-                          launchShadow(view, c->pb, b - 1, c->spill, c->traceBlocks, c->refillAt, c->stackDepth,
-                                       wwShadow, c->fetchRun, c->shadowWide, st));
-                if (b > 0 && conc) HIPCK(c, hipStreamWaitEvent(st, c->evJoin, 0));
-                TIMED(3, st,
-                      launchFinish(view, c->pb, ci, b, c->spill, c->traceBlocks, c->stackDepth, c->whileWhile,
-                                   c->shadowWide, left, c->numCU, st));
-                break;
-            }
-        }
         TIMED(0, st,
               launchExtend(view, c->pb, b, c->spill, c->traceBlocks, c->refillAt, c->stackDepth, c->whileWhile,
-                           c->fetchRun, st));
+                           c->fetchRun, cit.tailPaths, st));
         if (b > 0 && !conc)
             TIMED(1, st,
                   launchShadow(view, c->pb, b - 1, c->spill, c->traceBlocks, c->refillAt, c->stackDepth, wwShadow,
                                c->fetchRun, c->shadowWide, st));
         if (b > 0 && conc) HIPCK(c, hipStreamWaitEvent(st, c->evJoin, 0));
-        TIMED(2, st, launchShade(view, c->pb, ci, b, c->shadeBlocks, c->shadeWaves, st));
+        if (tail)
+            TIMED(3, st,
+                  launchFinish(view, c->pb, cit, b, c->spill, c->traceBlocks, c->stackDepth, c->whileWhile,
+                               c->shadowWide, c->numCU, st));
+        TIMED(2, st, launchShade(view, c->pb, cit, b, c->shadeBlocks, c->shadeWaves, st));
     }
 #undef TIMED
     return PRR_OK;
@@ -292,8 +291,10 @@ int prr_create(int device, prr_context **out) {
     if (e == hipSuccess) e = hipEventCreate(&c->ev1);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&c->evFork, hipEventDisableTiming);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&c->evJoin, hipEventDisableTiming);
-    if (e == hipSuccess) e = hipEventCreateWithFlags(&c->evCnt, hipEventDisableTiming);
-    if (e == hipSuccess) e = hipHostMalloc((void **) &c->cntHost, 64, hipHostMallocDefault);
+    for (int b = 0; b < prr_context::kMaxBounces && e == hipSuccess; b++)
+        e = hipEventCreateWithFlags(&c->evCnt[b], hipEventDisableTiming);
+    if (e == hipSuccess)
+        e = hipHostMalloc((void **) &c->cntHost, prr_context::kMaxBounces * sizeof(uint32_t), hipHostMallocDefault);
     if (e != hipSuccess) {
         std::string m = std::string("device init: ") + hipGetErrorString(e);
         delete c;
@@ -360,7 +361,8 @@ int prr_destroy(prr_context *c) {
     for (hipEvent_t e : c->evPool) (void) hipEventDestroy(e);
     if (c->evFork) (void) hipEventDestroy(c->evFork);
     if (c->evJoin) (void) hipEventDestroy(c->evJoin);
-    if (c->evCnt) (void) hipEventDestroy(c->evCnt);
+    for (hipEvent_t e : c->evCnt)
+        if (e) (void) hipEventDestroy(e);
     if (c->cntHost) (void) hipHostFree(c->cntHost);
     if (c->own) (void) hipStreamDestroy(c->own);
     if (c->aux) (void) hipStreamDestroy(c->aux);

@@ -39,11 +39,14 @@ struct prr_context {
     uint2 *spillAux = nullptr;
     // path completion: once a bounce's queue holds at most tailPaths entries
     // (0: never), one kFinish launch runs those paths to their end instead of
-    // three launches per remaining bounce. The host reads each bounce's count
-    // into cntHost (pinned) and waits on evCnt before choosing.
+    // three launches per remaining bounce. The kernels choose from the queue
+    // counts on the device; the host copies each bounce's count into cntHost
+    // (pinned, kMaxBounces entries) behind event evCnt[b] and stops
+    // enqueuing bounces once a copied count shows the switch.
+    static constexpr int kMaxBounces = 66;
     int64_t tailPaths = 1 << 18;
     uint32_t *cntHost = nullptr;
-    hipEvent_t evCnt = nullptr;
+    hipEvent_t evCnt[kMaxBounces] = {};
     // render buffers
     prr::PathBufs pb{};
     int64_t pbSlots = 0;

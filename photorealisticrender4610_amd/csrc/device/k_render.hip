@@ -202,10 +202,11 @@ __global__ void kCopyResults(PathBufs pb, int64_t n, int64_t first, float *L) {
 
 template <bool WW>
 __global__ __launch_bounds__(kTraceBlock) void kExtend(SceneView s, PathBufs pb, int bounce, uint2 *spill, int refillAt,
-                                                       int stackDepth, int wwCap, int fetchRun) {
+                                                       int stackDepth, int wwCap, int fetchRun, int64_t tailPaths) {
     extern __shared__ uint2 stk[];
     const int q = bounce & 1;
     const int64_t n = pb.qcnt[bounce];
+    if (n <= tailPaths) return;  // kFinish runs this bounce's paths (tailPaths > 0) or there are none
     uint32_t *fetch = &pb.fetch[2 * bounce];
     const int64_t lane = (int64_t) blockIdx.x * kTraceBlock + threadIdx.x;
     LaneStack st;
@@ -637,6 +638,7 @@ template <bool PHOTON, bool CACHE>
 __device__ __forceinline__ void shadeImpl(const SceneView &s, const PathBufs &pb, const ChunkInfo &ci, int bounce) {
     const int q = bounce & 1;
     const int64_t n = pb.qcnt[bounce];
+    if (n <= ci.tailPaths) return;  // kFinish runs this bounce's paths (tailPaths > 0) or there are none
     const int64_t stride = (int64_t) gridDim.x * kShadeBlock;
     const int64_t i0 = (int64_t) blockIdx.x * kShadeBlock + threadIdx.x;
     // uniform trip count per wave so every lane reaches the appends
@@ -741,10 +743,12 @@ PRR_DEV bool occluded(const SceneView &s, const V3 &o, const V3 &d, float tmax, 
 template <bool PHOTON, bool CACHE>
 __global__ __launch_bounds__(kTraceBlock, 3) void kFinish(SceneView s, PathBufs pb, ChunkInfo ci, int bounce0,
                                                        uint2 *spill, int stackDepth, int wwCap, int tree,
-                                                       int lanes) {
+                                                       int64_t residentWaves) {
     extern __shared__ uint2 stk[];
     const int q = bounce0 & 1;
     const int64_t n = pb.qcnt[bounce0];
+    // runs the first bounce whose queue holds at most tailPaths entries
+    if (n > ci.tailPaths || (bounce0 > 0 && pb.qcnt[bounce0 - 1] <= ci.tailPaths)) return;
     uint32_t *fetch = &pb.fetch[2 * bounce0];
     const int64_t lane = (int64_t) blockIdx.x * kTraceBlock + threadIdx.x;
     // Two LDS stack areas: the closest-hit stack (uint2 entries) and the
@@ -760,7 +764,10 @@ __global__ __launch_bounds__(kTraceBlock, 3) void kFinish(SceneView s, PathBufs 
     uint32_t nRays = 0, nShadow = 0, nTraced = 0;
     // Only the first `lanes` lanes of a wave take paths: a wave waits at each
     // bounce for its slowest path, so when few paths remain they are spread
-    // over more waves, fewer per wave.
+    // over the waves the GPU holds at once, fewer per wave (a power of two,
+    // 1..64).
+    int lanes = 1;
+    while (lanes < 64 && (int64_t) lanes * residentWaves < n) lanes *= 2;
     const bool mine = (int) laneId() < lanes;
     while (mine) {
         // every lane that reaches this point takes one entry (>= n: none left)
@@ -914,14 +921,14 @@ hipError_t launchCopyResults(const PathBufs &pb, int64_t n, int64_t first, float
     return hipGetLastError();
 }
 hipError_t launchExtend(const SceneView &s, const PathBufs &pb, int bounce, uint2 *spill, int traceGrid,
-                        int refillAt, int stackDepth, int whileWhile, int fetchRun, hipStream_t st) {
+                        int refillAt, int stackDepth, int whileWhile, int fetchRun, int64_t tailPaths, hipStream_t st) {
     const int wwCap = whileWhile > 1 ? whileWhile : 0;
     if (whileWhile)
         hipLaunchKernelGGL(kExtend<true>, dim3(traceGrid), dim3(kTraceBlock), stackDepth * kTraceBlock * sizeof(uint2),
-                           st, s, pb, bounce, spill, refillAt, stackDepth, wwCap, fetchRun);
+                           st, s, pb, bounce, spill, refillAt, stackDepth, wwCap, fetchRun, tailPaths);
     else
         hipLaunchKernelGGL(kExtend<false>, dim3(traceGrid), dim3(kTraceBlock), stackDepth * kTraceBlock * sizeof(uint2),
-                           st, s, pb, bounce, spill, refillAt, stackDepth, wwCap, fetchRun);
+                           st, s, pb, bounce, spill, refillAt, stackDepth, wwCap, fetchRun, tailPaths);
     return hipGetLastError();
 }
 hipError_t launchShadow(const SceneView &s, const PathBufs &pb, int bounce, uint2 *spill, int traceGrid,
@@ -968,36 +975,32 @@ hipError_t launchShade(const SceneView &s, const PathBufs &pb, const ChunkInfo &
               : shadeLaunch<false, false>(s, pb, ci, bounce, grid, waves, st);
     return hipGetLastError();
 }
-// Paths per wave: the remaining paths spread over the waves the GPU holds at
-// once (a power of two, 1..64).
-static int finishLanes(const void *kernel, size_t lds, int64_t paths, int numCU) {
+// Waves of kFinish the GPU holds at once.
+static int64_t finishResidentWaves(const void *kernel, size_t lds, int numCU) {
     int perCU = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, kernel, kTraceBlock, lds) != hipSuccess || perCU < 1)
         perCU = 1;
-    const int64_t waves = (int64_t) perCU * (kTraceBlock / 64) * (numCU > 0 ? numCU : 1);
-    const int64_t want = (paths + waves - 1) / waves;
-    int lanes = 1;
-    while (lanes < 64 && lanes < want) lanes *= 2;
-    return lanes;
+    return (int64_t) perCU * (kTraceBlock / 64) * (numCU > 0 ? numCU : 1);
 }
 template <bool P, bool C>
 static void finishLaunch(const SceneView &s, const PathBufs &pb, const ChunkInfo &ci, int bounce0, uint2 *spill,
-                         int grid, int stackDepth, int wwCap, int tree, int64_t paths, int numCU, hipStream_t st) {
+                         int grid, int stackDepth, int wwCap, int tree, int numCU, hipStream_t st) {
     // two stack areas (closest-hit and shadow rays)
     const size_t lds = 2 * stackDepth * kTraceBlock * sizeof(uint2);
-    const int lanes = finishLanes(reinterpret_cast<const void *>(&kFinish<P, C>), lds, paths, numCU);
+    const int64_t resident = finishResidentWaves(reinterpret_cast<const void *>(&kFinish<P, C>), lds, numCU);
+    // persistent: the blocks the GPU holds at once (fewer to start when the launch has nothing to do)
+    const int64_t blocks = resident / (kTraceBlock / 64);
+    grid = blocks < grid ? (int) blocks : grid;
     hipLaunchKernelGGL((kFinish<P, C>), dim3(grid), dim3(kTraceBlock), lds, st, s, pb, ci, bounce0, spill, stackDepth,
-                       wwCap, tree, lanes);
+                       wwCap, tree, resident);
 }
 hipError_t launchFinish(const SceneView &s, const PathBufs &pb, const ChunkInfo &ci, int bounce0, uint2 *spill,
-                        int traceGrid, int stackDepth, int whileWhile, int tree, int64_t paths, int numCU,
-                        hipStream_t st) {
+                        int traceGrid, int stackDepth, int whileWhile, int tree, int numCU, hipStream_t st) {
     const int wwCap = whileWhile > 1 ? whileWhile : 0;
     // two stack areas of 2 KB per LDS entry: at most 64 KB per block
     stackDepth = stackDepth < 16 ? stackDepth : 16;
     const bool photon = s.ph_count > 0, cache = ci.cache.query || ci.cache.train;
-#define PRR_FINISH(P, C) \
-    finishLaunch<P, C>(s, pb, ci, bounce0, spill, traceGrid, stackDepth, wwCap, tree, paths, numCU, st)
+#define PRR_FINISH(P, C) finishLaunch<P, C>(s, pb, ci, bounce0, spill, traceGrid, stackDepth, wwCap, tree, numCU, st)
     if (photon)
         cache ? PRR_FINISH(true, true) : PRR_FINISH(true, false);
     else

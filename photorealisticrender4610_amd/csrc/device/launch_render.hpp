@@ -13,20 +13,22 @@ hipError_t launchRaygenRays(const PathBufs &pb, int64_t n, const float *org, con
 hipError_t launchCopyResults(const PathBufs &pb, int64_t n, int64_t first, float *L, hipStream_t st);
 // whileWhile: 0 one node per work check; 1 interior nodes then a leaf; n > 1
 // the same with at most n interior steps per work check
+// tailPaths: the launch does nothing when the bounce's queue holds at most
+// that many entries (kFinish takes them)
 hipError_t launchExtend(const SceneView &s, const PathBufs &pb, int bounce, uint2 *spill, int traceGrid,
-                        int refillAt, int stackDepth, int whileWhile, int fetchRun, hipStream_t st);
+                        int refillAt, int stackDepth, int whileWhile, int fetchRun, int64_t tailPaths, hipStream_t st);
 // tree: 0 the binary tree, 1 the four-wide occlusion tree, 2 its quantised copy
 hipError_t launchShadow(const SceneView &s, const PathBufs &pb, int bounce, uint2 *spill, int traceGrid,
                         int refillAt, int stackDepth, int whileWhile, int fetchRun, int tree, hipStream_t st);
 // waves: minimum waves per SIMD the shading kernel is compiled for (1, 4 or 5)
 hipError_t launchShade(const SceneView &s, const PathBufs &pb, const ChunkInfo &ci, int bounce, int grid, int waves,
                        hipStream_t st);
-// path completion: every entry of bounce0's queue (`paths` of them) traced
-// and shaded to the end of its path in one launch (the shadow rays of
-// bounce0 - 1 resolved)
+// path completion: when bounce0 is the first bounce whose queue holds at
+// most ci.tailPaths entries, every entry traced and shaded to the end of its
+// path in one launch (the shadow rays of bounce0 - 1 resolved); otherwise
+// the launch does nothing
 hipError_t launchFinish(const SceneView &s, const PathBufs &pb, const ChunkInfo &ci, int bounce0, uint2 *spill,
-                        int traceGrid, int stackDepth, int whileWhile, int tree, int64_t paths, int numCU,
-                        hipStream_t st);
+                        int traceGrid, int stackDepth, int whileWhile, int tree, int numCU, hipStream_t st);
 hipError_t launchAccumulate(const PathBufs &pb, const ChunkInfo &ci, float4 *fb, hipStream_t st);
 hipError_t launchAccumulateAdaptive(const PathBufs &pb, const ChunkInfo &ci, float4 *fb, const AdaptiveBufs &ab,
                                     int minSpp, float thr, hipStream_t st);

@@ -92,6 +92,10 @@ struct ChunkInfo {
     CacheRec cache;
     // per-slot sample index (serving batches); null: sampleBase + slot / pixels
     const uint32_t *sampleOf = nullptr;
+    // Path completion: the first bounce whose queue holds at most tailPaths
+    // entries (0: never) is run by kFinish; kExtend and kShade skip it and
+    // every later bounce.
+    int64_t tailPaths = 0;
 };
 
 // Per-pixel Welford state of the adaptive sampler (Renderer.hpp:14-40):
