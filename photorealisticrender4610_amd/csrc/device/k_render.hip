@@ -379,15 +379,33 @@ struct ShadeOut {
     float tA = 0.f, tP = 0.f;
 };
 
+// Where shading finds the ray a path traced and its closest hit (t, u, v,
+// prim bits; prim -1 for a miss or a terminating entry): the queue entry and
+// the hit record of the wavefront kernels, read where shading needs them, or
+// values a kernel already holds.
+struct QueueRay {
+    const PathBufs &pb;
+    int q;
+    int64_t i;
+    PRR_DEV float4 hit() const { return pb.hit[i]; }
+    PRR_DEV V3 org() const { return ld3(pb.qOrg[q][i]); }
+    PRR_DEV V3 dir() const { return ld3(pb.qDir[q][i]); }
+};
+struct HeldRay {
+    float4 h, o4, d4;
+    PRR_DEV float4 hit() const { return h; }
+    PRR_DEV V3 org() const { return ld3(o4); }
+    PRR_DEV V3 dir() const { return ld3(d4); }
+};
+
 // Shading of the path in `slot` at `bounce` (Integrator.cpp:104-263, one
-// loop iteration): h is the closest hit of the ray (o, d) the path traced
-// (t, u, v, prim bits; prim -1 for a miss or a terminating entry).
+// loop iteration) for the ray and hit `ray` gives.
 // PHOTON / CACHE: whether the caustic photon gather and the radiance-cache
 // query / training code are compiled in (the kernel's registers are those of
 // its heaviest branch, so renders without them use a leaner variant).
-template <bool PHOTON, bool CACHE>
+template <bool PHOTON, bool CACHE, class Ray>
 __device__ __forceinline__ void shadePath(const SceneView &s, const PathBufs &pb, const ChunkInfo &ci, int bounce,
-                                          uint32_t slot, const float4 h, const V3 o, const V3 d, ShadeOut &so) {
+                                          uint32_t slot, const Ray &ray, ShadeOut &so) {
     bool done = false;
     uint32_t fl = pb.flags[slot];
     V3 L = ld3(pb.L[slot]);
@@ -396,17 +414,26 @@ __device__ __forceinline__ void shadePath(const SceneView &s, const PathBufs &pb
     Pcg rng;
     rng.state = pb.rng[slot];
     rng.inc = pcgInc((uint32_t) sIdx);
-    // 1. NEE of the previous bounce, in reference order
-    if (fl & F_PEND_A)
-        if (pb.vis[slot]) L += ld3(pb.pend[slot]);
-    if (fl & F_PEND_E)
-        if (pb.vis[pb.slots + slot]) L += ld3(pb.pend[pb.slots + slot]);
-    if (fl & F_PEND_P)
-        if (pb.vis[2 * pb.slots + slot]) L += ld3(pb.pend[2 * pb.slots + slot]);
+    // 1. NEE of the previous bounce, in reference order (a term and its
+    // visibility are read together: both wait only for the flags)
+    if (fl & F_PEND_A) {
+        const float4 c = pb.pend[slot];
+        if (pb.vis[slot]) L += ld3(c);
+    }
+    if (fl & F_PEND_E) {
+        const float4 c = pb.pend[pb.slots + slot];
+        if (pb.vis[pb.slots + slot]) L += ld3(c);
+    }
+    if (fl & F_PEND_P) {
+        const float4 c = pb.pend[2 * pb.slots + slot];
+        if (pb.vis[2 * pb.slots + slot]) L += ld3(c);
+    }
     if (fl & F_PEND_C) L += ld3(pb.pend[3 * pb.slots + slot]);
     fl &= ~F_PEND_ALL;
     done = (fl & F_TERMINATING) != 0u;
+    const float4 h = ray.hit();
     const int prim = __float_as_int(h.w);
+    const V3 o = ray.org(), d = ray.dir();
     // 2. the BSDF-sampled ray of the previous (diffuse) bounce
     if (!done && (fl & F_PREV_DIFF)) {
         const bool ind = bounce - 1 > 0;
@@ -637,8 +664,9 @@ __device__ __forceinline__ void shadePath(const SceneView &s, const PathBufs &pb
 template <bool PHOTON, bool CACHE>
 __device__ __forceinline__ void shadeImpl(const SceneView &s, const PathBufs &pb, const ChunkInfo &ci, int bounce) {
     const int q = bounce & 1;
-    const int64_t n = pb.qcnt[bounce];
-    if (n <= ci.tailPaths) return;  // kFinish runs this bounce's paths (tailPaths > 0) or there are none
+    const int64_t cnt = pb.qcnt[bounce];
+    // at most tailPaths entries: kFinish runs this bounce's paths (an empty loop here)
+    const int64_t n = cnt > ci.tailPaths ? cnt : 0;
     const int64_t stride = (int64_t) gridDim.x * kShadeBlock;
     const int64_t i0 = (int64_t) blockIdx.x * kShadeBlock + threadIdx.x;
     // uniform trip count per wave so every lane reaches the appends
@@ -650,7 +678,7 @@ __device__ __forceinline__ void shadeImpl(const SceneView &s, const PathBufs &pb
         ShadeOut so;
         if (i < n) {
             slot = pb.qSlot[q][i];
-            shadePath<PHOTON, CACHE>(s, pb, ci, bounce, slot, pb.hit[i], ld3(pb.qOrg[q][i]), ld3(pb.qDir[q][i]), so);
+            shadePath<PHOTON, CACHE>(s, pb, ci, bounce, slot, QueueRay{pb, q, i}, so);
         }
         const bool emitRay = so.emitRay, termEntry = so.termEntry, emitA = so.emitA, emitE = so.emitE,
                    emitP = so.emitP;
@@ -786,7 +814,7 @@ __global__ __launch_bounds__(kTraceBlock, 3) void kFinish(SceneView s, PathBufs 
                 nTraced++;
             }
             ShadeOut so;
-            shadePath<PHOTON, CACHE>(s, pb, ci, b, slot, h, ld3(o4), ld3(d4), so);
+            shadePath<PHOTON, CACHE>(s, pb, ci, b, slot, HeldRay{h, o4, d4}, so);
             if (so.emitA) pb.vis[slot] = occluded(s, so.sOA, so.sDA, so.tA, tree, wwCap, lst, rst) ? 0 : 1;
             if (so.emitE)
                 pb.vis[pb.slots + slot] =
