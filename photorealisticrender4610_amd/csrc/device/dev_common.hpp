@@ -700,16 +700,25 @@ PRR_DEV V3 photonRadiance(const SceneView &s, const V3 &x, const V3 &wo, const F
         }
         if (dup) continue;
         const uint32_t b = s.ph_start[h], e = s.ph_end[h];
-        for (uint32_t i = b; i < e; i++) {
-            const V3 p = ld3(s.ph_pos[i]);
-            if (norm(p - x) > r) continue;
-            const V3 wi = -ld3(s.ph_dir[i]);
-            const float c = dot(wi, fr.N);
-            if (c <= 1e-4f) continue;
-            V3 f;
-            float pdf;
-            evalPdf(lb, fr, wi, wo, f, pdf);
-            sum += f * ld3(s.ph_pow[i]) / c;
+        // positions four at a time (most photons of a cell lie outside r, so
+        // the scan is a chain of position loads: four are in flight at once);
+        // the photons within r are added in index order
+        for (uint32_t i0 = b; i0 < e; i0 += 4) {
+            V3 p[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) p[u] = i0 + u < e ? ld3(s.ph_pos[i0 + u]) : x + V3(4.f * r);
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                if (norm(p[u] - x) > r) continue;
+                const uint32_t i = i0 + u;
+                const V3 wi = -ld3(s.ph_dir[i]);
+                const float c = dot(wi, fr.N);
+                if (c <= 1e-4f) continue;
+                V3 f;
+                float pdf;
+                evalPdf(lb, fr, wi, wo, f, pdf);
+                sum += f * ld3(s.ph_pow[i]) / c;
+            }
         }
     }
     return sum / (kPi * r * r);
