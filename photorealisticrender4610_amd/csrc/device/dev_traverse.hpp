@@ -351,14 +351,21 @@ PRR_DEV bool closestLeaf(const SceneView &s, TravState &ts, LaneStack &st) {
             }
         }
     } else {
-        for (uint32_t i = 0; i < cnt; i++) {
-            const TriRec tr = loadTri(s.tris, first + i);
-            float t, u, v;
-            if (triHit(tr, ts.o, ts.d, t, u, v) && t < ts.h.t) {
-                ts.h.t = t;
-                ts.h.u = u;
-                ts.h.v = v;
-                ts.h.prim = tr.id;
+        // four triangles' loads in flight at once, tested in leaf order
+        for (uint32_t i = 0; i < cnt; i += 4) {
+            TriRec tr[4];
+#pragma unroll
+            for (int k = 0; k < 4; k++)
+                if (i + k < cnt) tr[k] = loadTri(s.tris, first + i + k);
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                float t, u, v;
+                if (i + k < cnt && triHit(tr[k], ts.o, ts.d, t, u, v) && t < ts.h.t) {
+                    ts.h.t = t;
+                    ts.h.u = u;
+                    ts.h.v = v;
+                    ts.h.prim = tr[k].id;
+                }
             }
         }
     }
@@ -480,8 +487,17 @@ PRR_DEV int anyLeaf(const SceneView &s, TravState &ts, Stack &st) {
             if (sphHit(s.sph[first + i], ts.o, ts.d, t) && t < ts.tmax) return kAnyHit;
         }
     } else {
-        for (uint32_t i = 0; i < cnt; i++)
-            if (triAny(loadTri(s.tris, first + i), ts.o, ts.d, ts.tmax)) return kAnyHit;
+        // four triangles' loads in flight at once
+        for (uint32_t i = 0; i < cnt; i += 4) {
+            TriRec tr[4];
+#pragma unroll
+            for (int k = 0; k < 4; k++)
+                if (i + k < cnt) tr[k] = loadTri(s.tris, first + i + k);
+            bool hit = false;
+#pragma unroll
+            for (int k = 0; k < 4; k++) hit = hit || (i + k < cnt && triAny(tr[k], ts.o, ts.d, ts.tmax));
+            if (hit) return kAnyHit;
+        }
     }
     return anyPop(ts, st);
 }
