@@ -303,7 +303,7 @@ int prr_create(int device, prr_context **out) {
     c->stream = c->own;
     c->numCU = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 1;
     // persistent traversal grid: 8 blocks of 256 lanes per CU (with 8-entry
-    // LDS stacks, 16 KB per block), a wave refills once 40 of its lanes are
+    // LDS stacks, 16 KB per block), a wave refills once 24 of its lanes are
     // idle, at most 8 interior steps (4 for shadow rays) per work check
     // (measured on the 1M-tri scene). PRR_TRACE_BLOCKS_PER_CU, PRR_STACK_LDS,
     // PRR_REFILL_AT, PRR_WHILE_WHILE(_ANY), PRR_FETCH_RUN, PRR_SHADOW_WIDE,

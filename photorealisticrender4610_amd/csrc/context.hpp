@@ -19,7 +19,7 @@ struct prr_context {
     std::string err;
     int numCU = 1;
     int traceBlocks = 1, shadeBlocks = 1;
-    int refillAt = 40;
+    int refillAt = 24;
     int stackDepth = 8;
     int whileWhile = 8;     // traversal iteration: 0 one node per work check, 1 interior nodes then a leaf,
                             // n > 1 the same with at most n interior steps per check
