@@ -136,19 +136,22 @@ constexpr uint32_t kNoChild = 0x7FFFFFFFu;
 // fma(q, scale, origin) (exact: q * scale is a power-of-two multiple). The
 // builder checks that every decoded box contains the child's box, so a ray
 // that enters a child's box enters its decoded box (the slab test is monotone
-// in the bounds). Leaf children refer to a DLeafQ (kLeafBit | index) whose
-// exact box is tested before the leaf's primitives.
+// in the bounds). A leaf child refers to its leaf record (kLeafBit, the
+// leaf's primitive count and sphere bit as in the binary tree, the record's
+// float4 offset as `first`), whose exact box is tested before the leaf's
+// primitives.
 //   w[0..2] origin xyz, w[3..5] scale xyz,
 //   w[6..11] lo x, hi x, lo y, hi y, lo z, hi z (child k in byte k),
 //   w[12..15] child refs (kNoChild: empty; decoded box inverted)
 struct __align__(16) DNode4Q {
     uint4 w0, w1, w2, w3;
 };
-// Leaf of the quantised tree: exact box [minX, maxX, minY, maxY], [minZ,
-// maxZ, leaf ref bits, 0].
-struct __align__(16) DLeafQ {
-    float4 a, b;
-};
+// Leaf record of the quantised tree, in float4s: the exact box [minX, maxX,
+// minY, maxY], [minZ, maxZ, binary-tree leaf ref bits, 0], then for a
+// triangle leaf a copy of its triangles (three float4 each, as in the
+// triangle array), so the box and the triangles are read together from one
+// place; a sphere leaf reaches its spheres through the leaf ref.
+constexpr int kLeafQHead = 2;
 
 enum MatType { MAT_DIFFUSE = 0, MAT_GLASS = 1, MAT_MIRROR = 2, MAT_EMISSIVE = 3 };
 
@@ -188,7 +191,7 @@ struct SceneView {
     const DNode *nodes;
     const DNode4 *nodes4;     // occlusion tree (four-wide)
     const DNode4Q *nodes4q;   // the same tree quantised (one line per node)
-    const DLeafQ *leavesq;    // its leaves with exact boxes
+    const float4 *leavesq;    // its leaf records (exact box + triangles)
     const float4 *tris;       // 3 float4 per leaf slot: v0.xyz e1.x | e1.yz e2.xy | e2.z id pad pad
     const float4 *sph;        // leaf-order spheres: center.xyz radius
     const int *sph_id;        // leaf-order sphere -> global prim id

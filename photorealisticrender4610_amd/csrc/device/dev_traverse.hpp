@@ -664,13 +664,31 @@ PRR_DEV int anyInterior4Q(const SceneView &s, TravState &ts, RefStack &st) {
 }
 
 // A leaf of the quantised tree: its exact box first (the binary tree's test
-// of that leaf, BVH.cpp:445-459), then its primitives.
+// of that leaf, BVH.cpp:445-459), then its primitives. A triangle leaf's
+// record holds its box and its triangles, read together (up to four
+// triangles at once) and tested in leaf order.
 PRR_DEV int anyLeafQ(const SceneView &s, TravState &ts, RefStack &st) {
-    const DLeafQ lf = s.leavesq[ts.cur & ~kLeafBit];
-    const float tb = slab(ts.r, lf.a.x, lf.a.y, lf.a.z, lf.a.w, lf.b.x, lf.b.y);
-    if (!(tb < ts.tmax)) return anyPop(ts, st);
-    ts.cur = __float_as_uint(lf.b.z);
-    return anyLeaf(s, ts, st);
+    const float4 *rec = s.leavesq + refFirst(ts.cur);
+    const uint32_t cnt = refCount(ts.cur);
+    const float4 a = rec[0], b = rec[1];
+    if (refSphere(ts.cur)) {
+        if (!(slab(ts.r, a.x, a.y, a.z, a.w, b.x, b.y) < ts.tmax)) return anyPop(ts, st);
+        ts.cur = __float_as_uint(b.z);
+        return anyLeaf(s, ts, st);
+    }
+    const float4 *tris = rec + kLeafQHead;
+    for (uint32_t i = 0; i < cnt; i += 4) {
+        TriRec tr[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++)
+            if (i + k < cnt) tr[k] = loadTri(tris, i + k);
+        if (i == 0 && !(slab(ts.r, a.x, a.y, a.z, a.w, b.x, b.y) < ts.tmax)) return anyPop(ts, st);
+        bool hit = false;
+#pragma unroll
+        for (int k = 0; k < 4; k++) hit = hit || (i + k < cnt && triAny(tr[k], ts.o, ts.d, ts.tmax));
+        if (hit) return kAnyHit;
+    }
+    return anyPop(ts, st);
 }
 
 PRR_DEV int anyStep4Q(const SceneView &s, TravState &ts, RefStack &st) {

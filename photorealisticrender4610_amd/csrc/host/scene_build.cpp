@@ -410,7 +410,7 @@ struct Widener {
     }
 };
 
-// Quantised copy of the occlusion tree (DNode4Q, DLeafQ). Per node and axis
+// Quantised copy of the occlusion tree (DNode4Q, leaf records). Per node and axis
 // the origin is the children's minimum and the scale a power of two with
 // extent / scale <= 255; each child's bytes are moved outwards until the
 // decoded bound fma(q, scale, origin), evaluated in float as on the device,
@@ -436,12 +436,22 @@ struct Quantiser {
         }
         return true;
     }
-    uint32_t leaf(const float *b, uint32_t ref) {
-        DLeafQ l;
-        l.a = make_float4(b[0], b[1], b[2], b[3]);
-        l.b = make_float4(b[4], b[5], __builtin_bit_cast(float, ref), 0.f);
-        S.leavesq.push_back(l);
-        return kLeafBit | (uint32_t) (S.leavesq.size() - 1);
+    // leaf record: exact box, the binary tree's leaf ref, then a copy of a
+    // triangle leaf's triangles; the child ref keeps the leaf's count and
+    // sphere bit and points at the record
+    bool leaf(const float *b, uint32_t ref, uint32_t &out, std::string &err) {
+        const size_t off = S.leavesq.size();
+        if (off > 0x07FFFFFFu) {
+            err = "occlusion tree: leaf records exceed 2^27 float4s";
+            return false;
+        }
+        S.leavesq.push_back(make_float4(b[0], b[1], b[2], b[3]));
+        S.leavesq.push_back(make_float4(b[4], b[5], __builtin_bit_cast(float, ref), 0.f));
+        if (!refSphere(ref))
+            for (uint32_t i = 0; i < refCount(ref); i++)
+                for (int j = 0; j < 3; j++) S.leavesq.push_back(S.tris[3 * (size_t) (refFirst(ref) + i) + j]);
+        out = makeLeafRef((uint32_t) off, refCount(ref), refSphere(ref));
+        return true;
     }
     bool run(std::string &err) {
         S.nodes4q.assign(S.nodes4.size(), DNode4Q{});
@@ -485,7 +495,8 @@ struct Quantiser {
             uint32_t refs[4] = {kNoChild, kNoChild, kNoChild, kNoChild};
             for (int k = 0; k < n; k++) {
                 const float b[6] = {lo3[0][k], hi3[0][k], lo3[1][k], hi3[1][k], lo3[2][k], hi3[2][k]};
-                refs[k] = refIsLeaf(rf[k]) ? leaf(b, rf[k]) : rf[k];
+                refs[k] = rf[k];
+                if (refIsLeaf(rf[k]) && !leaf(b, rf[k], refs[k], err)) return false;
             }
             DNode4Q &q = S.nodes4q[i];
             q.w0 = make_uint4(__builtin_bit_cast(uint32_t, org[0]), __builtin_bit_cast(uint32_t, org[1]),
@@ -494,7 +505,8 @@ struct Quantiser {
             q.w2 = make_uint4(wq[2], wq[3], wq[4], wq[5]);
             q.w3 = make_uint4(refs[0], refs[1], refs[2], refs[3]);
         }
-        S.root4q = refIsLeaf(S.root4) ? leaf(S.rootBox, S.root4) : S.root4;
+        S.root4q = S.root4;
+        if (refIsLeaf(S.root4) && !leaf(S.rootBox, S.root4, S.root4q, err)) return false;
         if (S.nodes4q.empty()) S.nodes4q.emplace_back();
         if (S.leavesq.empty()) S.leavesq.emplace_back();
         return true;

@@ -58,7 +58,7 @@ struct HostScene {
     std::vector<DNode> nodes;
     std::vector<DNode4> nodes4;    // occlusion tree
     std::vector<DNode4Q> nodes4q;  // the same tree quantised
-    std::vector<DLeafQ> leavesq;   // its leaves with exact boxes
+    std::vector<float4> leavesq;   // its leaf records: exact box, then the leaf's triangles
     std::vector<float4> tris;      // 3 per leaf slot
     std::vector<float4> sph;       // leaf-order spheres
     std::vector<int> sphId;        // leaf-order sphere -> global prim id
