@@ -76,21 +76,26 @@ struct CacheOpt {
 // ---------------------------------------------------------------------------
 struct CacheQuery {
     float p[3];      // normalised position
-    float ang[4];    // wo theta, wo phi, n theta, n phi (in [0,1])
+    float ang;       // the one angle this lane's groups use (see cacheQuery)
     float rough;
     float alb[3];
 };
 
-PRR_DEV float cacheUnitTheta(float y) { return acosf(fminf(1.f, fmaxf(-1.f, y))) * (1.f / kPi); }
-PRR_DEV float cacheUnitPhi(float x, float z) { return atan2f(z, x) * (0.5f / kPi) + 0.5f; }
-
-PRR_DEV CacheQuery cacheQuery(const CacheView &cv, const float *q) {
+// Lane group g (lane / 16) evaluates groups G = 4 kb + g, kb = 0..5, so it
+// needs one angle only: G 20 (normal theta) for g = 0, 21 (normal phi) for
+// g = 1, 18 (wo theta) for g = 2, 19 (wo phi) for g = 3. Each is one atan2:
+// theta = acos(y) = atan2(sqrt(1 - y^2), y) with y clamped to [-1, 1], phi =
+// atan2(z, x).
+PRR_DEV CacheQuery cacheQuery(const CacheView &cv, const float *q, int g) {
     CacheQuery r;
     for (int a = 0; a < 3; a++) r.p[a] = fminf(1.f, fmaxf(0.f, (q[a] - cv.bmin[a]) * cv.inv_ext[a]));
-    r.ang[0] = cacheUnitTheta(q[4]);
-    r.ang[1] = cacheUnitPhi(q[3], q[5]);
-    r.ang[2] = cacheUnitTheta(q[7]);
-    r.ang[3] = cacheUnitPhi(q[6], q[8]);
+    const bool theta = (g & 1) == 0;
+    const int v = g < 2 ? 6 : 3;  // normal or wo
+    const float y = fminf(1.f, fmaxf(-1.f, q[v + 1]));
+    const float ay = theta ? sqrtf(fmaxf(0.f, 1.f - y * y)) : q[v + 2];
+    const float ax = theta ? y : q[v];
+    const float t = atan2f(ay, ax);
+    r.ang = theta ? t * (1.f / kPi) : t * (0.5f / kPi) + 0.5f;
     r.rough = q[9];
     r.alb[0] = q[10];
     r.alb[1] = q[11];
@@ -98,9 +103,10 @@ PRR_DEV CacheQuery cacheQuery(const CacheView &cv, const float *q) {
     return r;
 }
 
+// One-blob bin j of s: exp(-0.5 (4 s - j - 0.5)^2), with the hardware exp.
 PRR_DEV float oneBlob(float s, int j) {
     const float d = 4.f * s - ((float) j + 0.5f);
-    return expf(-0.5f * d * d);
+    return __expf(-0.5f * d * d);
 }
 
 // Element k of a small array as a blend of finite values (exact: one term is
@@ -109,23 +115,30 @@ PRR_DEV float oneBlob(float s, int j) {
 PRR_DEV float pick3(const float (&v)[3], int k) {
     return v[0] * (float) (k == 0) + v[1] * (float) (k == 1) + v[2] * (float) (k == 2);
 }
-PRR_DEV float pick4(const float (&v)[4], int k) {
-    return v[0] * (float) (k == 0) + v[1] * (float) (k == 1) + v[2] * (float) (k == 2) + v[3] * (float) (k == 3);
+
+// sin(pi x) and cos(pi x) for x >= 0: the hardware's sin and cos of 2 pi r
+// (v_sin_f32 / v_cos_f32 take revolutions) on r = fract(x / 2), which is
+// exact, so the large octaves lose nothing to argument reduction.
+PRR_DEV void sinCosPi(float x, float &s, float &c) {
+    const float r = __builtin_amdgcn_fractf(0.5f * x);
+    s = __builtin_amdgcn_sinf(r);
+    c = __builtin_amdgcn_cosf(r);
 }
 
-// Input group G (features 4G..4G+3).
+// Input group G (features 4G..4G+3) of the lane group that evaluates it.
 PRR_DEV h4 cacheInputGroup(const CacheQuery &q, int G) {
     float v0, v1, v2, v3;
     if (G < 18) {
         const int k = 2 * (G / 3);
         const float x = pick3(q.p, G - 3 * (G / 3)) * (float) (1 << k);
-        const float s = sinpif(x), c = cospif(x);
+        float s, c;
+        sinCosPi(x, s, c);
         v0 = s;
         v1 = c;
         v2 = 2.f * s * c;
         v3 = c * c - s * s;
     } else if (G < 23) {
-        const float a = G < 22 ? pick4(q.ang, G - 18) : q.rough;
+        const float a = G < 22 ? q.ang : q.rough;
         v0 = oneBlob(a, 0);
         v1 = oneBlob(a, 1);
         v2 = oneBlob(a, 2);

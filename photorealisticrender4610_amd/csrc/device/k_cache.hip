@@ -105,7 +105,7 @@ __global__ __launch_bounds__(kCacheBlock) void kCacheInfer(CacheView cv, int64_t
         for (int qb = 0; qb < QB; qb++) {
             float qq[kCacheQ];
             loadQuery(q, base + qb * 16 + c, n, qq);
-            const CacheQuery cq = cacheQuery(cv, qq);
+            const CacheQuery cq = cacheQuery(cv, qq, g);
 #pragma unroll
             for (int kb = 0; kb < 6; kb++) x[qb][kb] = cacheInputFrag(cq, kb, g);
         }
@@ -261,7 +261,7 @@ __global__ __launch_bounds__(kCacheBlock) void kCacheTrain(CacheView cv, int64_t
         } else {
             loadQuery(q, 0, 0, qq);
         }
-        const CacheQuery cq = cacheQuery(cv, qq);
+        const CacheQuery cq = cacheQuery(cv, qq, g);
 #pragma unroll
         for (int kb = 0; kb < 6; kb++) x[qb][kb] = cacheInputFrag(cq, kb, g);
     }
