@@ -270,7 +270,9 @@ PRR_DEV float lambdaGGX(float NdotV, float alpha) {  // BRDFUtils.hpp:102-106
     return 0.5f * (-1.f + sqrtf(1.f + alpha * alpha * t2));
 }
 PRR_DEV V3 F_Schlick(float WoDotH, const V3 &F0) {  // BRDFUtils.hpp:143-146
-    float t = powf(smax(0.f, 1.f - WoDotH), 5.f);
+    // (1 - WoDotH)^5 by multiplication (within 2 ulp of pow(x, 5))
+    const float x = smax(0.f, 1.f - WoDotH), x2 = x * x;
+    float t = x2 * x2 * x;
     return F0 + (V3(1.f) - F0) * t;
 }
 PRR_DEV V3 sampleVNDF(const V3 &wo, float alpha, float u1, float u2) {  // BRDFUtils.hpp:163-181
