@@ -10,6 +10,7 @@
 #include <algorithm>
 #include <chrono>
 #include <condition_variable>
+#include <cstdlib>
 #include <cstring>
 #include <deque>
 #include <map>
@@ -79,6 +80,7 @@ struct prr_server {
     Segment *dSegs = nullptr;
     int64_t *dPixStart = nullptr;
     int segCap = 0;
+    double aging = 1.0;  // PRR_SERVE_AGING: weight of the waiting credit in PRR_SERVE_AGED
     float *dRgb = nullptr;
     int64_t rgbCap = 0;
     double now() const { return std::chrono::duration<double, std::milli>(Clock::now() - t0).count(); }
@@ -209,7 +211,7 @@ static void planBatch(prr_server *S, std::vector<Planned> &plan) {
         const double now = S->now();
         const double rate = S->stats.busy_ms > 0.0 ? (double) S->stats.samples / S->stats.busy_ms : 2e5;
         std::vector<std::pair<double, Request *>> key;
-        for (Request *r : order) key.emplace_back((double) r->remaining() - rate * (now - r->tSubmit), r);
+        for (Request *r : order) key.emplace_back((double) r->remaining() - S->aging * rate * (now - r->tSubmit), r);
         std::stable_sort(key.begin(), key.end(),
                          [](const std::pair<double, Request *> &a, const std::pair<double, Request *> &b) {
                              return a.first < b.first;
@@ -338,6 +340,7 @@ int prr_server_create(prr_context *c, const prr_scene *const *scenes, const int3
     if (!S) return fail(c, PRR_ERR_NOMEM, "out of host memory");
     S->ctx = c;
     S->p = *p;
+    if (const char *ev = getenv("PRR_SERVE_AGING")) S->aging = std::max(0.0, atof(ev));
     for (int i = 0; i < num_scenes; i++) {
         S->scenes.push_back(scenes[i]);
         S->depth.push_back(max_depth[i]);
