@@ -63,6 +63,23 @@ enum prr_tonemap_op {
     PRR_TONEMAP_ACES = 4
 };
 
+/* Per-triangle attribute flags (Triangle::hasSmoothNormals / hasTangents;
+ * PRR_TRI_UVS marks rows whose t0..t2 came from the asset). */
+enum prr_tri_flag {
+    PRR_TRI_SMOOTH_NORMALS = 1,  /* hit normal = normalize(n0*w + n1*u + n2*v); light-sample normal = normalize(n0+n1+n2) */
+    PRR_TRI_TANGENTS = 2,        /* per-vertex tangents: normal maps use the asset's tangent frame */
+    PRR_TRI_UVS = 4
+};
+
+/* Texture (TextureUtils.hpp:9-19): 8-bit texels, rows top to bottom,
+ * `channels` bytes per texel (>= 3 for colour lookups). */
+typedef struct prr_texture_desc {
+    int32_t width;
+    int32_t height;
+    int32_t channels;
+    const uint8_t* data;  /* host */
+} prr_texture_desc;
+
 /* Material record. Field meaning follows Material.hpp:42-47. */
 typedef struct prr_material_desc {
     int32_t type;          /* prr_material_type */
@@ -122,6 +139,17 @@ typedef struct prr_scene_desc {
     int32_t num_point_lights;
     const prr_point_light_desc* point_lights;
     prr_camera_desc camera;
+    /* Per-vertex attributes and textures (Triangle.hpp:20-31, Triangle.cpp:34-59,
+     * Material.cpp:3-26). All optional: a zero-initialised tail means flat
+     * triangles, uv (0,0) and untextured materials. */
+    const uint8_t* tri_flags;        /* NULL, or per triangle: prr_tri_flag bits */
+    const float* tri_normals;        /* 9 per triangle: n0, n1, n2 (PRR_TRI_SMOOTH_NORMALS) */
+    const float* tri_uvs;            /* 6 per triangle: t0, t1, t2 (PRR_TRI_UVS) */
+    const float* tri_tangents;       /* 12 per triangle: tangent xyz + handedness w per vertex (PRR_TRI_TANGENTS) */
+    const int32_t* material_textures;/* NULL, or 4 per material: base colour, metallic-roughness,
+                                        normal map, emissive; row in textures, -1 none */
+    int32_t num_textures;
+    const prr_texture_desc* textures;
 } prr_scene_desc;
 
 /* RenderSettings (Renderer.hpp:42-55) plus the Integrator parameters
@@ -439,6 +467,45 @@ typedef struct prr_cache_stats {
 int prr_cache_render(prr_context* ctx, const prr_scene* scene, prr_cache* cache,
                      const prr_render_params* params, const prr_cache_render_params* cache_params,
                      float* rgb_out, prr_cache_stats* stats);
+
+/* ------------------------------------------------------------------------
+ * glTF input (host only, no context)
+ * ------------------------------------------------------------------------ */
+
+/* GLTFLoader::load (GLTFLoader.cpp:140-281): one mesh per triangle primitive
+ * of every glTF mesh, in file order, positions as stored (node transforms are
+ * not applied, as in the reference). Each mesh has its own material:
+ * DiffuseMaterial defaults, then the primitive's glTF material (base colour,
+ * metallic and roughness factors, emissive factor, the four textures;
+ * GLTFLoader.cpp:64-138), then `material_type`'s copy rules
+ * (GLTFLoader.cpp:191-224: Glass keeps colour, emission and every texture;
+ * Mirror colour, emission and the emissive texture; Emissive a fresh
+ * material with zero emission). Textures are decoded to 8-bit RGBA (16-bit
+ * images reduced by >> 8). Normals are normalised per vertex. PNG images
+ * only; a JPEG image is an error. All arrays are owned by the result. */
+typedef struct prr_gltf_data {
+    int32_t num_meshes;
+    const int32_t* mesh_first;      /* first triangle row of each mesh */
+    const int32_t* mesh_count;      /* triangles per mesh */
+    const int32_t* mesh_material;   /* material row of each mesh */
+    int32_t num_triangles;
+    const float* positions;         /* 9 per triangle */
+    const float* normals;           /* 9 per triangle (zeros without PRR_TRI_SMOOTH_NORMALS) */
+    const float* uvs;               /* 6 per triangle */
+    const float* tangents;          /* 12 per triangle */
+    const uint8_t* tri_flags;       /* prr_tri_flag bits per triangle */
+    int32_t num_materials;
+    const prr_material_desc* materials;
+    const int32_t* material_textures;  /* 4 per material, as prr_scene_desc::material_textures */
+    int32_t num_textures;
+    const prr_texture_desc* textures;
+    void* impl;                     /* owner of the arrays */
+} prr_gltf_data;
+
+/* `material_type`: prr_material_type, GLTFLoader's MaterialType. On error
+ * *out is NULL and prr_last_error(NULL) names the file and the cause. */
+int prr_gltf_load(const char* path, int32_t material_type, prr_gltf_data** out);
+void prr_gltf_free(prr_gltf_data* data);
 
 #ifdef __cplusplus
 }

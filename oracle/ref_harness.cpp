@@ -3,7 +3,7 @@
 // TEST INFRASTRUCTURE ONLY.
 //
 // Usage: ref_harness <mode> <in.bundle> <out.bundle>
-//   modes: trace, bsdf, env, camera, tonemap, castray, image, welford, hdr,
+//   modes: trace, bsdf, env, camera, tonemap, castray, image, welford, hdr, gltf,
 //          bench (only in the ref_bench build: PRR_REF_BENCH)
 // A bundle is a sequence of records {char name[32]; int32 dtype; int64 count;
 // payload}, dtype 0=f32 1=i32 2=u64 3=u8 4=u32 (tests/golden/bundle.py).
@@ -37,6 +37,7 @@
 #define private public
 #include "BVH.hpp"
 #include "EnvMap.hpp"
+#include "GLTFLoader.hpp"
 #include "Integrator.hpp"
 #include "Material.hpp"
 #include "Renderer.hpp"
@@ -586,6 +587,51 @@ static int modeHdr(const Bundle &b, const char *out) {
     return 0;
 }
 
+// GLTFLoader::load of one file: per mesh its triangles (vertices, flat and
+// per-vertex normals, uvs, tangents, flags) and its material (class, factors,
+// the four textures' texels).
+static int modeGltf(const Bundle &b, const char *out) {
+    const auto &pa = b.at("path");
+    std::string path(reinterpret_cast<const char *>(pa.bytes.data()), pa.bytes.size());
+    const int type = b.at("type").as<int32_t>()[0];
+    const MaterialType mt[4] = {MaterialType::Diffuse, MaterialType::Glass, MaterialType::Mirror, MaterialType::Emit};
+    std::vector<MeshTriangle *> meshes = GLTFLoader::load(path, mt[type]);
+    std::vector<int32_t> counts, flags, mtype, tw;
+    std::vector<float> tri, mat;
+    std::vector<uint8_t> texels;
+    auto put3 = [&](const Vector3f &v) { tri.insert(tri.end(), {v.x, v.y, v.z}); };
+    auto put2 = [&](const Vector2f &v) { tri.insert(tri.end(), {v.x, v.y}); };
+    for (MeshTriangle *m : meshes) {
+        counts.push_back((int32_t) m->triangles.size());
+        for (const Triangle &t : m->triangles) {
+            put3(t.v0), put3(t.v1), put3(t.v2), put3(t.normal);
+            put3(t.n0), put3(t.n1), put3(t.n2);
+            put2(t.t0), put2(t.t1), put2(t.t2);
+            put3(t.tan0), put3(t.tan1), put3(t.tan2);
+            tri.insert(tri.end(), {t.tangentW0, t.tangentW1, t.tangentW2});
+            flags.push_back((t.hasSmoothNormals ? 1 : 0) | (t.hasTangents ? 2 : 0));
+        }
+        Material *M = m->m;
+        mtype.push_back(dynamic_cast<GlassMaterial *>(M) ? 1 : dynamic_cast<MirrorMaterial *>(M) ? 2
+                        : dynamic_cast<EmissiveMaterial *>(M) ? 3 : 0);
+        mat.insert(mat.end(), {M->baseColor.x, M->baseColor.y, M->baseColor.z, M->roughness, M->metallic, M->ior,
+                               M->m_emission.x, M->m_emission.y, M->m_emission.z});
+        for (const Texture *t : {&M->baseColorTex, &M->metallicRoughnessTex, &M->normalTex, &M->emissiveTex}) {
+            tw.insert(tw.end(), {t->width, t->height, t->channels, (int32_t) t->data.size()});
+            texels.insert(texels.end(), t->data.begin(), t->data.end());
+        }
+    }
+    Writer w(out);
+    w.i32("counts", counts);
+    w.i32("flags", flags);
+    w.f32("tri", tri);
+    w.i32("mtype", mtype);
+    w.f32("mat", mat);
+    w.i32("tex", tw);
+    w.u8("texels", texels);
+    return 0;
+}
+
 #ifdef PRR_REF_BENCH
 // Scene-level query counts without editing the reference: the link wraps
 // Scene::intersect / Scene::intersectP (-Wl,--wrap, oracle/Makefile), so every
@@ -673,6 +719,7 @@ int main(int argc, char **argv) {
     if (m == "image") return modeImage(b, argv[3]);
     if (m == "welford") return modeWelford(b, argv[3]);
     if (m == "hdr") return modeHdr(b, argv[3]);
+    if (m == "gltf") return modeGltf(b, argv[3]);
 #ifdef PRR_REF_BENCH
     if (m == "bench") return modeBench(b, argv[3]);
 #endif

@@ -43,6 +43,11 @@ class PointLightDesc(C.Structure):
     _fields_ = [("position", C.c_float * 3), ("intensity", C.c_float * 3)]
 
 
+class TextureDesc(C.Structure):
+    _fields_ = [("width", C.c_int32), ("height", C.c_int32), ("channels", C.c_int32),
+                ("data", C.POINTER(C.c_uint8))]
+
+
 class SceneDesc(C.Structure):
     _fields_ = [("num_objects", C.c_int32), ("objects", C.POINTER(ObjectDesc)),
                 ("num_triangles", C.c_int32), ("triangles", C.POINTER(C.c_float)),
@@ -50,7 +55,22 @@ class SceneDesc(C.Structure):
                 ("num_materials", C.c_int32), ("materials", C.POINTER(MaterialDesc)),
                 ("env", EnvDesc), ("background", C.c_float * 3),
                 ("num_point_lights", C.c_int32), ("point_lights", C.POINTER(PointLightDesc)),
-                ("camera", CameraDesc)]
+                ("camera", CameraDesc),
+                ("tri_flags", C.POINTER(C.c_uint8)), ("tri_normals", C.POINTER(C.c_float)),
+                ("tri_uvs", C.POINTER(C.c_float)), ("tri_tangents", C.POINTER(C.c_float)),
+                ("material_textures", C.POINTER(C.c_int32)), ("num_textures", C.c_int32),
+                ("textures", C.POINTER(TextureDesc))]
+
+
+class GltfData(C.Structure):
+    _fields_ = [("num_meshes", C.c_int32), ("mesh_first", C.POINTER(C.c_int32)),
+                ("mesh_count", C.POINTER(C.c_int32)), ("mesh_material", C.POINTER(C.c_int32)),
+                ("num_triangles", C.c_int32), ("positions", C.POINTER(C.c_float)),
+                ("normals", C.POINTER(C.c_float)), ("uvs", C.POINTER(C.c_float)),
+                ("tangents", C.POINTER(C.c_float)), ("tri_flags", C.POINTER(C.c_uint8)),
+                ("num_materials", C.c_int32), ("materials", C.POINTER(MaterialDesc)),
+                ("material_textures", C.POINTER(C.c_int32)), ("num_textures", C.c_int32),
+                ("textures", C.POINTER(TextureDesc)), ("impl", C.c_void_p)]
 
 
 class RenderParams(C.Structure):
@@ -166,8 +186,10 @@ SIGNATURES = {
     "prr_server_wait": [vp, i64, vp, C.POINTER(RequestStats)],
     "prr_server_get_stats": [vp, C.POINTER(ServerStats)],
     "prr_server_last_error": [vp],
+    "prr_gltf_load": [C.c_char_p, i32, C.POINTER(C.POINTER(GltfData))],
+    "prr_gltf_free": [C.POINTER(GltfData)],
 }
-_RESTYPES = {"prr_last_error": C.c_char_p, "prr_server_last_error": C.c_char_p}
+_RESTYPES = {"prr_last_error": C.c_char_p, "prr_server_last_error": C.c_char_p, "prr_gltf_free": None}
 
 
 def lib():
@@ -224,6 +246,32 @@ def scene_desc(sc):
         d.num_point_lights = len(pl)
         d.point_lights = C.cast(pa, C.POINTER(PointLightDesc))
         keep.append(pa)
+    if getattr(sc, "tri_flags", None) is not None:
+        fl = np.ascontiguousarray(sc.tri_flags, np.uint8).reshape(-1)
+        nrm = np.ascontiguousarray(sc.tri_normals, np.float32).reshape(-1, 9)
+        uvs = np.ascontiguousarray(sc.tri_uvs, np.float32).reshape(-1, 6)
+        tan = np.ascontiguousarray(sc.tri_tangents, np.float32).reshape(-1, 12)
+        if not (fl.shape[0] == nrm.shape[0] == uvs.shape[0] == tan.shape[0] == tris.shape[0]):
+            raise PrrError("per-triangle attribute arrays must have one row per triangle")
+        d.tri_flags = fl.ctypes.data_as(C.POINTER(C.c_uint8))
+        d.tri_normals = nrm.ctypes.data_as(C.POINTER(C.c_float))
+        d.tri_uvs = uvs.ctypes.data_as(C.POINTER(C.c_float))
+        d.tri_tangents = tan.ctypes.data_as(C.POINTER(C.c_float))
+        keep += [fl, nrm, uvs, tan]
+    if getattr(sc, "mat_tex", None) is not None:
+        mt = np.ascontiguousarray(sc.mat_tex, np.int32).reshape(-1, 4)
+        if mt.shape[0] != len(mats):
+            raise PrrError("mat_tex must have one row per material")
+        d.material_textures = mt.ctypes.data_as(C.POINTER(C.c_int32))
+        texs = [np.ascontiguousarray(t, np.uint8) for t in (sc.textures or [])]
+        ta = (TextureDesc * max(1, len(texs)))()
+        for i, t in enumerate(texs):
+            ta[i].height, ta[i].width = t.shape[0], t.shape[1]
+            ta[i].channels = t.shape[2] if t.ndim == 3 else 1
+            ta[i].data = t.ctypes.data_as(C.POINTER(C.c_uint8))
+        d.num_textures = len(texs)
+        d.textures = C.cast(ta, C.POINTER(TextureDesc))
+        keep += [mt, ta, texs]
     cam = np.asarray(sc.camera, np.float32)
     d.camera.eye[:] = [float(x) for x in cam[0:3]]
     d.camera.target[:] = [float(x) for x in cam[3:6]]
@@ -231,6 +279,48 @@ def scene_desc(sc):
     d.camera.fovh, d.camera.aperture, d.camera.focus_distance = float(cam[9]), float(cam[10]), float(cam[11])
     keep += [objs, tris, sph, mats, ma, oa, env]
     return d, keep
+
+
+def load_gltf(path, material_type=0):
+    """prr_gltf_load (GLTFLoader::load, GLTFLoader.cpp:140-281) as numpy
+    arrays: a dict with meshes [(first, count, material)], positions [n,9],
+    normals [n,9], uvs [n,6], tangents [n,12], flags [n], materials [m,12]
+    (the SceneData material row layout), mat_tex [m,4], textures [list of
+    (H, W, C) uint8]. Host only: needs no GPU."""
+    L = lib()
+    out = C.POINTER(GltfData)()
+    if L.prr_gltf_load(os.fsencode(path), int(material_type), C.byref(out)) != PRR_OK:
+        raise PrrError(L.prr_last_error(None).decode())
+    try:
+        g = out.contents
+        n, m = g.num_triangles, g.num_materials
+
+        def arr(ptr, count, dtype):
+            if count == 0:
+                return np.zeros(0, dtype)
+            return np.ctypeslib.as_array(ptr, shape=(count,)).astype(dtype, copy=True)
+        res = {
+            "meshes": [(g.mesh_first[i], g.mesh_count[i], g.mesh_material[i]) for i in range(g.num_meshes)],
+            "positions": arr(g.positions, 9 * n, np.float32).reshape(n, 9),
+            "normals": arr(g.normals, 9 * n, np.float32).reshape(n, 9),
+            "uvs": arr(g.uvs, 6 * n, np.float32).reshape(n, 6),
+            "tangents": arr(g.tangents, 12 * n, np.float32).reshape(n, 12),
+            "flags": arr(g.tri_flags, n, np.uint8),
+            "mat_tex": arr(g.material_textures, 4 * m, np.int32).reshape(m, 4),
+            "textures": [],
+        }
+        mats = np.zeros((m, 12), np.float32)
+        for i in range(m):
+            md = g.materials[i]
+            mats[i, :10] = [md.type, *md.base_color, md.roughness, md.metallic, md.ior, *md.emission]
+        res["materials"] = mats
+        for i in range(g.num_textures):
+            t = g.textures[i]
+            res["textures"].append(arr(t.data, t.width * t.height * t.channels, np.uint8)
+                                   .reshape(t.height, t.width, t.channels))
+        return res
+    finally:
+        L.prr_gltf_free(out)
 
 
 def _ptr(t):

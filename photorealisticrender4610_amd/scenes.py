@@ -37,6 +37,13 @@ class SceneData:
     env: Optional[np.ndarray] = None          # float32 [H, W, 3], row 0 = zenith
     point_lights: Optional[np.ndarray] = None  # float32 [np, 6]: position, intensity
     name: str = "scene"
+    # per-vertex attributes (all or none): prr_tri_flag bits, n0..n2, t0..t2, tangent xyzw x3
+    tri_flags: Optional[np.ndarray] = None     # uint8 [nt]
+    tri_normals: Optional[np.ndarray] = None   # float32 [nt, 9]
+    tri_uvs: Optional[np.ndarray] = None       # float32 [nt, 6]
+    tri_tangents: Optional[np.ndarray] = None  # float32 [nt, 12]
+    mat_tex: Optional[np.ndarray] = None       # int32 [nm, 4]: base colour, metal-rough, normal, emissive (-1 none)
+    textures: Optional[list] = None            # uint8 [H, W, C] each
 
     @property
     def num_triangles(self):
@@ -60,6 +67,16 @@ class SceneData:
             h, w, _ = self.env.shape
             out["env_wh"] = np.array([w, h], np.int32)
             out["env_pixels"] = self.env.astype(np.float32).ravel()
+        if self.tri_flags is not None:
+            out["tri_flags"] = self.tri_flags.astype(np.int32)
+            out["tri_attr"] = np.concatenate([self.tri_normals, self.tri_uvs, self.tri_tangents], 1) \
+                .astype(np.float32).ravel()
+        if self.mat_tex is not None:
+            out["mat_tex"] = self.mat_tex.astype(np.int32).ravel()
+            out["tex_whc"] = np.array([[t.shape[1], t.shape[0], t.shape[2]] for t in self.textures],
+                                      np.int32).reshape(-1)
+            out["texels"] = np.concatenate([t.ravel() for t in self.textures]) if self.textures \
+                else np.zeros(0, np.uint8)
         return out
 
 
@@ -74,21 +91,72 @@ class SceneBuilder:
         self.env = None
         self.points: List[List[float]] = []
         self.camera = np.array([0, 0, -1, 0, 0, 0, 0, 1, 0, 40.0, 0.0, -1.0], np.float32)
+        self._attrs: List[tuple] = []   # per mesh: (flags, normals, uvs, tangents) or None
+        self._mat_tex: List[List[int]] = []
+        self._textures: List[np.ndarray] = []
 
     def material(self, mtype=MAT_DIFFUSE, base=(0.8, 0.8, 0.8), roughness=0.5, metallic=0.0,
-                 ior=1.5, emission=(0.0, 0.0, 0.0)) -> int:
+                 ior=1.5, emission=(0.0, 0.0, 0.0), textures=(-1, -1, -1, -1)) -> int:
+        """textures: rows from :meth:`texture` for base colour, metallic-roughness
+        (G = roughness, B = metallic), normal map, emissive; -1 none."""
         self._mats.append([float(mtype), *map(float, base), float(roughness), float(metallic),
                            float(ior), *map(float, emission), 0.0, 0.0])
+        self._mat_tex.append([int(x) for x in textures])
         return len(self._mats) - 1
 
-    def mesh(self, tris, material: int) -> int:
+    def texture(self, img) -> int:
+        """8-bit texture [H, W, C] (TextureUtils.hpp:9-19); returns its row."""
+        t = np.ascontiguousarray(img, np.uint8)
+        if t.ndim != 3 or t.shape[2] < 1:
+            raise ValueError("texture must be [H, W, C] uint8")
+        self._textures.append(t)
+        return len(self._textures) - 1
+
+    def mesh(self, tris, material: int, normals=None, uvs=None, tangents=None) -> int:
+        """Triangles [n, 9]; optional per-vertex normals [n, 9] (normalised per
+        vertex, Triangle::hasSmoothNormals), uvs [n, 6], tangents [n, 12]
+        (xyz + handedness per vertex, Triangle::hasTangents)."""
         t = np.asarray(tris, np.float32).reshape(-1, 9)
         if t.shape[0] == 0:
             raise ValueError("empty mesh")
-        self._objs.append((OBJ_MESH, material, self._ntris, t.shape[0]))
+        n = t.shape[0]
+        if normals is None and uvs is None and tangents is None:
+            self._attrs.append(None)
+        else:
+            fl = np.zeros(n, np.uint8)
+            def rows(a, k, bit):
+                if a is None:
+                    return np.zeros((n, k), np.float32)
+                fl[:] |= bit
+                return np.asarray(a, np.float32).reshape(n, k)
+            self._attrs.append((fl, rows(normals, 9, 1), rows(uvs, 6, 4), rows(tangents, 12, 2)))
+        self._objs.append((OBJ_MESH, material, self._ntris, n))
         self._tris.append(t)
-        self._ntris += t.shape[0]
+        self._ntris += n
         return len(self._objs) - 1
+
+    def gltf(self, path, material_type=MAT_DIFFUSE, material: Optional[int] = None) -> List[int]:
+        """GLTFLoader::load (GLTFLoader.cpp:140-281) + scene.Add of each mesh:
+        one mesh per triangle primitive with its own material (or `material`,
+        the loader's override). Returns the object rows."""
+        from . import abi
+        g = abi.load_gltf(path, material_type)
+        base = len(self._textures)
+        self._textures += g["textures"]
+        out = []
+        for first, count, mrow in g["meshes"]:
+            if material is None:
+                m = g["materials"][mrow]
+                tx = [x + base if x >= 0 else -1 for x in g["mat_tex"][mrow]]
+                mid = self.material(int(m[0]), m[1:4], m[4], m[5], m[6], m[7:10], tx)
+            else:
+                mid = material
+            sl = slice(first, first + count)
+            fl = g["flags"][sl]
+            out.append(self.mesh(g["positions"][sl], mid,
+                                 normals=g["normals"][sl], uvs=g["uvs"][sl], tangents=g["tangents"][sl]))
+            self._attrs[-1][0][:] = fl
+        return out
 
     def sphere(self, center, radius, material: int) -> int:
         self._objs.append((OBJ_SPHERE, material, len(self._sph), 1))
@@ -113,7 +181,24 @@ class SceneBuilder:
             env=None if self.env is None else np.ascontiguousarray(self.env, np.float32),
             point_lights=np.array(self.points, np.float32).reshape(-1, 6) if self.points else None,
             name=name,
+            **self._attr_fields(),
         )
+
+    def _attr_fields(self):
+        out = {}
+        if any(a is not None for a in self._attrs):
+            parts = [a if a is not None else (np.zeros(t.shape[0], np.uint8), np.zeros((t.shape[0], 9), np.float32),
+                                              np.zeros((t.shape[0], 6), np.float32),
+                                              np.zeros((t.shape[0], 12), np.float32))
+                     for a, t in zip(self._attrs, self._tris)]
+            out["tri_flags"] = np.concatenate([p[0] for p in parts])
+            out["tri_normals"] = np.concatenate([p[1] for p in parts])
+            out["tri_uvs"] = np.concatenate([p[2] for p in parts])
+            out["tri_tangents"] = np.concatenate([p[3] for p in parts])
+        if self._textures or any(any(x >= 0 for x in r) for r in self._mat_tex):
+            out["mat_tex"] = np.array(self._mat_tex, np.int32).reshape(-1, 4)
+            out["textures"] = list(self._textures)
+        return out
 
 
 # ---------------------------------------------------------------------------

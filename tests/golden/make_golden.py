@@ -4,7 +4,7 @@ Runs in the development container only: builds oracle/_ref/ref_harness from
 the reference's own sources (oracle/Makefile target ``ref``), feeds it the
 seeded inputs below, and stores inputs + reference outputs together, so the
 tests never need the reference. Usage:
-python tests/golden/make_golden.py [trace bsdf env camera tonemap castray image welford hdr]
+python tests/golden/make_golden.py [trace bsdf env camera tonemap castray image welford hdr gltf]
 (no names: all).
 """
 import os
@@ -243,6 +243,196 @@ def gen_hdr(seed=71):
     save("hdr.npz", res)
 
 
+def png_bytes(px, ctype, depth, palette=None, trns=None, seed=0):
+    """PNG file bytes, one random filter type per row (0-4), for the decoder
+    cases: px is the raw sample array [H, W, channels] (palette indices for
+    colour type 3)."""
+    import struct
+    import zlib
+    rng = np.random.default_rng(seed)
+    h, w = px.shape[:2]
+    if depth == 16:
+        raw_rows = [px[y].astype(">u2").tobytes() for y in range(h)]
+    elif depth == 8:
+        raw_rows = [px[y].astype(np.uint8).tobytes() for y in range(h)]
+    else:
+        raw_rows = []
+        for y in range(h):
+            bits = "".join(format(int(v), "0%db" % depth) for v in px[y].ravel())
+            bits += "0" * (-len(bits) % 8)
+            raw_rows.append(bytes(int(bits[i:i + 8], 2) for i in range(0, len(bits), 8)))
+    chans = {0: 1, 2: 3, 3: 1, 4: 2, 6: 4}[ctype]
+    bpp = max(1, chans * depth // 8)
+    out, prev = bytearray(), bytes(len(raw_rows[0]))
+    for row in raw_rows:
+        ft = int(rng.integers(0, 5))
+        f = bytearray(len(row))
+        for i in range(len(row)):
+            a = row[i - bpp] if i >= bpp else 0
+            b = prev[i]
+            c = prev[i - bpp] if i >= bpp else 0
+            if ft == 0:
+                pr = 0
+            elif ft == 1:
+                pr = a
+            elif ft == 2:
+                pr = b
+            elif ft == 3:
+                pr = (a + b) >> 1
+            else:
+                pp = a + b - c
+                pa, pb, pc = abs(pp - a), abs(pp - b), abs(pp - c)
+                pr = a if (pa <= pb and pa <= pc) else (b if pb <= pc else c)
+            f[i] = (row[i] - pr) & 255
+        out += bytes([ft]) + f
+        prev = row
+
+    def chunk(t, d):
+        return struct.pack(">I", len(d)) + t + d + struct.pack(">I", zlib.crc32(t + d) & 0xFFFFFFFF)
+    data = b"\x89PNG\r\n\x1a\n" + chunk(b"IHDR", struct.pack(">IIBBBBB", w, h, depth, ctype, 0, 0, 0))
+    if palette is not None:
+        data += chunk(b"PLTE", bytes(np.asarray(palette, np.uint8).ravel()))
+    if trns is not None:
+        data += chunk(b"tRNS", bytes(trns))
+    return data + chunk(b"IDAT", zlib.compress(bytes(out), 6)) + chunk(b"IEND", b"")
+
+
+def gltf_assets(seed=81):
+    """Synthetic glTF assets for the loader: (file name, bytes, material type)
+    plus side files. textured.glb: two triangle primitives (float attributes,
+    u16 indices; normalised u16/u8 attributes, u8 indices, no material), a
+    line primitive (skipped), four PNG textures (RGB 8-bit, RGBA 16-bit,
+    palette 4-bit with tRNS, grey 2-bit). textured.gltf: external .bin buffer
+    and a base64 data: image, no index accessor."""
+    import base64
+    import json
+    import struct
+    rng = np.random.default_rng(seed)
+    pngs = [
+        png_bytes(rng.integers(0, 256, (5, 7, 3)), 2, 8, seed=1),
+        png_bytes(rng.integers(0, 65536, (4, 6, 4)), 6, 16, seed=2),
+        png_bytes(rng.integers(0, 16, (6, 5, 1)), 3, 4, palette=rng.integers(0, 256, (16, 3)),
+                  trns=list(rng.integers(0, 256, 9)), seed=3),
+        png_bytes(rng.integers(0, 4, (3, 9, 1)), 0, 2, seed=4),
+        png_bytes(rng.integers(0, 256, (4, 4, 2)), 4, 8, seed=5),
+    ]
+    nv = 12
+    pos = rng.normal(size=(nv, 3)).astype(np.float32)
+    nrm = rng.normal(size=(nv, 3)).astype(np.float32)
+    uv = rng.uniform(-1.5, 2.5, (nv, 2)).astype(np.float32)
+    tan = np.concatenate([rng.normal(size=(nv, 3)), rng.choice([-1.0, 1.0], (nv, 1))], 1).astype(np.float32)
+    idx16 = rng.integers(0, nv, 18).astype(np.uint16)
+    nrm16 = rng.integers(0, 65536, (nv, 3)).astype(np.uint16)
+    uv8 = rng.integers(0, 256, (nv, 2)).astype(np.uint8)
+    idx8 = rng.integers(0, nv, 9).astype(np.uint8)
+    blobs, views, accs = [], [], []
+
+    def add_blob(b, stride=0):
+        off = sum(len(x) for x in blobs)
+        pad = (-off) % 4
+        if pad:
+            blobs.append(b"\0" * pad)
+            off += pad
+        blobs.append(b)
+        v = {"buffer": 0, "byteOffset": off, "byteLength": len(b)}
+        if stride:
+            v["byteStride"] = stride
+        views.append(v)
+        return len(views) - 1
+
+    def add_acc(arr, ctype, typ, normalized=False, stride=0):
+        data = arr.tobytes()
+        if stride:  # interleave with padding bytes
+            row = arr.reshape(arr.shape[0], -1)
+            data = b"".join(r.tobytes() + b"\xAB" * (stride - r.nbytes) for r in row)
+        a = {"bufferView": add_blob(data, stride), "componentType": ctype, "count": int(arr.shape[0]), "type": typ}
+        if normalized:
+            a["normalized"] = True
+        accs.append(a)
+        return len(accs) - 1
+    A_pos = add_acc(pos, 5126, "VEC3", stride=16)
+    A_nrm = add_acc(nrm, 5126, "VEC3")
+    A_uv = add_acc(uv, 5126, "VEC2")
+    A_tan = add_acc(tan, 5126, "VEC4")
+    A_idx = add_acc(idx16, 5123, "SCALAR")
+    A_nrm16 = add_acc(nrm16, 5123, "VEC3", True)
+    A_uv8 = add_acc(uv8, 5121, "VEC2", True, stride=4)
+    A_idx8 = add_acc(idx8, 5121, "SCALAR")
+    img_views = [add_blob(p) for p in pngs]
+    doc = {
+        "asset": {"version": "2.0"},
+        "images": [{"bufferView": v, "mimeType": "image/png"} for v in img_views],
+        "textures": [{"source": i} for i in range(len(pngs))] + [{}],
+        "materials": [
+            {"pbrMetallicRoughness": {"baseColorFactor": [0.3, 0.6, 0.9, 1.0], "metallicFactor": 0.25,
+                                      "roughnessFactor": 0.75, "baseColorTexture": {"index": 0},
+                                      "metallicRoughnessTexture": {"index": 1}},
+             "normalTexture": {"index": 2}, "emissiveTexture": {"index": 3}, "emissiveFactor": [0.5, 0.25, 2.0]},
+            {"pbrMetallicRoughness": {"baseColorTexture": {"index": 4}}, "normalTexture": {"index": 5}},
+        ],
+        "meshes": [
+            {"primitives": [{"attributes": {"POSITION": A_pos, "NORMAL": A_nrm, "TEXCOORD_0": A_uv,
+                                            "TANGENT": A_tan}, "indices": A_idx, "material": 0},
+                            {"attributes": {"POSITION": A_pos}, "indices": A_idx, "mode": 1}]},
+            {"primitives": [{"attributes": {"POSITION": A_pos, "NORMAL": A_nrm16, "TEXCOORD_0": A_uv8},
+                             "indices": A_idx8},
+                            {"attributes": {"POSITION": A_pos, "TEXCOORD_0": A_uv}, "indices": A_idx,
+                             "material": 1}]},
+        ],
+    }
+    binb = b"".join(blobs)
+    binb += b"\0" * ((-len(binb)) % 4)
+    doc["buffers"] = [{"byteLength": len(binb)}]
+    js = json.dumps(doc).encode()
+    js += b" " * ((-len(js)) % 4)
+    glb = struct.pack("<III", 0x46546C67, 2, 12 + 8 + len(js) + 8 + len(binb))
+    glb += struct.pack("<II", len(js), 0x4E4F534A) + js + struct.pack("<II", len(binb), 0x004E4942) + binb
+    # .gltf: external buffer, no indices, data: URI image
+    doc2 = {
+        "asset": {"version": "2.0"},
+        "buffers": [{"uri": "textured%20data.bin", "byteLength": len(binb)}],
+        "bufferViews": views, "accessors": accs,
+        "images": [{"uri": "data:image/png;base64," + base64.b64encode(pngs[0]).decode()}],
+        "textures": [{"source": 0}],
+        "materials": [{"pbrMetallicRoughness": {"baseColorTexture": {"index": 0}, "metallicFactor": 0.0}}],
+        "meshes": [{"primitives": [{"attributes": {"POSITION": A_pos, "NORMAL": A_nrm}, "material": 0}]}],
+    }
+    doc["bufferViews"], doc["accessors"] = views, accs
+    js = json.dumps(doc).encode()
+    js += b" " * ((-len(js)) % 4)
+    glb = struct.pack("<III", 0x46546C67, 2, 12 + 8 + len(js) + 8 + len(binb))
+    glb += struct.pack("<II", len(js), 0x4E4F534A) + js + struct.pack("<II", len(binb), 0x004E4942) + binb
+    return [("textured.glb", glb), ("textured.gltf", json.dumps(doc2).encode()), ("textured data.bin", binb)]
+
+
+def gen_gltf():
+    """GLTFLoader::load through the reference's own loader (tinygltf +
+    stb_image) on the synthetic assets, each under every material type, and
+    on the reference's Cornell-box models."""
+    res = {}
+    with tempfile.TemporaryDirectory() as td:
+        files = gltf_assets()
+        for name, data in files:
+            with open(os.path.join(td, name), "wb") as f:
+                f.write(data)
+            res["file_" + name.replace(" ", "_")] = np.frombuffer(data, np.uint8)
+        cornell = sorted(f for f in os.listdir("/root/reference/assets/models") if f.endswith(".glb"))
+        for c in cornell:
+            with open(os.path.join("/root/reference/assets/models", c), "rb") as f:
+                data = f.read()
+            with open(os.path.join(td, c), "wb") as f:
+                f.write(data)
+            res["file_" + c] = np.frombuffer(data, np.uint8)
+        cases = [(n, t) for n in ("textured.glb", "textured.gltf") for t in range(4)] + [(c, 0) for c in cornell]
+        res["cases"] = np.frombuffer("\n".join("%s:%d" % c for c in cases).encode(), np.uint8)
+        for name, t in cases:
+            out = run("gltf", dict(path=np.frombuffer(os.path.join(td, name).encode(), np.uint8),
+                                   type=np.array([t], np.int32)))
+            for k, v in out.items():
+                res["%s:%d:%s" % (name.replace(" ", "_"), t, k)] = v
+    save("gltf.npz", res)
+
+
 def main():
     if not os.path.isdir("/root/reference"):
         print("reference sources absent; fixtures are committed, nothing to do")
@@ -276,6 +466,8 @@ def main():
         gen_welford()
     if want("hdr"):
         gen_hdr()
+    if want("gltf"):
+        gen_gltf()
 
 
 if __name__ == "__main__":

@@ -1,0 +1,119 @@
+"""glTF input (prr_gltf_load) against the reference's own GLTFLoader::load
+(GLTFLoader.cpp:140-281, tinygltf + stb_image), whose output on the same
+files is stored in tests/golden/gltf.npz (tests/golden/make_golden.py gltf):
+synthetic assets that cover GLB and .gltf containers, an external buffer
+with an escaped name, a base64 data: image, strided and normalised
+attributes, u8/u16 indices, missing indices, a skipped line primitive, every
+PNG colour type the assets use (RGB 8, RGBA 16, 4-bit palette with tRNS,
+2-bit grey, grey+alpha) under all four material types, and the reference's
+Cornell-box models. Host only: no GPU."""
+import os
+
+import numpy as np
+import pytest
+
+from photorealisticrender4610_amd import abi, scenes
+
+Z = np.load(os.path.join(os.path.dirname(__file__), "golden", "gltf.npz"))
+CASES = [c.split(":") for c in bytes(Z["cases"]).decode().split("\n")]
+
+
+@pytest.fixture(scope="module")
+def assets(tmp_path_factory):
+    d = tmp_path_factory.mktemp("gltf")
+    for k in Z.files:
+        if k.startswith("file_"):
+            name = k[5:].replace("textured_data.bin", "textured data.bin")
+            (d / name).write_bytes(Z[k].tobytes())
+    return d
+
+
+def _ref(name, t, key):
+    return Z["%s:%d:%s" % (name, t, key)]
+
+
+@pytest.mark.parametrize("name,t", [(n, int(t)) for n, t in CASES])
+def test_loader_matches_reference(assets, name, t):
+    g = abi.load_gltf(str(assets / name), t)
+    counts = _ref(name, t, "counts")
+    assert [c for _, c, _ in g["meshes"]] == counts.tolist()
+    n = int(counts.sum())
+    tri = _ref(name, t, "tri").reshape(n, 39)
+    # v0 v1 v2 | flat normal | n0 n1 n2 | t0 t1 t2 | tan0 tan1 tan2 | w0 w1 w2
+    assert np.array_equal(g["positions"], tri[:, 0:9])
+    flags = _ref(name, t, "flags")
+    smooth = (flags & 1) != 0
+    tang = (flags & 2) != 0
+    assert np.array_equal((g["flags"] & 1) != 0, smooth)
+    assert np.array_equal((g["flags"] & 2) != 0, tang)
+    assert np.array_equal(g["normals"][smooth], tri[smooth, 12:21])
+    assert np.array_equal(g["uvs"], tri[:, 21:27])
+    mine_tan = g["tangents"].reshape(n, 3, 4)
+    assert np.array_equal(mine_tan[tang][:, :, :3].reshape(-1, 9), tri[tang, 27:36])
+    assert np.array_equal(mine_tan[tang][:, :, 3], tri[tang, 36:39])
+    # the flat normal the light sampler uses for a smooth triangle:
+    # normalize(n0 + n1 + n2) (GLTFLoader.cpp:250), 1/sqrtf as Vector.hpp:103-111
+    if smooth.any():
+        nn = g["normals"][smooth].reshape(-1, 3, 3)
+        s = (nn[:, 0] + nn[:, 1]) + nn[:, 2]
+        m2 = (s[:, 0] * s[:, 0] + s[:, 1] * s[:, 1]) + s[:, 2] * s[:, 2]
+        inv = np.float32(1) / np.sqrt(m2)
+        assert np.array_equal(s * inv[:, None], tri[smooth, 9:12])
+    # materials: class, factors, textures
+    mt = _ref(name, t, "mtype")
+    mat = _ref(name, t, "mat").reshape(-1, 9)
+    tex = _ref(name, t, "tex").reshape(-1, 4, 4)
+    texels = _ref(name, t, "texels")
+    off = 0
+    for k, (_, _, mrow) in enumerate(g["meshes"]):
+        m = g["materials"][mrow]
+        assert int(m[0]) == mt[k]
+        assert np.array_equal(m[1:10], mat[k])
+        for j in range(4):
+            w, h, ch, nb = tex[k, j]
+            row = g["mat_tex"][mrow, j]
+            ref_px = texels[off:off + nb]
+            off += nb
+            if nb == 0:
+                assert row == -1
+                continue
+            assert row >= 0
+            px = g["textures"][row]
+            assert px.shape == (h, w, ch)
+            assert np.array_equal(px.ravel(), ref_px)
+
+
+def test_loader_errors(assets, tmp_path):
+    with pytest.raises(abi.PrrError, match="cannot open"):
+        abi.load_gltf(str(tmp_path / "missing.glb"))
+    bad = tmp_path / "bad.glb"
+    bad.write_bytes(b"glTF\x02\x00\x00\x00\x18\x00\x00\x00\x04\x00\x00\x00JSON{bad")
+    with pytest.raises(abi.PrrError, match="JSON"):
+        abi.load_gltf(str(bad))
+    # a JPEG image is refused with a clear message
+    data = bytearray(Z["file_textured.glb"].tobytes())
+    k = data.find(b"\x89PNG")
+    data[k:k + 2] = b"\xff\xd8"
+    jpg = tmp_path / "jpeg.glb"
+    jpg.write_bytes(bytes(data))
+    with pytest.raises(abi.PrrError, match="JPEG"):
+        abi.load_gltf(str(jpg))
+    with pytest.raises(abi.PrrError, match="material type"):
+        abi.load_gltf(str(assets / "left.glb"), 7)
+
+
+def test_scene_builder_gltf(assets):
+    """SceneBuilder.gltf adds one mesh per primitive with its own material and
+    shared texture rows; an override material replaces them."""
+    b = scenes.SceneBuilder()
+    m0 = b.material()
+    b.sphere((0, 0, 0), 1.0, m0)
+    objs = b.gltf(str(assets / "textured.glb"))
+    objs2 = b.gltf(str(assets / "left.glb"), material=m0)
+    sc = b.build()
+    assert len(objs) == 3 and len(objs2) == 1
+    assert sc.tri_flags.shape[0] == sc.num_triangles == 6 + 3 + 6 + 2
+    assert len(sc.textures) == 5 and sc.mat_tex.shape == (sc.materials.shape[0], 4)
+    assert sc.objects[objs2[0], 1] == m0
+    d, keep = abi.scene_desc(sc)
+    assert d.num_textures == 5 and bool(d.tri_flags)
