@@ -134,6 +134,15 @@ struct Ray {
 // ---------------------------------------------------------------------------
 enum LobeType { LOBE_DIFFUSE = 0, LOBE_SPECULAR = 1, LOBE_DELTA = 2, LOBE_ALL = 3 };
 
+// Texture (TextureUtils.hpp:9-19): 8-bit texels, rows top to bottom.
+struct Texture {
+    std::vector<unsigned char> data;
+    int width = 0;
+    int height = 0;
+    int channels = 4;
+    bool empty() const { return data.empty(); }
+};
+
 class Material {
   public:
     Vector3f m_emission = Vector3f(0.0f);
@@ -141,6 +150,12 @@ class Material {
     float roughness = 0.5f;
     float metallic = 0.0f;
     float ior = 1.5f;
+    // Material.hpp:48-51: sRGB colour, metallic-roughness (G, B), normal map,
+    // emissive (kept for the asset; castRay does not read it)
+    Texture emissiveTex;
+    Texture baseColorTex;
+    Texture normalTex;
+    Texture metallicRoughnessTex;
     virtual ~Material() = default;
     virtual int kind() const = 0;  // prr_material_type
     virtual bool isDelta() const { return false; }
@@ -193,6 +208,12 @@ class Triangle : public Object {
   public:
     Vector3f v0, v1, v2;
     Vector3f e1, e2;
+    Vector2f t0, t1, t2;        // per-vertex texture coordinates
+    Vector3f tan0, tan1, tan2;  // per-vertex tangents (xyz)
+    float tangentW0 = 1.f, tangentW1 = 1.f, tangentW2 = 1.f;  // glTF handedness
+    bool hasTangents = false;
+    Vector3f n0, n1, n2;  // per-vertex normals
+    bool hasSmoothNormals = false;
     Vector3f normal;
     float area;
     Material *m;
@@ -371,6 +392,17 @@ class Integrator {
     const Scene &scene;
     int maxDepth;
     float rrThreshold;
+};
+
+// GLTFLoader (GLTFLoader.hpp:12-37): one MeshTriangle per triangle
+// primitive, materials and textures as the reference loads them (PNG images;
+// JPEG throws). Returns heap objects the caller owns; throws
+// std::runtime_error on an unreadable or malformed file.
+enum class MaterialType { Diffuse, Glass, Mirror, Emit };
+class GLTFLoader {
+  public:
+    static std::vector<MeshTriangle *> load(const std::string &filename, MaterialType matType,
+                                            Material *overrideMat = nullptr);
 };
 
 namespace tonemap {

@@ -8,6 +8,7 @@
 
 #include "oracle_core.hpp"
 
+#include <array>
 #include <cstdio>
 #include <functional>
 
@@ -17,6 +18,11 @@ namespace orc {
 struct Tri {
     V3 v0, v1, v2, e1, e2, n;
     float area;
+    // per-vertex attributes (Triangle.hpp:16-22; GLTFLoader.cpp:244-268)
+    V3 n0, n1, n2, tan0, tan1, tan2;
+    float t0[2] = {0, 0}, t1[2] = {0, 0}, t2[2] = {0, 0};
+    float tw0 = 1.f, tw1 = 1.f, tw2 = 1.f;
+    bool smooth = false, hasTan = false;
     Tri() : area(0) {}
     Tri(const V3 &a, const V3 &b, const V3 &c) : v0(a), v1(b), v2(c) {
         e1 = v1 - v0;
@@ -60,7 +66,25 @@ struct Hit {
     int prim = -1;  // global primitive id
     int obj = -1;   // object index
     float u = 0.f, v = 0.f;
+    float tc[2] = {0.f, 0.f};  // Intersection::tcoords
+    V3 tangent;                // Intersection::tangent (hasTangent)
+    float hand = 1.f;
+    bool hasTan = false;
 };
+
+// Triangle::finalize's attribute part (Triangle.cpp:42-56)
+inline void finalizeTri(const Tri &tr, float u, float v, Hit &h) {
+    const float w0 = 1.f - u - v;
+    h.tc[0] = tr.t0[0] * w0 + tr.t1[0] * u + tr.t2[0] * v;
+    h.tc[1] = tr.t0[1] * w0 + tr.t1[1] * u + tr.t2[1] * v;
+    h.normal = tr.smooth ? normalize(tr.n0 * w0 + tr.n1 * u + tr.n2 * v) : tr.n;
+    if (tr.hasTan) {
+        h.tangent = normalize(tr.tan0 * w0 + tr.tan1 * u + tr.tan2 * v);
+        const float w = tr.tw0 * w0 + tr.tw1 * u + tr.tw2 * v;
+        h.hand = w >= 0.f ? 1.f : -1.f;
+        h.hasTan = true;
+    }
+}
 
 // Triangle::hitTest (Triangle.hpp:46-64)
 inline bool triHit(const Tri &tri, const Ray &r, float &t, float &u, float &v) {
@@ -473,8 +497,80 @@ struct PointLight {
     V3 pos, intensity;
 };
 
+// Material::buildShadingData (Material.cpp:3-26) with the glTF overload's
+// tangent frame (Material.hpp:61-74) and TextureUtils lookups.
+struct Tex {
+    int w = 0, h = 0, c = 0;
+    std::vector<uint8_t> d;
+};
+inline void uvToTexel(float u, float v, int w, int h, int &x0, int &y0, int &x1, int &y1, float &fx, float &fy) {
+    u = u - std::floor(u);  // TextureUtils.hpp:24-39
+    v = v - std::floor(v);
+    float px = u * w, py = v * h;
+    x0 = (int) px % w;
+    y0 = (int) py % h;
+    x1 = (x0 + 1) % w;
+    y1 = (y0 + 1) % h;
+    fx = px - std::floor(px);
+    fy = py - std::floor(py);
+}
+inline V3 texLookup(const Tex &t, const float uv[2], bool snorm) {
+    int x0, y0, x1, y1;
+    float fx, fy;
+    uvToTexel(uv[0], uv[1], t.w, t.h, x0, y0, x1, y1, fx, fy);
+    auto at = [&](int x, int y) {
+        const uint8_t *q = t.d.data() + ((size_t) y * t.w + x) * t.c;
+        if (snorm) return V3(q[0] / 255.f * 2.f - 1.f, q[1] / 255.f * 2.f - 1.f, q[2] / 255.f * 2.f - 1.f);
+        return V3(q[0] / 255.f, q[1] / 255.f, q[2] / 255.f);
+    };
+    V3 c00 = at(x0, y0), c10 = at(x1, y0), c01 = at(x0, y1), c11 = at(x1, y1);
+    V3 top = c00 + (c10 - c00) * fx;  // TextureUtils.hpp:61-66
+    V3 bottom = c01 + (c11 - c01) * fx;
+    return top + (bottom - top) * fy;
+}
+inline float srgbToLinear(float x) {  // TextureUtils.hpp:51-54
+    x = std::max(0.f, x);
+    return x <= 0.04045f ? x / 12.92f : std::pow((x + 0.055f) / 1.055f, 2.4f);
+}
+
 struct Scene {
     std::vector<Tri> tris;
+    std::vector<Tex> texs;
+    std::vector<std::array<int, 4>> matTex;  // per material (empty: untextured)
+
+    // ShadingData for `mat` at hit h, with Ng the normal the caller shades with
+    Shading shading(int mat, const Hit &h, const V3 &Ng) const {
+        Shading sd = makeShading(mats[mat], Ng);
+        if (matTex.empty()) return sd;
+        const std::array<int, 4> &tx = matTex[mat];
+        if (tx[0] >= 0) {
+            V3 c = texLookup(texs[tx[0]], h.tc, false);
+            sd.base = V3(srgbToLinear(c.x), srgbToLinear(c.y), srgbToLinear(c.z));
+        }
+        if (tx[1] >= 0) {
+            V3 c = texLookup(texs[tx[1]], h.tc, false);
+            sd.roughness = std::max(c.y, 0.08f);
+            sd.metallic = c.z;
+        }
+        if (tx[2] >= 0) {
+            V3 T, B;
+            bool ok = false;
+            if (h.hasTan) {
+                V3 tg = normalize(h.tangent);  // castRay normalises again (Integrator.cpp:145, 179)
+                T = tg - dot(Ng, tg) * Ng;
+                float tlen = std::sqrt(T.x * T.x + T.y * T.y + T.z * T.z);
+                if (tlen >= 1e-6f) {
+                    T = T / tlen;
+                    B = cross(Ng, T) * h.hand;
+                    ok = true;
+                }
+            }
+            if (!ok) buildTBN(Ng, T, B);
+            V3 tn = normalize(texLookup(texs[tx[2]], h.tc, true));
+            sd.N = normalize(tn.x * T + tn.y * B + tn.z * Ng);
+        }
+        return sd;
+    }
     std::vector<Sph> sphs;
     std::vector<Material> mats;
     std::vector<Obj> objs;
@@ -561,11 +657,11 @@ struct Scene {
                 h.happened = true;
                 h.t = t;
                 h.coords = r.o + t * r.d;
-                h.normal = tris[best].n;
                 h.prim = best;
                 h.obj = k;
                 h.u = bu;
                 h.v = bv;
+                finalizeTri(tris[best], bu, bv, h);
             }
         } else {
             if (cnt) cnt->spheres++;
@@ -715,15 +811,16 @@ V3 castRay(const Scene &sc, const Ray &ray0, R &rng, const IntegratorParams &ip,
         V3 hitPoint = inter.coords;
         V3 geoN = inter.normal;
         if (mat.isDelta()) {
-            BSDFSample bs = (mat.type == MAT_GLASS) ? glassSample(-cur.d, geoN, mat.ior, rng)
-                                                     : mirrorSample(-cur.d, geoN, mat.base);
+            const Shading dsd = sc.shading(sc.objs[inter.obj].mat, inter, geoN);
+            BSDFSample bs = (mat.type == MAT_GLASS) ? glassSample(-cur.d, dsd.N, mat.ior, rng)
+                                                     : mirrorSample(-cur.d, dsd.N, mat.base);
             if (bs.pdf < 1e-6f) break;
             beta = beta * bs.f / bs.pdf;
             bool trans = dot(bs.wi, geoN) < 0.f;
             V3 off = trans ? -geoN : geoN;
             cur = Ray{hitPoint + off * kEps, bs.wi};
             inter = sc.intersect(cur, cnt);
-            if (trans) beta = beta * mat.base;  // Integrator.cpp:161-165 (sd.baseColor)
+            if (trans) beta = beta * dsd.base;  // Integrator.cpp:161-165 (sd.baseColor)
             specular = true;
             continue;
         }
@@ -731,7 +828,7 @@ V3 castRay(const Scene &sc, const Ray &ray0, R &rng, const IntegratorParams &ip,
             bool camOrSpec = (bounce == 0) || specular;
             specular = false;
             V3 sN = dot(cur.d, geoN) < 0.f ? geoN : -geoN;
-            Shading sd = makeShading(mat, sN);
+            Shading sd = sc.shading(sc.objs[inter.obj].mat, inter, sN);
             V3 wo = -cur.d;
             bool indirect = bounce > 0;
             const bool scatters = (mat.type == MAT_DIFFUSE);  // EmissiveMaterial: f = pdf = 0

@@ -40,8 +40,35 @@ extern "C" orc_scene *orc_scene_create(const prr_scene_desc *d) {
     sc.tris.resize(d->num_triangles);
     for (int i = 0; i < d->num_triangles; i++) {
         const float *t = d->triangles + 9 * (size_t) i;
-        sc.tris[i] = Tri(v3(t), v3(t + 3), v3(t + 6));
+        Tri &tr = sc.tris[i] = Tri(v3(t), v3(t + 3), v3(t + 6));
+        const int fl = d->tri_flags ? d->tri_flags[i] : 0;
+        if (fl & PRR_TRI_SMOOTH_NORMALS) {  // GLTFLoader.cpp:244-251
+            const float *n = d->tri_normals + 9 * (size_t) i;
+            tr.n0 = v3(n), tr.n1 = v3(n + 3), tr.n2 = v3(n + 6);
+            tr.smooth = true;
+            tr.n = normalize(tr.n0 + tr.n1 + tr.n2);
+        }
+        if (fl & PRR_TRI_UVS) {
+            const float *q = d->tri_uvs + 6 * (size_t) i;
+            tr.t0[0] = q[0], tr.t0[1] = q[1], tr.t1[0] = q[2], tr.t1[1] = q[3], tr.t2[0] = q[4], tr.t2[1] = q[5];
+        }
+        if (fl & PRR_TRI_TANGENTS) {
+            const float *q = d->tri_tangents + 12 * (size_t) i;
+            tr.tan0 = v3(q), tr.tw0 = q[3], tr.tan1 = v3(q + 4), tr.tw1 = q[7], tr.tan2 = v3(q + 8), tr.tw2 = q[11];
+            tr.hasTan = true;
+        }
     }
+    for (int i = 0; i < d->num_textures; i++) {
+        const prr_texture_desc &t = d->textures[i];
+        Tex x;
+        x.w = t.width, x.h = t.height, x.c = t.channels;
+        x.d.assign(t.data, t.data + (size_t) t.width * t.height * t.channels);
+        sc.texs.push_back(std::move(x));
+    }
+    if (d->material_textures)
+        for (int i = 0; i < d->num_materials; i++)
+            sc.matTex.push_back({d->material_textures[4 * i], d->material_textures[4 * i + 1],
+                                 d->material_textures[4 * i + 2], d->material_textures[4 * i + 3]});
     sc.sphs.resize(d->num_spheres);
     for (int i = 0; i < d->num_spheres; i++) {
         const float *p = d->spheres + 4 * (size_t) i;
@@ -440,11 +467,12 @@ static bool tracePhoton(const Scene &sc, Ray ray, V3 power, int maxDepth, R &rng
         if (m.isEmissive()) return false;
         V3 N = h.normal;
         if (m.isDelta()) {
-            BSDFSample bs = (m.type == MAT_GLASS) ? glassSample(-ray.d, N, m.ior, rng)
-                                                   : mirrorSample(-ray.d, N, m.base);
+            const Shading sd = sc.shading(sc.objs[h.obj].mat, h, N);
+            BSDFSample bs = (m.type == MAT_GLASS) ? glassSample(-ray.d, sd.N, m.ior, rng)
+                                                   : mirrorSample(-ray.d, sd.N, m.base);
             power = power * bs.f / bs.pdf;
             bool trans = dot(bs.wi, N) < 0.f;
-            if (trans) power = power * m.base;
+            if (trans) power = power * sd.base;
             ray = Ray{h.coords + (trans ? -N : N) * kEps, bs.wi};
             caustic = true;
             continue;

@@ -35,6 +35,11 @@ class PointLightDesc(C.Structure):
     _fields_ = [("position", C.c_float * 3), ("intensity", C.c_float * 3)]
 
 
+class TextureDesc(C.Structure):
+    _fields_ = [("width", C.c_int32), ("height", C.c_int32), ("channels", C.c_int32),
+                ("data", C.POINTER(C.c_uint8))]
+
+
 class SceneDesc(C.Structure):
     _fields_ = [("num_objects", C.c_int32), ("objects", C.POINTER(ObjectDesc)),
                 ("num_triangles", C.c_int32), ("triangles", C.POINTER(C.c_float)),
@@ -42,7 +47,11 @@ class SceneDesc(C.Structure):
                 ("num_materials", C.c_int32), ("materials", C.POINTER(MaterialDesc)),
                 ("env", EnvDesc), ("background", C.c_float * 3),
                 ("num_point_lights", C.c_int32), ("point_lights", C.POINTER(PointLightDesc)),
-                ("camera", CameraDesc)]
+                ("camera", CameraDesc),
+                ("tri_flags", C.POINTER(C.c_uint8)), ("tri_normals", C.POINTER(C.c_float)),
+                ("tri_uvs", C.POINTER(C.c_float)), ("tri_tangents", C.POINTER(C.c_float)),
+                ("material_textures", C.POINTER(C.c_int32)), ("num_textures", C.c_int32),
+                ("textures", C.POINTER(TextureDesc))]
 
 
 class RenderParams(C.Structure):
@@ -101,6 +110,25 @@ def scene_desc(sc):
         d.num_point_lights = len(pl)
         d.point_lights = C.cast(pa, C.POINTER(PointLightDesc))
         keep.append(pa)
+    if getattr(sc, "tri_flags", None) is not None:
+        fl = np.ascontiguousarray(sc.tri_flags, np.uint8)
+        nrm = np.ascontiguousarray(sc.tri_normals, np.float32)
+        uvs = np.ascontiguousarray(sc.tri_uvs, np.float32)
+        tan = np.ascontiguousarray(sc.tri_tangents, np.float32)
+        d.tri_flags = fl.ctypes.data_as(C.POINTER(C.c_uint8))
+        d.tri_normals, d.tri_uvs, d.tri_tangents = _f(nrm), _f(uvs), _f(tan)
+        keep += [fl, nrm, uvs, tan]
+    if getattr(sc, "mat_tex", None) is not None:
+        mt = np.ascontiguousarray(sc.mat_tex, np.int32)
+        d.material_textures = mt.ctypes.data_as(C.POINTER(C.c_int32))
+        texs = [np.ascontiguousarray(t, np.uint8) for t in sc.textures]
+        ta = (TextureDesc * max(1, len(texs)))()
+        for i, t in enumerate(texs):
+            ta[i].height, ta[i].width, ta[i].channels = t.shape[0], t.shape[1], t.shape[2]
+            ta[i].data = t.ctypes.data_as(C.POINTER(C.c_uint8))
+        d.num_textures = len(texs)
+        d.textures = C.cast(ta, C.POINTER(TextureDesc))
+        keep += [mt, ta, texs]
     cam = np.asarray(sc.camera, np.float32)
     d.camera.eye[:] = [float(x) for x in cam[0:3]]
     d.camera.target[:] = [float(x) for x in cam[3:6]]

@@ -159,6 +159,29 @@ static void buildScene(const Bundle &b, RefScene &rs) {
     const int nMat = (int) M.n<float>() / 12;
     std::vector<Material *> mats;
     for (int i = 0; i < nMat; i++) mats.push_back(makeMaterial(M.as<float>() + 12 * i));
+    // textures: mat_tex (4 per material: base colour, metallic-roughness, normal, emissive)
+    if (b.count("mat_tex")) {
+        const int32_t *mt = b.at("mat_tex").as<int32_t>();
+        const int32_t *whc = b.at("tex_whc").as<int32_t>();
+        const uint8_t *px = b.at("texels").as<uint8_t>();
+        std::vector<size_t> off(1, 0);
+        for (size_t t = 0; t < b.at("tex_whc").n<int32_t>() / 3; t++)
+            off.push_back(off.back() + (size_t) whc[3 * t] * whc[3 * t + 1] * whc[3 * t + 2]);
+        for (int i = 0; i < nMat; i++) {
+            Texture *slots[4] = {&mats[i]->baseColorTex, &mats[i]->metallicRoughnessTex, &mats[i]->normalTex,
+                                 &mats[i]->emissiveTex};
+            for (int j = 0; j < 4; j++) {
+                const int t = mt[4 * i + j];
+                if (t < 0) continue;
+                slots[j]->width = whc[3 * t];
+                slots[j]->height = whc[3 * t + 1];
+                slots[j]->channels = whc[3 * t + 2];
+                slots[j]->data.assign(px + off[t], px + off[t + 1]);
+            }
+        }
+    }
+    const int32_t *tfl = b.count("tri_flags") ? b.at("tri_flags").as<int32_t>() : nullptr;
+    const float *tat = b.count("tri_attr") ? b.at("tri_attr").as<float>() : nullptr;
     rs.numTris = (int) T.n<float>() / 9;
     const float *sp = b.count("spheres") ? b.at("spheres").as<float>() : nullptr;
     for (int k = 0; k < nObj; k++) {
@@ -176,8 +199,33 @@ static void buildScene(const Bundle &b, RefScene &rs) {
                     mx = Vector3f(std::max(mx.x, p.x), std::max(mx.y, p.y), std::max(mx.z, p.z));
                 }
                 tris.emplace_back(v[0], v[1], v[2], mt);
-                tris.back().t1 = Vector2f(1.f, 0.f);  // tcoords = (u, v)
-                tris.back().t2 = Vector2f(0.f, 1.f);
+                Triangle &tr = tris.back();
+                if (!tat) {
+                    tr.t1 = Vector2f(1.f, 0.f);  // tcoords = (u, v)
+                    tr.t2 = Vector2f(0.f, 1.f);
+                } else {  // GLTFLoader.cpp:244-268: n0..n2 | t0..t2 | tangent xyz w x3
+                    const size_t g = (size_t) (o[2] + i);
+                    const float *a = tat + 27 * g;
+                    if (tfl[g] & 1) {
+                        tr.n0 = Vector3f(a[0], a[1], a[2]);
+                        tr.n1 = Vector3f(a[3], a[4], a[5]);
+                        tr.n2 = Vector3f(a[6], a[7], a[8]);
+                        tr.hasSmoothNormals = true;
+                        tr.normal = normalize(tr.n0 + tr.n1 + tr.n2);
+                    }
+                    tr.t0 = Vector2f(a[9], a[10]);
+                    tr.t1 = Vector2f(a[11], a[12]);
+                    tr.t2 = Vector2f(a[13], a[14]);
+                    if (tfl[g] & 2) {
+                        tr.tan0 = Vector3f(a[15], a[16], a[17]);
+                        tr.tangentW0 = a[18];
+                        tr.tan1 = Vector3f(a[19], a[20], a[21]);
+                        tr.tangentW1 = a[22];
+                        tr.tan2 = Vector3f(a[23], a[24], a[25]);
+                        tr.tangentW2 = a[26];
+                        tr.hasTangents = true;
+                    }
+                }
             }
             MeshTriangle *m = new MeshTriangle(tris, Bounds3(mn, mx), mt);
             for (int i = 0; i < (int) m->triangles.size(); i++) rs.primId[&m->triangles[i]] = o[2] + i;

@@ -416,6 +416,12 @@ int prr_scene_create(prr_context *c, const prr_scene_desc *d, prr_scene **out) {
     UP(v.env_marg, h.envMarg);
     UP(v.env_gm, h.envGuideM);
     UP(v.env_gc, h.envGuideC);
+    if (!h.tattr.empty()) UP(v.tattr, h.tattr);
+    if (!h.matTex.empty()) {
+        UP(v.mat_tex, h.matTex);
+        UP(v.tex_info, h.texInfo);
+        UP(v.texels, h.texels);
+    }
 #undef UP
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) {

@@ -155,6 +155,15 @@ constexpr int kLeafQHead = 2;
 
 enum MatType { MAT_DIFFUSE = 0, MAT_GLASS = 1, MAT_MIRROR = 2, MAT_EMISSIVE = 3 };
 
+// prim_n.w holds the material row in its low bits and the triangle's
+// per-vertex attribute flags (PRR_TRI_*) above kPrimFlagShift.
+constexpr int kPrimFlagShift = 28;
+constexpr int kPrimMatMask = (1 << kPrimFlagShift) - 1;
+constexpr int kPrimSmooth = 1, kPrimTangents = 2;
+// Per-triangle attribute record (float4 rows): n0.xyz n1.x | n1.yz n2.xy |
+// n2.z t0.xy t1.x | t1.y t2.xy - | tangent0.xyz w0 | tangent1 w1 | tangent2 w2
+constexpr int kTriAttrRows = 7;
+
 struct __align__(16) DMaterial {
     float4 base_type;   // base rgb, type (as float)
     float4 params;      // roughness (raw), metallic, ior, hasEmission (0/1)
@@ -232,6 +241,11 @@ struct SceneView {
     uint32_t ph_mask;           // buckets - 1 (power of two)
     float ph_radius, ph_cell;
     CamView cam;  // the scene's camera
+    // per-vertex attributes and textures (null: none)
+    const float4 *tattr;        // kTriAttrRows per triangle
+    const int4 *mat_tex;        // per material: base colour, metallic-roughness, normal, emissive rows (-1 none)
+    const int4 *tex_info;       // per texture: first texel byte, width, height, channels
+    const uint8_t *texels;
 };
 
 // ---------------------------------------------------------------------------
@@ -254,6 +268,119 @@ PRR_DEV Frame makeFrame(const V3 &N) {
     }
     f.B = cross(N, f.T);
     return f;
+}
+
+// ---------------------------------------------------------------------------
+// Per-vertex attributes and textures (Triangle.cpp:34-59, TextureUtils.hpp,
+// Material.cpp:3-26)
+// ---------------------------------------------------------------------------
+// Interpolated shading normal of a triangle with per-vertex normals:
+// normalize(n0 * w0 + n1 * u + n2 * v), w0 = 1 - u - v (Triangle.cpp:47-50).
+PRR_DEV V3 smoothNormal(const SceneView &s, int prim, float u, float v) {
+    const float4 *a = s.tattr + (int64_t) kTriAttrRows * prim;
+    const float4 r0 = a[0], r1 = a[1], r2 = a[2];
+    const float w0 = 1.f - u - v;
+    return normalize(V3(r0.x, r0.y, r0.z) * w0 + V3(r0.w, r1.x, r1.y) * u + V3(r1.z, r1.w, r2.x) * v);
+}
+
+// TextureUtils::uvToTexel (TextureUtils.hpp:24-39); a NaN coordinate reads texel 0.
+PRR_DEV void uvToTexel(float u, float v, int w, int h, int &x0, int &y0, int &x1, int &y1, float &fx, float &fy) {
+    u = u - floorf(u);
+    v = v - floorf(v);
+    float px = u * (float) w, py = v * (float) h;
+    if (!(px >= 0.f)) px = 0.f;
+    if (!(py >= 0.f)) py = 0.f;
+    x0 = min((int) px, w) % w;
+    y0 = min((int) py, h) % h;
+    x1 = (x0 + 1) % w;
+    y1 = (y0 + 1) % h;
+    fx = px - floorf(px);
+    fy = py - floorf(py);
+}
+
+// Bilinear lookup of channels 0..2, each texel read as b / 255, or as
+// b / 255 * 2 - 1 for a normal map (TextureUtils.hpp:41-103).
+template <bool SNORM>
+PRR_DEV V3 texBilerp(const SceneView &s, int t, float u, float v) {
+    const int4 ti = s.tex_info[t];
+    int x0, y0, x1, y1;
+    float fx, fy;
+    uvToTexel(u, v, ti.y, ti.z, x0, y0, x1, y1, fx, fy);
+    const uint8_t *px = s.texels + ti.x;
+    auto at = [&](int x, int y) {
+        const uint8_t *q = px + ((int64_t) y * ti.y + x) * ti.w;
+        if (SNORM) return V3(q[0] / 255.f * 2.f - 1.f, q[1] / 255.f * 2.f - 1.f, q[2] / 255.f * 2.f - 1.f);
+        return V3(q[0] / 255.f, q[1] / 255.f, q[2] / 255.f);
+    };
+    const V3 c00 = at(x0, y0), c10 = at(x1, y0), c01 = at(x0, y1), c11 = at(x1, y1);
+    const V3 top = c00 + (c10 - c00) * fx;
+    const V3 bottom = c01 + (c11 - c01) * fx;
+    return top + (bottom - top) * fy;
+}
+
+PRR_DEV float srgbToLinear(float x) {  // TextureUtils.hpp:51-54
+    x = fmaxf(0.f, x);
+    return x <= 0.04045f ? x / 12.92f : powf((x + 0.055f) / 1.055f, 2.4f);
+}
+
+// Material::buildShadingData for a textured material (params.w != 0):
+// base colour (sRGB -> linear), roughness (G) and metallic (B) replace the
+// material's constants; a normal map tilts N in the frame of the asset's
+// tangent (Gram-Schmidt against N, bitangent cross(N, T) * handedness;
+// Material.hpp:61-74) or, without one, of buildTBN(N). `N` is the normal the
+// caller shades with (the hit normal, or its flip towards the viewer).
+struct ShadeData {
+    V3 base;
+    float rough, metal;
+    V3 N;
+};
+PRR_DEV void texturedShading(const SceneView &s, int mat, int prim, int pflags, float u, float v, ShadeData &sd) {
+    const int4 tx = s.mat_tex[mat];
+    float tu = 0.f, tv = 0.f;  // Intersection::tcoords (spheres and rows without uvs: 0)
+    const bool tri = prim < s.num_tris && s.tattr != nullptr;
+    const float4 *a = tri ? s.tattr + (int64_t) kTriAttrRows * prim : nullptr;
+    const float w0 = 1.f - u - v;
+    if (tri) {
+        const float4 r2 = a[2], r3 = a[3];
+        tu = r2.y * w0 + r2.w * u + r3.y * v;
+        tv = r2.z * w0 + r3.x * u + r3.z * v;
+    }
+    if (tx.x >= 0) {
+        const V3 c = texBilerp<false>(s, tx.x, tu, tv);
+        sd.base = V3(srgbToLinear(c.x), srgbToLinear(c.y), srgbToLinear(c.z));
+    }
+    if (tx.y >= 0) {
+        const V3 c = texBilerp<false>(s, tx.y, tu, tv);  // G = roughness, B = metallic
+        sd.rough = c.y;
+        sd.metal = c.z;
+    }
+    if (tx.z >= 0) {
+        const V3 Ng = sd.N;
+        V3 T, B;
+        bool haveT = false;
+        if (tri && (pflags & kPrimTangents)) {
+            const float4 t0 = a[4], t1 = a[5], t2 = a[6];
+            // Triangle::finalize normalises the interpolated tangent, castRay again
+            const V3 tg = normalize(normalize(V3(t0.x, t0.y, t0.z) * w0 + V3(t1.x, t1.y, t1.z) * u +
+                                              V3(t2.x, t2.y, t2.z) * v));
+            const float hw = t0.w * w0 + t1.w * u + t2.w * v;
+            const float hand = hw >= 0.f ? 1.f : -1.f;
+            T = tg - dot(Ng, tg) * Ng;
+            const float tlen = sqrtf(T.x * T.x + T.y * T.y + T.z * T.z);
+            if (tlen >= 1e-6f) {
+                T = T / tlen;
+                B = cross(Ng, T) * hand;
+                haveT = true;
+            }
+        }
+        if (!haveT) {
+            const Frame f = makeFrame(Ng);
+            T = f.T;
+            B = f.B;
+        }
+        const V3 tn = normalize(texBilerp<true>(s, tx.z, tu, tv));
+        sd.N = normalize(tn.x * T + tn.y * B + tn.z * Ng);
+    }
 }
 PRR_DEV V3 toLocal(const Frame &f, const V3 &w) { return V3(dot(w, f.T), dot(w, f.B), dot(w, f.N)); }
 PRR_DEV V3 toWorld(const Frame &f, const V3 &l) { return l.x * f.T + l.y * f.B + l.z * f.N; }

@@ -47,12 +47,13 @@ __global__ __launch_bounds__(kTraceBlock) void kPhotonTrace(SceneView s, int64_t
             const HitOut h = traceClosest(s, o, d, st);
             if (h.prim < 0) break;
             const float4 pn = s.prim_n[h.prim];
-            const DMaterial m = s.mats[__float_as_int(pn.w)];
+            const int bits = __float_as_int(pn.w), pflags = bits >> kPrimFlagShift;
+            const DMaterial m = s.mats[bits & kPrimMatMask];
             const int mtype = (int) m.base_type.w;
             V3 P, N;
             if (h.prim < s.num_tris) {
                 P = o + h.t * d;
-                N = ld3(pn);
+                N = (pflags & kPrimSmooth) ? smoothNormal(s, h.prim, h.u, h.v) : ld3(pn);
             } else {
                 const float4 sc = s.sph_by_id[h.prim - s.num_tris];
                 P = V3(o + d * h.t);
@@ -61,11 +62,13 @@ __global__ __launch_bounds__(kTraceBlock) void kPhotonTrace(SceneView s, int64_t
             if (mtype == MAT_EMISSIVE) break;
             if (mtype == MAT_GLASS || mtype == MAT_MIRROR) {
                 const V3 base = ld3(m.base_type);
-                const BsdfSample bs = mtype == MAT_GLASS ? sampleGlass(-d, N, m.params.z, rng.next())
-                                                         : sampleMirror(-d, N, base);
+                ShadeData sd{base, m.params.x, m.params.y, N};
+                if (m.params.w != 0.f) texturedShading(s, bits & kPrimMatMask, h.prim, pflags, h.u, h.v, sd);
+                const BsdfSample bs = mtype == MAT_GLASS ? sampleGlass(-d, sd.N, m.params.z, rng.next())
+                                                         : sampleMirror(-d, sd.N, base);
                 power = power * bs.f / bs.pdf;
                 const bool trans = dot(bs.wi, N) < 0.f;
-                if (trans) power = power * base;
+                if (trans) power = power * sd.base;
                 o = P + (trans ? -N : N) * kEps;
                 d = bs.wi;
                 caustic = true;

@@ -302,16 +302,19 @@ __global__ __launch_bounds__(kTraceBlock) void kShadow(SceneView s, PathBufs pb,
 
 // ---------------------------------------------------------------------------
 struct HitGeom {
-    V3 P, Ng;
+    V3 P, Ng;  // Ng: Intersection::normal (per-vertex normals interpolated where the triangle has them)
     int mat;
+    int flags;
 };
-PRR_DEV HitGeom hitGeom(const SceneView &s, int prim, float t, const V3 &o, const V3 &d) {
+PRR_DEV HitGeom hitGeom(const SceneView &s, int prim, float t, const V3 &o, const V3 &d, float u, float v) {
     HitGeom g;
     const float4 pn = s.prim_n[prim];
-    g.mat = __float_as_int(pn.w);
+    const int bits = __float_as_int(pn.w);
+    g.mat = bits & kPrimMatMask;
+    g.flags = bits >> kPrimFlagShift;
     if (prim < s.num_tris) {
         g.P = o + t * d;  // Triangle::finalize (Triangle.cpp:34-59)
-        g.Ng = ld3(pn);
+        g.Ng = (g.flags & kPrimSmooth) ? smoothNormal(s, prim, u, v) : ld3(pn);
     } else {
         const float4 sc = s.sph_by_id[prim - s.num_tris];
         g.P = V3(o + d * t);  // Sphere.hpp:34-35
@@ -452,7 +455,7 @@ __device__ __forceinline__ void shadePath(const SceneView &s, const PathBufs &pb
             }
             done = true;
         } else {
-            const HitGeom g = hitGeom(s, prim, h.x, o, d);
+            const HitGeom g = hitGeom(s, prim, h.x, o, d, h.y, h.z);
             const DMaterial hm = s.mats[g.mat];
             if ((int) hm.base_type.w == MAT_EMISSIVE) {  // Integrator.cpp:231-242
                 const float cosT = smax(0.f, dot(-d, normalized(g.Ng)));
@@ -494,10 +497,12 @@ __device__ __forceinline__ void shadePath(const SceneView &s, const PathBufs &pb
         if (prim < 0) {
             done = true;
         } else {
-            const HitGeom g = hitGeom(s, prim, h.x, o, d);
+            const HitGeom g = hitGeom(s, prim, h.x, o, d, h.y, h.z);
             const DMaterial m = s.mats[g.mat];
             const int mtype = (int) m.base_type.w;
             const V3 base = ld3(m.base_type);
+            // Material::buildShadingData (Material.cpp:3-26): constants, or the textures
+            ShadeData sd{base, m.params.x, m.params.y, g.Ng};
             if (mtype == MAT_EMISSIVE) {
                 if (bounce == 0 || spec) {
                     const V3 c = beta * ld3(m.emission);
@@ -506,12 +511,13 @@ __device__ __forceinline__ void shadePath(const SceneView &s, const PathBufs &pb
                 // scatters nothing (EmissiveMaterial: f = pdf = 0)
                 done = true;
             } else if (mtype == MAT_GLASS || mtype == MAT_MIRROR) {
-                const BsdfSample bs = mtype == MAT_GLASS ? sampleGlass(-d, g.Ng, m.params.z, rng.next())
-                                                         : sampleMirror(-d, g.Ng, base);
+                if (m.params.w != 0.f) texturedShading(s, g.mat, prim, g.flags, h.y, h.z, sd);
+                const BsdfSample bs = mtype == MAT_GLASS ? sampleGlass(-d, sd.N, m.params.z, rng.next())
+                                                         : sampleMirror(-d, sd.N, base);
                 beta = beta * bs.f / bs.pdf;
                 const bool trans = dot(bs.wi, g.Ng) < 0.f;
                 const V3 off = trans ? -g.Ng : g.Ng;
-                if (trans) beta = beta * base;  // Integrator.cpp:161-165
+                if (trans) beta = beta * sd.base;  // Integrator.cpp:161-165
                 fl |= F_SPEC;
                 if (bounce + 1 >= ci.maxDepth) {
                     done = true;  // the loop ends before this ray is used
@@ -522,7 +528,9 @@ __device__ __forceinline__ void shadePath(const SceneView &s, const PathBufs &pb
                 }
             } else {
                 fl &= ~F_SPEC;
-                const V3 sN = dot(d, g.Ng) < 0.f ? g.Ng : -g.Ng;
+                sd.N = dot(d, g.Ng) < 0.f ? g.Ng : -g.Ng;
+                if (m.params.w != 0.f) texturedShading(s, g.mat, prim, g.flags, h.y, h.z, sd);
+                const V3 sN = sd.N;
                 const V3 wo = -d;
                 const CacheRec &cr = ci.cache;
                 bool cacheQuery = false;
@@ -534,14 +542,14 @@ __device__ __forceinline__ void shadePath(const SceneView &s, const PathBufs &pb
                         cacheQuery = true;
                         done = true;
                         const uint32_t k = waveAppend(cr.qCount, true);
-                        putQuery(cr.q + (int64_t) k * kCacheQueryFloats, g.P, wo, sN, m.params.x, base);
+                        putQuery(cr.q + (int64_t) k * kCacheQueryFloats, g.P, wo, sN, sd.rough, sd.base);
                         cr.qSlot[k] = slot;
                         cr.qBeta[k] = st4(beta, 0.f);
                     } else if ((fl & F_TRAIN) && bounce >= 1 && min3(beta) > 1e-3f) {
                         // target later: (final L - L so far) / beta
                         const uint32_t k = waveAppend(cr.tCount, true);
                         if (k < cr.tCap) {
-                            putQuery(cr.tq + (int64_t) k * kCacheQueryFloats, g.P, wo, sN, m.params.x, base);
+                            putQuery(cr.tq + (int64_t) k * kCacheQueryFloats, g.P, wo, sN, sd.rough, sd.base);
                             cr.tSlot[k] = slot;
                             cr.tBeta[k] = st4(beta, 0.f);
                             cr.tLpre[k] = st4(L, 0.f);
@@ -550,7 +558,7 @@ __device__ __forceinline__ void shadePath(const SceneView &s, const PathBufs &pb
                 }
                 if (!cacheQuery) {
                 const Frame fr = makeFrame(sN);
-                const Lobes lb = makeLobes(base, m.params.x, m.params.y, m.params.z);
+                const Lobes lb = makeLobes(sd.base, sd.rough, sd.metal, m.params.z);
                 const bool ind = bounce > 0;
                 // NEE area (Integrator.cpp:14-34, 78-87)
                 if (s.total_emit_area > 0.f) {

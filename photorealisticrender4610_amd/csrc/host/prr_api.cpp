@@ -212,6 +212,87 @@ Scene &Scene::operator=(Scene &&o) noexcept {
 }
 
 // ---------------------------------------------------------------------------
+// GLTFLoader::load (GLTFLoader.cpp:140-281) over prr_gltf_load: one
+// MeshTriangle per mesh of the result, each with a material of the requested
+// class holding the loaded factors and textures, or `overrideMat`.
+std::vector<MeshTriangle *> GLTFLoader::load(const std::string &filename, MaterialType matType,
+                                             Material *overrideMat) {
+    const int type = matType == MaterialType::Glass    ? PRR_MAT_GLASS
+                     : matType == MaterialType::Mirror ? PRR_MAT_MIRROR
+                     : matType == MaterialType::Emit   ? PRR_MAT_EMISSIVE
+                                                       : PRR_MAT_DIFFUSE;
+    prr_gltf_data *g = nullptr;
+    if (prr_gltf_load(filename.c_str(), type, &g) != PRR_OK)
+        throw std::runtime_error(std::string("GLTFLoader::load: ") + prr_last_error(nullptr));
+    std::unique_ptr<prr_gltf_data, void (*)(prr_gltf_data *)> own(g, prr_gltf_free);
+    auto tex = [&](int row, Texture &out) {
+        if (row < 0) return;
+        const prr_texture_desc &t = g->textures[row];
+        out.width = t.width;
+        out.height = t.height;
+        out.channels = t.channels;
+        out.data.assign(t.data, t.data + (size_t) t.width * t.height * t.channels);
+    };
+    std::vector<MeshTriangle *> out;
+    for (int k = 0; k < g->num_meshes; k++) {
+        Material *mat = overrideMat;
+        if (!mat) {
+            const prr_material_desc &md = g->materials[g->mesh_material[k]];
+            const int32_t *tx = g->material_textures + 4 * g->mesh_material[k];
+            Material *m = md.type == PRR_MAT_GLASS    ? (Material *) new GlassMaterial()
+                          : md.type == PRR_MAT_MIRROR ? (Material *) new MirrorMaterial()
+                          : md.type == PRR_MAT_EMISSIVE ? (Material *) new EmissiveMaterial()
+                                                        : (Material *) new DiffuseMaterial();
+            m->baseColor = Vector3f(md.base_color[0], md.base_color[1], md.base_color[2]);
+            m->roughness = md.roughness;
+            m->metallic = md.metallic;
+            m->ior = md.ior;
+            m->m_emission = Vector3f(md.emission[0], md.emission[1], md.emission[2]);
+            tex(tx[0], m->baseColorTex);
+            tex(tx[1], m->metallicRoughnessTex);
+            tex(tx[2], m->normalTex);
+            tex(tx[3], m->emissiveTex);
+            mat = m;
+        }
+        std::vector<Triangle> tris;
+        Vector3f mn(std::numeric_limits<float>::infinity()), mx(-std::numeric_limits<float>::infinity());
+        for (int i = g->mesh_first[k]; i < g->mesh_first[k] + g->mesh_count[k]; i++) {
+            const float *p = g->positions + 9 * (size_t) i;
+            Triangle t(Vector3f(p[0], p[1], p[2]), Vector3f(p[3], p[4], p[5]), Vector3f(p[6], p[7], p[8]), mat);
+            for (const Vector3f &v : {t.v0, t.v1, t.v2}) {
+                mn = Vector3f(std::min(mn.x, v.x), std::min(mn.y, v.y), std::min(mn.z, v.z));
+                mx = Vector3f(std::max(mx.x, v.x), std::max(mx.y, v.y), std::max(mx.z, v.z));
+            }
+            const uint8_t fl = g->tri_flags[i];
+            if (fl & PRR_TRI_SMOOTH_NORMALS) {
+                const float *n = g->normals + 9 * (size_t) i;
+                t.n0 = Vector3f(n[0], n[1], n[2]);
+                t.n1 = Vector3f(n[3], n[4], n[5]);
+                t.n2 = Vector3f(n[6], n[7], n[8]);
+                t.hasSmoothNormals = true;
+                t.normal = normalize(t.n0 + t.n1 + t.n2);
+            }
+            const float *q = g->uvs + 6 * (size_t) i;
+            t.t0 = Vector2f(q[0], q[1]);
+            t.t1 = Vector2f(q[2], q[3]);
+            t.t2 = Vector2f(q[4], q[5]);
+            if (fl & PRR_TRI_TANGENTS) {
+                const float *a = g->tangents + 12 * (size_t) i;
+                t.tan0 = Vector3f(a[0], a[1], a[2]);
+                t.tangentW0 = a[3];
+                t.tan1 = Vector3f(a[4], a[5], a[6]);
+                t.tangentW1 = a[7];
+                t.tan2 = Vector3f(a[8], a[9], a[10]);
+                t.tangentW2 = a[11];
+                t.hasTangents = true;
+            }
+            tris.push_back(t);
+        }
+        out.push_back(new MeshTriangle(std::move(tris), Bounds3(mn, mx), mat));
+    }
+    return out;
+}
+
 // MeshTriangle from a Wavefront OBJ file: "v x y z" and "f a b c ..." records
 // (vertex index first in each a/b/c group, 1-based, negative = relative to the
 // end); faces become triangle fans in file order.
@@ -282,14 +363,33 @@ void Scene::buildBVH() {
     std::vector<prr_material_desc> mats;
     std::unordered_map<const Material *, int> matId;
     std::vector<std::pair<Object *, int>> sphereObjs;  // (object, material)
+    std::vector<uint8_t> triFlags;
+    std::vector<float> triNormals, triUvs, triTan;
+    bool anyAttr = false;
     D.primObj.clear();
     D.primMat.clear();
     D.primNormal.clear();
     D.primSphere.clear();
+    std::vector<int32_t> matTex;
+    std::vector<prr_texture_desc> texs;
+    std::unordered_map<const Texture *, int> texId;
+    bool anyTex = false;
+    auto texture = [&](const Texture &t) -> int {
+        if (t.empty()) return -1;
+        auto it = texId.find(&t);
+        if (it != texId.end()) return it->second;
+        if ((size_t) t.width * t.height * t.channels != t.data.size())
+            throw std::runtime_error("Scene::buildBVH: texture size does not match its data");
+        texs.push_back(prr_texture_desc{t.width, t.height, t.channels, t.data.data()});
+        anyTex = true;
+        return texId[&t] = (int) texs.size() - 1;
+    };
     auto material = [&](Material *m) -> int {
         if (!m) throw std::runtime_error("Scene::buildBVH: object without material");
         auto it = matId.find(m);
         if (it != matId.end()) return it->second;
+        matTex.insert(matTex.end(), {texture(m->baseColorTex), texture(m->metallicRoughnessTex),
+                                     texture(m->normalTex), texture(m->emissiveTex)});
         prr_material_desc d{};
         d.type = m->kind();
         d.base_color[0] = m->baseColor.x;
@@ -317,6 +417,17 @@ void Scene::buildBVH() {
                     tris.push_back(v[k]->y);
                     tris.push_back(v[k]->z);
                 }
+                // per-vertex attributes (Triangle.hpp:16-22)
+                uint8_t fl = PRR_TRI_UVS;
+                if (t.hasSmoothNormals) fl |= PRR_TRI_SMOOTH_NORMALS;
+                if (t.hasTangents) fl |= PRR_TRI_TANGENTS;
+                anyAttr |= t.hasSmoothNormals || t.hasTangents || t.t0.x != 0.f || t.t0.y != 0.f ||
+                           t.t1.x != 0.f || t.t1.y != 0.f || t.t2.x != 0.f || t.t2.y != 0.f;
+                triFlags.push_back(fl);
+                triNormals.insert(triNormals.end(), {t.n0.x, t.n0.y, t.n0.z, t.n1.x, t.n1.y, t.n1.z, t.n2.x, t.n2.y, t.n2.z});
+                triUvs.insert(triUvs.end(), {t.t0.x, t.t0.y, t.t1.x, t.t1.y, t.t2.x, t.t2.y});
+                triTan.insert(triTan.end(), {t.tan0.x, t.tan0.y, t.tan0.z, t.tangentW0, t.tan1.x, t.tan1.y, t.tan1.z,
+                                             t.tangentW1, t.tan2.x, t.tan2.y, t.tan2.z, t.tangentW2});
                 D.primObj.push_back(&t);
                 D.primMat.push_back(mesh->m);
                 D.primNormal.push_back(t.normal);
@@ -355,6 +466,17 @@ void Scene::buildBVH() {
     d.spheres = sph.data();
     d.num_materials = (int32_t) mats.size();
     d.materials = mats.data();
+    if (anyAttr) {
+        d.tri_flags = triFlags.data();
+        d.tri_normals = triNormals.data();
+        d.tri_uvs = triUvs.data();
+        d.tri_tangents = triTan.data();
+    }
+    if (anyTex) {
+        d.material_textures = matTex.data();
+        d.num_textures = (int32_t) texs.size();
+        d.textures = texs.data();
+    }
     if (envMap.loaded) {
         env.resize(3 * envMap.pixels.size());
         for (size_t i = 0; i < envMap.pixels.size(); i++) {

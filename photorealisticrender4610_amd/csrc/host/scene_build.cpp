@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <atomic>
 #include <cmath>
+#include <cstring>
 #include <limits>
 #include <thread>
 
@@ -572,6 +573,51 @@ bool buildHostScene(const prr_scene_desc *d, HostScene &S, std::string &err) {
             return false;
         }
     }
+    if (d->num_materials >= (1 << kPrimFlagShift)) {
+        err = "too many materials";
+        return false;
+    }
+    if (d->tri_flags && d->num_triangles > 0) {
+        uint8_t any = 0;
+        for (int i = 0; i < d->num_triangles; i++) any |= d->tri_flags[i];
+        if (any & ~(PRR_TRI_SMOOTH_NORMALS | PRR_TRI_TANGENTS | PRR_TRI_UVS)) {
+            err = "tri_flags: unknown bits";
+            return false;
+        }
+        if (((any & PRR_TRI_SMOOTH_NORMALS) && !d->tri_normals) || ((any & PRR_TRI_UVS) && !d->tri_uvs) ||
+            ((any & PRR_TRI_TANGENTS) && !d->tri_tangents)) {
+            err = "tri_flags: attribute array missing";
+            return false;
+        }
+    }
+    if (d->num_textures < 0 || (d->num_textures > 0 && !d->textures)) {
+        err = "bad texture list";
+        return false;
+    }
+    for (int i = 0; i < d->num_textures; i++) {
+        const prr_texture_desc &t = d->textures[i];
+        if (t.width <= 0 || t.height <= 0 || t.channels < 1 || !t.data ||
+            (int64_t) t.width * t.height * t.channels > ((int64_t) 1 << 31)) {
+            err = "texture " + std::to_string(i) + ": bad size or data";
+            return false;
+        }
+    }
+    if (d->material_textures) {
+        for (int i = 0; i < d->num_materials; i++)
+            for (int j = 0; j < 4; j++) {
+                const int t = d->material_textures[4 * i + j];
+                if (t < -1 || t >= d->num_textures) {
+                    err = "material " + std::to_string(i) + ": texture row out of range";
+                    return false;
+                }
+                // colour, metallic-roughness and normal lookups read three channels
+                if (t >= 0 && j < 3 && d->textures[t].channels < 3) {
+                    err = "material " + std::to_string(i) + ": texture " + std::to_string(t) +
+                          " needs at least 3 channels";
+                    return false;
+                }
+            }
+    }
     if (d->env.width < 0 || d->env.height < 0 || (d->env.width > 0 && (d->env.height <= 0 || !d->env.pixels))) {
         err = "bad environment map";
         return false;
@@ -656,7 +702,9 @@ bool buildHostScene(const prr_scene_desc *d, HostScene &S, std::string &err) {
         if (o.kind == PRR_OBJ_MESH) {
             for (int i = 0; i < o.count; i++) {
                 const TriPre &t = tp[o.first + i];
-                S.primN[o.first + i] = make_float4(t.n.x, t.n.y, t.n.z, mbits);
+                const int fl = d->tri_flags ? d->tri_flags[o.first + i] : 0;
+                S.primN[o.first + i] =
+                    make_float4(t.n.x, t.n.y, t.n.z, __builtin_bit_cast(float, o.material | fl << kPrimFlagShift));
             }
         } else {
             S.primN[(size_t) d->num_triangles + o.first] = make_float4(0, 0, 0, mbits);
@@ -693,14 +741,51 @@ bool buildHostScene(const prr_scene_desc *d, HostScene &S, std::string &err) {
     if (S.nodes.empty()) S.nodes.emplace_back();
     if (S.sphById.empty()) S.sphById.push_back(make_float4(0, 0, 0, 0));
 
-    // materials
+    // per-vertex attributes: n0 n1 n2 | t0 t1 t2 | tangent xyz + w per vertex
+    S.tattr.clear();
+    if (d->tri_flags) {
+        S.tattr.resize((size_t) kTriAttrRows * d->num_triangles);
+        for (int64_t i = 0; i < d->num_triangles; i++) {
+            const int fl = d->tri_flags[i];
+            float a[28] = {0};
+            if (fl & PRR_TRI_SMOOTH_NORMALS) memcpy(a, d->tri_normals + 9 * i, 9 * sizeof(float));
+            if (fl & PRR_TRI_UVS) memcpy(a + 9, d->tri_uvs + 6 * i, 6 * sizeof(float));
+            if (fl & PRR_TRI_TANGENTS) memcpy(a + 16, d->tri_tangents + 12 * i, 12 * sizeof(float));
+            for (int r = 0; r < kTriAttrRows; r++)
+                S.tattr[(size_t) kTriAttrRows * i + r] = make_float4(a[4 * r], a[4 * r + 1], a[4 * r + 2], a[4 * r + 3]);
+        }
+    }
+    // textures: one byte pool
+    S.texInfo.clear();
+    S.texels.clear();
+    for (int i = 0; i < d->num_textures; i++) {
+        const prr_texture_desc &t = d->textures[i];
+        const size_t n = (size_t) t.width * t.height * t.channels;
+        if (S.texels.size() + n > ((size_t) 1 << 31)) {
+            err = "textures exceed 2 GiB";
+            return false;
+        }
+        S.texInfo.push_back(make_int4((int) S.texels.size(), t.width, t.height, t.channels));
+        S.texels.insert(S.texels.end(), t.data, t.data + n);
+    }
+    // materials; params.w = 1 when a colour, metallic-roughness or normal
+    // texture is attached (the emissive texture is not read by castRay)
     S.mats.resize(d->num_materials);
+    S.matTex.clear();
+    bool textured = false;
     for (int i = 0; i < d->num_materials; i++) {
         const prr_material_desc &m = d->materials[i];
+        const int4 tx = d->material_textures ? make_int4(d->material_textures[4 * i], d->material_textures[4 * i + 1],
+                                                         d->material_textures[4 * i + 2], d->material_textures[4 * i + 3])
+                                             : make_int4(-1, -1, -1, -1);
+        const bool tex = tx.x >= 0 || tx.y >= 0 || tx.z >= 0;
+        textured |= tex;
         S.mats[i].base_type = make_float4(m.base_color[0], m.base_color[1], m.base_color[2], (float) m.type);
-        S.mats[i].params = make_float4(m.roughness, m.metallic, m.ior, hasEmission(m) ? 1.f : 0.f);
+        S.mats[i].params = make_float4(m.roughness, m.metallic, m.ior, tex ? 1.f : 0.f);
         S.mats[i].emission = make_float4(m.emission[0], m.emission[1], m.emission[2], 0.f);
+        S.matTex.push_back(tx);
     }
+    if (!textured) S.matTex.clear();
     // emitters (Scene::buildBVH, Scene.cpp:19-25) and light trees
     S.totalEmitArea = 0.f;
     S.emitters.clear();
@@ -739,7 +824,15 @@ bool buildHostScene(const prr_scene_desc *d, HostScene &S, std::string &err) {
                 S.ltri.push_back(make_float4(t.v0.x, t.v0.y, t.v0.z, t.area));
                 S.ltri.push_back(make_float4(t.v1.x, t.v1.y, t.v1.z, 0.f));
                 S.ltri.push_back(make_float4(t.v2.x, t.v2.y, t.v2.z, 0.f));
-                S.ltri.push_back(make_float4(t.n.x, t.n.y, t.n.z, 0.f));
+                // Triangle::Sample's normal: normalize(n0 + n1 + n2) for a triangle
+                // with per-vertex normals (GLTFLoader.cpp:250), else the face normal
+                V3 n = t.n;
+                const int64_t g = o.first + li;
+                if (d->tri_flags && (d->tri_flags[g] & PRR_TRI_SMOOTH_NORMALS)) {
+                    const float *q = d->tri_normals + 9 * g;
+                    n = normalize(V3(q[0], q[1], q[2]) + V3(q[3], q[4], q[5]) + V3(q[6], q[7], q[8]));
+                }
+                S.ltri.push_back(make_float4(n.x, n.y, n.z, 0.f));
             }
         }
         S.emitters.push_back(e);

@@ -68,6 +68,11 @@ struct HostScene {
     std::vector<DEmitter> emitters;
     std::vector<DLightNode> lnodes;
     std::vector<float4> ltri;      // 4 per light leaf slot
+    // per-vertex attributes (empty: none): kTriAttrRows float4 per triangle
+    std::vector<float4> tattr;
+    std::vector<int4> matTex;      // per material: texture rows (-1 none); empty: untextured scene
+    std::vector<int4> texInfo;     // per texture: first texel byte, width, height, channels
+    std::vector<uint8_t> texels;
     std::vector<float4> envPx;
     std::vector<float> envCdf, envMarg;
     float envTotal = 0.f;

@@ -36,6 +36,19 @@ class _Scene:
             self.env = d["scene_env_pixels"].reshape(h, w, 3)
         self.point_lights = None
         self.name = "fixture"
+        self.tri_flags = self.tri_normals = self.tri_uvs = self.tri_tangents = None
+        self.mat_tex = self.textures = None
+        if "scene_tri_flags" in d:
+            a = d["scene_tri_attr"].reshape(-1, 27)
+            self.tri_flags = d["scene_tri_flags"].astype(np.uint8)
+            self.tri_normals, self.tri_uvs, self.tri_tangents = a[:, :9], a[:, 9:15], a[:, 15:]
+        if "scene_mat_tex" in d:
+            self.mat_tex = d["scene_mat_tex"].reshape(-1, 4)
+            whc = d["scene_tex_whc"].reshape(-1, 3)
+            px, off, self.textures = d["scene_texels"], 0, []
+            for w, h, c in whc:
+                self.textures.append(px[off:off + w * h * c].reshape(h, w, c))
+                off += w * h * c
 
 
 def fixture_scene(d):

@@ -460,6 +460,8 @@ def main():
     if want("castray"):
         gen_castray(box, 256, 256, 8, 3000, 201)
         gen_castray(field, 320, 240, 12, 3000, 202)
+    if want("castray") or want("textured"):
+        gen_castray(TS.textured_room(), 256, 256, 8, 4000, 203)
     if want("image"):
         gen_image(box, 64, 64, 8, 256)
     if want("welford"):

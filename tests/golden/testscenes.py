@@ -159,3 +159,75 @@ def sphere_dirs(n, seed):
     special /= np.linalg.norm(special, axis=1, keepdims=True)
     d[: len(special)] = special
     return d.astype(np.float32)
+
+
+def attr_sphere(center, radius, segments=12, bands=8):
+    """UV-sphere mesh with per-vertex normals, uvs and tangents (glTF-style
+    attributes): (tris [n,9], normals [n,9], uvs [n,6], tangents [n,12])."""
+    c = np.asarray(center, np.float64)
+    tris, nrm, uvs, tan = [], [], [], []
+
+    def vert(i, j):
+        th, ph = math.pi * i / bands, 2 * math.pi * j / segments
+        n = np.array([math.sin(th) * math.cos(ph), math.cos(th), math.sin(th) * math.sin(ph)])
+        t = np.array([-math.sin(ph), 0.0, math.cos(ph)])
+        return c + radius * n, n, (j / segments * 2.0, i / bands), np.append(t, 1.0 if j % 2 else -1.0)
+    for i in range(bands):
+        for j in range(segments):
+            a, b, cc, d = vert(i, j), vert(i, j + 1), vert(i + 1, j + 1), vert(i + 1, j)
+            for f in ((a, d, cc), (a, cc, b)):
+                tris.append(np.concatenate([v[0] for v in f]))
+                nrm.append(np.concatenate([v[1] for v in f]))
+                uvs.append(np.concatenate([v[2] for v in f]))
+                tan.append(np.concatenate([v[3] for v in f]))
+    f32 = lambda x: np.asarray(x, np.float32)
+    return f32(tris), f32(nrm), f32(uvs), f32(tan)
+
+
+def textured_room(seed=17):
+    """box_room with glTF-style content: a textured floor (sRGB colour
+    checker, metallic-roughness and normal maps, tangents), a smooth-normal
+    GGX sphere mesh with a metallic-roughness map and a normal map without
+    tangents (buildTBN frame), a smooth-normal glass sphere mesh tinted by a
+    colour texture, a textured analytic sphere (uv 0), and a ceiling light
+    whose per-vertex normals tilt (light-sample normal normalize(n0+n1+n2))."""
+    rng = np.random.default_rng(seed)
+    b = S.SceneBuilder()
+    b.background = np.zeros(3, np.float32)
+    checker = np.zeros((8, 8, 3), np.uint8)
+    checker[(np.add.outer(np.arange(8), np.arange(8)) % 2) == 0] = (230, 200, 60)
+    checker[(np.add.outer(np.arange(8), np.arange(8)) % 2) == 1] = (40, 90, 200)
+    t_col = b.texture(checker)
+    t_mr = b.texture(rng.integers(0, 256, (5, 7, 4)).astype(np.uint8))
+    nm = rng.normal(scale=0.35, size=(6, 6, 3)) + np.array([0, 0, 1.0])
+    nm /= np.linalg.norm(nm, axis=2, keepdims=True)
+    t_nrm = b.texture(np.clip((nm * 0.5 + 0.5) * 255 + 0.5, 0, 255).astype(np.uint8))
+    t_tint = b.texture(rng.integers(120, 256, (4, 4, 3)).astype(np.uint8))
+    grey = b.material(S.MAT_DIFFUSE, (0.73, 0.73, 0.73), 1.0, 0.0, 1.5)
+    red = b.material(S.MAT_DIFFUSE, (0.65, 0.05, 0.05), 1.0, 0.0, 1.5)
+    green = b.material(S.MAT_DIFFUSE, (0.12, 0.45, 0.15), 1.0, 0.0, 1.5)
+    floor_m = b.material(S.MAT_DIFFUSE, (1, 1, 1), 0.6, 0.2, 1.5, textures=(t_col, t_mr, t_nrm, -1))
+    ggx_m = b.material(S.MAT_DIFFUSE, (0.95, 0.64, 0.54), 0.3, 1.0, 1.5, textures=(-1, t_mr, t_nrm, -1))
+    glass_m = b.material(S.MAT_GLASS, (1, 1, 1), 0.5, 0.0, 1.5, textures=(t_tint, -1, -1, -1))
+    ball_m = b.material(S.MAT_DIFFUSE, (0.5, 0.5, 0.5), 0.5, 0.0, 1.5, textures=(t_col, -1, -1, -1))
+    light = b.material(S.MAT_EMISSIVE, (0.8, 0.8, 0.8), 0.5, 0.0, 1.5, (17.0, 12.0, 4.0))
+    fl = S.quad((-1, 0, -1), (-1, 0, 1), (1, 0, 1), (1, 0, -1))
+    fl_uv = np.array([[0, 0, 0, 3, 3, 3], [0, 0, 3, 3, 3, 0]], np.float32)
+    fl_n = np.tile(np.array([0, 1, 0], np.float32), (2, 3))
+    fl_t = np.tile(np.array([1, 0, 0, 1], np.float32), (2, 3))
+    b.mesh(fl, floor_m, normals=fl_n, uvs=fl_uv, tangents=fl_t)
+    b.mesh(S.quad((-1, 2, -1), (1, 2, -1), (1, 2, 1), (-1, 2, 1)), grey)
+    b.mesh(S.quad((-1, 0, -1), (1, 0, -1), (1, 2, -1), (-1, 2, -1)), grey)
+    b.mesh(S.quad((-1, 0, -1), (-1, 2, -1), (-1, 2, 1), (-1, 0, 1)), red)
+    b.mesh(S.quad((1, 0, -1), (1, 0, 1), (1, 2, 1), (1, 2, -1)), green)
+    lq = S.quad((-0.25, 1.98, -0.25), (0.25, 1.98, -0.25), (0.25, 1.98, 0.25), (-0.25, 1.98, 0.25))
+    ln = np.array([[0.2, -1, 0.1], [-0.1, -1, 0.3], [0.05, -1, -0.2]], np.float32)
+    ln /= np.linalg.norm(ln, axis=1, keepdims=True)
+    b.mesh(lq, light, normals=np.tile(ln.ravel(), (2, 1)))
+    t, n, uv, tg = attr_sphere((-0.45, 0.4, -0.3), 0.4)
+    b.mesh(t, ggx_m, normals=n, uvs=uv)
+    t, n, uv, tg = attr_sphere((0.45, 0.4, 0.25), 0.4)
+    b.mesh(t, glass_m, normals=n, uvs=uv, tangents=tg)
+    b.sphere((0.3, 1.5, -0.5), 0.2, ball_m)
+    b.set_camera((0, 1, 3.9), (0, 1, 0), (0, 1, 0), 40.0)
+    return b.build("textured_room")

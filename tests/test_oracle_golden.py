@@ -193,7 +193,7 @@ def test_tonemap_closed_forms():
     assert np.allclose(disp[:, 0], oetf, atol=2e-6)
 
 
-@pytest.mark.parametrize("name", ["castray_box_room.npz", "castray_mini_field.npz"])
+@pytest.mark.parametrize("name", ["castray_box_room.npz", "castray_mini_field.npz", "castray_textured_room.npz"])
 def test_castray_matches_reference(name):
     """Full per-path parity with the reference castRay on identical
     SplitMix64 streams: same draws consumed in the same order."""
@@ -207,6 +207,21 @@ def test_castray_matches_reference(name):
     assert exact.mean() > 0.995, exact.mean()
     assert close.mean() > 0.999, close.mean()
     assert (ref.sum(1) > 0).mean() > 0.3
+
+
+def test_castray_textured_features_matter():
+    """The textured fixture exercises what it is for: the same paths without
+    the textures, or without the per-vertex attributes, leave the
+    reference's radiance on many paths."""
+    g = load_golden("castray_textured_room.npz")
+    w, h, depth = (int(x) for x in g["cfg"])
+    ref = g["ref_L"].reshape(-1, 3)
+    for strip in ("mat_tex", "tri_flags"):
+        sc = fixture_scene(g)
+        setattr(sc, strip, None)
+        L = O.OracleScene(sc).castray_splitmix(w, h, g["pix"], g["seeds"], depth)
+        same = np.all(np.abs(L - ref) <= 1e-5 * np.maximum(np.abs(ref), 1e-3), axis=1)
+        assert same.mean() < 0.9, (strip, same.mean())
 
 
 def test_welford_stop_rule_matches_reference():
