@@ -135,6 +135,36 @@ static int obj(int argc, char **argv) {
     return 0;
 }
 
+// GLTFLoader::load (scenes/museum_final.cpp's call): per mesh its triangle
+// count, material class and factors, texture sizes, and a checksum of every
+// loaded float and texel, for comparison with the reference loader's output.
+static int gltf(int argc, char **argv) {
+    if (argc < 4) return 2;
+    const MaterialType mt[4] = {MaterialType::Diffuse, MaterialType::Glass, MaterialType::Mirror, MaterialType::Emit};
+    std::vector<MeshTriangle *> meshes = GLTFLoader::load(argv[2], mt[std::atoi(argv[3])]);
+    for (MeshTriangle *m : meshes) {
+        double sum = 0.0;
+        for (const Triangle &t : m->triangles) {
+            for (const Vector3f &v : {t.v0, t.v1, t.v2, t.normal, t.n0, t.n1, t.n2, t.tan0, t.tan1, t.tan2})
+                sum += (double) v.x + 2.0 * v.y + 3.0 * v.z;
+            for (const Vector2f &v : {t.t0, t.t1, t.t2}) sum += (double) v.x - 5.0 * v.y;
+            sum += t.tangentW0 + t.tangentW1 + t.tangentW2 + (t.hasSmoothNormals ? 7 : 0) + (t.hasTangents ? 11 : 0);
+        }
+        Material *M = m->m;
+        printf("mesh %zu kind %d mat %.9g %.9g %.9g %.9g %.9g %.9g %.9g %.9g %.9g sum %.17g", m->triangles.size(),
+               M->kind(), M->baseColor.x, M->baseColor.y, M->baseColor.z, M->roughness, M->metallic, M->ior,
+               M->m_emission.x, M->m_emission.y, M->m_emission.z, sum);
+        for (const Texture *t : {&M->baseColorTex, &M->metallicRoughnessTex, &M->normalTex, &M->emissiveTex}) {
+            uint64_t h = 0;
+            for (unsigned char c : t->data) h = h * 131 + c;
+            printf(" tex %d %d %d %llu", t->width, t->height, t->channels, (unsigned long long) h);
+        }
+        printf("\n");
+        delete m;
+    }
+    return 0;
+}
+
 int main(int argc, char **argv) {
     if (argc < 2) {
         fprintf(stderr, "usage: %s render|hdr|obj ...\n", argv[0]);
@@ -144,6 +174,7 @@ int main(int argc, char **argv) {
         if (!std::strcmp(argv[1], "render")) return render(argc, argv);
         if (!std::strcmp(argv[1], "hdr")) return hdr(argc, argv);
         if (!std::strcmp(argv[1], "obj")) return obj(argc, argv);
+        if (!std::strcmp(argv[1], "gltf")) return gltf(argc, argv);
     } catch (const std::exception &e) {
         fprintf(stderr, "error: %s\n", e.what());
         return 1;

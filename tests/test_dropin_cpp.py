@@ -68,6 +68,44 @@ def test_obj_mesh_triangle(tmp_path):
     assert [float(x) for x in f[5:11]] == [1.0, 2.0, 3.0, 3.0, 5.0, 4.0]
 
 
+def test_gltf_loader_cpp_matches_reference(tmp_path):
+    """GLTFLoader::load through the C++ header: the reference loader's
+    meshes, material classes, factors, per-vertex data and texels (golden
+    dump of GLTFLoader.cpp on the same file), via a checksum of each."""
+    import numpy as np
+    z = np.load(os.path.join(ROOT, "tests", "golden", "gltf.npz"))
+    (tmp_path / "textured.glb").write_bytes(z["file_textured.glb"].tobytes())
+    for t in range(4):
+        lines = _run("gltf", tmp_path / "textured.glb", t).stdout.strip().split("\n")
+        pre = "textured.glb:%d:" % t
+        counts, mt = z[pre + "counts"], z[pre + "mtype"]
+        mat, tex, texels = z[pre + "mat"].reshape(-1, 9), z[pre + "tex"].reshape(-1, 4, 4), z[pre + "texels"]
+        tri = z[pre + "tri"].reshape(-1, 39).astype(np.float64)
+        flags = z[pre + "flags"]
+        assert len(lines) == len(counts)
+        first, off = 0, 0
+        for k, line in enumerate(lines):
+            f = line.split()
+            assert int(f[1]) == counts[k] and int(f[3]) == mt[k]
+            assert np.array_equal(np.array(f[5:14], np.float32), mat[k])
+            r = tri[first:first + counts[k]]
+            vec = np.concatenate([r[:, 0:21], r[:, 27:36]], 1).reshape(-1, 10, 3)
+            uv = r[:, 21:27].reshape(-1, 3, 2)
+            fl = flags[first:first + counts[k]]
+            want = (vec[..., 0] + 2.0 * vec[..., 1] + 3.0 * vec[..., 2]).sum() + (uv[..., 0] - 5.0 * uv[..., 1]).sum() \
+                + r[:, 36:39].sum() + 7.0 * ((fl & 1) != 0).sum() + 11.0 * ((fl & 2) != 0).sum()
+            assert abs(float(f[15]) - want) <= 1e-9 * max(1.0, abs(want)), (float(f[15]), want)
+            for j in range(4):
+                w, h, c, nb = tex[k, j]
+                g = f[16 + 5 * j:21 + 5 * j]
+                hsh = 0
+                for b in texels[off:off + nb].tolist():
+                    hsh = (hsh * 131 + b) % (1 << 64)
+                off += nb
+                assert [int(x) for x in g[1:4]] == ([w, h, c] if nb else [0, 0, 4]) and int(g[4]) == hsh
+            first += counts[k]
+
+
 def test_obj_missing_file_raises(tmp_path):
     r = _run("obj", tmp_path / "none.obj", 0, 0, 0, check=False)
     assert r.returncode == 1 and "cannot open" in r.stderr
