@@ -186,7 +186,7 @@ class Object;
 struct Intersection {
     bool happened = false;
     Vector3f coords;
-    Vector2f tcoords;  // barycentric (u, v) of the hit triangle
+    Vector2f tcoords;  // t0 * (1 - u - v) + t1 * u + t2 * v of the hit triangle (0 without uvs)
     Vector3f normal;
     Vector3f tangent;
     float tangentHandedness = 1.f;

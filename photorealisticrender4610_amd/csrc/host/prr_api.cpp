@@ -546,9 +546,17 @@ Intersection Scene::intersect(const Ray &ray) const {
     it.material = dev_->primMat[prim];
     if (Sphere *s = dev_->primSphere[prim]) {
         it.normal = normalize(it.coords - s->center);
-    } else {
-        it.normal = dev_->primNormal[prim];
-        it.tcoords = Vector2f(h[7], h[8]);
+    } else {  // Triangle::finalize (Triangle.cpp:42-56)
+        const Triangle &tr = *static_cast<const Triangle *>(dev_->primObj[prim]);
+        const float u = h[7], v = h[8], w0 = 1.f - u - v;
+        it.tcoords = tr.t0 * w0 + tr.t1 * u + tr.t2 * v;
+        it.normal = tr.hasSmoothNormals ? normalize(tr.n0 * w0 + tr.n1 * u + tr.n2 * v) : dev_->primNormal[prim];
+        if (tr.hasTangents) {
+            it.tangent = normalize(tr.tan0 * w0 + tr.tan1 * u + tr.tan2 * v);
+            const float w = tr.tangentW0 * w0 + tr.tangentW1 * u + tr.tangentW2 * v;
+            it.tangentHandedness = w >= 0.f ? 1.f : -1.f;
+            it.hasTangent = true;
+        }
     }
     return it;
 }
