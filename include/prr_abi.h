@@ -246,6 +246,25 @@ int prr_scene_export_level(const prr_scene* scene, int32_t level, int32_t cap,
 int prr_trace_closest(prr_context* ctx, const prr_scene* scene, int64_t n,
                       const float* org, const float* dir,
                       int32_t* prim, float* t, float* u, float* v);
+/* No counterpart in the reference: prr_trace_closest for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc). Same kernel and results (bit for bit) as
+ * prr_trace_closest, but the traversal's global spill stacks live in a
+ * static device array of PRR_NOALLOC_SCRATCH_BYTES (32 MiB: 256 blocks of
+ * 256 lanes, 64 entries of 8 bytes per lane) that each GPU the library runs
+ * on holds from the first prr_create until the process ends, instead of the
+ * context's own allocation. The call allocates nothing (no hipMalloc, no
+ * hipFree). The scratch does not grow with n: larger batches loop over the
+ * same lanes; n above PRR_NOALLOC_MAX_RAYS is rejected. Every call on a GPU
+ * shares that GPU's array: calls that overlap on one GPU (two contexts or
+ * streams at once) race and must be serialised by the caller. Nothing in the
+ * array needs clearing between calls (each lane writes its stack before
+ * reading it). */
+#define PRR_NOALLOC_SCRATCH_BYTES (256ll * 256 * 64 * 8)
+#define PRR_NOALLOC_MAX_RAYS (((int64_t) 1 << 31) - 1)
+int prr_trace_closest_noalloc(prr_context* ctx, const prr_scene* scene, int64_t n,
+                              const float* org, const float* dir,
+                              int32_t* prim, float* t, float* u, float* v);
 /* Any hit in (1e-3, tmax) (BVH.cpp:421-473, Triangle.hpp:75-92). */
 int prr_trace_any(prr_context* ctx, const prr_scene* scene, int64_t n,
                   const float* org, const float* dir, const float* tmax, uint8_t* hit);

@@ -158,6 +158,7 @@ SIGNATURES = {
     "prr_scene_get_info": [vp, C.POINTER(SceneInfo)],
     "prr_scene_export_level": [vp, i32, i32, vp, vp, vp, C.POINTER(i32)],
     "prr_trace_closest": [vp, vp, i64, vp, vp, vp, vp, vp, vp],
+    "prr_trace_closest_noalloc": [vp, vp, i64, vp, vp, vp, vp, vp, vp],
     "prr_trace_any": [vp, vp, i64, vp, vp, vp, vp],
     "prr_bsdf_eval": [vp, i64, vp, vp, vp, vp, vp, vp],
     "prr_bsdf_sample": [vp, i64, vp, vp, vp, vp, vp, vp, vp, vp],
@@ -197,6 +198,14 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise PrrError("libprr.so is not built (%s); run __graft_entry__.build()" % LIB_PATH)
+        # torch ships its own HIP runtime under the same soname
+        # (libamdhip64.so.7) as the one libprr.so links: whichever loads first
+        # serves the process, and torch fails to find a device on the other.
+        # Load torch's first so both use it.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = C.CDLL(LIB_PATH)
         for name, args in SIGNATURES.items():
             fn = getattr(L, name)

@@ -3,6 +3,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -18,6 +19,12 @@
 using namespace prr;
 
 static thread_local std::string g_lastError = "";
+
+// Per device: its address of the static spill array behind
+// prr_trace_closest_noalloc, looked up once (hipGetSymbolAddress with the
+// device current) by the first prr_create on that device.
+static std::mutex g_noAllocMu;
+static uint2 *g_noAllocSpill[256] = {};
 
 int fail(prr_context *c, int code, const std::string &msg) {
     if (c) c->err = msg;
@@ -300,6 +307,20 @@ int prr_create(int device, prr_context **out) {
         delete c;
         return fail(nullptr, PRR_ERR_DEVICE, m);
     }
+    if (e == hipSuccess && device < 256) {
+        std::lock_guard<std::mutex> lk(g_noAllocMu);
+        if (!g_noAllocSpill[device]) {
+            uint2 *p = nullptr;
+            e = noAllocScratch(&p);
+            if (e == hipSuccess) g_noAllocSpill[device] = p;
+        }
+        c->noAllocSpill = g_noAllocSpill[device];
+    }
+    if (e != hipSuccess) {
+        std::string m = std::string("device init: ") + hipGetErrorString(e);
+        prr_destroy(c);
+        return fail(nullptr, PRR_ERR_DEVICE, m);
+    }
     c->stream = c->own;
     c->numCU = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 1;
     // persistent traversal grid: 8 blocks of 256 lanes per CU (with 8-entry
@@ -508,6 +529,19 @@ int prr_trace_closest(prr_context *c, const prr_scene *s, int64_t n, const float
     if (n < 0 || (n > 0 && (!org || !dir || !prim || !t))) return fail(c, PRR_ERR_INVALID, "bad arrays");
     HIPCK(c, hipSetDevice(c->device));
     HIPCK(c, launchTraceClosest(s->view, n, org, dir, prim, t, u, v, c->spill, c->spillLanes, c->stackDepth, c->stream));
+    return PRR_OK;
+}
+
+int prr_trace_closest_noalloc(prr_context *c, const prr_scene *s, int64_t n, const float *org, const float *dir,
+                              int32_t *prim, float *t, float *u, float *v) {
+    if (!c || !s) return fail(c, PRR_ERR_INVALID, "NULL context or scene");
+    if (n < 0 || n > PRR_NOALLOC_MAX_RAYS)
+        return fail(c, PRR_ERR_INVALID, "prr_trace_closest_noalloc: n outside [0, PRR_NOALLOC_MAX_RAYS]");
+    if (n > 0 && (!org || !dir || !prim || !t)) return fail(c, PRR_ERR_INVALID, "bad arrays");
+    if (!c->noAllocSpill) return fail(c, PRR_ERR_DEVICE, "no static spill array on this device");
+    HIPCK(c, hipSetDevice(c->device));
+    HIPCK(c, launchTraceClosestNoAlloc(s->view, n, org, dir, prim, t, u, v, c->noAllocSpill, c->stackDepth,
+                                       c->stream));
     return PRR_OK;
 }
 

@@ -30,6 +30,9 @@ struct prr_context {
     int64_t maxSlots = 1 << 24;  // paths per chunk (set at creation from free HBM)
     uint2 *spill = nullptr;
     int spillLanes = 0;
+    // this device's address of the static spill array of
+    // prr_trace_closest_noalloc (looked up once per device at prr_create)
+    uint2 *noAllocSpill = nullptr;
     // shadow rays of bounce b - 1 are traced on `aux` while the extension
     // rays of bounce b are traced on the render stream (each launch's tail
     // overlaps the other's bulk); the aux launches use their own spill region

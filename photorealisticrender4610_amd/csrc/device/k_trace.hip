@@ -62,6 +62,26 @@ hipError_t launchTraceClosest(const SceneView &s, int64_t n, const float *org, c
     return hipGetLastError();
 }
 
+// Spill region of prr_trace_closest_noalloc: the stacks of kNoAllocBlocks
+// blocks of kTraceBlock lanes, kSpill entries each (32 MiB), a file-scope
+// device array that every GPU the library runs on holds until the process
+// ends. Host code never passes the array's name to a kernel: that name is a
+// host-side stand-in, not a device address. noAllocScratch() returns the
+// current device's own address of it, which the launcher below takes.
+static __device__ __align__(256) uint2 scratch[(size_t) kNoAllocBlocks * kTraceBlock * kSpill];
+
+hipError_t noAllocScratch(uint2 **out) { return hipGetSymbolAddress((void **) out, HIP_SYMBOL(scratch)); }
+
+hipError_t launchTraceClosestNoAlloc(const SceneView &s, int64_t n, const float *org, const float *dir, int32_t *prim,
+                                     float *t, float *u, float *v, uint2 *scratchPtr, int stackDepth, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    // at most kNoAllocBlocks blocks: each lane's spill region lies inside the array
+    hipLaunchKernelGGL(kTraceClosest, dim3(gridFor(n, kNoAllocBlocks * kTraceBlock)), dim3(kTraceBlock),
+                       stackDepth * kTraceBlock * sizeof(uint2), st, s, n, org, dir, prim, t, u, v, scratchPtr,
+                       stackDepth);
+    return hipGetLastError();
+}
+
 hipError_t launchTraceAny(const SceneView &s, int64_t n, const float *org, const float *dir, const float *tmax,
                           uint8_t *hit, uint2 *spill, int spillLanes, int stackDepth, int quant, hipStream_t st) {
     if (n <= 0) return hipSuccess;

@@ -15,6 +15,15 @@ hipError_t launchTraceClosest(const SceneView &s, int64_t n, const float *org, c
 hipError_t launchTraceAny(const SceneView &s, int64_t n, const float *org, const float *dir, const float *tmax,
                           uint8_t *hit, uint2 *spill, int spillLanes, int stackDepth, int quant, hipStream_t st);
 
+// prr_trace_closest_noalloc: kTraceClosest with its spill region in a
+// static device array (kNoAllocBlocks * kTraceBlock lanes of kSpill entries)
+// instead of the context's allocation. noAllocScratch() gives the current
+// device's address of that array (call it once per device, at set-up).
+constexpr int kNoAllocBlocks = 256;
+hipError_t noAllocScratch(uint2 **out);
+hipError_t launchTraceClosestNoAlloc(const SceneView &s, int64_t n, const float *org, const float *dir, int32_t *prim,
+                                     float *t, float *u, float *v, uint2 *scratchPtr, int stackDepth, hipStream_t st);
+
 // k_misc.hip
 hipError_t launchBsdfEval(int64_t n, const float *wi, const float *wo, const float *nrm, const float *params,
                           float *f, float *pdf, hipStream_t st);
