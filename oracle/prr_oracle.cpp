@@ -34,6 +34,29 @@ static void put3(float *p, const V3 &v) {
 
 extern "C" int orc_version(void) { return 3; }
 
+// pcg32_srandom_r(initstate, initseq) followed by n pcg32_random_r outputs
+// (O'Neill 2014) and the float each maps to (top 24 bits * 2^-24), through
+// the Pcg32 the per-pixel streams use.
+extern "C" void orc_pcg32_sequence(uint64_t initstate, uint64_t initseq, int n, uint32_t *out, float *fout) {
+    Pcg32 r;
+    r.state = 0;
+    r.inc = (initseq << 1) | 1u;
+    r.next_u32();
+    r.state += initstate;
+    r.next_u32();
+    Pcg32 f = r;
+    for (int i = 0; i < n; i++) {
+        out[i] = r.next_u32();
+        fout[i] = f.next();
+    }
+}
+
+// The first n floats of the per-pixel stream Pcg32::make(seed, pixel, sample).
+extern "C" void orc_pcg32_stream(uint64_t seed, uint32_t pixel, uint32_t sample, int n, float *fout) {
+    Pcg32 r = Pcg32::make(seed, pixel, sample);
+    for (int i = 0; i < n; i++) fout[i] = r.next();
+}
+
 extern "C" orc_scene *orc_scene_create(const prr_scene_desc *d) {
     orc_scene *s = new orc_scene();
     Scene &sc = s->sc;
@@ -412,6 +435,8 @@ struct Photon {
 };
 struct orc_photon_map {
     std::vector<Photon> ph;
+    std::vector<int64_t> owner;  // emitting photon of each stored photon (reference rules)
+    std::vector<int32_t> depth;  // its trace depth there
     float r = 0.01f, cell = 0.02f;
     std::unordered_map<uint64_t, std::vector<int>> grid;
     static uint64_t key(int x, int y, int z) {
@@ -456,6 +481,16 @@ struct orc_photon_map {
         return sum / (kPi * r * r);
     }
 };
+
+// hash-grid cells of the stored photons (cell = 2r)
+static void gridInsert(orc_photon_map *pm) {
+    for (int i = 0; i < (int) pm->ph.size(); i++) {
+        const V3 &x = pm->ph[i].pos;
+        int cx = (int) std::floor(x.x / pm->cell), cy = (int) std::floor(x.y / pm->cell),
+            cz = (int) std::floor(x.z / pm->cell);
+        pm->grid[orc_photon_map::key(cx, cy, cz)].push_back(i);
+    }
+}
 
 template <class R>
 static bool tracePhoton(const Scene &sc, Ray ray, V3 power, int maxDepth, R &rng, Photon &out) {
@@ -519,13 +554,145 @@ extern "C" orc_photon_map *orc_photon_build(const orc_scene *s, int64_t num, flo
     }
     for (int64_t i = 0; i < num; i++)
         if (ok[i]) pm->ph.push_back(tmp[i]);
-    for (int i = 0; i < (int) pm->ph.size(); i++) {
-        const V3 &x = pm->ph[i].pos;
-        int cx = (int) std::floor(x.x / pm->cell), cy = (int) std::floor(x.y / pm->cell),
-            cz = (int) std::floor(x.z / pm->cell);
-        pm->grid[orc_photon_map::key(cx, cy, cz)].push_back(i);
-    }
+    gridInsert(pm);
     return pm;
+}
+
+// ---------------------------------------------------------------------------
+// The reference's own photon map: PhotonMap::emit's per-photon body
+// (Photon.cpp:166-180) and PhotonMap::trace (Photon.cpp:96-152) as written,
+// including what the build's point-light map changes: emission from
+// Scene::emissives through Scene::sampleLight and a cosine hemisphere about
+// the light normal with power Le * A / N (A = 1 / pdf); delta vertices shaded
+// with the normal flipped to face the photon (so glass is always "entered",
+// eta = 1 / ior, and transmission is not tinted); every diffuse hit after a
+// delta vertex stores the photon (depth > 0), then a Russian-roulette
+// diffuse continuation on max(power) with the weight f * cos / pdf (f already
+// holds the cosine). The trace recursion is a loop over depth.
+// ---------------------------------------------------------------------------
+template <class R>
+static void emitReferencePhoton(const Scene &sc, float numPhotons, int maxDepth, R &rng, std::vector<Photon> &out,
+                                std::vector<int32_t> *depths) {
+    V3 lp, ln;
+    int lm;
+    float pdf;
+    if (!sc.sampleLight(rng, lp, ln, lm, pdf)) return;
+    const float area = 1.0f / pdf;
+    // sampleCosineHemisphere (BRDFUtils.hpp:66-73)
+    const float r1 = rng.next();
+    const float r2 = rng.next();
+    const float r = std::sqrt(r1);
+    const float phi = 2.f * kPi * r2;
+    V3 dir = toWorld(V3(r * std::cos(phi), r * std::sin(phi), std::sqrt(std::max(0.f, 1.f - r1))), ln);
+    V3 power = sc.mats[lm].emission * area / numPhotons;
+    V3 pos = lp + ln * kEps;
+    bool caustic = false;
+    for (int depth = 0; depth <= maxDepth; depth++) {
+        const Hit h = sc.intersect(Ray{pos, dir}, nullptr);
+        if (!h.happened) return;
+        const int mi = sc.objs[h.obj].mat;
+        const Material &m = sc.mats[mi];
+        if (m.isEmissive()) return;
+        const V3 geoN = dot(dir, h.normal) < 0.0f ? h.normal : -h.normal;
+        if (m.isDelta()) {
+            const BSDFSample bs = (m.type == MAT_GLASS) ? glassSample(-dir, geoN, m.ior, rng)
+                                                        : mirrorSample(-dir, geoN, m.base);
+            caustic = true;
+            power = power * bs.f;
+            dir = bs.wi;
+            pos = h.coords + (dot(bs.wi, geoN) > 0.0f ? geoN : -geoN) * kEps;
+            continue;
+        }
+        if (depth > 0 && caustic) {
+            out.push_back(Photon{h.coords, dir, power});
+            if (depths) depths->push_back(depth);
+        }
+        float survival = std::min(1.0f, std::max({power.x, power.y, power.z}));
+        if (!(rng.next() < survival && depth < maxDepth)) return;
+        const Shading sd = sc.shading(mi, h, geoN);
+        const BSDFSample bs = diffuseSample(-dir, sd, rng);
+        if (bs.pdf < 1e-6f) return;
+        const float cosTheta = std::max(0.0f, dot(bs.wi, geoN));
+        const V3 weight = bs.f * cosTheta / bs.pdf;
+        power = power * weight;
+        survival = std::min(1.0f, std::max({power.x, power.y, power.z}));
+        if (rng.next() > survival) return;
+        power = power / survival;
+        pos = h.coords + geoN * kEps;
+        dir = bs.wi;
+    }
+}
+
+// Scene::buildPhotonMaps (Scene.cpp:8-13) with the reference's rules and the
+// GPU's PCG32 stream per photon (the point-light map's streams); stored
+// photons in (photon, depth) order.
+extern "C" orc_photon_map *orc_photon_emit(const orc_scene *s, int64_t num, float radius, int maxDepth, uint64_t seed,
+                                           int threads) {
+    const Scene &sc = s->sc;
+    orc_photon_map *pm = new orc_photon_map();
+    pm->r = radius;
+    pm->cell = 2.f * radius;
+    if (sc.totalEmitArea <= 0.f || num <= 0) return pm;
+    std::vector<std::vector<Photon>> per(num);
+    std::vector<std::vector<int32_t>> dep(num);
+#ifdef _OPENMP
+    if (threads > 0) omp_set_num_threads(threads);
+#endif
+#pragma omp parallel for schedule(dynamic, 1024)
+    for (int64_t i = 0; i < num; i++) {
+        Pcg32 rng = Pcg32::make(seed, (uint32_t) (i & 0xffffffffu), 0x7000000u + (uint32_t) (i >> 32));
+        emitReferencePhoton(sc, (float) num, maxDepth, rng, per[i], &dep[i]);
+    }
+    for (int64_t i = 0; i < num; i++)
+        for (size_t k = 0; k < per[i].size(); k++) {
+            pm->ph.push_back(per[i][k]);
+            pm->owner.push_back(i);
+            pm->depth.push_back(dep[i][k]);
+        }
+    gridInsert(pm);
+    return pm;
+}
+
+// The same with the reference's own SplitMix64 stream from states[k] per
+// photon and a given N (power Le * A / N): replays the golden harness.
+extern "C" int64_t orc_photon_emit_splitmix(const orc_scene *s, int64_t n, const uint64_t *states, float numPhotons,
+                                            int maxDepth, int64_t cap, float *out9, int32_t *owner) {
+    std::vector<Photon> ph;
+    std::vector<int32_t> own;
+    for (int64_t k = 0; k < n; k++) {
+        SplitMix64 rng{states[k]};
+        emitReferencePhoton(s->sc, numPhotons, maxDepth, rng, ph, nullptr);
+        own.resize(ph.size(), (int32_t) k);
+    }
+    for (int64_t i = 0; i < (int64_t) ph.size() && i < cap; i++) {
+        put3(out9 + 9 * i, ph[i].pos);
+        put3(out9 + 9 * i + 3, ph[i].dir);
+        put3(out9 + 9 * i + 6, ph[i].power);
+        owner[i] = own[i];
+    }
+    return (int64_t) ph.size();
+}
+
+// A photon map over given photons (pos, dir, power per row): the gather of
+// a map the caller supplies, e.g. the reference's own stored photons.
+extern "C" orc_photon_map *orc_photon_map_from(int64_t n, const float *ph9, float radius) {
+    orc_photon_map *pm = new orc_photon_map();
+    pm->r = radius;
+    pm->cell = 2.f * radius;
+    for (int64_t i = 0; i < n; i++)
+        pm->ph.push_back(Photon{v3(ph9 + 9 * i), v3(ph9 + 9 * i + 3), v3(ph9 + 9 * i + 6)});
+    gridInsert(pm);
+    return pm;
+}
+
+// Emitting photon and trace depth of the stored photons (reference rules).
+extern "C" int64_t orc_photon_owners(const orc_photon_map *pm, int64_t cap, int64_t *owner, int32_t *depth) {
+    const int64_t n = std::min<int64_t>(cap, (int64_t) pm->owner.size());
+    for (int64_t i = 0; i < n; i++) {
+        owner[i] = pm->owner[i];
+        depth[i] = pm->depth[i];
+    }
+    return (int64_t) pm->owner.size();
 }
 
 extern "C" void orc_photon_free(orc_photon_map *pm) { delete pm; }

@@ -98,6 +98,11 @@ void orc_castray_pcg(const orc_scene* s, int64_t n, const float* org, const floa
  * closest-hit interior expansions, closest-hit triangle tests, any-hit
  * interior expansions, any-hit triangle tests. threads: OpenMP
  * threads (<=0: default). Photon estimate if photon_map != NULL. */
+/* PCG32 (O'Neill 2014): pcg32_srandom_r(initstate, initseq) then n outputs
+ * of pcg32_random_r and their floats; and the per-pixel stream's floats. */
+void orc_pcg32_sequence(uint64_t initstate, uint64_t initseq, int n, uint32_t* out, float* fout);
+void orc_pcg32_stream(uint64_t seed, uint32_t pixel, uint32_t sample, int n, float* fout);
+
 typedef struct orc_photon_map orc_photon_map;
 void orc_render(const orc_scene* s, const prr_render_params* p, int row0, int row1,
                 const orc_photon_map* pm, int threads, float* mean, float* var,
@@ -122,6 +127,21 @@ void orc_welford(int nseq, int len, const float* seq, const int32_t* min_spp, co
  * hash-grid storage). */
 orc_photon_map* orc_photon_build(const orc_scene* s, int64_t num_photons, float radius,
                                  int max_depth, uint64_t seed, int threads);
+/* Scene::buildPhotonMaps with the reference's own rules (Photon.cpp:96-188:
+ * area emitters, flipped normal at delta vertices, storage at every diffuse
+ * hit after a delta vertex, Russian-roulette diffuse continuation), PCG32
+ * stream per photon as orc_photon_build. */
+orc_photon_map* orc_photon_emit(const orc_scene* s, int64_t num_photons, float radius, int max_depth,
+                                uint64_t seed, int threads);
+/* The same per photon k with SplitMix64 state states[k] and power divisor
+ * num_photons; writes up to cap stored photons (pos, dir, power) and their
+ * emitting photon; returns the count. */
+int64_t orc_photon_emit_splitmix(const orc_scene* s, int64_t n, const uint64_t* states, float num_photons,
+                                 int max_depth, int64_t cap, float* out9, int32_t* owner);
+/* a map over n given photons (pos, dir, power per row of ph9) */
+orc_photon_map* orc_photon_map_from(int64_t n, const float* ph9, float radius);
+/* emitting photon and trace depth per stored photon (orc_photon_emit maps) */
+int64_t orc_photon_owners(const orc_photon_map* pm, int64_t cap, int64_t* owner, int32_t* depth);
 void orc_photon_free(orc_photon_map* pm);
 int64_t orc_photon_count(const orc_photon_map* pm);
 /* copies up to cap photons: pos3, dir3, power3 (9 floats each) */

@@ -184,6 +184,16 @@ def lib(fast=False):
     L.orc_photon_build.restype = vp
     L.orc_photon_build.argtypes = [vp, C.c_int64, C.c_float, C.c_int, C.c_uint64, C.c_int]
     L.orc_photon_free.argtypes = [vp]
+    L.orc_photon_emit.restype = vp
+    L.orc_photon_emit.argtypes = [vp, C.c_int64, C.c_float, C.c_int, C.c_uint64, C.c_int]
+    L.orc_photon_emit_splitmix.restype = C.c_int64
+    L.orc_photon_emit_splitmix.argtypes = [vp, C.c_int64, vp, C.c_float, C.c_int, C.c_int64, vp, vp]
+    L.orc_photon_map_from.restype = vp
+    L.orc_photon_map_from.argtypes = [C.c_int64, vp, C.c_float]
+    L.orc_photon_owners.restype = C.c_int64
+    L.orc_photon_owners.argtypes = [vp, C.c_int64, vp, vp]
+    L.orc_pcg32_sequence.argtypes = [C.c_uint64, C.c_uint64, C.c_int, vp, vp]
+    L.orc_pcg32_stream.argtypes = [C.c_uint64, C.c_uint32, C.c_uint32, C.c_int, vp]
     L.orc_photon_count.restype = C.c_int64
     L.orc_photon_count.argtypes = [vp]
     L.orc_photon_export.restype = C.c_int64
@@ -341,9 +351,14 @@ def train_hash(pixel, sample):
 
 
 class OraclePhotonMap:
-    def __init__(self, scene: OracleScene, num, radius, max_depth, seed, threads=0):
+    """rules "point": the build's caustic map from the point light
+    (orc_photon_build); "reference": PhotonMap::emit's own rules from the
+    area emitters (orc_photon_emit)."""
+
+    def __init__(self, scene: OracleScene, num, radius, max_depth, seed, threads=0, rules="point"):
         self.L = scene.L
-        self.h = self.L.orc_photon_build(scene.h, num, radius, max_depth, seed, threads)
+        fn = self.L.orc_photon_emit if rules == "reference" else self.L.orc_photon_build
+        self.h = fn(scene.h, num, radius, max_depth, seed, threads)
 
     def __del__(self):
         try:
@@ -366,11 +381,56 @@ class OraclePhotonMap:
         self.L.orc_photon_radiance(self.h, pos.shape[0], _p(pos), _p(wo), _p(nrm), _p(params), _p(out))
         return out
 
+    @classmethod
+    def from_photons(cls, ph9, radius):
+        """A map over given photons (rows pos, dir, power)."""
+        self = cls.__new__(cls)
+        self.L = lib()
+        self._keep = np.ascontiguousarray(ph9, np.float32).reshape(-1, 9)
+        self.h = self.L.orc_photon_map_from(self._keep.shape[0], _p(self._keep), radius)
+        return self
+
+    def owners(self):
+        """(emitting photon, trace depth) per stored photon (reference rules)."""
+        n = self.count()
+        own = np.zeros(n, np.int64)
+        dep = np.zeros(n, np.int32)
+        self.L.orc_photon_owners(self.h, n, _p(own), _p(dep))
+        return own, dep
+
     def irradiance(self, pos):
         pos = _c(pos)
         out = np.zeros_like(pos)
         self.L.orc_photon_irradiance(self.h, pos.shape[0], _p(pos), _p(out))
         return out
+
+
+def photon_emit_splitmix(scene: OracleScene, states_u64, num_photons, max_depth):
+    """The reference's photon rules per photon k from SplitMix64 state
+    states_u64[k]: (photons [m, 9], owner [m])."""
+    st = np.ascontiguousarray(states_u64, np.uint64)
+    cap = 64 * max(1, st.shape[0])
+    out = np.zeros((cap, 9), np.float32)
+    own = np.zeros(cap, np.int32)
+    m = scene.L.orc_photon_emit_splitmix(scene.h, st.shape[0], _p(st), float(num_photons), max_depth, cap, _p(out),
+                                         _p(own))
+    assert m <= cap
+    return out[:m], own[:m]
+
+
+def pcg32_sequence(initstate, initseq, n):
+    """(uint32 outputs, floats) after pcg32_srandom_r(initstate, initseq)."""
+    L = lib()
+    out = np.zeros(n, np.uint32)
+    f = np.zeros(n, np.float32)
+    L.orc_pcg32_sequence(initstate, initseq, n, _p(out), _p(f))
+    return out, f
+
+
+def pcg32_stream(seed, pixel, sample, n):
+    f = np.zeros(n, np.float32)
+    lib().orc_pcg32_stream(seed, pixel, sample, n, _p(f))
+    return f
 
 
 def bsdf_eval(wi, wo, n, params):

@@ -3,7 +3,7 @@
 // TEST INFRASTRUCTURE ONLY.
 //
 // Usage: ref_harness <mode> <in.bundle> <out.bundle>
-//   modes: trace, bsdf, env, camera, tonemap, castray, image, welford, hdr, gltf,
+//   modes: trace, bsdf, env, camera, tonemap, castray, image, photon, welford, hdr, gltf,
 //          bench (only in the ref_bench build: PRR_REF_BENCH)
 // A bundle is a sequence of records {char name[32]; int32 dtype; int64 count;
 // payload}, dtype 0=f32 1=i32 2=u64 3=u8 4=u32 (tests/golden/bundle.py).
@@ -541,6 +541,61 @@ static int modeCastray(const Bundle &b, const char *out) {
     return 0;
 }
 
+// PhotonMap::emit / trace / build / estimateIrradiance (Photon.cpp:9-190)
+// as the reference runs them. Photon k is emitted by the reference's own
+// PhotonMap::emit(1, scene) on a fresh thread whose generator starts from
+// seeds[k] (one OpenMP thread: emit's team is then the calling thread), so
+// its draws are known; its stored photons are appended to the map's
+// caustic_map and recorded with owner k. num_photons = 1 makes the emitted
+// power Le * A (Photon.cpp:168-170), so the diffuse continuation with its
+// Russian roulette (Photon.cpp:126-150) is exercised too. Then the grid is
+// built and estimateIrradiance evaluated at the query points.
+static int modePhoton(const Bundle &b, const char *out) {
+    RefScene rs;
+    buildScene(b, rs);
+    const int32_t *cfg = b.at("cfg").as<int32_t>();  // maxDepth
+    PhotonMap pm;
+    pm.maxDepth = cfg[0];
+    pm.photon_radius = b.at("radius").as<float>()[0];
+    const auto &S = b.at("seeds");
+    std::vector<float> ph;
+    std::vector<int32_t> owner;
+    for (size_t k = 0; k < S.n<uint32_t>(); k++) {
+        const size_t before = pm.caustic_map.size();
+        onFreshThread(S.as<uint32_t>()[k], [&]() {
+#ifdef _OPENMP
+            omp_set_num_threads(2);  // emit uses omp_get_max_threads() - 1 threads
+#endif
+            pm.emit(1, rs.scene);
+        });
+        for (size_t i = before; i < pm.caustic_map.size(); i++) {
+            const Photon &q = pm.caustic_map[i];
+            for (const Vector3f &v : {q.position, q.direction, q.power}) {
+                ph.push_back(v.x);
+                ph.push_back(v.y);
+                ph.push_back(v.z);
+            }
+            owner.push_back((int32_t) k);
+        }
+    }
+    pm.build();
+    const auto &Q = b.at("queries");
+    const size_t nq = Q.n<float>() / 3;
+    std::vector<float> irr(3 * nq);
+    for (size_t i = 0; i < nq; i++) {
+        const float *q = Q.as<float>() + 3 * i;
+        Vector3f e = pm.estimateIrradiance(Vector3f(q[0], q[1], q[2]), Vector3f(0.f, 1.f, 0.f));
+        irr[3 * i] = e.x;
+        irr[3 * i + 1] = e.y;
+        irr[3 * i + 2] = e.z;
+    }
+    Writer w(out);
+    w.f32("photons", ph);
+    w.i32("owner", owner);
+    w.f32("irradiance", irr);
+    return 0;
+}
+
 static int modeImage(const Bundle &b, const char *out) {
     RefScene rs;
     buildScene(b, rs);
@@ -765,6 +820,7 @@ int main(int argc, char **argv) {
     if (m == "tonemap") return modeTonemap(b, argv[3]);
     if (m == "castray") return modeCastray(b, argv[3]);
     if (m == "image") return modeImage(b, argv[3]);
+    if (m == "photon") return modePhoton(b, argv[3]);
     if (m == "welford") return modeWelford(b, argv[3]);
     if (m == "hdr") return modeHdr(b, argv[3]);
     if (m == "gltf") return modeGltf(b, argv[3]);

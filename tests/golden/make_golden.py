@@ -4,7 +4,7 @@ Runs in the development container only: builds oracle/_ref/ref_harness from
 the reference's own sources (oracle/Makefile target ``ref``), feeds it the
 seeded inputs below, and stores inputs + reference outputs together, so the
 tests never need the reference. Usage:
-python tests/golden/make_golden.py [trace bsdf env camera tonemap castray image welford hdr gltf]
+python tests/golden/make_golden.py [trace bsdf env camera tonemap castray photon image welford hdr gltf]
 (no names: all).
 """
 import os
@@ -143,6 +143,28 @@ def gen_image(sc, w, h, depth, spp):
     res = scene_npz(sc)
     res.update(cfg=np.array([w, h, depth, spp], np.int32), ref_mean=out["mean"], ref_var=out["var"])
     save("image_%s.npz" % sc.name, res)
+
+
+def gen_photon(sc, n, radius, max_depth, seed):
+    """PhotonMap::emit(1, scene) per photon on a fresh seeded thread, then
+    build + estimateIrradiance at points near the stored photons."""
+    seeds = seeds_u32(n, seed)
+    arr = sc.bundle_arrays()
+    arr.update(cfg=np.array([max_depth], np.int32), radius=np.array([radius], np.float32), seeds=seeds,
+               queries=np.zeros(3, np.float32))
+    first = run("photon", arr)
+    ph = first["photons"].reshape(-1, 9)
+    rng = np.random.default_rng(seed + 1)
+    q = ph[rng.integers(0, ph.shape[0], 400), 0:3] + rng.normal(scale=0.5 * radius, size=(400, 3))
+    q = np.concatenate([q, rng.normal(scale=2.0, size=(40, 3))]).astype(np.float32)
+    arr["queries"] = q.ravel()
+    out = run("photon", arr)
+    assert np.array_equal(out["photons"], first["photons"])
+    res = scene_npz(sc)
+    res.update(cfg=np.array([max_depth], np.int32), radius=np.array([radius], np.float32), seeds=seeds,
+               queries=q, ref_photons=out["photons"].reshape(-1, 9), ref_owner=out["owner"],
+               ref_irradiance=out["irradiance"].reshape(-1, 3))
+    save("photon_%s.npz" % sc.name, res)
 
 
 def welford_sequences(seed=61, nseq=720, ln=64):
@@ -462,6 +484,10 @@ def main():
         gen_castray(field, 320, 240, 12, 3000, 202)
     if want("castray") or want("textured"):
         gen_castray(TS.textured_room(), 256, 256, 8, 4000, 203)
+    if want("photon"):
+        gen_photon(box, 3000, 0.05, 30, 211)
+        gen_photon(field, 12000, 0.2, 30, 212)
+        gen_photon(TS.textured_room(), 3000, 0.05, 30, 213)
     if want("image"):
         gen_image(box, 64, 64, 8, 256)
     if want("welford"):
