@@ -304,6 +304,28 @@ int prr_tonemap(prr_context* ctx, int32_t op, int64_t n, float exposure, const f
  * scene. Blocks. stored may be NULL. */
 int prr_photon_build(prr_context* ctx, prr_scene* scene, int64_t num_photons, float radius,
                      int32_t max_depth, uint64_t seed, int64_t* stored);
+/* Scene::buildPhotonMaps (Scene.cpp:8-13) = PhotonMap::emit + PhotonMap::build
+ * (Photon.cpp:156-190) with the reference's own rules: num photons emitted
+ * from Scene::emissives (Scene::sampleLight, a cosine hemisphere about the
+ * light normal, power Le * A / num), traced as PhotonMap::trace does
+ * (Photon.cpp:96-152: normal flipped to face the photon at delta vertices,
+ * every diffuse hit after a delta vertex stores the photon, then Russian
+ * roulette and a diffuse bounce), PCG32 stream per photon index. The map
+ * stays resident in the scene (replacing any earlier one) for
+ * prr_photon_irradiance; renders do not gather it (castRay never does).
+ * Blocks. Fails with PRR_ERR_UNSUPPORTED when the scene has no emitter. */
+int prr_photon_emit(prr_context* ctx, prr_scene* scene, int64_t num_photons, float radius,
+                    int32_t max_depth, uint64_t seed, int64_t* stored);
+/* PhotonMap::estimateIrradiance (Photon.cpp:82-94) at n points (device
+ * arrays): the power of the resident map's photons within its radius over
+ * pi r^2. normal is unused, as in the reference, and may be NULL. */
+int prr_photon_irradiance(prr_context* ctx, const prr_scene* scene, int64_t n, const float* pos,
+                          const float* normal, float* out);
+/* For a prr_photon_emit map: per stored photon (the order of
+ * prr_photon_export) the index of the photon that stored it and its trace
+ * depth there, to HOST arrays of cap entries; *count = photons stored. */
+int prr_photon_export_owners(prr_context* ctx, const prr_scene* scene, int64_t cap, int64_t* owner,
+                             int32_t* depth, int64_t* count);
 /* Copies up to cap stored photons (pos3, dir3, power3 = 9 floats each, bucket
  * order) to device memory out9; *count = photons stored. */
 int prr_photon_export(prr_context* ctx, const prr_scene* scene, int64_t cap, float* out9,

@@ -170,6 +170,9 @@ SIGNATURES = {
     "prr_cast_rays": [vp, vp, i64, vp, vp, C.c_uint64, i32, C.c_float, vp, C.POINTER(Stats)],
     "prr_photon_build": [vp, vp, i64, C.c_float, i32, C.c_uint64, C.POINTER(i64)],
     "prr_photon_export": [vp, vp, i64, vp, C.POINTER(i64)],
+    "prr_photon_emit": [vp, vp, i64, C.c_float, i32, C.c_uint64, C.POINTER(i64)],
+    "prr_photon_irradiance": [vp, vp, i64, vp, vp, vp],
+    "prr_photon_export_owners": [vp, vp, i64, vp, vp, C.POINTER(i64)],
     "prr_photon_radiance": [vp, vp, i64, vp, vp, vp, vp, vp],
     "prr_cache_create": [vp, C.POINTER(CacheParams), C.POINTER(vp)],
     "prr_cache_free": [vp, vp],
@@ -500,6 +503,37 @@ class Context:
                                           C.byref(stored)))
         scene.photons = int(num)
         return stored.value
+
+    def photon_emit(self, scene, num, radius, max_depth=30, seed=0xC2):
+        """Scene::buildPhotonMaps with the reference's own rules (area
+        emitters; prr_photon_emit): returns the photons stored. Renders do
+        not gather this map."""
+        stored = i64()
+        self._bind_stream()
+        self._done(lib().prr_photon_emit(self.h, scene.h, int(num), float(radius), int(max_depth), int(seed),
+                                         C.byref(stored)))
+        scene.photons = 0
+        return stored.value
+
+    def photon_owners(self, scene):
+        """(emitting photon int64, trace depth int32) per stored photon of a
+        prr_photon_emit map, in photon_export's order (numpy)."""
+        cnt = i64()
+        self._check(lib().prr_photon_export_owners(self.h, scene.h, 0, None, None, C.byref(cnt)), self.h)
+        own = np.zeros(max(cnt.value, 1), np.int64)
+        dep = np.zeros(max(cnt.value, 1), np.int32)
+        self._check(lib().prr_photon_export_owners(self.h, scene.h, cnt.value, own.ctypes.data_as(vp),
+                                                   dep.ctypes.data_as(vp), C.byref(cnt)), self.h)
+        return own[:cnt.value], dep[:cnt.value]
+
+    def photon_irradiance(self, scene, pos):
+        """PhotonMap::estimateIrradiance at points pos [n, 3]."""
+        T = self.torch
+        pos = self._f32(pos)
+        out = T.empty((pos.shape[0], 3), dtype=T.float32, device=self.device)
+        self._bind_stream()
+        self._done(lib().prr_photon_irradiance(self.h, scene.h, pos.shape[0], _ptr(pos), None, _ptr(out)))
+        return out
 
     def photon_export(self, scene):
         T = self.torch

@@ -707,6 +707,164 @@ int prr_photon_build(prr_context *c, prr_scene *s, int64_t num, float radius, in
     return PRR_OK;
 }
 
+// PhotonMap::emit + PhotonMap::build (Photon.cpp:156-190) as Scene::buildPhotonMaps
+// calls them (Scene.cpp:8-13), with the reference's own rules.
+int prr_photon_emit(prr_context *c, prr_scene *s, int64_t num, float radius, int32_t maxDepth, uint64_t seed,
+                    int64_t *stored) {
+    if (!c || !s) return fail(c, PRR_ERR_INVALID, "NULL context or scene");
+    if (num <= 0 || num > ((int64_t) 1 << 31) || !(radius > 0.f) || maxDepth < 0 || maxDepth > 64)
+        return fail(c, PRR_ERR_INVALID, "bad photon parameters");
+    if (!(s->view.total_emit_area > 0.f) || s->view.num_emitters < 1)
+        return fail(c, PRR_ERR_UNSUPPORTED, "photon emission needs an emissive object (Scene::emissives is empty)");
+    HIPCK(c, hipSetDevice(c->device));
+    hipStream_t st = c->stream;
+    HIPCK(c, hipStreamSynchronize(st));
+    freePhotons(s);
+    PhotonMap &pm = s->ph;
+    std::vector<void *> tmp;
+    auto T = [&](size_t bytes) -> void * {
+        void *p = nullptr;
+        if (hipMalloc(&p, std::max<size_t>(bytes, 16)) != hipSuccess) return nullptr;
+        tmp.push_back(p);
+        return p;
+    };
+    auto A = [&](size_t bytes) -> void * {
+        void *p = nullptr;
+        if (hipMalloc(&p, std::max<size_t>(bytes, 16)) != hipSuccess) return nullptr;
+        pm.allocs.push_back(p);
+        return p;
+    };
+    auto freeTmp = [&]() {
+        for (void *p : tmp) (void) hipFree(p);
+        tmp.clear();
+    };
+    SceneView v = s->view;
+    v.ph_count = 0;
+    // records: a photon stores at most once per diffuse hit; start at two per
+    // photon and trace again with the exact capacity if that overflows (the
+    // trace is deterministic)
+    uint32_t cap = (uint32_t) std::min<int64_t>(std::max<int64_t>(2 * num, 1024), 0x7fffffff), m = 0;
+    float4 *pos = nullptr, *dir = nullptr, *pw = nullptr;
+    uint64_t *key = nullptr;
+    uint32_t *cnt = (uint32_t *) T(sizeof(uint32_t));
+    for (int pass = 0; pass < 2; pass++) {
+        pos = (float4 *) T((size_t) cap * sizeof(float4));
+        dir = (float4 *) T((size_t) cap * sizeof(float4));
+        pw = (float4 *) T((size_t) cap * sizeof(float4));
+        key = (uint64_t *) T((size_t) cap * sizeof(uint64_t));
+        if (!cnt || !pos || !dir || !pw || !key) {
+            freeTmp();
+            return fail(c, PRR_ERR_NOMEM, "photon buffers");
+        }
+        hipError_t e = hipMemsetAsync(cnt, 0, sizeof(uint32_t), st);
+        if (e == hipSuccess)
+            e = launchPhotonTraceRef(v, num, seed, maxDepth, pos, dir, pw, key, cap, cnt, c->spill, c->spillLanes,
+                                     c->stackDepth, st);
+        if (e == hipSuccess) e = hipMemcpyAsync(&m, cnt, sizeof(uint32_t), hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess) e = hipStreamSynchronize(st);
+        if (e != hipSuccess) {
+            freeTmp();
+            return fail(c, PRR_ERR_DEVICE, std::string("photon trace: ") + hipGetErrorString(e));
+        }
+        if (m <= cap) break;
+        if (pass == 1 || m > 0x7fffffffu) {
+            freeTmp();
+            return fail(c, PRR_ERR_NOMEM, "photon records exceed the capacity");
+        }
+        cap = m;
+    }
+    uint32_t buckets = 16;
+    while (buckets < 2u * std::max<uint32_t>(m, 1u) && buckets < (1u << 30)) buckets <<= 1;
+    const float cell = 2.f * radius;
+    // 1. records in (photon, depth) order
+    uint32_t *v0 = (uint32_t *) T((size_t) m * 4), *v1 = (uint32_t *) T((size_t) m * 4);
+    uint64_t *ksorted = (uint64_t *) T((size_t) m * 8);
+    float4 *rpos = (float4 *) T((size_t) m * sizeof(float4)), *rdir = (float4 *) T((size_t) m * sizeof(float4));
+    float4 *rpw = (float4 *) T((size_t) m * sizeof(float4));
+    uint64_t *rkey = (uint64_t *) T((size_t) m * 8);
+    uint32_t *bk = (uint32_t *) T((size_t) m * 4), *bkOut = (uint32_t *) T((size_t) m * 4);
+    // 2. the grid, in bucket order
+    float4 *spos = (float4 *) A((size_t) m * sizeof(float4)), *sdir = (float4 *) A((size_t) m * sizeof(float4));
+    float4 *spw = (float4 *) A((size_t) m * sizeof(float4));
+    uint64_t *skey = (uint64_t *) A((size_t) m * 8);
+    uint32_t *bs = (uint32_t *) A((size_t) buckets * 4), *be = (uint32_t *) A((size_t) buckets * 4);
+    if (!v0 || !v1 || !ksorted || !rpos || !rdir || !rpw || !rkey || !bk || !bkOut || !spos || !sdir || !spw || !skey ||
+        !bs || !be) {
+        freeTmp();
+        freePhotons(s);
+        return fail(c, PRR_ERR_NOMEM, "photon grid buffers");
+    }
+    hipError_t e = launchPhotonIota(m, v0, st);
+    size_t tb = 0, tb2 = 0;
+    if (e == hipSuccess) e = photonSortByKey(nullptr, tb, key, ksorted, v0, v1, (int) m, st);
+    if (e == hipSuccess) e = photonSort(nullptr, tb2, bk, bkOut, v0, v1, (int) m, st);
+    void *temp = (e == hipSuccess) ? T(std::max(tb, tb2)) : nullptr;
+    if (e == hipSuccess && !temp) e = hipErrorOutOfMemory;
+    if (e == hipSuccess) e = photonSortByKey(temp, tb, key, ksorted, v0, v1, (int) m, st);
+    if (e == hipSuccess) e = launchPhotonGather(m, v1, pos, dir, pw, key, rpos, rdir, rpw, rkey, st);
+    if (e == hipSuccess) e = launchPhotonKeys(m, rpos, cell, buckets - 1u, bk, v0, st);
+    if (e == hipSuccess) e = photonSort(temp, tb2, bk, bkOut, v0, v1, (int) m, st);
+    if (e == hipSuccess) e = launchPhotonGather(m, v1, rpos, rdir, rpw, rkey, spos, sdir, spw, skey, st);
+    if (e == hipSuccess) e = hipMemsetAsync(bs, 0, (size_t) buckets * 4, st);
+    if (e == hipSuccess) e = hipMemsetAsync(be, 0, (size_t) buckets * 4, st);
+    if (e == hipSuccess) e = launchPhotonRanges(m, bkOut, bs, be, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    freeTmp();
+    if (e != hipSuccess) {
+        freePhotons(s);
+        return fail(c, PRR_ERR_DEVICE, std::string("photon grid: ") + hipGetErrorString(e));
+    }
+    pm.emitted = num;
+    pm.stored = m;
+    pm.radius = radius;
+    pm.maxDepth = maxDepth;
+    pm.seed = seed;
+    pm.rules = 1;
+    pm.keys = skey;
+    s->view.ph_pos = spos;
+    s->view.ph_dir = sdir;
+    s->view.ph_pow = spw;
+    s->view.ph_start = bs;
+    s->view.ph_end = be;
+    s->view.ph_count = m;
+    s->view.ph_mask = buckets - 1u;
+    s->view.ph_radius = radius;
+    s->view.ph_cell = cell;
+    if (stored) *stored = m;
+    return PRR_OK;
+}
+
+int prr_photon_export_owners(prr_context *c, const prr_scene *s, int64_t cap, int64_t *owner, int32_t *depth,
+                             int64_t *count) {
+    if (!c || !s || !count) return fail(c, PRR_ERR_INVALID, "NULL argument");
+    if (s->ph.rules != 1 || !s->ph.keys)
+        return fail(c, PRR_ERR_UNSUPPORTED, "no photon map of the reference's rules (prr_photon_emit)");
+    *count = s->ph.stored;
+    const int64_t k = std::min<int64_t>(cap, s->ph.stored);
+    if (k <= 0) return PRR_OK;
+    if (!owner || !depth) return fail(c, PRR_ERR_INVALID, "NULL output");
+    std::vector<uint64_t> h((size_t) k);
+    HIPCK(c, hipSetDevice(c->device));
+    HIPCK(c, hipMemcpyAsync(h.data(), s->ph.keys, (size_t) k * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCK(c, hipStreamSynchronize(c->stream));
+    for (int64_t i = 0; i < k; i++) {
+        owner[i] = (int64_t) (h[i] >> 7);
+        depth[i] = (int32_t) (h[i] & 127u);
+    }
+    return PRR_OK;
+}
+
+int prr_photon_irradiance(prr_context *c, const prr_scene *s, int64_t n, const float *pos, const float *normal,
+                          float *out) {
+    (void) normal;  // unused, as in PhotonMap::estimateIrradiance
+    if (!c || !s) return fail(c, PRR_ERR_INVALID, "NULL context or scene");
+    if (s->ph.stored <= 0 && s->ph.emitted <= 0) return fail(c, PRR_ERR_UNSUPPORTED, "no photon map");
+    if (n < 0 || (n > 0 && (!pos || !out))) return fail(c, PRR_ERR_INVALID, "bad arrays");
+    HIPCK(c, hipSetDevice(c->device));
+    HIPCK(c, launchPhotonIrradiance(s->view, n, pos, out, c->stream));
+    return PRR_OK;
+}
+
 int prr_photon_export(prr_context *c, const prr_scene *s, int64_t cap, float *out9, int64_t *count) {
     if (!c || !s || !count) return fail(c, PRR_ERR_INVALID, "NULL argument");
     *count = s->ph.stored;
@@ -745,6 +903,9 @@ int prr_render(prr_context *c, const prr_scene *s, const prr_render_params *p, f
     if (pixels > c->maxSlots) return fail(c, PRR_ERR_UNSUPPORTED, "image larger than the context's path slots");
     if (p->num_photons > 0 && s->ph.stored <= 0 && s->ph.emitted <= 0)
         return fail(c, PRR_ERR_INVALID, "num_photons > 0 but the scene has no photon map (prr_photon_build)");
+    if (p->num_photons > 0 && s->ph.rules == 1)
+        return fail(c, PRR_ERR_UNSUPPORTED,
+                    "the reference's photon map (prr_photon_emit) is not gathered by castRay; use prr_photon_build");
     HIPCK(c, hipSetDevice(c->device));
     // Welford stopping can only end a pixel early when the threshold is
     // positive and min_spp < spp (Renderer.cpp:92-96); otherwise every pixel

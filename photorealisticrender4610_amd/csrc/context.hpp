@@ -68,6 +68,11 @@ struct PhotonMap {
     float radius = 0.f;
     int maxDepth = 0;
     uint64_t seed = 0;
+    // 0: the build's caustic map from the point light (prr_photon_build),
+    // gathered by renders; 1: the reference's own map from the area emitters
+    // (prr_photon_emit), which castRay never gathers
+    int rules = 0;
+    uint64_t *keys = nullptr;  // rules 1: emitting photon * 128 + trace depth per stored photon (bucket order)
 };
 
 struct prr_scene {

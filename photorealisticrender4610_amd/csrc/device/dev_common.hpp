@@ -799,6 +799,56 @@ PRR_DEV V3 exposeLinear(const V3 &fb, float exposure) {  // Renderer.cpp:149-161
     return V3(smax(0.f, c.x), smax(0.f, c.y), smax(0.f, c.z));
 }
 
+// Scene::sampleLight (Scene.cpp:37-58) -> MeshTriangle / BVHAccel::Sample
+// (BVH.cpp:478-514) -> Triangle::Sample / Sphere::Sample.
+PRR_DEV void sampleLight(const SceneView &s, Pcg &rng, V3 &pos, V3 &nrm, int &mat) {
+    const float p = rng.next() * s.total_emit_area;
+    float acc = 0.f;
+    int chosen = s.num_emitters - 1;
+    for (int k = 0; k < s.num_emitters; k++) {
+        acc += s.emitters[k].area;
+        if (p <= acc) {
+            chosen = k;
+            break;
+        }
+    }
+    const DEmitter e = s.emitters[chosen];
+    mat = e.material;
+    if (e.kind == 0) {
+        float q = sqrtf(rng.next()) * e.total;
+        int ni = e.lt_root;
+        int slot = -1;
+        while (slot < 0) {
+            const DLightNode n = s.lnodes[ni];
+            if (n.count > 0) {
+                for (int i = 0; i < n.count; i++) {
+                    q -= s.ltri[4 * (n.first + i)].w;
+                    if (q <= 0.f || i == n.count - 1) {
+                        slot = n.first + i;
+                        break;
+                    }
+                }
+            } else if (q < n.left_area) {
+                ni = ni + 1;
+            } else {
+                q -= n.left_area;
+                ni = n.first;
+            }
+        }
+        const float x = sqrtf(rng.next()), y = rng.next();
+        const V3 v0 = ld3(s.ltri[4 * slot]), v1 = ld3(s.ltri[4 * slot + 1]), v2 = ld3(s.ltri[4 * slot + 2]);
+        pos = v0 * (1.0f - x) + v1 * (x * (1.0f - y)) + v2 * (x * y);
+        nrm = ld3(s.ltri[4 * slot + 3]);
+    } else {
+        const float4 sc = s.sph_by_id[e.lt_root];
+        const float theta = (float) (2.0 * (double) kPi * (double) rng.next());
+        const float phi = kPi * rng.next();
+        const V3 dir(cosf(phi), sinf(phi) * cosf(theta), sinf(phi) * sinf(theta));
+        pos = V3(sc.x, sc.y, sc.z) + sc.w * dir;
+        nrm = dir;
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Caustic photon map gather (build extension of Photon.cpp:82-94): the
 // photon density estimate weighted by the BSDF,
@@ -852,6 +902,33 @@ PRR_DEV V3 photonRadiance(const SceneView &s, const V3 &x, const V3 &wo, const F
                 sum += f * ld3(s.ph_pow[i]) / c;
             }
         }
+    }
+    return sum / (kPi * r * r);
+}
+
+// PhotonMap::estimateIrradiance (Photon.cpp:82-94): the power of the
+// photons within r of x over pi r^2 (the normal is unused, as there). The
+// same cells and buckets as photonRadiance.
+PRR_DEV V3 photonIrradiance(const SceneView &s, const V3 &x) {
+    V3 sum(0.f);
+    if (s.ph_count == 0) return sum;
+    const float r = s.ph_radius, cs = s.ph_cell;
+    const int x0 = (int) floorf((x.x - r) / cs), x1 = (int) floorf((x.x + r) / cs);
+    const int y0 = (int) floorf((x.y - r) / cs), y1 = (int) floorf((x.y + r) / cs);
+    const int z0 = (int) floorf((x.z - r) / cs), z1 = (int) floorf((x.z + r) / cs);
+    for (int k = 0; k < 8; k++) {
+        const int cx = x0 + (k & 1), cy = y0 + ((k >> 1) & 1), cz = z0 + (k >> 2);
+        if (cx > x1 || cy > y1 || cz > z1) continue;
+        const uint32_t h = photonHash(cx, cy, cz, s.ph_mask);
+        bool dup = false;
+        for (int j = 0; j < k; j++) {
+            const int jx = x0 + (j & 1), jy = y0 + ((j >> 1) & 1), jz = z0 + (j >> 2);
+            dup |= jx <= x1 && jy <= y1 && jz <= z1 && photonHash(jx, jy, jz, s.ph_mask) == h;
+        }
+        if (dup) continue;
+        const uint32_t b = s.ph_start[h], e = s.ph_end[h];
+        for (uint32_t i = b; i < e; i++)
+            if (norm(ld3(s.ph_pos[i]) - x) <= r) sum += ld3(s.ph_pow[i]);
     }
     return sum / (kPi * r * r);
 }

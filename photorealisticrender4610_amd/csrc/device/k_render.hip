@@ -323,56 +323,6 @@ PRR_DEV HitGeom hitGeom(const SceneView &s, int prim, float t, const V3 &o, cons
     return g;
 }
 
-// Scene::sampleLight (Scene.cpp:37-58) -> MeshTriangle / BVHAccel::Sample
-// (BVH.cpp:478-514) -> Triangle::Sample / Sphere::Sample.
-PRR_DEV void sampleLight(const SceneView &s, Pcg &rng, V3 &pos, V3 &nrm, int &mat) {
-    const float p = rng.next() * s.total_emit_area;
-    float acc = 0.f;
-    int chosen = s.num_emitters - 1;
-    for (int k = 0; k < s.num_emitters; k++) {
-        acc += s.emitters[k].area;
-        if (p <= acc) {
-            chosen = k;
-            break;
-        }
-    }
-    const DEmitter e = s.emitters[chosen];
-    mat = e.material;
-    if (e.kind == 0) {
-        float q = sqrtf(rng.next()) * e.total;
-        int ni = e.lt_root;
-        int slot = -1;
-        while (slot < 0) {
-            const DLightNode n = s.lnodes[ni];
-            if (n.count > 0) {
-                for (int i = 0; i < n.count; i++) {
-                    q -= s.ltri[4 * (n.first + i)].w;
-                    if (q <= 0.f || i == n.count - 1) {
-                        slot = n.first + i;
-                        break;
-                    }
-                }
-            } else if (q < n.left_area) {
-                ni = ni + 1;
-            } else {
-                q -= n.left_area;
-                ni = n.first;
-            }
-        }
-        const float x = sqrtf(rng.next()), y = rng.next();
-        const V3 v0 = ld3(s.ltri[4 * slot]), v1 = ld3(s.ltri[4 * slot + 1]), v2 = ld3(s.ltri[4 * slot + 2]);
-        pos = v0 * (1.0f - x) + v1 * (x * (1.0f - y)) + v2 * (x * y);
-        nrm = ld3(s.ltri[4 * slot + 3]);
-    } else {
-        const float4 sc = s.sph_by_id[e.lt_root];
-        const float theta = (float) (2.0 * (double) kPi * (double) rng.next());
-        const float phi = kPi * rng.next();
-        const V3 dir(cosf(phi), sinf(phi) * cosf(theta), sinf(phi) * sinf(theta));
-        pos = V3(sc.x, sc.y, sc.z) + sc.w * dir;
-        nrm = dir;
-    }
-}
-
 // What shading one path at one bounce emits: its next extension ray (or a
 // terminating entry that brings it back once more for its pending NEE) and
 // up to three shadow rays (area light, environment, point light).

@@ -4,7 +4,7 @@ Runs in the development container only: builds oracle/_ref/ref_harness from
 the reference's own sources (oracle/Makefile target ``ref``), feeds it the
 seeded inputs below, and stores inputs + reference outputs together, so the
 tests never need the reference. Usage:
-python tests/golden/make_golden.py [trace bsdf env camera tonemap castray photon image welford hdr gltf]
+python tests/golden/make_golden.py [trace bsdf env camera tonemap castray photon image image_field welford hdr gltf]
 (no names: all).
 """
 import os
@@ -165,6 +165,37 @@ def gen_photon(sc, n, radius, max_depth, seed):
                queries=q, ref_photons=out["photons"].reshape(-1, 9), ref_owner=out["owner"],
                ref_irradiance=out["irradiance"].reshape(-1, 3))
     save("photon_%s.npz" % sc.name, res)
+
+
+FIELD_IMAGE = dict(num_spheres=512, env_size=(2048, 1024), w=96, h=64, depth=12, spp=256)
+
+
+def scene_digest(sc):
+    """sha256 of the arrays a scene generator produced (tests regenerate the
+    scene and check it is the one the fixture was made from)."""
+    import hashlib
+    hsh = hashlib.sha256()
+    for k, v in sorted(sc.bundle_arrays().items()):
+        hsh.update(k.encode())
+        hsh.update(np.ascontiguousarray(v).tobytes())
+    return np.frombuffer(hsh.digest(), np.uint8)
+
+
+def gen_image_field():
+    """The reference's own estimator image of a sphere_field instance with the
+    full-size sky (2048 x 1024, 2-degree sun of peak 1e4): the scene is not
+    stored (its sky alone is 25 MB) but regenerated from its seed; the
+    digest pins it."""
+    c = FIELD_IMAGE
+    sc = S.sphere_field(num_spheres=c["num_spheres"], env_size=c["env_size"])
+    arr = sc.bundle_arrays()
+    cfg = np.array([c["w"], c["h"], c["depth"], c["spp"]], np.int32)
+    arr.update(cfg=cfg)
+    out = run("image", arr)
+    res = dict(cfg=cfg, num_spheres=np.array([c["num_spheres"]], np.int32),
+               env_size=np.array(c["env_size"], np.int32), digest=scene_digest(sc),
+               ref_mean=out["mean"], ref_var=out["var"])
+    save("image_sphere_field.npz", res)
 
 
 def welford_sequences(seed=61, nseq=720, ln=64):
@@ -490,6 +521,8 @@ def main():
         gen_photon(TS.textured_room(), 3000, 0.05, 30, 213)
     if want("image"):
         gen_image(box, 64, 64, 8, 256)
+    if want("image_field"):
+        gen_image_field()
     if want("welford"):
         gen_welford()
     if want("hdr"):

@@ -17,11 +17,14 @@ def ctx():
     c.close()
 
 
-def test_photon_map_vs_oracle(ctx):
-    sc = S.glass_torus(nu=64, nv=32)
-    print("case photon map 200k", flush=True)
+@pytest.mark.parametrize("nu,nv,n,r", [(64, 32, 200000, 0.03), (256, 128, 10_000_000, 0.01)])
+def test_photon_map_vs_oracle(ctx, nu, nv, n, r):
+    """The second case is the caustic workload's own shape: the 65,536-triangle
+    torus, 10M photons, gather radius 0.01."""
+    sc = S.glass_torus(nu=nu, nv=nv)
+    print("case photon map %d tris %d photons r=%g" % (sc.num_triangles, n, r), flush=True)
     scene = ctx.scene(sc)
-    n, r, seed = 200000, 0.03, 0xC2
+    seed = 0xC2
     stored = ctx.photon_build(scene, n, r, 30, seed)
     osc = O.OracleScene(sc)
     pm = O.OraclePhotonMap(osc, n, r, 30, seed, threads=8)
@@ -64,3 +67,65 @@ def test_render_with_photons_vs_oracle(ctx):
     # without the photon map the render has no caustic term
     plain, _ = ctx.render(scene, 64, 64, 2, 8, seed=5, photons=False)
     assert g.sum() > plain.cpu().numpy().sum()
+
+
+def _match_by_owner(g_ph, g_own, g_dep, o_ph, o_own, o_dep):
+    """Rows of the two maps keyed by (emitting photon, depth): fraction of
+    keys both hold, and the matched rows (GPU, oracle)."""
+    gk = g_own.astype(np.int64) * 128 + g_dep
+    ok = o_own.astype(np.int64) * 128 + o_dep
+    common, gi, oi = np.intersect1d(gk, ok, return_indices=True)
+    return common.size / max(ok.size, gk.size, 1), g_ph[gi], o_ph[oi]
+
+
+@pytest.mark.parametrize("case", ["box_room", "bright_box_room"])
+def test_reference_rules_photon_map_vs_oracle(ctx, case):
+    """prr_photon_emit (PhotonMap::emit/trace with the reference's rules, from
+    the area light) against the oracle's restatement, pinned bit for bit to
+    the reference's own photons: the same stored photons per (emitting
+    photon, depth) with the same PCG32 streams, and estimateIrradiance at
+    points of the caustic. The bright room (emission x 1e4) keeps the
+    photons' power near 1, so most survive the roulette and bounce on."""
+    sc = S.box_room()
+    n = 200000 if case == "box_room" else 50000
+    if case == "bright_box_room":
+        sc.materials = sc.materials.copy()
+        sc.materials[:, 7:10] *= 1e4
+    print("case reference photon map %s" % case, flush=True)
+    scene = ctx.scene(sc)
+    r, seed = 0.03, 0x5EED
+    stored = ctx.photon_emit(scene, n, r, 30, seed)
+    osc = O.OracleScene(sc)
+    pm = O.OraclePhotonMap(osc, n, r, 30, seed, threads=8, rules="reference")
+    o_ph = pm.export()
+    o_own, o_dep = pm.owners()
+    assert o_ph.shape[0] > 1000
+    assert abs(stored - o_ph.shape[0]) <= 0.01 * o_ph.shape[0], (stored, o_ph.shape[0])
+    g_ph = ctx.photon_export(scene).cpu().numpy()
+    g_own, g_dep = ctx.photon_owners(scene)
+    assert g_ph.shape[0] == stored == g_own.shape[0]
+    if case == "bright_box_room":
+        assert (o_dep > 1).mean() > 0.2  # stored again after diffuse bounces
+    frac, g, o = _match_by_owner(g_ph, g_own, g_dep, o_ph, o_own, o_dep)
+    assert frac > 0.98, frac
+    close = np.all(np.abs(g - o) <= 1e-4 * np.maximum(np.abs(o), 1e-3), axis=1)
+    assert close.mean() > 0.98, close.mean()
+    # estimateIrradiance at points near stored photons
+    rng = np.random.default_rng(6)
+    q = o_ph[rng.integers(0, o_ph.shape[0], 2000), 0:3] + rng.normal(scale=0.01, size=(2000, 3)).astype(np.float32)
+    ge = ctx.photon_irradiance(scene, q).cpu().numpy()
+    oe = pm.irradiance(q)
+    assert (oe > 0).any(1).mean() > 0.9
+    ok = np.all(np.abs(ge - oe) <= 1e-4 * np.abs(oe) + 1e-7 * np.abs(oe).max(), axis=1)
+    assert ok.mean() > 0.97, ok.mean()
+    # renders do not gather the reference's map (castRay never does)
+    with pytest.raises(Exception):
+        scene.photons = n
+        ctx.render(scene, 16, 16, 1, 4, photons=True)
+
+
+def test_photon_emit_needs_an_emitter(ctx):
+    sc = S.glass_torus(nu=16, nv=8)  # point light only
+    scene = ctx.scene(sc)
+    with pytest.raises(Exception, match="emissive"):
+        ctx.photon_emit(scene, 1000, 0.05)
