@@ -19,10 +19,15 @@
 //   Integrator Integrator.hpp:15-45, RenderSettings / Renderer Renderer.hpp:42-59,
 //   tonemap::ToneMapper ToneMapping.hpp:32-37 (+ ACES).
 //
-// Differences a caller sees: sampling is fixed-spp (maxSPP samples per pixel;
-// the reference's adaptive stop rule is not applied), the RNG is PCG32 per
-// (pixel, sample) instead of a thread-local SplitMix64, and there is no CPU
-// fallback: without a HIP device the calls throw std::runtime_error.
+//   BVHAccel BVH.hpp:86-96 (Intersect / IntersectP), Photon / PhotonMap
+//   Photon.hpp:11-47, Scene::sampleLight / pdfLight Scene.hpp:39-40.
+//
+// Differences a caller sees: the RNG is PCG32 per (pixel, sample) instead of
+// a thread-local SplitMix64 (Scene::sampleLight keeps a thread-local
+// SplitMix64 of its own), and there is no CPU fallback: without a HIP device
+// the calls throw std::runtime_error. The reference's headers are forwarded
+// under include/prr/reference/ (Scene.hpp, Renderer.hpp, ...), so a scene
+// script that includes them compiles with -I include/prr/reference.
 #pragma once
 
 #include <cmath>
@@ -343,15 +348,72 @@ struct EnvMap {
 namespace prr_detail {
 struct SceneDevice;
 }
+class Scene;
+
+// BVHAccel (BVH.hpp:86-96): the closest / any-hit queries of a device BVH.
+// Scene::buildBVH sets Scene::bvh to the scene's own; a BVHAccel built from
+// a list of objects uploads a scene of just those objects (the reference's
+// SAH rules: one object per scene-level leaf, four triangles per mesh leaf,
+// so maxPrimsInNode must be 1 and splitMethod SAH). Queries go one ray at a
+// time through the batch entry points (prr_trace_closest / prr_trace_any);
+// batch them there for throughput.
+class BVHAccel {
+  public:
+    enum class SplitMethod { NAIVE, SAH };
+    BVHAccel(std::vector<Object *> p, int maxPrimsInNode = 1, SplitMethod splitMethod = SplitMethod::SAH);
+    ~BVHAccel();
+    BVHAccel(const BVHAccel &) = delete;
+    BVHAccel &operator=(const BVHAccel &) = delete;
+    Intersection Intersect(const Ray &ray) const;
+    bool IntersectP(const Ray &ray, float tMax = std::numeric_limits<float>::infinity()) const;
+
+  private:
+    friend class Scene;
+    explicit BVHAccel(const Scene *scene) : scene_(scene) {}
+    const Scene *scene_ = nullptr;  // the scene whose device tree answers
+    Scene *owned_ = nullptr;        // built from an object list
+};
+
+// Photon (Photon.hpp:11-16)
+struct Photon {
+    Vector3f position;
+    Vector3f direction;
+    Vector3f power;
+    bool is_caustic = false;
+};
+
+// PhotonMap (Photon.hpp:32-47) on the device: emit() traces num_photons from
+// the scene's emitters with the reference's rules (Photon.cpp:96-188;
+// prr_photon_emit) and keeps the grid resident on the device; build() has
+// nothing left to do (the device grid is built by emit); caustic_map holds a
+// host copy of the stored photons; estimateIrradiance (Photon.cpp:82-94)
+// queries the device grid (prr_photon_irradiance). Scenes with only a point
+// light get the build's caustic map instead (prr_photon_build).
+class PhotonMap {
+  public:
+    std::vector<Photon> caustic_map;
+    float photon_radius = 0.01f;
+    int maxDepth = 30;
+    void emit(int num_photons, const Scene &scene);
+    void build() {}
+    Vector3f estimateIrradiance(Vector3f pos, Vector3f normal) const;
+
+  private:
+    friend class Scene;
+    const Scene *scene_ = nullptr;
+};
 
 class Scene {
   public:
     Vector3f backgroundColor = Vector3f(0.235294f, 0.67451f, 0.843137f);
     std::vector<Object *> objects;
+    std::vector<Object *> emissives;      // objects that hasEmit(), set by buildBVH (Scene.cpp:19-25)
+    BVHAccel *bvh = nullptr;              // set by buildBVH; owned by the scene
     std::vector<PointLight> pointLights;  // extension: NEE + photon emission (at most one)
     EnvMap envMap;
     Camera camera;
     float totalEmitArea = 0.f;
+    PhotonMap photon_map;
     Medium medium;
 
     Scene();
@@ -365,11 +427,18 @@ class Scene {
     const std::vector<Object *> &get_objects() const { return objects; }
     // Scene::buildBVH (Scene.cpp:15-27): host SAH build, device upload.
     void buildBVH();
-    // Scene::buildPhotonMaps (Scene.cpp:8-13): device caustic photon map from
-    // the point light (radius 0.01 as Photon.hpp:24).
+    // Scene::buildPhotonMaps (Scene.cpp:8-13): photon_map.emit + build. With
+    // emitters, the reference's own rules; with only a point light, the
+    // build's caustic map that Renderer::Render then gathers.
     void buildPhotonMaps(int num_photons);
     Intersection intersect(const Ray &ray) const;
     bool intersectP(const Ray &ray, float tMax = std::numeric_limits<float>::infinity()) const;
+    // Scene::sampleLight / pdfLight (Scene.cpp:37-62): a point on an emitter,
+    // picked by area (4 draws of a thread-local SplitMix64, as
+    // get_random_float), through prr_sample_light. pdf = 1 / totalEmitArea;
+    // nothing is written when the scene has no emitter.
+    void sampleLight(Intersection &pos, float &pdf) const;
+    float pdfLight(const Intersection &lightInter) const;
 
     prr_detail::SceneDevice *device() const { return dev_; }
 

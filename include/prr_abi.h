@@ -198,6 +198,9 @@ typedef struct prr_stats {
     uint64_t rays_closest_finish; /* closest-hit queries traced there (part of rays_closest) */
     double ms_finish;             /* its duration (with PRR_RENDER_KERNEL_TIMING) */
     int64_t launches_finish;
+    /* samples whose radiance was not finite and was replaced by 0
+       (Integrator.cpp:259-261, Renderer.cpp:80-83) */
+    uint64_t nan_samples;
 } prr_stats;
 
 typedef struct prr_scene_info {
@@ -281,6 +284,15 @@ int prr_bsdf_eval(prr_context* ctx, int64_t n, const float* wi, const float* wo,
 int prr_bsdf_sample(prr_context* ctx, int64_t n, const float* wo, const float* normal,
                     const float* params, const float* u, float* wi, float* f, float* pdf,
                     int32_t* lobe);
+/* Scene::sampleLight (Scene.cpp:37-58: emitter picked by area,
+ * MeshTriangle -> BVHAccel::Sample's sqrt(u) leaf walk, BVH.cpp:478-514, ->
+ * Triangle::Sample, Triangle.hpp:98-103; Sphere::Sample, Sphere.hpp:54-63)
+ * with explicit uniforms, 4 per sample (pick, sqrt(u), x, y; a sphere uses
+ * the first 3). Writes position and light normal (3 floats each), the row of
+ * Scene::emissives picked (emitter, may be NULL) and the area pdf
+ * 1 / totalEmitArea (pdf, may be NULL; Scene::pdfLight, Scene.cpp:60-62). */
+int prr_sample_light(prr_context* ctx, const prr_scene* scene, int64_t n, const float* u4,
+                     float* pos, float* normal, int32_t* emitter, float* pdf);
 /* EnvMap::importanceSample with explicit uniforms (EnvMap.hpp:145-169). */
 int prr_env_sample(prr_context* ctx, const prr_scene* scene, int64_t n, const float* u2,
                    float* dir, float* pdf);

@@ -471,9 +471,19 @@ static int modeCamera(const Bundle &b, const char *out) {
     const auto &pix = b.at("pix"), &offs = b.at("offs");
     const size_t n = pix.n<int32_t>() / 2;
     std::vector<float> o(3 * n), d(3 * n);
+    // with "seeds": each ray on a fresh thread whose generator starts from
+    // seeds[i] (the thin lens draws its two lens uniforms, Camera.hpp:87-88)
+    const Arr *S = b.count("seeds") ? &b.at("seeds") : nullptr;
     for (size_t i = 0; i < n; i++) {
-        Ray r = cam.generateRay(pix.as<int32_t>()[2 * i], pix.as<int32_t>()[2 * i + 1], wh[0], wh[1],
+        Ray r(Vector3f(0.f), Vector3f(0.f, 0.f, 1.f));
+        auto gen = [&]() {
+            r = cam.generateRay(pix.as<int32_t>()[2 * i], pix.as<int32_t>()[2 * i + 1], wh[0], wh[1],
                                 offs.as<float>()[2 * i], offs.as<float>()[2 * i + 1]);
+        };
+        if (S)
+            onFreshThread(S->as<uint32_t>()[i], gen);
+        else
+            gen();
         o[3 * i] = r.origin.x;
         o[3 * i + 1] = r.origin.y;
         o[3 * i + 2] = r.origin.z;

@@ -86,7 +86,8 @@ class Stats(C.Structure):
                 ("photons_stored", C.c_uint64), ("ms_total", C.c_double),
                 ("ms_extend", C.c_double), ("ms_shadow", C.c_double), ("ms_shade", C.c_double),
                 ("launches_extend", C.c_int64), ("launches_shadow", C.c_int64),
-                ("rays_closest_finish", C.c_uint64), ("ms_finish", C.c_double), ("launches_finish", C.c_int64)]
+                ("rays_closest_finish", C.c_uint64), ("ms_finish", C.c_double), ("launches_finish", C.c_int64),
+                ("nan_samples", C.c_uint64)]
 
 
 class SceneInfo(C.Structure):
@@ -162,6 +163,7 @@ SIGNATURES = {
     "prr_trace_any": [vp, vp, i64, vp, vp, vp, vp],
     "prr_bsdf_eval": [vp, i64, vp, vp, vp, vp, vp, vp],
     "prr_bsdf_sample": [vp, i64, vp, vp, vp, vp, vp, vp, vp, vp],
+    "prr_sample_light": [vp, vp, i64, vp, vp, vp, vp, vp],
     "prr_env_sample": [vp, vp, i64, vp, vp, vp],
     "prr_env_eval": [vp, vp, i64, vp, vp, vp],
     "prr_camera_rays": [vp, vp, i32, i32, i64, vp, vp, vp, vp],
@@ -442,6 +444,19 @@ class Context:
                                           _ptr(pdf), _ptr(lobe)))
         return wi, f, pdf, lobe
 
+    def sample_light(self, scene, u4):
+        """Scene::sampleLight with explicit uniforms (4 per sample): position,
+        light normal, emitter row, area pdf."""
+        T = self.torch
+        u4 = self._f32(u4)
+        n = u4.shape[0]
+        pos, nrm = (T.empty((n, 3), dtype=T.float32, device=self.device) for _ in range(2))
+        em = T.empty(n, dtype=T.int32, device=self.device)
+        pdf = T.empty(n, dtype=T.float32, device=self.device)
+        self._bind_stream()
+        self._done(lib().prr_sample_light(self.h, scene.h, n, _ptr(u4), _ptr(pos), _ptr(nrm), _ptr(em), _ptr(pdf)))
+        return pos, nrm, em, pdf
+
     def env_sample(self, scene, u2):
         T = self.torch
         u2 = self._f32(u2)
@@ -575,7 +590,7 @@ class Context:
                "photons_stored": st.photons_stored, "ms": st.ms_total, "ms_extend": st.ms_extend,
                "ms_shadow": st.ms_shadow, "ms_shade": st.ms_shade, "launches_extend": st.launches_extend,
                "launches_shadow": st.launches_shadow, "rays_closest_finish": st.rays_closest_finish,
-               "ms_finish": st.ms_finish, "launches_finish": st.launches_finish}
+               "ms_finish": st.ms_finish, "launches_finish": st.launches_finish, "nan_samples": st.nan_samples}
         if cnt is not None:
             res["spp"] = cnt
         return out, res

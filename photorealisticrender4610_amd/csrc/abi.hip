@@ -225,6 +225,7 @@ int finishStats(prr_context *c, const Timing &tm, uint64_t photons, prr_stats *s
     stats->photons_stored = photons;
     stats->ms_total = ms;
     stats->rays_closest_finish = h[4];
+    stats->nan_samples = h[5];
     stats->ms_extend = stats->ms_shadow = stats->ms_shade = stats->ms_finish = 0.0;
     stats->launches_extend = stats->launches_shadow = stats->launches_finish = 0;
     for (const auto &t : tm.timed) {
@@ -571,6 +572,17 @@ int prr_bsdf_sample(prr_context *c, int64_t n, const float *wo, const float *nrm
         return fail(c, PRR_ERR_INVALID, "bad arrays");
     HIPCK(c, hipSetDevice(c->device));
     HIPCK(c, launchBsdfSample(n, wo, nrm, params, u, wi, f, pdf, lobe, c->stream));
+    return PRR_OK;
+}
+
+int prr_sample_light(prr_context *c, const prr_scene *s, int64_t n, const float *u4, float *pos, float *nrm,
+                     int32_t *emitter, float *pdf) {
+    if (!c || !s) return fail(c, PRR_ERR_INVALID, "NULL context or scene");
+    if (s->view.num_emitters < 1 || !(s->view.total_emit_area > 0.f))
+        return fail(c, PRR_ERR_UNSUPPORTED, "scene has no emitter (Scene::emissives is empty)");
+    if (n < 0 || (n > 0 && (!u4 || !pos || !nrm))) return fail(c, PRR_ERR_INVALID, "bad arrays");
+    HIPCK(c, hipSetDevice(c->device));
+    HIPCK(c, launchSampleLight(s->view, n, u4, pos, nrm, emitter, pdf, c->stream));
     return PRR_OK;
 }
 

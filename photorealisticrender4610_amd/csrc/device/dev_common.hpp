@@ -801,7 +801,10 @@ PRR_DEV V3 exposeLinear(const V3 &fb, float exposure) {  // Renderer.cpp:149-161
 
 // Scene::sampleLight (Scene.cpp:37-58) -> MeshTriangle / BVHAccel::Sample
 // (BVH.cpp:478-514) -> Triangle::Sample / Sphere::Sample.
-PRR_DEV void sampleLight(const SceneView &s, Pcg &rng, V3 &pos, V3 &nrm, int &mat) {
+// R: any generator with next() (PCG32 in the renderer, explicit uniforms in
+// prr_sample_light). *emitter: the row of Scene::emissives picked.
+template <class R>
+PRR_DEV void sampleLight(const SceneView &s, R &rng, V3 &pos, V3 &nrm, int &mat, int *emitter = nullptr) {
     const float p = rng.next() * s.total_emit_area;
     float acc = 0.f;
     int chosen = s.num_emitters - 1;
@@ -814,6 +817,7 @@ PRR_DEV void sampleLight(const SceneView &s, Pcg &rng, V3 &pos, V3 &nrm, int &ma
     }
     const DEmitter e = s.emitters[chosen];
     mat = e.material;
+    if (emitter) *emitter = chosen;
     if (e.kind == 0) {
         float q = sqrtf(rng.next()) * e.total;
         int ni = e.lt_root;

@@ -20,6 +20,21 @@ struct FixedU {
     PRR_DEV float next() { return u[k++]; }
 };
 
+// Scene::sampleLight (Scene.cpp:37-58) with explicit uniforms: 4 per sample.
+__global__ void kSampleLight(SceneView s, int64_t n, const float *u, float *pos, float *nrm, int32_t *emitter,
+                             float *pdf) {
+    const int64_t i = (int64_t) blockIdx.x * kBlk + threadIdx.x;
+    if (i >= n) return;
+    FixedU r{u + 4 * i, 0};
+    V3 p, nn;
+    int mat = 0, e = 0;
+    sampleLight(s, r, p, nn, mat, &e);
+    stv(pos, i, p);
+    stv(nrm, i, nn);
+    if (emitter) emitter[i] = e;
+    if (pdf) pdf[i] = 1.0f / s.total_emit_area;  // Scene.cpp:46
+}
+
 __global__ void kBsdfEval(int64_t n, const float *wi, const float *wo, const float *nrm, const float *params, float *f,
                           float *pdf) {
     const int64_t i = (int64_t) blockIdx.x * kBlk + threadIdx.x;
@@ -140,6 +155,14 @@ hipError_t launchTonemap(int op, int64_t n, float exposure, const float *lin, fl
                          const AgxConst &agx, hipStream_t st) {
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(kTonemap, grid(n), dim3(kBlk), 0, st, op, n, exposure, lin, disp, rgb8, agx);
+    return hipGetLastError();
+}
+
+hipError_t launchSampleLight(const SceneView &s, int64_t n, const float *u, float *pos, float *nrm, int32_t *emitter,
+                            float *pdf, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(kSampleLight, dim3((unsigned) ((n + kBlk - 1) / kBlk)), dim3(kBlk), 0, st, s, n, u, pos, nrm,
+                       emitter, pdf);
     return hipGetLastError();
 }
 

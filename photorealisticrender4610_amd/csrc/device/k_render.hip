@@ -35,6 +35,13 @@ PRR_DEV uint32_t waveAppend(uint32_t *counter, bool want) {
     return base + (uint32_t) __popcll(below);
 }
 
+// Adds the wave's count of replaced (non-finite) samples to stats[5]: one
+// atomic per wave. Every lane of the wave calls it.
+PRR_DEV void countNan(const PathBufs &pb, uint32_t n) {
+    for (int o = 32; o > 0; o >>= 1) n += __shfl_xor(n, o);
+    if (laneId() == 0 && n) atomicAdd(&pb.stats[5], (unsigned long long) n);
+}
+
 PRR_DEV uint64_t pcgInc(uint32_t sample) { return ((uint64_t) sample << 1) | 1u; }
 
 // Pixel/sample hash selecting radiance-cache training paths (lowbias32 mix).
@@ -188,11 +195,15 @@ __global__ __launch_bounds__(kShadeBlock) void kRaygenRays(PathBufs pb, int64_t 
 
 __global__ void kCopyResults(PathBufs pb, int64_t n, int64_t first, float *L) {
     const int64_t i = (int64_t) blockIdx.x * 256 + threadIdx.x;
-    if (i >= n) return;
-    const float4 r = pb.result[i];
-    L[3 * (first + i)] = r.x;
-    L[3 * (first + i) + 1] = r.y;
-    L[3 * (first + i) + 2] = r.z;
+    uint32_t nan = 0;
+    if (i < n) {
+        const float4 r = pb.result[i];
+        L[3 * (first + i)] = r.x;
+        L[3 * (first + i) + 1] = r.y;
+        L[3 * (first + i) + 2] = r.z;
+        nan = r.w != 0.f ? 1u : 0u;
+    }
+    countNan(pb, nan);
 }
 
 // Persistent traversal: idle lanes refill from the queue once `refillAt`
@@ -608,8 +619,9 @@ __device__ __forceinline__ void shadePath(const SceneView &s, const PathBufs &pb
         done = false;
     }
     if (done) {
-        const V3 r = finite3(L) ? L : V3(0.f);  // Integrator.cpp:259-261
-        pb.result[slot] = st4(r, 0.f);
+        // Integrator.cpp:259-261; w = 1 marks the replaced sample (counted at accumulation)
+        const bool fin = finite3(L);
+        pb.result[slot] = st4(fin ? L : V3(0.f), fin ? 0.f : 1.f);
     } else {
         pb.L[slot] = st4(L, 0.f);
         pb.beta[slot] = st4(beta, 0.f);
@@ -814,10 +826,17 @@ __global__ void kCacheTargets(PathBufs pb, CacheRec cr, uint32_t r0, uint32_t r1
 // Renderer.cpp:84-99: framebuffer[m] += s in sample order.
 __global__ void kAccumulate(PathBufs pb, ChunkInfo ci, float4 *fb) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= ci.pixels) return;
-    V3 acc = ld3(fb[p]);
-    for (int k = 0; k < ci.spc; k++) acc += ld3(pb.result[(int64_t) k * ci.pixels + p]);
-    fb[p] = st4(acc, 0.f);
+    uint32_t nan = 0;
+    if (p < ci.pixels) {
+        V3 acc = ld3(fb[p]);
+        for (int k = 0; k < ci.spc; k++) {
+            const float4 r = pb.result[(int64_t) k * ci.pixels + p];
+            acc += ld3(r);
+            nan += r.w != 0.f ? 1u : 0u;
+        }
+        fb[p] = st4(acc, 0.f);
+    }
+    countNan(pb, nan);
 }
 
 // Welford::update (Renderer.hpp:20-26) on one channel.
@@ -833,20 +852,23 @@ PRR_DEV float welfordCi(int n, float mean, float m2) {
     return 1.96f * sqrtf(var / (float) n) / fabsf(mean);
 }
 
-// Renderer.cpp:74-99 with the adaptive stop rule: samples of the chunk are
-// taken in sample order until n >= minSpp and all three channel CIs are below
-// thr; later samples of a stopped pixel are discarded.
-__global__ void kAccumulateAdaptive(PathBufs pb, ChunkInfo ci, float4 *fb, AdaptiveBufs ab, int minSpp, float thr) {
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= ci.pixels) return;
+// Renderer.cpp:74-99 with the adaptive stop rule, for one pixel: samples of
+// the chunk are taken in sample order until n >= minSpp and all three
+// channel CIs are below thr; later samples of a stopped pixel are discarded.
+// Returns how many of the taken samples were replaced (non-finite).
+PRR_DEV uint32_t accumulateAdaptive(const PathBufs &pb, const ChunkInfo &ci, float4 *fb, const AdaptiveBufs &ab,
+                                    int minSpp, float thr, int e) {
     const int p = ci.pixList ? ci.pixList[e] : e;
     float4 m = ab.mean[p], q = ab.m2[p];
-    if (q.w != 0.f) return;
+    if (q.w != 0.f) return 0u;
     int n = __float_as_int(m.w);
     V3 acc = ld3(fb[p]);
     bool stop = false;
+    uint32_t nan = 0;
     for (int k = 0; k < ci.spc && !stop; k++) {
-        const V3 sv = ld3(pb.result[(int64_t) k * ci.pixels + e]);
+        const float4 r4 = pb.result[(int64_t) k * ci.pixels + e];
+        const V3 sv = ld3(r4);
+        nan += r4.w != 0.f ? 1u : 0u;
         acc += sv;
         n++;
         welfordUpdate(sv.x, n, m.x, q.x);
@@ -860,6 +882,14 @@ __global__ void kAccumulateAdaptive(PathBufs pb, ChunkInfo ci, float4 *fb, Adapt
     q.w = stop ? 1.f : 0.f;
     ab.mean[p] = m;
     ab.m2[p] = q;
+    return nan;
+}
+
+__global__ void kAccumulateAdaptive(PathBufs pb, ChunkInfo ci, float4 *fb, AdaptiveBufs ab, int minSpp, float thr) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t nan = 0;
+    if (e < ci.pixels) nan = accumulateAdaptive(pb, ci, fb, ab, minSpp, thr, e);
+    countNan(pb, nan);
 }
 
 // Active pixels (not stopped, fewer than spp samples) in pixel order within

@@ -25,6 +25,8 @@ hipError_t launchTraceClosestNoAlloc(const SceneView &s, int64_t n, const float 
                                      float *t, float *u, float *v, uint2 *scratchPtr, int stackDepth, hipStream_t st);
 
 // k_misc.hip
+hipError_t launchSampleLight(const SceneView &s, int64_t n, const float *u, float *pos, float *nrm, int32_t *emitter,
+                            float *pdf, hipStream_t st);
 hipError_t launchBsdfEval(int64_t n, const float *wi, const float *wo, const float *nrm, const float *params,
                           float *f, float *pdf, hipStream_t st);
 hipError_t launchBsdfSample(int64_t n, const float *wo, const float *nrm, const float *params, const float *u,

@@ -16,6 +16,7 @@
 #include <cstring>
 #include <memory>
 #include <mutex>
+#include <random>
 #include <stdexcept>
 #include <string>
 #include <unordered_map>
@@ -186,26 +187,36 @@ void EnvMap::set(int w, int h, const float *rgb) {
 Scene::Scene() : dev_(new prr_detail::SceneDevice) {}
 
 Scene::~Scene() {
+    delete bvh;
     if (dev_ && dev_->scene && prr_detail::g_ctx) prr_scene_free(prr_detail::g_ctx, dev_->scene);
     delete dev_;
 }
 
 Scene::Scene(Scene &&o) noexcept
-    : backgroundColor(o.backgroundColor), objects(std::move(o.objects)), pointLights(std::move(o.pointLights)),
-      envMap(std::move(o.envMap)), camera(o.camera), totalEmitArea(o.totalEmitArea), medium(o.medium),
-      dev_(o.dev_) {
+    : backgroundColor(o.backgroundColor), objects(std::move(o.objects)), emissives(std::move(o.emissives)),
+      bvh(o.bvh), pointLights(std::move(o.pointLights)), envMap(std::move(o.envMap)), camera(o.camera),
+      totalEmitArea(o.totalEmitArea), photon_map(std::move(o.photon_map)), medium(o.medium), dev_(o.dev_) {
     o.dev_ = new prr_detail::SceneDevice;
+    o.bvh = nullptr;
+    if (bvh) bvh->scene_ = this;
+    if (photon_map.scene_) photon_map.scene_ = this;
 }
 
 Scene &Scene::operator=(Scene &&o) noexcept {
     if (this != &o) {
         std::swap(dev_, o.dev_);
+        std::swap(bvh, o.bvh);
+        if (bvh) bvh->scene_ = this;
+        if (o.bvh) o.bvh->scene_ = &o;
         backgroundColor = o.backgroundColor;
         objects = std::move(o.objects);
+        emissives = std::move(o.emissives);
         pointLights = std::move(o.pointLights);
         envMap = std::move(o.envMap);
         camera = o.camera;
         totalEmitArea = o.totalEmitArea;
+        photon_map = std::move(o.photon_map);
+        if (photon_map.scene_) photon_map.scene_ = this;
         medium = o.medium;
     }
     return *this;
@@ -503,17 +514,137 @@ void Scene::buildBVH() {
     prr_scene_info info{};
     check(prr_scene_get_info(D.scene, &info), "prr_scene_get_info");
     totalEmitArea = info.total_emit_area;
+    // Scene.cpp:19-25: the emitters, in object order
+    emissives.clear();
+    for (Object *o : objects)
+        if (o->hasEmit()) emissives.push_back(o);
+    if (!bvh) bvh = new BVHAccel(this);
 }
 
-// Scene::buildPhotonMaps (Scene.cpp:8-13) with PhotonMap's radius 0.01 and
-// depth 30 (Photon.hpp:36-37), emitted from the point light.
+// Scene::buildPhotonMaps (Scene.cpp:8-13) with PhotonMap's radius and depth
+// (0.01, 30: Photon.hpp:36-37).
 void Scene::buildPhotonMaps(int num_photons) {
-    if (!dev_->scene) throw std::runtime_error("Scene::buildPhotonMaps before buildBVH");
-    int64_t stored = 0;
-    check(prr_photon_build(context(), dev_->scene, num_photons, 0.01f, 30, 0x9E3779B97F4A7C15ull, &stored),
-          "prr_photon_build");
-    dev_->photons = num_photons;
+    photon_map.emit(num_photons, *this);
+    photon_map.build();
 }
+
+// PhotonMap::emit: the reference's rules from the emitters (prr_photon_emit),
+// or, in a scene with only a point light, the build's caustic map
+// (prr_photon_build), which Renderer::Render gathers. The stored photons are
+// copied to caustic_map.
+void PhotonMap::emit(int num_photons, const Scene &scene) {
+    prr_detail::SceneDevice *D = scene.device();
+    if (!D->scene) throw std::runtime_error("PhotonMap::emit before Scene::buildBVH");
+    prr_context *ctx = context();
+    std::lock_guard<std::mutex> lk(D->mu);
+    int64_t stored = 0;
+    const uint64_t seed = 0x9E3779B97F4A7C15ull;
+    if (scene.totalEmitArea > 0.f) {
+        check(prr_photon_emit(ctx, D->scene, num_photons, photon_radius, maxDepth, seed, &stored), "prr_photon_emit");
+        D->photons = 0;
+    } else {
+        check(prr_photon_build(ctx, D->scene, num_photons, photon_radius, maxDepth, seed, &stored),
+              "prr_photon_build");
+        D->photons = num_photons;
+    }
+    scene_ = &scene;
+    caustic_map.assign((size_t) stored, Photon());
+    if (stored <= 0) return;
+    float *dev = (float *) D->buf.get((size_t) stored * 9 * sizeof(float));
+    int64_t cnt = 0;
+    check(prr_photon_export(ctx, D->scene, stored, dev, &cnt), "prr_photon_export");
+    check(prr_synchronize(ctx), "prr_synchronize");
+    std::vector<float> h((size_t) stored * 9);
+    if (hipMemcpy(h.data(), dev, h.size() * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess)
+        throw std::runtime_error("hipMemcpy");
+    for (int64_t i = 0; i < stored; i++) {
+        const float *q = h.data() + 9 * i;
+        caustic_map[i] = Photon{Vector3f(q[0], q[1], q[2]), Vector3f(q[3], q[4], q[5]), Vector3f(q[6], q[7], q[8]),
+                                true};
+    }
+}
+
+// PhotonMap::estimateIrradiance (Photon.cpp:82-94) on the device grid.
+Vector3f PhotonMap::estimateIrradiance(Vector3f pos, Vector3f normal) const {
+    (void) normal;
+    if (!scene_) return Vector3f(0.f);
+    prr_detail::SceneDevice *D = scene_->device();
+    prr_context *ctx = context();
+    std::lock_guard<std::mutex> lk(D->mu);
+    float *b = (float *) D->buf.get(256);
+    const float p[3] = {pos.x, pos.y, pos.z};
+    if (hipMemcpy(b, p, sizeof p, hipMemcpyHostToDevice) != hipSuccess) throw std::runtime_error("hipMemcpy");
+    check(prr_photon_irradiance(ctx, D->scene, 1, b, nullptr, b + 4), "prr_photon_irradiance");
+    check(prr_synchronize(ctx), "prr_synchronize");
+    float e[3];
+    if (hipMemcpy(e, b + 4, sizeof e, hipMemcpyDeviceToHost) != hipSuccess) throw std::runtime_error("hipMemcpy");
+    return Vector3f(e[0], e[1], e[2]);
+}
+
+// get_random_float (global.hpp:35-50): a thread-local SplitMix64 seeded from
+// std::random_device, top 24 bits.
+static float hostRandomFloat() {
+    thread_local uint64_t state = [] {
+        std::random_device rd;
+        return (uint64_t) rd() << 32 | rd();
+    }();
+    uint64_t z = (state += 0x9E3779B97F4A7C15ULL);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    z ^= (z >> 31);
+    return (float) (z >> 40) * 0x1p-24f;
+}
+
+void Scene::sampleLight(Intersection &pos, float &pdf) const {
+    if (totalEmitArea <= 0.f) return;  // Scene.cpp:38-39
+    if (!dev_->scene) throw std::runtime_error("Scene::sampleLight before buildBVH");
+    prr_context *ctx = context();
+    std::lock_guard<std::mutex> lk(dev_->mu);
+    float u[4];
+    for (float &x : u) x = hostRandomFloat();
+    float *b = (float *) dev_->buf.get(256);
+    if (hipMemcpy(b, u, sizeof u, hipMemcpyHostToDevice) != hipSuccess) throw std::runtime_error("hipMemcpy");
+    int32_t *em = (int32_t *) (b + 12);
+    check(prr_sample_light(ctx, dev_->scene, 1, b, b + 4, b + 8, em, b + 11), "prr_sample_light");
+    check(prr_synchronize(ctx), "prr_synchronize");
+    float h[13];
+    if (hipMemcpy(h, b, sizeof h, hipMemcpyDeviceToHost) != hipSuccess) throw std::runtime_error("hipMemcpy");
+    int32_t k;
+    std::memcpy(&k, &h[12], 4);
+    pos.coords = Vector3f(h[4], h[5], h[6]);
+    pos.normal = Vector3f(h[8], h[9], h[10]);
+    pos.happened = true;
+    if (k >= 0 && k < (int32_t) emissives.size()) {
+        Object *o = emissives[k];
+        pos.obj = o;
+        if (auto *mesh = dynamic_cast<MeshTriangle *>(o))
+            pos.material = mesh->m;
+        else if (auto *sph = dynamic_cast<Sphere *>(o))
+            pos.material = sph->m;
+    }
+    pdf = 1.0f / totalEmitArea;  // Scene.cpp:46
+}
+
+float Scene::pdfLight(const Intersection &lightInter) const {  // Scene.cpp:60-62
+    (void) lightInter;
+    return totalEmitArea > 0.f ? 1.f / totalEmitArea : 0.f;
+}
+
+// BVHAccel over a list of objects: a scene of just those objects.
+BVHAccel::BVHAccel(std::vector<Object *> p, int maxPrimsInNode, SplitMethod splitMethod) {
+    if (maxPrimsInNode != 1 || splitMethod != SplitMethod::SAH)
+        throw std::invalid_argument("BVHAccel: the device tree follows Scene::buildBVH (maxPrimsInNode 1, SAH)");
+    owned_ = new Scene();
+    owned_->objects = std::move(p);
+    owned_->buildBVH();
+    scene_ = owned_;
+}
+
+BVHAccel::~BVHAccel() { delete owned_; }
+
+Intersection BVHAccel::Intersect(const Ray &ray) const { return scene_->intersect(ray); }
+
+bool BVHAccel::IntersectP(const Ray &ray, float tMax) const { return scene_->intersectP(ray, tMax); }
 
 static void rayArrays(const Ray &ray, float *od) {
     od[0] = ray.origin.x, od[1] = ray.origin.y, od[2] = ray.origin.z;

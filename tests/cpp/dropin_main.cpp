@@ -8,6 +8,8 @@
 //       output files into <outdir>, the linear image to <outdir>/linear.f32
 //       and single-query results (Scene::intersect / intersectP,
 //       Integrator::castRay) to <outdir>/queries.txt
+//   dropin_main api <outdir>   BVHAccel, Scene::sampleLight / pdfLight and the
+//       photon map on the box room (<outdir>/api.txt, photons.f32)
 //   dropin_main hdr <file.hdr> <out.f32>     EnvMap::load; w, h, pixels
 //   dropin_main obj <file.obj> <dx> <dy> <dz> MeshTriangle from OBJ; prints
 //       triangle count, area and bounds
@@ -110,6 +112,57 @@ static int render(int argc, char **argv) {
     return 0;
 }
 
+// The per-query API of the reference on the box room: BVHAccel (the
+// scene's own and one built from the object list), Scene::sampleLight /
+// pdfLight and the photon map (Scene::buildPhotonMaps, caustic_map,
+// estimateIrradiance). Writes one line per query to <outdir>/api.txt.
+static int api(int argc, char **argv) {
+    if (argc < 3) return 2;
+    const std::string out = argv[2];
+    Scene scene = buildScene();
+    scene.camera.init(64, 48);
+    scene.buildBVH();
+    FILE *q = fopen((out + "/api.txt").c_str(), "w");
+    if (!q) return 1;
+    fprintf(q, "emissives %d pdf %.9g\n", (int) scene.emissives.size(), scene.pdfLight(Intersection()));
+    BVHAccel own(scene.objects, 1, BVHAccel::SplitMethod::SAH);
+    const Vector3f targets[4] = {Vector3f(-0.45f, 0.4f, -0.3f), Vector3f(0.45f, 0.4f, 0.25f),
+                                 Vector3f(0.f, 2.f, 0.f), Vector3f(0.f, 1.f, -1.f)};
+    for (const Vector3f &tg : targets) {
+        Ray ray(scene.camera.eye, normalize(tg - scene.camera.eye));
+        Intersection a = scene.intersect(ray), b = scene.bvh->Intersect(ray), c = own.Intersect(ray);
+        fprintf(q, "bvh %.9g %.9g %.9g %d %d %d\n", a.tnear, b.tnear, c.tnear, scene.bvh->IntersectP(ray, 100.f) ? 1 : 0,
+                own.IntersectP(ray, 100.f) ? 1 : 0, own.IntersectP(ray, 0.5f) ? 1 : 0);
+    }
+    for (int k = 0; k < 64; k++) {
+        Intersection it;
+        float pdf = 0.f;
+        scene.sampleLight(it, pdf);
+        fprintf(q, "light %.9g %.9g %.9g %.9g %.9g %.9g %.9g %.9g %d\n", it.coords.x, it.coords.y, it.coords.z,
+                it.normal.x, it.normal.y, it.normal.z, pdf, it.material ? it.material->m_emission.x : -1.f,
+                it.obj == scene.emissives[0] ? 1 : 0);
+    }
+    scene.photon_map.photon_radius = 0.05f;
+    scene.buildPhotonMaps(200000);
+    const auto &cm = scene.photon_map.caustic_map;
+    fprintf(q, "photons %d\n", (int) cm.size());
+    for (size_t k = 0; k < cm.size() && k < 40; k += 4) {
+        const Vector3f p = cm[k].position + Vector3f(0.01f, 0.f, 0.01f);
+        const Vector3f e = scene.photon_map.estimateIrradiance(p, Vector3f(0.f, 1.f, 0.f));
+        fprintf(q, "irr %.9g %.9g %.9g %.9g %.9g %.9g\n", p.x, p.y, p.z, e.x, e.y, e.z);
+    }
+    fclose(q);
+    FILE *f = fopen((out + "/photons.f32").c_str(), "wb");
+    if (!f) return 1;
+    for (const Photon &ph : cm) {
+        const float row[9] = {ph.position.x, ph.position.y, ph.position.z, ph.direction.x, ph.direction.y,
+                              ph.direction.z, ph.power.x, ph.power.y, ph.power.z};
+        fwrite(row, sizeof(float), 9, f);
+    }
+    fclose(f);
+    return 0;
+}
+
 static int hdr(int argc, char **argv) {
     if (argc < 4) return 2;
     EnvMap env;
@@ -172,6 +225,7 @@ int main(int argc, char **argv) {
     }
     try {
         if (!std::strcmp(argv[1], "render")) return render(argc, argv);
+        if (!std::strcmp(argv[1], "api")) return api(argc, argv);
         if (!std::strcmp(argv[1], "hdr")) return hdr(argc, argv);
         if (!std::strcmp(argv[1], "obj")) return obj(argc, argv);
         if (!std::strcmp(argv[1], "gltf")) return gltf(argc, argv);

@@ -107,6 +107,17 @@ def gen_camera(seed=41):
         out = run("camera", dict(camera=cam, wh=np.array([w, h], np.int32), pix=pix.ravel(), offs=offs.ravel()))
         res.update({name + "_camera": cam, name + "_wh": np.array([w, h], np.int32), name + "_pix": pix,
                     name + "_offs": offs, name + "_ref_org": out["org"], name + "_ref_dir": out["dir"]})
+    # thin lens (Camera.hpp:58-94): aperture 0.08, focus 3.2; each ray draws
+    # its lens uniforms from its own seeded generator
+    cam = np.array([0, 1, 3.9, 0, 1, 0, 0, 1, 0, 40, 0.08, 3.2], np.float32)
+    w, h, n = 256, 192, 1000
+    pix = np.stack([rng.integers(0, w, n), rng.integers(0, h, n)], 1).astype(np.int32)
+    offs = rng.random((n, 2)).astype(np.float32)
+    seeds = seeds_u32(n, seed + 1)
+    out = run("camera", dict(camera=cam, wh=np.array([w, h], np.int32), pix=pix.ravel(), offs=offs.ravel(),
+                             seeds=seeds))
+    res.update({"lens_camera": cam, "lens_wh": np.array([w, h], np.int32), "lens_pix": pix, "lens_offs": offs,
+                "lens_seeds": seeds, "lens_ref_org": out["org"], "lens_ref_dir": out["dir"]})
     save("camera.npz", res)
 
 

@@ -45,6 +45,15 @@ def test_trace_bit_exact_vs_reference(ctx, name):
     assert np.array_equal(anyh, g["ref_any"]), "any-hit differs from BVH::IntersectP"
     info = scene.info()
     assert info["total_emit_area"] == float(g["ref_total_emit_area"][0])
+    if info["total_emit_area"] <= 0:
+        return  # no emitter (the soup): nothing to sample
+    # Scene::sampleLight with the reference's own draws (4 per sample from
+    # each seeded generator), bit for bit
+    u4 = np.stack([O.splitmix_draws(int(s), 4) for s in g["light_seeds"]])
+    pos, nrm, em, pdf = (_np(x) for x in ctx.sample_light(scene, u4))
+    assert np.array_equal(pos.ravel(), g["ref_light_pos"])
+    assert np.array_equal(nrm.ravel(), g["ref_light_nrm"])
+    assert np.array_equal(pdf, g["ref_light_pdf"])
 
 
 @pytest.mark.parametrize("name", TRACE_SETS)
@@ -181,6 +190,17 @@ def test_camera_vs_reference(ctx):
         o, d = (_np(x) for x in ctx.camera_rays(scene, w, h, g[name + "_pix"], g[name + "_offs"]))
         assert np.array_equal(o.ravel(), g[name + "_ref_org"])
         assert np.array_equal(d.ravel(), g[name + "_ref_dir"])
+    # thin lens: the reference's lens draws passed as explicit uniforms; the
+    # disc sample's cos / sin are the device's (1e-5)
+    from test_oracle_golden import lens_offsets
+    b = scenes.SceneBuilder()
+    b.sphere((0, 0, 0), 1.0, b.material())
+    b.camera = g["lens_camera"]
+    scene = ctx.scene(b.build())
+    w, h = (int(x) for x in g["lens_wh"])
+    o, d = (_np(x) for x in ctx.camera_rays(scene, w, h, g["lens_pix"], lens_offsets(g)))
+    assert _close(o.ravel(), g["lens_ref_org"], 1e-5, 1e-6).all()
+    assert _close(d.ravel(), g["lens_ref_dir"], 1e-5, 1e-6).all()
 
 
 def test_tonemap_vs_reference(ctx):

@@ -168,6 +168,26 @@ def test_camera_matches_reference():
         assert np.array_equal(d.ravel(), g[name + "_ref_dir"])
 
 
+def lens_offsets(g):
+    """Pixel offsets plus the two lens uniforms each reference ray drew from
+    its seeded generator (Camera.hpp:87-88)."""
+    uv = np.stack([O.splitmix_draws(int(s), 2) for s in g["lens_seeds"]])
+    return np.concatenate([g["lens_offs"], uv], 1).astype(np.float32)
+
+
+def test_thin_lens_camera_matches_reference():
+    """Camera::generateRay with aperture > 0 (concentric disc sample on the
+    lens, ray through the focus plane; Camera.hpp:58-94) bit for bit."""
+    g = load_golden("camera.npz")
+    w, h = (int(x) for x in g["lens_wh"])
+    o, d = O.camera_rays(g["lens_camera"], w, h, g["lens_pix"], lens_offsets(g))
+    assert np.array_equal(o.ravel(), g["lens_ref_org"])
+    assert np.array_equal(d.ravel(), g["lens_ref_dir"])
+    # the lens spreads the origins over the aperture
+    spread = np.linalg.norm(o - np.array([0, 1, 3.9], np.float32), axis=1)
+    assert spread.max() <= 0.08 * 1.0001 and spread.mean() > 0.03
+
+
 def test_tonemap_matches_reference():
     g = load_golden("tonemap.npz")
     for op, name in enumerate(["raw", "standard", "agx", "pbrneutral"]):
