@@ -87,6 +87,7 @@ def reference_baseline(exe, sc, w, h, depth, target_s=12.0):
     closest, anyq, samples, used = (c[0] << 32) | c[1], (c[2] << 32) | c[3], (c[4] << 32) | c[5], c[6]
     rays = closest + anyq
     return {"value": round(rays / sec / 1e6, 3), "unit": "Mrays/s", "cores": used, "kind": "reference",
+            "cpu_model": cpu_model(),
             "sample": "the reference's own castRay / BVH / Render loop (oracle/_ref/ref_bench, -O3 -ffast-math "
                       "-march=x86-64-v3 OpenMP, no LTO) on rows %d.. of the %dx%d image at %d spp: %d samples, "
                       "%d rays (%d closest, %d any-hit) in %.1f s; its scene build (%.1f s) excluded"
@@ -123,23 +124,53 @@ def cpu_baseline(sc, w, h, depth, seed, target_s=12.0):
     dt = time.time() - t0
     rays = cnt["closest"] + cnt["any"]
     return {"value": round(rays / dt / 1e6, 3), "unit": "Mrays/s", "cores": threads, "kind": "port",
+            "cpu_model": cpu_model(),
             "sample": "rows %d-%d of the %dx%d image at %d spp (%d samples, %d rays) in %.1f s; "
                       "oracle scene build %.1f s excluded" % (r0, r0 + rows, w, h, spp, cnt["samples"], rays, dt,
                                                               build_s)}
 
 
 def traversal_bytes(sc, w, h, depth, seed):
-    """Algorithmic bytes per closest-hit ray, from the oracle's restatement of
-    BVHAccel::Intersect on a band of the same render: 64 B per interior
+    """Algorithmic bytes per ray, from the oracle's restatement of the
+    reference's traversal on a band of the same render: 64 B per interior
     expansion (two child boxes) + 48 B per triangle test (v0, e1, e2 + ids)
-    + 32 B per sphere test + 48 B ray in / hit out."""
+    + 48 B ray in / result out. Closest hit: BVHAccel::Intersect's counts;
+    any hit: BVHAccel::IntersectP's on the same binary tree (the device
+    traces shadow rays on a four-wide tree built from it, whose leaves are
+    the same)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle as O
     osc = O.OracleScene(sc)
     r0 = h // 2 - 4
     _, _, cnt = osc.render(w, h, 1, depth, seed, rows=(r0, r0 + 8))
     per_ray = (64.0 * cnt["inner"] + 48.0 * cnt["tris"]) / max(cnt["closest"], 1) + 48.0
-    return per_ray, cnt
+    per_any = (64.0 * cnt["inner_any"] + 48.0 * cnt["tris_any"]) / max(cnt["any"], 1) + 48.0
+    return per_ray, per_any, cnt
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def calibrate_copy(mib):
+    """A plain device copy of known size for calibrating rocprofv3's
+    FETCH_SIZE / WRITE_SIZE counters: a copy of B bytes fetches and writes B
+    bytes (run under rocprofv3 --pmc)."""
+    import torch
+    n = mib * (1 << 20) // 4
+    x = torch.rand(n, device="cuda")
+    y = torch.empty_like(x)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    log("copied %d MiB x3" % mib)
 
 
 def profiled_traffic(workload, kernel="kExtend"):
@@ -387,7 +418,11 @@ def main():
     ap.add_argument("--frames", type=int, default=0, help="radiance_cache: override the frame count")
     ap.add_argument("--spp", type=int, default=0, help="override spp (profiling runs only)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB (counter calibration under rocprofv3 --pmc)")
     args = ap.parse_args()
+    if args.calibrate_copy > 0:
+        return calibrate_copy(args.calibrate_copy)
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path (the render would shard "
               "by pixels/samples with one final reduce). Use --gpus 1.", file=sys.stderr)
@@ -430,6 +465,9 @@ def main():
     launches = 0
     t0 = time.perf_counter()
     stored = 0
+    anyr = 0
+    ms_sh = 0.0
+    launches_sh = 0
     for k in range(args.steps):
         stored = photon_pass(100 + k)
         _, st = ctx.render(scene, W, H, SPP, DEPTH, seed=seed + 100 + k, out=out, kernel_timing=True,
@@ -437,24 +475,42 @@ def main():
         ctx.tonemap("aces", out.view(-1, 3), 1.0)
         rays += st["rays_closest"] + st["rays_any"]
         closest += st["rays_closest"] - st["rays_closest_finish"]  # the rays kExtend traced
+        anyr += st["rays_any"] - st["rays_any_finish"]  # the rays kShadow traced
         ms_ext += st["ms_extend"]
         launches += st["launches_extend"]
-        log("step %d: %.1f ms (extend %.1f, shadow %.1f, shade %.1f, finish %.1f), %d rays" % (
-            k, st["ms"], st["ms_extend"], st["ms_shadow"], st["ms_shade"], st["ms_finish"],
+        ms_sh += st["ms_shadow"]
+        launches_sh += st["launches_shadow"]
+        log("step %d: %.1f ms (extend %.1f, shadow %.1f, shadow binning %.1f, shade %.1f, finish %.1f), %d rays" % (
+            k, st["ms"], st["ms_extend"], st["ms_shadow"], st["ms_sort"], st["ms_shade"], st["ms_finish"],
             st["rays_closest"] + st["rays_any"]))
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
     value = rays / dt / 1e6
 
-    per_ray, ocnt = traversal_bytes(sc, W, H, DEPTH, seed)
-    achieved = (closest * per_ray / 1e9) / (ms_ext / 1e3) if ms_ext > 0 else 0.0
-    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": profiled_traffic(args.workload) if args.spp == 0 else None,
-                "traffic_unit": "HBM bytes per launch (rocprofv3 --pmc, copy-calibrated)",
-                "kernel": "kExtend (closest-hit traversal)",
-                "bytes_per_ray": round(per_ray, 1),
-                "avg_launch_ms": round(ms_ext / max(launches, 1), 4)}
+    per_ray, per_any, ocnt = traversal_bytes(sc, W, H, DEPTH, seed)
+
+    def kernel_roofline(name, desc, ms, nlaunch, nrays, bytes_per_ray):
+        """achieved / frac: bytes the kernel moved between L2 and the memory
+        fabric per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated,
+        from the committed counter summary of this workload) over the average
+        launch time measured here with HIP events; the algorithmic bytes
+        (what a cacheless machine would move) are reported beside them."""
+        avg_ms = ms / max(nlaunch, 1)
+        traffic = profiled_traffic(args.workload, name) if args.spp == 0 else None
+        alg = nrays / max(nlaunch, 1) * bytes_per_ray
+        ach = traffic / (avg_ms / 1e3) / 1e9 if traffic and avg_ms > 0 else None
+        return {"bound": "hbm", "achieved": round(ach, 1) if ach else None, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(ach / HBM_PEAK_GBS, 4) if ach else None, "traffic": traffic,
+                "traffic_unit": "fabric bytes per launch (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE; copy-calibrated)",
+                "kernel": desc, "avg_launch_ms": round(avg_ms, 4), "launches_per_step": nlaunch // args.steps,
+                "summed_ms_per_step": round(ms / args.steps, 2),
+                "algorithmic": {"bytes_per_ray": round(bytes_per_ray, 1), "bytes_per_launch": int(alg),
+                                "GBps": round(alg / (avg_ms / 1e3) / 1e9, 1) if avg_ms > 0 else None}}
+    rl_ext = kernel_roofline("kExtend", "kExtend (closest-hit traversal, extension rays)", ms_ext, launches, closest,
+                             per_ray)
+    rl_sh = kernel_roofline("kShadow", "kShadow (any-hit traversal, shadow rays)", ms_sh, launches_sh, anyr, per_any)
+    # the dominant kernel by summed launch time (the two overlap on two streams)
+    roofline = rl_sh if ms_sh >= ms_ext else rl_ext
     cpu = None
     if not args.no_cpu:
         try:
@@ -472,6 +528,7 @@ def main():
         "rays_per_step": rays // args.steps,
         "photons_stored": stored if photons else None,
         "roofline": roofline,
+        "roofline_kernels": [rl_sh, rl_ext],
         "cpu_baseline": cpu,
     }
     if cpu:

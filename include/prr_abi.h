@@ -201,6 +201,8 @@ typedef struct prr_stats {
     /* samples whose radiance was not finite and was replaced by 0
        (Integrator.cpp:259-261, Renderer.cpp:80-83) */
     uint64_t nan_samples;
+    uint64_t rays_any_finish;     /* shadow rays traced by path completion (part of rays_any) */
+    double ms_sort;               /* shadow-ray binning before traversal (with PRR_RENDER_KERNEL_TIMING) */
 } prr_stats;
 
 typedef struct prr_scene_info {

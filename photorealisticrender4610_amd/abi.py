@@ -87,7 +87,7 @@ class Stats(C.Structure):
                 ("ms_extend", C.c_double), ("ms_shadow", C.c_double), ("ms_shade", C.c_double),
                 ("launches_extend", C.c_int64), ("launches_shadow", C.c_int64),
                 ("rays_closest_finish", C.c_uint64), ("ms_finish", C.c_double), ("launches_finish", C.c_int64),
-                ("nan_samples", C.c_uint64)]
+                ("nan_samples", C.c_uint64), ("rays_any_finish", C.c_uint64), ("ms_sort", C.c_double)]
 
 
 class SceneInfo(C.Structure):
@@ -590,7 +590,8 @@ class Context:
                "photons_stored": st.photons_stored, "ms": st.ms_total, "ms_extend": st.ms_extend,
                "ms_shadow": st.ms_shadow, "ms_shade": st.ms_shade, "launches_extend": st.launches_extend,
                "launches_shadow": st.launches_shadow, "rays_closest_finish": st.rays_closest_finish,
-               "ms_finish": st.ms_finish, "launches_finish": st.launches_finish, "nan_samples": st.nan_samples}
+               "ms_finish": st.ms_finish, "launches_finish": st.launches_finish, "nan_samples": st.nan_samples,
+               "rays_any_finish": st.rays_any_finish, "ms_sort": st.ms_sort}
         if cnt is not None:
             res["spp"] = cnt
         return out, res

@@ -122,8 +122,28 @@ int ensureRenderBufs(prr_context *c, int64_t slots, int maxDepth) {
     A(&p.qcnt, (size_t) maxDepth + 2);
     A(&p.sqcnt, (size_t) maxDepth + 2);
     A(&p.fetch, 2 * ((size_t) maxDepth + 2));
+    c->bins = ShadeBins{};
+    if (c->shadeSort) {
+        c->bins.cap = slots;
+        c->bins.blocks = 4 * c->numCU;
+        A(&c->bins.lists, 2 * slots);
+        A(&c->bins.counts, 2);
+        A(&c->bins.blockCounts, 2 * (size_t) c->bins.blocks);
+    }
+    c->sb = ShadowSortBufs{};
+    if (c->shadowSortFrom > 0) {
+        // as many entries as the shadow queue (up to three NEE rays per path)
+        ShadowSortBufs &b = c->sb;
+        b.cap = 3 * slots;
+        b.blocks = 4 * c->numCU;
+        A(&b.slot, 3 * slots);
+        A(&b.org, 3 * slots);
+        A(&b.dir, 3 * slots);
+        A(&b.hist, (size_t) shadowBinCount(c->shadowSortBits) * b.blocks);
+    }
     if (e != hipSuccess) {
         freeRenderBufs(c);
+        c->sb = ShadowSortBufs{};
         return fail(c, PRR_ERR_NOMEM, std::string("render buffers: ") + hipGetErrorString(e));
     }
     p.stats = c->stats;
@@ -166,7 +186,11 @@ int runBounces(prr_context *c, const SceneView &view, const ChunkInfo &ci, Timin
     // ended, while bounce b - 1's kernels still occupy the GPU) and stops
     // once it shows the switch, so no launch of a later bounce is wasted and
     // the GPU never waits for the host.
-    const bool conc = c->concurrentShadow && c->aux;
+    // Shadow rays on the second stream beside the extension rays hide each
+    // launch's tail behind the other's bulk, which pays for small chunks;
+    // large chunks keep the GPU full with one kernel at a time (the same frame
+    // time, measured) and each kernel's time is its own.
+    const bool conc = c->concurrentShadow && c->aux && (int64_t) ci.pixels * ci.spc <= c->concurrentMaxPaths;
     const int wwShadow = c->shadowWide ? c->whileWhileAny : c->whileWhile;
     const bool tail = c->tailPaths > 0 && c->cntHost && ci.maxDepth + 1 < prr_context::kMaxBounces;
     ChunkInfo cit = ci;
@@ -180,11 +204,21 @@ int runBounces(prr_context *c, const SceneView &view, const ChunkInfo &ci, Timin
             HIPCK(c, hipMemcpyAsync(c->cntHost + b, c->pb.qcnt + b, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
             HIPCK(c, hipEventRecord(c->evCnt[b], st));
         }
+        // shadow rays of bounce b - 1, binned first (from bounce shadowSortFrom
+        // on) on the render stream, ahead of the extension rays: a short pass
+        // that would otherwise wait for the persistent traversal grid's CUs
+        PathBufs pbS = c->pb;
+        if (b > 0 && c->shadowSortFrom > 0 && b - 1 >= c->shadowSortFrom && c->sb.cap > 0) {
+            pbS.sqSlot = c->sb.slot;
+            pbS.sqOrg = c->sb.org;
+            pbS.sqDir = c->sb.dir;
+            TIMED(4, st, launchShadowSort(view, c->pb, b - 1, c->shadowSortBits, c->sb, st));
+        }
         if (b > 0 && conc) {
             HIPCK(c, hipEventRecord(c->evFork, st));
             HIPCK(c, hipStreamWaitEvent(c->aux, c->evFork, 0));
             TIMED(1, c->aux,
-                  launchShadow(view, c->pb, b - 1, c->spillAux, c->traceBlocks, c->refillAt, c->stackDepth, wwShadow,
+                  launchShadow(view, pbS, b - 1, c->spillAux, c->traceBlocks, c->refillAt, c->stackDepth, wwShadow,
                                c->fetchRun, c->shadowWide, c->aux));
             HIPCK(c, hipEventRecord(c->evJoin, c->aux));
         }
@@ -193,14 +227,18 @@ int runBounces(prr_context *c, const SceneView &view, const ChunkInfo &ci, Timin
                            c->fetchRun, cit.tailPaths, st));
         if (b > 0 && !conc)
             TIMED(1, st,
-                  launchShadow(view, c->pb, b - 1, c->spill, c->traceBlocks, c->refillAt, c->stackDepth, wwShadow,
+                  launchShadow(view, pbS, b - 1, c->spill, c->traceBlocks, c->refillAt, c->stackDepth, wwShadow,
                                c->fetchRun, c->shadowWide, st));
         if (b > 0 && conc) HIPCK(c, hipStreamWaitEvent(st, c->evJoin, 0));
         if (tail)
             TIMED(3, st,
                   launchFinish(view, c->pb, cit, b, c->spill, c->traceBlocks, c->stackDepth, c->whileWhile,
                                c->shadowWide, c->numCU, st));
-        TIMED(2, st, launchShade(view, c->pb, cit, b, c->shadeBlocks, c->shadeWaves, st));
+        if (c->shadeSort)
+            TIMED(2, st, launchShadeSorted(view, c->pb, cit, b, c->shadeBlocks, c->shadeWaves, c->shadeLightWaves,
+                                           c->bins, st));
+        else
+            TIMED(2, st, launchShade(view, c->pb, cit, b, c->shadeBlocks, c->shadeWaves, st));
     }
 #undef TIMED
     return PRR_OK;
@@ -226,7 +264,8 @@ int finishStats(prr_context *c, const Timing &tm, uint64_t photons, prr_stats *s
     stats->ms_total = ms;
     stats->rays_closest_finish = h[4];
     stats->nan_samples = h[5];
-    stats->ms_extend = stats->ms_shadow = stats->ms_shade = stats->ms_finish = 0.0;
+    stats->rays_any_finish = h[6];
+    stats->ms_extend = stats->ms_shadow = stats->ms_shade = stats->ms_finish = stats->ms_sort = 0.0;
     stats->launches_extend = stats->launches_shadow = stats->launches_finish = 0;
     for (const auto &t : tm.timed) {
         float dt = 0.f;
@@ -239,6 +278,8 @@ int finishStats(prr_context *c, const Timing &tm, uint64_t photons, prr_stats *s
             stats->launches_shadow++;
         } else if (t.first == 2) {
             stats->ms_shade += dt;
+        } else if (t.first == 4) {
+            stats->ms_sort += dt;
         } else {
             stats->ms_finish += dt;
             stats->launches_finish++;
@@ -347,9 +388,17 @@ int prr_create(int device, prr_context **out) {
     if (const char *ev = getenv("PRR_WHILE_WHILE_ANY")) c->whileWhileAny = std::max(0, atoi(ev));
     if (const char *ev = getenv("PRR_SHADOW_WIDE")) c->shadowWide = std::max(0, std::min(2, atoi(ev)));
     if (const char *ev = getenv("PRR_CONCURRENT_SHADOW")) c->concurrentShadow = atoi(ev) != 0;
+    if (const char *ev = getenv("PRR_CONCURRENT_MAX_PATHS")) c->concurrentMaxPaths = std::max<int64_t>(0, atoll(ev));
     if (const char *ev = getenv("PRR_FETCH_RUN")) c->fetchRun = std::max(64, std::min(1 << 16, atoi(ev)));
     if (const char *ev = getenv("PRR_SHADE_WAVES")) c->shadeWaves = std::max(1, std::min(5, atoi(ev)));
+    if (const char *ev = getenv("PRR_SHADE_SORT")) c->shadeSort = atoi(ev) != 0;
+    if (const char *ev = getenv("PRR_SHADE_LIGHT_WAVES")) {
+        const int v = atoi(ev);
+        c->shadeLightWaves = v >= 8 ? 8 : (v >= 6 ? 6 : 4);
+    }
     if (const char *ev = getenv("PRR_TAIL_PATHS")) c->tailPaths = std::max<int64_t>(0, atoll(ev));
+    if (const char *ev = getenv("PRR_SHADOW_SORT")) c->shadowSortFrom = std::max(0, std::min(64, atoi(ev)));
+    if (const char *ev = getenv("PRR_SHADOW_SORT_BITS")) c->shadowSortBits = std::max(1, std::min(3, atoi(ev)));
     if (const char *ev = getenv("PRR_STACK_LDS")) {  // LDS stack entries per lane: 2, 4, 8, 16 or 32
         const int v = atoi(ev);
         if (v == 2 || v == 4 || v == 8 || v == 16 || v == 32) c->stackDepth = v;

@@ -27,6 +27,12 @@ struct prr_context {
     int fetchRun = 256;     // traversal work fetch: a wave claims at most this many queue entries per atomic
     int shadowWide = 2;     // shadow rays: 0 the binary tree, 1 the four-wide occlusion tree, 2 its quantised copy
     int shadeWaves = 4;     // shading kernel compiled for at least this many waves per SIMD (1, 4, 5)
+    // shading sorted by material: each bounce's queue split into diffuse-hit
+    // entries and the rest, shaded by two specialised kernels (the rest at
+    // shadeLightWaves waves per SIMD: 4, 6 or 8)
+    int shadeSort = 0;
+    int shadeLightWaves = 8;
+    prr::ShadeBins bins{};
     int64_t maxSlots = 1 << 24;  // paths per chunk (set at creation from free HBM)
     uint2 *spill = nullptr;
     int spillLanes = 0;
@@ -37,6 +43,7 @@ struct prr_context {
     // rays of bounce b are traced on the render stream (each launch's tail
     // overlaps the other's bulk); the aux launches use their own spill region
     int concurrentShadow = 1;
+    int64_t concurrentMaxPaths = (int64_t) 1 << 24;  // only chunks of at most this many paths
     hipStream_t aux = nullptr;
     hipEvent_t evFork = nullptr, evJoin = nullptr;
     uint2 *spillAux = nullptr;
@@ -50,6 +57,11 @@ struct prr_context {
     int64_t tailPaths = 1 << 18;
     uint32_t *cntHost = nullptr;
     hipEvent_t evCnt[kMaxBounces] = {};
+    // shadow-queue binning before the any-hit traversal of bounces >=
+    // shadowSortFrom (0: off) on a grid of 2^shadowSortBits cells per axis
+    int shadowSortFrom = 0;
+    int shadowSortBits = 2;
+    prr::ShadowSortBufs sb{};
     // render buffers
     prr::PathBufs pb{};
     int64_t pbSlots = 0;
@@ -97,7 +109,8 @@ int fail(prr_context *c, int code, const std::string &msg);
 namespace prr_host {
 
 // Paths in flight per chunk, at most. Path state is kPathBytes per slot, so
-// 2^27 slots hold ~47 GB of HBM: a whole 1080p frame at 64 spp in one chunk.
+// 2^27 slots hold ~47 GB of HBM: a whole 1080p frame at 64 spp in one chunk
+// (with shadow binning on, 108 B more per slot); the shading bins take 8 B.
 // Large chunks pay the per-bounce tail of the traversal kernels (the last few
 // long rays of each bounce) once per frame instead of once per chunk. The
 // context's own limit (prr_context::maxSlots) also keeps path state within
@@ -107,7 +120,8 @@ constexpr int64_t kPathBytes = 360;
 
 struct Timing {
     bool on = false;
-    std::vector<std::pair<int, int>> timed;  // (kind 0 extend / 1 shadow / 2 shade / 3 finish, first event)
+    std::vector<std::pair<int, int>> timed;  // (kind 0 extend / 1 shadow / 2 shade / 3 finish / 4 shadow sort,
+                                             // first event)
     size_t next = 0;
 };
 

@@ -367,7 +367,13 @@ struct HeldRay {
 // PHOTON / CACHE: whether the caustic photon gather and the radiance-cache
 // query / training code are compiled in (the kernel's registers are those of
 // its heaviest branch, so renders without them use a leaner variant).
-template <bool PHOTON, bool CACHE, class Ray>
+// KIND: kShadeAny any entry; kShadeLight only entries whose hit is not a
+// diffuse surface (miss, terminating entry, emitter, glass, mirror: the
+// diffuse vertex code is not compiled in); kShadeDiffuse only entries whose
+// hit is a diffuse surface.
+constexpr int kShadeAny = 0, kShadeLight = 1, kShadeDiffuse = 2;
+
+template <bool PHOTON, bool CACHE, int KIND, class Ray>
 __device__ __forceinline__ void shadePath(const SceneView &s, const PathBufs &pb, const ChunkInfo &ci, int bounce,
                                           uint32_t slot, const Ray &ray, ShadeOut &so) {
     bool done = false;
@@ -487,7 +493,7 @@ __device__ __forceinline__ void shadePath(const SceneView &s, const PathBufs &pb
                     so.newO = g.P + off * kEps;
                     so.newD = bs.wi;
                 }
-            } else {
+            } else if constexpr (KIND != kShadeLight) {
                 fl &= ~F_SPEC;
                 sd.N = dot(d, g.Ng) < 0.f ? g.Ng : -g.Ng;
                 if (m.params.w != 0.f) texturedShading(s, g.mat, prim, g.flags, h.y, h.z, sd);
@@ -631,12 +637,15 @@ __device__ __forceinline__ void shadePath(const SceneView &s, const PathBufs &pb
     so.termEntry = (fl & F_TERMINATING) != 0u;
 }
 
-template <bool PHOTON, bool CACHE>
-__device__ __forceinline__ void shadeImpl(const SceneView &s, const PathBufs &pb, const ChunkInfo &ci, int bounce) {
+// Shades the entries of bounce's queue (list == null), or the count[0]
+// entries list[0..) of it (a shading bin).
+template <bool PHOTON, bool CACHE, int KIND>
+__device__ __forceinline__ void shadeImpl(const SceneView &s, const PathBufs &pb, const ChunkInfo &ci, int bounce,
+                                          const uint32_t *list, const uint32_t *count) {
     const int q = bounce & 1;
-    const int64_t cnt = pb.qcnt[bounce];
+    const int64_t cnt = list ? (int64_t) *count : (int64_t) pb.qcnt[bounce];
     // at most tailPaths entries: kFinish runs this bounce's paths (an empty loop here)
-    const int64_t n = cnt > ci.tailPaths ? cnt : 0;
+    const int64_t n = pb.qcnt[bounce] > ci.tailPaths ? cnt : 0;
     const int64_t stride = (int64_t) gridDim.x * kShadeBlock;
     const int64_t i0 = (int64_t) blockIdx.x * kShadeBlock + threadIdx.x;
     // uniform trip count per wave so every lane reaches the appends
@@ -647,8 +656,9 @@ __device__ __forceinline__ void shadeImpl(const SceneView &s, const PathBufs &pb
         uint32_t slot = 0;
         ShadeOut so;
         if (i < n) {
-            slot = pb.qSlot[q][i];
-            shadePath<PHOTON, CACHE>(s, pb, ci, bounce, slot, QueueRay{pb, q, i}, so);
+            const int64_t e = list ? (int64_t) list[i] : i;
+            slot = pb.qSlot[q][e];
+            shadePath<PHOTON, CACHE, KIND>(s, pb, ci, bounce, slot, QueueRay{pb, q, e}, so);
         }
         const bool emitRay = so.emitRay, termEntry = so.termEntry, emitA = so.emitA, emitE = so.emitE,
                    emitP = so.emitP;
@@ -708,7 +718,151 @@ __device__ __forceinline__ void shadeImpl(const SceneView &s, const PathBufs &pb
 // registers, spilled if need be, for more latency hiding).
 template <int W, bool PHOTON, bool CACHE>
 __global__ __launch_bounds__(kShadeBlock, W) void kShade(SceneView s, PathBufs pb, ChunkInfo ci, int bounce) {
-    shadeImpl<PHOTON, CACHE>(s, pb, ci, bounce);
+    shadeImpl<PHOTON, CACHE, kShadeAny>(s, pb, ci, bounce, nullptr, nullptr);
+}
+
+// Shading sorted by material (the hit's material class): the bounce's queue
+// is split into the entries whose hit is a diffuse surface and the rest, and
+// kShadeLightList / kShadeDiffuseList shade the two lists. The light kernel
+// holds none of the diffuse vertex's code (NEE, BSDF sampling, photon
+// gather, cache query) and runs at more waves per SIMD; the diffuse kernel's
+// waves no longer wait on lanes that escaped or hit glass. Every entry is
+// shaded by the same code as in kShade, so images do not change.
+//
+// The split is a counting sort without global atomics (appends through one
+// counter per list serialise on it when every wave of a light kernel hits
+// it): block b takes chunk b of the queue; kShadeBinCount counts each
+// block's entries per list, kShadeBinScan turns the counts into offsets,
+// kShadeBinScatter writes each block's entries in chunk order (wave
+// ballots + prefix sums within the block).
+PRR_DEV bool diffuseHit(const SceneView &s, const PathBufs &pb, int q, int64_t i) {
+    const int prim = __float_as_int(pb.hit[i].w);
+    if (prim < 0 || pb.qOrg[q][i].w < 0.f) return false;
+    const int mat = __float_as_int(s.prim_n[prim].w) & kPrimMatMask;
+    return (int) s.mats[mat].base_type.w == MAT_DIFFUSE;
+}
+
+PRR_DEV void shadeChunk(int64_t n, int b, int nb, int64_t &c0, int64_t &c1) {
+    const int64_t per = ((n + nb - 1) / nb + 63) & ~(int64_t) 63;  // whole waves
+    c0 = (int64_t) b * per;
+    c0 = c0 < n ? c0 : n;
+    c1 = c0 + per < n ? c0 + per : n;
+}
+
+// per block: [0] entries of the light list, [1] of the diffuse list
+__global__ __launch_bounds__(kShadeBlock) void kShadeBinCount(SceneView s, PathBufs pb, ChunkInfo ci, int bounce,
+                                                              uint32_t *blockCounts) {
+    __shared__ uint32_t part[2][kShadeBlock / 64];
+    const int q = bounce & 1;
+    const int64_t cnt = pb.qcnt[bounce];
+    const int64_t n = cnt > ci.tailPaths ? cnt : 0;
+    int64_t c0, c1;
+    shadeChunk(n, blockIdx.x, gridDim.x, c0, c1);
+    uint32_t nl = 0, nd = 0;
+    for (int64_t i = c0 + threadIdx.x; i < c1; i += kShadeBlock) {
+        const bool d = diffuseHit(s, pb, q, i);
+        nd += d ? 1u : 0u;
+        nl += d ? 0u : 1u;
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        nl += __shfl_xor(nl, o);
+        nd += __shfl_xor(nd, o);
+    }
+    if (laneId() == 0) {
+        part[0][threadIdx.x >> 6] = nl;
+        part[1][threadIdx.x >> 6] = nd;
+    }
+    __syncthreads();
+    if (threadIdx.x < 2) {
+        uint32_t t = 0;
+        for (int w = 0; w < kShadeBlock / 64; w++) t += part[threadIdx.x][w];
+        blockCounts[(size_t) threadIdx.x * gridDim.x + blockIdx.x] = t;
+    }
+}
+
+// Exclusive scan of the 2 * nb block counts (light blocks, then diffuse
+// blocks) into offsets within each list; counts[0..1] = list sizes.
+__global__ __launch_bounds__(1024) void kShadeBinScan(uint32_t *blockCounts, int nb, uint32_t *counts) {
+    __shared__ uint32_t part[1024];
+    for (int list = 0; list < 2; list++) {
+        uint32_t *c = blockCounts + (size_t) list * nb;
+        const int t = threadIdx.x;
+        const int per = (nb + 1023) / 1024;
+        const int a = t * per, e = a + per < nb ? a + per : nb;
+        uint32_t sum = 0;
+        for (int i = a; i < e; i++) sum += c[i];
+        part[t] = sum;
+        __syncthreads();
+        for (int off = 1; off < 1024; off <<= 1) {
+            const uint32_t v = t >= off ? part[t - off] : 0u;
+            __syncthreads();
+            part[t] += v;
+            __syncthreads();
+        }
+        uint32_t run = part[t] - sum;
+        for (int i = a; i < e; i++) {
+            const uint32_t x = c[i];
+            c[i] = run;
+            run += x;
+        }
+        if (t == 1023) counts[list] = part[1023];
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(kShadeBlock) void kShadeBinScatter(SceneView s, PathBufs pb, ChunkInfo ci, int bounce,
+                                                                const uint32_t *blockCounts, uint32_t *lists,
+                                                                int64_t cap) {
+    __shared__ uint32_t wcnt[2][kShadeBlock / 64];
+    const int q = bounce & 1;
+    const int64_t cnt = pb.qcnt[bounce];
+    const int64_t n = cnt > ci.tailPaths ? cnt : 0;
+    int64_t c0, c1;
+    shadeChunk(n, blockIdx.x, gridDim.x, c0, c1);
+    uint32_t base[2] = {blockCounts[blockIdx.x], blockCounts[(size_t) gridDim.x + blockIdx.x]};
+    const int w = threadIdx.x >> 6;
+    const uint64_t below = (1ull << laneId()) - 1ull;
+    // block-uniform trip count (the barrier below)
+    for (int64_t bb = c0; bb < c1; bb += kShadeBlock) {
+        const int64_t i = bb + threadIdx.x;
+        const bool valid = i < c1;
+        const bool d = valid && diffuseHit(s, pb, q, i);
+        const uint64_t md = __ballot(d), ml = __ballot(valid && !d);
+        if (laneId() == 0) {
+            wcnt[0][w] = (uint32_t) __popcll(ml);
+            wcnt[1][w] = (uint32_t) __popcll(md);
+        }
+        __syncthreads();
+        uint32_t offl = base[0], offd = base[1], totl = 0, totd = 0;
+        for (int k = 0; k < kShadeBlock / 64; k++) {
+            const uint32_t a = wcnt[0][k], b = wcnt[1][k];
+            offl += k < w ? a : 0u;
+            offd += k < w ? b : 0u;
+            totl += a;
+            totd += b;
+        }
+        __syncthreads();
+        if (valid) {
+            if (d)
+                lists[cap + offd + (uint32_t) __popcll(md & below)] = (uint32_t) i;
+            else
+                lists[offl + (uint32_t) __popcll(ml & below)] = (uint32_t) i;
+        }
+        base[0] += totl;
+        base[1] += totd;
+    }
+}
+
+template <int W, bool PHOTON, bool CACHE>
+__global__ __launch_bounds__(kShadeBlock, W) void kShadeLightList(SceneView s, PathBufs pb, ChunkInfo ci, int bounce,
+                                                                  const uint32_t *lists, const uint32_t *counts) {
+    shadeImpl<PHOTON, CACHE, kShadeLight>(s, pb, ci, bounce, lists, counts);
+}
+template <int W, bool PHOTON, bool CACHE>
+__global__ __launch_bounds__(kShadeBlock, W) void kShadeDiffuseList(SceneView s, PathBufs pb, ChunkInfo ci, int bounce,
+                                                                    const uint32_t *lists, int64_t cap,
+                                                                    const uint32_t *counts) {
+    shadeImpl<PHOTON, CACHE, kShadeDiffuse>(s, pb, ci, bounce, lists + cap, counts + 1);
 }
 
 // ---------------------------------------------------------------------------
@@ -784,7 +938,7 @@ __global__ __launch_bounds__(kTraceBlock, 3) void kFinish(SceneView s, PathBufs 
                 nTraced++;
             }
             ShadeOut so;
-            shadePath<PHOTON, CACHE>(s, pb, ci, b, slot, HeldRay{h, o4, d4}, so);
+            shadePath<PHOTON, CACHE, kShadeAny>(s, pb, ci, b, slot, HeldRay{h, o4, d4}, so);
             if (so.emitA) pb.vis[slot] = occluded(s, so.sOA, so.sDA, so.tA, tree, wwCap, lst, rst) ? 0 : 1;
             if (so.emitE)
                 pb.vis[pb.slots + slot] =
@@ -805,7 +959,10 @@ __global__ __launch_bounds__(kTraceBlock, 3) void kFinish(SceneView s, PathBufs 
     }
     if (laneId() == 0) {
         if (nRays) atomicAdd(&pb.stats[0], (unsigned long long) nRays);
-        if (nShadow) atomicAdd(&pb.stats[1], (unsigned long long) nShadow);
+        if (nShadow) {
+            atomicAdd(&pb.stats[1], (unsigned long long) nShadow);
+            atomicAdd(&pb.stats[6], (unsigned long long) nShadow);
+        }
         if (nTraced) atomicAdd(&pb.stats[4], (unsigned long long) nTraced);
     }
 }
@@ -978,6 +1135,42 @@ static void shadeLaunch(const SceneView &s, const PathBufs &pb, const ChunkInfo 
         hipLaunchKernelGGL((kShade<4, P, C>), dim3(grid), dim3(kShadeBlock), 0, st, s, pb, ci, bounce);
     else
         hipLaunchKernelGGL((kShade<1, P, C>), dim3(grid), dim3(kShadeBlock), 0, st, s, pb, ci, bounce);
+}
+
+template <bool P, bool C>
+static void shadeSortedLaunch(const SceneView &s, const PathBufs &pb, const ChunkInfo &ci, int bounce, int grid,
+                              int waves, int lightWaves, const ShadeBins &sb, hipStream_t st) {
+    hipLaunchKernelGGL(kShadeBinCount, dim3(sb.blocks), dim3(kShadeBlock), 0, st, s, pb, ci, bounce, sb.blockCounts);
+    hipLaunchKernelGGL(kShadeBinScan, dim3(1), dim3(1024), 0, st, sb.blockCounts, sb.blocks, sb.counts);
+    hipLaunchKernelGGL(kShadeBinScatter, dim3(sb.blocks), dim3(kShadeBlock), 0, st, s, pb, ci, bounce,
+                       sb.blockCounts, sb.lists, sb.cap);
+    if (lightWaves >= 8)
+        hipLaunchKernelGGL((kShadeLightList<8, P, C>), dim3(grid), dim3(kShadeBlock), 0, st, s, pb, ci, bounce,
+                           sb.lists, sb.counts);
+    else if (lightWaves >= 6)
+        hipLaunchKernelGGL((kShadeLightList<6, P, C>), dim3(grid), dim3(kShadeBlock), 0, st, s, pb, ci, bounce,
+                           sb.lists, sb.counts);
+    else
+        hipLaunchKernelGGL((kShadeLightList<4, P, C>), dim3(grid), dim3(kShadeBlock), 0, st, s, pb, ci, bounce,
+                           sb.lists, sb.counts);
+    if (waves >= 5)
+        hipLaunchKernelGGL((kShadeDiffuseList<5, P, C>), dim3(grid), dim3(kShadeBlock), 0, st, s, pb, ci, bounce,
+                           sb.lists, sb.cap, sb.counts);
+    else
+        hipLaunchKernelGGL((kShadeDiffuseList<4, P, C>), dim3(grid), dim3(kShadeBlock), 0, st, s, pb, ci, bounce,
+                           sb.lists, sb.cap, sb.counts);
+}
+
+hipError_t launchShadeSorted(const SceneView &s, const PathBufs &pb, const ChunkInfo &ci, int bounce, int grid,
+                             int waves, int lightWaves, const ShadeBins &sb, hipStream_t st) {
+    const bool photon = s.ph_count > 0, cache = ci.cache.query || ci.cache.train;
+    if (photon)
+        cache ? shadeSortedLaunch<true, true>(s, pb, ci, bounce, grid, waves, lightWaves, sb, st)
+              : shadeSortedLaunch<true, false>(s, pb, ci, bounce, grid, waves, lightWaves, sb, st);
+    else
+        cache ? shadeSortedLaunch<false, true>(s, pb, ci, bounce, grid, waves, lightWaves, sb, st)
+              : shadeSortedLaunch<false, false>(s, pb, ci, bounce, grid, waves, lightWaves, sb, st);
+    return hipGetLastError();
 }
 
 hipError_t launchShade(const SceneView &s, const PathBufs &pb, const ChunkInfo &ci, int bounce, int grid, int waves,

@@ -20,9 +20,23 @@ hipError_t launchExtend(const SceneView &s, const PathBufs &pb, int bounce, uint
 // tree: 0 the binary tree, 1 the four-wide occlusion tree, 2 its quantised copy
 hipError_t launchShadow(const SceneView &s, const PathBufs &pb, int bounce, uint2 *spill, int traceGrid,
                         int refillAt, int stackDepth, int whileWhile, int fetchRun, int tree, hipStream_t st);
+// Shadow-queue binning (k_sort.hip): the entries of bounce's shadow queue
+// grouped by (direction octant, Morton code of the origin's cell on a grid of
+// 2^bits cells per axis over the scene box) into sb.slot / org / dir, which
+// hold the whole queue (sb.cap entries); shadowBinCount(bits) bins, whose
+// histograms take bins * sb.blocks words of sb.hist.
+int shadowBinCount(int bits);
+hipError_t launchShadowSort(const SceneView &s, const PathBufs &pb, int bounce, int bits, const ShadowSortBufs &sb,
+                            hipStream_t st);
 // waves: minimum waves per SIMD the shading kernel is compiled for (1, 4 or 5)
 hipError_t launchShade(const SceneView &s, const PathBufs &pb, const ChunkInfo &ci, int bounce, int grid, int waves,
                        hipStream_t st);
+// Shading sorted by material: the bounce's queue split into the entries
+// whose hit is a diffuse surface and the rest (sb), each list shaded by its
+// own kernel (lightWaves: waves per SIMD the non-diffuse kernel is compiled
+// for, 4, 6 or 8). Same results as launchShade.
+hipError_t launchShadeSorted(const SceneView &s, const PathBufs &pb, const ChunkInfo &ci, int bounce, int grid,
+                             int waves, int lightWaves, const ShadeBins &sb, hipStream_t st);
 // path completion: when bounce0 is the first bounce whose queue holds at
 // most ci.tailPaths entries, every entry traced and shaded to the end of its
 // path in one launch (the shadow rays of bounce0 - 1 resolved); otherwise

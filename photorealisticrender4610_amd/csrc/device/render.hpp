@@ -39,7 +39,31 @@ struct PathBufs {
     uint32_t *sqcnt;         // shadow-queue count per bounce
     uint32_t *fetch;         // persistent-kernel work counters: [2b] extend, [2b+1] shadow
     unsigned long long *stats;  // [0] closest rays, [1] any rays, [2] samples, [3] photons stored,
-                                // [4] closest rays traced by kFinish
+                                // [4] closest rays traced by kFinish, [5] samples replaced by 0
+                                // (non-finite), [6] shadow rays traced by kFinish
+};
+
+// Buffers of the shadow-queue binning (k_sort.hip): the binned queue (cap
+// entries, as many as the shadow queue) and the per-block bin histograms /
+// offsets of its counting sort (bins x blocks).
+struct ShadowSortBufs {
+    int64_t cap = 0;
+    int blocks = 0;
+    uint32_t *slot = nullptr;
+    float4 *org = nullptr;
+    float4 *dir = nullptr;
+    uint32_t *hist = nullptr;
+};
+
+// Shading bins (kShadeBin): queue indices of the entries whose hit is not a
+// diffuse surface in lists[0, counts[0]), of those whose hit is, in
+// lists[cap, cap + counts[1]).
+struct ShadeBins {
+    int64_t cap = 0;
+    int blocks = 0;                   // blocks of the counting sort (chunks of the queue)
+    uint32_t *lists = nullptr;        // 2 * cap
+    uint32_t *counts = nullptr;       // 2
+    uint32_t *blockCounts = nullptr;  // 2 * blocks
 };
 
 constexpr int kCacheQueryFloats = 13;  // position, wo, normal, roughness, albedo
