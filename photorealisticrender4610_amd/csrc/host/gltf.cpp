@@ -10,7 +10,9 @@
 // JPEG images are rejected with an error (no decoder here).
 #include <zlib.h>
 
+#include <charconv>
 #include <cmath>
+#include <cstddef>
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
@@ -48,7 +50,20 @@ struct Json {
         const Json *v = get(k);
         return v && v->kind == Num ? v->num : d;
     }
-    int intOr(const char *k, int d) const { return (int) numOr(k, d); }
+    int intOr(const char *k, int d) const {
+        const double x = numOr(k, d);
+        // an out-of-range or non-integral index reads as "none" (-1)
+        return (std::isfinite(x) && x >= -2147483648.0 && x <= 2147483647.0 && x == std::floor(x)) ? (int) x : -1;
+    }
+    // A byte count or offset: absent = 0; a negative, non-integral or
+    // non-finite value, or one above `limit`, is an error (the value is
+    // converted to size_t only once it is known to fit).
+    size_t sizeOr0(const char *k, size_t limit) const {
+        const double x = numOr(k, 0.0);
+        if (!(std::isfinite(x) && x >= 0.0 && x == std::floor(x) && x <= (double) limit))
+            throw std::runtime_error(std::string("glTF: bad ") + k);
+        return (size_t) x;
+    }
     const Json &at(size_t i) const {
         if (kind != Arr || i >= arr.size()) throw std::runtime_error("glTF: index out of range");
         return arr[i];
@@ -200,12 +215,15 @@ struct JsonReader {
             p += 4;
             return v;
         }
-        // number
+        // number: JSON's grammar characters only (a NUL byte ends it), read by
+        // std::from_chars, which does not depend on the process's locale
         const char *s = p;
-        while (p < e && (strchr("+-0123456789.eE", *p) != nullptr)) p++;
+        while (p < e && *p != '\0' && strchr("+-0123456789.eE", *p) != nullptr) p++;
         if (p == s) bad("unexpected character");
         v.kind = Json::Num;
-        v.num = strtod(std::string(s, p).c_str(), nullptr);
+        const char *b = (*s == '+') ? s + 1 : s;  // from_chars takes no leading '+'
+        const auto r = std::from_chars(b, p, v.num);
+        if (r.ec != std::errc() || r.ptr != p) bad("bad number");
         return v;
     }
 };
@@ -491,13 +509,22 @@ View accessorView(const Model &m, int idx) {
     v.nc = numComponents(type && type->kind == Json::Str ? type->str : "");
     v.ct = acc.intOr("componentType", 0);
     const int cs = componentSize(v.ct);
-    v.count = (size_t) acc.numOr("count", 0);
-    const size_t bvStride = (size_t) bv.numOr("byteStride", 0);
-    v.stride = bvStride ? bvStride : (size_t) v.nc * cs;
-    const size_t start = (size_t) bv.numOr("byteOffset", 0) + (size_t) acc.numOr("byteOffset", 0);
-    const size_t bvEnd = (size_t) bv.numOr("byteOffset", 0) + (size_t) bv.numOr("byteLength", 0);
     const auto &buf = m.buffers[(size_t) bi];
-    const size_t need = v.count ? start + (v.count - 1) * v.stride + (size_t) v.nc * cs : start;
+    // every offset, length and count is checked against the buffer before
+    // any arithmetic, and the arithmetic itself for overflow
+    const size_t bvOff = bv.sizeOr0("byteOffset", buf.size());
+    const size_t bvLen = bv.sizeOr0("byteLength", buf.size() - bvOff);
+    const size_t accOff = acc.sizeOr0("byteOffset", bvLen);
+    v.count = acc.sizeOr0("count", buf.size());
+    const size_t bvStride = bv.sizeOr0("byteStride", 255);
+    v.stride = bvStride ? bvStride : (size_t) v.nc * cs;
+    const size_t start = bvOff + accOff, bvEnd = bvOff + bvLen;  // both <= buf.size()
+    size_t need = start, span = 0;
+    if (v.count) {
+        if (__builtin_mul_overflow(v.count - 1, v.stride, &span) || __builtin_add_overflow(span, start, &need) ||
+            __builtin_add_overflow(need, (size_t) v.nc * cs, &need))
+            throw std::runtime_error("glTF: accessor exceeds its buffer");
+    }
     if (need > buf.size() || need > bvEnd) throw std::runtime_error("glTF: accessor exceeds its buffer");
     v.base = buf.data() + start;
     return v;
@@ -611,7 +638,7 @@ Model openModel(const std::string &path) {
             } else {
                 throw std::runtime_error("glTF: buffer without data");
             }
-            const size_t len = (size_t) b.numOr("byteLength", 0);
+            const size_t len = b.sizeOr0("byteLength", std::numeric_limits<uint32_t>::max());
             if (data.size() < len) throw std::runtime_error("glTF: buffer shorter than its byteLength");
             m.buffers.push_back(std::move(data));
         }
@@ -641,9 +668,10 @@ int textureOf(const Model &m, const std::string &dir, Loaded &L, std::map<int, i
         const Json &bv = bvs->at((size_t) bvi);
         const int bi = bv.intOr("buffer", -1);
         if (bi < 0 || (size_t) bi >= m.buffers.size()) throw std::runtime_error("glTF: buffer out of range");
-        const size_t o = (size_t) bv.numOr("byteOffset", 0), n = (size_t) bv.numOr("byteLength", 0);
-        if (o + n > m.buffers[(size_t) bi].size()) throw std::runtime_error("glTF: image view exceeds its buffer");
-        bytes.assign(m.buffers[(size_t) bi].begin() + (long) o, m.buffers[(size_t) bi].begin() + (long) (o + n));
+        const auto &buf = m.buffers[(size_t) bi];
+        const size_t o = bv.sizeOr0("byteOffset", buf.size());
+        const size_t n = bv.sizeOr0("byteLength", buf.size() - o);  // o + n <= size, no overflow
+        bytes.assign(buf.begin() + (std::ptrdiff_t) o, buf.begin() + (std::ptrdiff_t) (o + n));
     }
     const Image img = decodePng(bytes.data(), bytes.size());
     std::vector<uint8_t> t(img.px.size());

@@ -117,3 +117,85 @@ def test_scene_builder_gltf(assets):
     assert sc.objects[objs2[0], 1] == m0
     d, keep = abi.scene_desc(sc)
     assert d.num_textures == 5 and bool(d.tri_flags)
+
+
+def _tiny_gltf(tmp_path, name, edit=None):
+    """A .gltf of one triangle (positions in a base64 data: buffer), with
+    edit(doc) applied to its JSON before writing."""
+    import base64
+    import json
+    pos = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0]], np.float32).tobytes()
+    doc = {
+        "asset": {"version": "2.0"},
+        "buffers": [{"byteLength": len(pos), "uri": "data:application/octet-stream;base64,"
+                     + base64.b64encode(pos).decode()}],
+        "bufferViews": [{"buffer": 0, "byteOffset": 0, "byteLength": len(pos)}],
+        "accessors": [{"bufferView": 0, "componentType": 5126, "count": 3, "type": "VEC3"}],
+        "meshes": [{"primitives": [{"attributes": {"POSITION": 0}}]}],
+    }
+    if edit:
+        edit(doc)
+    p = tmp_path / name
+    p.write_text(json.dumps(doc))
+    return str(p)
+
+
+def test_loader_rejects_malformed_offsets_and_counts(tmp_path):
+    """Offsets, lengths and counts that would reach outside a buffer (huge,
+    negative, non-integral, or wrapping size_t arithmetic) are errors, never
+    out-of-bounds reads; the unedited file loads."""
+    g = abi.load_gltf(_tiny_gltf(tmp_path, "ok.gltf"))
+    assert g["positions"].shape == (1, 9)
+    edits = {
+        "huge_view_offset": lambda d: d["bufferViews"][0].update(byteOffset=2.0 ** 64 - 16),
+        "huge_accessor_offset": lambda d: d["accessors"][0].update(byteOffset=18446744073709551600),
+        "negative_count": lambda d: d["accessors"][0].update(count=-1),
+        "fractional_count": lambda d: d["accessors"][0].update(count=2.5),
+        "huge_count": lambda d: d["accessors"][0].update(count=2.0 ** 62),
+        "huge_stride": lambda d: d["bufferViews"][0].update(byteStride=2 ** 40),
+        "view_past_buffer": lambda d: d["bufferViews"][0].update(byteLength=10 ** 6),
+        "negative_length": lambda d: d["bufferViews"][0].update(byteLength=-36),
+    }
+    for name, edit in edits.items():
+        with pytest.raises(abi.PrrError, match="glTF"):
+            abi.load_gltf(_tiny_gltf(tmp_path, name + ".gltf", edit))
+
+
+def test_loader_rejects_image_view_offset_near_2_64(tmp_path):
+    """An image whose bufferView offset is near 2^64: o + n would wrap below
+    the buffer size; it is refused before any copy."""
+    def edit(d):
+        d["bufferViews"].append({"buffer": 0, "byteOffset": 18446744073709551615 - 8, "byteLength": 16})
+        d["images"] = [{"bufferView": 1, "mimeType": "image/png"}]
+        d["textures"] = [{"source": 0}]
+        d["materials"] = [{"pbrMetallicRoughness": {"baseColorTexture": {"index": 0}}}]
+        d["meshes"][0]["primitives"][0]["material"] = 0
+    with pytest.raises(abi.PrrError, match="glTF"):
+        abi.load_gltf(_tiny_gltf(tmp_path, "img.gltf", edit))
+
+
+def test_json_numbers_ignore_the_locale(tmp_path):
+    """Numbers are read by std::from_chars: a comma-decimal LC_NUMERIC in the
+    host process (setlocale) does not change them (skipped when no such
+    locale is installed)."""
+    import ctypes
+    import ctypes.util
+    libc = ctypes.CDLL(ctypes.util.find_library("c"))
+    libc.setlocale.restype = ctypes.c_char_p
+    LC_NUMERIC = 1
+    old = libc.setlocale(LC_NUMERIC, None)
+    got = None
+    for name in (b"de_DE.UTF-8", b"de_DE.utf8", b"fr_FR.UTF-8", b"ru_RU.UTF-8"):
+        if libc.setlocale(LC_NUMERIC, name):
+            got = name
+            break
+    if not got:
+        pytest.skip("no comma-decimal locale installed")
+    try:
+        def edit(d):
+            d["materials"] = [{"pbrMetallicRoughness": {"roughnessFactor": 0.25, "metallicFactor": 0.5}}]
+            d["meshes"][0]["primitives"][0]["material"] = 0
+        g = abi.load_gltf(_tiny_gltf(tmp_path, "loc.gltf", edit))
+        assert g["materials"][0][4] == np.float32(0.25) and g["materials"][0][5] == np.float32(0.5)
+    finally:
+        libc.setlocale(LC_NUMERIC, old)
