@@ -3,10 +3,11 @@
 // A worker thread owns the context. Each loop: admit pending requests in
 // arrival order while their framebuffers fit the HBM budget; plan one batch
 // of at most path_slots paths with one segment per admitted request (the
-// share of each set by the server's policy); run it per scene (segment ray generation, the wavefront bounces,
-// per-request accumulation in sample order); resolve and copy out the
-// requests the batch completed. Requests submitted meanwhile wait only for
-// the current batch to end.
+// share of each set by the server's policy); run it as one part per scene
+// (segment ray generation, the wavefront bounces, per-request accumulation in
+// sample order), the part of the most urgent request first; after each part,
+// resolve, copy out and hand over the requests it completed. Requests
+// submitted meanwhile wait only for the current batch to end.
 #include <algorithm>
 #include <chrono>
 #include <condition_variable>
@@ -87,31 +88,69 @@ struct prr_server {
 };
 
 
-// Runs one planned batch: per scene, segment ray generation, bounces and
-// accumulation; then resolves the requests it completes. Called without the
-// lock (the plan's requests are owned by the worker until it re-locks).
-static int runBatch(prr_server *S, std::vector<Planned> &plan, int64_t &rays) {
+// Bookkeeping of plan entries whose scene part has run (worker thread, under
+// the lock): sample progress, and for the requests the batch completes their
+// completion time, release of their budget, removal from the active list and
+// the wake-up of their waiters. A completed request may be deleted by its
+// waiter as soon as the lock is released, so nothing touches it afterwards.
+static void completeEntries(prr_server *S, std::vector<Planned *> &done, double t) {
+    for (Planned *pl : done) {
+        Request *r = pl->r;
+        r->batches++;
+        S->stats.samples += (int64_t) pl->seg.np * pl->seg.ns;
+        if (pl->seg.ns == 1 && pl->seg.p0 + pl->seg.np < r->pixels) {
+            r->pCur = pl->seg.p0 + pl->seg.np;
+        } else {
+            r->pCur = 0;
+            r->sCur += pl->seg.ns;
+        }
+        if (pl->finishes) {
+            r->done = true;
+            r->tDone = t;
+            S->admittedBytes -= r->bytes;
+            S->stats.requests_done++;
+        }
+        pl->r = nullptr;
+    }
+    S->active.erase(std::remove_if(S->active.begin(), S->active.end(), [](Request *r) { return r->done; }),
+                    S->active.end());
+    S->cvDone.notify_all();
+}
+
+// Runs one planned batch: one part per scene (segment ray generation, the
+// wavefront bounces, per-request accumulation in sample order), the parts in
+// the order of their most urgent request (the plan's order). The requests a
+// part completes are resolved and handed to their waiters as soon as that
+// part has run, not at the end of the batch. Called without the lock; takes
+// it only to complete entries.
+static int runBatch(prr_server *S, std::vector<Planned> &plan, std::unique_lock<std::mutex> &lk, int64_t &rays) {
     prr_context *c = S->ctx;
     hipStream_t st = c->stream;
     const int nseg = (int) plan.size();
     if (nseg > S->segCap) return fail(c, PRR_ERR_INVALID, "too many segments");
     HIPCK(c, hipMemsetAsync(c->stats, 0, 8 * sizeof(unsigned long long), st));
+    std::vector<int> sceneOrder;
+    for (auto &pl : plan)
+        if (std::find(sceneOrder.begin(), sceneOrder.end(), pl.r->d.scene) == sceneOrder.end())
+            sceneOrder.push_back(pl.r->d.scene);
     std::vector<Segment> segs;
     std::vector<int64_t> pixStart;
+    std::vector<Planned *> part;
     int off = 0;
-    for (size_t sc = 0; sc < S->scenes.size(); sc++) {
+    for (const int sc : sceneOrder) {
         segs.clear();
         pixStart.clear();
+        part.clear();
         int64_t slots = 0, pix = 0;
         for (auto &pl : plan) {
-            if (pl.r->d.scene != (int) sc) continue;
+            if (!pl.r || pl.r->d.scene != sc) continue;  // null: completed by an earlier part
             pl.seg.slot0 = slots;
             segs.push_back(pl.seg);
             pixStart.push_back(pix);
+            part.push_back(&pl);
             slots += (int64_t) pl.seg.np * pl.seg.ns;
             pix += pl.seg.np;
         }
-        if (segs.empty()) continue;
         Segment *dS = S->dSegs + off;
         int64_t *dP = S->dPixStart + off;
         off += (int) segs.size();
@@ -134,25 +173,29 @@ static int runBatch(prr_server *S, std::vector<Planned> &plan, int64_t &rays) {
         Timing tm;
         if ((rc = runBounces(c, view, ci, tm, st)) != PRR_OK) return rc;
         HIPCK(c, launchAccumulateSeg(c->pb, dS, dP, (int) segs.size(), pix, st));
-    }
-    // completed requests: mean = sums / spp (Renderer.cpp:99), copied out
-    for (auto &pl : plan) {
-        if (!pl.finishes) continue;
-        Request *r = pl.r;
-        if (r->pixels * 3 > S->rgbCap) {
+        // completed requests of this part: mean = sums / spp (Renderer.cpp:99), copied out
+        for (Planned *pl : part) {
+            if (!pl->finishes) continue;
+            Request *r = pl->r;
+            if (r->pixels * 3 > S->rgbCap) {
+                HIPCK(c, hipStreamSynchronize(st));
+                if (S->dRgb) HIPCK(c, hipFree(S->dRgb));
+                S->dRgb = nullptr;
+                S->rgbCap = 0;
+                HIPCK(c, hipMalloc(&S->dRgb, (size_t) r->pixels * 3 * sizeof(float)));
+                S->rgbCap = r->pixels * 3;
+            }
+            HIPCK(c, launchResolve(r->fb, (int) r->pixels, r->d.spp, S->dRgb, st));
+            r->rgb.resize((size_t) r->pixels * 3);
+            HIPCK(c, hipMemcpyAsync(r->rgb.data(), S->dRgb, r->rgb.size() * sizeof(float), hipMemcpyDeviceToHost, st));
             HIPCK(c, hipStreamSynchronize(st));
-            if (S->dRgb) HIPCK(c, hipFree(S->dRgb));
-            S->dRgb = nullptr;
-            S->rgbCap = 0;
-            HIPCK(c, hipMalloc(&S->dRgb, (size_t) r->pixels * 3 * sizeof(float)));
-            S->rgbCap = r->pixels * 3;
+            HIPCK(c, hipFree(r->fb));
+            r->fb = nullptr;
         }
-        HIPCK(c, launchResolve(r->fb, (int) r->pixels, r->d.spp, S->dRgb, st));
-        r->rgb.resize((size_t) r->pixels * 3);
-        HIPCK(c, hipMemcpyAsync(r->rgb.data(), S->dRgb, r->rgb.size() * sizeof(float), hipMemcpyDeviceToHost, st));
-        HIPCK(c, hipStreamSynchronize(st));
-        HIPCK(c, hipFree(r->fb));
-        r->fb = nullptr;
+        const double t = S->now();
+        lk.lock();
+        completeEntries(S, part, t);
+        lk.unlock();
     }
     unsigned long long h[3] = {0, 0, 0};
     HIPCK(c, hipMemcpyAsync(h, c->stats, sizeof(h), hipMemcpyDeviceToHost, st));
@@ -273,14 +316,14 @@ static void workerLoop(prr_server *S) {
         lk.unlock();
         const double tb = S->now();
         int64_t rays = 0;
-        const int rc = runBatch(S, plan, rays);
+        const int rc = runBatch(S, plan, lk, rays);
         const double te = S->now();
         lk.lock();
         S->stats.batches++;
         S->stats.busy_ms += te - tb;
         S->stats.rays += rays;
         if (rc != PRR_OK) {
-            // the device failed: every admitted request fails with the message
+            // the device failed: every request still admitted fails with the message
             S->err = S->ctx->err;
             for (Request *r : S->active) {
                 r->failed = r->done = true;
@@ -294,25 +337,6 @@ static void workerLoop(prr_server *S) {
             S->cvDone.notify_all();
             continue;
         }
-        for (auto &pl : plan) {
-            Request *r = pl.r;
-            r->batches++;
-            S->stats.samples += (int64_t) pl.seg.np * pl.seg.ns;
-            if (pl.seg.ns == 1 && pl.seg.p0 + pl.seg.np < r->pixels) {
-                r->pCur = pl.seg.p0 + pl.seg.np;
-            } else {
-                r->pCur = 0;
-                r->sCur += pl.seg.ns;
-            }
-            if (pl.finishes) {
-                r->done = true;
-                r->tDone = te;
-                S->admittedBytes -= r->bytes;
-                S->stats.requests_done++;
-            }
-        }
-        S->active.erase(std::remove_if(S->active.begin(), S->active.end(), [](Request *r) { return r->done; }),
-                        S->active.end());
         S->cvDone.notify_all();
     }
 }
