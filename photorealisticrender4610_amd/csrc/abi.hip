@@ -369,7 +369,7 @@ int prr_create(int device, prr_context **out) {
     // LDS stacks, 16 KB per block), a wave refills once 24 of its lanes are
     // idle, at most 8 interior steps (4 for shadow rays) per work check
     // (measured on the 1M-tri scene). PRR_TRACE_BLOCKS_PER_CU, PRR_STACK_LDS,
-    // PRR_REFILL_AT, PRR_WHILE_WHILE(_ANY), PRR_FETCH_RUN, PRR_SHADOW_WIDE,
+    // PRR_REFILL_AT, PRR_WHILE_WHILE(_ANY), PRR_FETCH_RUN, PRR_SHADOW_WIDE, PRR_SHADOW_FAST,
     // PRR_CONCURRENT_SHADOW, PRR_SHADE_WAVES and PRR_TAIL_PATHS override the
     // measured defaults for tuning sweeps.
     int tbpc = 8;
@@ -387,6 +387,7 @@ int prr_create(int device, prr_context **out) {
     if (const char *ev = getenv("PRR_WHILE_WHILE")) c->whileWhile = std::max(0, atoi(ev));
     if (const char *ev = getenv("PRR_WHILE_WHILE_ANY")) c->whileWhileAny = std::max(0, atoi(ev));
     if (const char *ev = getenv("PRR_SHADOW_WIDE")) c->shadowWide = std::max(0, std::min(2, atoi(ev)));
+    if (const char *ev = getenv("PRR_SHADOW_FAST")) c->shadowFast = atoi(ev) != 0;
     if (const char *ev = getenv("PRR_CONCURRENT_SHADOW")) c->concurrentShadow = atoi(ev) != 0;
     if (const char *ev = getenv("PRR_CONCURRENT_MAX_PATHS")) c->concurrentMaxPaths = std::max<int64_t>(0, atoll(ev));
     if (const char *ev = getenv("PRR_FETCH_RUN")) c->fetchRun = std::max(64, std::min(1 << 16, atoi(ev)));
@@ -507,6 +508,15 @@ int prr_scene_create(prr_context *c, const prr_scene_desc *d, prr_scene **out) {
     v.num_emitters = h.totalEmitArea > 0.f ? (int) h.emitters.size() : 0;
     v.total_emit_area = h.totalEmitArea;
     for (int k = 0; k < 6; k++) v.root_box[k] = h.rootBox[k];
+    {
+        float R = 0.f;  // the extent the quantiser's margin was sized for
+        for (int a = 0; a < 3; a++) R = std::max(R, h.rootBox[2 * a + 1] - h.rootBox[2 * a]);
+        for (int a = 0; a < 3; a++) {
+            v.safe_box[2 * a] = h.rootBox[2 * a] - R;
+            v.safe_box[2 * a + 1] = h.rootBox[2 * a + 1] + R;
+        }
+        v.quant_fast = c->shadowFast ? 1 : 0;
+    }
     v.env_w = h.envW;
     v.env_h = h.envH;
     v.env_total = h.envTotal;

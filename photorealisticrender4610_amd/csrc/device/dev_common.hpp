@@ -223,6 +223,11 @@ struct SceneView {
     int num_emitters;
     float total_emit_area;
     float root_box[6];        // [minX,maxX,minY,maxY,minZ,maxZ]
+    // ray origins for which shadow rays may use the short form of the
+    // quantised node test (the scene box widened by its extent on every
+    // side); quant_fast 0 keeps every ray on the full form
+    float safe_box[6];
+    int quant_fast;
     int env_w, env_h;
     float env_total;
     float background[3];

@@ -307,6 +307,10 @@ struct TravState {
     float tmax;  // any-hit range
     uint32_t cur;
     HitOut h;
+    // quantised occlusion tree: this ray may take the short form of the node
+    // test (anyPrepQ), with 1/d clamped to +-2^100
+    bool fast;
+    float invc[3];
 };
 
 template <class Stack>
@@ -537,21 +541,11 @@ PRR_DEV int anyIter(const SceneView &s, TravState &ts, LaneStack &st, int cap) {
     return anyLeaf(s, ts, st);
 }
 
-// Any-hit at a node of the occlusion tree (DNode4): the nearest child whose
-// box is entered before tmax next, the other entered children pushed (in
-// child order), or a pop when none is entered. Empty slots hold an inverted
-// box, which the slab test never enters. The answer equals
-// BVHAccel::IntersectP's on the binary tree: the occlusion tree keeps its
-// leaves and skips only boxes that contain their children's boxes.
-PRR_DEV int anyInterior4(const SceneView &s, TravState &ts, RefStack &st) {
-    const DNode4 *n = s.nodes4 + ts.cur;
-    const float4 lx = n->lox, hx = n->hix, ly = n->loy, hy = n->hiy, lz = n->loz, hz = n->hiz;
-    const uint4 rf = n->ref;
+// Children of a four-wide node entered with box entry t0..t3 (infinity: not
+// entered): the nearest next, the other entered ones pushed in child order,
+// or a pop when none is entered before tmax.
+PRR_DEV int anyChoose4(TravState &ts, RefStack &st, const uint4 rf, float t0, float t1, float t2, float t3) {
     const float tm = ts.tmax;
-    const float t0 = slab(ts.r, lx.x, hx.x, ly.x, hy.x, lz.x, hz.x);
-    const float t1 = slab(ts.r, lx.y, hx.y, ly.y, hy.y, lz.y, hz.y);
-    const float t2 = slab(ts.r, lx.z, hx.z, ly.z, hy.z, lz.z, hz.z);
-    const float t3 = slab(ts.r, lx.w, hx.w, ly.w, hy.w, lz.w, hz.w);
     // nearest entered child (slot index, so equal refs never arise)
     int bk = -1;
     float bt = tm;
@@ -572,6 +566,22 @@ PRR_DEV int anyInterior4(const SceneView &s, TravState &ts, RefStack &st) {
     st.push3(p0, p1, p2, m);
     ts.cur = bk == 0 ? rf.x : (bk == 1 ? rf.y : (bk == 2 ? rf.z : rf.w));
     return kAnyGo;
+}
+
+// Any-hit at a node of the occlusion tree (DNode4): the nearest child whose
+// box is entered before tmax next, the other entered children pushed (in
+// child order), or a pop when none is entered. Empty slots hold an inverted
+// box, which the slab test never enters. The answer equals
+// BVHAccel::IntersectP's on the binary tree: the occlusion tree keeps its
+// leaves and skips only boxes that contain their children's boxes.
+PRR_DEV int anyInterior4(const SceneView &s, TravState &ts, RefStack &st) {
+    const DNode4 *n = s.nodes4 + ts.cur;
+    const float4 lx = n->lox, hx = n->hix, ly = n->loy, hy = n->hiy, lz = n->loz, hz = n->hiz;
+    const float t0 = slab(ts.r, lx.x, hx.x, ly.x, hy.x, lz.x, hz.x);
+    const float t1 = slab(ts.r, lx.y, hx.y, ly.y, hy.y, lz.y, hz.y);
+    const float t2 = slab(ts.r, lx.z, hx.z, ly.z, hy.z, lz.z, hz.z);
+    const float t3 = slab(ts.r, lx.w, hx.w, ly.w, hy.w, lz.w, hz.w);
+    return anyChoose4(ts, st, n->ref, t0, t1, t2, t3);
 }
 
 PRR_DEV int anyStep4(const SceneView &s, TravState &ts, RefStack &st) {
@@ -641,26 +651,59 @@ PRR_DEV int anyInterior4Q(const SceneView &s, TravState &ts, RefStack &st) {
            decQ(nzw, k, sz, oz), decQ(fzw, k, sz, oz))
     const float t0 = PRR_TQ(0), t1 = PRR_TQ(1), t2 = PRR_TQ(2), t3 = PRR_TQ(3);
 #undef PRR_TQ
-    const uint4 rf = nq.w3;
-    const float tm = ts.tmax;
-    int bk = -1;
-    float bt = tm;
-    bk = t0 < bt ? 0 : bk;
-    bt = t0 < bt ? t0 : bt;
-    bk = t1 < bt ? 1 : bk;
-    bt = t1 < bt ? t1 : bt;
-    bk = t2 < bt ? 2 : bk;
-    bt = t2 < bt ? t2 : bt;
-    bk = t3 < bt ? 3 : bk;
-    if (bk < 0) return anyPop(ts, st);
-    const bool h0 = t0 < tm && bk != 0, h1 = t1 < tm && bk != 1, h2 = t2 < tm && bk != 2, h3 = t3 < tm && bk != 3;
-    const uint32_t p0 = h0 ? rf.x : (h1 ? rf.y : (h2 ? rf.z : rf.w));
-    const uint32_t p1 = h0 ? (h1 ? rf.y : (h2 ? rf.z : rf.w)) : (h1 ? (h2 ? rf.z : rf.w) : rf.w);
-    const uint32_t p2 = bk == 3 ? rf.z : rf.w;
-    const int m = (int) h0 + (int) h1 + (int) h2 + (int) h3;
-    st.push3(p0, p1, p2, m);
-    ts.cur = bk == 0 ? rf.x : (bk == 1 ? rf.y : (bk == 2 ? rf.z : rf.w));
-    return kAnyGo;
+    return anyChoose4(ts, st, nq.w3, t0, t1, t2, t3);
+}
+
+// Whether a shadow ray may take the short form of the quantised node test:
+// its origin inside the scene's safe box (NaN fails every comparison) and its
+// largest direction component in [2^-10, 2^10]. Also sets the clamped 1/d.
+PRR_DEV void anyPrepQ(const SceneView &s, TravState &ts) {
+    const float m = fmaxf(fmaxf(fabsf(ts.d.x), fabsf(ts.d.y)), fabsf(ts.d.z));
+    const bool in = ts.o.x >= s.safe_box[0] && ts.o.x <= s.safe_box[1] && ts.o.y >= s.safe_box[2] &&
+                    ts.o.y <= s.safe_box[3] && ts.o.z >= s.safe_box[4] && ts.o.z <= s.safe_box[5];
+    ts.fast = s.quant_fast != 0 && in && m >= 0x1p-10f && m <= 0x1p10f;
+#pragma unroll
+    for (int a = 0; a < 3; a++) ts.invc[a] = fminf(fmaxf(ts.r.inv[a], -0x1p100f), 0x1p100f);
+}
+
+// Plane k of a quantised word in the short form: fma(q, scale / d, (origin - o) / d).
+PRR_DEV float planeF(uint32_t w, int k, float a, float b) {
+    return __builtin_fmaf((float) ((w >> (8 * k)) & 255u), a, b);
+}
+
+// anyInterior4Q in the short form, for rays with ts.fast: per axis
+// a = scale * (1/d) and b = (origin - o) * (1/d) once per node, then each of
+// a child's six planes is one fma of its byte instead of a decode, a
+// subtraction and a product. Against the reference's own test of any leaf
+// below (t = (plane - o) * (1/d), rounded twice), this form's entry and exit
+// times for the decoded box differ from the exact ones for that box by at
+// most about 6 ulps of the distances involved (the decoded plane is at most
+// the node extent from the node origin, which is within the scene extent R
+// plus the largest coordinate of the origin), and the quantiser widened
+// every decoded box by 32 ulps of those distances, so the decoded box's
+// entry is never later and its exit never earlier than the reference's for a
+// leaf it contains. A 1/d clamped to +-2^100 (d = 0 or tiny) keeps the
+// planes' order on that axis with exit times above 2^80 whenever the origin
+// lies in the slab, while the largest direction component (>= 2^-10) bounds
+// every exit time the reference could compare against below that. Every leaf
+// the exact test enters is therefore reached; the leaf test is exact, so the
+// answer is unchanged.
+PRR_DEV int anyInterior4QF(const SceneView &s, TravState &ts, RefStack &st) {
+    const DNode4Q nq = s.nodes4q[ts.cur];
+    const float ax = __uint_as_float(nq.w0.w) * ts.invc[0], bx = (__uint_as_float(nq.w0.x) - ts.r.o[0]) * ts.invc[0];
+    const float ay = __uint_as_float(nq.w1.x) * ts.invc[1], by = (__uint_as_float(nq.w0.y) - ts.r.o[1]) * ts.invc[1];
+    const float az = __uint_as_float(nq.w1.y) * ts.invc[2], bz = (__uint_as_float(nq.w0.z) - ts.r.o[2]) * ts.invc[2];
+    const uint32_t nxw = ts.r.oct[0] ? nq.w1.w : nq.w1.z, fxw = ts.r.oct[0] ? nq.w1.z : nq.w1.w;
+    const uint32_t nyw = ts.r.oct[1] ? nq.w2.y : nq.w2.x, fyw = ts.r.oct[1] ? nq.w2.x : nq.w2.y;
+    const uint32_t nzw = ts.r.oct[2] ? nq.w2.w : nq.w2.z, fzw = ts.r.oct[2] ? nq.w2.z : nq.w2.w;
+    float t[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const float tmin = fmaxf(fmaxf(0.f, planeF(nxw, k, ax, bx)), fmaxf(planeF(nyw, k, ay, by), planeF(nzw, k, az, bz)));
+        const float tmax = fminf(planeF(fxw, k, ax, bx), fminf(planeF(fyw, k, ay, by), planeF(fzw, k, az, bz)));
+        t[k] = tmin <= tmax ? tmin : __builtin_inff();
+    }
+    return anyChoose4(ts, st, nq.w3, t[0], t[1], t[2], t[3]);
 }
 
 // A leaf of the quantised tree: its exact box first (the binary tree's test
@@ -691,14 +734,20 @@ PRR_DEV int anyLeafQ(const SceneView &s, TravState &ts, RefStack &st) {
     return anyPop(ts, st);
 }
 
+PRR_DEV int anyNode4Q(const SceneView &s, TravState &ts, RefStack &st) {
+    return ts.fast ? anyInterior4QF(s, ts, st) : anyInterior4Q(s, ts, st);
+}
+
+// Steps of IntersectP on the quantised occlusion tree; the ray's TravState
+// was prepared by anyPrepQ after travBegin.
 PRR_DEV int anyStep4Q(const SceneView &s, TravState &ts, RefStack &st) {
-    return refIsLeaf(ts.cur) ? anyLeafQ(s, ts, st) : anyInterior4Q(s, ts, st);
+    return refIsLeaf(ts.cur) ? anyLeafQ(s, ts, st) : anyNode4Q(s, ts, st);
 }
 
 PRR_DEV int anyIter4Q(const SceneView &s, TravState &ts, RefStack &st, int cap) {
     int k = 0;
     while (!refIsLeaf(ts.cur)) {
-        const int r = anyInterior4Q(s, ts, st);
+        const int r = anyNode4Q(s, ts, st);
         if (r != kAnyGo) return r;
         if (++k == cap) return kAnyGo;
     }
@@ -708,6 +757,7 @@ PRR_DEV int anyIter4Q(const SceneView &s, TravState &ts, RefStack &st, int cap) 
 PRR_DEV bool traceAny4Q(const SceneView &s, const V3 &o, const V3 &d, float tMax, RefStack &st) {
     TravState ts;
     travBegin(ts, o, d, tMax, s.root4q, st);
+    anyPrepQ(s, ts);
     if (!anyBegin(s, ts)) return false;
     int r;
     while ((r = anyStep4Q(s, ts, st)) == kAnyGo) {

@@ -286,6 +286,7 @@ __global__ __launch_bounds__(kTraceBlock) void kShadow(SceneView s, PathBufs pb,
                 const float4 o4 = pb.sqOrg[got];
                 travBegin(ts, ld3(o4), ld3(pb.sqDir[got]), o4.w, TREE == 2 ? s.root4q : (TREE == 1 ? s.root4 : s.root),
                           st);
+                if constexpr (TREE == 2) anyPrepQ(s, ts);
                 if (anyBegin(s, ts)) {
                     idx = got;
                     sk = k;

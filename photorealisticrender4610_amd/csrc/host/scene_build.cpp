@@ -415,7 +415,14 @@ struct Widener {
 // the origin is the children's minimum and the scale a power of two with
 // extent / scale <= 255; each child's bytes are moved outwards until the
 // decoded bound fma(q, scale, origin), evaluated in float as on the device,
-// lies outside the child's box, and the scale doubles if a byte runs out.
+// lies outside the child's box widened by a margin, and the scale doubles if
+// a byte runs out. The margin, 2^-19 * (node extent + scene extent R + largest
+// scene coordinate), covers the rounding of the device's short form of the
+// slab test (t = fma(q, scale / d, (origin - o) / d), anyInterior4QF) for ray
+// origins within R of the scene box: it is at least 32 float ulps of every
+// distance involved, while the short form errs by at most about 6 (see the
+// device code), so a leaf the reference's exact test enters is entered by
+// every decoded ancestor either way.
 struct Quantiser {
     HostScene &S;
     static float dec(uint32_t q, float s, float o) { return std::fma((float) q, s, o); }
@@ -454,9 +461,20 @@ struct Quantiser {
         out = makeLeafRef((uint32_t) off, refCount(ref), refSphere(ref));
         return true;
     }
+    // a float at or below x (at or above when up)
+    static float roundOut(double x, bool up) {
+        float f = (float) x;
+        if (up ? (double) f < x : (double) f > x) f = std::nextafter(f, up ? HUGE_VALF : -HUGE_VALF);
+        return f;
+    }
     bool run(std::string &err) {
         S.nodes4q.assign(S.nodes4.size(), DNode4Q{});
         S.leavesq.clear();
+        double R = 0.0, M = 0.0;  // scene extent, largest coordinate magnitude
+        for (int a = 0; a < 3; a++) {
+            R = std::max(R, (double) S.rootBox[2 * a + 1] - (double) S.rootBox[2 * a]);
+            M = std::max(M, std::max(std::fabs((double) S.rootBox[2 * a]), std::fabs((double) S.rootBox[2 * a + 1])));
+        }
         for (size_t i = 0; i < S.nodes4.size(); i++) {
             const DNode4 &w = S.nodes4[i];
             const float lo3[3][4] = {{w.lox.x, w.lox.y, w.lox.z, w.lox.w},
@@ -471,10 +489,21 @@ struct Quantiser {
             float org[3] = {0.f, 0.f, 0.f}, scl[3] = {1.f, 1.f, 1.f};
             uint32_t wq[6] = {0xFFFFFFFFu, 0u, 0xFFFFFFFFu, 0u, 0xFFFFFFFFu, 0u};
             for (int a = 0; a < 3 && n > 0; a++) {
-                float mn = lo3[a][0], mx = hi3[a][0];
+                float mn0 = lo3[a][0], mx0 = hi3[a][0];
                 for (int k = 1; k < n; k++) {
-                    mn = std::min(mn, lo3[a][k]);
-                    mx = std::max(mx, hi3[a][k]);
+                    mn0 = std::min(mn0, lo3[a][k]);
+                    mx0 = std::max(mx0, hi3[a][k]);
+                }
+                const double margin = std::ldexp((double) mx0 - (double) mn0 + R + M, -19);
+                float lo[4], hi[4];  // the children's bounds widened by the margin
+                for (int k = 0; k < n; k++) {
+                    lo[k] = roundOut((double) lo3[a][k] - margin, false);
+                    hi[k] = roundOut((double) hi3[a][k] + margin, true);
+                }
+                float mn = lo[0], mx = hi[0];
+                for (int k = 1; k < n; k++) {
+                    mn = std::min(mn, lo[k]);
+                    mx = std::max(mx, hi[k]);
                 }
                 org[a] = mn;
                 const double ext = (double) mx - (double) mn;
@@ -486,7 +515,7 @@ struct Quantiser {
                 bool ok = false;
                 for (int tries = 0; tries < 8 && !ok; tries++, e++) {
                     scl[a] = std::ldexp(1.f, e);
-                    ok = axis(lo3[a], hi3[a], n, org[a], scl[a], wq[2 * a], wq[2 * a + 1]);
+                    ok = axis(lo, hi, n, org[a], scl[a], wq[2 * a], wq[2 * a + 1]);
                 }
                 if (!ok) {
                     err = "occlusion tree: a child box does not fit its node's grid";

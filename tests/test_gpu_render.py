@@ -147,7 +147,9 @@ def test_shadow_trees_agree(ctx, monkeypatch):
     image bit for bit, and batch any-hit queries agree with the oracle on
     each: an any-hit answer does not depend on the tree, the occlusion tree
     skips only boxes that contain their children's boxes, and the quantised
-    boxes contain the exact ones with the exact leaf test kept."""
+    boxes contain the exact ones with the exact leaf test kept. The same holds
+    with the short form of the quantised node test switched off
+    (PRR_SHADOW_FAST=0: every ray on the full decode)."""
     from photorealisticrender4610_amd import abi
     import testscenes as TS
     sc = TS.mini_field()
@@ -168,8 +170,9 @@ def test_shadow_trees_agree(ctx, monkeypatch):
     tmax = (rng.random(n) * np.linalg.norm(b[3:] - b[:3])).astype(np.float32)
     want = O.OracleScene(sc).any(org, d, tmax)
     assert np.array_equal(ctx.trace_any(scene, org, d, tmax).cpu().numpy(), want)
-    for mode in ("1", "0"):
-        monkeypatch.setenv("PRR_SHADOW_WIDE", mode)
+    for var, mode in (("PRR_SHADOW_WIDE", "1"), ("PRR_SHADOW_WIDE", "0"), ("PRR_SHADOW_FAST", "0")):
+        monkeypatch.delenv("PRR_SHADOW_WIDE", raising=False)
+        monkeypatch.setenv(var, mode)
         c2 = abi.Context(0)
         try:
             s2 = c2.scene(sc)
@@ -179,9 +182,55 @@ def test_shadow_trees_agree(ctx, monkeypatch):
             del s2
         finally:
             c2.close()
-        assert np.array_equal(img_q, img), mode
+        assert np.array_equal(img_q, img), (var, mode)
         assert st_q["rays_any"] == st["rays_any"] and st_q["rays_closest"] == st["rays_closest"]
-        assert np.array_equal(anyh, want), mode
+        assert np.array_equal(anyh, want), (var, mode)
+
+
+def test_shadow_short_form_boundaries(ctx):
+    """The short form of the quantised node test (fma of a plane byte with
+    scale / d and (origin - o) / d, against boxes widened by a margin) at its
+    boundaries, against the oracle's IntersectP: origins on leaf-box planes
+    (triangle vertices) with axis directions, which make the reference's
+    1/d infinite and some of its plane times NaN; direction components of
+    1e-35 (1/d clamped to +-2^100); origins far outside the region the margin
+    covers and directions scaled by 1e-6 or 1e6 (those rays keep the full
+    decode); ranges of 0, the hit distance and infinity."""
+    import testscenes as TS
+    print("case shadow short form", flush=True)
+    sc = TS.mini_field()
+    scene = ctx.scene(sc)
+    orc = O.OracleScene(sc)
+    info = scene.info()
+    b = np.asarray(info["bounds"], np.float32)
+    ext = float(np.max(b[3:] - b[:3]))
+    rng = np.random.default_rng(41)
+    tris = np.asarray(sc.tris, np.float32).reshape(-1, 3, 3)
+    verts = tris[rng.integers(0, len(tris), 3000), rng.integers(0, 3, 3000)]
+    axes = np.array([(1, 0, 0), (-1, 0, 0), (0, 1, 0), (0, -1, 0), (0, 0, 1), (0, 0, -1),
+                     (0.70710677, 0, -0.70710677), (0, 0.70710677, 0.70710677)], np.float32)
+    o1 = np.repeat(verts, len(axes), 0)
+    d1 = np.tile(axes, (len(verts), 1))
+    n2 = 20000
+    o2 = (b[:3] + rng.random((n2, 3)) * (b[3:] - b[:3])).astype(np.float32)
+    d2 = rng.normal(size=(n2, 3)).astype(np.float32)
+    d2 /= np.linalg.norm(d2, axis=1, keepdims=True)
+    d2[: n2 // 4, rng.integers(0, 3)] = 1e-35
+    d2[n2 // 4: n2 // 2] *= np.float32(1e-6)
+    d2[n2 // 2: 3 * n2 // 4] *= np.float32(1e6)
+    far = (b[:3] + b[3:]) / 2 + rng.normal(size=(n2 // 4, 3)).astype(np.float32) * 5 * ext
+    o2[3 * n2 // 4:] = far
+    tgt = b[:3] + rng.random((n2 // 4, 3)) * (b[3:] - b[:3])
+    d2[3 * n2 // 4:] = (tgt - far) / np.linalg.norm(tgt - far, axis=1, keepdims=True)
+    org = np.concatenate([o1, o2]).astype(np.float32)
+    d = np.concatenate([d1, d2]).astype(np.float32)
+    n = len(org)
+    prim, t, _, _ = (x.cpu().numpy() for x in ctx.trace_closest(scene, org, d))
+    for tm in (np.zeros(n), t, np.full(n, np.inf), rng.random(n) * 2 * ext):
+        tm = np.asarray(tm, np.float32)
+        got = ctx.trace_any(scene, org, d, tm).cpu().numpy()
+        assert np.array_equal(got, orc.any(org, d, tm))
+    assert (prim >= 0).mean() > 0.3
 
 
 def test_render_deterministic(ctx):
