@@ -748,8 +748,12 @@ static int modeGltf(const Bundle &b, const char *out) {
 #ifdef PRR_REF_BENCH
 // Scene-level query counts without editing the reference: the link wraps
 // Scene::intersect / Scene::intersectP (-Wl,--wrap, oracle/Makefile), so every
-// call from castRay's translation unit passes through these counters.
+// call from castRay's translation unit passes through these counters. The
+// link-time-optimised build (PRR_REF_BENCH_NOCOUNT) has no wrap and counts
+// nothing: its samples per second are converted with the counting build's
+// queries per sample.
 static thread_local uint64_t t_closest = 0, t_any = 0;
+#ifndef PRR_REF_BENCH_NOCOUNT
 Intersection realIntersect(const Scene *, const Ray &) __asm__("__real__ZNK5Scene9intersectERK3Ray");
 bool realIntersectP(const Scene *, const Ray &, float) __asm__("__real__ZNK5Scene10intersectPERK3Rayf");
 Intersection wrapIntersect(const Scene *s, const Ray &r) __asm__("__wrap__ZNK5Scene9intersectERK3Ray");
@@ -762,6 +766,7 @@ bool wrapIntersectP(const Scene *s, const Ray &r, float t) {
     t_any++;
     return realIntersectP(s, r, t);
 }
+#endif
 
 // The reference's own Render sample loop (Renderer.cpp:63-99, fixed spp,
 // rows handed out one at a time) on rows [row0, row1), timed with the wall
