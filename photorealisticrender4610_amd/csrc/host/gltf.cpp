@@ -6,7 +6,8 @@
 // buffers (GLB BIN chunk, files next to the asset, base64 data: URIs),
 // accessors, and a PNG decoder on zlib's inflate. Images are decoded to four
 // channels as the reference's loader requests them (req_comp 4, 16-bit
-// images kept at 16 bits and reduced with >> 8 per GLTFLoader.cpp:77-84).
+// images kept at 16 bits, then reduced as GLTFLoader.cpp:77-137 does per
+// texture slot).
 // JPEG images are rejected with an error (no decoder here).
 #include <zlib.h>
 
@@ -647,14 +648,21 @@ Model openModel(const std::string &path) {
     return m;
 }
 
-// image index -> texture row (8-bit RGBA after the reference's >> 8)
-int textureOf(const Model &m, const std::string &dir, Loaded &L, std::map<int, int> &cache, int texIdx) {
+// image index -> texture row, 8-bit RGBA. A 16-bit image becomes 8-bit by
+// >> 8 (GLTFLoader.cpp:81-88) in the colour, metallic-roughness and normal
+// slots; the emissive slot keeps the decoded image's bytes as they are
+// (GLTFLoader.cpp:131-137: two little-endian bytes per channel, read with the
+// default four one-byte channels per texel), of which texel fetches reach the
+// first width * height * 4, the row kept here.
+int textureOf(const Model &m, const std::string &dir, Loaded &L, std::map<int, int> &cache, int texIdx,
+              bool emissive = false) {
     const Json *texs = m.j.get("textures");
     if (texIdx < 0 || !texs || (size_t) texIdx >= texs->size()) return -1;
     const int src = texs->at((size_t) texIdx).intOr("source", -1);
     const Json *imgs = m.j.get("images");
     if (src < 0 || !imgs || (size_t) src >= imgs->size()) return -1;
-    auto it = cache.find(src);
+    const int key = 2 * src + (emissive ? 1 : 0);
+    auto it = cache.find(key);
     if (it != cache.end()) return it->second;
     const Json &im = imgs->at((size_t) src);
     std::vector<uint8_t> bytes;
@@ -675,12 +683,19 @@ int textureOf(const Model &m, const std::string &dir, Loaded &L, std::map<int, i
     }
     const Image img = decodePng(bytes.data(), bytes.size());
     std::vector<uint8_t> t(img.px.size());
-    for (size_t i = 0; i < t.size(); i++) t[i] = (uint8_t) (img.bits == 16 ? img.px[i] >> 8 : img.px[i]);
+    if (img.bits == 16 && emissive) {
+        for (size_t i = 0; i < t.size(); i++) {
+            const uint32_t v = img.px[i / 2];
+            t[i] = (uint8_t) ((i & 1) ? v >> 8 : v & 0xFFu);
+        }
+    } else {
+        for (size_t i = 0; i < t.size(); i++) t[i] = (uint8_t) (img.bits == 16 ? img.px[i] >> 8 : img.px[i]);
+    }
     L.texData.push_back(std::move(t));
     prr_texture_desc td{img.w, img.h, 4, nullptr};
     L.tex.push_back(td);
     const int row = (int) L.tex.size() - 1;
-    cache[src] = row;
+    cache[key] = row;
     return row;
 }
 
@@ -766,7 +781,7 @@ void load(const std::string &path, int matType, Loaded &L) {
                 tx[0] = textureOf(m, dir, L, texCache, texIndex(&P, "baseColorTexture"));
                 tx[1] = textureOf(m, dir, L, texCache, texIndex(&P, "metallicRoughnessTexture"));
                 tx[2] = textureOf(m, dir, L, texCache, texIndex(&gm, "normalTexture"));
-                tx[3] = textureOf(m, dir, L, texCache, texIndex(&gm, "emissiveTexture"));
+                tx[3] = textureOf(m, dir, L, texCache, texIndex(&gm, "emissiveTexture"), true);
             }
             if (matType == PRR_MAT_GLASS) {  // baseColor, emission, every texture; GlassMaterial defaults
                 md.type = PRR_MAT_GLASS;

@@ -366,7 +366,8 @@ def gltf_assets(seed=81):
     plus side files. textured.glb: two triangle primitives (float attributes,
     u16 indices; normalised u16/u8 attributes, u8 indices, no material), a
     line primitive (skipped), four PNG textures (RGB 8-bit, RGBA 16-bit,
-    palette 4-bit with tRNS, grey 2-bit). textured.gltf: external .bin buffer
+    palette 4-bit with tRNS, grey 2-bit); the 16-bit image is also the second
+    material's emissive texture, which the reference keeps undecimated. textured.gltf: external .bin buffer
     and a base64 data: image, no index accessor."""
     import base64
     import json
@@ -432,7 +433,8 @@ def gltf_assets(seed=81):
                                       "roughnessFactor": 0.75, "baseColorTexture": {"index": 0},
                                       "metallicRoughnessTexture": {"index": 1}},
              "normalTexture": {"index": 2}, "emissiveTexture": {"index": 3}, "emissiveFactor": [0.5, 0.25, 2.0]},
-            {"pbrMetallicRoughness": {"baseColorTexture": {"index": 4}}, "normalTexture": {"index": 5}},
+            {"pbrMetallicRoughness": {"baseColorTexture": {"index": 4}}, "normalTexture": {"index": 5},
+             "emissiveTexture": {"index": 1}},
         ],
         "meshes": [
             {"primitives": [{"attributes": {"POSITION": A_pos, "NORMAL": A_nrm, "TEXCOORD_0": A_uv,

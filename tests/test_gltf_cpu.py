@@ -80,6 +80,9 @@ def test_loader_matches_reference(assets, name, t):
             assert row >= 0
             px = g["textures"][row]
             assert px.shape == (h, w, ch)
+            if nb == 2 * w * h * ch:  # a 16-bit emissive image, kept as its raw bytes; fetches reach the first half
+                assert j == 3
+                ref_px = ref_px[: w * h * ch]
             assert np.array_equal(px.ravel(), ref_px)
 
 
@@ -113,10 +116,12 @@ def test_scene_builder_gltf(assets):
     sc = b.build()
     assert len(objs) == 3 and len(objs2) == 1
     assert sc.tri_flags.shape[0] == sc.num_triangles == 6 + 3 + 6 + 2
-    assert len(sc.textures) == 5 and sc.mat_tex.shape == (sc.materials.shape[0], 4)
+    # five images; the 16-bit one twice: reduced to 8 bits for its
+    # metallic-roughness use, its raw bytes for its emissive use
+    assert len(sc.textures) == 6 and sc.mat_tex.shape == (sc.materials.shape[0], 4)
     assert sc.objects[objs2[0], 1] == m0
     d, keep = abi.scene_desc(sc)
-    assert d.num_textures == 5 and bool(d.tri_flags)
+    assert d.num_textures == 6 and bool(d.tri_flags)
 
 
 def _tiny_gltf(tmp_path, name, edit=None):
