@@ -224,7 +224,7 @@ int runBounces(prr_context *c, const SceneView &view, const ChunkInfo &ci, Timin
         }
         TIMED(0, st,
               launchExtend(view, c->pb, b, c->spill, c->traceBlocks, c->refillAt, c->stackDepth, c->whileWhile,
-                           c->fetchRun, cit.tailPaths, st));
+                           c->speculate, c->fetchRun, cit.tailPaths, st));
         if (b > 0 && !conc)
             TIMED(1, st,
                   launchShadow(view, pbS, b - 1, c->spill, c->traceBlocks, c->refillAt, c->stackDepth, wwShadow,
@@ -388,6 +388,7 @@ int prr_create(int device, prr_context **out) {
     if (const char *ev = getenv("PRR_WHILE_WHILE_ANY")) c->whileWhileAny = std::max(0, atoi(ev));
     if (const char *ev = getenv("PRR_SHADOW_WIDE")) c->shadowWide = std::max(0, std::min(2, atoi(ev)));
     if (const char *ev = getenv("PRR_SHADOW_FAST")) c->shadowFast = atoi(ev) != 0;
+    if (const char *ev = getenv("PRR_SPECULATE")) c->speculate = atoi(ev) != 0;
     if (const char *ev = getenv("PRR_CONCURRENT_SHADOW")) c->concurrentShadow = atoi(ev) != 0;
     if (const char *ev = getenv("PRR_CONCURRENT_MAX_PATHS")) c->concurrentMaxPaths = std::max<int64_t>(0, atoll(ev));
     if (const char *ev = getenv("PRR_FETCH_RUN")) c->fetchRun = std::max(64, std::min(1 << 16, atoi(ev)));

@@ -306,6 +306,7 @@ struct TravState {
     V3 o, d;
     float tmax;  // any-hit range
     uint32_t cur;
+    float curT;  // closest hit, speculative iteration: entry t of cur's box
     HitOut h;
     // quantised occlusion tree: this ray may take the short form of the node
     // test (anyPrepQ), with 1/d clamped to +-2^100
@@ -320,6 +321,7 @@ PRR_DEV void travBegin(TravState &ts, const V3 &o, const V3 &d, float tmax, uint
     ts.tmax = tmax;
     ts.r = precompute(o, d);
     ts.cur = root;
+    ts.curT = 0.f;
     ts.h.t = __builtin_inff();
     ts.h.u = ts.h.v = 0.f;
     ts.h.prim = -1;
@@ -341,9 +343,8 @@ PRR_DEV bool popOne(TravState &ts, LaneStack &st) {
     return false;
 }
 
-// BVHAccel::Intersect at a leaf (ts.cur): test its primitives, then pop.
-PRR_DEV bool closestLeaf(const SceneView &s, TravState &ts, LaneStack &st) {
-    const uint32_t cur = ts.cur;
+// BVHAccel::Intersect's primitive tests at leaf `cur` (in leaf order).
+PRR_DEV void closestPrims(const SceneView &s, TravState &ts, uint32_t cur) {
     const uint32_t first = refFirst(cur), cnt = refCount(cur);
     if (refSphere(cur)) {
         for (uint32_t i = 0; i < cnt; i++) {
@@ -373,6 +374,11 @@ PRR_DEV bool closestLeaf(const SceneView &s, TravState &ts, LaneStack &st) {
             }
         }
     }
+}
+
+// BVHAccel::Intersect at a leaf (ts.cur): test its primitives, then pop.
+PRR_DEV bool closestLeaf(const SceneView &s, TravState &ts, LaneStack &st) {
+    closestPrims(s, ts, ts.cur);
     return popOne(ts, st);
 }
 
@@ -454,6 +460,87 @@ PRR_DEV bool closestIter(const SceneView &s, TravState &ts, LaneStack &st, int c
     if (done) return true;
     if (!refIsLeaf(cur)) return false;  // step cap reached
     return closestLeaf(s, ts, st);
+}
+
+// Speculative while-while iteration (Aila and Laine 2009): a lane that
+// reaches a leaf sets it aside and keeps stepping through interior nodes
+// until it reaches a second leaf (or its stack runs out, or the step cap),
+// so fewer lanes sit idle while the rest of the wave finishes its descent;
+// then it tests the set-aside leaf and the second one, in that order.
+//
+// The result is BVHAccel::Intersect's, ids and ties included. The steps taken
+// after the first leaf use the closest t from before that leaf's tests, which
+// is never smaller than the reference's at the same point, so they visit a
+// superset of the reference's nodes in the same order (near/far choices
+// depend on boxes only) and push no entry the reference pops uncullled. A
+// leaf is tested only if its own entry t is still below the closest hit when
+// its turn comes: the slab test is monotone in the box bounds and every box
+// of the tree contains its children's, so its ancestors' entry t are no
+// larger and the reference, whose closest t at their visits was no smaller,
+// entered them and reaches the leaf with the same closest t. Both therefore
+// test the same leaves in the same order with the same running closest hit.
+PRR_DEV bool closestIterSpec(const SceneView &s, TravState &ts, LaneStack &st, int cap) {
+    uint32_t cur = ts.cur;
+    float curT = ts.curT;  // entry t of cur's box (0: the root)
+    uint32_t aside = kCulled;  // set-aside leaf (kCulled: none)
+    float asideT = 0.f;
+    int sp = st.sp;
+    const float best = ts.h.t;
+    const char *nodes = reinterpret_cast<const char *>(s.nodes);
+    uint2 *lds = st.lds + st.lane;
+    const int dmask = st.depth - 1;
+    bool done = false;
+    const int lim = cap > 0 ? cap : -1;
+    for (int k = 0; !done && k != lim; k++) {
+        bool pop = true;
+        if (refIsLeaf(cur)) {
+            if (aside != kCulled) break;  // two leaves in hand
+            aside = cur;
+            asideT = curT;
+        } else if (cur != kCulled) {
+            const DNode n = *reinterpret_cast<const DNode *>(nodes + (cur << 6));
+            const float tL = slab(ts.r, n.a.x, n.a.y, n.a.z, n.a.w, n.b.x, n.b.y);
+            const float tR = slab(ts.r, n.b.z, n.b.w, n.c.x, n.c.y, n.c.z, n.c.w);
+            const bool lFirst = tL <= tR;
+            const float tNear = lFirst ? tL : tR, tFar = lFirst ? tR : tL;
+            const uint32_t rNear = (uint32_t) (lFirst ? n.d.x : n.d.y), rFar = (uint32_t) (lFirst ? n.d.y : n.d.x);
+            if (tFar < best) {  // LaneStack::push
+                uint2 *slot = lds + (sp & dmask) * kTraceBlock;
+                if (sp >= st.depth) st.spill[sp - st.depth] = *slot;
+                *slot = make_uint2(rFar, __float_as_uint(tFar));
+                sp++;
+            }
+            pop = !(tNear < best);
+            cur = rNear;
+            curT = tNear;
+        }
+        if (pop) {
+            done = sp == 0;
+            if (!done) {  // LaneStack::pop
+                sp--;
+                uint2 *slot = lds + (sp & dmask) * kTraceBlock;
+                const uint2 e = *slot;
+                if (sp >= st.depth) *slot = st.spill[sp - st.depth];
+                curT = __uint_as_float(e.y);
+                cur = curT < best ? e.x : kCulled;
+            }
+        }
+    }
+    st.sp = sp;
+    if (aside != kCulled && asideT < ts.h.t) closestPrims(s, ts, aside);
+    if (!done && refIsLeaf(cur)) {
+        if (curT < ts.h.t) closestPrims(s, ts, cur);
+        // the next entry (BVH.cpp:399-407), culled against the updated closest t
+        done = st.sp == 0;
+        if (!done) {
+            const uint2 e = st.pop();
+            curT = __uint_as_float(e.y);
+            cur = curT < ts.h.t ? e.x : kCulled;
+        }
+    }
+    ts.cur = cur;
+    ts.curT = curT;
+    return done;
 }
 
 // Root test of BVHAccel::IntersectP; false = the ray misses the scene box.

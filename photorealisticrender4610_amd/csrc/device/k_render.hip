@@ -209,9 +209,11 @@ __global__ void kCopyResults(PathBufs pb, int64_t n, int64_t first, float *L) {
 // Persistent traversal: idle lanes refill from the queue once `refillAt`
 // lanes of the wave are idle (64: only when the whole wave is idle), taking
 // consecutive entries from the wave's claimed run (waveTake), which costs an
-// atomic on the bounce's counter only when the run is used up.
+// atomic on the bounce's counter only when the run is used up. ITER: 0 one
+// node per work check, 1 while-while (closestIter), 2 speculative while-while
+// (closestIterSpec).
 
-template <bool WW>
+template <int ITER>
 __global__ __launch_bounds__(kTraceBlock) void kExtend(SceneView s, PathBufs pb, int bounce, uint2 *spill, int refillAt,
                                                        int stackDepth, int wwCap, int fetchRun, int64_t tailPaths) {
     extern __shared__ uint2 stk[];
@@ -250,7 +252,9 @@ __global__ __launch_bounds__(kTraceBlock) void kExtend(SceneView s, PathBufs pb,
             }
         }
         if (__ballot(idx >= 0 || !exhausted) == 0) break;
-        if (idx >= 0 && (WW ? closestIter(s, ts, st, wwCap) : closestStep(s, ts, st))) {
+        if (idx >= 0 && (ITER == 2   ? closestIterSpec(s, ts, st, wwCap)
+                         : ITER == 1 ? closestIter(s, ts, st, wwCap)
+                                     : closestStep(s, ts, st))) {
             pb.hit[idx] = make_float4(ts.h.t, ts.h.u, ts.h.v, __int_as_float(ts.h.prim));
             idx = -1;
         }
@@ -1095,14 +1099,19 @@ hipError_t launchCopyResults(const PathBufs &pb, int64_t n, int64_t first, float
     return hipGetLastError();
 }
 hipError_t launchExtend(const SceneView &s, const PathBufs &pb, int bounce, uint2 *spill, int traceGrid,
-                        int refillAt, int stackDepth, int whileWhile, int fetchRun, int64_t tailPaths, hipStream_t st) {
+                        int refillAt, int stackDepth, int whileWhile, bool speculate, int fetchRun, int64_t tailPaths,
+                        hipStream_t st) {
     const int wwCap = whileWhile > 1 ? whileWhile : 0;
-    if (whileWhile)
-        hipLaunchKernelGGL(kExtend<true>, dim3(traceGrid), dim3(kTraceBlock), stackDepth * kTraceBlock * sizeof(uint2),
-                           st, s, pb, bounce, spill, refillAt, stackDepth, wwCap, fetchRun, tailPaths);
+    const size_t lds = stackDepth * kTraceBlock * sizeof(uint2);
+    if (whileWhile && speculate)
+        hipLaunchKernelGGL(kExtend<2>, dim3(traceGrid), dim3(kTraceBlock), lds, st, s, pb, bounce, spill, refillAt,
+                           stackDepth, wwCap, fetchRun, tailPaths);
+    else if (whileWhile)
+        hipLaunchKernelGGL(kExtend<1>, dim3(traceGrid), dim3(kTraceBlock), lds, st, s, pb, bounce, spill, refillAt,
+                           stackDepth, wwCap, fetchRun, tailPaths);
     else
-        hipLaunchKernelGGL(kExtend<false>, dim3(traceGrid), dim3(kTraceBlock), stackDepth * kTraceBlock * sizeof(uint2),
-                           st, s, pb, bounce, spill, refillAt, stackDepth, wwCap, fetchRun, tailPaths);
+        hipLaunchKernelGGL(kExtend<0>, dim3(traceGrid), dim3(kTraceBlock), lds, st, s, pb, bounce, spill, refillAt,
+                           stackDepth, wwCap, fetchRun, tailPaths);
     return hipGetLastError();
 }
 hipError_t launchShadow(const SceneView &s, const PathBufs &pb, int bounce, uint2 *spill, int traceGrid,

@@ -187,6 +187,44 @@ def test_shadow_trees_agree(ctx, monkeypatch):
         assert np.array_equal(anyh, want), (var, mode)
 
 
+def test_speculative_extension_rays_agree(ctx, monkeypatch):
+    """Extension rays traced with speculative while-while iteration
+    (PRR_SPECULATE=1: a lane that reaches a leaf sets it aside and steps on to
+    a second one, each leaf re-checked against the closest hit when its turn
+    comes) give the same images and ray counts bit for bit as the plain
+    while-while iteration, on scenes with triangles and spheres, glass and
+    mirror chains, and the full 1M-triangle scene."""
+    from photorealisticrender4610_amd import abi, scenes
+    import testscenes as TS
+    print("case speculative extension rays", flush=True)
+    cases = [(scenes.box_room(), 128, 96, 8, 8), (TS.mini_field(), 160, 120, 4, 12),
+             (scenes.sphere_field(), 320, 180, 4, 12), (scenes.glass_torus(), 128, 128, 4, 12)]
+    # every bounce in the per-bounce kernels (no path-completion launch), so
+    # all extension rays go through kExtend
+    monkeypatch.setenv("PRR_TAIL_PATHS", "0")
+    c1 = abi.Context(0)
+    want = []
+    try:
+        for sc, w, h, spp, depth in cases:
+            scene = c1.scene(sc)
+            img, st = c1.render(scene, w, h, spp, depth, seed=21)
+            want.append((img.cpu().numpy(), st["rays_closest"], st["rays_any"]))
+            del scene
+    finally:
+        c1.close()
+    monkeypatch.setenv("PRR_SPECULATE", "1")
+    c2 = abi.Context(0)
+    try:
+        for (sc, w, h, spp, depth), (img0, rc, ra) in zip(cases, want):
+            s2 = c2.scene(sc)
+            img, st = c2.render(s2, w, h, spp, depth, seed=21)
+            assert np.array_equal(img.cpu().numpy(), img0), sc.name
+            assert st["rays_closest"] == rc and st["rays_any"] == ra, sc.name
+            del s2
+    finally:
+        c2.close()
+
+
 def test_shadow_short_form_boundaries(ctx):
     """The short form of the quantised node test (fma of a plane byte with
     scale / d and (origin - o) / d, against boxes widened by a margin) at its
