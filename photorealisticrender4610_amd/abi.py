@@ -93,7 +93,8 @@ class Stats(C.Structure):
 class SceneInfo(C.Structure):
     _fields_ = [("num_nodes", C.c_int32), ("num_leaf_prims", C.c_int32), ("max_depth", C.c_int32),
                 ("num_emitters", C.c_int32), ("total_emit_area", C.c_float), ("bounds", C.c_float * 6),
-                ("device_bytes", C.c_int64), ("num_wide_nodes", C.c_int32), ("wide_levels_kept", C.c_int32)]
+                ("device_bytes", C.c_int64), ("num_wide_nodes", C.c_int32), ("wide_levels_kept", C.c_int32),
+                ("two_level", C.c_int32), ("reserved", C.c_int32)]
 
 
 class CacheParams(C.Structure):
@@ -626,7 +627,7 @@ class Scene:
         return {"num_nodes": inf.num_nodes, "num_leaf_prims": inf.num_leaf_prims, "max_depth": inf.max_depth,
                 "num_emitters": inf.num_emitters, "total_emit_area": inf.total_emit_area,
                 "bounds": list(inf.bounds), "device_bytes": inf.device_bytes, "num_wide_nodes": inf.num_wide_nodes,
-                "wide_levels_kept": inf.wide_levels_kept}
+                "wide_levels_kept": inf.wide_levels_kept, "two_level": bool(inf.two_level)}
 
     def export_level(self, level):
         n = i32()

@@ -224,7 +224,7 @@ int runBounces(prr_context *c, const SceneView &view, const ChunkInfo &ci, Timin
         }
         TIMED(0, st,
               launchExtend(view, c->pb, b, c->spill, c->traceBlocks, c->refillAt, c->stackDepth, c->whileWhile,
-                           c->speculate, c->fetchRun, cit.tailPaths, st));
+                           c->speculate, c->twoLevel, c->fetchRun, cit.tailPaths, st));
         if (b > 0 && !conc)
             TIMED(1, st,
                   launchShadow(view, pbS, b - 1, c->spill, c->traceBlocks, c->refillAt, c->stackDepth, wwShadow,
@@ -389,6 +389,7 @@ int prr_create(int device, prr_context **out) {
     if (const char *ev = getenv("PRR_SHADOW_WIDE")) c->shadowWide = std::max(0, std::min(2, atoi(ev)));
     if (const char *ev = getenv("PRR_SHADOW_FAST")) c->shadowFast = atoi(ev) != 0;
     if (const char *ev = getenv("PRR_SPECULATE")) c->speculate = atoi(ev) != 0;
+    if (const char *ev = getenv("PRR_TWO_LEVEL")) c->twoLevel = atoi(ev) != 0;
     if (const char *ev = getenv("PRR_CONCURRENT_SHADOW")) c->concurrentShadow = atoi(ev) != 0;
     if (const char *ev = getenv("PRR_CONCURRENT_MAX_PATHS")) c->concurrentMaxPaths = std::max<int64_t>(0, atoll(ev));
     if (const char *ev = getenv("PRR_FETCH_RUN")) c->fetchRun = std::max(64, std::min(1 << 16, atoi(ev)));
@@ -472,6 +473,7 @@ int prr_scene_create(prr_context *c, const prr_scene_desc *d, prr_scene **out) {
 #define UP(dst, src)                                                               \
     if (e == hipSuccess) e = upload(s, (std::remove_const<std::remove_reference<decltype(*dst)>::type>::type **) &dst, src, c->stream)
     UP(v.nodes, h.nodes);
+    if (!h.nodes2.empty()) UP(v.nodes2, h.nodes2);
     UP(v.nodes4, h.nodes4);
     UP(v.nodes4q, h.nodes4q);
     UP(v.leavesq, h.leavesq);
@@ -561,6 +563,8 @@ int prr_scene_get_info(const prr_scene *s, prr_scene_info *o) {
     o->device_bytes = s->bytes;
     o->num_wide_nodes = (int32_t) h.nodes4.size();
     o->wide_levels_kept = h.wideLevelsKept;
+    o->two_level = h.nodes2.empty() ? 0 : 1;
+    o->reserved = 0;
     return PRR_OK;
 }
 

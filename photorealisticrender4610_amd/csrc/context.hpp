@@ -28,6 +28,7 @@ struct prr_context {
     int shadowWide = 2;     // shadow rays: 0 the binary tree, 1 the four-wide occlusion tree, 2 its quantised copy
     bool shadowFast = true; // quantised tree: short form of the node test for rays in the scene's safe region
     bool speculate = false; // extension rays: speculative while-while iteration
+    bool twoLevel = false;  // extension rays: two binary levels per step on the two-level records
     int shadeWaves = 4;     // shading kernel compiled for at least this many waves per SIMD (1, 4, 5)
     // shading sorted by material: each bounce's queue split into diffuse-hit
     // entries and the rest, shaded by two specialised kernels (the rest at

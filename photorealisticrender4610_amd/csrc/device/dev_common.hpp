@@ -108,6 +108,17 @@ struct __align__(16) DNode {
     float4 c;  // R.miny R.maxy R.minz R.maxz
     int4 d;    // Lref Rref pad pad
 };
+// Two levels of the binary tree in one 128-byte line (index = the binary
+// node's): the boxes of the left child's two children (a0 b0 c0, laid out as
+// a DNode's pair) and of the right child's (a1 b1 c1); a child that is a leaf
+// stands as the first of its pair with an empty (inverted) second box. Each
+// child's box is the union of its pair's, exactly as in the binary tree.
+struct __align__(16) DNode2 {
+    float4 a0, b0, c0;
+    float4 a1, b1, c1;
+    uint4 gref;  // grandchild refs: left pair, right pair
+    uint4 cref;  // left child ref, right child ref, 0, 0
+};
 // Child reference: bit31 = leaf; leaf: bits 28..30 = count-1, bit 27 =
 // sphere leaf, bits 0..26 = first leaf-order primitive slot.
 constexpr uint32_t kLeafBit = 0x80000000u;
@@ -198,6 +209,7 @@ struct CamView {
 
 struct SceneView {
     const DNode *nodes;
+    const DNode2 *nodes2;     // the same tree two levels per record (null: not built)
     const DNode4 *nodes4;     // occlusion tree (four-wide)
     const DNode4Q *nodes4q;   // the same tree quantised (one line per node)
     const float4 *leavesq;    // its leaf records (exact box + triangles)

@@ -462,6 +462,119 @@ PRR_DEV bool closestIter(const SceneView &s, TravState &ts, LaneStack &st, int c
     return closestLeaf(s, ts, st);
 }
 
+// Entry and exit times of a box's six planes for one ray, as slab() forms
+// them ((bound - o) * inv, near bound chosen by the ray's octant), with the
+// NaN of a plane through the origin of a ray parallel to it (0 * inf) read as
+// -inf for a near plane and +inf for a far one: slab() ignores such a term,
+// and so do these values in its max / min.
+struct Planes {
+    float nx, fx, ny, fy, nz, fz;
+};
+PRR_DEV Planes planesOf(const RayPre &r, float x0, float x1, float y0, float y1, float z0, float z1) {
+    Planes p;
+    p.nx = fmaxf(((r.oct[0] ? x1 : x0) - r.o[0]) * r.inv[0], -__builtin_inff());
+    p.fx = fminf(((r.oct[0] ? x0 : x1) - r.o[0]) * r.inv[0], __builtin_inff());
+    p.ny = fmaxf(((r.oct[1] ? y1 : y0) - r.o[1]) * r.inv[1], -__builtin_inff());
+    p.fy = fminf(((r.oct[1] ? y0 : y1) - r.o[1]) * r.inv[1], __builtin_inff());
+    p.nz = fmaxf(((r.oct[2] ? z1 : z0) - r.o[2]) * r.inv[2], -__builtin_inff());
+    p.fz = fminf(((r.oct[2] ? z0 : z1) - r.o[2]) * r.inv[2], __builtin_inff());
+    return p;
+}
+// slab()'s result from the plane times.
+PRR_DEV float entryOf(const Planes &p) {
+    const float tmin = fmaxf(fmaxf(0.f, p.nx), fmaxf(p.ny, p.nz));
+    const float tmax = fminf(p.fx, fminf(p.fy, p.fz));
+    return tmin <= tmax ? tmin : __builtin_inff();
+}
+// Plane times of the union of two boxes: (bound - o) * inv is monotone in the
+// bound (non-decreasing for inv > 0, non-increasing for inv < 0, and so are
+// the NaN readings above), so the union's near plane time is the smaller of
+// the two near times and its far time the larger, bit for bit what slab()
+// computes from the union's own bounds.
+PRR_DEV Planes unionOf(const Planes &a, const Planes &b) {
+    Planes p;
+    p.nx = fminf(a.nx, b.nx);
+    p.fx = fmaxf(a.fx, b.fx);
+    p.ny = fminf(a.ny, b.ny);
+    p.fy = fmaxf(a.fy, b.fy);
+    p.nz = fminf(a.nz, b.nz);
+    p.fz = fmaxf(a.fz, b.fz);
+    return p;
+}
+
+// closestIter on the two-level records (DNode2): one 128-byte line per step
+// for up to two levels of BVHAccel::Intersect. At binary node cur the
+// children's boxes are the unions of their pairs' (unionOf), so tL and tR are
+// slab()'s; the far child is pushed with its entry t and the near one
+// entered as in closestIter, and when the near child is interior its own
+// step follows at once from the same line (its children are the near pair),
+// with the same closest t, since no leaf lies between the two steps. Pushes,
+// pops and leaves are therefore the reference's, in its order.
+PRR_DEV bool closestIter2(const SceneView &s, TravState &ts, LaneStack &st, int cap) {
+    uint32_t cur = ts.cur;
+    int sp = st.sp;
+    const float best = ts.h.t;
+    uint2 *lds = st.lds + st.lane;
+    const int dmask = st.depth - 1;
+    bool done = false;
+    const int lim = cap > 0 ? cap : -1;
+    for (int k = 0; !done && k != lim && !refIsLeaf(cur); k++) {
+        bool pop = true;
+        if (cur != kCulled) {
+            const DNode2 n = s.nodes2[cur];
+            const Planes g0 = planesOf(ts.r, n.a0.x, n.a0.y, n.a0.z, n.a0.w, n.b0.x, n.b0.y);
+            const Planes g1 = planesOf(ts.r, n.b0.z, n.b0.w, n.c0.x, n.c0.y, n.c0.z, n.c0.w);
+            const Planes g2 = planesOf(ts.r, n.a1.x, n.a1.y, n.a1.z, n.a1.w, n.b1.x, n.b1.y);
+            const Planes g3 = planesOf(ts.r, n.b1.z, n.b1.w, n.c1.x, n.c1.y, n.c1.z, n.c1.w);
+            const float tL = entryOf(unionOf(g0, g1)), tR = entryOf(unionOf(g2, g3));
+            const bool lFirst = tL <= tR;
+            const float tNear = lFirst ? tL : tR, tFar = lFirst ? tR : tL;
+            const uint32_t rNear = lFirst ? n.cref.x : n.cref.y, rFar = lFirst ? n.cref.y : n.cref.x;
+            if (tFar < best) {  // LaneStack::push
+                uint2 *slot = lds + (sp & dmask) * kTraceBlock;
+                if (sp >= st.depth) st.spill[sp - st.depth] = *slot;
+                *slot = make_uint2(rFar, __float_as_uint(tFar));
+                sp++;
+            }
+            if (tNear < best) {
+                pop = false;
+                cur = rNear;
+                if (!refIsLeaf(rNear)) {  // the near child's step: its children are the near pair
+                    const float tA = entryOf(lFirst ? g0 : g2), tB = entryOf(lFirst ? g1 : g3);
+                    const uint32_t rA = lFirst ? n.gref.x : n.gref.z, rB = lFirst ? n.gref.y : n.gref.w;
+                    const bool aFirst = tA <= tB;
+                    const float tN2 = aFirst ? tA : tB, tF2 = aFirst ? tB : tA;
+                    if (tF2 < best) {
+                        uint2 *slot = lds + (sp & dmask) * kTraceBlock;
+                        if (sp >= st.depth) st.spill[sp - st.depth] = *slot;
+                        *slot = make_uint2(aFirst ? rB : rA, __float_as_uint(tF2));
+                        sp++;
+                    }
+                    if (tN2 < best)
+                        cur = aFirst ? rA : rB;
+                    else
+                        pop = true;
+                }
+            }
+        }
+        if (pop) {
+            done = sp == 0;
+            if (!done) {  // LaneStack::pop
+                sp--;
+                uint2 *slot = lds + (sp & dmask) * kTraceBlock;
+                const uint2 e = *slot;
+                if (sp >= st.depth) *slot = st.spill[sp - st.depth];
+                cur = __uint_as_float(e.y) < best ? e.x : kCulled;
+            }
+        }
+    }
+    ts.cur = cur;
+    st.sp = sp;
+    if (done) return true;
+    if (!refIsLeaf(cur)) return false;  // step cap reached
+    return closestLeaf(s, ts, st);
+}
+
 // Speculative while-while iteration (Aila and Laine 2009): a lane that
 // reaches a leaf sets it aside and keeps stepping through interior nodes
 // until it reaches a second leaf (or its stack runs out, or the step cap),

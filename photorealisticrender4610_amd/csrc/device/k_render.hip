@@ -211,7 +211,7 @@ __global__ void kCopyResults(PathBufs pb, int64_t n, int64_t first, float *L) {
 // consecutive entries from the wave's claimed run (waveTake), which costs an
 // atomic on the bounce's counter only when the run is used up. ITER: 0 one
 // node per work check, 1 while-while (closestIter), 2 speculative while-while
-// (closestIterSpec).
+// (closestIterSpec), 3 while-while on the two-level records (closestIter2).
 
 template <int ITER>
 __global__ __launch_bounds__(kTraceBlock) void kExtend(SceneView s, PathBufs pb, int bounce, uint2 *spill, int refillAt,
@@ -252,7 +252,8 @@ __global__ __launch_bounds__(kTraceBlock) void kExtend(SceneView s, PathBufs pb,
             }
         }
         if (__ballot(idx >= 0 || !exhausted) == 0) break;
-        if (idx >= 0 && (ITER == 2   ? closestIterSpec(s, ts, st, wwCap)
+        if (idx >= 0 && (ITER == 3   ? closestIter2(s, ts, st, wwCap)
+                         : ITER == 2 ? closestIterSpec(s, ts, st, wwCap)
                          : ITER == 1 ? closestIter(s, ts, st, wwCap)
                                      : closestStep(s, ts, st))) {
             pb.hit[idx] = make_float4(ts.h.t, ts.h.u, ts.h.v, __int_as_float(ts.h.prim));
@@ -1099,11 +1100,14 @@ hipError_t launchCopyResults(const PathBufs &pb, int64_t n, int64_t first, float
     return hipGetLastError();
 }
 hipError_t launchExtend(const SceneView &s, const PathBufs &pb, int bounce, uint2 *spill, int traceGrid,
-                        int refillAt, int stackDepth, int whileWhile, bool speculate, int fetchRun, int64_t tailPaths,
-                        hipStream_t st) {
+                        int refillAt, int stackDepth, int whileWhile, bool speculate, bool twoLevel, int fetchRun,
+                        int64_t tailPaths, hipStream_t st) {
     const int wwCap = whileWhile > 1 ? whileWhile : 0;
     const size_t lds = stackDepth * kTraceBlock * sizeof(uint2);
-    if (whileWhile && speculate)
+    if (whileWhile && twoLevel && s.nodes2)
+        hipLaunchKernelGGL(kExtend<3>, dim3(traceGrid), dim3(kTraceBlock), lds, st, s, pb, bounce, spill, refillAt,
+                           stackDepth, wwCap, fetchRun, tailPaths);
+    else if (whileWhile && speculate)
         hipLaunchKernelGGL(kExtend<2>, dim3(traceGrid), dim3(kTraceBlock), lds, st, s, pb, bounce, spill, refillAt,
                            stackDepth, wwCap, fetchRun, tailPaths);
     else if (whileWhile)

@@ -15,10 +15,11 @@ hipError_t launchCopyResults(const PathBufs &pb, int64_t n, int64_t first, float
 // the same with at most n interior steps per work check
 // tailPaths: the launch does nothing when the bounce's queue holds at most
 // that many entries (kFinish takes them); speculate: speculative while-while
-// iteration (whileWhile > 0 only)
+// iteration; twoLevel: while-while on the two-level records where the scene
+// has them (both need whileWhile > 0)
 hipError_t launchExtend(const SceneView &s, const PathBufs &pb, int bounce, uint2 *spill, int traceGrid,
-                        int refillAt, int stackDepth, int whileWhile, bool speculate, int fetchRun, int64_t tailPaths,
-                        hipStream_t st);
+                        int refillAt, int stackDepth, int whileWhile, bool speculate, bool twoLevel, int fetchRun,
+                        int64_t tailPaths, hipStream_t st);
 // tree: 0 the binary tree, 1 the four-wide occlusion tree, 2 its quantised copy
 hipError_t launchShadow(const SceneView &s, const PathBufs &pb, int bounce, uint2 *spill, int traceGrid,
                         int refillAt, int stackDepth, int whileWhile, int fetchRun, int tree, hipStream_t st);

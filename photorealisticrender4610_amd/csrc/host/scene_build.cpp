@@ -325,6 +325,62 @@ struct Merger {
     }
 };
 
+// Two-level records of the merged binary tree (DNode2): for every interior
+// node, its children's children. The device rebuilds a child's box as the
+// union of its pair's, which must equal the child's own box bit for bit (the
+// reference's node bounds are unions of its primitives' bounds, so they do);
+// false if some box differs, and then no record is kept.
+bool buildTwoLevel(HostScene &S) {
+    const float inf = std::numeric_limits<float>::infinity();
+    S.nodes2.assign(S.nodes.size(), DNode2{});
+    auto boxesOf = [](const DNode &n, float *l, float *r) {
+        const float lb[6] = {n.a.x, n.a.y, n.a.z, n.a.w, n.b.x, n.b.y};
+        const float rb[6] = {n.b.z, n.b.w, n.c.x, n.c.y, n.c.z, n.c.w};
+        std::memcpy(l, lb, sizeof(lb));
+        std::memcpy(r, rb, sizeof(rb));
+    };
+    for (size_t i = 0; i < S.nodes.size(); i++) {
+        const DNode &n = S.nodes[i];
+        float cb[2][6];
+        boxesOf(n, cb[0], cb[1]);
+        const uint32_t cr[2] = {(uint32_t) n.d.x, (uint32_t) n.d.y};
+        float pb[2][2][6];
+        uint32_t gr[2][2];
+        for (int c = 0; c < 2; c++) {
+            if (refIsLeaf(cr[c])) {
+                std::memcpy(pb[c][0], cb[c], sizeof(cb[c]));
+                for (int a = 0; a < 3; a++) {
+                    pb[c][1][2 * a] = inf;
+                    pb[c][1][2 * a + 1] = -inf;
+                }
+                gr[c][0] = cr[c];
+                gr[c][1] = cr[c];
+            } else {
+                const DNode &m = S.nodes[cr[c]];
+                boxesOf(m, pb[c][0], pb[c][1]);
+                gr[c][0] = (uint32_t) m.d.x;
+                gr[c][1] = (uint32_t) m.d.y;
+                for (int a = 0; a < 3; a++)
+                    if (std::min(pb[c][0][2 * a], pb[c][1][2 * a]) != cb[c][2 * a] ||
+                        std::max(pb[c][0][2 * a + 1], pb[c][1][2 * a + 1]) != cb[c][2 * a + 1]) {
+                        S.nodes2.clear();
+                        return false;
+                    }
+            }
+        }
+        DNode2 &q = S.nodes2[i];
+        q.a0 = make_float4(pb[0][0][0], pb[0][0][1], pb[0][0][2], pb[0][0][3]);
+        q.b0 = make_float4(pb[0][0][4], pb[0][0][5], pb[0][1][0], pb[0][1][1]);
+        q.c0 = make_float4(pb[0][1][2], pb[0][1][3], pb[0][1][4], pb[0][1][5]);
+        q.a1 = make_float4(pb[1][0][0], pb[1][0][1], pb[1][0][2], pb[1][0][3]);
+        q.b1 = make_float4(pb[1][0][4], pb[1][0][5], pb[1][1][0], pb[1][1][1]);
+        q.c1 = make_float4(pb[1][1][2], pb[1][1][3], pb[1][1][4], pb[1][1][5]);
+        q.gref = make_uint4(gr[0][0], gr[0][1], gr[1][0], gr[1][1]);
+        q.cref = make_uint4(cr[0], cr[1], 0u, 0u);
+    }
+    return true;
+}
+
 // Occlusion tree: the merged binary tree with levels collapsed into nodes of
 // up to four children, keeping its leaves. An any-hit query only asks whether
 // some reached leaf holds an occluder. The slab test's arithmetic is monotone
@@ -762,6 +818,7 @@ bool buildHostScene(const prr_scene_desc *d, HostScene &S, std::string &err) {
     }
     if (S.nodes4.empty()) S.nodes4.emplace_back();
     if (!Quantiser{S}.run(err)) return false;
+    buildTwoLevel(S);  // no records (binary steps only) if a box is not the union of its children's
     if (S.tris.empty()) S.tris.push_back(make_float4(0, 0, 0, 0));  // keep buffers non-empty
     if (S.sph.empty()) {
         S.sph.push_back(make_float4(0, 0, 0, 0));

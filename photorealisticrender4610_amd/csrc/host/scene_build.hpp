@@ -56,6 +56,7 @@ struct HostScene {
     std::vector<float> objArea;
     // device staging
     std::vector<DNode> nodes;
+    std::vector<DNode2> nodes2;    // two levels per record, per binary node (empty: a box not the union of its children's)
     std::vector<DNode4> nodes4;    // occlusion tree
     std::vector<DNode4Q> nodes4q;  // the same tree quantised
     std::vector<float4> leavesq;   // its leaf records: exact box, then the leaf's triangles

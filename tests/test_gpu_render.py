@@ -187,16 +187,19 @@ def test_shadow_trees_agree(ctx, monkeypatch):
         assert np.array_equal(anyh, want), (var, mode)
 
 
-def test_speculative_extension_rays_agree(ctx, monkeypatch):
+@pytest.mark.parametrize("var", ["PRR_SPECULATE", "PRR_TWO_LEVEL"])
+def test_extension_ray_iterations_agree(ctx, monkeypatch, var):
     """Extension rays traced with speculative while-while iteration
     (PRR_SPECULATE=1: a lane that reaches a leaf sets it aside and steps on to
     a second one, each leaf re-checked against the closest hit when its turn
-    comes) give the same images and ray counts bit for bit as the plain
-    while-while iteration, on scenes with triangles and spheres, glass and
-    mirror chains, and the full 1M-triangle scene."""
+    comes) or on the two-level records (PRR_TWO_LEVEL=1: two binary levels per
+    128-byte line, the children's boxes rebuilt as unions of their pairs) give
+    the same images and ray counts bit for bit as the plain while-while
+    iteration, on scenes with triangles and spheres, glass and mirror chains,
+    and the full 1M-triangle scene."""
     from photorealisticrender4610_amd import abi, scenes
     import testscenes as TS
-    print("case speculative extension rays", flush=True)
+    print("case extension-ray iteration %s" % var, flush=True)
     cases = [(scenes.box_room(), 128, 96, 8, 8), (TS.mini_field(), 160, 120, 4, 12),
              (scenes.sphere_field(), 320, 180, 4, 12), (scenes.glass_torus(), 128, 128, 4, 12)]
     # every bounce in the per-bounce kernels (no path-completion launch), so
@@ -207,12 +210,14 @@ def test_speculative_extension_rays_agree(ctx, monkeypatch):
     try:
         for sc, w, h, spp, depth in cases:
             scene = c1.scene(sc)
+            if var == "PRR_TWO_LEVEL":
+                assert scene.info()["two_level"], sc.name
             img, st = c1.render(scene, w, h, spp, depth, seed=21)
             want.append((img.cpu().numpy(), st["rays_closest"], st["rays_any"]))
             del scene
     finally:
         c1.close()
-    monkeypatch.setenv("PRR_SPECULATE", "1")
+    monkeypatch.setenv(var, "1")
     c2 = abi.Context(0)
     try:
         for (sc, w, h, spp, depth), (img0, rc, ra) in zip(cases, want):
