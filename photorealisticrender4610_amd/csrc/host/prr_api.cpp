@@ -10,6 +10,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <cerrno>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -147,6 +148,12 @@ bool EnvMap::load(const std::string &path) {
     int H = 0, W = 0;
     if (!fgets(line, sizeof line, f) || sscanf(line, "-Y %d +X %d", &H, &W) != 2 || W <= 0 || H <= 0) {
         fprintf(stderr, "[EnvMap] Failed to load: %s (unsupported resolution line)\n", path.c_str());
+        return false;
+    }
+    // a header is not trusted with the allocation: at most 2^15 on a side and
+    // 2^28 pixels (a 32768 x 8192 sky), checked before any pixel is read
+    if (W > (1 << 15) || H > (1 << 15) || (int64_t) W * H > ((int64_t) 1 << 28)) {
+        fprintf(stderr, "[EnvMap] Failed to load: %s (%d x %d pixels is too large)\n", path.c_str(), W, H);
         return false;
     }
     std::vector<Vector3f> px((size_t) W * H);
@@ -325,11 +332,13 @@ MeshTriangle::MeshTriangle(const std::string &filename, Vector3f offset, Materia
             while (*q) {
                 while (*q == ' ' || *q == '\t') q++;
                 if (!*q || *q == '\n' || *q == '\r') break;
-                int v = (int) std::strtol(q, &q, 10);
-                if (v < 0) v = (int) verts.size() + v + 1;
-                if (v < 1 || v > (int) verts.size())
+                errno = 0;
+                long v = std::strtol(q, &q, 10);
+                const long n = (long) verts.size();
+                if (errno == ERANGE || v == 0 || v > n || v < -n)  // 1-based, or -1 = the last vertex
                     throw std::runtime_error("MeshTriangle: bad face index in " + filename);
-                idx.push_back(v - 1);
+                if (v < 0) v = n + v + 1;
+                idx.push_back((int) (v - 1));
                 while (*q && *q != ' ' && *q != '\t' && *q != '\n' && *q != '\r') q++;
             }
             for (size_t k = 2; k < idx.size(); k++) {

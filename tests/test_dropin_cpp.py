@@ -114,6 +114,29 @@ def test_obj_missing_file_raises(tmp_path):
     assert r.returncode == 1 and "cannot open" in r.stderr
 
 
+def test_obj_rejects_out_of_range_indices(tmp_path):
+    """Face indices out of range (0, past the end, before the first vertex,
+    or too large for a long) are errors, not reads outside the vertex list."""
+    for face in ("f 1 2 4", "f 0 1 2", "f -4 1 2", "f 1 2 99999999999999999999999", "f 1 2 -9223372036854775808"):
+        obj = tmp_path / "bad.obj"
+        obj.write_text("v 0 0 0\nv 1 0 0\nv 0 1 0\n" + face + "\n")
+        r = _run("obj", obj, 0, 0, 0, check=False)
+        assert r.returncode == 1 and "bad face index" in r.stderr, face
+    ok = tmp_path / "ok.obj"
+    ok.write_text("v 0 0 0\nv 1 0 0\nv 0 1 0\nf -3 -2 -1\n")
+    assert _run("obj", ok, 0, 0, 0).stdout.split()[1] == "1"
+
+
+def test_envmap_rejects_oversized_headers(tmp_path):
+    """A resolution line larger than 2^15 on a side (or 2^28 pixels) is refused
+    before any allocation."""
+    for res in (b"-Y 2000000000 +X 2000000000", b"-Y 32769 +X 4", b"-Y 32768 +X 16384"):
+        bad = tmp_path / "big.hdr"
+        bad.write_bytes(b"#?RADIANCE\nFORMAT=32-bit_rle_rgbe\n\n" + res + b"\n\x00\x00\x00\x00")
+        r = _run("hdr", bad, tmp_path / "o.f32", check=False)
+        assert r.returncode == 1 and "too large" in r.stderr, res
+
+
 def _png_pixels(path):
     data = open(path, "rb").read()
     assert data[:8] == b"\x89PNG\r\n\x1a\n"
