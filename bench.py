@@ -83,7 +83,7 @@ def reference_baseline(exe, sc, w, h, depth, target_s=12.0):
         with tempfile.TemporaryDirectory() as td:
             fin, fout = os.path.join(td, "in.bin"), os.path.join(td, "out.bin")
             write_bundle(fin, arr)
-            env = dict(os.environ, OMP_NUM_THREADS=str(threads))
+            env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="spread")
             subprocess.run([binary, "bench", fin, fout], check=True, env=env, timeout=600,
                            stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
             out = read_bundle(fout)
@@ -96,7 +96,7 @@ def reference_baseline(exe, sc, w, h, depth, target_s=12.0):
         rows = samples // (w * spp)
         _, _, closest, anyq, csamples, _ = run(exe, 0, rows, 0.0)
         rays = (closest + anyq) * samples / max(csamples, 1)
-        what = ("oracle/_ref/ref_bench_lto: -O3 -ffast-math -march=x86-64-v3 -flto, OpenMP; rows 0..%d of the "
+        what = ("oracle/_ref/ref_bench_lto: -O3 -ffast-math -march=x86-64-v3 -flto, OpenMP, OMP_PROC_BIND=spread; rows 0..%d of the "
                 "%dx%d image at %d spp: %d samples in %.1f s; queries per sample (%.3f closest, %.3f any-hit) counted "
                 "by oracle/_ref/ref_bench on the same rows" % (rows, w, h, spp, samples, sec, closest / csamples,
                                                                anyq / csamples))
