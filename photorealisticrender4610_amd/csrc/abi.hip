@@ -973,7 +973,8 @@ int prr_render(prr_context *c, const prr_scene *s, const prr_render_params *p, f
                prr_stats *stats) {
     if (!c || !s || !p || !rgb) return fail(c, PRR_ERR_INVALID, "NULL argument");
     if (p->width <= 0 || p->height <= 0 || p->spp <= 0 || p->max_depth <= 0 || p->max_depth > 64 ||
-        p->sample_offset < 0 || p->min_spp < 0)
+        p->sample_offset < 0 || p->min_spp < 0 ||
+        (int64_t) p->sample_offset + p->spp > INT32_MAX)
         return fail(c, PRR_ERR_INVALID, "bad render parameters");
     const int64_t pixels = (int64_t) p->width * p->height;
     if (pixels > c->maxSlots) return fail(c, PRR_ERR_UNSUPPORTED, "image larger than the context's path slots");

@@ -311,7 +311,7 @@ int prr_cache_render(prr_context *c, const prr_scene *s, prr_cache *k, const prr
                      const prr_cache_render_params *cp, float *rgb, prr_cache_stats *cs) {
     if (!c || !s || !k || !p || !cp || !rgb) return fail(c, PRR_ERR_INVALID, "NULL argument");
     if (p->width <= 0 || p->height <= 0 || p->spp <= 0 || p->max_depth <= 0 || p->max_depth > 64 ||
-        p->sample_offset < 0)
+        p->sample_offset < 0 || (int64_t) p->sample_offset + p->spp > INT32_MAX)
         return fail(c, PRR_ERR_INVALID, "bad render parameters");
     if (p->variance_threshold > 0.f && p->min_spp < p->spp)
         return fail(c, PRR_ERR_UNSUPPORTED, "adaptive sampling is not available with the cache");

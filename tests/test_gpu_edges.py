@@ -147,6 +147,8 @@ def test_bad_parameters_raise(ctx):
     for args in ((0, 8, 1, 4), (8, 0, 1, 4), (8, 8, 0, 4), (8, 8, 1, 0), (8, 8, 1, 65)):
         with pytest.raises(abi.PrrError):
             ctx.render(scene, *args)
+    with pytest.raises(abi.PrrError):  # sample indices past 2^31 - 1
+        ctx.render(scene, 8, 8, 16, 4, sample_offset=2 ** 31 - 10)
     b = scenes.SceneBuilder()
     b.material()
     with pytest.raises(Exception):
