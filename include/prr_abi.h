@@ -215,8 +215,8 @@ typedef struct prr_scene_info {
     int64_t device_bytes;      /* HBM held by the scene */
     int32_t num_wide_nodes;    /* nodes of the four-wide occlusion tree (shadow rays, prr_trace_any) */
     int32_t wide_levels_kept;  /* binary levels left uncollapsed there (a child box outside its parent's) */
-    int32_t two_level;         /* 1: two-level records of the binary tree built (every box the union of its
-                                  children's; PRR_TWO_LEVEL extension-ray steps), 0: not built */
+    int32_t two_level;         /* 1: two-level records of the binary tree on the device (a context created
+                                  with PRR_TWO_LEVEL=1, and every box the union of its children's), 0: not */
     int32_t reserved;
 } prr_scene_info;
 

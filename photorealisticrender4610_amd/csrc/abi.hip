@@ -473,7 +473,7 @@ int prr_scene_create(prr_context *c, const prr_scene_desc *d, prr_scene **out) {
 #define UP(dst, src)                                                               \
     if (e == hipSuccess) e = upload(s, (std::remove_const<std::remove_reference<decltype(*dst)>::type>::type **) &dst, src, c->stream)
     UP(v.nodes, h.nodes);
-    if (!h.nodes2.empty()) UP(v.nodes2, h.nodes2);
+    if (c->twoLevel && !h.nodes2.empty()) UP(v.nodes2, h.nodes2);  // read only with PRR_TWO_LEVEL
     UP(v.nodes4, h.nodes4);
     UP(v.nodes4q, h.nodes4q);
     UP(v.leavesq, h.leavesq);
@@ -531,6 +531,7 @@ int prr_scene_create(prr_context *c, const prr_scene_desc *d, prr_scene **out) {
     }
     v.num_points = h.numPoints;
     v.cam = h.cam;
+    if (!v.nodes2) std::vector<DNode2>().swap(s->hs.nodes2);  // host copy not kept when not uploaded
     s->agx = agxConst();
     *out = s;
     return PRR_OK;
@@ -563,7 +564,7 @@ int prr_scene_get_info(const prr_scene *s, prr_scene_info *o) {
     o->device_bytes = s->bytes;
     o->num_wide_nodes = (int32_t) h.nodes4.size();
     o->wide_levels_kept = h.wideLevelsKept;
-    o->two_level = h.nodes2.empty() ? 0 : 1;
+    o->two_level = s->view.nodes2 ? 1 : 0;
     o->reserved = 0;
     return PRR_OK;
 }
