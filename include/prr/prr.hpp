@@ -324,6 +324,9 @@ struct Camera {
 
 // Medium (Volume.hpp:7-66): carried for source compatibility; the reference's
 // castRay does not use it and neither does the device integrator.
+// Volume.hpp:7-66. As in the reference, Scene::medium is carried and not used
+// by Integrator::castRay; intersect and the phase functions run on the device
+// (prr_medium_intersect, prr_phase_hg, prr_phase_hg_sample).
 struct Medium {
     float sigma_a = 0.0f;
     float sigma_s = 0.0f;
@@ -332,7 +335,30 @@ struct Medium {
     bool bounded = false;
     Bounds3 bounds;
     bool active() const { return sigma_t > 1e-6f; }
+    // Volume.hpp:17-44: entry and exit t of the ray in the medium's box (0 and
+    // infinity when unbounded); false, outputs untouched, when it misses.
+    bool intersect(const Ray &ray, float &t_enter, float &t_exit) const;
+    static Medium dust() {  // Volume.hpp:46-53
+        Medium m;
+        m.sigma_s = 0.05f;
+        m.sigma_a = 0.005f;
+        m.sigma_t = m.sigma_s + m.sigma_a;
+        m.g = 0.6f;
+        return m;
+    }
+    static Medium fog() {  // Volume.hpp:55-62
+        Medium m;
+        m.sigma_s = 0.1f;
+        m.sigma_a = 0.01f;
+        m.sigma_t = m.sigma_s + m.sigma_a;
+        m.g = 0.0f;
+        return m;
+    }
 };
+// Volume.cpp:3-24: Henyey-Greenstein phase function, and a direction drawn
+// from it about wo with two draws from the calling thread's generator.
+float phaseHG(float cos_theta, float g);
+Vector3f samplePhaseHG(const Vector3f &wo, float g);
 
 struct EnvMap {
     std::vector<Vector3f> pixels;  // linear RGB, row 0 = zenith

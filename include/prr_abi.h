@@ -289,6 +289,22 @@ int prr_bsdf_eval(prr_context* ctx, int64_t n, const float* wi, const float* wo,
 int prr_bsdf_sample(prr_context* ctx, int64_t n, const float* wo, const float* normal,
                     const float* params, const float* u, float* wi, float* f, float* pdf,
                     int32_t* lobe);
+/* Participating media (Volume.hpp:7-66, Volume.cpp:3-24; Scene::medium is
+ * not used by the reference's castRay, so these are batch functions only).
+ * Medium::intersect (Volume.hpp:17-44) per ray against its own medium:
+ * bounds 6 floats per ray (pMin xyz, pMax xyz as Bounds3 holds them),
+ * bounded 0: the ray enters at 0 and leaves at infinity. hit[i] = the
+ * function's result; t_enter / t_exit are written only where it is 1 (the
+ * reference leaves its outputs untouched otherwise). */
+int prr_medium_intersect(prr_context* ctx, int64_t n, const float* org, const float* dir,
+                         const float* bounds, const int32_t* bounded, float* t_enter, float* t_exit,
+                         uint8_t* hit);
+/* phaseHG(cos_theta, g) (Volume.cpp:3-6). */
+int prr_phase_hg(prr_context* ctx, int64_t n, const float* cos_theta, const float* g, float* out);
+/* samplePhaseHG(wo, g) (Volume.cpp:8-24) with explicit uniforms, 2 per entry
+ * (cos theta, then phi); wi is a unit direction (3 floats). */
+int prr_phase_hg_sample(prr_context* ctx, int64_t n, const float* wo, const float* g, const float* u2,
+                        float* wi);
 /* Scene::sampleLight (Scene.cpp:37-58: emitter picked by area,
  * MeshTriangle -> BVHAccel::Sample's sqrt(u) leaf walk, BVH.cpp:478-514, ->
  * Triangle::Sample, Triangle.hpp:98-103; Sphere::Sample, Sphere.hpp:54-63)

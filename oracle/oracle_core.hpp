@@ -186,6 +186,57 @@ inline void buildTBN(const V3 &N, V3 &T, V3 &B) {
     }
     B = cross(N, T);
 }
+// ---------------------------------------------------------------------------
+// Participating media (Volume.hpp:7-66, Volume.cpp:3-24); the reference's
+// integrator does not call them.
+// ---------------------------------------------------------------------------
+// Medium::intersect (Volume.hpp:17-44): std::max / std::min as written (the
+// first argument wins a comparison with NaN), dirIsNeg = direction > 0. The
+// bounds and the origin are read through Vector3f's const operator[], which
+// returns double (Vector.hpp:67,78-80), so each plane time is formed in double
+// and rounded to float once.
+inline bool mediumIntersect(bool bounded, const float *lo, const float *hi, const V3 &o, const V3 &d, float &tEnter,
+                            float &tExit) {
+    if (!bounded) {
+        tEnter = 0.0f;
+        tExit = std::numeric_limits<float>::infinity();
+        return true;
+    }
+    const V3 inv(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+    float te = -std::numeric_limits<float>::max(), tx = std::numeric_limits<float>::max();
+    for (int i = 0; i < 3; i++) {
+        float tmin = (float) (((double) lo[i] - (double) o[i]) * (double) inv[i]);
+        float tmax = (float) (((double) hi[i] - (double) o[i]) * (double) inv[i]);
+        if (!(d[i] > 0)) std::swap(tmin, tmax);
+        te = std::max(tmin, te);
+        tx = std::min(tmax, tx);
+    }
+    if (te > tx || tx < 0.0f) return false;
+    tEnter = te;
+    tExit = tx;
+    return true;
+}
+// phaseHG (Volume.cpp:3-6), in float (M_PI is a float literal, global.hpp:9-10).
+inline float phaseHG(float cosTheta, float g) {
+    const float denom = 1.0f + g * g - 2.0f * g * cosTheta;
+    return (1.0f - g * g) / (4.0f * kPi * std::pow(denom, 1.5f));
+}
+// samplePhaseHG (Volume.cpp:8-24) with its two draws u1 (cos theta), u2 (phi).
+inline V3 samplePhaseHG(const V3 &wo, float g, float u1, float u2) {
+    float cosTheta;
+    if (std::abs(g) < 1e-3f) {
+        cosTheta = 1.0f - 2.0f * u1;
+    } else {
+        const float sqr = (1.0f - g * g) / (1.0f - g + 2.0f * g * u1);
+        cosTheta = (1.0f + g * g - sqr * sqr) / (2.0f * g);
+    }
+    const float sinTheta = std::sqrt(std::max(0.0f, 1.0f - cosTheta * cosTheta));
+    const float phi = 2.0f * kPi * u2;
+    V3 T, B;
+    buildTBN(wo, T, B);
+    return normalize(sinTheta * std::cos(phi) * T + sinTheta * std::sin(phi) * B + cosTheta * wo);
+}
+
 // BRDFUtils.hpp:35-49
 inline V3 toWorld(const V3 &l, const V3 &N) {
     V3 T, B;

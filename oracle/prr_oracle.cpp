@@ -315,6 +315,27 @@ extern "C" void orc_bsdf_sample(int64_t n, const float *wo, const float *nrm, co
     }
 }
 
+extern "C" void orc_medium(int64_t n, const float *org, const float *dir, const float *bounds,
+                           const int32_t *bounded, float *t_enter, float *t_exit, uint8_t *hit) {
+    for (int64_t i = 0; i < n; i++) {
+        float te = t_enter[i], tx = t_exit[i];
+        hit[i] = mediumIntersect(bounded[i] != 0, bounds + 6 * i, bounds + 6 * i + 3, v3(org + 3 * i),
+                                 v3(dir + 3 * i), te, tx)
+                     ? 1
+                     : 0;
+        t_enter[i] = te;
+        t_exit[i] = tx;
+    }
+}
+
+extern "C" void orc_phase_hg(int64_t n, const float *cos_theta, const float *g, float *out) {
+    for (int64_t i = 0; i < n; i++) out[i] = phaseHG(cos_theta[i], g[i]);
+}
+
+extern "C" void orc_phase_hg_sample(int64_t n, const float *wo, const float *g, const float *u2, float *wi) {
+    for (int64_t i = 0; i < n; i++) put3(wi + 3 * i, samplePhaseHG(v3(wo + 3 * i), g[i], u2[2 * i], u2[2 * i + 1]));
+}
+
 extern "C" void orc_ggx_terms(int64_t n, const float *in4, float *out6) {
     for (int64_t i = 0; i < n; i++) {
         float NdotH = in4[4 * i], NdotWo = in4[4 * i + 1], NdotWi = in4[4 * i + 2], alpha = in4[4 * i + 3];

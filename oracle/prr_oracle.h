@@ -65,6 +65,12 @@ void orc_bsdf_eval(int64_t n, const float* wi, const float* wo, const float* nrm
                    const float* params, float* f, float* pdf);
 void orc_bsdf_sample(int64_t n, const float* wo, const float* nrm, const float* params,
                      const float* u3, float* wi, float* f, float* pdf, int32_t* lobe);
+/* Medium::intersect (bounds: min xyz, max xyz per entry; t_enter / t_exit kept
+ * on a miss), phaseHG, samplePhaseHG with two draws per entry */
+void orc_medium(int64_t n, const float* org, const float* dir, const float* bounds, const int32_t* bounded,
+                float* t_enter, float* t_exit, uint8_t* hit);
+void orc_phase_hg(int64_t n, const float* cos_theta, const float* g, float* out);
+void orc_phase_hg_sample(int64_t n, const float* wo, const float* g, const float* u2, float* wi);
 void orc_ggx_terms(int64_t n, const float* in4, float* out6); /* NdotH,NdotWo,NdotWi,alpha ->
                                                                  D,Lambda(wo),G1(wo),G2,pdfVNDF(WoDotH=NdotWo),Fschlick(F0=0.04) */
 

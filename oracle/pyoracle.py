@@ -169,6 +169,9 @@ def lib(fast=False):
     L.orc_env_eval.argtypes = [C.c_int, C.c_int, vp, C.c_int64, vp, vp, vp]
     L.orc_camera_rays.argtypes = [C.POINTER(CameraDesc), C.c_int, C.c_int, C.c_int64, vp, vp, vp, vp]
     L.orc_tonemap.argtypes = [C.c_int, C.c_int64, C.c_float, vp, vp, vp]
+    L.orc_medium.argtypes = [C.c_int64, vp, vp, vp, vp, vp, vp, vp]
+    L.orc_phase_hg.argtypes = [C.c_int64, vp, vp, vp]
+    L.orc_phase_hg_sample.argtypes = [C.c_int64, vp, vp, vp, vp]
     L.orc_castray_splitmix.argtypes = [vp, C.c_int, C.c_int, C.c_int64, vp, vp, C.c_int, C.c_float, vp]
     L.orc_castray_pcg.argtypes = [vp, C.c_int64, vp, vp, C.c_uint64, C.c_int, C.c_float, vp]
     L.orc_render.argtypes = [vp, C.POINTER(RenderParams), C.c_int, C.c_int, vp, C.c_int, vp, vp, vp, vp]
@@ -453,6 +456,35 @@ def bsdf_sample(wo, n, params, u3):
     lobe = np.zeros(k, np.int32)
     L.orc_bsdf_sample(k, _p(wo), _p(n), _p(params), _p(u3), _p(wi), _p(f), _p(pdf), _p(lobe))
     return wi, f, pdf, lobe
+
+
+def medium_intersect(org, d, bounds, bounded, fill=-7.0):
+    """Medium::intersect per entry: (t_enter, t_exit, hit); t values stay `fill` on a miss."""
+    L = lib()
+    org, d, bounds = _c(org), _c(d), _c(bounds)
+    bounded = np.ascontiguousarray(bounded, np.int32)
+    k = org.shape[0]
+    te = np.full(k, fill, np.float32)
+    tx = np.full(k, fill, np.float32)
+    hit = np.zeros(k, np.uint8)
+    L.orc_medium(k, _p(org), _p(d), _p(bounds), _p(bounded), _p(te), _p(tx), _p(hit))
+    return te, tx, hit
+
+
+def phase_hg(cos, g):
+    L = lib()
+    cos, g = _c(cos), _c(g)
+    out = np.zeros(cos.shape[0], np.float32)
+    L.orc_phase_hg(cos.shape[0], _p(cos), _p(g), _p(out))
+    return out
+
+
+def phase_hg_sample(wo, g, u2):
+    L = lib()
+    wo, g, u2 = _c(wo), _c(g), _c(u2)
+    wi = np.zeros((wo.shape[0], 3), np.float32)
+    L.orc_phase_hg_sample(wo.shape[0], _p(wo), _p(g), _p(u2), _p(wi))
+    return wi
 
 
 def ggx_terms(in4):

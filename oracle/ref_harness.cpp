@@ -405,6 +405,49 @@ static int modeBsdf(const Bundle &b, const char *out) {
     return 0;
 }
 
+// Medium::intersect (Volume.hpp:17-44), phaseHG and samplePhaseHG
+// (Volume.cpp:3-24): per entry a ray against a box (bounded or not), the HG
+// phase function at (cos, g), and a phase sample about wo with its two draws.
+static int modeVolume(const Bundle &b, const char *out) {
+    const auto &O = b.at("org"), &D = b.at("dir"), &B = b.at("bounds"), &BD = b.at("bounded");
+    const auto &C = b.at("cos"), &G = b.at("g"), &WO = b.at("wo"), &S = b.at("seeds");
+    const size_t n = S.n<uint32_t>();
+    std::vector<float> te(n), tx(n), phase(n), swi(3 * n), draws(2 * n);
+    std::vector<int32_t> hit(n);
+    for (size_t i = 0; i < n; i++) {
+        const float *o = O.as<float>() + 3 * i, *d = D.as<float>() + 3 * i, *bb = B.as<float>() + 6 * i;
+        Medium m;
+        m.bounded = BD.as<int32_t>()[i] != 0;
+        m.bounds = Bounds3(Vector3f(bb[0], bb[1], bb[2]), Vector3f(bb[3], bb[4], bb[5]));
+        Ray r(Vector3f(o[0], o[1], o[2]), Vector3f(d[0], d[1], d[2]));
+        float a = -7.f, e = -7.f;  // untouched on a miss
+        hit[i] = m.intersect(r, a, e) ? 1 : 0;
+        te[i] = a;
+        tx[i] = e;
+        phase[i] = phaseHG(C.as<float>()[i], G.as<float>()[i]);
+        const Vector3f wo(WO.as<float>()[3 * i], WO.as<float>()[3 * i + 1], WO.as<float>()[3 * i + 2]);
+        const uint32_t seed = S.as<uint32_t>()[i];
+        onFreshThread(seed, [&]() {
+            const Vector3f w = samplePhaseHG(wo, G.as<float>()[i]);
+            swi[3 * i] = w.x;
+            swi[3 * i + 1] = w.y;
+            swi[3 * i + 2] = w.z;
+        });
+        onFreshThread(seed, [&]() {
+            draws[2 * i] = get_random_float();
+            draws[2 * i + 1] = get_random_float();
+        });
+    }
+    Writer w(out);
+    w.f32("t_enter", te);
+    w.f32("t_exit", tx);
+    w.i32("hit", hit);
+    w.f32("phase", phase);
+    w.f32("sample_wi", swi);
+    w.f32("draws", draws);
+    return 0;
+}
+
 static int modeEnv(const Bundle &b, const char *out) {
     const int32_t *wh = b.at("env_wh").as<int32_t>();
     const float *px = b.at("env_pixels").as<float>();
@@ -837,6 +880,7 @@ int main(int argc, char **argv) {
     if (m == "image") return modeImage(b, argv[3]);
     if (m == "photon") return modePhoton(b, argv[3]);
     if (m == "welford") return modeWelford(b, argv[3]);
+    if (m == "volume") return modeVolume(b, argv[3]);
     if (m == "hdr") return modeHdr(b, argv[3]);
     if (m == "gltf") return modeGltf(b, argv[3]);
 #ifdef PRR_REF_BENCH

@@ -163,6 +163,9 @@ SIGNATURES = {
     "prr_trace_closest_noalloc": [vp, vp, i64, vp, vp, vp, vp, vp, vp],
     "prr_trace_any": [vp, vp, i64, vp, vp, vp, vp],
     "prr_bsdf_eval": [vp, i64, vp, vp, vp, vp, vp, vp],
+    "prr_medium_intersect": [vp, i64, vp, vp, vp, vp, vp, vp, vp],
+    "prr_phase_hg": [vp, i64, vp, vp, vp],
+    "prr_phase_hg_sample": [vp, i64, vp, vp, vp, vp],
     "prr_bsdf_sample": [vp, i64, vp, vp, vp, vp, vp, vp, vp, vp],
     "prr_sample_light": [vp, vp, i64, vp, vp, vp, vp, vp],
     "prr_env_sample": [vp, vp, i64, vp, vp, vp],
@@ -421,6 +424,37 @@ class Context:
         self._bind_stream()
         self._done(lib().prr_trace_any(self.h, scene.h, n, _ptr(org), _ptr(d), _ptr(tmax), _ptr(hit)))
         return hit
+
+    def medium_intersect(self, org, d, bounds, bounded, fill=-7.0):
+        """Medium::intersect per ray: (t_enter, t_exit, hit); t values stay
+        `fill` where hit is 0."""
+        T = self.torch
+        org, d, bounds = self._f32(org), self._f32(d), self._f32(bounds)
+        bounded = T.as_tensor(np.ascontiguousarray(bounded, np.int32), device=self.device).contiguous()
+        n = org.shape[0]
+        te = T.full((n,), fill, dtype=T.float32, device=self.device)
+        tx = T.full((n,), fill, dtype=T.float32, device=self.device)
+        hit = T.empty(n, dtype=T.uint8, device=self.device)
+        self._bind_stream()
+        self._done(lib().prr_medium_intersect(self.h, n, _ptr(org), _ptr(d), _ptr(bounds), _ptr(bounded), _ptr(te),
+                                              _ptr(tx), _ptr(hit)))
+        return te, tx, hit
+
+    def phase_hg(self, cos, g):
+        T = self.torch
+        cos, g = self._f32(cos), self._f32(g)
+        out = T.empty(cos.shape[0], dtype=T.float32, device=self.device)
+        self._bind_stream()
+        self._done(lib().prr_phase_hg(self.h, cos.shape[0], _ptr(cos), _ptr(g), _ptr(out)))
+        return out
+
+    def phase_hg_sample(self, wo, g, u2):
+        T = self.torch
+        wo, g, u2 = self._f32(wo), self._f32(g), self._f32(u2)
+        wi = T.empty((wo.shape[0], 3), dtype=T.float32, device=self.device)
+        self._bind_stream()
+        self._done(lib().prr_phase_hg_sample(self.h, wo.shape[0], _ptr(wo), _ptr(g), _ptr(u2), _ptr(wi)))
+        return wi
 
     def bsdf_eval(self, wi, wo, nrm, params):
         T = self.torch

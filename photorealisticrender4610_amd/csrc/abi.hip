@@ -630,6 +630,32 @@ int prr_bsdf_eval(prr_context *c, int64_t n, const float *wi, const float *wo, c
     return PRR_OK;
 }
 
+int prr_medium_intersect(prr_context *c, int64_t n, const float *org, const float *dir, const float *bounds,
+                         const int32_t *bounded, float *t_enter, float *t_exit, uint8_t *hit) {
+    if (!c) return fail(nullptr, PRR_ERR_INVALID, "context is NULL");
+    if (n < 0 || (n > 0 && (!org || !dir || !bounds || !bounded || !t_enter || !t_exit || !hit)))
+        return fail(c, PRR_ERR_INVALID, "bad arrays");
+    HIPCK(c, hipSetDevice(c->device));
+    HIPCK(c, launchMedium(n, org, dir, bounds, bounded, t_enter, t_exit, hit, c->stream));
+    return PRR_OK;
+}
+
+int prr_phase_hg(prr_context *c, int64_t n, const float *cos_theta, const float *g, float *out) {
+    if (!c) return fail(nullptr, PRR_ERR_INVALID, "context is NULL");
+    if (n < 0 || (n > 0 && (!cos_theta || !g || !out))) return fail(c, PRR_ERR_INVALID, "bad arrays");
+    HIPCK(c, hipSetDevice(c->device));
+    HIPCK(c, launchPhaseHG(n, cos_theta, g, out, c->stream));
+    return PRR_OK;
+}
+
+int prr_phase_hg_sample(prr_context *c, int64_t n, const float *wo, const float *g, const float *u2, float *wi) {
+    if (!c) return fail(nullptr, PRR_ERR_INVALID, "context is NULL");
+    if (n < 0 || (n > 0 && (!wo || !g || !u2 || !wi))) return fail(c, PRR_ERR_INVALID, "bad arrays");
+    HIPCK(c, hipSetDevice(c->device));
+    HIPCK(c, launchPhaseHGSample(n, wo, g, u2, wi, c->stream));
+    return PRR_OK;
+}
+
 int prr_bsdf_sample(prr_context *c, int64_t n, const float *wo, const float *nrm, const float *params, const float *u,
                     float *wi, float *f, float *pdf, int32_t *lobe) {
     if (!c) return fail(nullptr, PRR_ERR_INVALID, "context is NULL");

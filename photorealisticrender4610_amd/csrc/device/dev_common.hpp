@@ -271,6 +271,48 @@ struct SceneView {
 struct Frame {
     V3 T, B, N;
 };
+
+// ---------------------------------------------------------------------------
+// Participating media (Volume.hpp:7-66, Volume.cpp:3-24). The reference's
+// integrator does not call them; they are here as batch entry points.
+// ---------------------------------------------------------------------------
+// Medium::intersect (Volume.hpp:17-44). Vector3f's const operator[] returns
+// double (Vector.hpp:67,78-80), so each plane time is formed in double from
+// the float bounds, origin and 1/d and rounded to float once; std::max(a, b)
+// and std::min(a, b) return a when the comparison fails (NaN); dirIsNeg
+// holds direction > 0.
+PRR_DEV bool mediumIntersect(bool bounded, const float *lo, const float *hi, const V3 &o, const V3 &d, float &tEnter,
+                             float &tExit) {
+    if (!bounded) {
+        tEnter = 0.0f;
+        tExit = __builtin_inff();
+        return true;
+    }
+    const float ov[3] = {o.x, o.y, o.z}, dv[3] = {d.x, d.y, d.z};
+    float te = -3.40282347e38f, tx = 3.40282347e38f;
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        const float inv = 1.0f / dv[i];
+        float t0 = (float) (((double) lo[i] - (double) ov[i]) * (double) inv);
+        float t1 = (float) (((double) hi[i] - (double) ov[i]) * (double) inv);
+        if (!(dv[i] > 0.f)) {
+            const float t = t0;
+            t0 = t1;
+            t1 = t;
+        }
+        te = t0 < te ? te : t0;  // std::max(t_min, tEnter)
+        tx = tx < t1 ? tx : t1;  // std::min(t_max, tExit)
+    }
+    if (te > tx || tx < 0.0f) return false;
+    tEnter = te;
+    tExit = tx;
+    return true;
+}
+// phaseHG (Volume.cpp:3-6), in float (M_PI is a float literal, global.hpp:9-10).
+PRR_DEV float phaseHG(float cosTheta, float g) {
+    const float denom = 1.0f + g * g - 2.0f * g * cosTheta;
+    return (1.0f - g * g) / (4.0f * kPi * powf(denom, 1.5f));
+}
 // buildTBN (BRDFUtils.hpp:15-30); the reference rebuilds it in every
 // toLocal/toWorld call, it is built once here (identical values).
 PRR_DEV Frame makeFrame(const V3 &N) {
@@ -285,6 +327,21 @@ PRR_DEV Frame makeFrame(const V3 &N) {
     }
     f.B = cross(N, f.T);
     return f;
+}
+// samplePhaseHG (Volume.cpp:8-24) with its two draws: u1 for cos theta, u2
+// for phi; the frame is buildTBN(wo).
+PRR_DEV V3 samplePhaseHG(const V3 &wo, float g, float u1, float u2) {
+    float cosTheta;
+    if (fabsf(g) < 1e-3f) {
+        cosTheta = 1.0f - 2.0f * u1;
+    } else {
+        const float sqr = (1.0f - g * g) / (1.0f - g + 2.0f * g * u1);
+        cosTheta = (1.0f + g * g - sqr * sqr) / (2.0f * g);
+    }
+    const float sinTheta = sqrtf(fmaxf(0.0f, 1.0f - cosTheta * cosTheta));
+    const float phi = 2.0f * kPi * u2;
+    const Frame f = makeFrame(wo);
+    return normalize(sinTheta * cosf(phi) * f.T + sinTheta * sinf(phi) * f.B + cosTheta * wo);
 }
 
 // ---------------------------------------------------------------------------

@@ -51,6 +51,29 @@ __global__ void kBsdfEval(int64_t n, const float *wi, const float *wo, const flo
     pdf[i] = pv;
 }
 
+__global__ void kMedium(int64_t n, const float *org, const float *dir, const float *bounds, const int32_t *bounded,
+                        float *tEnter, float *tExit, uint8_t *hit) {
+    const int64_t i = (int64_t) blockIdx.x * kBlk + threadIdx.x;
+    if (i >= n) return;
+    float te = 0.f, tx = 0.f;
+    const bool h = mediumIntersect(bounded[i] != 0, bounds + 6 * i, bounds + 6 * i + 3, ldv(org, i), ldv(dir, i), te, tx);
+    hit[i] = h ? 1 : 0;
+    if (h) {  // the reference leaves its outputs untouched on a miss
+        tEnter[i] = te;
+        tExit[i] = tx;
+    }
+}
+
+__global__ void kPhaseHG(int64_t n, const float *cosTheta, const float *g, float *out) {
+    const int64_t i = (int64_t) blockIdx.x * kBlk + threadIdx.x;
+    if (i < n) out[i] = phaseHG(cosTheta[i], g[i]);
+}
+
+__global__ void kPhaseHGSample(int64_t n, const float *wo, const float *g, const float *u, float *wi) {
+    const int64_t i = (int64_t) blockIdx.x * kBlk + threadIdx.x;
+    if (i < n) stv(wi, i, samplePhaseHG(ldv(wo, i), g[i], u[2 * i], u[2 * i + 1]));
+}
+
 __global__ void kBsdfSample(int64_t n, const float *wo, const float *nrm, const float *params, const float *u,
                             float *wi, float *f, float *pdf, int32_t *lobe) {
     const int64_t i = (int64_t) blockIdx.x * kBlk + threadIdx.x;
@@ -133,6 +156,22 @@ hipError_t launchBsdfSample(int64_t n, const float *wo, const float *nrm, const 
                             float *wi, float *f, float *pdf, int32_t *lobe, hipStream_t st) {
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(kBsdfSample, grid(n), dim3(kBlk), 0, st, n, wo, nrm, params, u, wi, f, pdf, lobe);
+    return hipGetLastError();
+}
+hipError_t launchMedium(int64_t n, const float *org, const float *dir, const float *bounds, const int32_t *bounded,
+                        float *tEnter, float *tExit, uint8_t *hit, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(kMedium, grid(n), dim3(kBlk), 0, st, n, org, dir, bounds, bounded, tEnter, tExit, hit);
+    return hipGetLastError();
+}
+hipError_t launchPhaseHG(int64_t n, const float *cosTheta, const float *g, float *out, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(kPhaseHG, grid(n), dim3(kBlk), 0, st, n, cosTheta, g, out);
+    return hipGetLastError();
+}
+hipError_t launchPhaseHGSample(int64_t n, const float *wo, const float *g, const float *u, float *wi, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(kPhaseHGSample, grid(n), dim3(kBlk), 0, st, n, wo, g, u, wi);
     return hipGetLastError();
 }
 hipError_t launchEnvSample(const SceneView &s, int64_t n, const float *u2, float *dir, float *pdf, hipStream_t st) {

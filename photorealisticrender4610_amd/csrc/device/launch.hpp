@@ -27,6 +27,10 @@ hipError_t launchTraceClosestNoAlloc(const SceneView &s, int64_t n, const float 
 // k_misc.hip
 hipError_t launchSampleLight(const SceneView &s, int64_t n, const float *u, float *pos, float *nrm, int32_t *emitter,
                             float *pdf, hipStream_t st);
+hipError_t launchMedium(int64_t n, const float *org, const float *dir, const float *bounds, const int32_t *bounded,
+                        float *tEnter, float *tExit, uint8_t *hit, hipStream_t st);
+hipError_t launchPhaseHG(int64_t n, const float *cosTheta, const float *g, float *out, hipStream_t st);
+hipError_t launchPhaseHGSample(int64_t n, const float *wo, const float *g, const float *u, float *wi, hipStream_t st);
 hipError_t launchBsdfEval(int64_t n, const float *wi, const float *wo, const float *nrm, const float *params,
                           float *f, float *pdf, hipStream_t st);
 hipError_t launchBsdfSample(int64_t n, const float *wo, const float *nrm, const float *params, const float *u,

@@ -604,6 +604,71 @@ static float hostRandomFloat() {
     return (float) (z >> 40) * 0x1p-24f;
 }
 
+// Device scratch of the media functions (no scene to hold it), taken under
+// its lock; kept until the process ends (never freed during static
+// destruction, when the HIP runtime may already be gone).
+static prr_detail::DevBuf &mediaBuf(std::unique_lock<std::mutex> &lk) {
+    static std::mutex mu;
+    static prr_detail::DevBuf *buf = new prr_detail::DevBuf;
+    lk = std::unique_lock<std::mutex>(mu);
+    return *buf;
+}
+
+// Medium::intersect (Volume.hpp:17-44) through prr_medium_intersect.
+bool Medium::intersect(const Ray &ray, float &t_enter, float &t_exit) const {
+    prr_context *ctx = context();
+    std::unique_lock<std::mutex> lk;
+    prr_detail::DevBuf &buf = mediaBuf(lk);
+    float h[16] = {ray.origin.x,    ray.origin.y,    ray.origin.z,    ray.direction.x, ray.direction.y,
+                   ray.direction.z, bounds.pMin.x,   bounds.pMin.y,   bounds.pMin.z,   bounds.pMax.x,
+                   bounds.pMax.y,   bounds.pMax.z,   0.f,             t_enter,         t_exit,  0.f};
+    const int32_t bd = bounded ? 1 : 0;
+    std::memcpy(&h[12], &bd, 4);
+    float *d = (float *) buf.get(sizeof h);
+    if (hipMemcpy(d, h, sizeof h, hipMemcpyHostToDevice) != hipSuccess) throw std::runtime_error("hipMemcpy");
+    check(prr_medium_intersect(ctx, 1, d, d + 3, d + 6, (const int32_t *) (d + 12), d + 13, d + 14,
+                               (uint8_t *) (d + 15)),
+          "prr_medium_intersect");
+    check(prr_synchronize(ctx), "prr_synchronize");
+    if (hipMemcpy(h, d, sizeof h, hipMemcpyDeviceToHost) != hipSuccess) throw std::runtime_error("hipMemcpy");
+    uint8_t hit;
+    std::memcpy(&hit, &h[15], 1);
+    if (!hit) return false;
+    t_enter = h[13];
+    t_exit = h[14];
+    return true;
+}
+
+// phaseHG (Volume.cpp:3-6) through prr_phase_hg.
+float phaseHG(float cos_theta, float g) {
+    prr_context *ctx = context();
+    std::unique_lock<std::mutex> lk;
+    prr_detail::DevBuf &buf = mediaBuf(lk);
+    float h[3] = {cos_theta, g, 0.f};
+    float *d = (float *) buf.get(sizeof h);
+    if (hipMemcpy(d, h, sizeof h, hipMemcpyHostToDevice) != hipSuccess) throw std::runtime_error("hipMemcpy");
+    check(prr_phase_hg(ctx, 1, d, d + 1, d + 2), "prr_phase_hg");
+    check(prr_synchronize(ctx), "prr_synchronize");
+    if (hipMemcpy(h, d, sizeof h, hipMemcpyDeviceToHost) != hipSuccess) throw std::runtime_error("hipMemcpy");
+    return h[2];
+}
+
+// samplePhaseHG (Volume.cpp:8-24): its two draws from the calling thread's
+// generator, then prr_phase_hg_sample.
+Vector3f samplePhaseHG(const Vector3f &wo, float g) {
+    prr_context *ctx = context();
+    const float u1 = hostRandomFloat(), u2 = hostRandomFloat();
+    std::unique_lock<std::mutex> lk;
+    prr_detail::DevBuf &buf = mediaBuf(lk);
+    float h[9] = {wo.x, wo.y, wo.z, g, u1, u2, 0.f, 0.f, 0.f};
+    float *d = (float *) buf.get(sizeof h);
+    if (hipMemcpy(d, h, sizeof h, hipMemcpyHostToDevice) != hipSuccess) throw std::runtime_error("hipMemcpy");
+    check(prr_phase_hg_sample(ctx, 1, d, d + 3, d + 4, d + 6), "prr_phase_hg_sample");
+    check(prr_synchronize(ctx), "prr_synchronize");
+    if (hipMemcpy(h, d, sizeof h, hipMemcpyDeviceToHost) != hipSuccess) throw std::runtime_error("hipMemcpy");
+    return Vector3f(h[6], h[7], h[8]);
+}
+
 void Scene::sampleLight(Intersection &pos, float &pdf) const {
     if (totalEmitArea <= 0.f) return;  // Scene.cpp:38-39
     if (!dev_->scene) throw std::runtime_error("Scene::sampleLight before buildBVH");

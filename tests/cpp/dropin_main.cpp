@@ -114,8 +114,9 @@ static int render(int argc, char **argv) {
 
 // The per-query API of the reference on the box room: BVHAccel (the
 // scene's own and one built from the object list), Scene::sampleLight /
-// pdfLight and the photon map (Scene::buildPhotonMaps, caustic_map,
-// estimateIrradiance). Writes one line per query to <outdir>/api.txt.
+// pdfLight, the photon map (Scene::buildPhotonMaps, caustic_map,
+// estimateIrradiance) and the media functions (Medium::intersect, phaseHG,
+// samplePhaseHG). Writes one line per query to <outdir>/api.txt.
 static int api(int argc, char **argv) {
     if (argc < 3) return 2;
     const std::string out = argv[2];
@@ -150,6 +151,24 @@ static int api(int argc, char **argv) {
         const Vector3f p = cm[k].position + Vector3f(0.01f, 0.f, 0.01f);
         const Vector3f e = scene.photon_map.estimateIrradiance(p, Vector3f(0.f, 1.f, 0.f));
         fprintf(q, "irr %.9g %.9g %.9g %.9g %.9g %.9g\n", p.x, p.y, p.z, e.x, e.y, e.z);
+    }
+    // Medium (Volume.hpp): the dust preset bounded by a box, the fog preset
+    // unbounded, and the phase functions
+    Medium dust = Medium::dust();
+    dust.bounded = true;
+    dust.bounds = Bounds3(Vector3f(-1.f, 0.f, -1.f), Vector3f(1.f, 2.f, 1.f));
+    const Medium fog = Medium::fog();
+    for (const Vector3f &tg : targets) {
+        Ray ray(scene.camera.eye, normalize(tg - scene.camera.eye));
+        float a = -7.f, b = -7.f, c = -7.f, d = -7.f;
+        const int h0 = dust.intersect(ray, a, b) ? 1 : 0, h1 = fog.intersect(ray, c, d) ? 1 : 0;
+        fprintf(q, "medium %d %.9g %.9g %d %.9g %.9g\n", h0, a, b, h1, c, d);
+    }
+    fprintf(q, "phase %.9g %.9g %.9g %d %d\n", phaseHG(0.3f, dust.g), phaseHG(-1.f, 0.f), phaseHG(1.f, 0.6f),
+            dust.active() ? 1 : 0, Medium().active() ? 1 : 0);
+    for (int k = 0; k < 32; k++) {
+        const Vector3f w = samplePhaseHG(Vector3f(0.f, 0.f, 1.f), dust.g);
+        fprintf(q, "hg %.9g %.9g %.9g\n", w.x, w.y, w.z);
     }
     fclose(q);
     FILE *f = fopen((out + "/photons.f32").c_str(), "wb");

@@ -71,6 +71,43 @@ def gen_bsdf(n=3000, seed=21):
     save("bsdf.npz", res)
 
 
+def gen_volume(n=3000, seed=41):
+    """Medium::intersect, phaseHG and samplePhaseHG (Volume.hpp:17-44,
+    Volume.cpp:3-24): rays against boxes (a quarter unbounded; axis-aligned
+    directions with zero components; origins on the box planes and inside),
+    HG at random and extreme (cos, g) including |g| < 1e-3, and phase samples
+    about random unit wo with the reference's own draws."""
+    rng = np.random.default_rng(seed)
+    lo = rng.uniform(-2, 1, (n, 3)).astype(np.float32)
+    hi = (lo + rng.uniform(0.01, 3, (n, 3))).astype(np.float32)
+    org = rng.uniform(-4, 4, (n, 3)).astype(np.float32)
+    k = n // 8
+    org[:k] = lo[:k]  # on the min corner's planes
+    org[k:2 * k] = ((lo[k:2 * k] + hi[k:2 * k]) / 2).astype(np.float32)  # inside
+    d = rng.normal(size=(n, 3)).astype(np.float32)
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    axes = np.array([(1, 0, 0), (-1, 0, 0), (0, 1, 0), (0, -1, 0), (0, 0, 1), (0, 0, -1), (0.6, 0, 0.8)],
+                    np.float32)
+    d[2 * k:3 * k] = axes[rng.integers(0, len(axes), k)]
+    d[:k // 2] = axes[rng.integers(0, len(axes), k // 2)]  # on the planes and parallel to two of them: 0 * inf
+    d[3 * k:3 * k + 50] = np.array([-0.0, 1.0, 0.0], np.float32)  # a negative zero component
+    bounded = (rng.random(n) > 0.25).astype(np.int32)
+    cos = rng.uniform(-1, 1, n).astype(np.float32)
+    g = rng.uniform(-0.95, 0.95, n).astype(np.float32)
+    g[:200] = rng.uniform(-9e-4, 9e-4, 200).astype(np.float32)  # the isotropic branch
+    g[200:260] = np.float32(0.0)
+    cos[260:300] = np.float32(1.0)
+    cos[300:340] = np.float32(-1.0)
+    wo = TS.sphere_dirs(n, seed + 1).astype(np.float32)
+    seeds = seeds_u32(n, seed + 2)
+    bounds = np.concatenate([lo, hi], 1).astype(np.float32)
+    out = run("volume", dict(org=org.ravel(), dir=d.ravel(), bounds=bounds.ravel(), bounded=bounded, cos=cos, g=g,
+                             wo=wo.ravel(), seeds=seeds))
+    res = dict(org=org, dir=d, bounds=bounds, bounded=bounded, cos=cos, g=g, wo=wo, seeds=seeds)
+    res.update({"ref_" + k: v for k, v in out.items()})
+    save("volume.npz", res)
+
+
 def gen_env(seed=31):
     envs = {
         "sky": S.sky(64, 32, sun_sigma_deg=8.0, sun_peak=60.0),
@@ -542,6 +579,8 @@ def main():
         gen_hdr()
     if want("gltf"):
         gen_gltf()
+    if want("volume"):
+        gen_volume()
 
 
 if __name__ == "__main__":

@@ -214,3 +214,27 @@ def test_tonemap_vs_reference(ctx):
     disp, _ = (_np(x) for x in ctx.tonemap("aces", lin, 0.7))
     odisp, _ = O.tonemap(4, lin, 0.7)
     assert _close(disp, odisp, 2e-5, 2e-6).all()
+
+
+def test_media_vs_reference(ctx):
+    """prr_medium_intersect, prr_phase_hg and prr_phase_hg_sample against the
+    reference's own Medium::intersect, phaseHG and samplePhaseHG
+    (Volume.hpp:17-44, Volume.cpp:3-24) at its recorded draws: the box test
+    bit for bit (hit flags, entry / exit times including the NaN of a ray on a
+    box plane and parallel to it, untouched outputs on a miss); the phase
+    function and the samples within 1e-5 (powf, sinf and cosf on the device
+    against the host's)."""
+    print("case media", flush=True)
+    g = load_golden("volume.npz")
+    te, tx, hit = (_np(x) for x in ctx.medium_intersect(g["org"], g["dir"], g["bounds"], g["bounded"]))
+    assert np.array_equal(hit, g["ref_hit"].astype(np.uint8))
+    assert np.array_equal(te, g["ref_t_enter"], equal_nan=True)
+    assert np.array_equal(tx, g["ref_t_exit"], equal_nan=True)
+    p = _np(ctx.phase_hg(g["cos"], g["g"]))
+    assert np.all(np.abs(p - g["ref_phase"]) <= 1e-5 * np.abs(g["ref_phase"]))
+    wi = _np(ctx.phase_hg_sample(g["wo"], g["g"], g["ref_draws"].reshape(-1, 2)))
+    ref = g["ref_sample_wi"].reshape(-1, 3)
+    assert np.all(np.abs(wi - ref) <= 1e-5 * np.maximum(np.abs(ref), 1.0))
+    print("media: phase max rel err %.2e, sample max abs err %.2e, bit-exact samples %.4f"
+          % (np.max(np.abs(p - g["ref_phase"]) / np.abs(g["ref_phase"])), np.max(np.abs(wi - ref)),
+             np.mean(np.all(wi == ref, axis=1))), flush=True)

@@ -289,3 +289,37 @@ def test_welford_closed_forms():
             assert np.all(ci(k) < 0.2 + 1e-4)
         if k > 4:
             assert not np.all(ci(k - 1) < 0.2 - 1e-4)
+
+
+def test_media_match_reference():
+    """Medium::intersect, phaseHG and samplePhaseHG (Volume.hpp:17-44,
+    Volume.cpp:3-24) against the reference's own outputs: the box test bit for
+    bit, NaN entry / exit times included (rays on a box plane and parallel to
+    it), the phase function and the phase samples at the reference's draws."""
+    g = load_golden("volume.npz")
+    te, tx, hit = O.medium_intersect(g["org"], g["dir"], g["bounds"], g["bounded"])
+    assert np.array_equal(hit, g["ref_hit"].astype(np.uint8))
+    assert np.array_equal(te, g["ref_t_enter"], equal_nan=True)
+    assert np.array_equal(tx, g["ref_t_exit"], equal_nan=True)
+    assert np.isnan(g["ref_t_exit"]).any() and (g["bounded"] == 0).any() and (hit == 0).any()
+    assert np.array_equal(O.phase_hg(g["cos"], g["g"]), g["ref_phase"])
+    wi = O.phase_hg_sample(g["wo"], g["g"], g["ref_draws"].reshape(-1, 2))
+    assert np.array_equal(wi, g["ref_sample_wi"].reshape(-1, 3))
+
+
+def test_media_closed_forms():
+    """The HG phase function integrates to one over the sphere and is 1/(4 pi)
+    at g = 0; its samples' cos theta about wo follow the inverse of the HG
+    CDF (mean cos theta = g)."""
+    rng = np.random.default_rng(3)
+    for gv in (-0.7, -0.2, 0.0, 0.3, 0.85):
+        mu = np.linspace(-1, 1, 200001).astype(np.float32)
+        p = O.phase_hg(mu, np.full_like(mu, gv)).astype(np.float64)
+        integral = 2 * np.pi * np.trapezoid(p, mu.astype(np.float64))
+        assert abs(integral - 1.0) < 2e-3, (gv, integral)
+        n = 200000
+        wo = np.tile(np.array([[0.0, 0.0, 1.0]], np.float32), (n, 1))
+        wi = O.phase_hg_sample(wo, np.full(n, gv, np.float32), rng.random((n, 2)).astype(np.float32))
+        assert abs(wi[:, 2].mean() - gv) < 5e-3, (gv, wi[:, 2].mean())
+        assert np.allclose(np.linalg.norm(wi, axis=1), 1.0, atol=1e-5)
+    assert abs(O.phase_hg(np.zeros(1, np.float32), np.zeros(1, np.float32))[0] - 1 / (4 * np.pi)) < 1e-8
