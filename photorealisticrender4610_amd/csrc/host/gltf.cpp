@@ -4,12 +4,13 @@
 //
 // Own parsers, no third-party code: a small JSON reader, the GLB container,
 // buffers (GLB BIN chunk, files next to the asset, base64 data: URIs),
-// accessors, and a PNG decoder on zlib's inflate. Images are decoded to four
-// channels as the reference's loader requests them (req_comp 4, 16-bit
-// images kept at 16 bits, then reduced as GLTFLoader.cpp:77-137 does per
-// texture slot).
-// JPEG images are rejected with an error (no decoder here).
+// accessors, a PNG decoder on zlib's inflate and a JPEG decoder (jpeg.cpp).
+// Images are decoded to four channels as the reference's loader requests them
+// (req_comp 4, 16-bit images kept at 16 bits, then reduced as
+// GLTFLoader.cpp:77-137 does per texture slot).
 #include <zlib.h>
+
+#include "jpeg.hpp"
 
 #include <charconv>
 #include <cmath>
@@ -321,11 +322,7 @@ struct Image {
 
 Image decodePng(const uint8_t *p, size_t n) {
     static const uint8_t sig[8] = {137, 80, 78, 71, 13, 10, 26, 10};
-    if (n < 8 || memcmp(p, sig, 8)) {
-        if (n >= 2 && p[0] == 0xFF && p[1] == 0xD8)
-            throw std::runtime_error("glTF: JPEG images are not supported (re-encode the texture as PNG)");
-        throw std::runtime_error("glTF: unknown image format (PNG expected)");
-    }
+    if (n < 8 || memcmp(p, sig, 8)) throw std::runtime_error("glTF: unknown image format (PNG or JPEG expected)");
     size_t off = 8;
     int w = 0, h = 0, depth = 0, ctype = -1, interlace = 0;
     std::vector<uint8_t> idat, plte, trns;
@@ -648,6 +645,19 @@ Model openModel(const std::string &path) {
     return m;
 }
 
+// PNG, or JPEG (jpeg.cpp: 8 bits, four channels, alpha 255).
+Image decodeImage(const uint8_t *p, size_t n) {
+    if (n >= 2 && p[0] == 0xFF && p[1] == 0xD8) {
+        Image img;
+        std::vector<uint8_t> rgba;
+        prr_host::decodeJpeg(p, n, img.w, img.h, rgba);
+        img.bits = 8;
+        img.px.assign(rgba.begin(), rgba.end());
+        return img;
+    }
+    return decodePng(p, n);
+}
+
 // image index -> texture row, 8-bit RGBA. A 16-bit image becomes 8-bit by
 // >> 8 (GLTFLoader.cpp:81-88) in the colour, metallic-roughness and normal
 // slots; the emissive slot keeps the decoded image's bytes as they are
@@ -681,7 +691,7 @@ int textureOf(const Model &m, const std::string &dir, Loaded &L, std::map<int, i
         const size_t n = bv.sizeOr0("byteLength", buf.size() - o);  // o + n <= size, no overflow
         bytes.assign(buf.begin() + (std::ptrdiff_t) o, buf.begin() + (std::ptrdiff_t) (o + n));
     }
-    const Image img = decodePng(bytes.data(), bytes.size());
+    const Image img = decodeImage(bytes.data(), bytes.size());
     std::vector<uint8_t> t(img.px.size());
     if (img.bits == 16 && emissive) {
         for (size_t i = 0; i < t.size(); i++) {

@@ -4,7 +4,8 @@ Runs in the development container only: builds oracle/_ref/ref_harness from
 the reference's own sources (oracle/Makefile target ``ref``), feeds it the
 seeded inputs below, and stores inputs + reference outputs together, so the
 tests never need the reference. Usage:
-python tests/golden/make_golden.py [trace bsdf env camera tonemap castray photon image image_field welford hdr gltf]
+python tests/golden/make_golden.py [trace bsdf env camera tonemap castray photon image image_field welford hdr gltf
+volume jpeg]
 (no names: all).
 """
 import os
@@ -536,6 +537,244 @@ def gen_gltf():
     save("gltf.npz", res)
 
 
+JPEG_WINDOW = 48  # side of the texel windows kept for a large image
+
+# zigzag position -> row-major index in an 8x8 block (T.81 Figure 5)
+ZIGZAG = np.array([0, 1, 8, 16, 9, 2, 3, 10, 17, 24, 32, 25, 18, 11, 4, 5, 12, 19, 26, 33, 40, 48, 41, 34, 27, 20,
+                   13, 6, 7, 14, 21, 28, 35, 42, 49, 56, 57, 50, 43, 36, 29, 22, 15, 23, 30, 37, 44, 51, 58, 59,
+                   52, 45, 38, 31, 39, 46, 53, 60, 61, 54, 47, 55, 62, 63])
+
+
+def jpeg_encode(px, sampling, ids=(1, 2, 3), jfif=True, adobe=None, restart=0):
+    """A baseline JPEG of px [H, W, C] (C 1 or 3, samples stored as given, no
+    colour transform) with the given (h, v) sampling factor per component,
+    for layouts Pillow does not write. Chroma planes are decimated, Huffman
+    codes have fixed lengths (DC categories 4 bits, the 162 AC run/size
+    symbols 8 bits), one quantisation table; restart = MCUs per interval."""
+    import struct
+    from scipy.fft import dctn
+    h_img, w_img, nc = px.shape
+    hmax = max(s[0] for s in sampling)
+    vmax = max(s[1] for s in sampling)
+    mx, my = -(-w_img // (8 * hmax)), -(-h_img // (8 * vmax))
+    full = np.pad(px.astype(np.float64), ((0, my * 8 * vmax - h_img), (0, mx * 8 * hmax - w_img), (0, 0)), "edge")
+    q = np.clip(2 + np.arange(64) // 3, 1, 255)  # zigzag order
+    planes = []
+    for c, (h, v) in enumerate(sampling):
+        p = full[::vmax // v, ::hmax // h, c] - 128.0
+        blocks = p.reshape(my * v, 8, mx * h, 8).transpose(0, 2, 1, 3)
+        coef = dctn(blocks, axes=(2, 3), norm="ortho").reshape(my * v, mx * h, 64)[:, :, ZIGZAG]
+        planes.append(np.rint(coef / q).astype(np.int64))
+    ac_syms = [0x00, 0xF0] + [(r << 4) | s for r in range(16) for s in range(1, 11)]
+    ac_code = {sym: i for i, sym in enumerate(ac_syms)}
+    out = bytearray()
+    acc = [0, 0]  # bit buffer, bit count
+
+    def put(val, n):
+        acc[0] = (acc[0] << n) | (val & ((1 << n) - 1))
+        acc[1] += n
+        while acc[1] >= 8:
+            b = (acc[0] >> (acc[1] - 8)) & 0xFF
+            out.append(b)
+            if b == 0xFF:
+                out.append(0)
+            acc[1] -= 8
+        acc[0] &= (1 << acc[1]) - 1
+
+    def flush():
+        if acc[1]:
+            put((1 << (8 - acc[1])) - 1, 8 - acc[1])
+
+    def value(v):
+        s = int(abs(v)).bit_length()
+        return s, (v if v >= 0 else v + (1 << s) - 1)
+
+    pred = [0] * nc
+
+    def block(c, blk):
+        s, bits = value(int(blk[0]) - pred[c])
+        pred[c] = int(blk[0])
+        put(s, 4)
+        put(bits, s)
+        run = 0
+        for k in range(1, 64):
+            if blk[k] == 0:
+                run += 1
+                continue
+            while run > 15:
+                put(ac_code[0xF0], 8)
+                run -= 16
+            s, bits = value(int(blk[k]))
+            put(ac_code[(run << 4) | s], 8)
+            put(bits, s)
+            run = 0
+        if run:
+            put(ac_code[0x00], 8)
+
+    def seg(marker, body):
+        return bytes([0xFF, marker]) + struct.pack(">H", len(body) + 2) + body
+    head = b"\xff\xd8"
+    if jfif:
+        head += seg(0xE0, b"JFIF\0\x01\x01\0\0\x01\0\x01\0\0")
+    if adobe is not None:
+        head += seg(0xEE, b"Adobe\0\x64\0\0\0\0" + bytes([adobe]))  # version 100, flags, transform
+    head += seg(0xDB, b"\x00" + bytes(int(x) for x in q))
+    head += seg(0xC0, struct.pack(">BHHB", 8, h_img, w_img, nc)
+                + b"".join(bytes([ids[c], (h << 4) | v, 0]) for c, (h, v) in enumerate(sampling)))
+    head += seg(0xC4, b"\x00" + bytes([0, 0, 0, 12] + [0] * 12) + bytes(range(12))
+                + b"\x10" + bytes([0] * 7 + [162] + [0] * 8) + bytes(ac_syms))
+    if restart:
+        head += seg(0xDD, struct.pack(">H", restart))
+    head += seg(0xDA, bytes([nc]) + b"".join(bytes([ids[c], 0]) for c in range(nc)) + b"\x00\x3f\x00")
+    if nc == 1:  # non-interleaved: the data blocks only
+        h, v = sampling[0]
+        cw, ch = -(-w_img * h // hmax), -(-h_img * v // vmax)
+        units = [[(0, bx, by)] for by in range(-(-ch // 8)) for bx in range(-(-cw // 8))]
+    else:
+        units = [[(c, x * h + bx, y * v + by) for c, (h, v) in enumerate(sampling)
+                  for by in range(v) for bx in range(h)] for y in range(my) for x in range(mx)]
+    for i, mcu in enumerate(units):
+        if restart and i and i % restart == 0:
+            flush()
+            out.extend(bytes([0xFF, 0xD0 + (i // restart - 1) % 8]))
+            pred = [0] * nc
+        for c, bx, by in mcu:
+            block(c, planes[c][by, bx])
+    flush()
+    return head + bytes(out) + b"\xff\xd9"
+
+
+def jpeg_images(seed=91):
+    """JPEG files for the loader, encoded by Pillow (libjpeg): (label, bytes).
+    Baseline 4:2:0 / 4:2:2 / 4:4:4, optimised Huffman tables, restart
+    intervals in blocks and in rows, grey, progressive colour and grey, a
+    saturated high-quality image (IDCT ringing past 0 and 255), odd sizes
+    (partial MCUs), and the reference's own progressive photograph
+    (assets/spot/hmap.jpg, 800 x 800, 4:2:0)."""
+    import io
+    from PIL import Image
+    rng = np.random.default_rng(seed)
+
+    def smooth(h, w, ch):
+        y, x = np.mgrid[0:h, 0:w].astype(np.float64)
+        planes = [128 + 90 * np.sin(x / rng.uniform(3, 9) + rng.uniform(0, 6))
+                  * np.cos(y / rng.uniform(3, 9) + rng.uniform(0, 6)) for _ in range(ch)]
+        a = np.stack(planes, -1) + rng.normal(0, 12, (h, w, ch))
+        return np.clip(np.rint(a), 0, 255).astype(np.uint8)
+
+    def enc(px, **kw):
+        b = io.BytesIO()
+        Image.fromarray(px[..., 0] if px.shape[2] == 1 else px).save(b, "JPEG", **kw)
+        return b.getvalue()
+    out = [
+        ("420_q90", enc(smooth(23, 37, 3), quality=90, subsampling=2)),
+        ("444_opt", enc(smooth(16, 16, 3), quality=95, subsampling=0, optimize=True)),
+        ("422_rst_blocks", enc(smooth(29, 41, 3), quality=75, subsampling=1, restart_marker_blocks=3)),
+        ("420_rst_rows", enc(smooth(33, 33, 3), quality=80, subsampling=2, restart_marker_rows=1)),
+        ("grey", enc(smooth(19, 25, 1), quality=85)),
+        ("420_progressive", enc(smooth(31, 53, 3), quality=80, subsampling=2, progressive=True)),
+        ("444_progressive_opt", enc(smooth(12, 20, 3), quality=90, subsampling=0, progressive=True,
+                                    optimize=True)),
+        ("grey_progressive", enc(smooth(17, 9, 1), quality=70, progressive=True)),
+        ("saturated_q100", enc((rng.integers(0, 2, (24, 24, 3)) * 255).astype(np.uint8), quality=100,
+                               subsampling=0)),
+    ]
+    # layouts Pillow does not write (own baseline encoder)
+    out += [
+        ("440", jpeg_encode(smooth(21, 30, 3), [(1, 2), (1, 1), (1, 1)])),
+        ("411", jpeg_encode(smooth(18, 45, 3), [(4, 1), (1, 1), (1, 1)], restart=2)),
+        ("h1v4", jpeg_encode(smooth(37, 14, 3), [(1, 4), (1, 1), (1, 1)])),
+        ("h3v1", jpeg_encode(smooth(11, 29, 3), [(3, 1), (1, 1), (1, 1)])),
+        ("mixed_41_21_11", jpeg_encode(smooth(26, 35, 3), [(4, 1), (2, 1), (1, 1)])),
+        ("mixed_22_21_11", jpeg_encode(smooth(26, 35, 3), [(2, 2), (2, 1), (1, 1)], restart=1)),
+        ("mixed_22_11_22", jpeg_encode(smooth(19, 23, 3), [(2, 2), (1, 1), (2, 2)])),
+        ("chroma_width_1", jpeg_encode(smooth(9, 2, 3), [(2, 1), (1, 1), (1, 1)])),
+        ("chroma_1x1_hv", jpeg_encode(smooth(2, 2, 3), [(2, 2), (1, 1), (1, 1)])),
+        ("rgb_ids", jpeg_encode(smooth(13, 17, 3), [(1, 1)] * 3, ids=(82, 71, 66))),
+        ("adobe_rgb", jpeg_encode(smooth(15, 12, 3), [(1, 1)] * 3, jfif=False, adobe=0)),
+        ("adobe_rgb_with_jfif", jpeg_encode(smooth(15, 12, 3), [(1, 1)] * 3, adobe=0)),
+        ("adobe_ycc", jpeg_encode(smooth(10, 10, 3), [(2, 2), (1, 1), (1, 1)], jfif=False, adobe=1)),
+        ("grey_22", jpeg_encode(smooth(21, 27, 1), [(2, 2)], restart=3)),
+        ("ycck", jpeg_encode(smooth(14, 18, 4), [(2, 2), (1, 1), (1, 1), (2, 2)], ids=(1, 2, 3, 4), jfif=False,
+                             adobe=2)),
+        ("ycbcr_k_ignored", jpeg_encode(smooth(14, 18, 4), [(1, 1)] * 4, ids=(1, 2, 3, 4))),
+    ]
+    # CMYK as Pillow writes it (libjpeg, Adobe marker, transform 0)
+    out.append(("cmyk", (lambda b: (Image.fromarray(smooth(16, 24, 4), "CMYK").save(b, "JPEG", quality=85),
+                                    b.getvalue())[1])(io.BytesIO())))
+    with open("/root/reference/assets/spot/hmap.jpg", "rb") as f:
+        out.append(("hmap", f.read()))
+    return out
+
+
+def jpeg_glb(images):
+    """A GLB with one triangle primitive per image, each with a material whose
+    base-colour texture is that image (image/jpeg in the BIN chunk)."""
+    import json
+    import struct
+    pos = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0]], np.float32).tobytes()
+    blobs, views = [pos], [{"buffer": 0, "byteOffset": 0, "byteLength": len(pos)}]
+    off = len(pos)
+    for _, b in images:
+        pad = (-off) % 4
+        blobs.append(b"\0" * pad)
+        off += pad
+        views.append({"buffer": 0, "byteOffset": off, "byteLength": len(b)})
+        blobs.append(b)
+        off += len(b)
+    binb = b"".join(blobs)
+    binb += b"\0" * ((-len(binb)) % 4)
+    k = len(images)
+    doc = {
+        "asset": {"version": "2.0"},
+        "buffers": [{"byteLength": len(binb)}],
+        "bufferViews": views,
+        "accessors": [{"bufferView": 0, "componentType": 5126, "count": 3, "type": "VEC3"}],
+        "images": [{"bufferView": 1 + i, "mimeType": "image/jpeg"} for i in range(k)],
+        "textures": [{"source": i} for i in range(k)],
+        "materials": [{"pbrMetallicRoughness": {"baseColorTexture": {"index": i}}} for i in range(k)],
+        "meshes": [{"primitives": [{"attributes": {"POSITION": 0}, "material": i} for i in range(k)]}],
+    }
+    js = json.dumps(doc).encode()
+    js += b" " * ((-len(js)) % 4)
+    glb = struct.pack("<III", 0x46546C67, 2, 12 + 8 + len(js) + 8 + len(binb))
+    return glb + struct.pack("<II", len(js), 0x4E4F534A) + js + struct.pack("<II", len(binb), 0x004E4942) + binb
+
+
+def gen_jpeg():
+    """GLTFLoader::load (tinygltf + stb_image) on a GLB of JPEG textures: per
+    image its size and texels (RGBA8). A large image keeps three windows of
+    JPEG_WINDOW^2 texels (corner, centre, far corner) and its per-channel
+    texel sums."""
+    images = jpeg_images()
+    glb = jpeg_glb(images)
+    res = {"file_jpeg.glb": np.frombuffer(glb, np.uint8),
+           "labels": np.frombuffer("\n".join(n for n, _ in images).encode(), np.uint8)}
+    with tempfile.TemporaryDirectory() as td:
+        path = os.path.join(td, "jpeg.glb")
+        with open(path, "wb") as f:
+            f.write(glb)
+        out = run("gltf", dict(path=np.frombuffer(path.encode(), np.uint8), type=np.array([0], np.int32)))
+    tex = out["tex"].reshape(-1, 4, 4)
+    texels = out["texels"]
+    off = 0
+    for i, (label, _) in enumerate(images):
+        w, h, ch, nb = (int(v) for v in tex[i, 0])
+        assert ch == 4 and nb == w * h * 4 and not tex[i, 1:, 3].any()
+        px = texels[off:off + nb].reshape(h, w, 4)
+        off += nb
+        res["wh_" + label] = np.array([w, h], np.int32)
+        if w * h <= 64 * 64:
+            res["texels_" + label] = px
+            continue
+        s = JPEG_WINDOW
+        wins = np.array([[0, 0], [(h - s) // 2, (w - s) // 2], [h - s, w - s]], np.int32)
+        res["windows_" + label] = wins
+        res["texels_" + label] = np.stack([px[y:y + s, x:x + s] for y, x in wins])
+        res["sums_" + label] = px.reshape(-1, 4).sum(0, dtype=np.int64)
+    save("jpeg.npz", res)
+
+
 def main():
     if not os.path.isdir("/root/reference"):
         print("reference sources absent; fixtures are committed, nothing to do")
@@ -581,6 +820,8 @@ def main():
         gen_gltf()
     if want("volume"):
         gen_volume()
+    if want("jpeg"):
+        gen_jpeg()
 
 
 if __name__ == "__main__":

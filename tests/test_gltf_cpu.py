@@ -93,7 +93,8 @@ def test_loader_errors(assets, tmp_path):
     bad.write_bytes(b"glTF\x02\x00\x00\x00\x18\x00\x00\x00\x04\x00\x00\x00JSON{bad")
     with pytest.raises(abi.PrrError, match="JSON"):
         abi.load_gltf(str(bad))
-    # a JPEG image is refused with a clear message
+    # a PNG whose signature is overwritten by a JPEG one is not a JPEG: an
+    # error naming JPEG (JPEG decoding itself: test_jpeg_cpu.py)
     data = bytearray(Z["file_textured.glb"].tobytes())
     k = data.find(b"\x89PNG")
     data[k:k + 2] = b"\xff\xd8"
