@@ -110,21 +110,28 @@ int ensureRenderBufs(prr_context *c, int64_t slots, int maxDepth) {
     A(&p.flags, slots);
     A(&p.vis, 3 * slots);
     A(&p.result, slots);
+    // segmented queues: kQSegs segments of qcap (ray) / sqcap (shadow) entries
+    p.qcap = segCapacity(slots, 1);
+    p.sqcap = segCapacity(slots, 3);
+    const size_t qEntries = (size_t) kQSegs * p.qcap, sqEntries = (size_t) kQSegs * p.sqcap;
     for (int k = 0; k < 2; k++) {
-        A(&p.qSlot[k], slots);
-        A(&p.qOrg[k], slots);
-        A(&p.qDir[k], slots);
+        A(&p.qSlot[k], qEntries);
+        A(&p.qOrg[k], qEntries);
+        A(&p.qDir[k], qEntries);
     }
-    A(&p.hit, slots);
-    A(&p.sqSlot, 3 * slots);
-    A(&p.sqOrg, 3 * slots);
-    A(&p.sqDir, 3 * slots);
+    A(&p.hit, qEntries);
+    A(&p.sqSlot, sqEntries);
+    A(&p.sqOrg, sqEntries);
+    A(&p.sqDir, sqEntries);
     A(&p.qcnt, (size_t) maxDepth + 2);
     A(&p.sqcnt, (size_t) maxDepth + 2);
+    A(&p.segc, ((size_t) maxDepth + 2) * kQSegs);
+    A(&p.qpre, ((size_t) maxDepth + 2) * (kQSegs + 1));
+    A(&p.sqpre, ((size_t) maxDepth + 2) * (kQSegs + 1));
     A(&p.fetch, 2 * ((size_t) maxDepth + 2));
     c->bins = ShadeBins{};
     if (c->shadeSort) {
-        c->bins.cap = slots;
+        c->bins.cap = slots;  // a list holds at most the queue's entries
         c->bins.blocks = 4 * c->numCU;
         A(&c->bins.lists, 2 * slots);
         A(&c->bins.counts, 2);
@@ -132,7 +139,7 @@ int ensureRenderBufs(prr_context *c, int64_t slots, int maxDepth) {
     }
     c->sb = ShadowSortBufs{};
     if (c->shadowSortFrom > 0) {
-        // as many entries as the shadow queue (up to three NEE rays per path)
+        // as many entries as the shadow queue holds (up to three NEE rays per path), in order
         ShadowSortBufs &b = c->sb;
         b.cap = 3 * slots;
         b.blocks = 4 * c->numCU;
@@ -208,10 +215,12 @@ int runBounces(prr_context *c, const SceneView &view, const ChunkInfo &ci, Timin
         // on) on the render stream, ahead of the extension rays: a short pass
         // that would otherwise wait for the persistent traversal grid's CUs
         PathBufs pbS = c->pb;
+        bool denseS = false;
         if (b > 0 && c->shadowSortFrom > 0 && b - 1 >= c->shadowSortFrom && c->sb.cap > 0) {
             pbS.sqSlot = c->sb.slot;
             pbS.sqOrg = c->sb.org;
             pbS.sqDir = c->sb.dir;
+            denseS = true;
             TIMED(4, st, launchShadowSort(view, c->pb, b - 1, c->shadowSortBits, c->sb, st));
         }
         if (b > 0 && conc) {
@@ -219,7 +228,7 @@ int runBounces(prr_context *c, const SceneView &view, const ChunkInfo &ci, Timin
             HIPCK(c, hipStreamWaitEvent(c->aux, c->evFork, 0));
             TIMED(1, c->aux,
                   launchShadow(view, pbS, b - 1, c->spillAux, c->traceBlocks, c->refillAt, c->stackDepth, wwShadow,
-                               c->fetchRun, c->shadowWide, c->aux));
+                               c->fetchRun, c->shadowWide, denseS, c->aux));
             HIPCK(c, hipEventRecord(c->evJoin, c->aux));
         }
         TIMED(0, st,
@@ -228,7 +237,7 @@ int runBounces(prr_context *c, const SceneView &view, const ChunkInfo &ci, Timin
         if (b > 0 && !conc)
             TIMED(1, st,
                   launchShadow(view, pbS, b - 1, c->spill, c->traceBlocks, c->refillAt, c->stackDepth, wwShadow,
-                               c->fetchRun, c->shadowWide, st));
+                               c->fetchRun, c->shadowWide, denseS, st));
         if (b > 0 && conc) HIPCK(c, hipStreamWaitEvent(st, c->evJoin, 0));
         if (tail)
             TIMED(3, st,
@@ -239,6 +248,7 @@ int runBounces(prr_context *c, const SceneView &view, const ChunkInfo &ci, Timin
                                            c->bins, st));
         else
             TIMED(2, st, launchShade(view, c->pb, cit, b, c->shadeBlocks, c->shadeWaves, st));
+        HIPCK(c, launchQueueScan(c->pb, b, st));
     }
 #undef TIMED
     return PRR_OK;
@@ -247,6 +257,7 @@ int runBounces(prr_context *c, const SceneView &view, const ChunkInfo &ci, Timin
 int resetChunk(prr_context *c, int maxDepth, hipStream_t st) {
     HIPCK(c, hipMemsetAsync(c->pb.qcnt, 0, ((size_t) maxDepth + 2) * sizeof(uint32_t), st));
     HIPCK(c, hipMemsetAsync(c->pb.sqcnt, 0, ((size_t) maxDepth + 2) * sizeof(uint32_t), st));
+    HIPCK(c, hipMemsetAsync(c->pb.segc, 0, ((size_t) maxDepth + 2) * kQSegs * sizeof(unsigned long long), st));
     HIPCK(c, hipMemsetAsync(c->pb.fetch, 0, 2 * ((size_t) maxDepth + 2) * sizeof(uint32_t), st));
     return PRR_OK;
 }

@@ -72,6 +72,44 @@ PRR_DEV V3 clampInd(const V3 &c, float k) {  // Integrator.cpp:105-111
 }
 PRR_DEV float mis(float a, float b) { return (a * a) / (a * a + b * b + 1e-6f); }  // Integrator.hpp:30-32
 
+// Queue 0 holds the n camera (or caller) rays in order, entry w at index w:
+// its prefix is that of a queue filled segment by segment. Also counts the
+// rays and samples.
+PRR_DEV void queue0Written(const PathBufs &pb, int64_t n) {
+    if (blockIdx.x != 0) return;
+    for (int k = threadIdx.x; k <= kQSegs; k += blockDim.x) {
+        const int64_t p = (int64_t) k * pb.qcap;
+        pb.qpre[k] = (uint32_t) (p < n ? p : n);
+    }
+    if (threadIdx.x == 0) {
+        pb.qcnt[0] = (uint32_t) n;
+        atomicAdd(&pb.stats[0], (unsigned long long) n);
+        atomicAdd(&pb.stats[2], (unsigned long long) n);
+    }
+}
+
+// After shading pass b: the prefix counts and totals of the ray queue of
+// bounce b + 1 (block 0) and of the shadow queue of bounce b (block 1), each
+// count clamped to the segment capacity (a producer never passes it).
+__global__ __launch_bounds__(64) void kQueueScan(PathBufs pb, int bounce) {
+    static_assert(kQSegs == 64, "one lane per segment");
+    const bool sh = blockIdx.x == 1;
+    const int b = sh ? bounce : bounce + 1;
+    const int64_t cap = sh ? pb.sqcap : pb.qcap;
+    const int k = (int) threadIdx.x;
+    const unsigned long long pair = pb.segc[(int64_t) bounce * kQSegs + k];
+    const uint32_t c = sh ? (uint32_t) (pair >> 32) : (uint32_t) pair;
+    uint32_t v = (int64_t) c < cap ? c : (uint32_t) cap;
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t t = __shfl_up(v, o);
+        if (k >= o) v += t;
+    }
+    uint32_t *pre = (sh ? pb.sqpre : pb.qpre) + (int64_t) b * (kQSegs + 1);
+    pre[k + 1] = v;
+    if (k == 0) pre[0] = 0u;
+    if (k == 63) (sh ? pb.sqcnt : pb.qcnt)[b] = v;
+}
+
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(kShadeBlock) void kRaygen(SceneView s, PathBufs pb, ChunkInfo ci) {
     const int64_t n = (int64_t) ci.pixels * ci.spc;
@@ -95,11 +133,7 @@ __global__ __launch_bounds__(kShadeBlock) void kRaygen(SceneView s, PathBufs pb,
         pb.qOrg[0][slot] = st4(o, 0.f);
         pb.qDir[0][slot] = st4(d, 0.f);
     }
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        pb.qcnt[0] = (uint32_t) n;
-        atomicAdd(&pb.stats[0], (unsigned long long) n);
-        atomicAdd(&pb.stats[2], (unsigned long long) n);
-    }
+    queue0Written(pb, n);
 }
 
 // ---------------------------------------------------------------------------
@@ -140,11 +174,7 @@ __global__ __launch_bounds__(kShadeBlock) void kRaygenSeg(PathBufs pb, const Seg
         pb.qOrg[0][slot] = st4(o, 0.f);
         pb.qDir[0][slot] = st4(d, 0.f);
     }
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        pb.qcnt[0] = (uint32_t) n;
-        atomicAdd(&pb.stats[0], (unsigned long long) n);
-        atomicAdd(&pb.stats[2], (unsigned long long) n);
-    }
+    queue0Written(pb, n);
 }
 
 // Renderer.cpp:84-99 per request pixel: its ns samples of the batch added in
@@ -186,11 +216,7 @@ __global__ __launch_bounds__(kShadeBlock) void kRaygenRays(PathBufs pb, int64_t 
         pb.qOrg[0][slot] = make_float4(org[3 * k], org[3 * k + 1], org[3 * k + 2], 0.f);
         pb.qDir[0][slot] = make_float4(dir[3 * k], dir[3 * k + 1], dir[3 * k + 2], 0.f);
     }
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        pb.qcnt[0] = (uint32_t) n;
-        atomicAdd(&pb.stats[0], (unsigned long long) n);
-        atomicAdd(&pb.stats[2], (unsigned long long) n);
-    }
+    queue0Written(pb, n);
 }
 
 __global__ void kCopyResults(PathBufs pb, int64_t n, int64_t first, float *L) {
@@ -217,9 +243,11 @@ template <int ITER>
 __global__ __launch_bounds__(kTraceBlock) void kExtend(SceneView s, PathBufs pb, int bounce, uint2 *spill, int refillAt,
                                                        int stackDepth, int wwCap, int fetchRun, int64_t tailPaths) {
     extern __shared__ uint2 stk[];
+    __shared__ uint32_t spre[kQSegs + 1];
     const int q = bounce & 1;
     const int64_t n = pb.qcnt[bounce];
     if (n <= tailPaths) return;  // kFinish runs this bounce's paths (tailPaths > 0) or there are none
+    loadPrefix(spre, pb.qpre + (int64_t) bounce * (kQSegs + 1));
     uint32_t *fetch = &pb.fetch[2 * bounce];
     const int64_t lane = (int64_t) blockIdx.x * kTraceBlock + threadIdx.x;
     LaneStack st;
@@ -242,12 +270,13 @@ __global__ __launch_bounds__(kTraceBlock) void kExtend(SceneView s, PathBufs pb,
             if (got >= n) {
                 exhausted = true;
             } else {
-                const float4 o4 = pb.qOrg[q][got];
+                const int64_t e = segEntry(spre, pb.qcap, got);
+                const float4 o4 = pb.qOrg[q][e];
                 if (o4.w < 0.f) {  // terminating entry: no ray
-                    pb.hit[got] = make_float4(__builtin_inff(), 0.f, 0.f, __int_as_float(-1));
+                    pb.hit[e] = make_float4(__builtin_inff(), 0.f, 0.f, __int_as_float(-1));
                 } else {
-                    idx = got;
-                    travBegin(ts, ld3(o4), ld3(pb.qDir[q][got]), 0.f, s.root, st);
+                    idx = e;
+                    travBegin(ts, ld3(o4), ld3(pb.qDir[q][e]), 0.f, s.root, st);
                 }
             }
         }
@@ -265,9 +294,11 @@ __global__ __launch_bounds__(kTraceBlock) void kExtend(SceneView s, PathBufs pb,
 // TREE: 0 the binary tree, 1 the four-wide occlusion tree, 2 its quantised copy.
 template <bool WW, int TREE>
 __global__ __launch_bounds__(kTraceBlock) void kShadow(SceneView s, PathBufs pb, int bounce, uint2 *spill, int refillAt,
-                                                       int stackDepth, int wwCap, int fetchRun) {
+                                                       int stackDepth, int wwCap, int fetchRun, bool dense) {
     extern __shared__ uint2 stk[];
+    __shared__ uint32_t spre[kQSegs + 1];
     const int64_t n = pb.sqcnt[bounce];
+    loadPrefix(spre, pb.sqpre + (int64_t) bounce * (kQSegs + 1));
     uint32_t *fetch = &pb.fetch[2 * bounce + 1];
     const int64_t lane = (int64_t) blockIdx.x * kTraceBlock + threadIdx.x;
     using Stack = typename std::conditional<TREE != 0, RefStack, LaneStack>::type;
@@ -287,9 +318,10 @@ __global__ __launch_bounds__(kTraceBlock) void kShadow(SceneView s, PathBufs pb,
             if (got >= n) {
                 exhausted = true;
             } else {
-                const uint32_t k = pb.sqSlot[got];
-                const float4 o4 = pb.sqOrg[got];
-                travBegin(ts, ld3(o4), ld3(pb.sqDir[got]), o4.w, TREE == 2 ? s.root4q : (TREE == 1 ? s.root4 : s.root),
+                const int64_t e = dense ? got : segEntry(spre, pb.sqcap, got);
+                const uint32_t k = pb.sqSlot[e];
+                const float4 o4 = pb.sqOrg[e];
+                travBegin(ts, ld3(o4), ld3(pb.sqDir[e]), o4.w, TREE == 2 ? s.root4q : (TREE == 1 ? s.root4 : s.root),
                           st);
                 if constexpr (TREE == 2) anyPrepQ(s, ts);
                 if (anyBegin(s, ts)) {
@@ -648,10 +680,12 @@ __device__ __forceinline__ void shadePath(const SceneView &s, const PathBufs &pb
 template <bool PHOTON, bool CACHE, int KIND>
 __device__ __forceinline__ void shadeImpl(const SceneView &s, const PathBufs &pb, const ChunkInfo &ci, int bounce,
                                           const uint32_t *list, const uint32_t *count) {
+    __shared__ uint32_t spre[kQSegs + 1];
     const int q = bounce & 1;
     const int64_t cnt = list ? (int64_t) *count : (int64_t) pb.qcnt[bounce];
     // at most tailPaths entries: kFinish runs this bounce's paths (an empty loop here)
     const int64_t n = pb.qcnt[bounce] > ci.tailPaths ? cnt : 0;
+    if (!list) loadPrefix(spre, pb.qpre + (int64_t) bounce * (kQSegs + 1));
     const int64_t stride = (int64_t) gridDim.x * kShadeBlock;
     const int64_t i0 = (int64_t) blockIdx.x * kShadeBlock + threadIdx.x;
     // uniform trip count per wave so every lane reaches the appends
@@ -662,47 +696,62 @@ __device__ __forceinline__ void shadeImpl(const SceneView &s, const PathBufs &pb
         uint32_t slot = 0;
         ShadeOut so;
         if (i < n) {
-            const int64_t e = list ? (int64_t) list[i] : i;
+            const int64_t e = list ? (int64_t) list[i] : segEntry(spre, pb.qcap, i);
             slot = pb.qSlot[q][e];
             shadePath<PHOTON, CACHE, KIND>(s, pb, ci, bounce, slot, QueueRay{pb, q, e}, so);
         }
         const bool emitRay = so.emitRay, termEntry = so.termEntry, emitA = so.emitA, emitE = so.emitE,
                    emitP = so.emitP;
-        // queue appends (all lanes of the wave take part)
-        const uint32_t qi = waveAppend(&pb.qcnt[bounce + 1], emitRay);
-        if (emitRay) {
-            pb.qSlot[q ^ 1][qi] = slot;
-            pb.qOrg[q ^ 1][qi] = termEntry ? make_float4(0.f, 0.f, 0.f, -1.f) : st4(so.newO, 0.f);
-            pb.qDir[q ^ 1][qi] = termEntry ? make_float4(0.f, 0.f, 0.f, 0.f) : st4(so.newD, 0.f);
-        }
-        // the wave's shadow rays (area, env, point) take one atomic together
+        // queue appends (all lanes of the wave take part): the wave's rays
+        // and shadow rays (area, env, point) go to the segment of its input
+        // position, reserved with one atomic on the segment's counter pair
         {
+            const int seg = (int) ((wb >> 6) % kQSegs);
+            const uint64_t mR = __ballot(emitRay);
             const uint64_t mA = __ballot(emitA), mE = __ballot(emitE), mP = __ballot(emitP);
+            const uint32_t nR = (uint32_t) __popcll(mR);
             const uint32_t nA = (uint32_t) __popcll(mA), nE = (uint32_t) __popcll(mE);
             const uint32_t total = nA + nE + (uint32_t) __popcll(mP);
-            uint32_t base = 0;
-            if (total) {
-                if (laneId() == 0) base = atomicAdd(&pb.sqcnt[bounce], total);
-                base = __shfl(base, 0);
+            unsigned long long pair = 0;
+            if (nR | total) {
+                if (laneId() == 0)
+                    pair = atomicAdd(&pb.segc[(int64_t) bounce * kQSegs + seg],
+                                     ((unsigned long long) total << 32) | nR);
+                pair = __shfl(pair, 0);
             }
             const uint64_t below = (1ull << laneId()) - 1ull;
+            const int64_t qi = (int64_t) (uint32_t) pair + __popcll(mR & below);
+            if (emitRay && qi < pb.qcap) {
+                const int64_t e = (int64_t) seg * pb.qcap + qi;
+                pb.qSlot[q ^ 1][e] = slot;
+                pb.qOrg[q ^ 1][e] = termEntry ? make_float4(0.f, 0.f, 0.f, -1.f) : st4(so.newO, 0.f);
+                pb.qDir[q ^ 1][e] = termEntry ? make_float4(0.f, 0.f, 0.f, 0.f) : st4(so.newD, 0.f);
+            }
+            const uint32_t base = (uint32_t) (pair >> 32);
+            const int64_t s0 = (int64_t) seg * pb.sqcap;
             if (emitA) {
-                const uint32_t ia = base + (uint32_t) __popcll(mA & below);
-                pb.sqSlot[ia] = slot;
-                pb.sqOrg[ia] = st4(so.sOA, so.tA);
-                pb.sqDir[ia] = st4(so.sDA, 0.f);
+                const int64_t ia = (int64_t) base + __popcll(mA & below);
+                if (ia < pb.sqcap) {
+                    pb.sqSlot[s0 + ia] = slot;
+                    pb.sqOrg[s0 + ia] = st4(so.sOA, so.tA);
+                    pb.sqDir[s0 + ia] = st4(so.sDA, 0.f);
+                }
             }
             if (emitE) {
-                const uint32_t ie = base + nA + (uint32_t) __popcll(mE & below);
-                pb.sqSlot[ie] = slot | (1u << 30);
-                pb.sqOrg[ie] = st4(so.sOE, __builtin_inff());
-                pb.sqDir[ie] = st4(so.sDE, 0.f);
+                const int64_t ie = (int64_t) base + nA + __popcll(mE & below);
+                if (ie < pb.sqcap) {
+                    pb.sqSlot[s0 + ie] = slot | (1u << 30);
+                    pb.sqOrg[s0 + ie] = st4(so.sOE, __builtin_inff());
+                    pb.sqDir[s0 + ie] = st4(so.sDE, 0.f);
+                }
             }
             if (emitP) {
-                const uint32_t ip = base + nA + nE + (uint32_t) __popcll(mP & below);
-                pb.sqSlot[ip] = slot | (2u << 30);
-                pb.sqOrg[ip] = st4(so.sOP, so.tP);
-                pb.sqDir[ip] = st4(so.sDP, 0.f);
+                const int64_t ip = (int64_t) base + nA + nE + __popcll(mP & below);
+                if (ip < pb.sqcap) {
+                    pb.sqSlot[s0 + ip] = slot | (2u << 30);
+                    pb.sqOrg[s0 + ip] = st4(so.sOP, so.tP);
+                    pb.sqDir[s0 + ip] = st4(so.sDP, 0.f);
+                }
             }
         }
         nRays += (emitRay && !termEntry) ? 1u : 0u;
@@ -759,14 +808,16 @@ PRR_DEV void shadeChunk(int64_t n, int b, int nb, int64_t &c0, int64_t &c1) {
 __global__ __launch_bounds__(kShadeBlock) void kShadeBinCount(SceneView s, PathBufs pb, ChunkInfo ci, int bounce,
                                                               uint32_t *blockCounts) {
     __shared__ uint32_t part[2][kShadeBlock / 64];
+    __shared__ uint32_t spre[kQSegs + 1];
     const int q = bounce & 1;
     const int64_t cnt = pb.qcnt[bounce];
     const int64_t n = cnt > ci.tailPaths ? cnt : 0;
+    loadPrefix(spre, pb.qpre + (int64_t) bounce * (kQSegs + 1));
     int64_t c0, c1;
     shadeChunk(n, blockIdx.x, gridDim.x, c0, c1);
     uint32_t nl = 0, nd = 0;
     for (int64_t i = c0 + threadIdx.x; i < c1; i += kShadeBlock) {
-        const bool d = diffuseHit(s, pb, q, i);
+        const bool d = diffuseHit(s, pb, q, segEntry(spre, pb.qcap, i));
         nd += d ? 1u : 0u;
         nl += d ? 0u : 1u;
     }
@@ -820,9 +871,11 @@ __global__ __launch_bounds__(kShadeBlock) void kShadeBinScatter(SceneView s, Pat
                                                                 const uint32_t *blockCounts, uint32_t *lists,
                                                                 int64_t cap) {
     __shared__ uint32_t wcnt[2][kShadeBlock / 64];
+    __shared__ uint32_t spre[kQSegs + 1];
     const int q = bounce & 1;
     const int64_t cnt = pb.qcnt[bounce];
     const int64_t n = cnt > ci.tailPaths ? cnt : 0;
+    loadPrefix(spre, pb.qpre + (int64_t) bounce * (kQSegs + 1));
     int64_t c0, c1;
     shadeChunk(n, blockIdx.x, gridDim.x, c0, c1);
     uint32_t base[2] = {blockCounts[blockIdx.x], blockCounts[(size_t) gridDim.x + blockIdx.x]};
@@ -832,7 +885,8 @@ __global__ __launch_bounds__(kShadeBlock) void kShadeBinScatter(SceneView s, Pat
     for (int64_t bb = c0; bb < c1; bb += kShadeBlock) {
         const int64_t i = bb + threadIdx.x;
         const bool valid = i < c1;
-        const bool d = valid && diffuseHit(s, pb, q, i);
+        const int64_t e = valid ? segEntry(spre, pb.qcap, i) : 0;
+        const bool d = valid && diffuseHit(s, pb, q, e);
         const uint64_t md = __ballot(d), ml = __ballot(valid && !d);
         if (laneId() == 0) {
             wcnt[0][w] = (uint32_t) __popcll(ml);
@@ -850,9 +904,9 @@ __global__ __launch_bounds__(kShadeBlock) void kShadeBinScatter(SceneView s, Pat
         __syncthreads();
         if (valid) {
             if (d)
-                lists[cap + offd + (uint32_t) __popcll(md & below)] = (uint32_t) i;
+                lists[cap + offd + (uint32_t) __popcll(md & below)] = (uint32_t) e;
             else
-                lists[offl + (uint32_t) __popcll(ml & below)] = (uint32_t) i;
+                lists[offl + (uint32_t) __popcll(ml & below)] = (uint32_t) e;
         }
         base[0] += totl;
         base[1] += totd;
@@ -903,10 +957,12 @@ __global__ __launch_bounds__(kTraceBlock, 3) void kFinish(SceneView s, PathBufs 
                                                        uint2 *spill, int stackDepth, int wwCap, int tree,
                                                        int64_t residentWaves) {
     extern __shared__ uint2 stk[];
+    __shared__ uint32_t spre[kQSegs + 1];
     const int q = bounce0 & 1;
     const int64_t n = pb.qcnt[bounce0];
     // runs the first bounce whose queue holds at most tailPaths entries
     if (n > ci.tailPaths || (bounce0 > 0 && pb.qcnt[bounce0 - 1] <= ci.tailPaths)) return;
+    loadPrefix(spre, pb.qpre + (int64_t) bounce0 * (kQSegs + 1));
     uint32_t *fetch = &pb.fetch[2 * bounce0];
     const int64_t lane = (int64_t) blockIdx.x * kTraceBlock + threadIdx.x;
     // Two LDS stack areas: the closest-hit stack (uint2 entries) and the
@@ -929,8 +985,9 @@ __global__ __launch_bounds__(kTraceBlock, 3) void kFinish(SceneView s, PathBufs 
     const bool mine = (int) laneId() < lanes;
     while (mine) {
         // every lane that reaches this point takes one entry (>= n: none left)
-        const int64_t i = waveTake(wr, fetch, n, true, lanes, waves, lanes);
-        if (i >= n) break;
+        const int64_t w = waveTake(wr, fetch, n, true, lanes, waves, lanes);
+        if (w >= n) break;
+        const int64_t i = segEntry(spre, pb.qcap, w);
         const uint32_t slot = pb.qSlot[q][i];
         float4 o4 = pb.qOrg[q][i], d4 = pb.qDir[q][i];
         for (int b = bounce0;; b++) {
@@ -1119,12 +1176,13 @@ hipError_t launchExtend(const SceneView &s, const PathBufs &pb, int bounce, uint
     return hipGetLastError();
 }
 hipError_t launchShadow(const SceneView &s, const PathBufs &pb, int bounce, uint2 *spill, int traceGrid,
-                        int refillAt, int stackDepth, int whileWhile, int fetchRun, int tree, hipStream_t st) {
+                        int refillAt, int stackDepth, int whileWhile, int fetchRun, int tree, bool dense,
+                        hipStream_t st) {
     const int wwCap = whileWhile > 1 ? whileWhile : 0;
     const size_t lds = stackDepth * kTraceBlock * sizeof(uint2);
 #define PRR_SHADOW(W, V)                                                                                             \
     hipLaunchKernelGGL((kShadow<W, V>), dim3(traceGrid), dim3(kTraceBlock), lds, st, s, pb, bounce, spill, refillAt, \
-                       stackDepth, wwCap, fetchRun)
+                       stackDepth, wwCap, fetchRun, dense)
     if (whileWhile && tree == 2)
         PRR_SHADOW(true, 2);
     else if (whileWhile && tree == 1)
@@ -1138,6 +1196,10 @@ hipError_t launchShadow(const SceneView &s, const PathBufs &pb, int bounce, uint
     else
         PRR_SHADOW(false, 0);
 #undef PRR_SHADOW
+    return hipGetLastError();
+}
+hipError_t launchQueueScan(const PathBufs &pb, int bounce, hipStream_t st) {
+    hipLaunchKernelGGL(kQueueScan, dim3(2), dim3(64), 0, st, pb, bounce);
     return hipGetLastError();
 }
 template <bool P, bool C>

@@ -84,15 +84,17 @@ PRR_DEV uint32_t waveBinAdd(uint32_t *cnt, uint32_t bin, bool valid) {
 __global__ __launch_bounds__(kBinBlock) void kBinHist(PathBufs pb, int bounce, BinGrid g, int bins,
                                                       uint32_t *__restrict__ hist) {
     extern __shared__ uint32_t cnt[];
+    __shared__ uint32_t spre[kQSegs + 1];
     for (int k = threadIdx.x; k < bins; k += kBinBlock) cnt[k] = 0u;
-    __syncthreads();
+    loadPrefix(spre, pb.sqpre + (int64_t) bounce * (kQSegs + 1));
     int64_t c0, c1;
     chunkOf(pb.sqcnt[bounce], blockIdx.x, gridDim.x, c0, c1);
     // wave-uniform trip count: every lane reaches the ballots
     for (int64_t w = c0 + (threadIdx.x & ~63); w < c1; w += kBinBlock) {
         const int64_t i = w + (threadIdx.x & 63);
         const bool valid = i < c1;
-        const uint32_t bin = valid ? binOf(g, pb.sqOrg[i], pb.sqDir[i]) : 0u;
+        const int64_t e = valid ? segEntry(spre, pb.sqcap, i) : 0;
+        const uint32_t bin = valid ? binOf(g, pb.sqOrg[e], pb.sqDir[e]) : 0u;
         (void) waveBinAdd(cnt, bin, valid);
     }
     __syncthreads();
@@ -129,8 +131,9 @@ __global__ __launch_bounds__(kBinBlock) void kBinScatter(PathBufs pb, int bounce
                                                          const uint32_t *__restrict__ base, uint32_t *__restrict__ slot,
                                                          float4 *__restrict__ org, float4 *__restrict__ dir) {
     extern __shared__ uint32_t cur[];
+    __shared__ uint32_t spre[kQSegs + 1];
     for (int k = threadIdx.x; k < bins; k += kBinBlock) cur[k] = base[(size_t) k * gridDim.x + blockIdx.x];
-    __syncthreads();
+    loadPrefix(spre, pb.sqpre + (int64_t) bounce * (kQSegs + 1));
     int64_t c0, c1;
     chunkOf(pb.sqcnt[bounce], blockIdx.x, gridDim.x, c0, c1);
     for (int64_t w = c0 + (threadIdx.x & ~63); w < c1; w += kBinBlock) {
@@ -139,9 +142,10 @@ __global__ __launch_bounds__(kBinBlock) void kBinScatter(PathBufs pb, int bounce
         float4 o = make_float4(0.f, 0.f, 0.f, 0.f), d = o;
         uint32_t sl = 0;
         if (valid) {
-            o = pb.sqOrg[i];
-            d = pb.sqDir[i];
-            sl = pb.sqSlot[i];
+            const int64_t e = segEntry(spre, pb.sqcap, i);
+            o = pb.sqOrg[e];
+            d = pb.sqDir[e];
+            sl = pb.sqSlot[e];
         }
         const uint32_t j = waveBinAdd(cur, valid ? binOf(g, o, d) : 0u, valid);
         if (valid) {

@@ -20,9 +20,15 @@ hipError_t launchCopyResults(const PathBufs &pb, int64_t n, int64_t first, float
 hipError_t launchExtend(const SceneView &s, const PathBufs &pb, int bounce, uint2 *spill, int traceGrid,
                         int refillAt, int stackDepth, int whileWhile, bool speculate, bool twoLevel, int fetchRun,
                         int64_t tailPaths, hipStream_t st);
-// tree: 0 the binary tree, 1 the four-wide occlusion tree, 2 its quantised copy
+// tree: 0 the binary tree, 1 the four-wide occlusion tree, 2 its quantised copy;
+// dense: pb's shadow queue arrays hold the queue in order (the binned copy),
+// not in segments
 hipError_t launchShadow(const SceneView &s, const PathBufs &pb, int bounce, uint2 *spill, int traceGrid,
-                        int refillAt, int stackDepth, int whileWhile, int fetchRun, int tree, hipStream_t st);
+                        int refillAt, int stackDepth, int whileWhile, int fetchRun, int tree, bool dense,
+                        hipStream_t st);
+// after shading pass `bounce`: prefix counts and totals of the ray queue of
+// bounce + 1 and of the shadow queue of bounce (segmented queues)
+hipError_t launchQueueScan(const PathBufs &pb, int bounce, hipStream_t st);
 // Shadow-queue binning (k_sort.hip): the entries of bounce's shadow queue
 // grouped by (direction octant, Morton code of the origin's cell on a grid of
 // 2^bits cells per axis over the scene box) into sb.slot / org / dir, which

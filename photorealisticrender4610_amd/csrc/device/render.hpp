@@ -17,6 +17,18 @@ constexpr uint32_t F_PEND_C = 64u;       // caustic photon estimate (no shadow r
 constexpr uint32_t F_TRAIN = 128u;       // radiance-cache training path: full depth, records its vertices
 constexpr uint32_t F_PEND_ALL = F_PEND_A | F_PEND_E | F_PEND_P | F_PEND_C;
 
+// Segmented queues. A bounce's ray queue (and its shadow queue) is split into
+// kQSegs segments: segment k holds its count[k] entries at
+// [k * cap, k * cap + count[k]). A producer wave appends its rays and shadow
+// rays to the segment of its input position, with one 64-bit atomic on that
+// segment's counter pair (appends of every wave of the GPU to one counter
+// serialise on that address, ~9 ns per wave, which bounded shading).
+// Consumers see the queue as the dense range [0, total) and find an entry
+// through the prefix of the counts (kQueueScan writes it, with the total,
+// after each shading pass). Queues written in order (camera rays) carry the
+// prefix of a queue filled segment by segment, which maps w to entry w.
+constexpr int kQSegs = 64;
+
 struct PathBufs {
     int64_t slots;           // capacity (paths per chunk)
     float4 *L;               // radiance so far (xyz)
@@ -35,8 +47,14 @@ struct PathBufs {
     uint32_t *sqSlot;        // shadow queue: slot | kind << 30
     float4 *sqOrg;           // origin, tmax
     float4 *sqDir;
-    uint32_t *qcnt;          // ray-queue count per bounce
+    uint32_t *qcnt;          // ray-queue count per bounce (total over its segments)
     uint32_t *sqcnt;         // shadow-queue count per bounce
+    unsigned long long *segc;  // per shading pass b and segment: entries appended to the ray queue of
+                               // b + 1 (low 32 bits) and to the shadow queue of b (high 32 bits)
+    uint32_t *qpre;          // kQSegs + 1 prefix counts per bounce (ray queues)
+    uint32_t *sqpre;         // the same for shadow queues
+    int64_t qcap;            // entries per ray-queue segment
+    int64_t sqcap;           // entries per shadow-queue segment
     uint32_t *fetch;         // persistent-kernel work counters: [2b] extend, [2b+1] shadow
     unsigned long long *stats;  // [0] closest rays, [1] any rays, [2] samples, [3] photons stored,
                                 // [4] closest rays traced by kFinish, [5] samples replaced by 0
@@ -131,5 +149,37 @@ struct AdaptiveBufs {
     int32_t *list;       // active pixel list
     uint32_t *count;     // active pixel count
 };
+
+// Entries per segment of a queue whose producer appends at most perEntry
+// entries for each of at most `entries` input positions. The waves that
+// append to segment k are those that shade input positions
+// [64 j, 64 j + 64) with j % kQSegs == k; one producer pass has at most
+// ceil(ceil(entries / 64) / kQSegs) of them. Two slack groups cover the two
+// list passes of material-sorted shading, whose positions both start at 0.
+inline int64_t segCapacity(int64_t entries, int perEntry) {
+    const int64_t groups = (entries + 63) / 64;
+    return ((groups + kQSegs - 1) / kQSegs + 2) * 64 * perEntry;
+}
+
+// Copies a queue's kQSegs + 1 prefix counts to LDS. Every thread of the
+// block calls it (it ends with a barrier).
+PRR_DEV void loadPrefix(uint32_t *spre, const uint32_t *pre) {
+    for (int k = threadIdx.x; k <= kQSegs; k += blockDim.x) spre[k] = pre[k];
+    __syncthreads();
+}
+
+// Entry index of dense position w (< total) of a segmented queue: the last
+// segment whose prefix is at most w (an empty segment's prefix equals the
+// next one's, so it is never chosen), then w's offset in it. Queue indices
+// fit in 32 bits (at most kQSegs * cap entries, a little over the 2^27 path
+// slots of a chunk), so the arithmetic is 32-bit.
+PRR_DEV int64_t segEntry(const uint32_t *spre, int64_t cap, int64_t w) {
+    const uint32_t w32 = (uint32_t) w;
+    uint32_t k = 0;
+#pragma unroll
+    for (uint32_t step = kQSegs / 2; step > 0; step >>= 1)
+        if (spre[k + step] <= w32) k += step;
+    return (int64_t) (k * (uint32_t) cap + (w32 - spre[k]));
+}
 
 }  // namespace prr
