@@ -55,6 +55,15 @@ PRR_DEV uint32_t trainHash(uint32_t p, uint32_t s) {
     return h;
 }
 
+// Flags of a path at its first bounce: a radiance-cache training path when
+// the render trains the cache and the hash of its pixel and sample selects it.
+PRR_DEV uint32_t pathFlags0(const ChunkInfo &ci, uint32_t slot, uint32_t sIdx) {
+    if (!ci.cache.train) return 0u;
+    const int e = (int) (slot % (uint32_t) ci.pixels);
+    const int p = ci.pixList ? ci.pixList[e] : e;
+    return (trainHash((uint32_t) p, sIdx) & ci.cache.trainMask) == 0u ? F_TRAIN : 0u;
+}
+
 // One radiance-cache query / training record: position, wo, shading normal,
 // roughness, albedo (the cache's 13 query floats).
 PRR_DEV void putQuery(float *q, const V3 &P, const V3 &wo, const V3 &n, float rough, const V3 &alb) {
@@ -123,11 +132,6 @@ __global__ __launch_bounds__(kShadeBlock) void kRaygen(SceneView s, PathBufs pb,
         const float oy = rng.next();
         V3 o, d;
         cameraRay(s.cam, p % ci.W, p / ci.W, ci.W, ci.H, ox, oy, rng, o, d);
-        pb.L[slot] = make_float4(0.f, 0.f, 0.f, 0.f);
-        pb.beta[slot] = make_float4(1.f, 1.f, 1.f, 0.f);
-        pb.flags[slot] = ci.cache.train && (trainHash((uint32_t) p, (uint32_t) sIdx) & ci.cache.trainMask) == 0u
-                             ? F_TRAIN
-                             : 0u;
         pb.rng[slot] = rng.state;
         pb.qSlot[0][slot] = (uint32_t) slot;
         pb.qOrg[0][slot] = st4(o, 0.f);
@@ -165,9 +169,6 @@ __global__ __launch_bounds__(kShadeBlock) void kRaygenSeg(PathBufs pb, const Seg
         const float oy = rng.next();
         V3 o, d;
         cameraRay(sg.cam, p % sg.W, p / sg.W, sg.W, sg.H, ox, oy, rng, o, d);
-        pb.L[slot] = make_float4(0.f, 0.f, 0.f, 0.f);
-        pb.beta[slot] = make_float4(1.f, 1.f, 1.f, 0.f);
-        pb.flags[slot] = 0u;
         pb.rng[slot] = rng.state;
         sampleOf[slot] = (uint32_t) sIdx;
         pb.qSlot[0][slot] = (uint32_t) slot;
@@ -208,9 +209,6 @@ __global__ __launch_bounds__(kShadeBlock) void kRaygenRays(PathBufs pb, int64_t 
     for (int64_t slot = (int64_t) blockIdx.x * kShadeBlock + threadIdx.x; slot < n; slot += stride) {
         const int64_t k = first + slot;
         const Pcg rng = Pcg::make(seed, (uint32_t) k, 0u);
-        pb.L[slot] = make_float4(0.f, 0.f, 0.f, 0.f);
-        pb.beta[slot] = make_float4(1.f, 1.f, 1.f, 0.f);
-        pb.flags[slot] = 0u;
         pb.rng[slot] = rng.state;
         pb.qSlot[0][slot] = (uint32_t) slot;
         pb.qOrg[0][slot] = make_float4(org[3 * k], org[3 * k + 1], org[3 * k + 2], 0.f);
@@ -355,9 +353,10 @@ struct HitGeom {
     int mat;
     int flags;
 };
-PRR_DEV HitGeom hitGeom(const SceneView &s, int prim, float t, const V3 &o, const V3 &d, float u, float v) {
+// pn: the hit primitive's prim_n row.
+PRR_DEV HitGeom hitGeom(const SceneView &s, int prim, const float4 &pn, float t, const V3 &o, const V3 &d, float u,
+                        float v) {
     HitGeom g;
-    const float4 pn = s.prim_n[prim];
     const int bits = __float_as_int(pn.w);
     g.mat = bits & kPrimMatMask;
     g.flags = bits >> kPrimFlagShift;
@@ -381,18 +380,9 @@ struct ShadeOut {
     float tA = 0.f, tP = 0.f;
 };
 
-// Where shading finds the ray a path traced and its closest hit (t, u, v,
-// prim bits; prim -1 for a miss or a terminating entry): the queue entry and
-// the hit record of the wavefront kernels, read where shading needs them, or
-// values a kernel already holds.
-struct QueueRay {
-    const PathBufs &pb;
-    int q;
-    int64_t i;
-    PRR_DEV float4 hit() const { return pb.hit[i]; }
-    PRR_DEV V3 org() const { return ld3(pb.qOrg[q][i]); }
-    PRR_DEV V3 dir() const { return ld3(pb.qDir[q][i]); }
-};
+// The ray a path traced and its closest hit (t, u, v, prim bits; prim -1 for
+// a miss or a terminating entry), as read from the queue entry and the hit
+// record, or as a kernel holds them.
 struct HeldRay {
     float4 h, o4, d4;
     PRR_DEV float4 hit() const { return h; }
@@ -415,37 +405,61 @@ template <bool PHOTON, bool CACHE, int KIND, class Ray>
 __device__ __forceinline__ void shadePath(const SceneView &s, const PathBufs &pb, const ChunkInfo &ci, int bounce,
                                           uint32_t slot, const Ray &ray, ShadeOut &so) {
     bool done = false;
-    uint32_t fl = pb.flags[slot];
-    V3 L = ld3(pb.L[slot]);
-    V3 beta = ld3(pb.beta[slot]);
-    const int sIdx = ci.sampleOf ? (int) ci.sampleOf[slot] : ci.sampleBase + (int) (slot / (uint32_t) ci.pixels);
-    Pcg rng;
-    rng.state = pb.rng[slot];
-    rng.inc = pcgInc((uint32_t) sIdx);
-    // 1. NEE of the previous bounce, in reference order (a term and its
-    // visibility are read together: both wait only for the flags)
-    if (fl & F_PEND_A) {
-        const float4 c = pb.pend[slot];
-        if (pb.vis[slot]) L += ld3(c);
-    }
-    if (fl & F_PEND_E) {
-        const float4 c = pb.pend[pb.slots + slot];
-        if (pb.vis[pb.slots + slot]) L += ld3(c);
-    }
-    if (fl & F_PEND_P) {
-        const float4 c = pb.pend[2 * pb.slots + slot];
-        if (pb.vis[2 * pb.slots + slot]) L += ld3(c);
-    }
-    if (fl & F_PEND_C) L += ld3(pb.pend[3 * pb.slots + slot]);
-    fl &= ~F_PEND_ALL;
-    done = (fl & F_TERMINATING) != 0u;
     const float4 h = ray.hit();
     const int prim = __float_as_int(h.w);
     const V3 o = ray.org(), d = ray.dir();
+    // Every load that depends only on the slot and the hit is issued here,
+    // together, ahead of the branches that use it: the path state, the hit
+    // primitive's row and, past bounce 0, the pending NEE terms of the kinds
+    // the scene has and the BSDF factor (each would otherwise wait for the
+    // flags, one dependent round trip more).
+    const float4 pn = prim >= 0 ? s.prim_n[prim] : make_float4(0.f, 0.f, 0.f, 0.f);
+    const int sIdx = ci.sampleOf ? (int) ci.sampleOf[slot] : ci.sampleBase + (int) (slot / (uint32_t) ci.pixels);
+    uint32_t fl;
+    V3 L, beta;
+    if (bounce == 0) {
+        // a path's first bounce starts from L = 0, beta = 1 (Integrator.cpp:
+        // castRay's locals); the camera-ray kernels store only the ray and
+        // the random-number state
+        fl = pathFlags0(ci, slot, (uint32_t) sIdx);
+        L = V3(0.f);
+        beta = V3(1.f);
+    } else {
+        fl = pb.flags[slot];
+        L = ld3(pb.L[slot]);
+        beta = ld3(pb.beta[slot]);
+    }
+    Pcg rng;
+    rng.state = pb.rng[slot];
+    rng.inc = pcgInc((uint32_t) sIdx);
+    float4 cA = make_float4(0.f, 0.f, 0.f, 0.f), cE = cA, cP = cA, cC = cA, bs4 = cA;
+    uint8_t vA = 0, vE = 0, vP = 0;
+    if (bounce > 0) {
+        if (s.total_emit_area > 0.f) {
+            cA = pb.pend[slot];
+            vA = pb.vis[slot];
+        }
+        if (s.env_w > 0) {
+            cE = pb.pend[pb.slots + slot];
+            vE = pb.vis[pb.slots + slot];
+        }
+        if (s.num_points > 0) {
+            cP = pb.pend[2 * pb.slots + slot];
+            vP = pb.vis[2 * pb.slots + slot];
+        }
+        if (PHOTON && s.ph_count > 0) cC = pb.pend[3 * pb.slots + slot];
+        bs4 = pb.bscale[slot];
+    }
+    // 1. NEE of the previous bounce, in reference order
+    if ((fl & F_PEND_A) && vA) L += ld3(cA);
+    if ((fl & F_PEND_E) && vE) L += ld3(cE);
+    if ((fl & F_PEND_P) && vP) L += ld3(cP);
+    if (fl & F_PEND_C) L += ld3(cC);
+    fl &= ~F_PEND_ALL;
+    done = (fl & F_TERMINATING) != 0u;
     // 2. the BSDF-sampled ray of the previous (diffuse) bounce
     if (!done && (fl & F_PREV_DIFF)) {
         const bool ind = bounce - 1 > 0;
-        const float4 bs4 = pb.bscale[slot];
         const V3 bscale = ld3(bs4);
         const float bpdf = bs4.w;
         if (prim < 0) {  // Integrator.cpp:217-229
@@ -460,7 +474,7 @@ __device__ __forceinline__ void shadePath(const SceneView &s, const PathBufs &pb
             }
             done = true;
         } else {
-            const HitGeom g = hitGeom(s, prim, h.x, o, d, h.y, h.z);
+            const HitGeom g = hitGeom(s, prim, pn, h.x, o, d, h.y, h.z);
             const DMaterial hm = s.mats[g.mat];
             if ((int) hm.base_type.w == MAT_EMISSIVE) {  // Integrator.cpp:231-242
                 const float cosT = smax(0.f, dot(-d, normalized(g.Ng)));
@@ -502,7 +516,7 @@ __device__ __forceinline__ void shadePath(const SceneView &s, const PathBufs &pb
         if (prim < 0) {
             done = true;
         } else {
-            const HitGeom g = hitGeom(s, prim, h.x, o, d, h.y, h.z);
+            const HitGeom g = hitGeom(s, prim, pn, h.x, o, d, h.y, h.z);
             const DMaterial m = s.mats[g.mat];
             const int mtype = (int) m.base_type.w;
             const V3 base = ld3(m.base_type);
@@ -697,8 +711,10 @@ __device__ __forceinline__ void shadeImpl(const SceneView &s, const PathBufs &pb
         ShadeOut so;
         if (i < n) {
             const int64_t e = list ? (int64_t) list[i] : segEntry(spre, pb.qcap, i);
-            slot = pb.qSlot[q][e];
-            shadePath<PHOTON, CACHE, KIND>(s, pb, ci, bounce, slot, QueueRay{pb, q, e}, so);
+            // queue 0 holds the camera rays in slot order (entry e is slot e)
+            slot = bounce == 0 ? (uint32_t) e : pb.qSlot[q][e];
+            const HeldRay ray{pb.hit[e], pb.qOrg[q][e], pb.qDir[q][e]};
+            shadePath<PHOTON, CACHE, KIND>(s, pb, ci, bounce, slot, ray, so);
         }
         const bool emitRay = so.emitRay, termEntry = so.termEntry, emitA = so.emitA, emitE = so.emitE,
                    emitP = so.emitP;
