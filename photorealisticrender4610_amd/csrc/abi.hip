@@ -103,7 +103,6 @@ int ensureRenderBufs(prr_context *c, int64_t slots, int maxDepth) {
     };
     A(&p.L, slots);
     A(&p.beta, slots);
-    A(&p.bscale, slots);
     A(&p.prevP, slots);
     A(&p.pend, 4 * slots);
     A(&p.rng, slots);

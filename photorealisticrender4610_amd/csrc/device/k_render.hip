@@ -417,6 +417,7 @@ __device__ __forceinline__ void shadePath(const SceneView &s, const PathBufs &pb
     const int sIdx = ci.sampleOf ? (int) ci.sampleOf[slot] : ci.sampleBase + (int) (slot / (uint32_t) ci.pixels);
     uint32_t fl;
     V3 L, beta;
+    float bpdf = 0.f;  // BSDF pdf of the ray a diffuse vertex sampled (F_PREV_DIFF)
     if (bounce == 0) {
         // a path's first bounce starts from L = 0, beta = 1 (Integrator.cpp:
         // castRay's locals); the camera-ray kernels store only the ray and
@@ -427,12 +428,14 @@ __device__ __forceinline__ void shadePath(const SceneView &s, const PathBufs &pb
     } else {
         fl = pb.flags[slot];
         L = ld3(pb.L[slot]);
-        beta = ld3(pb.beta[slot]);
+        const float4 b4 = pb.beta[slot];
+        beta = ld3(b4);
+        bpdf = b4.w;
     }
     Pcg rng;
     rng.state = pb.rng[slot];
     rng.inc = pcgInc((uint32_t) sIdx);
-    float4 cA = make_float4(0.f, 0.f, 0.f, 0.f), cE = cA, cP = cA, cC = cA, bs4 = cA;
+    float4 cA = make_float4(0.f, 0.f, 0.f, 0.f), cE = cA, cP = cA, cC = cA;
     uint8_t vA = 0, vE = 0, vP = 0;
     if (bounce > 0) {
         if (s.total_emit_area > 0.f) {
@@ -448,7 +451,6 @@ __device__ __forceinline__ void shadePath(const SceneView &s, const PathBufs &pb
             vP = pb.vis[2 * pb.slots + slot];
         }
         if (PHOTON && s.ph_count > 0) cC = pb.pend[3 * pb.slots + slot];
-        bs4 = pb.bscale[slot];
     }
     // 1. NEE of the previous bounce, in reference order
     if ((fl & F_PEND_A) && vA) L += ld3(cA);
@@ -457,19 +459,19 @@ __device__ __forceinline__ void shadePath(const SceneView &s, const PathBufs &pb
     if (fl & F_PEND_C) L += ld3(cC);
     fl &= ~F_PEND_ALL;
     done = (fl & F_TERMINATING) != 0u;
-    // 2. the BSDF-sampled ray of the previous (diffuse) bounce
+    // 2. the BSDF-sampled ray of the previous (diffuse) bounce; the stored
+    // throughput is already beta * betaScale (the product every use below
+    // starts with, Integrator.cpp:217-246)
     if (!done && (fl & F_PREV_DIFF)) {
         const bool ind = bounce - 1 > 0;
-        const V3 bscale = ld3(bs4);
-        const float bpdf = bs4.w;
         if (prim < 0) {  // Integrator.cpp:217-229
             if (s.env_w > 0) {
                 const V3 envL = envLookup(s, d);
                 const float w = mis(bpdf, envPdf(s, d));
-                const V3 c = beta * bscale * w * envL;
+                const V3 c = beta * w * envL;
                 L += ind ? clampInd(c, ci.clampInd) : c;
             } else {
-                const V3 c = beta * bscale * V3(s.background[0], s.background[1], s.background[2]);
+                const V3 c = beta * V3(s.background[0], s.background[1], s.background[2]);
                 L += ind ? clampInd(c, ci.clampInd) : c;
             }
             done = true;
@@ -478,17 +480,20 @@ __device__ __forceinline__ void shadePath(const SceneView &s, const PathBufs &pb
             const DMaterial hm = s.mats[g.mat];
             if ((int) hm.base_type.w == MAT_EMISSIVE) {  // Integrator.cpp:231-242
                 const float cosT = smax(0.f, dot(-d, normalized(g.Ng)));
-                const V3 dd = g.P - ld3(pb.prevP[slot]);
-                const float hd2 = dot(dd, dd);
-                const float lpA = s.total_emit_area > 0.f ? 1.f / s.total_emit_area : 0.f;
-                const float lpSA = lpA * hd2 / smax(cosT, 1e-4f);
+                // light pdf of the hit point (0 without area emitters: the
+                // diffuse vertex is then not stored)
+                float lpSA = 0.f;
+                if (s.total_emit_area > 0.f) {
+                    const V3 dd = g.P - ld3(pb.prevP[slot]);
+                    const float hd2 = dot(dd, dd);
+                    lpSA = (1.f / s.total_emit_area) * hd2 / smax(cosT, 1e-4f);
+                }
                 const float w = mis(bpdf, lpSA);
-                const V3 c = beta * bscale * w * ld3(hm.emission);
+                const V3 c = beta * w * ld3(hm.emission);
                 L += ind ? clampInd(c, ci.clampInd) : c;
                 done = true;
-            } else {
-                beta = beta * bscale;
-                if (!finite3(beta)) done = true;
+            } else if (!finite3(beta)) {  // beta = beta * betaScale (Integrator.cpp:244-246)
+                done = true;
             }
         }
         if (!done && bounce - 1 >= 3) {  // Russian roulette (Integrator.cpp:250-256)
@@ -659,8 +664,10 @@ __device__ __forceinline__ void shadePath(const SceneView &s, const PathBufs &pb
                 } else {
                     const V3 bscale = bs.f / bs.pdf;
                     const V3 off = dot(bs.wi, sN) >= 0.f ? sN : -sN;
-                    pb.bscale[slot] = st4(bscale, bs.pdf);
-                    pb.prevP[slot] = st4(g.P, 0.f);
+                    // the next bounce's throughput, beta * betaScale, and the pdf
+                    beta = beta * bscale;
+                    bpdf = bs.pdf;
+                    if (s.total_emit_area > 0.f) pb.prevP[slot] = st4(g.P, 0.f);
                     fl |= F_PREV_DIFF;
                     so.emitRay = true;
                     so.newO = g.P + off * kEps;
@@ -682,7 +689,7 @@ __device__ __forceinline__ void shadePath(const SceneView &s, const PathBufs &pb
         pb.result[slot] = st4(fin ? L : V3(0.f), fin ? 0.f : 1.f);
     } else {
         pb.L[slot] = st4(L, 0.f);
-        pb.beta[slot] = st4(beta, 0.f);
+        pb.beta[slot] = st4(beta, bpdf);
         pb.rng[slot] = rng.state;
     }
     pb.flags[slot] = fl;

@@ -32,9 +32,9 @@ constexpr int kQSegs = 64;
 struct PathBufs {
     int64_t slots;           // capacity (paths per chunk)
     float4 *L;               // radiance so far (xyz)
-    float4 *beta;            // throughput (xyz)
-    float4 *bscale;          // pending BSDF factor f/pdf (xyz), w = BSDF pdf
-    float4 *prevP;           // diffuse vertex that shot the BSDF ray (xyz)
+    float4 *beta;            // throughput (xyz; after a diffuse vertex, times its BSDF factor f/pdf),
+                             // w = that vertex's BSDF pdf
+    float4 *prevP;           // diffuse vertex that shot the BSDF ray (xyz; only with area emitters)
     float4 *pend;            // 4 * slots: area, env, point NEE and caustic-photon contributions
     uint64_t *rng;           // PCG32 state
     uint32_t *flags;
