@@ -106,7 +106,6 @@ int ensureRenderBufs(prr_context *c, int64_t slots, int maxDepth) {
     A(&p.prevP, slots);
     A(&p.pend, 4 * slots);
     A(&p.rng, slots);
-    A(&p.flags, slots);
     A(&p.vis, 3 * slots);
     A(&p.result, slots);
     // segmented queues: kQSegs segments of qcap (ray) / sqcap (shadow) entries

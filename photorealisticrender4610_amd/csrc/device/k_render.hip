@@ -426,8 +426,9 @@ __device__ __forceinline__ void shadePath(const SceneView &s, const PathBufs &pb
         L = V3(0.f);
         beta = V3(1.f);
     } else {
-        fl = pb.flags[slot];
-        L = ld3(pb.L[slot]);
+        const float4 l4 = pb.L[slot];
+        L = ld3(l4);
+        fl = __float_as_uint(l4.w);
         const float4 b4 = pb.beta[slot];
         beta = ld3(b4);
         bpdf = b4.w;
@@ -688,11 +689,10 @@ __device__ __forceinline__ void shadePath(const SceneView &s, const PathBufs &pb
         const bool fin = finite3(L);
         pb.result[slot] = st4(fin ? L : V3(0.f), fin ? 0.f : 1.f);
     } else {
-        pb.L[slot] = st4(L, 0.f);
+        pb.L[slot] = st4(L, __uint_as_float(fl));
         pb.beta[slot] = st4(beta, bpdf);
         pb.rng[slot] = rng.state;
     }
-    pb.flags[slot] = fl;
     so.termEntry = (fl & F_TERMINATING) != 0u;
 }
 

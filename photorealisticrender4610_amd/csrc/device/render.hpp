@@ -31,13 +31,12 @@ constexpr int kQSegs = 64;
 
 struct PathBufs {
     int64_t slots;           // capacity (paths per chunk)
-    float4 *L;               // radiance so far (xyz)
+    float4 *L;               // radiance so far (xyz), w = the path's flags (bits)
     float4 *beta;            // throughput (xyz; after a diffuse vertex, times its BSDF factor f/pdf),
                              // w = that vertex's BSDF pdf
     float4 *prevP;           // diffuse vertex that shot the BSDF ray (xyz; only with area emitters)
     float4 *pend;            // 4 * slots: area, env, point NEE and caustic-photon contributions
     uint64_t *rng;           // PCG32 state
-    uint32_t *flags;
     uint8_t *vis;            // 3 * slots: visibility of the pending NEE rays
     float4 *result;          // final L per slot
     uint32_t *qSlot[2];      // ray queue (ping-pong)
