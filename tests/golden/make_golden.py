@@ -4,7 +4,7 @@ Runs in the development container only: builds oracle/_ref/ref_harness from
 the reference's own sources (oracle/Makefile target ``ref``), feeds it the
 seeded inputs below, and stores inputs + reference outputs together, so the
 tests never need the reference. Usage:
-python tests/golden/make_golden.py [trace bsdf env camera tonemap castray photon image image_field welford hdr gltf
+python tests/golden/make_golden.py [trace bsdf env camera tonemap castray textured textured_edges photon image image_field welford hdr gltf
 volume jpeg]
 (no names: all).
 """
@@ -804,6 +804,8 @@ def main():
         gen_castray(field, 320, 240, 12, 3000, 202)
     if want("castray") or want("textured"):
         gen_castray(TS.textured_room(), 256, 256, 8, 4000, 203)
+    if want("castray") or want("textured_edges"):
+        gen_castray(TS.textured_edges(), 256, 256, 8, 4000, 204)
     if want("photon"):
         gen_photon(box, 3000, 0.05, 30, 211)
         gen_photon(field, 12000, 0.2, 30, 212)

@@ -184,13 +184,20 @@ def attr_sphere(center, radius, segments=12, bands=8):
     return f32(tris), f32(nrm), f32(uvs), f32(tan)
 
 
-def textured_room(seed=17):
+def textured_room(seed=17, edges=False):
     """box_room with glTF-style content: a textured floor (sRGB colour
     checker, metallic-roughness and normal maps, tangents), a smooth-normal
     GGX sphere mesh with a metallic-roughness map and a normal map without
     tangents (buildTBN frame), a smooth-normal glass sphere mesh tinted by a
     colour texture, a textured analytic sphere (uv 0), and a ceiling light
-    whose per-vertex normals tilt (light-sample normal normalize(n0+n1+n2))."""
+    whose per-vertex normals tilt (light-sample normal normalize(n0+n1+n2)).
+
+    edges=True (textured_edges): the tangent frame's edge cases under the
+    normal maps (Material.hpp:62-74). The floor's left half has tangents
+    parallel to its normal (Gram-Schmidt leaves nothing: the buildTBN
+    fallback); its right half has handedness -1 on one triangle and mixed
+    signs on the other (the interpolated w changes sign inside it); the GGX
+    sphere carries tangents whose handedness alternates per column."""
     rng = np.random.default_rng(seed)
     b = S.SceneBuilder()
     b.background = np.zeros(3, np.float32)
@@ -211,11 +218,21 @@ def textured_room(seed=17):
     glass_m = b.material(S.MAT_GLASS, (1, 1, 1), 0.5, 0.0, 1.5, textures=(t_tint, -1, -1, -1))
     ball_m = b.material(S.MAT_DIFFUSE, (0.5, 0.5, 0.5), 0.5, 0.0, 1.5, textures=(t_col, -1, -1, -1))
     light = b.material(S.MAT_EMISSIVE, (0.8, 0.8, 0.8), 0.5, 0.0, 1.5, (17.0, 12.0, 4.0))
-    fl = S.quad((-1, 0, -1), (-1, 0, 1), (1, 0, 1), (1, 0, -1))
-    fl_uv = np.array([[0, 0, 0, 3, 3, 3], [0, 0, 3, 3, 3, 0]], np.float32)
     fl_n = np.tile(np.array([0, 1, 0], np.float32), (2, 3))
-    fl_t = np.tile(np.array([1, 0, 0, 1], np.float32), (2, 3))
-    b.mesh(fl, floor_m, normals=fl_n, uvs=fl_uv, tangents=fl_t)
+    if not edges:
+        fl = S.quad((-1, 0, -1), (-1, 0, 1), (1, 0, 1), (1, 0, -1))
+        fl_uv = np.array([[0, 0, 0, 3, 3, 3], [0, 0, 3, 3, 3, 0]], np.float32)
+        fl_t = np.tile(np.array([1, 0, 0, 1], np.float32), (2, 3))
+        b.mesh(fl, floor_m, normals=fl_n, uvs=fl_uv, tangents=fl_t)
+    else:
+        def uv_of(q):
+            return np.stack([1.5 * (q[:, 0::3] + 1), 1.5 * (q[:, 2::3] + 1)], 2).reshape(-1, 6).astype(np.float32)
+        left = S.quad((-1, 0, -1), (-1, 0, 1), (0, 0, 1), (0, 0, -1))
+        right = S.quad((0, 0, -1), (0, 0, 1), (1, 0, 1), (1, 0, -1))
+        t_par = np.tile(np.array([0, 1, 0, 1], np.float32), (2, 3))
+        t_neg = np.array([[0.8, 0, 0.6, -1] * 3, [1, 0, 0, 1, 1, 0, 0, -1, 1, 0, 0, -1]], np.float32)
+        b.mesh(left, floor_m, normals=fl_n, uvs=uv_of(left), tangents=t_par)
+        b.mesh(right, floor_m, normals=fl_n, uvs=uv_of(right), tangents=t_neg)
     b.mesh(S.quad((-1, 2, -1), (1, 2, -1), (1, 2, 1), (-1, 2, 1)), grey)
     b.mesh(S.quad((-1, 0, -1), (1, 0, -1), (1, 2, -1), (-1, 2, -1)), grey)
     b.mesh(S.quad((-1, 0, -1), (-1, 2, -1), (-1, 2, 1), (-1, 0, 1)), red)
@@ -225,9 +242,13 @@ def textured_room(seed=17):
     ln /= np.linalg.norm(ln, axis=1, keepdims=True)
     b.mesh(lq, light, normals=np.tile(ln.ravel(), (2, 1)))
     t, n, uv, tg = attr_sphere((-0.45, 0.4, -0.3), 0.4)
-    b.mesh(t, ggx_m, normals=n, uvs=uv)
+    b.mesh(t, ggx_m, normals=n, uvs=uv, tangents=tg if edges else None)
     t, n, uv, tg = attr_sphere((0.45, 0.4, 0.25), 0.4)
     b.mesh(t, glass_m, normals=n, uvs=uv, tangents=tg)
     b.sphere((0.3, 1.5, -0.5), 0.2, ball_m)
     b.set_camera((0, 1, 3.9), (0, 1, 0), (0, 1, 0), 40.0)
-    return b.build("textured_room")
+    return b.build("textured_edges" if edges else "textured_room")
+
+
+def textured_edges(seed=17):
+    return textured_room(seed, edges=True)

@@ -38,12 +38,44 @@ def test_cast_rays_textured_vs_oracle(ctx):
     g = ctx.cast_rays(scene, org, d, 8, seed=11).cpu().numpy()
     o = O.OracleScene(sc).castray_pcg(org, d, 11, 8)
     close = np.all(np.abs(g - o) <= 1e-4 * np.maximum(np.abs(o), 1e-2), axis=1)
-    assert close.mean() > 0.98, close.mean()
+    assert close.mean() > 0.995, close.mean()
     assert (o.sum(1) > 0).mean() > 0.5
     # without its textures the device image is another one
     sc.mat_tex = sc.textures = None
     g2 = ctx.cast_rays(ctx.scene(sc), org, d, 8, seed=11).cpu().numpy()
     assert np.mean(np.all(np.abs(g2 - o) <= 1e-4 * np.maximum(np.abs(o), 1e-2), axis=1)) < 0.9
+
+
+@pytest.mark.parametrize("which", ["textured_room", "textured_edges"])
+def test_cast_rays_textured_depth1_and_totals(ctx, which):
+    """Bounds on the paths that leave the 1e-4 band of the test above. At
+    depth 1 (camera hit, its NEE and emission; no bounce for a branch flip to
+    compound over) every path agrees; at depth 8 all but 0.1 % do and the
+    summed radiance of all rays agrees to 1e-5 per channel (measured: 100 %,
+    99.98 %, ~1e-8), so a rarely taken branch (the
+    degenerate-tangent fallback, handedness -1, uv wrap at texel edges)
+    cannot hide in the unmatched share. textured_edges holds those tangent
+    cases under normal maps; the oracle is pinned to the reference's castRay
+    on it (test_oracle_golden.py)."""
+    import testscenes as TS
+    print("case cast_rays depth1/totals", which, flush=True)
+    sc = getattr(TS, which)()
+    scene = ctx.scene(sc)
+    orc = O.OracleScene(sc)
+    org, d = _camera_rays(40000, 5)
+    g1 = ctx.cast_rays(scene, org, d, 1, seed=13).cpu().numpy()
+    o1 = orc.castray_pcg(org, d, 13, 1)
+    close1 = np.all(np.abs(g1 - o1) <= 1e-4 * np.maximum(np.abs(o1), 1e-2), axis=1)
+    g8 = ctx.cast_rays(scene, org, d, 8, seed=13).cpu().numpy()
+    o8 = orc.castray_pcg(org, d, 13, 8)
+    close8 = np.all(np.abs(g8 - o8) <= 1e-4 * np.maximum(np.abs(o8), 1e-2), axis=1)
+    rel = np.abs(g8.astype(np.float64).sum(0) - o8.astype(np.float64).sum(0)) / o8.astype(np.float64).sum(0)
+    print("depth 1 within 1e-4: %.5f; depth 8: %.5f; summed radiance rel. diff %s" % (close1.mean(), close8.mean(), rel),
+          flush=True)
+    assert close1.mean() >= 0.9995, close1.mean()
+    assert close8.mean() >= 0.999, close8.mean()
+    assert np.all(rel <= 1e-5), rel
+    assert (o1.sum(1) > 0).mean() > 0.5
 
 
 def test_render_textured_finish_equals_wavefront(monkeypatch):
@@ -113,5 +145,5 @@ def test_cornell_glb_scene_vs_oracle(ctx, tmp_path):
     g = ctx.cast_rays(scene, org, d, 8, seed=13).cpu().numpy()
     o = O.OracleScene(sc).castray_pcg(org, d, 13, 8)
     close = np.all(np.abs(g - o) <= 1e-4 * np.maximum(np.abs(o), 1e-2), axis=1)
-    assert close.mean() > 0.98, close.mean()
+    assert close.mean() > 0.995, close.mean()
     assert (o.sum(1) > 0).mean() > 0.5

@@ -213,7 +213,8 @@ def test_tonemap_closed_forms():
     assert np.allclose(disp[:, 0], oetf, atol=2e-6)
 
 
-@pytest.mark.parametrize("name", ["castray_box_room.npz", "castray_mini_field.npz", "castray_textured_room.npz"])
+@pytest.mark.parametrize("name", ["castray_box_room.npz", "castray_mini_field.npz", "castray_textured_room.npz",
+                                  "castray_textured_edges.npz"])
 def test_castray_matches_reference(name):
     """Full per-path parity with the reference castRay on identical
     SplitMix64 streams: same draws consumed in the same order."""
@@ -242,6 +243,33 @@ def test_castray_textured_features_matter():
         L = O.OracleScene(sc).castray_splitmix(w, h, g["pix"], g["seeds"], depth)
         same = np.all(np.abs(L - ref) <= 1e-5 * np.maximum(np.abs(ref), 1e-3), axis=1)
         assert same.mean() < 0.9, (strip, same.mean())
+
+
+def test_castray_tangent_edges_matter():
+    """The tangent edge-case fixture reaches its cases: paths meet the floor
+    half whose tangents are parallel to its normal (the buildTBN fallback,
+    Material.hpp:67-70) and the tangents of handedness -1 and of mixed sign
+    (Triangle.cpp:52-54); flipping every handedness, or giving the
+    degenerate half a usable tangent, moves the reference's radiance on many
+    paths, so the oracle's treatment of each is pinned by the fixture."""
+    g = load_golden("castray_textured_edges.npz")
+    w, h, depth = (int(x) for x in g["cfg"])
+    ref = g["ref_L"].reshape(-1, 3)
+    L = O.OracleScene(fixture_scene(g)).castray_splitmix(w, h, g["pix"], g["seeds"], depth)
+    base = np.all(np.abs(L - ref) <= 1e-5 * np.maximum(np.abs(ref), 1e-3), axis=1)
+    for change in ("flip", "tangent"):
+        sc = fixture_scene(g)
+        tt = sc.tri_tangents.reshape(-1, 3, 4).copy()
+        if change == "flip":
+            tt[..., 3] = -tt[..., 3]
+        else:
+            par = np.all(np.abs(tt[..., :3] - np.array([0, 1, 0], np.float32)) == 0, axis=2).all(1)
+            assert par.sum() == 2
+            tt[par, :, :3] = np.array([1, 0, 0], np.float32)
+        sc.tri_tangents = tt.reshape(sc.tri_tangents.shape)
+        L2 = O.OracleScene(sc).castray_splitmix(w, h, g["pix"], g["seeds"], depth)
+        same = np.all(np.abs(L2 - ref) <= 1e-5 * np.maximum(np.abs(ref), 1e-3), axis=1)
+        assert same.mean() < base.mean() - 0.02, (change, same.mean(), base.mean())
 
 
 def test_welford_stop_rule_matches_reference():
