@@ -107,7 +107,7 @@ def reference_baseline(exe, sc, w, h, depth, target_s=12.0):
                 "image at %d spp: %d samples, %d rays (%d closest, %d any-hit) in %.1f s"
                 % (w, h, spp, samples, rays, closest, anyq, sec))
     return {"value": round(rays / sec / 1e6, 3), "unit": "Mrays/s", "cores": used, "kind": "reference",
-            "cpu_model": cpu_model(),
+            "msamples_per_s": round(samples / sec / 1e6, 3), "cpu_model": cpu_model(),
             "sample": "the reference's own castRay / BVH / Render loop (%s); its scene build (%.1f s) excluded"
                       % (what, build_s)}
 
@@ -142,7 +142,7 @@ def cpu_baseline(sc, w, h, depth, seed, target_s=12.0):
     dt = time.time() - t0
     rays = cnt["closest"] + cnt["any"]
     return {"value": round(rays / dt / 1e6, 3), "unit": "Mrays/s", "cores": threads, "kind": "port",
-            "cpu_model": cpu_model(),
+            "msamples_per_s": round(cnt["samples"] / dt / 1e6, 3), "cpu_model": cpu_model(),
             "sample": "rows %d-%d of the %dx%d image at %d spp (%d samples, %d rays) in %.1f s; "
                       "oracle scene build %.1f s excluded" % (r0, r0 + rows, w, h, spp, cnt["samples"], rays, dt,
                                                               build_s)}
@@ -478,7 +478,7 @@ def main():
         ctx.tonemap("aces", out.view(-1, 3), 1.0)
         log("warmup %d: %.1f ms, %d rays" % (k, st["ms"], st["rays_closest"] + st["rays_any"]))
     torch.cuda.synchronize()
-    rays = closest = 0
+    rays = closest = skipped = 0
     ms_ext = 0.0
     launches = 0
     t0 = time.perf_counter()
@@ -492,6 +492,7 @@ def main():
                            photons=photons)
         ctx.tonemap("aces", out.view(-1, 3), 1.0)
         rays += st["rays_closest"] + st["rays_any"]
+        skipped += st["rays_any_skipped"]  # NEE terms exactly zero: shadow rays not traced
         closest += st["rays_closest"] - st["rays_closest_finish"]  # the rays kExtend traced
         anyr += st["rays_any"] - st["rays_any_finish"]  # the rays kShadow traced
         ms_ext += st["ms_extend"]
@@ -544,6 +545,9 @@ def main():
                    "triangles": sc.num_triangles},
         "msamples_per_s": round(W * H * SPP * args.steps / dt / 1e6, 2),
         "rays_per_step": rays // args.steps,
+        # shadow queries of the reference's loop whose NEE term is exactly zero
+        # (the sample cannot depend on them): not traced, not counted in value
+        "shadow_queries_skipped_per_step": skipped // args.steps,
         "photons_stored": stored if photons else None,
         "roofline": roofline,
         "roofline_kernels": [rl_sh, rl_ext],
@@ -551,6 +555,8 @@ def main():
     }
     if cpu:
         line["gpu_over_cpu"] = round(value / cpu["value"], 1)
+        if cpu.get("msamples_per_s"):  # the same image work on both sides
+            line["gpu_over_cpu_samples"] = round(line["msamples_per_s"] / cpu["msamples_per_s"], 1)
     print(json.dumps(line), flush=True)
 
 

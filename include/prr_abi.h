@@ -203,6 +203,11 @@ typedef struct prr_stats {
     uint64_t nan_samples;
     uint64_t rays_any_finish;     /* shadow rays traced by path completion (part of rays_any) */
     double ms_sort;               /* shadow-ray binning before traversal (with PRR_RENDER_KERNEL_TIMING) */
+    /* NEE queries the reference makes whose term is exactly zero in every
+       channel (the BSDF vanishes for the light direction): the sample cannot
+       depend on their answer, so their shadow rays are not traced (not part
+       of rays_any) */
+    uint64_t rays_any_skipped;
 } prr_stats;
 
 typedef struct prr_scene_info {

@@ -87,7 +87,8 @@ class Stats(C.Structure):
                 ("ms_extend", C.c_double), ("ms_shadow", C.c_double), ("ms_shade", C.c_double),
                 ("launches_extend", C.c_int64), ("launches_shadow", C.c_int64),
                 ("rays_closest_finish", C.c_uint64), ("ms_finish", C.c_double), ("launches_finish", C.c_int64),
-                ("nan_samples", C.c_uint64), ("rays_any_finish", C.c_uint64), ("ms_sort", C.c_double)]
+                ("nan_samples", C.c_uint64), ("rays_any_finish", C.c_uint64), ("ms_sort", C.c_double),
+                ("rays_any_skipped", C.c_uint64)]
 
 
 class SceneInfo(C.Structure):
@@ -626,7 +627,8 @@ class Context:
                "ms_shadow": st.ms_shadow, "ms_shade": st.ms_shade, "launches_extend": st.launches_extend,
                "launches_shadow": st.launches_shadow, "rays_closest_finish": st.rays_closest_finish,
                "ms_finish": st.ms_finish, "launches_finish": st.launches_finish, "nan_samples": st.nan_samples,
-               "rays_any_finish": st.rays_any_finish, "ms_sort": st.ms_sort}
+               "rays_any_finish": st.rays_any_finish, "ms_sort": st.ms_sort,
+               "rays_any_skipped": st.rays_any_skipped}
         if cnt is not None:
             res["spp"] = cnt
         return out, res
@@ -756,7 +758,8 @@ class Cache:
         c._bind_stream()
         c._done(lib().prr_cache_render(c.h, scene.h, self.h, C.byref(p), C.byref(cp), _ptr(out), C.byref(st)))
         r = st.render
-        return out, {"rays_closest": r.rays_closest, "rays_any": r.rays_any, "samples": r.samples, "ms": r.ms_total,
+        return out, {"rays_closest": r.rays_closest, "rays_any": r.rays_any, "rays_any_skipped": r.rays_any_skipped,
+                     "samples": r.samples, "ms": r.ms_total,
                      "ms_extend": r.ms_extend, "ms_shadow": r.ms_shadow, "ms_shade": r.ms_shade,
                      "ms_finish": r.ms_finish, "queries": st.queries, "train_samples": st.train_samples, "train_steps": st.train_steps,
                      "fell_back": st.fell_back, "train_loss": st.train_loss, "diverged": st.diverged,

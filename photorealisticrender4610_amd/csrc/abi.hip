@@ -274,6 +274,7 @@ int finishStats(prr_context *c, const Timing &tm, uint64_t photons, prr_stats *s
     stats->rays_closest_finish = h[4];
     stats->nan_samples = h[5];
     stats->rays_any_finish = h[6];
+    stats->rays_any_skipped = h[7];
     stats->ms_extend = stats->ms_shadow = stats->ms_shade = stats->ms_finish = stats->ms_sort = 0.0;
     stats->launches_extend = stats->launches_shadow = stats->launches_finish = 0;
     for (const auto &t : tm.timed) {

@@ -378,7 +378,16 @@ struct ShadeOut {
     bool emitRay = false, termEntry = false, emitA = false, emitE = false, emitP = false;
     V3 newO, newD, sOA, sDA, sOE, sDE, sOP, sDP;
     float tA = 0.f, tP = 0.f;
+    uint32_t skipped = 0;  // NEE terms exactly zero: their shadow rays are not traced
 };
+
+// An NEE term that is exactly zero in every channel (the BSDF vanishes for
+// the light direction, e.g. below the surface) cannot change the sample: the
+// reference adds it to L only when its shadow ray is unoccluded, and L + 0 is
+// L (L starts at +0 and is never -0). Such a term's shadow ray is not traced
+// and the term is not stored. A NaN term compares unequal to zero and takes
+// its shadow ray as before.
+PRR_DEV bool zeroTerm(const V3 &c) { return c.x == 0.f && c.y == 0.f && c.z == 0.f; }
 
 // The ray a path traced and its closest hit (t, u, v, prim bits; prim -1 for
 // a miss or a terminating entry), as read from the queue entry and the hit
@@ -603,12 +612,16 @@ __device__ __forceinline__ void shadePath(const SceneView &s, const PathBufs &pb
                         const float w = mis(pdfSA, bp);
                         V3 c = beta * (w * ld3(s.mats[lm].emission) * f / pdfSA);
                         c = ind ? clampInd(c, ci.clampInd) : c;
-                        pb.pend[slot] = st4(c, 0.f);
-                        fl |= F_PEND_A;
-                        so.emitA = true;
-                        so.sOA = g.P + wi * kEps;
-                        so.sDA = wi;
-                        so.tA = dist - 2.f * kEps;
+                        if (zeroTerm(c)) {
+                            so.skipped++;
+                        } else {
+                            pb.pend[slot] = st4(c, 0.f);
+                            fl |= F_PEND_A;
+                            so.emitA = true;
+                            so.sOA = g.P + wi * kEps;
+                            so.sDA = wi;
+                            so.tA = dist - 2.f * kEps;
+                        }
                     }
                 }
                 // NEE environment (Integrator.cpp:39-66, 93-102)
@@ -624,11 +637,15 @@ __device__ __forceinline__ void shadePath(const SceneView &s, const PathBufs &pb
                         const float w = mis(pe, bp);
                         V3 c = beta * (w * Le * f / (pe + 1e-6f));
                         c = ind ? clampInd(c, ci.clampInd) : c;
-                        pb.pend[pb.slots + slot] = st4(c, 0.f);
-                        fl |= F_PEND_E;
-                        so.emitE = true;
-                        so.sOE = g.P + dir * kEps;
-                        so.sDE = dir;
+                        if (zeroTerm(c)) {
+                            so.skipped++;
+                        } else {
+                            pb.pend[pb.slots + slot] = st4(c, 0.f);
+                            fl |= F_PEND_E;
+                            so.emitE = true;
+                            so.sOE = g.P + dir * kEps;
+                            so.sDE = dir;
+                        }
                     }
                 }
                 // NEE point light (build extension; no draws)
@@ -643,12 +660,16 @@ __device__ __forceinline__ void shadePath(const SceneView &s, const PathBufs &pb
                     evalPdf(lb, fr, wi, wo, f, bp);
                     V3 c = beta * (V3(s.point_int[0], s.point_int[1], s.point_int[2]) * f / d2);
                     c = ind ? clampInd(c, ci.clampInd) : c;
-                    pb.pend[2 * pb.slots + slot] = st4(c, 0.f);
-                    fl |= F_PEND_P;
-                    so.emitP = true;
-                    so.sOP = g.P + wi * kEps;
-                    so.sDP = wi;
-                    so.tP = dist - 2.f * kEps;
+                    if (zeroTerm(c)) {
+                        so.skipped++;
+                    } else {
+                        pb.pend[2 * pb.slots + slot] = st4(c, 0.f);
+                        fl |= F_PEND_P;
+                        so.emitP = true;
+                        so.sOP = g.P + wi * kEps;
+                        so.sDP = wi;
+                        so.tP = dist - 2.f * kEps;
+                    }
                 }
                 // caustic photon estimate at camera / delta-reached vertices
                 // (build extension; no draws)
@@ -711,7 +732,8 @@ __device__ __forceinline__ void shadeImpl(const SceneView &s, const PathBufs &pb
     const int64_t i0 = (int64_t) blockIdx.x * kShadeBlock + threadIdx.x;
     // uniform trip count per wave so every lane reaches the appends
     const int64_t waveBase = i0 - (threadIdx.x & 63);
-    uint32_t nRays = 0, nShadow = 0;
+    // the wave's query counts (wave-uniform: sums of ballot popcounts)
+    uint32_t nRays = 0, nShadow = 0, nSkip = 0;
     for (int64_t wb = waveBase; wb < n; wb += stride) {
         const int64_t i = wb + (threadIdx.x & 63);
         uint32_t slot = 0;
@@ -735,6 +757,7 @@ __device__ __forceinline__ void shadeImpl(const SceneView &s, const PathBufs &pb
             const uint32_t nR = (uint32_t) __popcll(mR);
             const uint32_t nA = (uint32_t) __popcll(mA), nE = (uint32_t) __popcll(mE);
             const uint32_t total = nA + nE + (uint32_t) __popcll(mP);
+            nShadow += total;
             unsigned long long pair = 0;
             if (nR | total) {
                 if (laneId() == 0)
@@ -777,18 +800,16 @@ __device__ __forceinline__ void shadeImpl(const SceneView &s, const PathBufs &pb
                 }
             }
         }
-        nRays += (emitRay && !termEntry) ? 1u : 0u;
-        nShadow += (emitA ? 1u : 0u) + (emitE ? 1u : 0u) + (emitP ? 1u : 0u);
+        nRays += (uint32_t) __popcll(__ballot(emitRay && !termEntry));
+        const uint32_t sk = so.skipped;
+        nSkip += (uint32_t) (__popcll(__ballot(sk >= 1u)) + __popcll(__ballot(sk >= 2u)) + __popcll(__ballot(sk >= 3u)));
     }
     // query counts: one atomic per wave and launch (contended device-scope
     // atomics per wave-iteration would serialise the kernel)
-    for (int o = 32; o > 0; o >>= 1) {
-        nRays += __shfl_xor(nRays, o);
-        nShadow += __shfl_xor(nShadow, o);
-    }
     if (laneId() == 0) {
         if (nRays) atomicAdd(&pb.stats[0], (unsigned long long) nRays);
         if (nShadow) atomicAdd(&pb.stats[1], (unsigned long long) nShadow);
+        if (nSkip) atomicAdd(&pb.stats[7], (unsigned long long) nSkip);
     }
 }
 
@@ -998,7 +1019,7 @@ __global__ __launch_bounds__(kTraceBlock, 3) void kFinish(SceneView s, PathBufs 
     RefStack rst = makeStack<RefStack>(stk + stackDepth * kTraceBlock, spill + lane * kSpill, threadIdx.x, stackDepth);
     WaveRange wr;
     const int64_t waves = (int64_t) gridDim.x * (kTraceBlock / 64);
-    uint32_t nRays = 0, nShadow = 0, nTraced = 0;
+    uint32_t nRays = 0, nShadow = 0, nTraced = 0, nSkip = 0;
     // Only the first `lanes` lanes of a wave take paths: a wave waits at each
     // bounce for its slowest path, so when few paths remain they are spread
     // over the waves the GPU holds at once, fewer per wave (a power of two,
@@ -1032,6 +1053,7 @@ __global__ __launch_bounds__(kTraceBlock, 3) void kFinish(SceneView s, PathBufs 
             if (so.emitP) pb.vis[2 * pb.slots + slot] = occluded(s, so.sOP, so.sDP, so.tP, tree, wwCap, lst, rst) ? 0 : 1;
             nRays += (so.emitRay && !so.termEntry) ? 1u : 0u;
             nShadow += (so.emitA ? 1u : 0u) + (so.emitE ? 1u : 0u) + (so.emitP ? 1u : 0u);
+            nSkip += so.skipped;
             // the wavefront shades bounces 0..maxDepth, never the queue past it
             if (!so.emitRay || b >= ci.maxDepth) break;
             o4 = so.termEntry ? make_float4(0.f, 0.f, 0.f, -1.f) : st4(so.newO, 0.f);
@@ -1042,6 +1064,7 @@ __global__ __launch_bounds__(kTraceBlock, 3) void kFinish(SceneView s, PathBufs 
         nRays += __shfl_xor(nRays, o);
         nShadow += __shfl_xor(nShadow, o);
         nTraced += __shfl_xor(nTraced, o);
+        nSkip += __shfl_xor(nSkip, o);
     }
     if (laneId() == 0) {
         if (nRays) atomicAdd(&pb.stats[0], (unsigned long long) nRays);
@@ -1050,6 +1073,7 @@ __global__ __launch_bounds__(kTraceBlock, 3) void kFinish(SceneView s, PathBufs 
             atomicAdd(&pb.stats[6], (unsigned long long) nShadow);
         }
         if (nTraced) atomicAdd(&pb.stats[4], (unsigned long long) nTraced);
+        if (nSkip) atomicAdd(&pb.stats[7], (unsigned long long) nSkip);
     }
 }
 

@@ -57,7 +57,8 @@ struct PathBufs {
     uint32_t *fetch;         // persistent-kernel work counters: [2b] extend, [2b+1] shadow
     unsigned long long *stats;  // [0] closest rays, [1] any rays, [2] samples, [3] photons stored,
                                 // [4] closest rays traced by kFinish, [5] samples replaced by 0
-                                // (non-finite), [6] shadow rays traced by kFinish
+                                // (non-finite), [6] shadow rays traced by kFinish, [7] NEE terms exactly
+                                // zero, whose shadow rays are not traced
 };
 
 // Buffers of the shadow-queue binning (k_sort.hip): the binned queue (cap
