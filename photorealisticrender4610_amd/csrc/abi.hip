@@ -114,13 +114,13 @@ int ensureRenderBufs(prr_context *c, int64_t slots, int maxDepth) {
     const size_t qEntries = (size_t) kQSegs * p.qcap, sqEntries = (size_t) kQSegs * p.sqcap;
     for (int k = 0; k < 2; k++) {
         A(&p.qSlot[k], qEntries);
-        A(&p.qOrg[k], qEntries);
-        A(&p.qDir[k], qEntries);
+        A(&p.qOrg[k], 3 * qEntries);
+        A(&p.qDir[k], 3 * qEntries);
     }
     A(&p.hit, qEntries);
     A(&p.sqSlot, sqEntries);
     A(&p.sqOrg, sqEntries);
-    A(&p.sqDir, sqEntries);
+    A(&p.sqDir, 3 * sqEntries);
     A(&p.qcnt, (size_t) maxDepth + 2);
     A(&p.sqcnt, (size_t) maxDepth + 2);
     A(&p.segc, ((size_t) maxDepth + 2) * kQSegs);
@@ -143,7 +143,7 @@ int ensureRenderBufs(prr_context *c, int64_t slots, int maxDepth) {
         b.blocks = 4 * c->numCU;
         A(&b.slot, 3 * slots);
         A(&b.org, 3 * slots);
-        A(&b.dir, 3 * slots);
+        A(&b.dir, 9 * slots);
         A(&b.hist, (size_t) shadowBinCount(c->shadowSortBits) * b.blocks);
     }
     if (e != hipSuccess) {

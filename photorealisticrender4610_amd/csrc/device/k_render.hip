@@ -134,8 +134,8 @@ __global__ __launch_bounds__(kShadeBlock) void kRaygen(SceneView s, PathBufs pb,
         cameraRay(s.cam, p % ci.W, p / ci.W, ci.W, ci.H, ox, oy, rng, o, d);
         pb.rng[slot] = rng.state;
         pb.qSlot[0][slot] = (uint32_t) slot;
-        pb.qOrg[0][slot] = st4(o, 0.f);
-        pb.qDir[0][slot] = st4(d, 0.f);
+        stv3(pb.qOrg[0], slot, o);
+        stv3(pb.qDir[0], slot, d);
     }
     queue0Written(pb, n);
 }
@@ -172,8 +172,8 @@ __global__ __launch_bounds__(kShadeBlock) void kRaygenSeg(PathBufs pb, const Seg
         pb.rng[slot] = rng.state;
         sampleOf[slot] = (uint32_t) sIdx;
         pb.qSlot[0][slot] = (uint32_t) slot;
-        pb.qOrg[0][slot] = st4(o, 0.f);
-        pb.qDir[0][slot] = st4(d, 0.f);
+        stv3(pb.qOrg[0], slot, o);
+        stv3(pb.qDir[0], slot, d);
     }
     queue0Written(pb, n);
 }
@@ -211,8 +211,8 @@ __global__ __launch_bounds__(kShadeBlock) void kRaygenRays(PathBufs pb, int64_t 
         const Pcg rng = Pcg::make(seed, (uint32_t) k, 0u);
         pb.rng[slot] = rng.state;
         pb.qSlot[0][slot] = (uint32_t) slot;
-        pb.qOrg[0][slot] = make_float4(org[3 * k], org[3 * k + 1], org[3 * k + 2], 0.f);
-        pb.qDir[0][slot] = make_float4(dir[3 * k], dir[3 * k + 1], dir[3 * k + 2], 0.f);
+        stv3(pb.qOrg[0], slot, V3(org[3 * k], org[3 * k + 1], org[3 * k + 2]));
+        stv3(pb.qDir[0], slot, V3(dir[3 * k], dir[3 * k + 1], dir[3 * k + 2]));
     }
     queue0Written(pb, n);
 }
@@ -269,12 +269,12 @@ __global__ __launch_bounds__(kTraceBlock) void kExtend(SceneView s, PathBufs pb,
                 exhausted = true;
             } else {
                 const int64_t e = segEntry(spre, pb.qcap, got);
-                const float4 o4 = pb.qOrg[q][e];
-                if (o4.w < 0.f) {  // terminating entry: no ray
+                const V3 o = ldv3(pb.qOrg[q], e);
+                if (isnan(o.x)) {  // no ray (a ray with a NaN origin hits no primitive: the same record)
                     pb.hit[e] = make_float4(__builtin_inff(), 0.f, 0.f, __int_as_float(-1));
                 } else {
                     idx = e;
-                    travBegin(ts, ld3(o4), ld3(pb.qDir[q][e]), 0.f, s.root, st);
+                    travBegin(ts, o, ldv3(pb.qDir[q], e), 0.f, s.root, st);
                 }
             }
         }
@@ -319,7 +319,7 @@ __global__ __launch_bounds__(kTraceBlock) void kShadow(SceneView s, PathBufs pb,
                 const int64_t e = dense ? got : segEntry(spre, pb.sqcap, got);
                 const uint32_t k = pb.sqSlot[e];
                 const float4 o4 = pb.sqOrg[e];
-                travBegin(ts, ld3(o4), ld3(pb.sqDir[e]), o4.w, TREE == 2 ? s.root4q : (TREE == 1 ? s.root4 : s.root),
+                travBegin(ts, ld3(o4), ldv3(pb.sqDir, e), o4.w, TREE == 2 ? s.root4q : (TREE == 1 ? s.root4 : s.root),
                           st);
                 if constexpr (TREE == 2) anyPrepQ(s, ts);
                 if (anyBegin(s, ts)) {
@@ -393,10 +393,11 @@ PRR_DEV bool zeroTerm(const V3 &c) { return c.x == 0.f && c.y == 0.f && c.z == 0
 // a miss or a terminating entry), as read from the queue entry and the hit
 // record, or as a kernel holds them.
 struct HeldRay {
-    float4 h, o4, d4;
+    float4 h;
+    V3 o, d;
     PRR_DEV float4 hit() const { return h; }
-    PRR_DEV V3 org() const { return ld3(o4); }
-    PRR_DEV V3 dir() const { return ld3(d4); }
+    PRR_DEV V3 org() const { return o; }
+    PRR_DEV V3 dir() const { return d; }
 };
 
 // Shading of the path in `slot` at `bounce` (Integrator.cpp:104-263, one
@@ -742,7 +743,7 @@ __device__ __forceinline__ void shadeImpl(const SceneView &s, const PathBufs &pb
             const int64_t e = list ? (int64_t) list[i] : segEntry(spre, pb.qcap, i);
             // queue 0 holds the camera rays in slot order (entry e is slot e)
             slot = bounce == 0 ? (uint32_t) e : pb.qSlot[q][e];
-            const HeldRay ray{pb.hit[e], pb.qOrg[q][e], pb.qDir[q][e]};
+            const HeldRay ray{pb.hit[e], ldv3(pb.qOrg[q], e), ldv3(pb.qDir[q], e)};
             shadePath<PHOTON, CACHE, KIND>(s, pb, ci, bounce, slot, ray, so);
         }
         const bool emitRay = so.emitRay, termEntry = so.termEntry, emitA = so.emitA, emitE = so.emitE,
@@ -770,8 +771,8 @@ __device__ __forceinline__ void shadeImpl(const SceneView &s, const PathBufs &pb
             if (emitRay && qi < pb.qcap) {
                 const int64_t e = (int64_t) seg * pb.qcap + qi;
                 pb.qSlot[q ^ 1][e] = slot;
-                pb.qOrg[q ^ 1][e] = termEntry ? make_float4(0.f, 0.f, 0.f, -1.f) : st4(so.newO, 0.f);
-                pb.qDir[q ^ 1][e] = termEntry ? make_float4(0.f, 0.f, 0.f, 0.f) : st4(so.newD, 0.f);
+                stv3(pb.qOrg[q ^ 1], e, termEntry ? noRayOrigin() : so.newO);
+                stv3(pb.qDir[q ^ 1], e, termEntry ? V3(0.f) : so.newD);
             }
             const uint32_t base = (uint32_t) (pair >> 32);
             const int64_t s0 = (int64_t) seg * pb.sqcap;
@@ -780,7 +781,7 @@ __device__ __forceinline__ void shadeImpl(const SceneView &s, const PathBufs &pb
                 if (ia < pb.sqcap) {
                     pb.sqSlot[s0 + ia] = slot;
                     pb.sqOrg[s0 + ia] = st4(so.sOA, so.tA);
-                    pb.sqDir[s0 + ia] = st4(so.sDA, 0.f);
+                    stv3(pb.sqDir, s0 + ia, so.sDA);
                 }
             }
             if (emitE) {
@@ -788,7 +789,7 @@ __device__ __forceinline__ void shadeImpl(const SceneView &s, const PathBufs &pb
                 if (ie < pb.sqcap) {
                     pb.sqSlot[s0 + ie] = slot | (1u << 30);
                     pb.sqOrg[s0 + ie] = st4(so.sOE, __builtin_inff());
-                    pb.sqDir[s0 + ie] = st4(so.sDE, 0.f);
+                    stv3(pb.sqDir, s0 + ie, so.sDE);
                 }
             }
             if (emitP) {
@@ -796,7 +797,7 @@ __device__ __forceinline__ void shadeImpl(const SceneView &s, const PathBufs &pb
                 if (ip < pb.sqcap) {
                     pb.sqSlot[s0 + ip] = slot | (2u << 30);
                     pb.sqOrg[s0 + ip] = st4(so.sOP, so.tP);
-                    pb.sqDir[s0 + ip] = st4(so.sDP, 0.f);
+                    stv3(pb.sqDir, s0 + ip, so.sDP);
                 }
             }
         }
@@ -835,8 +836,9 @@ __global__ __launch_bounds__(kShadeBlock, W) void kShade(SceneView s, PathBufs p
 // kShadeBinScatter writes each block's entries in chunk order (wave
 // ballots + prefix sums within the block).
 PRR_DEV bool diffuseHit(const SceneView &s, const PathBufs &pb, int q, int64_t i) {
+    // an entry without a ray has a miss record (prim -1)
     const int prim = __float_as_int(pb.hit[i].w);
-    if (prim < 0 || pb.qOrg[q][i].w < 0.f) return false;
+    if (prim < 0) return false;
     const int mat = __float_as_int(s.prim_n[prim].w) & kPrimMatMask;
     return (int) s.mats[mat].base_type.w == MAT_DIFFUSE;
 }
@@ -1033,19 +1035,19 @@ __global__ __launch_bounds__(kTraceBlock, 3) void kFinish(SceneView s, PathBufs 
         if (w >= n) break;
         const int64_t i = segEntry(spre, pb.qcap, w);
         const uint32_t slot = pb.qSlot[q][i];
-        float4 o4 = pb.qOrg[q][i], d4 = pb.qDir[q][i];
+        V3 o = ldv3(pb.qOrg[q], i), d = ldv3(pb.qDir[q], i);
         for (int b = bounce0;; b++) {
             float4 h = make_float4(__builtin_inff(), 0.f, 0.f, __int_as_float(-1));
-            if (!(o4.w < 0.f)) {  // w < 0: terminating entry, no ray
+            if (!isnan(o.x)) {  // a NaN origin: no ray (terminating entry)
                 TravState ts;
-                travBegin(ts, ld3(o4), ld3(d4), 0.f, s.root, lst);
+                travBegin(ts, o, d, 0.f, s.root, lst);
                 while (!closestIter(s, ts, lst, wwCap)) {
                 }
                 h = make_float4(ts.h.t, ts.h.u, ts.h.v, __int_as_float(ts.h.prim));
                 nTraced++;
             }
             ShadeOut so;
-            shadePath<PHOTON, CACHE, kShadeAny>(s, pb, ci, b, slot, HeldRay{h, o4, d4}, so);
+            shadePath<PHOTON, CACHE, kShadeAny>(s, pb, ci, b, slot, HeldRay{h, o, d}, so);
             if (so.emitA) pb.vis[slot] = occluded(s, so.sOA, so.sDA, so.tA, tree, wwCap, lst, rst) ? 0 : 1;
             if (so.emitE)
                 pb.vis[pb.slots + slot] =
@@ -1056,8 +1058,8 @@ __global__ __launch_bounds__(kTraceBlock, 3) void kFinish(SceneView s, PathBufs 
             nSkip += so.skipped;
             // the wavefront shades bounces 0..maxDepth, never the queue past it
             if (!so.emitRay || b >= ci.maxDepth) break;
-            o4 = so.termEntry ? make_float4(0.f, 0.f, 0.f, -1.f) : st4(so.newO, 0.f);
-            d4 = st4(so.newD, 0.f);
+            o = so.termEntry ? noRayOrigin() : so.newO;
+            d = so.newD;
         }
     }
     for (int o = 32; o > 0; o >>= 1) {

@@ -44,7 +44,7 @@ struct BinGrid {
     int bits;
 };
 
-PRR_DEV uint32_t binOf(const BinGrid &g, const float4 &o, const float4 &d) {
+PRR_DEV uint32_t binOf(const BinGrid &g, const float4 &o, const V3 &d) {
     const uint32_t mort = spread3(cellOf(o.x, g.lo[0], g.scale[0], g.top)) |
                           (spread3(cellOf(o.y, g.lo[1], g.scale[1], g.top)) << 1) |
                           (spread3(cellOf(o.z, g.lo[2], g.scale[2], g.top)) << 2);
@@ -94,7 +94,7 @@ __global__ __launch_bounds__(kBinBlock) void kBinHist(PathBufs pb, int bounce, B
         const int64_t i = w + (threadIdx.x & 63);
         const bool valid = i < c1;
         const int64_t e = valid ? segEntry(spre, pb.sqcap, i) : 0;
-        const uint32_t bin = valid ? binOf(g, pb.sqOrg[e], pb.sqDir[e]) : 0u;
+        const uint32_t bin = valid ? binOf(g, pb.sqOrg[e], ldv3(pb.sqDir, e)) : 0u;
         (void) waveBinAdd(cnt, bin, valid);
     }
     __syncthreads();
@@ -129,7 +129,7 @@ __global__ __launch_bounds__(kBinScanThreads) void kBinScan(uint32_t *__restrict
 
 __global__ __launch_bounds__(kBinBlock) void kBinScatter(PathBufs pb, int bounce, BinGrid g, int bins,
                                                          const uint32_t *__restrict__ base, uint32_t *__restrict__ slot,
-                                                         float4 *__restrict__ org, float4 *__restrict__ dir) {
+                                                         float4 *__restrict__ org, float *__restrict__ dir) {
     extern __shared__ uint32_t cur[];
     __shared__ uint32_t spre[kQSegs + 1];
     for (int k = threadIdx.x; k < bins; k += kBinBlock) cur[k] = base[(size_t) k * gridDim.x + blockIdx.x];
@@ -139,19 +139,20 @@ __global__ __launch_bounds__(kBinBlock) void kBinScatter(PathBufs pb, int bounce
     for (int64_t w = c0 + (threadIdx.x & ~63); w < c1; w += kBinBlock) {
         const int64_t i = w + (threadIdx.x & 63);
         const bool valid = i < c1;
-        float4 o = make_float4(0.f, 0.f, 0.f, 0.f), d = o;
+        float4 o = make_float4(0.f, 0.f, 0.f, 0.f);
+        V3 d(0.f);
         uint32_t sl = 0;
         if (valid) {
             const int64_t e = segEntry(spre, pb.sqcap, i);
             o = pb.sqOrg[e];
-            d = pb.sqDir[e];
+            d = ldv3(pb.sqDir, e);
             sl = pb.sqSlot[e];
         }
         const uint32_t j = waveBinAdd(cur, valid ? binOf(g, o, d) : 0u, valid);
         if (valid) {
             slot[j] = sl;
             org[j] = o;
-            dir[j] = d;
+            stv3(dir, j, d);
         }
     }
 }

@@ -40,12 +40,13 @@ struct PathBufs {
     uint8_t *vis;            // 3 * slots: visibility of the pending NEE rays
     float4 *result;          // final L per slot
     uint32_t *qSlot[2];      // ray queue (ping-pong)
-    float4 *qOrg[2];         // origin, w < 0: no ray (terminating entry)
-    float4 *qDir[2];
+    float *qOrg[2];          // origin, 3 floats per entry; a NaN x marks an entry without a ray
+                             // (a path that comes back only for its pending NEE)
+    float *qDir[2];          // direction, 3 floats per entry
     float4 *hit;             // t, u, v, prim (bits), aligned with the queue being shaded
     uint32_t *sqSlot;        // shadow queue: slot | kind << 30
     float4 *sqOrg;           // origin, tmax
-    float4 *sqDir;
+    float *sqDir;            // direction, 3 floats per entry
     uint32_t *qcnt;          // ray-queue count per bounce (total over its segments)
     uint32_t *sqcnt;         // shadow-queue count per bounce
     unsigned long long *segc;  // per shading pass b and segment: entries appended to the ray queue of
@@ -69,7 +70,7 @@ struct ShadowSortBufs {
     int blocks = 0;
     uint32_t *slot = nullptr;
     float4 *org = nullptr;
-    float4 *dir = nullptr;
+    float *dir = nullptr;  // 3 floats per entry
     uint32_t *hist = nullptr;
 };
 
@@ -149,6 +150,20 @@ struct AdaptiveBufs {
     int32_t *list;       // active pixel list
     uint32_t *count;     // active pixel count
 };
+
+// Three-float records (queue origins and directions).
+PRR_DEV V3 ldv3(const float *a, int64_t e) {
+    const float *p = a + 3 * e;
+    return V3(p[0], p[1], p[2]);
+}
+PRR_DEV void stv3(float *a, int64_t e, const V3 &v) {
+    float *p = a + 3 * e;
+    p[0] = v.x;
+    p[1] = v.y;
+    p[2] = v.z;
+}
+// Origin of a ray-queue entry without a ray.
+PRR_DEV V3 noRayOrigin() { return V3(__builtin_nanf(""), 0.f, 0.f); }
 
 // Entries per segment of a queue whose producer appends at most perEntry
 // entries for each of at most `entries` input positions. The waves that
