@@ -380,7 +380,7 @@ int prr_create(int device, prr_context **out) {
     // idle, at most 8 interior steps (4 for shadow rays) per work check
     // (measured on the 1M-tri scene). PRR_TRACE_BLOCKS_PER_CU, PRR_STACK_LDS,
     // PRR_REFILL_AT, PRR_WHILE_WHILE(_ANY), PRR_FETCH_RUN, PRR_SHADOW_WIDE, PRR_SHADOW_FAST,
-    // PRR_CONCURRENT_SHADOW, PRR_SHADE_WAVES and PRR_TAIL_PATHS override the
+    // PRR_CONCURRENT_SHADOW, PRR_SHADE_WAVES, PRR_TAIL_PATHS and PRR_BAND_ROWS override the
     // measured defaults for tuning sweeps.
     int tbpc = 8;
     if (const char *ev = getenv("PRR_TRACE_BLOCKS_PER_CU")) tbpc = std::max(1, std::min(16, atoi(ev)));
@@ -410,6 +410,7 @@ int prr_create(int device, prr_context **out) {
         c->shadeLightWaves = v >= 8 ? 8 : (v >= 6 ? 6 : 4);
     }
     if (const char *ev = getenv("PRR_TAIL_PATHS")) c->tailPaths = std::max<int64_t>(0, atoll(ev));
+    if (const char *ev = getenv("PRR_BAND_ROWS")) c->bandRows = std::max(1, std::min(64, atoi(ev)));
     if (const char *ev = getenv("PRR_SHADOW_SORT")) c->shadowSortFrom = std::max(0, std::min(64, atoi(ev)));
     if (const char *ev = getenv("PRR_SHADOW_SORT_BITS")) c->shadowSortBits = std::max(1, std::min(3, atoi(ev)));
     if (const char *ev = getenv("PRR_STACK_LDS")) {  // LDS stack entries per lane: 2, 4, 8, 16 or 32
@@ -1056,6 +1057,7 @@ int prr_render(prr_context *c, const prr_scene *s, const prr_render_params *p, f
         const int per = (int) std::max<int64_t>(1, std::min<int64_t>(count, c->pbSlots / npix));
         for (int k = 0; k < count; k += per) {
             ci.pixList = list;
+            ci.bandRows = list ? 0 : c->bandRows;
             ci.pixels = (int) npix;
             ci.spc = std::min(per, count - k);
             ci.sampleBase = p->sample_offset + s0 + k;

@@ -58,6 +58,9 @@ struct prr_context {
     // enqueuing bounces once a copied count shows the switch.
     static constexpr int kMaxBounces = 66;
     int64_t tailPaths = 1 << 18;
+    // camera paths of a full-image render in bands of bandRows image rows,
+    // column by column within a band (ChunkInfo::bandRows; 1: scanline order)
+    int bandRows = 8;
     uint32_t *cntHost = nullptr;
     hipEvent_t evCnt[kMaxBounces] = {};
     // shadow-queue binning before the any-hit traversal of bounces >=

@@ -59,8 +59,7 @@ PRR_DEV uint32_t trainHash(uint32_t p, uint32_t s) {
 // the render trains the cache and the hash of its pixel and sample selects it.
 PRR_DEV uint32_t pathFlags0(const ChunkInfo &ci, uint32_t slot, uint32_t sIdx) {
     if (!ci.cache.train) return 0u;
-    const int e = (int) (slot % (uint32_t) ci.pixels);
-    const int p = ci.pixList ? ci.pixList[e] : e;
+    const int p = pixelOf(ci, (int) (slot % (uint32_t) ci.pixels));
     return (trainHash((uint32_t) p, sIdx) & ci.cache.trainMask) == 0u ? F_TRAIN : 0u;
 }
 
@@ -124,8 +123,7 @@ __global__ __launch_bounds__(kShadeBlock) void kRaygen(SceneView s, PathBufs pb,
     const int64_t n = (int64_t) ci.pixels * ci.spc;
     const int64_t stride = (int64_t) gridDim.x * kShadeBlock;
     for (int64_t slot = (int64_t) blockIdx.x * kShadeBlock + threadIdx.x; slot < n; slot += stride) {
-        const int e = (int) (slot % ci.pixels);
-        const int p = ci.pixList ? ci.pixList[e] : e;
+        const int p = pixelOf(ci, (int) (slot % ci.pixels));
         const int sIdx = ci.sampleBase + (int) (slot / ci.pixels);
         Pcg rng = Pcg::make(ci.seed, (uint32_t) p, (uint32_t) sIdx);
         const float ox = rng.next();  // Renderer.cpp:75-76
@@ -1094,12 +1092,13 @@ __global__ void kCacheTargets(PathBufs pb, CacheRec cr, uint32_t r0, uint32_t r1
 
 // Renderer.cpp:84-99: framebuffer[m] += s in sample order.
 __global__ void kAccumulate(PathBufs pb, ChunkInfo ci, float4 *fb) {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
     uint32_t nan = 0;
-    if (p < ci.pixels) {
+    if (e < ci.pixels) {
+        const int p = pixelOf(ci, e);
         V3 acc = ld3(fb[p]);
         for (int k = 0; k < ci.spc; k++) {
-            const float4 r = pb.result[(int64_t) k * ci.pixels + p];
+            const float4 r = pb.result[(int64_t) k * ci.pixels + e];
             acc += ld3(r);
             nan += r.w != 0.f ? 1u : 0u;
         }
@@ -1127,7 +1126,7 @@ PRR_DEV float welfordCi(int n, float mean, float m2) {
 // Returns how many of the taken samples were replaced (non-finite).
 PRR_DEV uint32_t accumulateAdaptive(const PathBufs &pb, const ChunkInfo &ci, float4 *fb, const AdaptiveBufs &ab,
                                     int minSpp, float thr, int e) {
-    const int p = ci.pixList ? ci.pixList[e] : e;
+    const int p = pixelOf(ci, e);
     float4 m = ab.mean[p], q = ab.m2[p];
     if (q.w != 0.f) return 0u;
     int n = __float_as_int(m.w);

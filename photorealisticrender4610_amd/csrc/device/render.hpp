@@ -129,9 +129,13 @@ struct ChunkInfo {
     uint64_t seed;
     int maxDepth;
     float clampInd;      // Integrator::indirectClamp
-    // Pixel of slot entry e = slot % pixels: pixList[e], or e when null
-    // (adaptive passes render only the pixels still sampling).
+    // Pixel of slot entry e = slot % pixels (pixelOf): pixList[e] (adaptive
+    // passes render only the pixels still sampling); with no list, e in
+    // bands of bandRows image rows, column by column within a band, so the
+    // 64 lanes of a wave take an 8 x 8 block of pixels at bandRows = 8
+    // (bandRows <= 1: scanline order).
     const int32_t *pixList = nullptr;
+    int bandRows = 0;
     CacheRec cache;
     // per-slot sample index (serving batches); null: sampleBase + slot / pixels
     const uint32_t *sampleOf = nullptr;
@@ -140,6 +144,16 @@ struct ChunkInfo {
     // every later bounce.
     int64_t tailPaths = 0;
 };
+
+PRR_DEV int pixelOf(const ChunkInfo &ci, int e) {
+    if (ci.pixList) return ci.pixList[e];
+    if (ci.bandRows <= 1) return e;
+    const int span = ci.bandRows * ci.W;  // entries of a full band
+    const int band = e / span, local = e - band * span;
+    const int rows = min(ci.bandRows, ci.H - band * ci.bandRows);
+    const int x = local / rows;
+    return (band * ci.bandRows + (local - x * rows)) * ci.W + x;
+}
 
 // Per-pixel Welford state of the adaptive sampler (Renderer.hpp:14-40):
 // mean.xyz / m2.xyz per channel, mean.w = sample count (int bits),
