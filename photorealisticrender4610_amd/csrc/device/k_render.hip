@@ -57,9 +57,17 @@ PRR_DEV uint32_t trainHash(uint32_t p, uint32_t s) {
 
 // Flags of a path at its first bounce: a radiance-cache training path when
 // the render trains the cache and the hash of its pixel and sample selects it.
+// Training renders always run the shading kernels compiled with the cache
+// (CACHE), which find the pixel through the camera-path bands. The kernels
+// compiled without it never see ci.cache.train set; their copy keeps the
+// plain entry-to-pixel lookup, which is never reached there and which the
+// compiler schedules without spills, where the band lookup (or none) left
+// 16-24 B per lane of spills in kShade.
+template <bool CACHE>
 PRR_DEV uint32_t pathFlags0(const ChunkInfo &ci, uint32_t slot, uint32_t sIdx) {
     if (!ci.cache.train) return 0u;
-    const int p = pixelOf(ci, (int) (slot % (uint32_t) ci.pixels));
+    const int e = (int) (slot % (uint32_t) ci.pixels);
+    const int p = CACHE ? pixelOf(ci, e) : (ci.pixList ? ci.pixList[e] : e);
     return (trainHash((uint32_t) p, sIdx) & ci.cache.trainMask) == 0u ? F_TRAIN : 0u;
 }
 
@@ -430,7 +438,7 @@ __device__ __forceinline__ void shadePath(const SceneView &s, const PathBufs &pb
         // a path's first bounce starts from L = 0, beta = 1 (Integrator.cpp:
         // castRay's locals); the camera-ray kernels store only the ray and
         // the random-number state
-        fl = pathFlags0(ci, slot, (uint32_t) sIdx);
+        fl = pathFlags0<CACHE>(ci, slot, (uint32_t) sIdx);
         L = V3(0.f);
         beta = V3(1.f);
     } else {

@@ -126,6 +126,7 @@ struct ChunkInfo {
     int pixels;          // W * H
     int spc;             // samples of this chunk per pixel
     int sampleBase;      // absolute index of the chunk's first sample
+    int bandRows = 0;    // camera-path bands (pixelOf); in the padding before seed
     uint64_t seed;
     int maxDepth;
     float clampInd;      // Integrator::indirectClamp
@@ -135,7 +136,6 @@ struct ChunkInfo {
     // 64 lanes of a wave take an 8 x 8 block of pixels at bandRows = 8
     // (bandRows <= 1: scanline order).
     const int32_t *pixList = nullptr;
-    int bandRows = 0;
     CacheRec cache;
     // per-slot sample index (serving batches); null: sampleBase + slot / pixels
     const uint32_t *sampleOf = nullptr;
