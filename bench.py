@@ -205,6 +205,29 @@ def profiled_traffic(workload, kernel="kExtend"):
     return int(m.group(1)) if m else None
 
 
+def profiled_pipeline(workload, kernel="kExtend"):
+    """Busy fractions of the vector-memory pipeline for `kernel` on this
+    workload, from the memory-pipeline table of the same committed counter
+    summary (rocprofv3 --pmc TA_TA_BUSY_sum, TD_TD_BUSY_sum,
+    TCP_TOTAL_CACHE_ACCESSES_sum over GRBM_GUI_ACTIVE): the texture address and
+    data units' busy share of the kernel's cycles and the vector L1's tag
+    lookups per CU per cycle (its peak is one). None without such a table."""
+    import re
+    path = os.path.join(ROOT, "profiles", "pmc_traversal_%s.md" % workload)
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("| `%s<" % kernel) and line.count("|") == 11:
+            c = [x.strip() for x in line.strip().strip("|").split("|")]
+            try:
+                return {"ta_busy": round(float(c[2]) / 100, 3), "td_busy": round(float(c[4]) / 100, 3),
+                        "l1_lookups_per_cu_cycle": float(c[6]), "l1_lookups_per_load": float(c[7]),
+                        "source": "rocprofv3 --pmc, profiles/pmc_traversal_%s.md" % workload}
+            except ValueError:
+                return None
+    return None
+
+
 def rel_mse(x, ref):
     """mean over pixels of sum over channels of (x - ref)^2 / (ref^2 + 1e-2)."""
     return float((((x - ref) ** 2) / (ref ** 2 + 1e-2)).sum(-1).mean())
@@ -524,7 +547,10 @@ def main():
                 "kernel": desc, "avg_launch_ms": round(avg_ms, 4), "launches_per_step": nlaunch // args.steps,
                 "summed_ms_per_step": round(ms / args.steps, 2),
                 "algorithmic": {"bytes_per_ray": round(bytes_per_ray, 1), "bytes_per_launch": int(alg),
-                                "GBps": round(alg / (avg_ms / 1e3) / 1e9, 1) if avg_ms > 0 else None}}
+                                "GBps": round(alg / (avg_ms / 1e3) / 1e9, 1) if avg_ms > 0 else None},
+                # what binds the kernel: each lane's 16-B record load is one L1
+                # lookup, and the vector-memory units run near their rate
+                "pipeline": profiled_pipeline(args.workload, name) if args.spp == 0 else None}
     rl_ext = kernel_roofline("kExtend", "kExtend (closest-hit traversal, extension rays)", ms_ext, launches, closest,
                              per_ray)
     rl_sh = kernel_roofline("kShadow", "kShadow (any-hit traversal, shadow rays)", ms_sh, launches_sh, anyr, per_any)
