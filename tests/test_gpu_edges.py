@@ -124,14 +124,15 @@ def test_single_triangle_and_spheres_only(ctx):
 
 
 def test_tiny_and_ragged_images(ctx):
-    """1x1 and 3x2 images, 1 and 3 spp, depth 1 and 8: per-path parity with
-    the oracle (same PCG streams)."""
+    """1x1 and 3x2 images, 1 and 3 spp, depth 1 and 8, and 37x21, whose camera
+    paths fill two 8-row bands and a last band of 5 rows: per-path parity
+    with the oracle (same PCG streams)."""
     from photorealisticrender4610_amd import scenes
     print("case edges: tiny images", flush=True)
     sc = scenes.box_room()
     scene = ctx.scene(sc)
     orc = O.OracleScene(sc)
-    for w, h, spp, depth in ((1, 1, 1, 1), (1, 1, 3, 8), (3, 2, 2, 8), (17, 5, 1, 4)):
+    for w, h, spp, depth in ((1, 1, 1, 1), (1, 1, 3, 8), (3, 2, 2, 8), (17, 5, 1, 4), (37, 21, 2, 8)):
         img, st = ctx.render(scene, w, h, spp, depth, seed=9)
         ref, _, _ = orc.render(w, h, spp, depth, 9)
         g = img.cpu().numpy()
