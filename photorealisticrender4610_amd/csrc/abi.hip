@@ -198,12 +198,18 @@ int runBounces(prr_context *c, const SceneView &view, const ChunkInfo &ci, Timin
     const bool conc = c->concurrentShadow && c->aux && (int64_t) ci.pixels * ci.spc <= c->concurrentMaxPaths;
     const int wwShadow = c->shadowWide ? c->whileWhileAny : c->whileWhile;
     const bool tail = c->tailPaths > 0 && c->cntHost && ci.maxDepth + 1 < prr_context::kMaxBounces;
+    // A radiance-cache render stops its other paths at the query bounce, and
+    // only its training paths (one in train_every) go on: completing those in
+    // one launch once they number at most 2^19 saves their per-bounce
+    // launches (measured: 12.2 -> 12.0 ms per 1080p 4-spp frame with
+    // full-depth training; plain frames are best at the default).
+    const int64_t tailAt = ci.cache.query ? std::max<int64_t>(c->tailPaths, int64_t(1) << 19) : c->tailPaths;
     ChunkInfo cit = ci;
-    cit.tailPaths = tail ? c->tailPaths : 0;
+    cit.tailPaths = tail ? tailAt : 0;
     for (int b = 0; b <= ci.maxDepth; b++) {
         if (tail && b > 0) {
             HIPCK(c, hipEventSynchronize(c->evCnt[b - 1]));
-            if ((int64_t) c->cntHost[b - 1] <= c->tailPaths) break;
+            if ((int64_t) c->cntHost[b - 1] <= tailAt) break;
         }
         if (tail) {
             HIPCK(c, hipMemcpyAsync(c->cntHost + b, c->pb.qcnt + b, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
